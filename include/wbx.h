@@ -1,0 +1,155 @@
+/*
+ * wbx.h -- C ABI of the MI355X (gfx950) DTW / distance library (libwbx.so).
+ *
+ * Drop-in for the batch drivers of wildboar's Cython metric layer
+ * (src/wildboar/distance/_cdistance.pyx) for metric = "euclidean", "dtw" and
+ * "wdtw".  Plain C types only: pointers, sizes, status codes.  No exceptions
+ * cross this boundary, nothing here calls Python, and every function may be
+ * called with the GIL released.
+ *
+ * Conventions
+ *   - Return value: WBX_OK (0) or a negative WBX_E* code; the message is in
+ *     wbx_last_error() (thread-local).
+ *   - Inputs are borrowed; outputs are caller-owned and fully written.
+ *   - Time-series arrays follow the reference's TSArray layout
+ *     (src/wildboar/utils/__init__.pxd:4): C-contiguous (n, dims, t) float64;
+ *     `dim` selects the row &X[i, dim, 0].
+ *   - Calls on one device are serialised by a per-device lock; distinct
+ *     devices run concurrently.
+ */
+#ifndef WBX_H
+#define WBX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WBX_OK 0
+#define WBX_EINVAL (-1)        /* bad argument: maps to ValueError        */
+#define WBX_ENOMEM (-2)        /* device/host allocation failed           */
+#define WBX_EDEVICE (-3)       /* HIP runtime error / no device           */
+#define WBX_EUNSUPPORTED (-4)  /* valid request this build cannot serve   */
+
+/* Metric kinds: the names of the reference's _METRICS registry
+ * (distance/_distance.py:185-204) this library serves. */
+#define WBX_EUCLIDEAN 0 /* EuclideanMetric,  distance/_metric.pyx:381-409   */
+#define WBX_DTW 1       /* DtwMetric,        distance/_elastic.pyx:3126-3217 */
+#define WBX_WDTW 2      /* WeightedDtwMetric, distance/_elastic.pyx:3322-3341 */
+
+/* Arithmetic modes. */
+#define WBX_FP64 0        /* float64, bit-identical to the reference            */
+#define WBX_FP32 1        /* float32 cells; argmin/labels certified exact      */
+#define WBX_FP32_ACC64 2  /* float32 differences, float64 accumulated cost     */
+
+/* Input element types for the device-pointer entry points. */
+#define WBX_F64 0
+#define WBX_F32 1
+
+typedef struct wbx_metric {
+    int32_t kind;      /* WBX_EUCLIDEAN / WBX_DTW / WBX_WDTW                 */
+    int32_t precision; /* WBX_FP64 / WBX_FP32 / WBX_FP32_ACC64                */
+    double r;          /* Sakoe-Chiba window fraction in [0, 1] (dtw, wdtw)  */
+    double g;          /* WDTW penalty control, >= 0                          */
+} wbx_metric;
+
+/* ----------------------------------------------------------------- runtime */
+
+/* Select and initialise a device (creates its stream and scratch arena). */
+int wbx_init(int device);
+/* Thread-local message of the last failed call ("" if none). */
+const char *wbx_last_error(void);
+/* Library version string. */
+const char *wbx_version(void);
+/* Number of HIP devices visible (0 when there is none). */
+int wbx_device_count(void);
+
+/* No counterpart in the reference (tuning / measurement).  Kernel family:
+ * 0 = fastest available (default), 1 = the plain one-pair-per-block kernel
+ * for every DTW request (the "plain" baseline of the benchmarks). */
+int wbx_set_policy(int policy);
+/* No counterpart in the reference (measurement).  While enabled, HIP events
+ * are recorded on the launch stream around every distance kernel;
+ * wbx_prof_take waits for them, returns the summed milliseconds, the number
+ * of kernels and the kind of the last one (1 euclidean, 2 lane-per-pair DTW,
+ * 3 plain DTW, 4 batched DTW, 5 subsequence scan), and clears the record. */
+int wbx_prof_enable(int on);
+int wbx_prof_take(double *total_ms, int64_t *launches, int *last_kind);
+
+/* ------------------------------------------- host-pointer drop-ins (float64)
+ * Synchronous.  Inputs and outputs are host memory.  These replace the
+ * reference's batch drivers one for one.                                    */
+
+/* Replaces _pairwise_distance (distance/_cdistance.pyx:1184-1205) and, when
+ * Y == NULL, _singleton_pairwise_distance (_cdistance.pyx:1248-1267).
+ * out is (nx, ny) (or (nx, nx) for Y == NULL, zero diagonal, i<j computed
+ * as D(X[i], X[j]) and mirrored). */
+int wbx_pairwise(const wbx_metric *metric, const double *X, int64_t nx,
+                 int64_t dims, int64_t tx, const double *Y, int64_t ny,
+                 int64_t ty, int64_t dim, double *out);
+
+/* Replaces _paired_distance (distance/_cdistance.pyx:1632-1652):
+ * out[i] = D(Y[i], X[i]) where X, Y are paired_distance's x and y
+ * (the reference swaps the arguments; this matters for WDTW). */
+int wbx_paired(const wbx_metric *metric, const double *X, const double *Y,
+               int64_t n, int64_t dims, int64_t t, int64_t dim, double *out);
+
+/* Replaces _argmin_distance (distance/_cdistance.pyx:1348-1378): the k
+ * nearest Y rows of each X row, in the reference's heap order (Heap,
+ * utils/_misc.pyx:18-107).  lower_bound (nx, ny) may be NULL.  idx is
+ * (nx, k) int64, dist is (nx, k). */
+int wbx_argmin(const wbx_metric *metric, const double *X, int64_t nx,
+               int64_t dims, int64_t tx, const double *Y, int64_t ny,
+               int64_t ty, int64_t dim, int64_t k, const double *lower_bound,
+               int64_t *idx, double *dist);
+
+/* Replaces _pairwise_subsequence_distance (distance/_cdistance.pyx:1018-1037):
+ * shapelets are packed back to back in S (shapelet j starts at s_off[j] and
+ * has s_len[j] values); dist and idx are (n, n_s): minimum distance over the
+ * windows of X[i, dim] and the first window index attaining it. */
+int wbx_subsequence_pairwise(const wbx_metric *metric, const double *S,
+                             const int64_t *s_off, const int64_t *s_len,
+                             int64_t n_s, const double *X, int64_t n,
+                             int64_t dims, int64_t t, int64_t dim, double *dist,
+                             int64_t *idx);
+
+/* ------------------------------------------- device-pointer entry points
+ * Inputs are device pointers of element type `dtype` (WBX_F64 / WBX_F32);
+ * series i of X starts at X + i * x_stride.  Outputs are device memory.
+ * Work is enqueued on `stream` (a hipStream_t; NULL = the library's own
+ * stream) and the call returns once it is enqueued on that stream.          */
+
+/* Pairwise / singleton (Y == NULL) as wbx_pairwise; out is (nx, ny) f64. */
+int wbx_dev_pairwise(const wbx_metric *metric, int dtype, const void *X,
+                     int64_t nx, int64_t x_stride, int64_t tx, const void *Y,
+                     int64_t ny, int64_t y_stride, int64_t ty, double *out,
+                     void *stream);
+
+/* Paired as wbx_paired: out[i] = D(Y[i], X[i]). */
+int wbx_dev_paired(const wbx_metric *metric, int dtype, const void *X,
+                   int64_t x_stride, const void *Y, int64_t y_stride,
+                   int64_t n, int64_t t, double *out, void *stream);
+
+/* Argmin as wbx_argmin (heap order), device outputs idx (nx, k) int64 and
+ * dist (nx, k) f64. */
+int wbx_dev_argmin(const wbx_metric *metric, int dtype, const void *X,
+                   int64_t nx, int64_t x_stride, int64_t tx, const void *Y,
+                   int64_t ny, int64_t y_stride, int64_t ty, int64_t k,
+                   const double *lower_bound, int64_t *idx, double *dist,
+                   void *stream);
+
+/* Subsequence scan as wbx_subsequence_pairwise; S is packed shapelets of
+ * element type dtype, s_off / s_len are device int64 arrays. */
+int wbx_dev_subsequence_pairwise(const wbx_metric *metric, int dtype,
+                                 const void *S, const int64_t *s_off,
+                                 const int64_t *s_len, int64_t n_s,
+                                 const void *X, int64_t n, int64_t x_stride,
+                                 int64_t t, double *dist, int64_t *idx,
+                                 void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WBX_H */

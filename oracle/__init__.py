@@ -1,0 +1,259 @@
+"""Float64 CPU oracle for wildboar's distance hot path.
+
+TEST INFRASTRUCTURE ONLY.  This package is the checker: only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import it.  The product (``wildboar_amd``) never imports it.
+
+The arithmetic lives in ``oracle.c`` (a restatement of
+``distance/_elastic.pyx:869-940`` and friends; every function there cites the
+reference lines it follows).  This module is a thin ctypes layer over it with
+the reference's driver semantics:
+
+- ``pairwise``   : ``_pairwise_distance``   (``_cdistance.pyx:1184-1205``)
+- ``singleton``  : ``_singleton_pairwise_distance`` (``:1248-1267``)
+- ``paired``     : ``_paired_distance``     (``:1632-1652``, D(y[i], x[i]))
+- ``argmin``     : ``_argmin_distance``     (``:1348-1378``, heap order)
+- ``subsequence``: ``_pairwise_subsequence_distance`` (``:1018-1037``)
+- ``dtw_grad``   : path-backtrack gradient (``_elastic.pyx:1011-1073`` +
+  ``dtw.py:395-423``)
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "lib", "liboracle.so")
+
+EUCLIDEAN, DTW, WDTW = 0, 1, 2
+_KINDS = {"euclidean": EUCLIDEAN, "dtw": DTW, "wdtw": WDTW}
+
+_lib = None
+
+
+def build():
+    """Compile oracle.c into oracle/lib/liboracle.so (gcc, -O2, no FMA)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    d, i64, i32 = ctypes.c_double, ctypes.c_int64, ctypes.c_int
+    p = ctypes.c_void_p
+    lib.orc_compute_r.restype = i64
+    lib.orc_compute_r.argtypes = [i64, d]
+    lib.orc_wdtw_weights.restype = None
+    lib.orc_wdtw_weights.argtypes = [i64, d, p]
+    lib.orc_dtw_distance.restype = d
+    lib.orc_dtw_distance.argtypes = [p, i64, p, i64, i64, p, p, p, d]
+    lib.orc_euclidean_distance.restype = d
+    lib.orc_euclidean_distance.argtypes = [p, i64, p, i64, d, p]
+    lib.orc_metric_distance.restype = d
+    lib.orc_metric_distance.argtypes = [i32, d, d, p, i64, p, i64, i64]
+    lib.orc_pairwise.restype = i32
+    lib.orc_pairwise.argtypes = [i32, d, d, p, i64, i64, i64, p, i64, i64, i64, p, i32]
+    lib.orc_singleton.restype = i32
+    lib.orc_singleton.argtypes = [i32, d, d, p, i64, i64, i64, i64, p, i32]
+    lib.orc_paired.restype = i32
+    lib.orc_paired.argtypes = [i32, d, d, p, p, i64, i64, i64, i64, p, i32]
+    lib.orc_argmin.restype = i32
+    lib.orc_argmin.argtypes = [i32, d, d, p, i64, i64, i64, p, i64, i64, i64, i64, p, p, p, i32]
+    lib.orc_subsequence_pairwise.restype = i32
+    lib.orc_subsequence_pairwise.argtypes = [i32, d, d, p, p, p, i64, p, i64, i64, i64, i64, p, p, i32]
+    lib.orc_argmin_replay.restype = None
+    lib.orc_argmin_replay.argtypes = [p, i64, i64, p, p, p]
+    lib.orc_dtw_cost_matrix.restype = i32
+    lib.orc_dtw_cost_matrix.argtypes = [p, i64, p, i64, i64, p, p]
+    lib.orc_dtw_mapping.restype = i64
+    lib.orc_dtw_mapping.argtypes = [p, i64, i64, p, p]
+    lib.orc_dtw_grad.restype = d
+    lib.orc_dtw_grad.argtypes = [p, i64, p, i64, d, p, p, p]
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _ts(a):
+    """(n, dims, t) C-contiguous float64, as utils/validation.py:_check_ts_array."""
+    a = np.asarray(a, dtype=np.float64)
+    if a.ndim == 1:
+        a = a.reshape(1, 1, -1)
+    elif a.ndim == 2:
+        a = a.reshape(a.shape[0], 1, a.shape[1])
+    return np.ascontiguousarray(a)
+
+
+def _kind(metric):
+    return _KINDS[metric]
+
+
+def default_threads():
+    return os.cpu_count() or 1
+
+
+def compute_r(length, r):
+    return _load().orc_compute_r(int(length), float(r))
+
+
+def wdtw_weights(n, g):
+    out = np.empty(n, dtype=np.float64)
+    _load().orc_wdtw_weights(n, float(g), out.ctypes.data)
+    return out
+
+
+def dtw_distance(x, y, band, weights=None, min_dist=np.inf):
+    """Raw dtw_distance (squared cost) with an integer band ``band``."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    n = max(x.shape[0], y.shape[0]) + 1
+    cost = np.empty(n)
+    cost_prev = np.empty(n)
+    if weights is not None:
+        weights = np.ascontiguousarray(weights, dtype=np.float64)
+    return _load().orc_dtw_distance(
+        x.ctypes.data, x.shape[0], y.ctypes.data, y.shape[0], int(band),
+        cost.ctypes.data, cost_prev.ctypes.data, _ptr(weights), float(min_dist))
+
+
+def metric_distance(x, y, metric="dtw", r=1.0, g=0.05, weight_n=0):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    return _load().orc_metric_distance(_kind(metric), float(r), float(g),
+                                       x.ctypes.data, x.shape[0],
+                                       y.ctypes.data, y.shape[0], int(weight_n))
+
+
+def pairwise(X, Y=None, metric="dtw", r=1.0, g=0.05, dim=0, n_threads=None):
+    n_threads = n_threads or default_threads()
+    X = _ts(X)
+    lib = _load()
+    if Y is None:
+        out = np.empty((X.shape[0], X.shape[0]))
+        rc = lib.orc_singleton(_kind(metric), r, g, X.ctypes.data, X.shape[0],
+                               X.shape[1], X.shape[2], dim, out.ctypes.data,
+                               n_threads)
+    else:
+        Y = _ts(Y)
+        out = np.empty((X.shape[0], Y.shape[0]))
+        rc = lib.orc_pairwise(_kind(metric), r, g, X.ctypes.data, X.shape[0],
+                              X.shape[1], X.shape[2], Y.ctypes.data, Y.shape[0],
+                              Y.shape[2], dim, out.ctypes.data, n_threads)
+    if rc != 0:
+        raise MemoryError("oracle allocation failed")
+    return out
+
+
+def paired(X, Y, metric="dtw", r=1.0, g=0.05, dim=0, n_threads=None):
+    """out[i] = D(Y[i], X[i]) -- the reference's argument swap."""
+    n_threads = n_threads or default_threads()
+    X = _ts(X)
+    Y = _ts(Y)
+    out = np.empty(X.shape[0])
+    rc = _load().orc_paired(_kind(metric), r, g, X.ctypes.data, Y.ctypes.data,
+                            X.shape[0], X.shape[1], X.shape[2], dim,
+                            out.ctypes.data, n_threads)
+    if rc != 0:
+        raise MemoryError("oracle allocation failed")
+    return out
+
+
+def argmin(X, Y, k=1, metric="dtw", r=1.0, g=0.05, dim=0, lower_bound=None,
+           n_threads=None):
+    n_threads = n_threads or default_threads()
+    X = _ts(X)
+    Y = _ts(Y)
+    k = min(k, Y.shape[0])
+    idx = np.zeros((X.shape[0], k), dtype=np.int64)
+    dist = np.zeros((X.shape[0], k))
+    if lower_bound is not None:
+        lower_bound = np.ascontiguousarray(lower_bound, dtype=np.float64)
+    rc = _load().orc_argmin(_kind(metric), r, g, X.ctypes.data, X.shape[0],
+                            X.shape[1], X.shape[2], Y.ctypes.data, Y.shape[0],
+                            Y.shape[2], dim, k, _ptr(lower_bound),
+                            idx.ctypes.data, dist.ctypes.data, n_threads)
+    if rc != 0:
+        raise MemoryError("oracle allocation failed")
+    return idx, dist
+
+
+def argmin_replay(row, k, lb_row=None):
+    row = np.ascontiguousarray(row, dtype=np.float64)
+    idx = np.zeros(k, dtype=np.int64)
+    dist = np.zeros(k)
+    if lb_row is not None:
+        lb_row = np.ascontiguousarray(lb_row, dtype=np.float64)
+    _load().orc_argmin_replay(row.ctypes.data, row.shape[0], k, _ptr(lb_row),
+                              idx.ctypes.data, dist.ctypes.data)
+    return idx, dist
+
+
+def pack_shapelets(shapelets):
+    lens = np.array([len(s) for s in shapelets], dtype=np.int64)
+    offs = np.zeros(len(shapelets), dtype=np.int64)
+    if len(shapelets) > 1:
+        offs[1:] = np.cumsum(lens)[:-1]
+    packed = np.concatenate([np.asarray(s, dtype=np.float64) for s in shapelets])
+    return np.ascontiguousarray(packed), offs, lens
+
+
+def subsequence(shapelets, X, metric="dtw", r=1.0, g=0.05, dim=0, n_threads=None):
+    """(dist, idx), each (n_samples, n_shapelets)."""
+    n_threads = n_threads or default_threads()
+    X = _ts(X)
+    S, offs, lens = pack_shapelets(shapelets)
+    n = X.shape[0]
+    dist = np.empty((n, len(shapelets)))
+    idx = np.empty((n, len(shapelets)), dtype=np.int64)
+    rc = _load().orc_subsequence_pairwise(
+        _kind(metric), r, g, S.ctypes.data, offs.ctypes.data, lens.ctypes.data,
+        len(shapelets), X.ctypes.data, n, X.shape[1], X.shape[2], dim,
+        dist.ctypes.data, idx.ctypes.data, n_threads)
+    if rc != 0:
+        raise MemoryError("oracle allocation failed")
+    return dist, idx
+
+
+def cost_matrix(x, y, band, weights=None):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.empty((x.shape[0], y.shape[0]))
+    if weights is not None:
+        weights = np.ascontiguousarray(weights, dtype=np.float64)
+    _load().orc_dtw_cost_matrix(x.ctypes.data, x.shape[0], y.ctypes.data,
+                                y.shape[0], int(band), _ptr(weights),
+                                out.ctypes.data)
+    return out
+
+
+def mapping(A):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0] + A.shape[1]
+    pi = np.empty(n, dtype=np.int64)
+    pj = np.empty(n, dtype=np.int64)
+    m = _load().orc_dtw_mapping(A.ctypes.data, A.shape[0], A.shape[1],
+                                pi.ctypes.data, pj.ctypes.data)
+    return pi[:m][::-1].copy(), pj[:m][::-1].copy()
+
+
+def dtw_grad(x, y, r, weights=None):
+    """(D, dD/dx, dD/dy) along the backtracked optimal path."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    gx = np.zeros(x.shape[0])
+    gy = np.zeros(y.shape[0])
+    if weights is not None:
+        weights = np.ascontiguousarray(weights, dtype=np.float64)
+    D = _load().orc_dtw_grad(x.ctypes.data, x.shape[0], y.ctypes.data,
+                             y.shape[0], float(r), _ptr(weights),
+                             gx.ctypes.data, gy.ctypes.data)
+    return D, gx, gy

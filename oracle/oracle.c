@@ -1,0 +1,602 @@
+/*
+ * oracle.c -- float64 CPU restatement of wildboar's distance hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, the smoke
+ * check in __graft_entry__.py and the cpu_baseline leg of bench.py may load it.
+ * The product (wildboar_amd/, libwbx.so) never links, loads or calls it.
+ *
+ * Every function restates one reference function; the citation names the
+ * reference file:line it follows (paths relative to wildboar's src/wildboar/).
+ * Arithmetic is kept in the reference's order: the per-cell update is
+ *     cost = min(min(up, left), diag) + (v * v) * w,   v = X[i] - Y[j]
+ * with no FMA contraction (build with -ffp-contract=off, no -march=native,
+ * as the reference's default build: setup.py -O2).
+ *
+ * Pinning (what checks this file, see tests/test_oracle.py):
+ *   - dtw_distance: pure-Python masked full-matrix brute force (all lengths
+ *     1..9, unequal lengths, r grid), r=0 == euclidean bitwise, r=1 ==
+ *     textbook unconstrained DTW, D(x,x)=0, symmetry, monotone in r.
+ *   - wdtw: brute force with an explicit weight vector incl. the row-0
+ *     weight-index quirk; g=0 => weights 0.5 => WDTW^2 == 0.5*DTW^2 exactly.
+ *   - euclidean / argmin heap: the argmin_distance docstring known answer
+ *     (distance/_distance.py:1380-1387) and brute-force argpartition.
+ *   - subsequence scans: brute force over windows; euclidean == dtw(r=0);
+ *     pairwise == pairwise_subsequence.T at equal lengths.
+ *   - alignment/backtrack/gradient: central finite differences at tie-free
+ *     points, and the r=0 closed form dD/dx_i = (x_i - y_i) / D.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_EUCLIDEAN 0
+#define ORC_DTW 1
+#define ORC_WDTW 2
+
+static inline int64_t imax(int64_t a, int64_t b) { return a > b ? a : b; }
+static inline int64_t imin(int64_t a, int64_t b) { return a < b ? a : b; }
+/* Cython's min() on C doubles: ((b < a) ? b : a); inputs are never NaN. */
+static inline double dmin(double a, double b) { return b < a ? b : a; }
+
+/* _compute_r, distance/_elastic.pyx:1924-1925 */
+int64_t orc_compute_r(int64_t length, double r) {
+    double v = floor((double)length * r);
+    return (int64_t)(v > 1.0 ? v : 1.0);
+}
+
+/* WeightedDtwMetric.reset, distance/_elastic.pyx:3334-3341 (and the
+ * subsequence variant _elastic.pyx:2368-2378): libc exp, n = array width. */
+void orc_wdtw_weights(int64_t n, double g, double *out) {
+    for (int64_t i = 0; i < n; i++)
+        out[i] = 1.0 / (1.0 + exp(-g * ((double)i - (double)n / 2.0)));
+}
+
+/* dtw_distance, distance/_elastic.pyx:869-940.  Returns the squared cost;
+ * INFINITY when a row minimum exceeds min_dist (early abandon, :933-934). */
+double orc_dtw_distance(const double *X, int64_t x_length, const double *Y,
+                        int64_t y_length, int64_t r, double *cost,
+                        double *cost_prev, const double *weight_vector,
+                        double min_dist) {
+    int64_t i, j, j_start, j_stop;
+    double x, y, z, v, w = 1.0, min_cost, *tmp;
+    int64_t max_len = imax(0, y_length - x_length) + r;
+    int64_t min_len = imax(0, x_length - y_length);
+
+    v = X[0] - Y[0];
+    if (weight_vector != NULL) w = weight_vector[0];
+    cost_prev[0] = v * v * w;
+    for (i = 1; i < imin(y_length, max_len); i++) {
+        v = X[0] - Y[i];
+        if (weight_vector != NULL) w = weight_vector[i - 1]; /* row-0 quirk */
+        cost_prev[i] = cost_prev[i - 1] + v * v * w;
+    }
+    if (max_len < y_length) cost_prev[max_len] = INFINITY;
+
+    for (i = 1; i < x_length; i++) {
+        j_start = imax(0, i - min_len - r + 1);
+        j_stop = imin(y_length, i + max_len);
+        if (j_start > 0) cost[j_start - 1] = INFINITY;
+        min_cost = INFINITY;
+        for (j = j_start; j < j_stop; j++) {
+            x = cost_prev[j];
+            if (j > 0) {
+                y = cost[j - 1];
+                z = cost_prev[j - 1];
+            } else {
+                y = INFINITY;
+                z = INFINITY;
+            }
+            v = X[i] - Y[j];
+            if (weight_vector != NULL) w = weight_vector[labs(i - j)];
+            cost[j] = dmin(dmin(x, y), z) + v * v * w;
+            if (cost[j] < min_cost) min_cost = cost[j];
+        }
+        if (min_cost > min_dist) return INFINITY;
+        if (j_stop < y_length) cost[j_stop] = INFINITY;
+        tmp = cost; cost = cost_prev; cost_prev = tmp;
+    }
+    return cost_prev[y_length - 1];
+}
+
+/* euclidean_distance, distance/_metric.pyx:654-683 (partial-sum abandon). */
+double orc_euclidean_distance(const double *S, int64_t s_length,
+                              const double *T, int64_t t_length,
+                              double min_dist, int64_t *index) {
+    double dist = 0, x;
+    for (int64_t i = 0; i < t_length - s_length + 1; i++) {
+        dist = 0;
+        for (int64_t j = 0; j < s_length; j++) {
+            if (dist >= min_dist) break;
+            x = T[i + j];
+            x -= S[j];
+            dist += x * x;
+        }
+        if (dist < min_dist) {
+            min_dist = dist;
+            if (index != NULL) *index = i;
+        }
+    }
+    return sqrt(min_dist);
+}
+
+/* dtw_subsequence_distance, distance/_elastic.pyx:622-655. */
+double orc_dtw_subsequence_distance(const double *S, int64_t s_length,
+                                    const double *T, int64_t t_length,
+                                    int64_t r, double *cost, double *cost_prev,
+                                    const double *weight_vector,
+                                    int64_t *index) {
+    double dist, min_dist = INFINITY;
+    for (int64_t i = 0; i < t_length - s_length + 1; i++) {
+        dist = orc_dtw_distance(S, s_length, T + i, s_length, r, cost,
+                                cost_prev, weight_vector, min_dist);
+        if (dist < min_dist) {
+            if (index != NULL) *index = i;
+            min_dist = dist;
+        }
+    }
+    return sqrt(min_dist);
+}
+
+/* ------------------------------------------------------------------------
+ * Metric objects: DtwMetric (_elastic.pyx:3126-3217), WeightedDtwMetric
+ * (:3322-3341), EuclideanMetric (_metric.pyx:381-409).  A Metric owns its
+ * scratch rows (reset) and per-array weights.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int kind;
+    double r, g;
+    double *cost, *cost_prev, *weights;
+} orc_metric;
+
+static int metric_reset(orc_metric *m, int64_t tx, int64_t ty) {
+    int64_t n = imax(tx, ty);
+    m->cost = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    m->cost_prev = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    m->weights = NULL;
+    if (m->kind == ORC_WDTW) {
+        m->weights = (double *)malloc(sizeof(double) * (size_t)n);
+        if (m->weights) orc_wdtw_weights(n, m->g, m->weights);
+    }
+    return (m->cost && m->cost_prev && (m->kind != ORC_WDTW || m->weights)) ? 0 : -1;
+}
+
+static void metric_free(orc_metric *m) {
+    free(m->cost); free(m->cost_prev); free(m->weights);
+    m->cost = m->cost_prev = m->weights = NULL;
+}
+
+/* Metric._distance */
+static double metric_distance(orc_metric *m, const double *x, int64_t lx,
+                              const double *y, int64_t ly) {
+    if (m->kind == ORC_EUCLIDEAN)
+        return orc_euclidean_distance(x, lx, y, ly, INFINITY, NULL);
+    return sqrt(orc_dtw_distance(x, lx, y, ly, orc_compute_r(imin(lx, ly), m->r),
+                                 m->cost, m->cost_prev, m->weights, INFINITY));
+}
+
+/* Metric._eadistance: True iff d < *min_dist, then *min_dist = d. */
+static int metric_eadistance(orc_metric *m, const double *x, int64_t lx,
+                             const double *y, int64_t ly, double *min_dist) {
+    double dist;
+    if (m->kind == ORC_EUCLIDEAN)
+        dist = orc_euclidean_distance(x, lx, y, ly, min_dist[0] * min_dist[0], NULL);
+    else
+        dist = sqrt(orc_dtw_distance(x, lx, y, ly, orc_compute_r(imin(lx, ly), m->r),
+                                     m->cost, m->cost_prev, m->weights,
+                                     min_dist[0] * min_dist[0]));
+    if (dist < min_dist[0]) {
+        min_dist[0] = dist;
+        return 1;
+    }
+    return 0;
+}
+
+/* Single-pair entry point used by the tests. */
+double orc_metric_distance(int kind, double r, double g, const double *x,
+                           int64_t lx, const double *y, int64_t ly,
+                           int64_t weight_n) {
+    orc_metric m = {kind, r, g, NULL, NULL, NULL};
+    int64_t n = imax(lx, ly);
+    if (weight_n > n) n = weight_n;
+    if (metric_reset(&m, n, n) != 0) { metric_free(&m); return NAN; }
+    double d = metric_distance(&m, x, lx, y, ly);
+    metric_free(&m);
+    return d;
+}
+
+/* ------------------------------------------------------------------------
+ * Heap, utils/_misc.pyx:18-107: keeps the k smallest (value, index), root is
+ * the largest; replaces the root only when root.value > value.
+ * ---------------------------------------------------------------------- */
+typedef struct { double value; int64_t index; } orc_heap_el;
+
+static void heap_shift_down(orc_heap_el *heap, int64_t startpos, int64_t pos) {
+    orc_heap_el newelement = heap[pos];
+    while (pos > startpos) {
+        int64_t parentpos = (pos - 1) >> 1;
+        if (newelement.value > heap[parentpos].value) {
+            heap[pos] = heap[parentpos];
+            pos = parentpos;
+            continue;
+        }
+        break;
+    }
+    heap[pos] = newelement;
+}
+
+static void heap_shift_up(orc_heap_el *heap, int64_t pos, int64_t endpos) {
+    int64_t startpos = pos;
+    orc_heap_el newelement = heap[pos];
+    int64_t childpos = 2 * pos + 1;
+    while (childpos < endpos) {
+        int64_t rightpos = childpos + 1;
+        if (rightpos < endpos && heap[childpos].value < heap[rightpos].value)
+            childpos = rightpos;
+        heap[pos] = heap[childpos];
+        pos = childpos;
+        childpos = 2 * pos + 1;
+    }
+    heap[pos] = newelement;
+    heap_shift_down(heap, startpos, pos);
+}
+
+static void heap_push(orc_heap_el *heap, int64_t *n, int64_t max_n,
+                      int64_t index, double value) {
+    orc_heap_el e = {value, index};
+    if (*n == 0) {
+        heap[0] = e;
+        *n = 1;
+    } else if (*n < max_n) {
+        heap[*n] = e;
+        *n += 1;
+        heap_shift_down(heap, 0, *n - 1);
+    } else if (heap[0].value > e.value) {
+        heap[0] = e;
+        heap_shift_up(heap, 0, max_n);
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * Batch drivers, distance/_cdistance.pyx, run over contiguous row blocks as
+ * utils/_parallel.py:7-43 (partition_n_jobs) with one thread per block.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int op;                 /* 0 pairwise, 1 singleton, 2 paired, 3 argmin, 4 subseq */
+    int kind;
+    double r, g;
+    const double *X, *Y;
+    int64_t nx, ny, dims, tx, ty, dim;
+    double *out;
+    /* argmin */
+    int64_t k;
+    const double *lower_bound;
+    int64_t *idx;
+    /* subsequence */
+    const double *S;
+    const int64_t *s_off, *s_len;
+    int64_t n_s;
+    /* job */
+    int64_t offset, batch;
+    int status;
+} orc_job;
+
+#define ROW(A, i, dims, t, dim) ((A) + ((int64_t)(i) * (dims) + (dim)) * (t))
+
+static void *run_job(void *arg) {
+    orc_job *jb = (orc_job *)arg;
+    orc_metric m = {jb->kind, jb->r, jb->g, NULL, NULL, NULL};
+    int64_t i, j;
+    jb->status = 0;
+    if (jb->op == 4) {
+        /* _PairwiseSubsequenceDistance.__call__, _cdistance.pyx:1001-1015.
+         * SubsequenceMetric.reset(X) sizes scratch/weights by X.shape[2]. */
+        m.cost = (double *)malloc(sizeof(double) * (size_t)(jb->tx + 1));
+        m.cost_prev = (double *)malloc(sizeof(double) * (size_t)(jb->tx + 1));
+        if (jb->kind == ORC_WDTW) {
+            m.weights = (double *)malloc(sizeof(double) * (size_t)jb->tx);
+            if (m.weights) orc_wdtw_weights(jb->tx, jb->g, m.weights);
+        }
+        if (!m.cost || !m.cost_prev || (jb->kind == ORC_WDTW && !m.weights)) {
+            jb->status = -1;
+            metric_free(&m);
+            return NULL;
+        }
+        for (i = jb->offset; i < jb->offset + jb->batch; i++) {
+            const double *x = ROW(jb->X, i, jb->dims, jb->tx, jb->dim);
+            for (j = 0; j < jb->n_s; j++) {
+                const double *s = jb->S + jb->s_off[j];
+                int64_t sl = jb->s_len[j], min_index = -1;
+                double d;
+                if (jb->kind == ORC_EUCLIDEAN)
+                    /* EuclideanSubsequenceMetric._distance, _metric.pyx:30-41 */
+                    d = orc_euclidean_distance(s, sl, x, jb->tx, INFINITY, &min_index);
+                else
+                    /* DtwSubsequenceMetric._distance, _elastic.pyx:2242-2263 */
+                    d = orc_dtw_subsequence_distance(s, sl, x, jb->tx,
+                                                     orc_compute_r(sl, jb->r), m.cost,
+                                                     m.cost_prev, m.weights, &min_index);
+                jb->out[i * jb->n_s + j] = d;
+                jb->idx[i * jb->n_s + j] = min_index;
+            }
+        }
+        metric_free(&m);
+        return NULL;
+    }
+
+    if (metric_reset(&m, jb->tx, jb->ty) != 0) {
+        jb->status = -1;
+        metric_free(&m);
+        return NULL;
+    }
+    if (jb->op == 0) {
+        /* _PairwiseDistanceBatch.__call__, _cdistance.pyx:1170-1181 */
+        for (i = jb->offset; i < jb->offset + jb->batch; i++)
+            for (j = 0; j < jb->ny; j++)
+                jb->out[i * jb->ny + j] = metric_distance(
+                    &m, ROW(jb->X, i, jb->dims, jb->tx, jb->dim), jb->tx,
+                    ROW(jb->Y, j, jb->dims, jb->ty, jb->dim), jb->ty);
+    } else if (jb->op == 1) {
+        /* _SingletonPairwiseDistanceBatch.__call__, _cdistance.pyx:1231-1245 */
+        for (i = jb->offset; i < jb->offset + jb->batch; i++)
+            for (j = i + 1; j < jb->nx; j++) {
+                double d = metric_distance(
+                    &m, ROW(jb->X, i, jb->dims, jb->tx, jb->dim), jb->tx,
+                    ROW(jb->X, j, jb->dims, jb->tx, jb->dim), jb->tx);
+                jb->out[i * jb->nx + j] = d;
+                jb->out[j * jb->nx + i] = d;
+            }
+    } else if (jb->op == 2) {
+        /* _PairedDistanceBatch.__call__, _cdistance.pyx:1623-1629; the
+         * driver binds its x to the caller's y (:1632-1652): D(Y[i], X[i]). */
+        for (i = jb->offset; i < jb->offset + jb->batch; i++)
+            jb->out[i] = metric_distance(
+                &m, ROW(jb->Y, i, jb->dims, jb->ty, jb->dim), jb->ty,
+                ROW(jb->X, i, jb->dims, jb->tx, jb->dim), jb->tx);
+    } else if (jb->op == 3) {
+        /* _ArgMinBatch.__call__, _cdistance.pyx:1302-1345 */
+        orc_heap_el *heap = (orc_heap_el *)malloc(sizeof(orc_heap_el) * (size_t)jb->k);
+        if (!heap) {
+            jb->status = -1;
+            metric_free(&m);
+            return NULL;
+        }
+        for (i = jb->offset; i < jb->offset + jb->batch; i++) {
+            double distance = INFINITY;
+            int64_t n_el = 0;
+            for (j = 0; j < jb->ny; j++) {
+                if (jb->lower_bound != NULL && jb->lower_bound[i * jb->ny + j] >= distance)
+                    continue;
+                if (metric_eadistance(&m, ROW(jb->X, i, jb->dims, jb->tx, jb->dim), jb->tx,
+                                      ROW(jb->Y, j, jb->dims, jb->ty, jb->dim), jb->ty,
+                                      &distance)) {
+                    heap_push(heap, &n_el, jb->k, j, distance);
+                    distance = (n_el == jb->k) ? heap[0].value : INFINITY;
+                }
+            }
+            /* np.zeros output; a heap that never filled leaves zeros */
+            for (j = 0; j < jb->k; j++) {
+                jb->idx[i * jb->k + j] = (j < n_el) ? heap[j].index : 0;
+                jb->out[i * jb->k + j] = (j < n_el) ? heap[j].value : 0.0;
+            }
+        }
+        free(heap);
+    }
+    metric_free(&m);
+    return NULL;
+}
+
+/* partition_n_jobs + run_in_parallel, utils/_parallel.py:7-43 */
+static int run_parallel(orc_job *proto, int64_t n_work, int n_threads) {
+    if (n_threads < 1) n_threads = 1;
+    if (n_threads > n_work) n_threads = (int)(n_work > 0 ? n_work : 1);
+    orc_job *jobs = (orc_job *)calloc((size_t)n_threads, sizeof(orc_job));
+    pthread_t *th = (pthread_t *)calloc((size_t)n_threads, sizeof(pthread_t));
+    if (!jobs || !th) { free(jobs); free(th); return -1; }
+    int64_t batch = n_work / n_threads, overflow = n_work % n_threads, offset = 0;
+    for (int t = 0; t < n_threads; t++) {
+        jobs[t] = *proto;
+        jobs[t].offset = offset;
+        jobs[t].batch = batch + (t < overflow ? 1 : 0);
+        offset += jobs[t].batch;
+    }
+    int rc = 0;
+    if (n_threads == 1) {
+        run_job(&jobs[0]);
+    } else {
+        for (int t = 0; t < n_threads; t++)
+            if (pthread_create(&th[t], NULL, run_job, &jobs[t]) != 0) {
+                run_job(&jobs[t]);
+                th[t] = 0;
+            }
+        for (int t = 0; t < n_threads; t++)
+            if (th[t]) pthread_join(th[t], NULL);
+    }
+    for (int t = 0; t < n_threads; t++)
+        if (jobs[t].status != 0) rc = -1;
+    free(jobs);
+    free(th);
+    return rc;
+}
+
+/* _pairwise_distance, _cdistance.pyx:1184-1205: out[nx, ny] */
+int orc_pairwise(int kind, double r, double g, const double *X, int64_t nx,
+                 int64_t dims, int64_t tx, const double *Y, int64_t ny,
+                 int64_t ty, int64_t dim, double *out, int n_threads) {
+    orc_job jb;
+    memset(&jb, 0, sizeof jb);
+    jb.op = 0; jb.kind = kind; jb.r = r; jb.g = g;
+    jb.X = X; jb.Y = Y; jb.nx = nx; jb.ny = ny; jb.dims = dims;
+    jb.tx = tx; jb.ty = ty; jb.dim = dim; jb.out = out;
+    return run_parallel(&jb, nx, n_threads);
+}
+
+/* _singleton_pairwise_distance, _cdistance.pyx:1248-1267: zeros + i<j mirror */
+int orc_singleton(int kind, double r, double g, const double *X, int64_t n,
+                  int64_t dims, int64_t t, int64_t dim, double *out,
+                  int n_threads) {
+    orc_job jb;
+    memset(&jb, 0, sizeof jb);
+    memset(out, 0, sizeof(double) * (size_t)(n * n));
+    jb.op = 1; jb.kind = kind; jb.r = r; jb.g = g;
+    jb.X = X; jb.Y = X; jb.nx = n; jb.ny = n; jb.dims = dims;
+    jb.tx = t; jb.ty = t; jb.dim = dim; jb.out = out;
+    return run_parallel(&jb, n, n_threads);
+}
+
+/* _paired_distance, _cdistance.pyx:1632-1652: out[i] = D(Y[i], X[i]) where
+ * X, Y are the caller's (paired_distance's) x and y. */
+int orc_paired(int kind, double r, double g, const double *X, const double *Y,
+               int64_t n, int64_t dims, int64_t t, int64_t dim, double *out,
+               int n_threads) {
+    orc_job jb;
+    memset(&jb, 0, sizeof jb);
+    jb.op = 2; jb.kind = kind; jb.r = r; jb.g = g;
+    jb.X = X; jb.Y = Y; jb.nx = n; jb.ny = n; jb.dims = dims;
+    jb.tx = t; jb.ty = t; jb.dim = dim; jb.out = out;
+    return run_parallel(&jb, n, n_threads);
+}
+
+/* _argmin_distance, _cdistance.pyx:1348-1378: heap order outputs. */
+int orc_argmin(int kind, double r, double g, const double *X, int64_t nx,
+               int64_t dims, int64_t tx, const double *Y, int64_t ny,
+               int64_t ty, int64_t dim, int64_t k, const double *lower_bound,
+               int64_t *idx, double *dist, int n_threads) {
+    orc_job jb;
+    memset(&jb, 0, sizeof jb);
+    jb.op = 3; jb.kind = kind; jb.r = r; jb.g = g;
+    jb.X = X; jb.Y = Y; jb.nx = nx; jb.ny = ny; jb.dims = dims;
+    jb.tx = tx; jb.ty = ty; jb.dim = dim; jb.out = dist; jb.idx = idx;
+    jb.k = k; jb.lower_bound = lower_bound;
+    return run_parallel(&jb, nx, n_threads);
+}
+
+/* _pairwise_subsequence_distance, _cdistance.pyx:1018-1037: dist and idx
+ * are (n_samples, n_shapelets); shapelets are packed back to back. */
+int orc_subsequence_pairwise(int kind, double r, double g, const double *S,
+                             const int64_t *s_off, const int64_t *s_len,
+                             int64_t n_s, const double *X, int64_t n,
+                             int64_t dims, int64_t t, int64_t dim, double *dist,
+                             int64_t *idx, int n_threads) {
+    orc_job jb;
+    memset(&jb, 0, sizeof jb);
+    jb.op = 4; jb.kind = kind; jb.r = r; jb.g = g;
+    jb.X = X; jb.nx = n; jb.dims = dims; jb.tx = t; jb.dim = dim;
+    jb.out = dist; jb.idx = idx; jb.S = S; jb.s_off = s_off; jb.s_len = s_len;
+    jb.n_s = n_s;
+    return run_parallel(&jb, n, n_threads);
+}
+
+/* Heap replay used to check a device-side replay: feed one row of exact
+ * distances through _ArgMinBatch's acceptance rule (_cdistance.pyx:1318-1345)
+ * with the lower-bound skip (:1329-1332). */
+void orc_argmin_replay(const double *row, int64_t ny, int64_t k,
+                       const double *lb_row, int64_t *idx, double *dist) {
+    orc_heap_el *heap = (orc_heap_el *)malloc(sizeof(orc_heap_el) * (size_t)k);
+    double distance = INFINITY;
+    int64_t n_el = 0;
+    for (int64_t j = 0; j < ny; j++) {
+        if (lb_row != NULL && lb_row[j] >= distance) continue;
+        if (row[j] < distance) {
+            heap_push(heap, &n_el, k, j, row[j]);
+            distance = (n_el == k) ? heap[0].value : INFINITY;
+        }
+    }
+    for (int64_t j = 0; j < k; j++) {
+        idx[j] = (j < n_el) ? heap[j].index : 0;
+        dist[j] = (j < n_el) ? heap[j].value : 0.0;
+    }
+    free(heap);
+}
+
+/* ------------------------------------------------------------------------
+ * Backward oracle.
+ *
+ * orc_dtw_cost_matrix: the full banded cost matrix of dtw_distance (same
+ * band, same arithmetic and row-0 weight quirk as _elastic.pyx:869-940),
+ * out-of-band cells set to +inf.  For DTW at equal lengths its in-band cells
+ * equal _dtw_alignment (_elastic.pyx:1011-1073).
+ *
+ * orc_dtw_mapping: dtw_mapping's backtrack (distance/dtw.py:395-423): from
+ * (n-1, m-1), step to argmin([diag, up, left]) with the first minimum winning.
+ *
+ * orc_dtw_grad: gradient of D = sqrt(cost) along that path:
+ *   dD/dx_i = sum_{(i,j) in P} w_ij (x_i - y_j) / D,  dD/dy_j = -sum ... / D,
+ * with w_ij the weight the forward applied to that cell; 0 when D == 0.
+ * ---------------------------------------------------------------------- */
+int orc_dtw_cost_matrix(const double *X, int64_t lx, const double *Y,
+                        int64_t ly, int64_t r, const double *weights,
+                        double *out) {
+    int64_t max_len = imax(0, ly - lx) + r;
+    int64_t min_len = imax(0, lx - ly);
+    double v, w = 1.0;
+    for (int64_t q = 0; q < lx * ly; q++) out[q] = INFINITY;
+    v = X[0] - Y[0];
+    if (weights) w = weights[0];
+    out[0] = v * v * w;
+    for (int64_t j = 1; j < imin(ly, max_len); j++) {
+        v = X[0] - Y[j];
+        if (weights) w = weights[j - 1];
+        out[j] = out[j - 1] + v * v * w;
+    }
+    for (int64_t i = 1; i < lx; i++) {
+        int64_t j_start = imax(0, i - min_len - r + 1);
+        int64_t j_stop = imin(ly, i + max_len);
+        for (int64_t j = j_start; j < j_stop; j++) {
+            double up = out[(i - 1) * ly + j];
+            double left = j > 0 ? out[i * ly + j - 1] : INFINITY;
+            double diag = j > 0 ? out[(i - 1) * ly + j - 1] : INFINITY;
+            v = X[i] - Y[j];
+            if (weights) w = weights[labs(i - j)];
+            out[i * ly + j] = dmin(dmin(up, left), diag) + v * v * w;
+        }
+    }
+    return 0;
+}
+
+int64_t orc_dtw_mapping(const double *A, int64_t lx, int64_t ly,
+                        int64_t *pi, int64_t *pj) {
+    int64_t i = lx - 1, j = ly - 1, n = 0;
+    while (i > 0 || j > 0) {
+        pi[n] = i; pj[n] = j; n++;
+        double o_diag = (i > 0 && j > 0) ? A[(i - 1) * ly + j - 1] : INFINITY;
+        double o_up = (i > 0) ? A[(i - 1) * ly + j] : INFINITY;
+        double o_left = (j > 0) ? A[i * ly + j - 1] : INFINITY;
+        /* np.argmin: first minimum */
+        int move = 0;
+        double best = o_diag;
+        if (o_up < best) { best = o_up; move = 1; }
+        if (o_left < best) { best = o_left; move = 2; }
+        if (move == 0) { i--; j--; }
+        else if (move == 1) { i--; }
+        else { j--; }
+    }
+    pi[n] = 0; pj[n] = 0; n++;
+    return n;
+}
+
+/* Returns D; writes gx[lx], gy[ly] (accumulated, caller zeroes). */
+double orc_dtw_grad(const double *X, int64_t lx, const double *Y, int64_t ly,
+                    double r, const double *weights, double *gx, double *gy) {
+    int64_t band = orc_compute_r(imin(lx, ly), r);
+    double *A = (double *)malloc(sizeof(double) * (size_t)(lx * ly));
+    int64_t *pi = (int64_t *)malloc(sizeof(int64_t) * (size_t)(lx + ly));
+    int64_t *pj = (int64_t *)malloc(sizeof(int64_t) * (size_t)(lx + ly));
+    if (!A || !pi || !pj) { free(A); free(pi); free(pj); return NAN; }
+    orc_dtw_cost_matrix(X, lx, Y, ly, band, weights, A);
+    double D = sqrt(A[lx * ly - 1]);
+    int64_t n = orc_dtw_mapping(A, lx, ly, pi, pj);
+    if (D > 0) {
+        for (int64_t q = 0; q < n; q++) {
+            int64_t i = pi[q], j = pj[q];
+            double w = 1.0;
+            if (weights) w = (i == 0 && j > 0) ? weights[j - 1] : weights[labs(i - j)];
+            double t = w * (X[i] - Y[j]) / D;
+            gx[i] += t;
+            gy[j] -= t;
+        }
+    }
+    free(A); free(pi); free(pj);
+    return D;
+}

@@ -1,0 +1,181 @@
+"""GPU parity: libwbx.so against the float64 oracle on the same seeded inputs.
+
+Bars
+- FP64 mode: bit-identical to the oracle (assert_array_equal), indices exact.
+- FP32 mode: distances within a max relative error of 1e-5 of the oracle
+  (absolute 1e-6 near zero); argmin indices, heap order and returned
+  distances exact (certified re-rank).
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from wildboar_amd import _lib
+from wildboar_amd import distance as wd
+
+pytestmark = pytest.mark.gpu
+
+FP32_RTOL = 1e-5
+
+
+def rw(seed, n, t):
+    rng = np.random.default_rng(seed)
+    x = np.cumsum(rng.standard_normal((n, t)), axis=1)
+    return (x - x.mean(axis=1, keepdims=True)) / x.std(axis=1, keepdims=True)
+
+
+def assert_fp32_close(got, want):
+    got = np.asarray(got)
+    want = np.asarray(want)
+    err = np.abs(got - want) / np.maximum(np.abs(want), 1e-1)
+    assert np.isfinite(got).all()
+    assert err.max() <= FP32_RTOL, float(err.max())
+
+
+def setup_module(module):
+    _lib.set_policy(0)
+
+
+# ----------------------------------------------------------------- FP64 exact
+@pytest.mark.parametrize("metric,params", [
+    ("dtw", {"r": 0.1}), ("dtw", {"r": 0.0}), ("dtw", {"r": 1.0}), ("dtw", {"r": 0.37}),
+    ("wdtw", {"r": 0.1, "g": 0.05}), ("wdtw", {"r": 1.0, "g": 0.5}), ("euclidean", {}),
+])
+def test_pairwise_fp64_bitwise_cfg1(metric, params):
+    X = rw(0, 100, 128)
+    Y = rw(1, 100, 128)
+    got = wd.pairwise_distance(X, Y, metric=metric, metric_params=params)
+    want = oracle.pairwise(X, Y, metric=metric, r=params.get("r", 1.0), g=params.get("g", 0.05))
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("lx,ly,r", [(1, 1, 0.5), (1, 7, 0.2), (9, 1, 0.9), (30, 47, 0.1),
+                                      (47, 30, 0.1), (64, 200, 0.05), (200, 64, 1.0),
+                                      (33, 33, 0.0)])
+@pytest.mark.parametrize("metric", ["dtw", "wdtw"])
+def test_pairwise_fp64_unequal_lengths(lx, ly, r, metric):
+    X = np.random.default_rng(2).standard_normal((5, lx))
+    Y = np.random.default_rng(3).standard_normal((6, ly))
+    got = wd.pairwise_distance(X, Y, metric=metric, metric_params={"r": r})
+    want = oracle.pairwise(X, Y, metric=metric, r=r)
+    np.testing.assert_array_equal(np.atleast_2d(got), want)
+
+
+@pytest.mark.parametrize("metric", ["dtw", "wdtw", "euclidean"])
+def test_singleton_and_paired_fp64(metric):
+    X = rw(4, 37, 60)
+    Y = rw(5, 37, 60)
+    params = {} if metric == "euclidean" else {"r": 0.2}
+    got = wd.pairwise_distance(X, metric=metric, metric_params=params)
+    np.testing.assert_array_equal(got, oracle.pairwise(X, None, metric=metric, r=0.2))
+    got = wd.paired_distance(X, Y, metric=metric, metric_params=params)
+    np.testing.assert_array_equal(got, oracle.paired(X, Y, metric=metric, r=0.2))
+
+
+def test_multivariate_dims_fp64():
+    rng = np.random.default_rng(6)
+    X = rng.standard_normal((7, 3, 40))
+    Y = rng.standard_normal((5, 3, 40))
+    for d in range(3):
+        got = wd.pairwise_distance(X, Y, dim=d, metric="dtw", metric_params={"r": 0.3})
+        np.testing.assert_array_equal(got, oracle.pairwise(X, Y, metric="dtw", r=0.3, dim=d))
+    full = wd.pairwise_distance(X, Y, dim="full", metric="dtw", metric_params={"r": 0.3})
+    assert full.shape == (3, 7, 5)
+    mean = wd.pairwise_distance(X, Y, dim="mean", metric="dtw", metric_params={"r": 0.3})
+    np.testing.assert_allclose(mean, full.mean(axis=0), rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("metric", ["dtw", "wdtw", "euclidean"])
+@pytest.mark.parametrize("k", [1, 3, 7])
+def test_argmin_fp64_heap_order(metric, k):
+    X = rw(7, 40, 64)
+    Y = rw(8, 90, 64)
+    params = {} if metric == "euclidean" else {"r": 0.15}
+    idx, dist = wd.argmin_distance(X, Y, k=k, metric=metric, metric_params=params,
+                                   return_distance=True)
+    oi, od = oracle.argmin(X, Y, k=k, metric=metric, r=0.15)
+    np.testing.assert_array_equal(idx, oi)
+    np.testing.assert_array_equal(dist, od)
+
+
+def test_argmin_lower_bound_and_docstring():
+    X = np.array([[1, 2, 3, 4], [10, 1, 2, 3]], float)
+    Y = np.array([[1, 2, 11, 2], [2, 4, 6, 7], [10, 11, 2, 3]], float)
+    idx, dist = wd.argmin_distance(X, Y, k=2, return_distance=True)
+    assert idx.tolist() == [[0, 1], [1, 2]]
+    np.testing.assert_allclose(dist, [[8.24621125, 4.79583152], [10.24695077, 10.0]], atol=1e-8)
+    Xr = rw(9, 20, 50)
+    Yr = rw(10, 60, 50)
+    lb = np.random.default_rng(11).uniform(0, 3, (20, 60))
+    idx = wd.argmin_distance(Xr, Yr, k=4, metric="dtw", metric_params={"r": 0.1},
+                             lower_bound=lb)
+    oi, _ = oracle.argmin(Xr, Yr, k=4, metric="dtw", r=0.1, lower_bound=lb)
+    np.testing.assert_array_equal(idx, oi)
+
+
+@pytest.mark.parametrize("metric", ["dtw", "wdtw", "euclidean"])
+def test_subsequence_fp64(metric):
+    X = rw(12, 9, 100)
+    rng = np.random.default_rng(13)
+    shapelets = [X[rng.integers(9), s:s + L].copy() for s, L in [(3, 5), (40, 17), (0, 33), (70, 1)]]
+    params = {} if metric == "euclidean" else {"r": 0.1}
+    d, i = wd.pairwise_subsequence_distance(shapelets, X, metric=metric, metric_params=params,
+                                            return_index=True)
+    od, oi = oracle.subsequence(shapelets, X, metric=metric, r=0.1)
+    np.testing.assert_array_equal(d, od)
+    np.testing.assert_array_equal(i, oi)
+
+
+# ----------------------------------------------------------------- FP32
+@pytest.mark.parametrize("n,r", [(128, 0.1), (512, 0.1), (512, 0.05), (256, 0.1), (128, 0.0),
+                                 (100, 0.3), (64, 0.85), (512, 0.2)])
+def test_pairwise_fp32_dtw(n, r):
+    X = rw(20, 64, n)
+    Y = rw(21, 70, n)
+    got = wd.pairwise_distance(X, Y, metric="dtw", metric_params={"r": r}, precision="fp32")
+    want = oracle.pairwise(X, Y, metric="dtw", r=r)
+    assert_fp32_close(got, want)
+
+
+@pytest.mark.parametrize("lx,ly,r", [(100, 130, 0.1), (130, 100, 0.1), (37, 80, 0.2),
+                                      (5, 3, 1.0), (1, 9, 0.5)])
+def test_pairwise_fp32_unequal(lx, ly, r):
+    X = np.random.default_rng(22).standard_normal((33, lx))
+    Y = np.random.default_rng(23).standard_normal((65, ly))
+    got = wd.pairwise_distance(X, Y, metric="dtw", metric_params={"r": r}, precision="fp32")
+    assert_fp32_close(np.atleast_2d(got), oracle.pairwise(X, Y, metric="dtw", r=r))
+
+
+def test_pairwise_fp32_plain_kernel_matches():
+    X = rw(24, 40, 200)
+    Y = rw(25, 50, 200)
+    want = oracle.pairwise(X, Y, metric="dtw", r=0.1)
+    try:
+        _lib.set_policy(1)
+        plain = wd.pairwise_distance(X, Y, metric="dtw", metric_params={"r": 0.1}, precision="fp32")
+    finally:
+        _lib.set_policy(0)
+    assert_fp32_close(plain, want)
+
+
+@pytest.mark.parametrize("metric", ["dtw", "euclidean", "wdtw"])
+@pytest.mark.parametrize("k", [1, 5])
+def test_argmin_fp32_certified_exact(metric, k):
+    X = rw(26, 150, 128)
+    Y = rw(27, 300, 128)
+    params = {} if metric == "euclidean" else {"r": 0.1}
+    idx, dist = wd.argmin_distance(X, Y, k=k, metric=metric, metric_params=params,
+                                   return_distance=True, precision="fp32")
+    oi, od = oracle.argmin(X, Y, k=k, metric=metric, r=0.1)
+    np.testing.assert_array_equal(idx, oi)
+    np.testing.assert_array_equal(dist, od)
+
+
+def test_argmin_fp32_float32_inputs_and_ties():
+    X = rw(28, 50, 96).astype(np.float32)
+    Y = np.repeat(rw(29, 20, 96).astype(np.float32), 3, axis=0)  # exact duplicates -> ties
+    idx = wd.argmin_distance(X, Y, k=1, metric="dtw", metric_params={"r": 0.1}, precision="fp32")
+    oi, _ = oracle.argmin(X.astype(np.float64), Y.astype(np.float64), k=1, metric="dtw", r=0.1)
+    np.testing.assert_array_equal(idx, oi)
+    assert (idx % 3 == 0).all()

@@ -1,0 +1,287 @@
+"""Pins the float64 oracle (oracle/oracle.c) before anything is compared to it.
+
+Independent checks, each chosen so a plausible slip in the restatement fails:
+- a pure-Python masked full-matrix DTW written from the band definition
+  (all lengths 1..8 x 1..8, a grid of r, with and without weights) -- catches
+  band-edge, index and row-0 mistakes;
+- r = 0 is euclidean bit for bit (diagonal-only band, same summation order);
+- r = 1 is the textbook unconstrained DTW;
+- D(x, x) = 0, symmetry of DTW, monotone non-increasing in r;
+- WDTW with g = 0 has every weight exactly 0.5, so WDTW^2 == 0.5 * DTW^2;
+- the argmin_distance docstring known answer (distance/_distance.py:1380-1387);
+- argmin equals a brute-force "k smallest, first index on ties" selection;
+- subsequence scans against brute force over windows, euclidean == dtw(r=0),
+  pairwise == pairwise_subsequence.T at equal lengths
+  (tests/wildboar/distance/test_distance.py:949-958, 1440-1477);
+- the path-backtrack gradient against central finite differences and the
+  r = 0 closed form dD/dx_i = (x_i - y_i) / D.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+
+INF = math.inf
+
+
+def band_w(lx, ly, r):
+    return max(math.floor(min(lx, ly) * r), 1)
+
+
+def brute_dtw(x, y, r, weights=None):
+    """Full-matrix DTW with an explicit band mask, from the definition.
+
+    Cell (i, j) is in the band iff
+      -(max(0, ly-lx) + w - 1) <= i - j <= max(0, lx-ly) + w - 1.
+    Costs outside the band or the matrix are +inf; (0, 0) starts the path;
+    row-0 cells weigh with w[j-1] (distance/_elastic.pyx:898-903), every
+    other cell with w[|i-j|].
+    """
+    lx, ly = len(x), len(y)
+    w = band_w(lx, ly, r)
+    hi = max(0, lx - ly) + w - 1      # i - j <= hi
+    lo = -(max(0, ly - lx) + w - 1)   # i - j >= lo
+    C = [[INF] * ly for _ in range(lx)]
+    for i in range(lx):
+        for j in range(ly):
+            if not (lo <= i - j <= hi):
+                continue
+            v = float(x[i]) - float(y[j])
+            if weights is None:
+                wt = 1.0
+            elif i == 0 and j > 0:
+                wt = float(weights[j - 1])
+            else:
+                wt = float(weights[abs(i - j)])
+            if i == 0 and j == 0:
+                C[i][j] = v * v * wt
+                continue
+            up = C[i - 1][j] if i > 0 else INF
+            left = C[i][j - 1] if j > 0 else INF
+            diag = C[i - 1][j - 1] if (i > 0 and j > 0) else INF
+            m = left if left < up else up
+            m = diag if diag < m else m
+            C[i][j] = m + v * v * wt
+    return C[lx - 1][ly - 1]
+
+
+def textbook_dtw(x, y):
+    n, m = len(x), len(y)
+    D = np.full((n + 1, m + 1), np.inf)
+    D[0, 0] = 0.0
+    for i in range(1, n + 1):
+        for j in range(1, m + 1):
+            D[i, j] = (x[i - 1] - y[j - 1]) ** 2 + min(D[i - 1, j], D[i, j - 1], D[i - 1, j - 1])
+    return D[n, m]
+
+
+def rw(rng, n, t):
+    x = np.cumsum(rng.standard_normal((n, t)), axis=1)
+    x = (x - x.mean(axis=1, keepdims=True)) / x.std(axis=1, keepdims=True)
+    return x
+
+
+R_GRID = [0.0, 0.05, 0.1, 0.25, 0.5, 0.75, 1.0]
+
+
+def test_brute_force_all_small_shapes():
+    rng = np.random.default_rng(1)
+    for lx in range(1, 9):
+        for ly in range(1, 9):
+            x = rng.standard_normal(lx)
+            y = rng.standard_normal(ly)
+            for r in R_GRID:
+                band = oracle.compute_r(min(lx, ly), r)
+                got = oracle.dtw_distance(x, y, band)
+                want = brute_dtw(x, y, r)
+                assert got == want, (lx, ly, r, got, want)
+
+
+def test_brute_force_weighted_with_row0_quirk():
+    rng = np.random.default_rng(2)
+    for lx in range(1, 8):
+        for ly in range(1, 8):
+            x = rng.standard_normal(lx)
+            y = rng.standard_normal(ly)
+            wts = rng.uniform(0.1, 2.0, max(lx, ly))
+            for r in (0.0, 0.3, 1.0):
+                band = oracle.compute_r(min(lx, ly), r)
+                got = oracle.dtw_distance(x, y, band, weights=wts)
+                assert got == brute_dtw(x, y, r, wts), (lx, ly, r)
+
+
+def test_weights_are_libm_formula():
+    for n, g in [(1, 0.05), (7, 0.5), (128, 0.05), (513, 1.3)]:
+        w = oracle.wdtw_weights(n, g)
+        ref = [1.0 / (1.0 + math.exp(-g * (i - n / 2.0))) for i in range(n)]
+        assert w.tolist() == ref
+
+
+def test_r0_is_euclidean_bitwise():
+    rng = np.random.default_rng(3)
+    for t in (1, 2, 17, 128):
+        x, y = rng.standard_normal((2, t))
+        d_dtw = oracle.metric_distance(x, y, "dtw", r=0.0)
+        d_euc = oracle.metric_distance(x, y, "euclidean")
+        assert d_dtw == d_euc
+
+
+def test_r1_is_textbook_dtw():
+    rng = np.random.default_rng(4)
+    for lx, ly in [(5, 5), (9, 4), (3, 11), (32, 32)]:
+        x = rng.standard_normal(lx)
+        y = rng.standard_normal(ly)
+        got = oracle.metric_distance(x, y, "dtw", r=1.0) ** 2
+        assert got == pytest.approx(textbook_dtw(x, y), rel=1e-12)
+
+
+def test_invariants_identity_symmetry_monotone():
+    rng = np.random.default_rng(5)
+    X = rw(rng, 6, 40)
+    for a in range(6):
+        assert oracle.metric_distance(X[a], X[a], "dtw", r=0.3) == 0.0
+        for b in range(6):
+            assert oracle.metric_distance(X[a], X[b], "dtw", r=0.2) == oracle.metric_distance(
+                X[b], X[a], "dtw", r=0.2)
+            prev = INF
+            for r in R_GRID:
+                d = oracle.metric_distance(X[a], X[b], "dtw", r=r)
+                assert d <= prev
+                prev = d
+
+
+def test_wdtw_g0_is_half_dtw_and_asymmetric():
+    rng = np.random.default_rng(6)
+    X = rw(rng, 4, 30)
+    for a in range(4):
+        for b in range(4):
+            dw = oracle.metric_distance(X[a], X[b], "wdtw", r=0.4, g=0.0)
+            dd = oracle.metric_distance(X[a], X[b], "dtw", r=0.4)
+            assert dw * dw == pytest.approx(0.5 * dd * dd, rel=1e-15)
+    asym = [abs(oracle.metric_distance(X[a], X[b], "wdtw", r=0.4, g=0.3)
+                - oracle.metric_distance(X[b], X[a], "wdtw", r=0.4, g=0.3))
+            for a in range(4) for b in range(4)]
+    assert max(asym) > 0.0
+
+
+def test_argmin_docstring_known_answer():
+    X = np.array([[1, 2, 3, 4], [10, 1, 2, 3]], float)
+    Y = np.array([[1, 2, 11, 2], [2, 4, 6, 7], [10, 11, 2, 3]], float)
+    idx, dist = oracle.argmin(X, Y, k=2, metric="euclidean")
+    assert idx.tolist() == [[0, 1], [1, 2]]
+    np.testing.assert_allclose(dist, [[8.24621125, 4.79583152], [10.24695077, 10.0]], atol=1e-8)
+
+
+@pytest.mark.parametrize("metric", ["dtw", "euclidean", "wdtw"])
+def test_argmin_is_k_smallest(metric):
+    rng = np.random.default_rng(7)
+    X = rw(rng, 7, 24)
+    Y = rw(rng, 19, 24)
+    D = oracle.pairwise(X, Y, metric=metric, r=0.2, g=0.1)
+    for k in (1, 3, 5):
+        idx, dist = oracle.argmin(X, Y, k=k, metric=metric, r=0.2, g=0.1)
+        for i in range(7):
+            order = np.argsort(D[i], kind="stable")[:k]
+            assert sorted(idx[i].tolist()) == sorted(order.tolist())
+            np.testing.assert_array_equal(np.sort(dist[i]), np.sort(D[i][order]))
+            ri, rd = oracle.argmin_replay(D[i], k)
+            assert ri.tolist() == idx[i].tolist() and rd.tolist() == dist[i].tolist()
+
+
+def test_argmin_ties_first_index():
+    X = np.zeros((1, 4))
+    Y = np.zeros((5, 4))
+    idx, _ = oracle.argmin(X, Y, k=1, metric="dtw", r=0.5)
+    assert idx.tolist() == [[0]]
+
+
+def test_singleton_and_paired_orientation():
+    rng = np.random.default_rng(8)
+    X = rw(rng, 5, 20)
+    Y = rw(rng, 5, 20)
+    S = oracle.pairwise(X, None, metric="wdtw", r=0.5, g=0.3)
+    for i in range(5):
+        assert S[i, i] == 0.0
+        for j in range(i + 1, 5):
+            d = oracle.metric_distance(X[i], X[j], "wdtw", r=0.5, g=0.3)
+            assert S[i, j] == d and S[j, i] == d
+    P = oracle.paired(X, Y, metric="wdtw", r=0.5, g=0.3)
+    for i in range(5):
+        assert P[i] == oracle.metric_distance(Y[i], X[i], "wdtw", r=0.5, g=0.3)
+
+
+@pytest.mark.parametrize("metric", ["dtw", "euclidean", "wdtw"])
+def test_subsequence_brute_force(metric):
+    rng = np.random.default_rng(9)
+    X = rw(rng, 4, 30)
+    shapelets = [rng.standard_normal(L) for L in (1, 3, 7, 12)]
+    dist, idx = oracle.subsequence(shapelets, X, metric=metric, r=0.25, g=0.2)
+    wts = oracle.wdtw_weights(30, 0.2) if metric == "wdtw" else None
+    for i in range(4):
+        for j, s in enumerate(shapelets):
+            L = len(s)
+            best, bi = INF, 0
+            for w0 in range(30 - L + 1):
+                win = X[i, w0:w0 + L]
+                if metric == "euclidean":
+                    c = 0.0
+                    for q in range(L):
+                        v = win[q] - s[q]
+                        c += v * v
+                else:
+                    c = brute_dtw(s, win, 0.25, wts)
+                if c < best:
+                    best, bi = c, w0
+            assert idx[i, j] == bi
+            assert dist[i, j] == math.sqrt(best)
+
+
+def test_subsequence_euclidean_is_dtw_r0_and_pairwise_transposed():
+    rng = np.random.default_rng(10)
+    X = rw(rng, 5, 25)
+    S = rw(rng, 3, 9)
+    de, ie = oracle.subsequence(list(S), X, metric="euclidean")
+    dd, id_ = oracle.subsequence(list(S), X, metric="dtw", r=0.0)
+    np.testing.assert_array_equal(de, dd)
+    np.testing.assert_array_equal(ie, id_)
+    Z = rw(rng, 3, 25)
+    dsub, _ = oracle.subsequence(list(Z), X, metric="dtw", r=0.3)
+    np.testing.assert_array_equal(dsub, oracle.pairwise(Z, X, metric="dtw", r=0.3).T)
+
+
+def test_gradient_finite_differences_and_closed_form():
+    rng = np.random.default_rng(11)
+    x = rw(rng, 1, 24)[0]
+    y = rw(rng, 1, 24)[0]
+    D, gx, gy = oracle.dtw_grad(x, y, 0.2)
+    assert D == oracle.metric_distance(x, y, "dtw", r=0.2)
+    eps = 1e-6
+    for i in range(24):
+        xp, xm = x.copy(), x.copy()
+        xp[i] += eps
+        xm[i] -= eps
+        fd = (oracle.metric_distance(xp, y, "dtw", r=0.2)
+              - oracle.metric_distance(xm, y, "dtw", r=0.2)) / (2 * eps)
+        assert fd == pytest.approx(gx[i], rel=1e-5, abs=1e-7)
+        yp, ym = y.copy(), y.copy()
+        yp[i] += eps
+        ym[i] -= eps
+        fd = (oracle.metric_distance(x, yp, "dtw", r=0.2)
+              - oracle.metric_distance(x, ym, "dtw", r=0.2)) / (2 * eps)
+        assert fd == pytest.approx(gy[i], rel=1e-5, abs=1e-7)
+    D0, g0x, g0y = oracle.dtw_grad(x, y, 0.0)
+    np.testing.assert_allclose(g0x, (x - y) / D0, rtol=1e-14)
+    np.testing.assert_allclose(g0y, -(x - y) / D0, rtol=1e-14)
+
+
+def test_mapping_tie_rule_prefers_diag_then_up():
+    A = np.array([[0.0, 1.0], [1.0, 0.0]])
+    pi, pj = oracle.mapping(A)
+    assert list(zip(pi, pj)) == [(0, 0), (1, 1)]
+    A = np.array([[0.0, 0.0, 5.0], [0.0, 0.0, 0.0]])
+    pi, pj = oracle.mapping(A)
+    # from (1,2): diag (0,1)=0 ties up (0,2)=5? no: diag 0 < up 5, left (1,1)=0 ties diag -> diag
+    assert list(zip(pi, pj))[-2] == (0, 1)

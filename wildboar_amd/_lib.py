@@ -1,0 +1,122 @@
+"""ctypes binding of libwbx.so (the C ABI declared in include/wbx.h).
+
+The library is the only compute path: when it is missing, or when a call needs
+a GPU and none is present, the call raises -- there is no CPU fallback.
+"""
+
+import ctypes
+import os
+import subprocess
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libwbx.so")
+CSRC = os.path.join(_HERE, "csrc")
+
+WBX_OK, WBX_EINVAL, WBX_ENOMEM, WBX_EDEVICE, WBX_EUNSUPPORTED = 0, -1, -2, -3, -4
+EUCLIDEAN, DTW, WDTW = 0, 1, 2
+FP64, FP32, FP32_ACC64 = 0, 1, 2
+F64, F32 = 0, 1
+
+KINDS = {"euclidean": EUCLIDEAN, "dtw": DTW, "wdtw": WDTW}
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "fp32_acc64": FP32_ACC64}
+
+EXPORTS = (
+    "wbx_init", "wbx_last_error", "wbx_version", "wbx_device_count",
+    "wbx_pairwise", "wbx_paired", "wbx_argmin", "wbx_subsequence_pairwise",
+    "wbx_dev_pairwise", "wbx_dev_paired", "wbx_dev_argmin",
+    "wbx_dev_subsequence_pairwise", "wbx_set_policy", "wbx_prof_enable", "wbx_prof_take",
+)
+
+
+class WbxMetric(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("r", ctypes.c_double), ("g", ctypes.c_double)]
+
+
+class WbxError(RuntimeError):
+    pass
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def build(jobs=8):
+    """Compile every HIP source under csrc/ for gfx950 into lib/libwbx.so."""
+    subprocess.run(["make", "-s", "-C", CSRC, f"-j{jobs}"], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise WbxError(
+                f"{LIB_PATH} is not built; build it with wildboar_amd._lib.build() "
+                "(hipcc --offload-arch=gfx950)")
+        lib = ctypes.CDLL(LIB_PATH)
+        p, i64, i32, d = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+        m = ctypes.POINTER(WbxMetric)
+        sig = {
+            "wbx_init": (i32, [i32]),
+            "wbx_last_error": (ctypes.c_char_p, []),
+            "wbx_version": (ctypes.c_char_p, []),
+            "wbx_device_count": (i32, []),
+            "wbx_set_policy": (i32, [i32]),
+            "wbx_prof_enable": (i32, [i32]),
+            "wbx_prof_take": (i32, [p, p, p]),
+            "wbx_pairwise": (i32, [m, p, i64, i64, i64, p, i64, i64, i64, p]),
+            "wbx_paired": (i32, [m, p, p, i64, i64, i64, i64, p]),
+            "wbx_argmin": (i32, [m, p, i64, i64, i64, p, i64, i64, i64, i64, p, p, p]),
+            "wbx_subsequence_pairwise": (i32, [m, p, p, p, i64, p, i64, i64, i64, i64, p, p]),
+            "wbx_dev_pairwise": (i32, [m, i32, p, i64, i64, i64, p, i64, i64, i64, p, p]),
+            "wbx_dev_paired": (i32, [m, i32, p, i64, p, i64, i64, i64, p, p]),
+            "wbx_dev_argmin": (i32, [m, i32, p, i64, i64, i64, p, i64, i64, i64, i64, p, p, p, p]),
+            "wbx_dev_subsequence_pairwise": (i32, [m, i32, p, p, p, i64, p, i64, i64, i64, p, p, p]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def check(rc):
+    """Raise the Python exception the reference raises for a status code."""
+    if rc == WBX_OK:
+        return
+    msg = load().wbx_last_error().decode("utf-8", "replace")
+    if rc == WBX_EINVAL:
+        raise ValueError(msg)
+    if rc == WBX_ENOMEM:
+        raise MemoryError(msg)
+    if rc == WBX_EUNSUPPORTED:
+        raise NotImplementedError(msg)
+    raise WbxError(msg or f"wbx call failed with status {rc}")
+
+
+def metric_struct(kind, precision, r=1.0, g=0.05):
+    return WbxMetric(int(kind), int(precision), float(r), float(g))
+
+
+def set_policy(policy):
+    """0: fastest kernels (default); 1: the plain one-pair-per-block kernel only."""
+    check(load().wbx_set_policy(int(policy)))
+
+
+def prof_enable(on=True):
+    check(load().wbx_prof_enable(1 if on else 0))
+
+
+def prof_take():
+    """(summed kernel milliseconds, number of kernels, kind of the last one)."""
+    ms = ctypes.c_double(0.0)
+    n = ctypes.c_int64(0)
+    kind = ctypes.c_int(0)
+    check(load().wbx_prof_take(ctypes.byref(ms), ctypes.byref(n), ctypes.byref(kind)))
+    return ms.value, n.value, kind.value

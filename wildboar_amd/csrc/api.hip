@@ -1,0 +1,696 @@
+// api.hip -- the extern "C" entry points declared in include/wbx.h.
+//
+// Dispatch per request:
+//   FP64 (parity)  euclidean -> exact tiled sum; dtw/wdtw -> plain kernel.
+//   FP32           euclidean -> FP32 tiled sum; dtw -> lane-per-pair register
+//                  kernel when the band fits (B <= 112), plain FP32 kernel
+//                  otherwise; wdtw -> plain FP32 kernel.
+//   argmin FP32    FP32 distances -> certified candidate set -> exact FP64
+//                  re-evaluation of the candidates -> heap replay, so indices
+//                  and distances equal the reference's.
+#include <atomic>
+#include <math.h>
+#include <mutex>
+#include <stdlib.h>
+#include <string.h>
+#include <type_traits>
+#include <vector>
+
+#include "wbx_internal.h"
+
+namespace wbx {
+const char *last_error();
+}
+
+using namespace wbx;
+
+namespace {
+
+std::atomic<int> g_policy{0};  // 0 = fastest kernel available, 1 = plain kernel only
+std::mutex g_dev_mu[64];
+
+struct DevGuard {
+    std::unique_lock<std::mutex> lk;
+    explicit DevGuard(int dev) : lk(g_dev_mu[dev & 63]) {}
+};
+
+// Carves one workspace reservation into aligned regions.
+struct Carve {
+    size_t total = 0;
+    size_t take(size_t bytes) {
+        size_t o = total;
+        total += (bytes + 255) & ~(size_t)255;
+        return o;
+    }
+};
+
+int check_metric(const wbx_metric *m) {
+    if (!m) return fail(WBX_EINVAL, "metric is NULL");
+    if (m->kind < WBX_EUCLIDEAN || m->kind > WBX_WDTW)
+        return fail(WBX_EINVAL, "unknown metric kind %d", m->kind);
+    if (m->precision < WBX_FP64 || m->precision > WBX_FP32_ACC64)
+        return fail(WBX_EINVAL, "unknown precision %d", m->precision);
+    if (m->precision == WBX_FP32_ACC64)
+        return fail(WBX_EUNSUPPORTED, "precision FP32_ACC64 is not available in this build");
+    if (m->kind != WBX_EUCLIDEAN && !(m->r >= 0.0 && m->r <= 1.0))
+        return fail(WBX_EINVAL, "r == %g, must be in [0, 1]", m->r);
+    if (m->kind == WBX_WDTW && !(m->g >= 0.0))
+        return fail(WBX_EINVAL, "g == %g, must be >= 0", m->g);
+    return WBX_OK;
+}
+
+hipStream_t pick_stream(DeviceCtx *ctx, void *stream) {
+    return stream ? (hipStream_t)stream : ctx->stream;
+}
+
+// Orders this call after the arena's previous user (possibly on another stream).
+int begin(DeviceCtx *ctx, hipStream_t st) {
+    WBX_HIP_CHECK(hipStreamWaitEvent(st, ctx->last, 0));
+    return WBX_OK;
+}
+int end(DeviceCtx *ctx, hipStream_t st) {
+    WBX_HIP_CHECK(hipEventRecord(ctx->last, st));
+    return WBX_OK;
+}
+
+// WDTW weights 1 / (1 + exp(-g (i - n/2))) with the host libm exp, as
+// WeightedDtwMetric.reset (distance/_elastic.pyx:3334-3341) computes them.
+int get_weights(DeviceCtx *ctx, int64_t n, double g, const double **w64, const float **w32) {
+    if (ctx->w_n != n || ctx->w_g != g) {
+        WBX_HIP_CHECK(hipDeviceSynchronize());
+        if (ctx->w64) (void)hipFree(ctx->w64);
+        if (ctx->w32) (void)hipFree(ctx->w32);
+        ctx->w64 = nullptr;
+        ctx->w32 = nullptr;
+        ctx->w_n = -1;
+        std::vector<double> h((size_t)n);
+        std::vector<float> hf((size_t)n);
+        for (int64_t i = 0; i < n; ++i) {
+            h[i] = 1.0 / (1.0 + exp(-g * ((double)i - (double)n / 2.0)));
+            hf[i] = (float)h[i];
+        }
+        if (hipMalloc(&ctx->w64, sizeof(double) * n) != hipSuccess ||
+            hipMalloc(&ctx->w32, sizeof(float) * n) != hipSuccess)
+            return fail(WBX_ENOMEM, "weights allocation failed");
+        WBX_HIP_CHECK(hipMemcpy(ctx->w64, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        WBX_HIP_CHECK(hipMemcpy(ctx->w32, hf.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+        ctx->w_n = n;
+        ctx->w_g = g;
+    }
+    *w64 = ctx->w64;
+    *w32 = ctx->w32;
+    return WBX_OK;
+}
+
+template <typename T>
+Series<T> series(const T *base, int64_t stride, int64_t len) {
+    Series<T> s;
+    s.base = base;
+    s.stride = stride;
+    s.len = (int)len;
+    s.lens = nullptr;
+    return s;
+}
+
+// Optional timing of the distance kernels: HIP events recorded on the launch
+// stream around each one, summed on request (wbx_prof_take).
+struct ProfState {
+    std::mutex mu;
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    std::vector<int> kind;
+} g_prof;
+
+struct ProfScope {
+    hipEvent_t a = nullptr, b = nullptr;
+    hipStream_t st;
+    int k;
+    ProfScope(hipStream_t s, int kind_) : st(s), k(kind_) {
+        std::lock_guard<std::mutex> lk(g_prof.mu);
+        if (!g_prof.on) return;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            a = b = nullptr;
+            return;
+        }
+        (void)hipEventRecord(a, st);
+    }
+    ~ProfScope() {
+        if (!a) return;
+        (void)hipEventRecord(b, st);
+        std::lock_guard<std::mutex> lk(g_prof.mu);
+        g_prof.ev.push_back({a, b});
+        g_prof.kind.push_back(k);
+    }
+};
+
+enum { KERN_EUCLID = 1, KERN_LANE = 2, KERN_PLAIN = 3 };
+
+template <typename T>
+int run_pairs(const wbx_metric *m, const PairSpec &ps, Series<T> xs, Series<T> ys,
+              const T *weights, T *sq, hipStream_t st) {
+    if (m->kind == WBX_EUCLIDEAN) {
+        ProfScope pf(st, KERN_EUCLID);
+        return launch_euclid_pairs<T>(ps, xs, ys, sq, st);
+    }
+    if constexpr (std::is_same<T, float>::value) {
+        if (g_policy.load() == 0 && !weights && !xs.lens && !ys.lens &&
+            lane_supported(xs.len, ys.len, m->r, 0)) {
+            ProfScope pf(st, KERN_LANE);
+            return launch_lane_dtw(ps, xs, ys, m->r, nullptr, sq, st);
+        }
+    }
+    ProfScope pf(st, KERN_PLAIN);
+    return launch_plain_dtw<T>(ps, xs, ys, m->r, weights, sq, st);
+}
+
+// Inputs of element type `dtype` brought to the compute type T (dense copy
+// when a conversion is needed, the caller's memory otherwise).
+template <typename T>
+struct Staged {
+    const T *ptr;
+    int64_t stride;
+};
+
+template <typename T>
+int stage(int dtype, const void *src, int64_t n, int64_t len, int64_t stride, char *ws,
+          size_t off, Staged<T> *out, hipStream_t st) {
+    const bool same = (dtype == WBX_F64) == std::is_same<T, double>::value;
+    if (same) {
+        out->ptr = (const T *)src;
+        out->stride = stride;
+        return WBX_OK;
+    }
+    T *dst = (T *)(ws + off);
+    int rc = (dtype == WBX_F64)
+                 ? launch_convert_rows<double, T>((const double *)src, n, len, stride, dst, len, st)
+                 : launch_convert_rows<float, T>((const float *)src, n, len, stride, dst, len, st);
+    out->ptr = dst;
+    out->stride = len;
+    return rc;
+}
+
+size_t stage_bytes(int dtype, bool want_double, int64_t n, int64_t len) {
+    const bool same = (dtype == WBX_F64) == want_double;
+    return same ? 0 : (size_t)n * (size_t)len * (want_double ? 8 : 4);
+}
+
+#define RC(x)              \
+    do {                   \
+        int _rc = (x);     \
+        if (_rc) return _rc; \
+    } while (0)
+
+// -------------------------------------------------------------- pairwise
+template <typename T>
+int pairwise_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *X, int64_t nx,
+                  int64_t xstr, int64_t tx, const void *Y, int64_t ny, int64_t ystr, int64_t ty,
+                  bool singleton, double *out, hipStream_t st) {
+    PairSpec ps{};
+    if (singleton) {
+        ps.mode = PAIR_TRIANGLE;
+        ps.ny = nx;
+        ps.n_pairs = nx * (nx - 1) / 2;
+    } else {
+        ps.mode = PAIR_GRID;
+        ps.ny = ny;
+        ps.n_pairs = nx * ny;
+    }
+    const bool dbl = std::is_same<T, double>::value;
+    Carve cv;
+    const size_t o_x = cv.take(stage_bytes(dtype, dbl, nx, tx));
+    const size_t o_y = singleton ? 0 : cv.take(stage_bytes(dtype, dbl, ny, ty));
+    const size_t o_sq = cv.take((size_t)ps.n_pairs * sizeof(T));
+    void *wsp = nullptr;
+    RC(ws_reserve(ctx, cv.total, &wsp));
+    char *ws = (char *)wsp;
+    Staged<T> sx, sy;
+    RC(stage<T>(dtype, X, nx, tx, xstr, ws, o_x, &sx, st));
+    if (singleton)
+        sy = sx;
+    else
+        RC(stage<T>(dtype, Y, ny, ty, ystr, ws, o_y, &sy, st));
+    const T *w = nullptr;
+    if (m->kind == WBX_WDTW) {
+        const double *w64;
+        const float *w32;
+        RC(get_weights(ctx, tx > ty ? tx : ty, m->g, &w64, &w32));
+        w = dbl ? (const T *)w64 : (const T *)w32;
+    }
+    T *sq = (T *)(ws + o_sq);
+    RC(run_pairs<T>(m, ps, series<T>(sx.ptr, sx.stride, tx), series<T>(sy.ptr, sy.stride, ty), w,
+                    sq, st));
+    if (singleton) RC(launch_fill_f64(out, nx * nx, 0.0, st));
+    return launch_finish<T>(ps, sq, out, singleton ? nx : ny, st);
+}
+
+// -------------------------------------------------------------- argmin
+int argmin_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *X, int64_t nx,
+                int64_t xstr, int64_t tx, const void *Y, int64_t ny, int64_t ystr, int64_t ty,
+                int64_t k, const double *lb, int64_t *idx, double *dist, hipStream_t st) {
+    const int64_t np = nx * ny;
+    const bool certified = m->precision == WBX_FP32 && k <= 32;
+    Carve cv;
+    const size_t o_x64 = cv.take(stage_bytes(dtype, true, nx, tx));
+    const size_t o_y64 = cv.take(stage_bytes(dtype, true, ny, ty));
+    const size_t o_d64 = cv.take((size_t)np * 8);
+    const size_t o_heap = cv.take((size_t)nx * (size_t)k * 16);
+    size_t o_x32 = 0, o_y32 = 0, o_sq32 = 0, o_cnt = 0, o_off = 0, o_pa = 0, o_pb = 0, o_sq64 = 0,
+           o_mx = 0;
+    if (certified) {
+        o_x32 = cv.take(stage_bytes(dtype, false, nx, tx));
+        o_y32 = cv.take(stage_bytes(dtype, false, ny, ty));
+        o_sq32 = cv.take((size_t)np * 4);
+        o_cnt = cv.take((size_t)nx * 8);
+        o_off = cv.take((size_t)(nx + 1) * 8);
+        o_pa = cv.take((size_t)np * 4);
+        o_pb = cv.take((size_t)np * 4);
+        o_mx = cv.take(8);
+    } else {
+        o_sq64 = cv.take((size_t)np * 8);
+    }
+    void *wsp = nullptr;
+    RC(ws_reserve(ctx, cv.total, &wsp));
+    char *ws = (char *)wsp;
+    Staged<double> x64, y64;
+    RC(stage<double>(dtype, X, nx, tx, xstr, ws, o_x64, &x64, st));
+    RC(stage<double>(dtype, Y, ny, ty, ystr, ws, o_y64, &y64, st));
+    const double *w64 = nullptr;
+    const float *w32 = nullptr;
+    if (m->kind == WBX_WDTW) RC(get_weights(ctx, tx > ty ? tx : ty, m->g, &w64, &w32));
+    double *D64 = (double *)(ws + o_d64);
+    if (!certified) {
+        PairSpec ps{PAIR_GRID, np, ny, nullptr, nullptr};
+        double *sq = (double *)(ws + o_sq64);
+        wbx_metric m64 = *m;
+        m64.precision = WBX_FP64;
+        RC(run_pairs<double>(&m64, ps, series<double>(x64.ptr, x64.stride, tx),
+                             series<double>(y64.ptr, y64.stride, ty), w64, sq, st));
+        RC(launch_finish<double>(ps, sq, D64, ny, st));
+    } else {
+        Staged<float> x32, y32;
+        RC(stage<float>(dtype, X, nx, tx, xstr, ws, o_x32, &x32, st));
+        RC(stage<float>(dtype, Y, ny, ty, ystr, ws, o_y32, &y32, st));
+        float *sq32 = (float *)(ws + o_sq32);
+        PairSpec ps{PAIR_GRID, np, ny, nullptr, nullptr};
+        RC(run_pairs<float>(m, ps, series<float>(x32.ptr, x32.stride, tx),
+                            series<float>(y32.ptr, y32.stride, ty), w32, sq32, st));
+        double absmax = 0.0;
+        const bool exact_inputs = dtype == WBX_F32;
+        if (!exact_inputs) {
+            unsigned *mx = (unsigned *)(ws + o_mx);
+            WBX_HIP_CHECK(hipMemsetAsync(mx, 0, 4, st));
+            RC(launch_absmax_rows<double>((const double *)X, nx, tx, xstr, mx, st));
+            RC(launch_absmax_rows<double>((const double *)Y, ny, ty, ystr, mx, st));
+            unsigned hb = 0;
+            WBX_HIP_CHECK(hipMemcpyAsync(&hb, mx, 4, hipMemcpyDeviceToHost, st));
+            WBX_HIP_CHECK(hipStreamSynchronize(st));
+            float f;
+            memcpy(&f, &hb, 4);
+            absmax = (double)f;
+        }
+        ErrBound eb = make_err_bound((int)tx, (int)ty, m->kind == WBX_WDTW, absmax, exact_inputs);
+        int64_t total = 0;
+        int32_t *pa = (int32_t *)(ws + o_pa), *pb = (int32_t *)(ws + o_pb);
+        RC(launch_argmin_candidates(sq32, nx, ny, (int)k, eb, (int64_t *)(ws + o_cnt),
+                                    (int64_t *)(ws + o_off), pa, pb, &total, st));
+        RC(launch_fill_f64(D64, np, WBX_INF_D, st));
+        // exact FP64 values for the candidates; sq64 reuses the sq32+cnt area
+        // only if large enough, so give it the pa/pb-sized tail instead.
+        double *sq64 = (double *)(ws + o_sq32);  // np*4 bytes >= total*8 when total <= np/2
+        double *tmp = nullptr;
+        if ((size_t)total * 8 > (size_t)np * 4) {
+            if (hipMalloc(&tmp, (size_t)total * 8) != hipSuccess)
+                return fail(WBX_ENOMEM, "candidate buffer allocation failed");
+            sq64 = tmp;
+        }
+        PairSpec pl{PAIR_LIST, total, ny, pa, pb};
+        wbx_metric m64 = *m;
+        m64.precision = WBX_FP64;
+        int rc = run_pairs<double>(&m64, pl, series<double>(x64.ptr, x64.stride, tx),
+                                   series<double>(y64.ptr, y64.stride, ty), w64, sq64, st);
+        if (!rc) rc = launch_finish<double>(pl, sq64, D64, ny, st);
+        if (tmp) {
+            (void)hipStreamSynchronize(st);
+            (void)hipFree(tmp);
+        }
+        if (rc) return rc;
+    }
+    return launch_argmin_replay(D64, nx, ny, k, lb, ws + o_heap, idx, dist, st);
+}
+
+// -------------------------------------------------------------- subsequence
+int subseq_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
+                const int64_t *s_off, const int64_t *s_len, int64_t n_s, const void *X,
+                int64_t n, int64_t xstr, int64_t t, double *dist, int64_t *idx, hipStream_t st) {
+    // Shapelet lengths drive the LDS size: read them back.
+    std::vector<int64_t> hl((size_t)n_s), ho((size_t)n_s);
+    WBX_HIP_CHECK(hipMemcpyAsync(hl.data(), s_len, sizeof(int64_t) * n_s, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(ho.data(), s_off, sizeof(int64_t) * n_s, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));
+    int64_t max_len = 0, total = 0;
+    for (int64_t j = 0; j < n_s; ++j) {
+        if (hl[j] < 1 || hl[j] > t)
+            return fail(WBX_EINVAL, "Invalid subsequnce shape (%lld > %lld)", (long long)hl[j],
+                        (long long)t);
+        if (hl[j] > max_len) max_len = hl[j];
+        if (ho[j] + hl[j] > total) total = ho[j] + hl[j];
+    }
+    // Exact path (FP64 arithmetic on float64-upcast values).
+    Carve cv;
+    const size_t o_x = cv.take(stage_bytes(dtype, true, n, t));
+    const size_t o_s = cv.take(stage_bytes(dtype, true, 1, total));
+    void *wsp = nullptr;
+    RC(ws_reserve(ctx, cv.total, &wsp));
+    char *ws = (char *)wsp;
+    Staged<double> x64, s64;
+    RC(stage<double>(dtype, X, n, t, xstr, ws, o_x, &x64, st));
+    RC(stage<double>(dtype, S, 1, total, total, ws, o_s, &s64, st));
+    const double *w64 = nullptr;
+    const float *w32 = nullptr;
+    if (m->kind == WBX_WDTW) RC(get_weights(ctx, t, m->g, &w64, &w32));
+    Shapelets sh{s64.ptr, s_off, s_len, n_s};
+    return launch_subseq_exact<double>(m->kind, m->r, sh, (int)max_len,
+                                       series<double>(x64.ptr, x64.stride, t), n, w64, dist, idx,
+                                       st);
+}
+
+// Host-pointer wrapper helpers.
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t bytes) {
+        if (bytes == 0) bytes = 8;
+        if (hipMalloc(&p, bytes) != hipSuccess)
+            return fail(WBX_ENOMEM, "device allocation of %zu bytes failed", bytes);
+        return WBX_OK;
+    }
+};
+
+int upload(DevBuf &b, const void *h, size_t bytes, hipStream_t st) {
+    RC(b.alloc(bytes));
+    if (h && bytes) WBX_HIP_CHECK(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, st));
+    return WBX_OK;
+}
+
+int check_sizes(int64_t nx, int64_t tx) {
+    if (nx < 1) return fail(WBX_EINVAL, "need at least one series, got %lld", (long long)nx);
+    if (tx < 1) return fail(WBX_EINVAL, "need at least one timestep, got %lld", (long long)tx);
+    if (tx > (1 << 20)) return fail(WBX_EUNSUPPORTED, "series longer than 2^20");
+    return WBX_OK;
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+const char *wbx_last_error(void) { return last_error(); }
+
+const char *wbx_version(void) { return "wbx 0.1.0 (gfx950)"; }
+
+int wbx_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int wbx_init(int device) {
+    WBX_HIP_CHECK(hipSetDevice(device));
+    return ctx_current() ? WBX_OK : WBX_EDEVICE;
+}
+
+// Not part of the reference interface: selects the kernel family (0 = the
+// fastest available, 1 = the plain one-pair-per-block kernel everywhere).
+int wbx_set_policy(int policy) {
+    if (policy < 0 || policy > 1) return fail(WBX_EINVAL, "policy must be 0 or 1");
+    g_policy.store(policy);
+    return WBX_OK;
+}
+
+// Not part of the reference interface: kernel timing for benchmarks.
+// wbx_prof_enable(1) starts recording HIP events around every distance kernel;
+// wbx_prof_take waits for them and returns the summed milliseconds, the number
+// of launches and the kind of the last one (1 euclid, 2 lane, 3 plain), then
+// clears the record.
+int wbx_prof_enable(int on) {
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    g_prof.on = on != 0;
+    return WBX_OK;
+}
+
+int wbx_prof_take(double *total_ms, int64_t *launches, int *last_kind) {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    std::vector<int> kinds;
+    {
+        std::lock_guard<std::mutex> lk(g_prof.mu);
+        ev.swap(g_prof.ev);
+        kinds.swap(g_prof.kind);
+    }
+    double tot = 0.0;
+    int rc = WBX_OK;
+    for (auto &e : ev) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(e.second) != hipSuccess ||
+            hipEventElapsedTime(&ms, e.first, e.second) != hipSuccess)
+            rc = fail(WBX_EDEVICE, "event timing failed");
+        tot += ms;
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = (int64_t)ev.size();
+    if (last_kind) *last_kind = kinds.empty() ? 0 : kinds.back();
+    return rc;
+}
+
+int wbx_dev_pairwise(const wbx_metric *metric, int dtype, const void *X, int64_t nx,
+                     int64_t x_stride, int64_t tx, const void *Y, int64_t ny, int64_t y_stride,
+                     int64_t ty, double *out, void *stream) {
+    RC(check_metric(metric));
+    RC(check_sizes(nx, tx));
+    const bool singleton = (Y == nullptr);
+    if (singleton) {
+        ny = nx;
+        ty = tx;
+        y_stride = x_stride;
+    } else {
+        RC(check_sizes(ny, ty));
+    }
+    if (metric->kind == WBX_EUCLIDEAN && !singleton && tx != ty)
+        return fail(WBX_EINVAL, "Illegal n_timestep (%lld != %lld) for non-elastic distance measure",
+                    (long long)tx, (long long)ty);
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    DevGuard g(ctx->device);
+    hipStream_t st = pick_stream(ctx, stream);
+    RC(begin(ctx, st));
+    int rc = metric->precision == WBX_FP64
+                 ? pairwise_impl<double>(ctx, metric, dtype, X, nx, x_stride, tx, Y, ny, y_stride,
+                                         ty, singleton, out, st)
+                 : pairwise_impl<float>(ctx, metric, dtype, X, nx, x_stride, tx, Y, ny, y_stride,
+                                        ty, singleton, out, st);
+    int rc2 = end(ctx, st);
+    return rc ? rc : rc2;
+}
+
+int wbx_dev_paired(const wbx_metric *metric, int dtype, const void *X, int64_t x_stride,
+                   const void *Y, int64_t y_stride, int64_t n, int64_t t, double *out,
+                   void *stream) {
+    RC(check_metric(metric));
+    RC(check_sizes(n, t));
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    DevGuard g(ctx->device);
+    hipStream_t st = pick_stream(ctx, stream);
+    RC(begin(ctx, st));
+    // _paired_distance binds its rows to the caller's y: out[i] = D(Y[i], X[i]).
+    PairSpec ps{PAIR_DIAG, n, n, nullptr, nullptr};
+    int rc;
+    const bool dbl = metric->precision == WBX_FP64;
+    Carve cv;
+    const size_t o_x = cv.take(stage_bytes(dtype, dbl, n, t));
+    const size_t o_y = cv.take(stage_bytes(dtype, dbl, n, t));
+    const size_t o_sq = cv.take((size_t)n * 8);
+    void *wsp = nullptr;
+    rc = ws_reserve(ctx, cv.total, &wsp);
+    if (!rc) {
+        char *ws = (char *)wsp;
+        const double *w64 = nullptr;
+        const float *w32 = nullptr;
+        if (metric->kind == WBX_WDTW) rc = get_weights(ctx, t, metric->g, &w64, &w32);
+        if (!rc && dbl) {
+            Staged<double> sx, sy;
+            rc = stage<double>(dtype, X, n, t, x_stride, ws, o_x, &sx, st);
+            if (!rc) rc = stage<double>(dtype, Y, n, t, y_stride, ws, o_y, &sy, st);
+            if (!rc)
+                rc = run_pairs<double>(metric, ps, series<double>(sy.ptr, sy.stride, t),
+                                       series<double>(sx.ptr, sx.stride, t), w64,
+                                       (double *)(ws + o_sq), st);
+            if (!rc) rc = launch_finish<double>(ps, (double *)(ws + o_sq), out, n, st);
+        } else if (!rc) {
+            Staged<float> sx, sy;
+            rc = stage<float>(dtype, X, n, t, x_stride, ws, o_x, &sx, st);
+            if (!rc) rc = stage<float>(dtype, Y, n, t, y_stride, ws, o_y, &sy, st);
+            if (!rc)
+                rc = run_pairs<float>(metric, ps, series<float>(sy.ptr, sy.stride, t),
+                                      series<float>(sx.ptr, sx.stride, t), w32,
+                                      (float *)(ws + o_sq), st);
+            if (!rc) rc = launch_finish<float>(ps, (float *)(ws + o_sq), out, n, st);
+        }
+    }
+    int rc2 = end(ctx, st);
+    return rc ? rc : rc2;
+}
+
+int wbx_dev_argmin(const wbx_metric *metric, int dtype, const void *X, int64_t nx,
+                   int64_t x_stride, int64_t tx, const void *Y, int64_t ny, int64_t y_stride,
+                   int64_t ty, int64_t k, const double *lower_bound, int64_t *idx, double *dist,
+                   void *stream) {
+    RC(check_metric(metric));
+    RC(check_sizes(nx, tx));
+    RC(check_sizes(ny, ty));
+    if (k < 1) return fail(WBX_EINVAL, "k must be >= 1, got %lld", (long long)k);
+    if (k > ny) k = ny;
+    if (metric->kind == WBX_EUCLIDEAN && tx != ty)
+        return fail(WBX_EINVAL, "Illegal n_timestep (%lld != %lld) for non-elastic distance measure.",
+                    (long long)tx, (long long)ty);
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    DevGuard g(ctx->device);
+    hipStream_t st = pick_stream(ctx, stream);
+    RC(begin(ctx, st));
+    int rc = argmin_impl(ctx, metric, dtype, X, nx, x_stride, tx, Y, ny, y_stride, ty, k,
+                         lower_bound, idx, dist, st);
+    int rc2 = end(ctx, st);
+    return rc ? rc : rc2;
+}
+
+int wbx_dev_subsequence_pairwise(const wbx_metric *metric, int dtype, const void *S,
+                                 const int64_t *s_off, const int64_t *s_len, int64_t n_s,
+                                 const void *X, int64_t n, int64_t x_stride, int64_t t,
+                                 double *dist, int64_t *idx, void *stream) {
+    RC(check_metric(metric));
+    RC(check_sizes(n, t));
+    if (n_s < 1) return fail(WBX_EINVAL, "Subsequence y cannot be empty.");
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    DevGuard g(ctx->device);
+    hipStream_t st = pick_stream(ctx, stream);
+    RC(begin(ctx, st));
+    int rc = subseq_impl(ctx, metric, dtype, S, s_off, s_len, n_s, X, n, x_stride, t, dist, idx,
+                         st);
+    int rc2 = end(ctx, st);
+    return rc ? rc : rc2;
+}
+
+// ------------------------------------------------------------ host drop-ins
+int wbx_pairwise(const wbx_metric *metric, const double *X, int64_t nx, int64_t dims, int64_t tx,
+                 const double *Y, int64_t ny, int64_t ty, int64_t dim, double *out) {
+    RC(check_metric(metric));
+    RC(check_sizes(nx, tx));
+    if (dims < 1 || dim < 0 || dim >= dims)
+        return fail(WBX_EINVAL, "The parameter dim must be 0 <= dim < n_dims");
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    hipStream_t st = ctx->stream;
+    const bool singleton = Y == nullptr;
+    DevBuf dx, dy, dout;
+    RC(upload(dx, X, sizeof(double) * nx * dims * tx, st));
+    if (!singleton) RC(upload(dy, Y, sizeof(double) * ny * dims * ty, st));
+    const int64_t ocols = singleton ? nx : ny;
+    RC(dout.alloc(sizeof(double) * nx * ocols));
+    RC(wbx_dev_pairwise(metric, WBX_F64, (const double *)dx.p + dim * tx, nx, dims * tx, tx,
+                        singleton ? nullptr : (const double *)dy.p + dim * ty, ny, dims * ty, ty,
+                        (double *)dout.p, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(out, dout.p, sizeof(double) * nx * ocols, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));
+    return WBX_OK;
+}
+
+int wbx_paired(const wbx_metric *metric, const double *X, const double *Y, int64_t n,
+               int64_t dims, int64_t t, int64_t dim, double *out) {
+    RC(check_metric(metric));
+    RC(check_sizes(n, t));
+    if (dims < 1 || dim < 0 || dim >= dims)
+        return fail(WBX_EINVAL, "The parameter dim must be 0 <= dim < n_dims");
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    hipStream_t st = ctx->stream;
+    DevBuf dx, dy, dout;
+    RC(upload(dx, X, sizeof(double) * n * dims * t, st));
+    RC(upload(dy, Y, sizeof(double) * n * dims * t, st));
+    RC(dout.alloc(sizeof(double) * n));
+    RC(wbx_dev_paired(metric, WBX_F64, (const double *)dx.p + dim * t, dims * t,
+                      (const double *)dy.p + dim * t, dims * t, n, t, (double *)dout.p, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(out, dout.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));
+    return WBX_OK;
+}
+
+int wbx_argmin(const wbx_metric *metric, const double *X, int64_t nx, int64_t dims, int64_t tx,
+               const double *Y, int64_t ny, int64_t ty, int64_t dim, int64_t k,
+               const double *lower_bound, int64_t *idx, double *dist) {
+    RC(check_metric(metric));
+    RC(check_sizes(nx, tx));
+    RC(check_sizes(ny, ty));
+    if (dims < 1 || dim < 0 || dim >= dims)
+        return fail(WBX_EINVAL, "The parameter dim must be 0 <= dim < n_dims");
+    if (k < 1) return fail(WBX_EINVAL, "k must be >= 1, got %lld", (long long)k);
+    if (k > ny) k = ny;
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    hipStream_t st = ctx->stream;
+    DevBuf dx, dy, dlb, didx, ddist;
+    RC(upload(dx, X, sizeof(double) * nx * dims * tx, st));
+    RC(upload(dy, Y, sizeof(double) * ny * dims * ty, st));
+    if (lower_bound) RC(upload(dlb, lower_bound, sizeof(double) * nx * ny, st));
+    RC(didx.alloc(sizeof(int64_t) * nx * k));
+    RC(ddist.alloc(sizeof(double) * nx * k));
+    RC(wbx_dev_argmin(metric, WBX_F64, (const double *)dx.p + dim * tx, nx, dims * tx, tx,
+                      (const double *)dy.p + dim * ty, ny, dims * ty, ty, k,
+                      lower_bound ? (const double *)dlb.p : nullptr, (int64_t *)didx.p,
+                      (double *)ddist.p, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(idx, didx.p, sizeof(int64_t) * nx * k, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(dist, ddist.p, sizeof(double) * nx * k, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));
+    return WBX_OK;
+}
+
+int wbx_subsequence_pairwise(const wbx_metric *metric, const double *S, const int64_t *s_off,
+                             const int64_t *s_len, int64_t n_s, const double *X, int64_t n,
+                             int64_t dims, int64_t t, int64_t dim, double *dist, int64_t *idx) {
+    RC(check_metric(metric));
+    RC(check_sizes(n, t));
+    if (n_s < 1) return fail(WBX_EINVAL, "Subsequence y cannot be empty.");
+    if (dims < 1 || dim < 0 || dim >= dims)
+        return fail(WBX_EINVAL, "The parameter dim must be 0 <= dim < n_dims");
+    int64_t total = 0;
+    for (int64_t j = 0; j < n_s; ++j) {
+        if (s_len[j] < 1 || s_len[j] > t)
+            return fail(WBX_EINVAL, "Invalid subsequnce shape (%lld > %lld)", (long long)s_len[j],
+                        (long long)t);
+        if (s_off[j] < 0) return fail(WBX_EINVAL, "negative shapelet offset");
+        if (s_off[j] + s_len[j] > total) total = s_off[j] + s_len[j];
+    }
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    hipStream_t st = ctx->stream;
+    DevBuf dx, ds, doff, dlen, ddist, didx;
+    RC(upload(dx, X, sizeof(double) * n * dims * t, st));
+    RC(upload(ds, S, sizeof(double) * total, st));
+    RC(upload(doff, s_off, sizeof(int64_t) * n_s, st));
+    RC(upload(dlen, s_len, sizeof(int64_t) * n_s, st));
+    RC(ddist.alloc(sizeof(double) * n * n_s));
+    RC(didx.alloc(sizeof(int64_t) * n * n_s));
+    RC(wbx_dev_subsequence_pairwise(metric, WBX_F64, ds.p, (const int64_t *)doff.p,
+                                    (const int64_t *)dlen.p, n_s, (const double *)dx.p + dim * t, n,
+                                    dims * t, t, (double *)ddist.p, (int64_t *)didx.p, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(dist, ddist.p, sizeof(double) * n * n_s, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(idx, didx.p, sizeof(int64_t) * n * n_s, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));
+    return WBX_OK;
+}
+
+}  // extern "C"

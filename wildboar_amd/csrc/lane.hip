@@ -1,0 +1,156 @@
+// lane.hip -- FP32 DTW with one pair per lane and the band row in registers.
+//
+// For pairs that share one geometry (lx, ly, w) -- every pair of a
+// pairwise/paired call on a fixed-width array, every window of one shapelet --
+// each of the 64 lanes of a wave runs the whole DP of its own pair.  The row
+// is kept in band coordinates (slot k <-> column j = i + k - off), where the
+// three predecessors of slot k are slot k+1 of the previous row (up), slot
+// k-1 of this row (left) and slot k of the previous row (diag).  Those are
+// static register indices, so a fully unrolled row is BP cells of
+//     d = x_i - y_j ; m = min3(up, left, diag) ; c = fma(d, d, m)
+// i.e. three VALU instructions per cell and no LDS or cross-lane traffic.
+//
+// The y values of a row are a window y[i-off .. i-off+BP) that slides by one
+// per row.  The kernel keeps a window of BP+U-1 registers, runs U rows against
+// it (row u of the group reads yw[k+u]) and then shifts it by U: the shift
+// costs (BP+U-1)/U moves per U*BP cells.  Columns outside [0, ly) read +inf,
+// which makes their cells +inf, and the row state before row 0 is +inf with a
+// 0 in slot off, which turns the first row into the reference's prefix sum.
+// Slots >= B (the padding up to BP) compute throw-away values; slot B is reset
+// to +inf after every row so slot B-1 never sees them.
+#include "wbx_internal.h"
+
+namespace wbx {
+
+struct LaneGeo {
+    int lx, ly, off, B, kf;
+};
+
+__device__ __forceinline__ float ldy(const float *__restrict__ Y, int j, int ly) {
+    int jc = j < 0 ? 0 : (j >= ly ? ly - 1 : j);
+    float v = Y[jc];
+    return (unsigned)j < (unsigned)ly ? v : WBX_INF_F;
+}
+
+// pen[q] is +inf for slot BP-8+q == B and -inf otherwise (uniform values), so
+// max(row, pen) resets slot B without a branch.
+template <int BP>
+__device__ __forceinline__ void lane_row(float (&row)[BP], const float *yw, float xi,
+                                         const float (&pen)[8]) {
+    float left = WBX_INF_F;
+#pragma unroll
+    for (int k = 0; k < BP; ++k) {
+        const float up = (k + 1 < BP) ? row[k + 1] : WBX_INF_F;
+        const float m = fminf(fminf(up, left), row[k]);
+        const float d = xi - yw[k];
+        const float c = __builtin_fmaf(d, d, m);
+        row[k] = c;
+        left = c;
+    }
+#pragma unroll
+    for (int q = 1; q < 8; ++q) row[BP - 8 + q] = fmaxf(row[BP - 8 + q], pen[q]);
+}
+
+template <int BP, int U>
+__global__ __launch_bounds__(64) void k_lane_dtw(PairSpec ps, Series<float> xs,
+                                                 Series<float> ys, LaneGeo g,
+                                                 float *__restrict__ sq) {
+    constexpr int W = BP + U - 1;
+    const int64_t p0 = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    const int64_t p = p0 < ps.n_pairs ? p0 : ps.n_pairs - 1;
+    int64_t a, b;
+    decode_pair(ps, p, a, b);
+    const float *__restrict__ X = xs.row(a);
+    const float *__restrict__ Y = ys.row(b);
+    const int lx = g.lx, ly = g.ly, off = g.off, B = g.B;
+
+    float row[BP];
+#pragma unroll
+    for (int k = 0; k < BP; ++k) row[k] = (k == off) ? 0.0f : WBX_INF_F;
+    float pen[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) pen[q] = (BP - 8 + q == B) ? WBX_INF_F : -WBX_INF_F;
+
+    float yw[W];
+#pragma unroll
+    for (int m = 0; m < W; ++m) yw[m] = ldy(Y, m - off, ly);
+
+    // Leading rows one at a time so the rest is a whole number of U-groups.
+    const int rem = lx % U;
+    for (int i = 0; i < rem; ++i) {
+        lane_row<BP>(row, yw, X[i], pen);
+#pragma unroll
+        for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
+        yw[W - 1] = ldy(Y, i + 1 - off + W - 1, ly);
+    }
+    for (int i0 = rem; i0 < lx; i0 += U) {
+        float xv[U], yn[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            xv[u] = X[i0 + u];
+            yn[u] = ldy(Y, i0 + U - off + W - U + u, ly);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) lane_row<BP>(row, yw + u, xv[u], pen);
+#pragma unroll
+        for (int m = 0; m < W - U; ++m) yw[m] = yw[m + U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) yw[W - U + u] = yn[u];
+    }
+
+    float res = WBX_INF_F;
+#pragma unroll
+    for (int k = 0; k < BP; ++k)
+        if (k == g.kf) res = row[k];
+    if (p0 < ps.n_pairs) sq[p0] = res;
+}
+
+// The y load for group i0 fetches window slots W-U .. W-1 of the *next* group:
+// next window starts at i0 + U - off, its slot W-U+u is y[i0 + U - off + W-U+u].
+
+template <int BP, int U>
+static void launch_one(const PairSpec &ps, Series<float> xs, Series<float> ys, LaneGeo g,
+                       float *sq, hipStream_t st) {
+    dim3 grid((unsigned)((ps.n_pairs + 63) / 64));
+    hipLaunchKernelGGL((k_lane_dtw<BP, U>), grid, dim3(64), 0, st, ps, xs, ys, g, sq);
+}
+
+constexpr int LANE_MAX_BP = 112;
+
+bool lane_supported(int lx, int ly, double r, int weighted) {
+    if (weighted) return false;
+    Band bd = make_band(lx, ly, r);
+    return bd.B <= LANE_MAX_BP && lx >= 1 && ly >= 1;
+}
+
+int launch_lane_dtw(const PairSpec &ps, Series<float> xs, Series<float> ys, double r,
+                    const float *weights, float *sq, hipStream_t st) {
+    if (ps.n_pairs <= 0) return WBX_OK;
+    if (weights) return fail(WBX_EUNSUPPORTED, "lane kernel: weighted DTW not supported");
+    if (xs.lens || ys.lens) return fail(WBX_EUNSUPPORTED, "lane kernel: needs one geometry");
+    Band bd = make_band(xs.len, ys.len, r);
+    if (bd.B > LANE_MAX_BP) return fail(WBX_EUNSUPPORTED, "lane kernel: band %d too wide", bd.B);
+    LaneGeo g{bd.lx, bd.ly, bd.off, bd.B, bd.ly - bd.lx + bd.off};
+    const int BP = (bd.B + 7) / 8 * 8;
+    switch (BP) {
+        case 8: launch_one<8, 8>(ps, xs, ys, g, sq, st); break;
+        case 16: launch_one<16, 8>(ps, xs, ys, g, sq, st); break;
+        case 24: launch_one<24, 8>(ps, xs, ys, g, sq, st); break;
+        case 32: launch_one<32, 8>(ps, xs, ys, g, sq, st); break;
+        case 40: launch_one<40, 8>(ps, xs, ys, g, sq, st); break;
+        case 48: launch_one<48, 8>(ps, xs, ys, g, sq, st); break;
+        case 56: launch_one<56, 8>(ps, xs, ys, g, sq, st); break;
+        case 64: launch_one<64, 8>(ps, xs, ys, g, sq, st); break;
+        case 72: launch_one<72, 8>(ps, xs, ys, g, sq, st); break;
+        case 80: launch_one<80, 8>(ps, xs, ys, g, sq, st); break;
+        case 88: launch_one<88, 8>(ps, xs, ys, g, sq, st); break;
+        case 96: launch_one<96, 4>(ps, xs, ys, g, sq, st); break;
+        case 104: launch_one<104, 4>(ps, xs, ys, g, sq, st); break;
+        case 112: launch_one<112, 4>(ps, xs, ys, g, sq, st); break;
+        default: return fail(WBX_EUNSUPPORTED, "lane kernel: band %d", bd.B);
+    }
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
+
+}  // namespace wbx

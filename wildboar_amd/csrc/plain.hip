@@ -1,0 +1,306 @@
+// plain.hip -- exact (FP64 bit-identical) kernels and small helpers.
+//
+//  k_plain_dtw    one pair per 64-lane block, anti-diagonal sweep of the
+//                 reference's band with three diagonals in LDS.  Any lengths,
+//                 any band, WDTW weights with the row-0 index quirk.  This is
+//                 the FP64 parity kernel and the "plain per-pair" baseline.
+//  k_euclid_tile  16x16 pairs per block, LDS tiles, each lane sums its pair
+//                 sequentially in the reference's order (exact in FP64).
+//  k_finish       sqrt + scatter into the driver's output layout.
+//  k_argmin_replay  _ArgMinBatch's heap acceptance rule, one lane per x row.
+//
+// Arithmetic order per cell is the reference's (distance/_elastic.pyx:925-928):
+//     cost = min(min(up, left), diag) + (v * v) * w,   v = x_i - y_j
+// and this file is compiled with contraction off, so no FMA appears.
+#pragma clang fp contract(off)
+
+#include "block_dtw.h"
+
+namespace wbx {
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_plain_dtw(PairSpec ps, Series<T> xs,
+                                                  Series<T> ys, double r,
+                                                  const T *__restrict__ weights,
+                                                  int cap, T *__restrict__ sq) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *buf = reinterpret_cast<T *>(smem_raw);
+    const int64_t p = (int64_t)blockIdx.x + (int64_t)blockIdx.y * gridDim.x;
+    if (p >= ps.n_pairs) return;
+    int64_t a, b;
+    decode_pair(ps, p, a, b);
+    const int lx = xs.length(a), ly = ys.length(b);
+    const T res = block_dtw<T>(xs.row(a), lx, ys.row(b), ly, band_w(lx, ly, r), weights, buf,
+                               buf + cap, buf + 2 * cap);
+    if (threadIdx.x == 0) sq[p] = res;
+}
+
+template <typename T>
+static int set_lds(const void *fn, size_t lds) {
+    if (lds > 64 * 1024) {
+        if (lds > 160 * 1024)
+            return fail(WBX_EUNSUPPORTED, "plain kernel needs %zu bytes of LDS", lds);
+        WBX_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    return WBX_OK;
+}
+
+template <typename T>
+int launch_plain_dtw(const PairSpec &ps, Series<T> xs, Series<T> ys, double r,
+                     const T *weights, T *sq, hipStream_t st) {
+    if (ps.n_pairs <= 0) return WBX_OK;
+    const int cap = xs.len;  // every row length of xs is <= xs.len
+    const size_t lds = (size_t)3 * (size_t)cap * sizeof(T);
+    int rc = set_lds<T>((const void *)k_plain_dtw<T>, lds);
+    if (rc) return rc;
+    const int64_t nb = ps.n_pairs;
+    dim3 grid((unsigned)(nb < 65536 ? nb : 65536), (unsigned)((nb + 65535) / 65536));
+    hipLaunchKernelGGL(k_plain_dtw<T>, grid, dim3(64), lds, st, ps, xs, ys, r, weights, cap, sq);
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
+
+template int launch_plain_dtw<float>(const PairSpec &, Series<float>, Series<float>, double,
+                                     const float *, float *, hipStream_t);
+template int launch_plain_dtw<double>(const PairSpec &, Series<double>, Series<double>, double,
+                                      const double *, double *, hipStream_t);
+
+// ------------------------------------------------------------- euclidean
+// euclidean_distance (distance/_metric.pyx:654-683) with S = x, T = y: windows
+// of y; per window dist = sum_j (y[i+j] - x[j])^2 accumulated in order.  Equal
+// lengths give one window.  Pairs are tiled 16 x 16 per block for GRID mode;
+// other modes use one lane per pair straight from memory.
+constexpr int ET = 16;   // tile edge
+constexpr int EC = 64;   // timesteps per LDS chunk
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_euclid_grid(Series<T> xs, int64_t nx,
+                                                     Series<T> ys, int64_t ny,
+                                                     int t, T *__restrict__ sq) {
+    __shared__ T sx[ET][EC + 1];
+    __shared__ T sy[ET][EC + 1];
+    const int tx = threadIdx.x % ET, ty = threadIdx.x / ET;
+    const int64_t a0 = (int64_t)blockIdx.y * ET, b0 = (int64_t)blockIdx.x * ET;
+    const int64_t a = a0 + ty, b = b0 + tx;
+    T dist = (T)0;
+    for (int c0 = 0; c0 < t; c0 += EC) {
+        const int cn = t - c0 < EC ? t - c0 : EC;
+        for (int q = threadIdx.x; q < ET * EC; q += 256) {
+            const int rr = q / EC, cc = q % EC;
+            if (cc < cn) {
+                sx[rr][cc] = (a0 + rr < nx) ? xs.row(a0 + rr)[c0 + cc] : (T)0;
+                sy[rr][cc] = (b0 + rr < ny) ? ys.row(b0 + rr)[c0 + cc] : (T)0;
+            }
+        }
+        __syncthreads();
+        for (int cc = 0; cc < cn; ++cc) {
+            T v = sy[tx][cc];
+            v -= sx[ty][cc];
+            dist += v * v;
+        }
+        __syncthreads();
+    }
+    if (a < nx && b < ny) sq[a * ny + b] = dist;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_euclid_pairs(PairSpec ps, Series<T> xs,
+                                                      Series<T> ys, T *__restrict__ sq) {
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= ps.n_pairs) return;
+    int64_t a, b;
+    decode_pair(ps, p, a, b);
+    const T *__restrict__ S = xs.row(a);
+    const T *__restrict__ Tt = ys.row(b);
+    const int sl = xs.length(a), tl = ys.length(b);
+    T min_dist = inf_of<T>();
+    for (int i = 0; i < tl - sl + 1; ++i) {
+        T dist = (T)0;
+        for (int j = 0; j < sl; ++j) {
+            if (dist >= min_dist) break;
+            T v = Tt[i + j];
+            v -= S[j];
+            dist += v * v;
+        }
+        if (dist < min_dist) min_dist = dist;
+    }
+    sq[p] = min_dist;
+}
+
+template <typename T>
+int launch_euclid_pairs(const PairSpec &ps, Series<T> xs, Series<T> ys, T *sq,
+                        hipStream_t st) {
+    if (ps.n_pairs <= 0) return WBX_OK;
+    const bool fixed = xs.lens == nullptr && ys.lens == nullptr && xs.len == ys.len;
+    if (ps.mode == PAIR_GRID && fixed) {
+        const int64_t nx = ps.n_pairs / ps.ny, ny = ps.ny;
+        dim3 grid((unsigned)((ny + ET - 1) / ET), (unsigned)((nx + ET - 1) / ET));
+        if (grid.y > 65535) return fail(WBX_EUNSUPPORTED, "too many rows for euclidean tiles");
+        hipLaunchKernelGGL(k_euclid_grid<T>, grid, dim3(256), 0, st, xs, nx, ys, ny, xs.len, sq);
+    } else {
+        dim3 grid((unsigned)((ps.n_pairs + 255) / 256));
+        hipLaunchKernelGGL(k_euclid_pairs<T>, grid, dim3(256), 0, st, ps, xs, ys, sq);
+    }
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
+
+template int launch_euclid_pairs<float>(const PairSpec &, Series<float>, Series<float>, float *,
+                                        hipStream_t);
+template int launch_euclid_pairs<double>(const PairSpec &, Series<double>, Series<double>,
+                                         double *, hipStream_t);
+
+// ------------------------------------------------------------- finish
+// GRID: out[a * ld + b];  TRIANGLE: out[a*ld+b] = out[b*ld+a] (diagonal is
+// zeroed by the caller);  DIAG/LIST: out[p].
+template <typename T>
+__global__ __launch_bounds__(256) void k_finish(PairSpec ps, const T *__restrict__ sq,
+                                                double *__restrict__ out, int64_t ld) {
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= ps.n_pairs) return;
+    const double v = sqrt((double)sq[p]);
+    if (ps.mode == PAIR_GRID) {
+        out[p] = v;
+    } else if (ps.mode == PAIR_TRIANGLE) {
+        int64_t a, b;
+        decode_pair(ps, p, a, b);
+        out[a * ld + b] = v;
+        out[b * ld + a] = v;
+    } else if (ps.mode == PAIR_LIST) {
+        out[(int64_t)ps.pa[p] * ld + ps.pb[p]] = v;
+    } else {
+        out[p] = v;
+    }
+}
+
+template <typename T>
+int launch_finish(const PairSpec &ps, const T *sq, double *out, int64_t ld, hipStream_t st) {
+    if (ps.n_pairs <= 0) return WBX_OK;
+    dim3 grid((unsigned)((ps.n_pairs + 255) / 256));
+    hipLaunchKernelGGL(k_finish<T>, grid, dim3(256), 0, st, ps, sq, out, ld);
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
+template int launch_finish<float>(const PairSpec &, const float *, double *, int64_t, hipStream_t);
+template int launch_finish<double>(const PairSpec &, const double *, double *, int64_t,
+                                   hipStream_t);
+
+// ------------------------------------------------------------- argmin
+// _ArgMinBatch.__call__ (distance/_cdistance.pyx:1318-1345) on exact distances:
+// j is pushed iff not skipped by the lower bound (lb >= threshold) and
+// D[j] < threshold; threshold is +inf until the heap holds k elements and the
+// heap maximum afterwards.  Heap = utils/_misc.pyx:18-107.  Early abandoning
+// in the reference never changes which j are pushed (a pair is abandoned only
+// when its cost exceeds the threshold), so replaying exact values reproduces
+// the reference's heap, element order included.
+struct HeapEl {
+    double value;
+    int64_t index;
+};
+
+__device__ void heap_shift_down(HeapEl *h, int64_t startpos, int64_t pos) {
+    HeapEl ne = h[pos];
+    while (pos > startpos) {
+        int64_t parent = (pos - 1) >> 1;
+        if (ne.value > h[parent].value) {
+            h[pos] = h[parent];
+            pos = parent;
+            continue;
+        }
+        break;
+    }
+    h[pos] = ne;
+}
+
+__device__ void heap_shift_up(HeapEl *h, int64_t pos, int64_t endpos) {
+    int64_t startpos = pos;
+    HeapEl ne = h[pos];
+    int64_t child = 2 * pos + 1;
+    while (child < endpos) {
+        int64_t right = child + 1;
+        if (right < endpos && h[child].value < h[right].value) child = right;
+        h[pos] = h[child];
+        pos = child;
+        child = 2 * pos + 1;
+    }
+    h[pos] = ne;
+    heap_shift_down(h, startpos, pos);
+}
+
+__global__ __launch_bounds__(64) void k_argmin_replay(const double *__restrict__ D, int64_t nx,
+                                                      int64_t ny, int64_t k,
+                                                      const double *__restrict__ lb,
+                                                      HeapEl *__restrict__ heaps,
+                                                      int64_t *__restrict__ idx,
+                                                      double *__restrict__ dist) {
+    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (i >= nx) return;
+    HeapEl *h = heaps + i * k;
+    const double *row = D + i * ny;
+    double threshold = WBX_INF_D;
+    int64_t n = 0;
+    for (int64_t j = 0; j < ny; ++j) {
+        if (lb && lb[i * ny + j] >= threshold) continue;
+        const double v = row[j];
+        if (v < threshold) {
+            HeapEl e{v, j};
+            if (n == 0) {
+                h[0] = e;
+                n = 1;
+            } else if (n < k) {
+                h[n] = e;
+                n += 1;
+                heap_shift_down(h, 0, n - 1);
+            } else if (h[0].value > v) {
+                h[0] = e;
+                heap_shift_up(h, 0, k);
+            }
+            threshold = (n == k) ? h[0].value : WBX_INF_D;
+        }
+    }
+    for (int64_t q = 0; q < k; ++q) {
+        idx[i * k + q] = q < n ? h[q].index : 0;
+        dist[i * k + q] = q < n ? h[q].value : 0.0;
+    }
+}
+
+int launch_argmin_replay(const double *D, int64_t nx, int64_t ny, int64_t k,
+                         const double *lower_bound, void *heaps, int64_t *idx, double *dist,
+                         hipStream_t st) {
+    if (nx <= 0) return WBX_OK;
+    hipLaunchKernelGGL(k_argmin_replay, dim3((unsigned)((nx + 63) / 64)), dim3(64), 0, st, D, nx,
+                       ny, k, lower_bound, (HeapEl *)heaps, idx, dist);
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
+
+// ------------------------------------------------------------- conversions
+__global__ void k_f64_to_f32(const double *__restrict__ s, float *__restrict__ d, int64_t n) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * blockDim.x)
+        d[q] = (float)s[q];
+}
+__global__ void k_f32_to_f64(const float *__restrict__ s, double *__restrict__ d, int64_t n) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * blockDim.x)
+        d[q] = (double)s[q];
+}
+
+int launch_f64_to_f32(const double *src, float *dst, int64_t n, hipStream_t st) {
+    if (n <= 0) return WBX_OK;
+    int64_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)(nb < 8192 ? nb : 8192)), dim3(256), 0, st,
+                       src, dst, n);
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
+int launch_f32_to_f64(const float *src, double *dst, int64_t n, hipStream_t st) {
+    if (n <= 0) return WBX_OK;
+    int64_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_f32_to_f64, dim3((unsigned)(nb < 8192 ? nb : 8192)), dim3(256), 0, st,
+                       src, dst, n);
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
+
+}  // namespace wbx

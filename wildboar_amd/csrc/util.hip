@@ -1,0 +1,94 @@
+// util.hip -- strided row conversion, fills and max-|x| reduction.
+#include "wbx_internal.h"
+
+namespace wbx {
+
+template <typename S, typename D>
+__global__ __launch_bounds__(256) void k_convert_rows(const S *__restrict__ src, int64_t n,
+                                                      int64_t len, int64_t sstride,
+                                                      D *__restrict__ dst, int64_t dstride) {
+    const int64_t total = n * len;
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < total;
+         q += (int64_t)gridDim.x * 256) {
+        const int64_t i = q / len, c = q - i * len;
+        dst[i * dstride + c] = (D)src[i * sstride + c];
+    }
+}
+
+template <typename S, typename D>
+int launch_convert_rows(const S *src, int64_t n, int64_t len, int64_t sstride, D *dst,
+                        int64_t dstride, hipStream_t st) {
+    const int64_t total = n * len;
+    if (total <= 0) return WBX_OK;
+    int64_t nb = (total + 255) / 256;
+    if (nb > 16384) nb = 16384;
+    hipLaunchKernelGGL((k_convert_rows<S, D>), dim3((unsigned)nb), dim3(256), 0, st, src, n, len,
+                       sstride, dst, dstride);
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
+
+template int launch_convert_rows<double, float>(const double *, int64_t, int64_t, int64_t, float *,
+                                                int64_t, hipStream_t);
+template int launch_convert_rows<float, double>(const float *, int64_t, int64_t, int64_t, double *,
+                                                int64_t, hipStream_t);
+template int launch_convert_rows<double, double>(const double *, int64_t, int64_t, int64_t,
+                                                 double *, int64_t, hipStream_t);
+template int launch_convert_rows<float, float>(const float *, int64_t, int64_t, int64_t, float *,
+                                               int64_t, hipStream_t);
+
+__global__ __launch_bounds__(256) void k_fill_f64(double *__restrict__ dst, int64_t n, double v) {
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * 256)
+        dst[q] = v;
+}
+
+int launch_fill_f64(double *dst, int64_t n, double value, hipStream_t st) {
+    if (n <= 0) return WBX_OK;
+    int64_t nb = (n + 255) / 256;
+    if (nb > 16384) nb = 16384;
+    hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)nb), dim3(256), 0, st, dst, n, value);
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
+
+// Non-negative floats order like their bit patterns, so an unsigned atomicMax
+// on the bits of fabsf(v) is a max reduction.
+template <typename S>
+__global__ __launch_bounds__(256) void k_absmax_rows(const S *__restrict__ src, int64_t n,
+                                                     int64_t len, int64_t stride,
+                                                     unsigned *__restrict__ out) {
+    float m = 0.0f;
+    const int64_t total = n * len;
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < total;
+         q += (int64_t)gridDim.x * 256) {
+        const int64_t i = q / len, c = q - i * len;
+        // round up so the float bound is never below the double value
+        const double v = fabs((double)src[i * stride + c]);
+        float f = (float)v;
+        if ((double)f < v) f = __uint_as_float(__float_as_uint(f) + 1u);  // next float up
+        m = fmaxf(m, f);
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+template <typename S>
+int launch_absmax_rows(const S *src, int64_t n, int64_t len, int64_t stride, unsigned *out,
+                       hipStream_t st) {
+    const int64_t total = n * len;
+    if (total <= 0) return WBX_OK;
+    int64_t nb = (total + 255) / 256;
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(k_absmax_rows<S>, dim3((unsigned)nb), dim3(256), 0, st, src, n, len, stride,
+                       out);
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
+
+template int launch_absmax_rows<double>(const double *, int64_t, int64_t, int64_t, unsigned *,
+                                        hipStream_t);
+template int launch_absmax_rows<float>(const float *, int64_t, int64_t, int64_t, unsigned *,
+                                       hipStream_t);
+
+}  // namespace wbx

@@ -1,0 +1,214 @@
+// wbx_internal.h -- shared definitions for the gfx950 DTW / distance kernels.
+//
+// Band geometry follows wildboar's dtw_distance (distance/_elastic.pyx:869-940):
+//   w        = max(floor(min(lx, ly) * r), 1)                 (_compute_r, :1924)
+//   min_len  = max(0, lx - ly),  max_len = max(0, ly - lx) + w
+//   row i spans columns [max(0, i - min_len - w + 1), min(ly, i + max_len))
+// In band coordinates a cell (i, j) lives in slot k = j - i + off with
+// off = min_len + w - 1, k in [0, B), B = min_len + max_len + w - 1.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/wbx.h"
+
+#define WBX_INF_F __builtin_inff()
+#define WBX_INF_D __builtin_inf()
+
+namespace wbx {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char *fmt, ...);
+int fail(int code, const char *fmt, ...);
+
+#define WBX_HIP_CHECK(expr)                                                  \
+    do {                                                                     \
+        hipError_t _e = (expr);                                              \
+        if (_e != hipSuccess)                                                \
+            return ::wbx::fail(WBX_EDEVICE, "%s failed: %s (%s:%d)", #expr,  \
+                               hipGetErrorString(_e), __FILE__, __LINE__);   \
+    } while (0)
+
+// ---------------------------------------------------------------- geometry
+struct Band {
+    int lx, ly, w, off, B;
+};
+
+__host__ __device__ inline int band_w(int lx, int ly, double r) {
+    int m = lx < ly ? lx : ly;
+    double v = floor((double)m * r);
+    return (int)(v > 1.0 ? v : 1.0);
+}
+
+__host__ __device__ inline Band make_band(int lx, int ly, double r) {
+    Band b;
+    b.lx = lx;
+    b.ly = ly;
+    b.w = band_w(lx, ly, r);
+    int min_len = lx > ly ? lx - ly : 0;
+    int max_len = (ly > lx ? ly - lx : 0) + b.w;
+    b.off = min_len + b.w - 1;
+    b.B = min_len + max_len + b.w - 1;
+    return b;
+}
+
+// Exact number of band cells of the reference's loops (row 0 included).
+__host__ __device__ inline int64_t band_cells(int lx, int ly, double r) {
+    int w = band_w(lx, ly, r);
+    int min_len = lx > ly ? lx - ly : 0;
+    int max_len = (ly > lx ? ly - lx : 0) + w;
+    int64_t cells = ly < max_len ? ly : max_len;  // row 0
+    for (int i = 1; i < lx; i++) {
+        int js = i - min_len - w + 1;
+        if (js < 0) js = 0;
+        int je = i + max_len;
+        if (je > ly) je = ly;
+        cells += je - js;
+    }
+    return cells;
+}
+
+// ---------------------------------------------------------------- pairs
+// How a flat pair index p maps onto (a, b) = (row series, column series).
+enum PairMode : int {
+    PAIR_GRID = 0,      // p -> (p / ny, p % ny)          pairwise X x Y
+    PAIR_TRIANGLE = 1,  // p -> (i, j), i < j, row-major  singleton X x X
+    PAIR_DIAG = 2,      // p -> (p, p)                    paired
+    PAIR_LIST = 3,      // p -> (pa[p], pb[p])            explicit list
+};
+
+struct PairSpec {
+    int mode;
+    int64_t n_pairs;
+    int64_t ny;  // GRID: columns; TRIANGLE: n
+    const int32_t *pa, *pb;
+};
+
+__host__ __device__ inline void decode_pair(const PairSpec &ps, int64_t p,
+                                            int64_t &a, int64_t &b) {
+    switch (ps.mode) {
+        case PAIR_GRID:
+            a = p / ps.ny;
+            b = p - a * ps.ny;
+            break;
+        case PAIR_TRIANGLE: {
+            const int64_t n = ps.ny;
+            const double q = (double)(2 * n - 1);
+            int64_t i = (int64_t)floor((q - sqrt(q * q - 8.0 * (double)p)) * 0.5);
+            if (i < 0) i = 0;
+            while (i > 0 && i * (2 * n - i - 1) / 2 > p) i--;
+            while ((i + 1) * (2 * n - i - 2) / 2 <= p) i++;
+            a = i;
+            b = p - i * (2 * n - i - 1) / 2 + i + 1;
+            break;
+        }
+        case PAIR_DIAG:
+            a = p;
+            b = p;
+            break;
+        default:
+            a = ps.pa[p];
+            b = ps.pb[p];
+            break;
+    }
+}
+
+// Series set: base pointer + stride between series, optional per-series length.
+template <typename T>
+struct Series {
+    const T *base;
+    int64_t stride;
+    int len;              // fixed length when lens == nullptr
+    const int32_t *lens;  // per-series length (EOS-trimmed) or nullptr
+    __device__ inline const T *row(int64_t i) const { return base + i * stride; }
+    __device__ inline int length(int64_t i) const { return lens ? lens[i] : len; }
+};
+
+// ---------------------------------------------------------------- context
+struct Workspace {
+    void *ptr = nullptr;
+    size_t size = 0;
+};
+
+// Per-device state: device id, a default stream, a grow-only scratch arena,
+// the event that marks the end of the arena's last user, and a one-entry
+// cache of WDTW weights.
+struct DeviceCtx {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipEvent_t last = nullptr;
+    Workspace ws;
+    int n_cu = 0;
+    int64_t w_n = -1;
+    double w_g = 0.0;
+    double *w64 = nullptr;
+    float *w32 = nullptr;
+};
+
+DeviceCtx *ctx_current();              // current device's context (created lazily)
+int ws_reserve(DeviceCtx *ctx, size_t bytes, void **out);
+
+// ---------------------------------------------------------------- launchers
+// Plain one-pair-per-block DTW (any band, any lengths, WDTW quirk): writes the
+// squared cost of each pair into sq[p].
+template <typename T>
+int launch_plain_dtw(const PairSpec &ps, Series<T> xs, Series<T> ys, double r,
+                     const T *weights, T *sq, hipStream_t st);
+
+// Euclidean: one thread per pair, sequential sum in the reference's order.
+template <typename T>
+int launch_euclid_pairs(const PairSpec &ps, Series<T> xs, Series<T> ys, T *sq,
+                        hipStream_t st);
+
+// Fast FP32 lane-per-pair kernel for pairs that share one geometry.
+// Returns WBX_EUNSUPPORTED when the band is too wide for it.
+int launch_lane_dtw(const PairSpec &ps, Series<float> xs, Series<float> ys,
+                    double r, const float *weights, float *sq, hipStream_t st);
+bool lane_supported(int lx, int ly, double r, int weighted);
+
+// sqrt + scatter of per-pair squared costs into the driver's output layout.
+template <typename T>
+int launch_finish(const PairSpec &ps, const T *sq, double *out, int64_t ld,
+                  hipStream_t st);
+
+// Heap replay (_ArgMinBatch acceptance rule) per x row over a distance matrix.
+// heaps: nx * k * 16 bytes of scratch.
+int launch_argmin_replay(const double *D, int64_t nx, int64_t ny, int64_t k,
+                         const double *lower_bound, void *heaps, int64_t *idx, double *dist,
+                         hipStream_t st);
+
+// Conversions and fills.
+int launch_f64_to_f32(const double *src, float *dst, int64_t n, hipStream_t st);
+int launch_f32_to_f64(const float *src, double *dst, int64_t n, hipStream_t st);
+// dst[i * dstride + c] = (D)src[i * sstride + c] for i < n, c < len.
+template <typename S, typename D>
+int launch_convert_rows(const S *src, int64_t n, int64_t len, int64_t sstride, D *dst,
+                        int64_t dstride, hipStream_t st);
+int launch_fill_f64(double *dst, int64_t n, double value, hipStream_t st);
+// *out = max |src[i * stride + c]| as float bits (out must be zeroed first).
+template <typename S>
+int launch_absmax_rows(const S *src, int64_t n, int64_t len, int64_t stride, unsigned *out,
+                       hipStream_t st);
+
+// ---------------------------------------------------------------- subsequence
+struct Shapelets {
+    const void *S;        // packed values (element type of the launch)
+    const int64_t *off;   // device: start of shapelet j
+    const int64_t *len;   // device: length of shapelet j
+    int64_t n;
+};
+template <typename T>
+int launch_subseq_exact(int kind, double r, Shapelets sh, int max_len, Series<T> xs, int64_t n,
+                        const T *weights, double *dist, int64_t *idx, hipStream_t st);
+
+// ---------------------------------------------------------------- re-rank
+struct ErrBound {
+    double e_up, e_dn, A, e64;
+};
+ErrBound make_err_bound(int lx, int ly, int weighted, double absmax, int inputs_exact);
+int launch_argmin_candidates(const float *sq, int64_t nx, int64_t ny, int k, const ErrBound &eb,
+                             int64_t *counts, int64_t *offs, int32_t *pa, int32_t *pb,
+                             int64_t *total_host, hipStream_t st);
+
+}  // namespace wbx

@@ -1,0 +1,122 @@
+"""Device-tensor entry points: torch CUDA (ROCm) tensors in, torch tensors out.
+
+Each function calls the matching ``wbx_dev_*`` C entry point with the tensors'
+device pointers and enqueues the work on torch's current stream, so results
+are ordered with surrounding torch work.  Inputs are (n, t) float32 or float64
+tensors whose last dimension is contiguous.  No host copies, no CPU fallback.
+"""
+
+import torch
+
+from . import _lib
+from .distance import _check_g, _check_r
+
+
+def _dtype(t):
+    if t.dtype == torch.float32:
+        return _lib.F32
+    if t.dtype == torch.float64:
+        return _lib.F64
+    raise TypeError(f"expected float32 or float64 tensor, got {t.dtype}")
+
+
+def _rows(t, name):
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if t.dim() != 2:
+        raise ValueError(f"{name} must be 2-D (n, t), got shape {tuple(t.shape)}")
+    if t.stride(1) != 1:
+        t = t.contiguous()
+    return t
+
+
+def _metric(metric, r, g, precision):
+    kind = _lib.KINDS[metric]
+    prec = _lib.PRECISIONS[precision]
+    if kind != _lib.EUCLIDEAN:
+        r = _check_r(r)
+    if kind == _lib.WDTW:
+        g = _check_g(g)
+    return _lib.metric_struct(kind, prec, r, g)
+
+
+def _stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def pairwise(X, Y=None, *, metric="dtw", r=1.0, g=0.05, precision="fp32", out=None):
+    """(nx, ny) float64 distances; ``Y=None`` gives the symmetric singleton matrix."""
+    X = _rows(X, "X")
+    if Y is not None:
+        Y = _rows(Y, "Y")
+        if Y.dtype != X.dtype:
+            raise TypeError("X and Y must have the same dtype")
+    nx = X.shape[0]
+    ny = nx if Y is None else Y.shape[0]
+    if out is None:
+        out = torch.empty((nx, ny), dtype=torch.float64, device=X.device)
+    m = _metric(metric, r, g, precision)
+    with torch.cuda.device(X.device):
+        _lib.check(_lib.load().wbx_dev_pairwise(
+            m, _dtype(X), X.data_ptr(), nx, X.stride(0), X.shape[1],
+            None if Y is None else Y.data_ptr(), ny, 0 if Y is None else Y.stride(0),
+            X.shape[1] if Y is None else Y.shape[1], out.data_ptr(), _stream()))
+    return out
+
+
+def paired(X, Y, *, metric="dtw", r=1.0, g=0.05, precision="fp32", out=None):
+    """out[i] = D(Y[i], X[i]) (the reference's paired orientation)."""
+    X = _rows(X, "X")
+    Y = _rows(Y, "Y")
+    if X.shape != Y.shape:
+        raise ValueError("X and Y must have the same shape")
+    if out is None:
+        out = torch.empty(X.shape[0], dtype=torch.float64, device=X.device)
+    m = _metric(metric, r, g, precision)
+    with torch.cuda.device(X.device):
+        _lib.check(_lib.load().wbx_dev_paired(
+            m, _dtype(X), X.data_ptr(), X.stride(0), Y.data_ptr(), Y.stride(0), X.shape[0],
+            X.shape[1], out.data_ptr(), _stream()))
+    return out
+
+
+def argmin(X, Y, k=1, *, metric="dtw", r=1.0, g=0.05, precision="fp32", lower_bound=None):
+    """(idx int64 (nx, k), dist float64 (nx, k)) in the reference's heap order."""
+    X = _rows(X, "X")
+    Y = _rows(Y, "Y")
+    k = min(int(k), Y.shape[0])
+    idx = torch.empty((X.shape[0], k), dtype=torch.int64, device=X.device)
+    dist = torch.empty((X.shape[0], k), dtype=torch.float64, device=X.device)
+    lb = None
+    if lower_bound is not None:
+        lb = lower_bound.to(device=X.device, dtype=torch.float64).contiguous()
+    m = _metric(metric, r, g, precision)
+    with torch.cuda.device(X.device):
+        _lib.check(_lib.load().wbx_dev_argmin(
+            m, _dtype(X), X.data_ptr(), X.shape[0], X.stride(0), X.shape[1], Y.data_ptr(),
+            Y.shape[0], Y.stride(0), Y.shape[1], k, None if lb is None else lb.data_ptr(),
+            idx.data_ptr(), dist.data_ptr(), _stream()))
+    return idx, dist
+
+
+def subsequence(shapelets, X, *, metric="dtw", r=1.0, g=0.05, precision="fp32"):
+    """(dist (n, n_shapelets) float64, idx (n, n_shapelets) int64)."""
+    X = _rows(X, "X")
+    if isinstance(shapelets, (list, tuple)):
+        lens = torch.tensor([int(s.shape[0]) for s in shapelets], dtype=torch.int64)
+        packed = torch.cat([s.reshape(-1).to(device=X.device, dtype=X.dtype) for s in shapelets])
+    else:
+        raise TypeError("shapelets must be a list of 1-D tensors")
+    offs = torch.zeros_like(lens)
+    offs[1:] = torch.cumsum(lens, 0)[:-1]
+    lens = lens.to(X.device)
+    offs = offs.to(X.device)
+    n, ns = X.shape[0], len(shapelets)
+    dist = torch.empty((n, ns), dtype=torch.float64, device=X.device)
+    idx = torch.empty((n, ns), dtype=torch.int64, device=X.device)
+    m = _metric(metric, r, g, precision)
+    with torch.cuda.device(X.device):
+        _lib.check(_lib.load().wbx_dev_subsequence_pairwise(
+            m, _dtype(X), packed.data_ptr(), offs.data_ptr(), lens.data_ptr(), ns, X.data_ptr(),
+            n, X.stride(0), X.shape[1], dist.data_ptr(), idx.data_ptr(), _stream()))
+    return dist, idx
