@@ -76,6 +76,10 @@ int wbx_set_policy(int policy);
  * 3 plain DTW, 4 batched DTW, 5 subsequence scan), and clears the record. */
 int wbx_prof_enable(int on);
 int wbx_prof_take(double *total_ms, int64_t *launches, int *last_kind);
+/* No counterpart in the reference (measurement).  Runs a VALU probe made of
+ * the DTW cell's instructions (sub, min3, fma) on every SIMD of the current
+ * device and returns the achieved lane-instructions per second. */
+int wbx_prof_valu_peak(double *lane_instr_per_s);
 
 /* ------------------------------------------- host-pointer drop-ins (float64)
  * Synchronous.  Inputs and outputs are host memory.  These replace the

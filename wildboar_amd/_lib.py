@@ -26,6 +26,7 @@ EXPORTS = (
     "wbx_pairwise", "wbx_paired", "wbx_argmin", "wbx_subsequence_pairwise",
     "wbx_dev_pairwise", "wbx_dev_paired", "wbx_dev_argmin",
     "wbx_dev_subsequence_pairwise", "wbx_set_policy", "wbx_prof_enable", "wbx_prof_take",
+    "wbx_prof_valu_peak",
 )
 
 
@@ -69,6 +70,7 @@ def load():
             "wbx_set_policy": (i32, [i32]),
             "wbx_prof_enable": (i32, [i32]),
             "wbx_prof_take": (i32, [p, p, p]),
+            "wbx_prof_valu_peak": (i32, [p]),
             "wbx_pairwise": (i32, [m, p, i64, i64, i64, p, i64, i64, i64, p]),
             "wbx_paired": (i32, [m, p, p, i64, i64, i64, i64, p]),
             "wbx_argmin": (i32, [m, p, i64, i64, i64, p, i64, i64, i64, i64, p, p, p]),
@@ -120,3 +122,10 @@ def prof_take():
     kind = ctypes.c_int(0)
     check(load().wbx_prof_take(ctypes.byref(ms), ctypes.byref(n), ctypes.byref(kind)))
     return ms.value, n.value, kind.value
+
+
+def valu_peak():
+    """Measured VALU lane-instructions per second of the DTW cell mix."""
+    v = ctypes.c_double(0.0)
+    check(load().wbx_prof_valu_peak(ctypes.byref(v)))
+    return v.value
