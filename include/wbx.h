@@ -121,8 +121,10 @@ int wbx_subsequence_pairwise(const wbx_metric *metric, const double *S,
 /* ------------------------------------------- device-pointer entry points
  * Inputs are device pointers of element type `dtype` (WBX_F64 / WBX_F32);
  * series i of X starts at X + i * x_stride.  Outputs are device memory.
- * Work is enqueued on `stream` (a hipStream_t; NULL = the library's own
- * stream) and the call returns once it is enqueued on that stream.          */
+ * Work is enqueued on `stream` (a hipStream_t; NULL = HIP's default null
+ * stream) and the call returns once it is enqueued on that stream.  Calls
+ * that must read a count back to the host (argmin in FP32 mode, the
+ * subsequence scan) synchronise `stream` before returning.                  */
 
 /* Pairwise / singleton (Y == NULL) as wbx_pairwise; out is (nx, ny) f64. */
 int wbx_dev_pairwise(const wbx_metric *metric, int dtype, const void *X,
@@ -151,6 +153,35 @@ int wbx_dev_subsequence_pairwise(const wbx_metric *metric, int dtype,
                                  const void *X, int64_t n, int64_t x_stride,
                                  int64_t t, double *dist, int64_t *idx,
                                  void *stream);
+
+/* ------------------------------------------- batched uneven pairs
+ * No single reference driver: this is the batch form of DtwMetric._distance
+ * (distance/_elastic.pyx:3167-3186) over pairs of different lengths, each with
+ * its own r.  Series are padded to t_pad with the end-of-series marker of
+ * utils/variable_len.py:17-115 (which the reference's distance API rejects,
+ * utils/validation.py:574-577); out[p] = D(X[p][:len_x[p]], Y[p][:len_y[p]]). */
+
+/* len[i] = index of the first end-of-series marker of row i (t if none). */
+int wbx_dev_eos_lengths(int dtype, const void *X, int64_t n, int64_t stride,
+                        int64_t t, int32_t *len, void *stream);
+
+/* Host-side plan: lengths and per-pair r are host arrays (r_per_pair NULL =
+ * metric->r for all).  policy 0 = fastest kernels, 1 = plain kernel only.
+ * The plan owns its device buffers; free it with wbx_batch_free. */
+int wbx_batch_create(const wbx_metric *metric, int64_t n_pairs, int64_t t_pad,
+                     const int32_t *len_x, const int32_t *len_y,
+                     const double *r_per_pair, int policy, void **plan);
+int wbx_batch_free(void *plan);
+/* Algorithmic band cells of the plan and how many pairs each kernel takes. */
+int wbx_batch_info(void *plan, int64_t *cells, int64_t *n_strip,
+                   int64_t *n_plain);
+/* Enqueue one run on `stream` (device pointers, dtype WBX_F32 / WBX_F64).
+ * use_graph != 0: capture the run into a hipGraph the first time these
+ * arguments are seen and replay the graph on later calls with the same
+ * arguments.  out is n_pairs float64. */
+int wbx_batch_run(void *plan, int dtype, const void *X, int64_t x_stride,
+                  const void *Y, int64_t y_stride, double *out, void *stream,
+                  int use_graph);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,131 @@
+"""GPU parity of the batched uneven-length DTW against the float64 oracle.
+
+Each pair is checked against DtwMetric on the unpadded series
+(oracle.metric_distance, band from the pair's own min length and r).
+Bars: FP64 bit-identical; FP32 max relative error 1e-5 (absolute 1e-6 floor).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from wildboar_amd import _lib
+from wildboar_amd.batch import DtwBatch, eos_lengths
+from wildboar_amd.variable_len import EOS, get_variable_length
+
+pytestmark = pytest.mark.gpu
+
+FP32_RTOL = 1e-5
+
+
+def make_pairs(seed, n, lo, hi, t_pad, spikes=0.0):
+    rng = np.random.default_rng(seed)
+    lx = rng.integers(lo, hi + 1, n)
+    ly = rng.integers(lo, hi + 1, n)
+    X = np.full((n, t_pad), EOS, dtype=np.float64)
+    Y = np.full((n, t_pad), EOS, dtype=np.float64)
+    for p in range(n):
+        for arr, L in ((X, lx[p]), (Y, ly[p])):
+            s = np.cumsum(rng.standard_normal(L))
+            s = (s - s.mean()) / (s.std() if L > 1 and s.std() > 0 else 1.0)
+            if spikes:
+                mask = rng.random(L) < spikes
+                s = s + mask * rng.laplace(0.0, 1.0, L)
+            arr[p, :L] = s
+    r = rng.uniform(0.05, 0.2, n)
+    return X, Y, lx.astype(np.int32), ly.astype(np.int32), r
+
+
+def oracle_batch(X, Y, lx, ly, r, metric="dtw", g=0.05, weight_n=0):
+    return np.array([oracle.metric_distance(X[p, :lx[p]], Y[p, :ly[p]], metric, r=r[p], g=g,
+                                            weight_n=weight_n) for p in range(len(lx))])
+
+
+def rel_err(got, want):
+    return np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-1))
+
+
+def test_eos_lengths_match_reference_rule():
+    X, Y, lx, ly, _ = make_pairs(1, 50, 1, 90, 96)
+    for arr, L in ((X, lx), (Y, ly)):
+        np.testing.assert_array_equal(get_variable_length(arr), L)
+        for dt in (torch.float32, torch.float64):
+            t = torch.from_numpy(arr).to("cuda", dt)
+            np.testing.assert_array_equal(eos_lengths(t).cpu().numpy(), L)
+
+
+@pytest.mark.parametrize("seed,n,lo,hi,t_pad", [(2, 300, 20, 300, 320), (3, 120, 1, 40, 48),
+                                                 (4, 60, 200, 700, 700)])
+def test_batch_fp32_strip_vs_oracle(seed, n, lo, hi, t_pad):
+    X, Y, lx, ly, r = make_pairs(seed, n, lo, hi, t_pad, spikes=0.05)
+    want = oracle_batch(X, Y, lx, ly, r)
+    Xd = torch.from_numpy(X.astype(np.float32)).cuda()
+    Yd = torch.from_numpy(Y.astype(np.float32)).cuda()
+    plan = DtwBatch(lx, ly, t_pad, r, precision="fp32")
+    got = plan.run(Xd, Yd, graph=False).cpu().numpy()
+    want32 = oracle_batch(X.astype(np.float32).astype(np.float64),
+                          Y.astype(np.float32).astype(np.float64), lx, ly, r)
+    assert plan.n_strip > 0
+    assert rel_err(got, want32) <= FP32_RTOL
+    assert rel_err(got, want) <= 1e-4
+
+
+def test_batch_fp32_plain_policy_matches():
+    X, Y, lx, ly, r = make_pairs(5, 80, 10, 200, 200)
+    want = oracle_batch(X.astype(np.float32).astype(np.float64),
+                        Y.astype(np.float32).astype(np.float64), lx, ly, r)
+    Xd = torch.from_numpy(X.astype(np.float32)).cuda()
+    Yd = torch.from_numpy(Y.astype(np.float32)).cuda()
+    plan = DtwBatch(lx, ly, 200, r, precision="fp32", plain=True)
+    assert plan.n_strip == 0
+    assert rel_err(plan.run(Xd, Yd, graph=False).cpu().numpy(), want) <= FP32_RTOL
+
+
+def test_batch_fp64_bitwise():
+    X, Y, lx, ly, r = make_pairs(6, 70, 1, 150, 160)
+    want = oracle_batch(X, Y, lx, ly, r)
+    plan = DtwBatch(lx, ly, 160, r, precision="fp64")
+    got = plan.run(torch.from_numpy(X).cuda(), torch.from_numpy(Y).cuda(), graph=False)
+    np.testing.assert_array_equal(got.cpu().numpy(), want)
+
+
+def test_batch_wdtw_fp64_bitwise_weights_by_array_width():
+    X, Y, lx, ly, r = make_pairs(7, 40, 5, 60, 64)
+    want = oracle_batch(X, Y, lx, ly, r, metric="wdtw", g=0.1, weight_n=64)
+    plan = DtwBatch(lx, ly, 64, r, metric="wdtw", g=0.1, precision="fp64")
+    got = plan.run(torch.from_numpy(X).cuda(), torch.from_numpy(Y).cuda(), graph=False)
+    np.testing.assert_array_equal(got.cpu().numpy(), want)
+
+
+def test_batch_graph_replays_match_eager():
+    X, Y, lx, ly, r = make_pairs(8, 200, 30, 400, 400, spikes=0.05)
+    Xd = torch.from_numpy(X.astype(np.float32)).cuda()
+    Yd = torch.from_numpy(Y.astype(np.float32)).cuda()
+    plan = DtwBatch(lx, ly, 400, r, precision="fp32")
+    eager = plan.run(Xd, Yd, graph=False).clone()
+    torch.cuda.synchronize()
+    with pytest.raises(ValueError):  # no capture on the null stream
+        plan.run(Xd, Yd, graph=True)
+    side = torch.cuda.Stream()
+    out = torch.empty_like(eager)
+    for rep in range(4):
+        with torch.cuda.stream(side):
+            out.fill_(-1.0)
+            plan.run(Xd, Yd, out=out, graph=True)
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager), rep
+        # eager work between replays
+        Zd = (Xd * 2.0).sum()
+        again = plan.run(Xd, Yd, graph=False)
+        torch.cuda.synchronize()
+        assert torch.isfinite(Zd) and torch.equal(again, eager)
+
+
+def test_batch_rejects_bad_lengths():
+    with pytest.raises(ValueError):
+        DtwBatch([0, 3], [3, 3], 4, 0.1)
+    with pytest.raises(ValueError):
+        DtwBatch([5], [3], 4, 0.1)
+    with pytest.raises(ValueError):
+        DtwBatch([2], [3], 4, [1.5])

@@ -26,7 +26,8 @@ EXPORTS = (
     "wbx_pairwise", "wbx_paired", "wbx_argmin", "wbx_subsequence_pairwise",
     "wbx_dev_pairwise", "wbx_dev_paired", "wbx_dev_argmin",
     "wbx_dev_subsequence_pairwise", "wbx_set_policy", "wbx_prof_enable", "wbx_prof_take",
-    "wbx_prof_valu_peak",
+    "wbx_prof_valu_peak", "wbx_dev_eos_lengths", "wbx_batch_create", "wbx_batch_free",
+    "wbx_batch_info", "wbx_batch_run",
 )
 
 
@@ -71,6 +72,11 @@ def load():
             "wbx_prof_enable": (i32, [i32]),
             "wbx_prof_take": (i32, [p, p, p]),
             "wbx_prof_valu_peak": (i32, [p]),
+            "wbx_dev_eos_lengths": (i32, [i32, p, i64, i64, i64, p, p]),
+            "wbx_batch_create": (i32, [m, i64, i64, p, p, p, i32, p]),
+            "wbx_batch_free": (i32, [p]),
+            "wbx_batch_info": (i32, [p, p, p, p]),
+            "wbx_batch_run": (i32, [p, i32, p, i64, p, i64, p, p, i32]),
             "wbx_pairwise": (i32, [m, p, i64, i64, i64, p, i64, i64, i64, p]),
             "wbx_paired": (i32, [m, p, p, i64, i64, i64, i64, p]),
             "wbx_argmin": (i32, [m, p, i64, i64, i64, p, i64, i64, i64, i64, p, p, p]),

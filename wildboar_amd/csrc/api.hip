@@ -60,7 +60,8 @@ int check_metric(const wbx_metric *m) {
 }
 
 hipStream_t pick_stream(DeviceCtx *ctx, void *stream) {
-    return stream ? (hipStream_t)stream : ctx->stream;
+    (void)ctx;
+    return (hipStream_t)stream;  // NULL is the default (null) stream
 }
 
 // Orders this call after the arena's previous user (possibly on another stream).
@@ -70,35 +71,6 @@ int begin(DeviceCtx *ctx, hipStream_t st) {
 }
 int end(DeviceCtx *ctx, hipStream_t st) {
     WBX_HIP_CHECK(hipEventRecord(ctx->last, st));
-    return WBX_OK;
-}
-
-// WDTW weights 1 / (1 + exp(-g (i - n/2))) with the host libm exp, as
-// WeightedDtwMetric.reset (distance/_elastic.pyx:3334-3341) computes them.
-int get_weights(DeviceCtx *ctx, int64_t n, double g, const double **w64, const float **w32) {
-    if (ctx->w_n != n || ctx->w_g != g) {
-        WBX_HIP_CHECK(hipDeviceSynchronize());
-        if (ctx->w64) (void)hipFree(ctx->w64);
-        if (ctx->w32) (void)hipFree(ctx->w32);
-        ctx->w64 = nullptr;
-        ctx->w32 = nullptr;
-        ctx->w_n = -1;
-        std::vector<double> h((size_t)n);
-        std::vector<float> hf((size_t)n);
-        for (int64_t i = 0; i < n; ++i) {
-            h[i] = 1.0 / (1.0 + exp(-g * ((double)i - (double)n / 2.0)));
-            hf[i] = (float)h[i];
-        }
-        if (hipMalloc(&ctx->w64, sizeof(double) * n) != hipSuccess ||
-            hipMalloc(&ctx->w32, sizeof(float) * n) != hipSuccess)
-            return fail(WBX_ENOMEM, "weights allocation failed");
-        WBX_HIP_CHECK(hipMemcpy(ctx->w64, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-        WBX_HIP_CHECK(hipMemcpy(ctx->w32, hf.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-        ctx->w_n = n;
-        ctx->w_g = g;
-    }
-    *w64 = ctx->w64;
-    *w32 = ctx->w32;
     return WBX_OK;
 }
 
@@ -233,7 +205,7 @@ int pairwise_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *X,
     if (m->kind == WBX_WDTW) {
         const double *w64;
         const float *w32;
-        RC(get_weights(ctx, tx > ty ? tx : ty, m->g, &w64, &w32));
+        RC(cached_weights(ctx, tx > ty ? tx : ty, m->g, &w64, &w32));
         w = dbl ? (const T *)w64 : (const T *)w32;
     }
     T *sq = (T *)(ws + o_sq);
@@ -276,7 +248,7 @@ int argmin_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *X, i
     RC(stage<double>(dtype, Y, ny, ty, ystr, ws, o_y64, &y64, st));
     const double *w64 = nullptr;
     const float *w32 = nullptr;
-    if (m->kind == WBX_WDTW) RC(get_weights(ctx, tx > ty ? tx : ty, m->g, &w64, &w32));
+    if (m->kind == WBX_WDTW) RC(cached_weights(ctx, tx > ty ? tx : ty, m->g, &w64, &w32));
     double *D64 = (double *)(ws + o_d64);
     if (!certified) {
         PairSpec ps{PAIR_GRID, np, ny, nullptr, nullptr};
@@ -367,7 +339,7 @@ int subseq_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
     RC(stage<double>(dtype, S, 1, total, total, ws, o_s, &s64, st));
     const double *w64 = nullptr;
     const float *w32 = nullptr;
-    if (m->kind == WBX_WDTW) RC(get_weights(ctx, t, m->g, &w64, &w32));
+    if (m->kind == WBX_WDTW) RC(cached_weights(ctx, t, m->g, &w64, &w32));
     Shapelets sh{s64.ptr, s_off, s_len, n_s};
     return launch_subseq_exact<double>(m->kind, m->r, sh, (int)max_len,
                                        series<double>(x64.ptr, x64.stride, t), n, w64, dist, idx,
@@ -519,7 +491,7 @@ int wbx_dev_paired(const wbx_metric *metric, int dtype, const void *X, int64_t x
         char *ws = (char *)wsp;
         const double *w64 = nullptr;
         const float *w32 = nullptr;
-        if (metric->kind == WBX_WDTW) rc = get_weights(ctx, t, metric->g, &w64, &w32);
+        if (metric->kind == WBX_WDTW) rc = cached_weights(ctx, t, metric->g, &w64, &w32);
         if (!rc && dbl) {
             Staged<double> sx, sy;
             rc = stage<double>(dtype, X, n, t, x_stride, ws, o_x, &sx, st);

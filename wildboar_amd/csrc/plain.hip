@@ -21,6 +21,7 @@ namespace wbx {
 template <typename T>
 __global__ __launch_bounds__(64) void k_plain_dtw(PairSpec ps, Series<T> xs,
                                                   Series<T> ys, double r,
+                                                  const double *__restrict__ rp,
                                                   const T *__restrict__ weights,
                                                   int cap, T *__restrict__ sq) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -30,7 +31,8 @@ __global__ __launch_bounds__(64) void k_plain_dtw(PairSpec ps, Series<T> xs,
     int64_t a, b;
     decode_pair(ps, p, a, b);
     const int lx = xs.length(a), ly = ys.length(b);
-    const T res = block_dtw<T>(xs.row(a), lx, ys.row(b), ly, band_w(lx, ly, r), weights, buf,
+    const double rr = rp ? rp[a] : r;
+    const T res = block_dtw<T>(xs.row(a), lx, ys.row(b), ly, band_w(lx, ly, rr), weights, buf,
                                buf + cap, buf + 2 * cap);
     if (threadIdx.x == 0) sq[p] = res;
 }
@@ -47,7 +49,7 @@ static int set_lds(const void *fn, size_t lds) {
 
 template <typename T>
 int launch_plain_dtw(const PairSpec &ps, Series<T> xs, Series<T> ys, double r,
-                     const T *weights, T *sq, hipStream_t st) {
+                     const T *weights, T *sq, hipStream_t st, const double *rp) {
     if (ps.n_pairs <= 0) return WBX_OK;
     const int cap = xs.len;  // every row length of xs is <= xs.len
     const size_t lds = (size_t)3 * (size_t)cap * sizeof(T);
@@ -55,15 +57,15 @@ int launch_plain_dtw(const PairSpec &ps, Series<T> xs, Series<T> ys, double r,
     if (rc) return rc;
     const int64_t nb = ps.n_pairs;
     dim3 grid((unsigned)(nb < 65536 ? nb : 65536), (unsigned)((nb + 65535) / 65536));
-    hipLaunchKernelGGL(k_plain_dtw<T>, grid, dim3(64), lds, st, ps, xs, ys, r, weights, cap, sq);
+    hipLaunchKernelGGL(k_plain_dtw<T>, grid, dim3(64), lds, st, ps, xs, ys, r, rp, weights, cap, sq);
     WBX_HIP_CHECK(hipGetLastError());
     return WBX_OK;
 }
 
 template int launch_plain_dtw<float>(const PairSpec &, Series<float>, Series<float>, double,
-                                     const float *, float *, hipStream_t);
+                                     const float *, float *, hipStream_t, const double *);
 template int launch_plain_dtw<double>(const PairSpec &, Series<double>, Series<double>, double,
-                                      const double *, double *, hipStream_t);
+                                      const double *, double *, hipStream_t, const double *);
 
 // ------------------------------------------------------------- euclidean
 // euclidean_distance (distance/_metric.pyx:654-683) with S = x, T = y: windows

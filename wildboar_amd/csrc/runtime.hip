@@ -1,4 +1,5 @@
 // runtime.hip -- error state, per-device context and scratch arena.
+#include <math.h>
 #include <mutex>
 #include <stdarg.h>
 #include <stdio.h>
@@ -79,6 +80,35 @@ int ws_reserve(DeviceCtx *ctx, size_t bytes, void **out) {
         ctx->ws.size = bytes;
     }
     *out = ctx->ws.ptr;
+    return WBX_OK;
+}
+
+// WDTW weights 1 / (1 + exp(-g (i - n/2))) with the host libm exp, as
+// WeightedDtwMetric.reset (distance/_elastic.pyx:3334-3341) computes them.
+int cached_weights(DeviceCtx *ctx, int64_t n, double g, const double **w64, const float **w32) {
+    if (ctx->w_n != n || ctx->w_g != g) {
+        WBX_HIP_CHECK(hipDeviceSynchronize());
+        if (ctx->w64) (void)hipFree(ctx->w64);
+        if (ctx->w32) (void)hipFree(ctx->w32);
+        ctx->w64 = nullptr;
+        ctx->w32 = nullptr;
+        ctx->w_n = -1;
+        std::vector<double> h((size_t)n);
+        std::vector<float> hf((size_t)n);
+        for (int64_t i = 0; i < n; ++i) {
+            h[i] = 1.0 / (1.0 + exp(-g * ((double)i - (double)n / 2.0)));
+            hf[i] = (float)h[i];
+        }
+        if (hipMalloc(&ctx->w64, sizeof(double) * n) != hipSuccess ||
+            hipMalloc(&ctx->w32, sizeof(float) * n) != hipSuccess)
+            return fail(WBX_ENOMEM, "weights allocation failed");
+        WBX_HIP_CHECK(hipMemcpy(ctx->w64, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        WBX_HIP_CHECK(hipMemcpy(ctx->w32, hf.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+        ctx->w_n = n;
+        ctx->w_g = g;
+    }
+    *w64 = ctx->w64;
+    *w32 = ctx->w32;
     return WBX_OK;
 }
 

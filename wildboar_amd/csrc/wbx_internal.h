@@ -148,13 +148,16 @@ struct DeviceCtx {
 
 DeviceCtx *ctx_current();              // current device's context (created lazily)
 int ws_reserve(DeviceCtx *ctx, size_t bytes, void **out);
+// WDTW weights for width n and penalty g, computed on the host with libm exp
+// and cached on the device (one entry per device).
+int cached_weights(DeviceCtx *ctx, int64_t n, double g, const double **w64, const float **w32);
 
 // ---------------------------------------------------------------- launchers
 // Plain one-pair-per-block DTW (any band, any lengths, WDTW quirk): writes the
-// squared cost of each pair into sq[p].
+// squared cost of each pair into sq[p].  rp (optional) gives r per x row.
 template <typename T>
 int launch_plain_dtw(const PairSpec &ps, Series<T> xs, Series<T> ys, double r,
-                     const T *weights, T *sq, hipStream_t st);
+                     const T *weights, T *sq, hipStream_t st, const double *rp = nullptr);
 
 // Euclidean: one thread per pair, sequential sum in the reference's order.
 template <typename T>
@@ -177,6 +180,11 @@ int launch_finish(const PairSpec &ps, const T *sq, double *out, int64_t ld,
 int launch_argmin_replay(const double *D, int64_t nx, int64_t ny, int64_t k,
                          const double *lower_bound, void *heaps, int64_t *idx, double *dist,
                          hipStream_t st);
+
+// End-of-series lengths: len[i] = index of the first EOS of row i (or t).
+template <typename T>
+int launch_eos_lengths(const T *X, int64_t n, int64_t stride, int t, int32_t *len,
+                       hipStream_t st);
 
 // Conversions and fills.
 int launch_f64_to_f32(const double *src, float *dst, int64_t n, hipStream_t st);
