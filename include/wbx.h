@@ -183,6 +183,32 @@ int wbx_batch_run(void *plan, int dtype, const void *X, int64_t x_stride,
                   const void *Y, int64_t y_stride, double *out, void *stream,
                   int use_graph);
 
+/* ------------------------------------------- differentiable DTW
+ * No single reference driver: DTW distances with gradients along the optimal
+ * path of dtw_mapping's backtrack (distance/dtw.py:362-427, ties diag > up >
+ * left), for DTW barycenter averaging (the role of dtw_average,
+ * distance/dtw.py:429-690) and the counterfactual explainers.  FP32, metric
+ * dtw, all series of X of length tx and of Y of length ty.
+ * mode 0: all pairs, D is (nx, ny); mode 1: D[p] = DTW(X[p], Y[p]), nx == ny.
+ * The forward writes `state` (state_bytes) that the backward reads; the
+ * backward needs `scratch` (scratch_bytes).  G is dL/dD in D's layout; dX
+ * (nx, tx) and dY (ny, ty) are float32 with row strides ldx, ldy. */
+int wbx_dtw_train_sizes(const wbx_metric *metric, int mode, int64_t nx,
+                        int64_t ny, int64_t tx, int64_t ty,
+                        int64_t *state_bytes, int64_t *scratch_bytes);
+int wbx_dev_dtw_train_forward(const wbx_metric *metric, int mode,
+                              const float *X, int64_t nx, int64_t x_stride,
+                              int64_t tx, const float *Y, int64_t ny,
+                              int64_t y_stride, int64_t ty, double *D,
+                              void *state, void *stream);
+int wbx_dev_dtw_train_backward(const wbx_metric *metric, int mode,
+                               const float *X, int64_t nx, int64_t x_stride,
+                               int64_t tx, const float *Y, int64_t ny,
+                               int64_t y_stride, int64_t ty, const double *G,
+                               const void *state, float *dX, int64_t ldx,
+                               float *dY, int64_t ldy, void *scratch,
+                               void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,142 @@
+"""GPU checks of the differentiable DTW (wildboar_amd.dtw).
+
+- forward values within 1e-5 relative of the float64 oracle;
+- gradients against the float64 path-backtrack gradient (oracle.dtw_grad,
+  restating _dtw_alignment + dtw_mapping) on tie-free pairs: every step of the
+  float64 path wins by a relative margin > 1e-4, so float32 costs pick the
+  same path; tolerance 1e-4 relative to the gradient's max magnitude;
+- central finite differences (eps = 1e-4) of the float64 distance;
+- bitwise determinism of the backward;
+- DTW barycenter averaging lowers the mean distance.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from wildboar_amd.dtw import dtw, dtw_barycenter, paired_dtw
+
+pytestmark = pytest.mark.gpu
+
+
+def rw(seed, n, t):
+    rng = np.random.default_rng(seed)
+    x = np.cumsum(rng.standard_normal((n, t)), axis=1)
+    return ((x - x.mean(axis=1, keepdims=True)) / x.std(axis=1, keepdims=True)).astype(np.float32)
+
+
+def path_margin(x, y, r):
+    """Smallest relative gap between the chosen and the next-best option
+    along the float64 backtrack (inf when the path is forced everywhere)."""
+    band = oracle.compute_r(min(len(x), len(y)), r)
+    A = oracle.cost_matrix(x, y, band)
+    pi, pj = oracle.mapping(A)
+    best = math.inf
+    for i, j in zip(pi[1:], pj[1:]):
+        opts = sorted([A[i - 1, j - 1] if i > 0 and j > 0 else math.inf,
+                       A[i - 1, j] if i > 0 else math.inf,
+                       A[i, j - 1] if j > 0 else math.inf])
+        if math.isfinite(opts[1]) and opts[0] > 0:
+            best = min(best, (opts[1] - opts[0]) / opts[0])
+    return best
+
+
+def test_forward_values_match_oracle():
+    X = rw(1, 64, 96)
+    Y = rw(2, 48, 96)
+    D = dtw(torch.from_numpy(X).cuda(), torch.from_numpy(Y).cuda(), 0.1).cpu().numpy()
+    want = oracle.pairwise(X.astype(np.float64), Y.astype(np.float64), metric="dtw", r=0.1)
+    assert np.max(np.abs(D - want) / want) <= 1e-5
+    P = paired_dtw(torch.from_numpy(X[:48]).cuda(), torch.from_numpy(Y).cuda(), 0.1)
+    wantp = np.array([oracle.metric_distance(X[i], Y[i], "dtw", r=0.1) for i in range(48)])
+    assert np.max(np.abs(P.cpu().numpy() - wantp) / wantp) <= 1e-5
+
+
+@pytest.mark.parametrize("t,r", [(64, 0.1), (128, 0.1), (100, 0.3), (40, 0.0)])
+def test_paired_gradient_vs_backtrack_oracle(t, r):
+    X = rw(3, 40, t)
+    Y = rw(4, 40, t)
+    Xd = torch.from_numpy(X).cuda().requires_grad_(True)
+    Yd = torch.from_numpy(Y).cuda().requires_grad_(True)
+    G = torch.from_numpy(np.random.default_rng(5).uniform(0.5, 1.5, 40)).cuda()
+    (paired_dtw(Xd, Yd, r) * G).sum().backward()
+    gX = Xd.grad.cpu().numpy()
+    gY = Yd.grad.cpu().numpy()
+    checked = 0
+    for i in range(40):
+        x64, y64 = X[i].astype(np.float64), Y[i].astype(np.float64)
+        if path_margin(x64, y64, r) <= 1e-4:
+            continue
+        _, ox, oy = oracle.dtw_grad(x64, y64, r)
+        g = float(G[i])
+        scale = max(np.abs(ox).max(), 1e-3)
+        assert np.max(np.abs(gX[i] - g * ox)) <= 1e-4 * scale * max(g, 1.0), i
+        assert np.max(np.abs(gY[i] - g * oy)) <= 1e-4 * scale * max(g, 1.0), i
+        checked += 1
+    assert checked >= 20
+
+
+def test_grid_gradient_sums_pairs():
+    X = rw(6, 12, 64)
+    Y = rw(7, 9, 64)
+    Xd = torch.from_numpy(X).cuda().requires_grad_(True)
+    Yd = torch.from_numpy(Y).cuda().requires_grad_(True)
+    G = np.random.default_rng(8).uniform(0.1, 1.0, (12, 9))
+    (dtw(Xd, Yd, 0.2) * torch.from_numpy(G).cuda()).sum().backward()
+    wx = np.zeros_like(X, dtype=np.float64)
+    wy = np.zeros_like(Y, dtype=np.float64)
+    for a in range(12):
+        for b in range(9):
+            x64, y64 = X[a].astype(np.float64), Y[b].astype(np.float64)
+            assert path_margin(x64, y64, 0.2) > 1e-5, (a, b)
+            _, ox, oy = oracle.dtw_grad(x64, y64, 0.2)
+            wx[a] += G[a, b] * ox
+            wy[b] += G[a, b] * oy
+    np.testing.assert_allclose(Xd.grad.cpu().numpy(), wx, rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(Yd.grad.cpu().numpy(), wy, rtol=1e-4, atol=1e-4)
+
+
+def test_gradient_central_differences_eps_1e4():
+    X = rw(9, 8, 48)
+    Y = rw(10, 8, 48)
+    Xd = torch.from_numpy(X).cuda().requires_grad_(True)
+    paired_dtw(Xd, torch.from_numpy(Y).cuda(), 0.1).sum().backward()
+    g = Xd.grad.cpu().numpy()
+    eps = 1e-4
+    done = 0
+    for i in range(8):
+        x64, y64 = X[i].astype(np.float64), Y[i].astype(np.float64)
+        if path_margin(x64, y64, 0.1) <= 1e-3:
+            continue
+        for q in range(0, 48, 5):
+            xp, xm = x64.copy(), x64.copy()
+            xp[q] += eps
+            xm[q] -= eps
+            fd = (oracle.metric_distance(xp, y64, "dtw", r=0.1)
+                  - oracle.metric_distance(xm, y64, "dtw", r=0.1)) / (2 * eps)
+            assert abs(fd - g[i, q]) <= 1e-3 * max(1.0, abs(fd)), (i, q, fd, g[i, q])
+        done += 1
+    assert done >= 4
+
+
+def test_backward_is_deterministic():
+    X = torch.from_numpy(rw(11, 256, 128)).cuda()
+    Y = torch.from_numpy(rw(12, 16, 128)).cuda()
+    grads = []
+    for _ in range(2):
+        Xd = X.clone().requires_grad_(True)
+        Yd = Y.clone().requires_grad_(True)
+        dtw(Xd, Yd, 0.1).sum().backward()
+        grads.append((Xd.grad.clone(), Yd.grad.clone()))
+    assert torch.equal(grads[0][0], grads[1][0]) and torch.equal(grads[0][1], grads[1][1])
+
+
+def test_barycenter_lowers_cost():
+    X = torch.from_numpy(rw(13, 64, 128)).cuda()
+    mean, hist = dtw_barycenter(X, 0.1, steps=60, lr=1e-2, return_history=True)
+    assert mean.shape == (128,)
+    assert hist[-1] < hist[0] * 0.95
+    assert np.isfinite(hist).all()

@@ -27,7 +27,8 @@ EXPORTS = (
     "wbx_dev_pairwise", "wbx_dev_paired", "wbx_dev_argmin",
     "wbx_dev_subsequence_pairwise", "wbx_set_policy", "wbx_prof_enable", "wbx_prof_take",
     "wbx_prof_valu_peak", "wbx_dev_eos_lengths", "wbx_batch_create", "wbx_batch_free",
-    "wbx_batch_info", "wbx_batch_run",
+    "wbx_batch_info", "wbx_batch_run", "wbx_dtw_train_sizes", "wbx_dev_dtw_train_forward",
+    "wbx_dev_dtw_train_backward",
 )
 
 
@@ -77,6 +78,10 @@ def load():
             "wbx_batch_free": (i32, [p]),
             "wbx_batch_info": (i32, [p, p, p, p]),
             "wbx_batch_run": (i32, [p, i32, p, i64, p, i64, p, p, i32]),
+            "wbx_dtw_train_sizes": (i32, [m, i32, i64, i64, i64, i64, p, p]),
+            "wbx_dev_dtw_train_forward": (i32, [m, i32, p, i64, i64, i64, p, i64, i64, i64, p, p, p]),
+            "wbx_dev_dtw_train_backward": (i32, [m, i32, p, i64, i64, i64, p, i64, i64, i64, p, p,
+                                                 p, i64, p, i64, p, p]),
             "wbx_pairwise": (i32, [m, p, i64, i64, i64, p, i64, i64, i64, p]),
             "wbx_paired": (i32, [m, p, p, i64, i64, i64, i64, p]),
             "wbx_argmin": (i32, [m, p, i64, i64, i64, p, i64, i64, i64, i64, p, p, p]),

@@ -18,19 +18,10 @@
 // 0 in slot off, which turns the first row into the reference's prefix sum.
 // Slots >= B (the padding up to BP) compute throw-away values; slot B is reset
 // to +inf after every row so slot B-1 never sees them.
-#include "wbx_internal.h"
+#include "lane_common.h"
 
 namespace wbx {
 
-struct LaneGeo {
-    int lx, ly, off, B, kf;
-};
-
-__device__ __forceinline__ float ldy(const float *__restrict__ Y, int j, int ly) {
-    int jc = j < 0 ? 0 : (j >= ly ? ly - 1 : j);
-    float v = Y[jc];
-    return (unsigned)j < (unsigned)ly ? v : WBX_INF_F;
-}
 
 // pen[q] is +inf for slot BP-8+q == B and -inf otherwise (uniform values), so
 // max(row, pen) resets slot B without a branch.
