@@ -179,3 +179,36 @@ def test_argmin_fp32_float32_inputs_and_ties():
     oi, _ = oracle.argmin(X.astype(np.float64), Y.astype(np.float64), k=1, metric="dtw", r=0.1)
     np.testing.assert_array_equal(idx, oi)
     assert (idx % 3 == 0).all()
+
+
+# ----------------------------------------------------------------- subsequence FP32
+@pytest.mark.parametrize("metric,r", [("dtw", 0.1), ("dtw", 0.3), ("euclidean", 0.1)])
+def test_subsequence_fp32_certified_exact(metric, r):
+    X = rw(30, 23, 300)
+    rng = np.random.default_rng(31)
+    shapelets = []
+    for L in (16, 17, 33, 64, 90, 128, 5, 1):
+        i = int(rng.integers(23))
+        s = int(rng.integers(0, 300 - L + 1))
+        shapelets.append(X[i, s:s + L].copy())
+    params = {} if metric == "euclidean" else {"r": r}
+    d, i = wd.pairwise_subsequence_distance(shapelets, X, metric=metric, metric_params=params,
+                                            return_index=True, precision="fp32")
+    od, oi = oracle.subsequence(shapelets, X, metric=metric, r=r)
+    np.testing.assert_array_equal(i, oi)
+    np.testing.assert_array_equal(d, od)
+
+
+def test_subsequence_fp32_ties_overflow_and_wide_band():
+    # periodic series: many windows tie exactly -> candidate lists overflow
+    t = np.arange(256)
+    X = np.stack([np.sin(2 * np.pi * t / 16.0), np.cos(2 * np.pi * t / 32.0)]).astype(np.float32)
+    shapelets = [X[0, 3:35].copy(), X[1, 10:42].copy(), X[0, 0:100].copy()]
+    for metric, params in (("dtw", {"r": 0.2}), ("dtw", {"r": 1.0}), ("euclidean", {})):
+        d, i = wd.pairwise_subsequence_distance(shapelets, X.astype(np.float64), metric=metric,
+                                                metric_params=params, return_index=True,
+                                                precision="fp32")
+        od, oi = oracle.subsequence(shapelets, X.astype(np.float64), metric=metric,
+                                    r=params.get("r", 1.0))
+        np.testing.assert_array_equal(i, oi)
+        np.testing.assert_array_equal(d, od)

@@ -8,6 +8,7 @@
 //   argmin FP32    FP32 distances -> certified candidate set -> exact FP64
 //                  re-evaluation of the candidates -> heap replay, so indices
 //                  and distances equal the reference's.
+#include <algorithm>
 #include <atomic>
 #include <math.h>
 #include <mutex>
@@ -16,6 +17,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "subseq_fast.h"
 #include "wbx_internal.h"
 
 namespace wbx {
@@ -115,7 +117,7 @@ struct ProfScope {
     }
 };
 
-enum { KERN_EUCLID = 1, KERN_LANE = 2, KERN_PLAIN = 3 };
+enum { KERN_EUCLID = 1, KERN_LANE = 2, KERN_PLAIN = 3, KERN_BATCH = 4, KERN_SUBSEQ = 5 };
 
 template <typename T>
 int run_pairs(const wbx_metric *m, const PairSpec &ps, Series<T> xs, Series<T> ys,
@@ -311,6 +313,121 @@ int argmin_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *X, i
 }
 
 // -------------------------------------------------------------- subsequence
+// FP32 scan + certified float64 finish (subseq_fast.hip).  Shapelets are
+// grouped into band classes (BP = band slots rounded up to 8); each class is
+// one launch over every sample.
+int subseq_fast_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
+                     const int64_t *s_off, const int64_t *s_len, int64_t n_s,
+                     const std::vector<int64_t> &hl, const void *X, int64_t n, int64_t xstr,
+                     int64_t t, int64_t total, int64_t max_len, double *dist, int64_t *idx,
+                     hipStream_t st) {
+    const bool euclid = m->kind == WBX_EUCLIDEAN;
+    std::vector<std::vector<int32_t>> cls(9);  // BP/8 = 1..8 fast; 0 = exact rescan
+    for (int64_t j = 0; j < n_s; ++j) {
+        int c = 1;
+        if (!euclid) {
+            const int w = band_w((int)hl[j], (int)hl[j], m->r);
+            const int bp = (2 * w - 1 + 7) / 8 * 8;
+            c = bp <= 64 ? bp / 8 : 0;
+        }
+        cls[c].push_back((int32_t)j);
+    }
+    int64_t max_tasks = 0;
+    for (auto &v : cls) max_tasks = std::max<int64_t>(max_tasks, n * (int64_t)v.size());
+    Carve cv;
+    const size_t o_x32 = cv.take(stage_bytes(dtype, false, n, t));
+    const size_t o_s32 = cv.take(stage_bytes(dtype, false, 1, total));
+    const size_t o_x64 = cv.take(stage_bytes(dtype, true, n, t));
+    const size_t o_s64 = cv.take(stage_bytes(dtype, true, 1, total));
+    const size_t o_cand = cv.take((size_t)max_tasks * sizeof(SubCand));
+    const size_t o_cost = cv.take((size_t)max_tasks * kMaxCand * 8);
+    const size_t o_ovf = cv.take((size_t)max_tasks * 4);
+    const size_t o_list = cv.take((size_t)n_s * 4);
+    const size_t o_cnt = cv.take(64);
+    void *wsp = nullptr;
+    RC(ws_reserve(ctx, cv.total, &wsp));
+    char *ws = (char *)wsp;
+    Staged<float> x32, s32;
+    Staged<double> x64, s64;
+    RC(stage<float>(dtype, X, n, t, xstr, ws, o_x32, &x32, st));
+    RC(stage<float>(dtype, S, 1, total, total, ws, o_s32, &s32, st));
+    RC(stage<double>(dtype, X, n, t, xstr, ws, o_x64, &x64, st));
+    RC(stage<double>(dtype, S, 1, total, total, ws, o_s64, &s64, st));
+    double absmax = 0.0;
+    const bool exact_inputs = dtype == WBX_F32;
+    int *cnt = (int *)(ws + o_cnt);
+    if (!exact_inputs) {
+        unsigned *mx = (unsigned *)(cnt + 4);
+        RC(launch_fill_i32(cnt, 16, 0, st));
+        RC(launch_absmax_rows<double>((const double *)X, n, t, xstr, mx, st));
+        RC(launch_absmax_rows<double>((const double *)S, 1, total, total, mx, st));
+        unsigned hb = 0;
+        WBX_HIP_CHECK(hipMemcpyAsync(&hb, mx, 4, hipMemcpyDeviceToHost, st));
+        WBX_HIP_CHECK(hipStreamSynchronize(st));
+        float f;
+        memcpy(&f, &hb, 4);
+        absmax = (double)f;
+    }
+    int32_t *list = (int32_t *)(ws + o_list);
+    SubCand *cands = (SubCand *)(ws + o_cand);
+    double *cost = (double *)(ws + o_cost);
+    int32_t *ovf = (int32_t *)(ws + o_ovf);
+    for (int c = 1; c <= 8; ++c) {
+        const std::vector<int32_t> &v = cls[c];
+        if (v.empty()) continue;
+        int Lmax = 0;
+        for (int32_t j : v) Lmax = std::max<int>(Lmax, (int)hl[j]);
+        // path cells P: 2 L - 1 for the DTW recursion, L summed terms for euclidean
+        const int P = euclid ? Lmax + 1 : Lmax;
+        const ErrBound eb = make_err_bound(P, P, 0, absmax, exact_inputs);
+        const double e64 = eb.e64;
+        SubGeo sg;
+        sg.S = s32.ptr;
+        sg.off = s_off;
+        sg.len = s_len;
+        sg.X = x32.ptr;
+        sg.xstride = x32.stride;
+        sg.T = (int)t;
+        sg.r = m->r;
+        sg.rho = (float)(eb.e_up / eb.e_dn * (1.0 + 3.0 * e64) * (1.0 + 1e-6));
+        sg.A = (float)(eb.A * (1.0 + 1e-6));
+        WBX_HIP_CHECK(hipMemcpyAsync(list, v.data(), sizeof(int32_t) * v.size(),
+                                     hipMemcpyHostToDevice, st));
+        TaskMap tm{list, (int32_t)v.size()};
+        const int64_t nt = n * (int64_t)v.size();
+        {
+            ProfScope pf(st, KERN_SUBSEQ);
+            RC(launch_subseq_fast(m->kind, c * 8, tm, nt, sg, cands, st));
+        }
+        RC(launch_subseq_exact_cands(m->kind, c * 8, tm, cands, nt, s64.ptr, s_off, s_len, x64.ptr,
+                                     x64.stride, (int)t, m->r, cost, st));
+        RC(launch_fill_i32(cnt, 1, 0, st));
+        RC(launch_subseq_select(tm, cands, nt, cost, n_s, dist, idx, ovf, cnt, st));
+        int novf = 0;
+        WBX_HIP_CHECK(hipMemcpyAsync(&novf, cnt, 4, hipMemcpyDeviceToHost, st));
+        WBX_HIP_CHECK(hipStreamSynchronize(st));
+        RC(launch_subseq_rescan(tm, ovf, cnt, novf, m->kind, s64.ptr, s_off, s_len, x64.ptr,
+                                x64.stride, (int)t, m->r, (int)max_len, n_s, dist, idx, st));
+    }
+    if (!cls[0].empty()) {  // bands too wide for the FP32 kernels: exact rescan
+        const std::vector<int32_t> &v = cls[0];
+        const int64_t nt = n * (int64_t)v.size();
+        std::vector<int32_t> all((size_t)nt);
+        for (int64_t q = 0; q < nt; ++q) all[q] = (int32_t)q;
+        WBX_HIP_CHECK(hipMemcpyAsync(list, v.data(), sizeof(int32_t) * v.size(),
+                                     hipMemcpyHostToDevice, st));
+        WBX_HIP_CHECK(hipMemcpyAsync(ovf, all.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice,
+                                     st));
+        const int ntv = (int)nt;
+        WBX_HIP_CHECK(hipMemcpyAsync(cnt, &ntv, 4, hipMemcpyHostToDevice, st));
+        TaskMap tm{list, (int32_t)v.size()};
+        RC(launch_subseq_rescan(tm, ovf, cnt, ntv, m->kind, s64.ptr, s_off, s_len, x64.ptr,
+                                x64.stride, (int)t, m->r, (int)max_len, n_s, dist, idx, st));
+        WBX_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    return WBX_OK;
+}
+
 int subseq_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
                 const int64_t *s_off, const int64_t *s_len, int64_t n_s, const void *X,
                 int64_t n, int64_t xstr, int64_t t, double *dist, int64_t *idx, hipStream_t st) {
@@ -327,23 +444,28 @@ int subseq_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
         if (hl[j] > max_len) max_len = hl[j];
         if (ho[j] + hl[j] > total) total = ho[j] + hl[j];
     }
-    // Exact path (FP64 arithmetic on float64-upcast values).
-    Carve cv;
-    const size_t o_x = cv.take(stage_bytes(dtype, true, n, t));
-    const size_t o_s = cv.take(stage_bytes(dtype, true, 1, total));
-    void *wsp = nullptr;
-    RC(ws_reserve(ctx, cv.total, &wsp));
-    char *ws = (char *)wsp;
-    Staged<double> x64, s64;
-    RC(stage<double>(dtype, X, n, t, xstr, ws, o_x, &x64, st));
-    RC(stage<double>(dtype, S, 1, total, total, ws, o_s, &s64, st));
     const double *w64 = nullptr;
     const float *w32 = nullptr;
     if (m->kind == WBX_WDTW) RC(cached_weights(ctx, t, m->g, &w64, &w32));
-    Shapelets sh{s64.ptr, s_off, s_len, n_s};
-    return launch_subseq_exact<double>(m->kind, m->r, sh, (int)max_len,
-                                       series<double>(x64.ptr, x64.stride, t), n, w64, dist, idx,
-                                       st);
+    if (m->precision == WBX_FP64 || m->kind == WBX_WDTW) {
+        // Exact path (FP64 arithmetic on float64-upcast values).
+        Carve cv;
+        const size_t o_x = cv.take(stage_bytes(dtype, true, n, t));
+        const size_t o_s = cv.take(stage_bytes(dtype, true, 1, total));
+        void *wsp = nullptr;
+        RC(ws_reserve(ctx, cv.total, &wsp));
+        char *ws = (char *)wsp;
+        Staged<double> x64, s64;
+        RC(stage<double>(dtype, X, n, t, xstr, ws, o_x, &x64, st));
+        RC(stage<double>(dtype, S, 1, total, total, ws, o_s, &s64, st));
+        Shapelets sh{s64.ptr, s_off, s_len, n_s};
+        ProfScope pf(st, KERN_SUBSEQ);
+        return launch_subseq_exact<double>(m->kind, m->r, sh, (int)max_len,
+                                           series<double>(x64.ptr, x64.stride, t), n, w64, dist,
+                                           idx, st);
+    }
+    return subseq_fast_impl(ctx, m, dtype, S, s_off, s_len, n_s, hl, X, n, xstr, t, total,
+                            max_len, dist, idx, st);
 }
 
 // Host-pointer wrapper helpers.

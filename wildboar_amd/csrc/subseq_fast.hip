@@ -1,0 +1,414 @@
+// subseq_fast.hip -- FP32 subsequence scans with a certified exact finish.
+//
+// Semantics are those of _PairwiseSubsequenceDistance (distance/_cdistance.pyx:
+// 1001-1015): for sample i and shapelet s, the minimum over windows of the
+// squared cost (dtw_subsequence_distance, distance/_elastic.pyx:622-655, or
+// euclidean_distance, distance/_metric.pyx:654-683), the first window that
+// attains it, and sqrt of that minimum.
+//
+// Pass 1 (FP32): one (sample, shapelet) task per 64-lane wave, lane l scans
+//   window ws = 64 b + l of block b.  For DTW each lane runs the register-band
+//   recursion of lane.hip on its window (all lanes share the shapelet, so the
+//   geometry is uniform); for euclidean each lane sums (x - s)^2 over its
+//   window.  A window is kept as a candidate when its FP32 cost c satisfies
+//   c <= tau(c*) = rho (c* + A) + A, c* the smallest FP32 cost so far: by the
+//   bound in rerank.hip no other window can hold the float64 minimum.  Up to
+//   kMaxCand candidates per task are stored in window order; more sets an
+//   overflow flag.
+// Pass 2 (FP64): one candidate per lane, evaluated with the reference's exact
+//   arithmetic (DTW: min(min(up, left), diag) + v*v, no FMA; euclidean: the
+//   sequential sum), then per task the first strictly smallest candidate wins.
+// Overflowed tasks are rescanned exactly (subseq.hip).
+#pragma clang fp contract(off)
+
+#include "block_dtw.h"
+#include "lane_common.h"
+#include "subseq_fast.h"
+
+namespace wbx {
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Append this block's candidates (lanes with `take`) after `count` entries.
+__device__ __forceinline__ void push_candidates(SubCand &cand_local, bool take, int ws,
+                                                SubCand *__restrict__ out) {
+    const uint64_t mask = __ballot(take);
+    const int before = __popcll(mask & ((1ull << threadIdx.x) - 1ull));
+    const int base = cand_local.count;
+    if (take && base + before < kMaxCand) out->win[base + before] = ws;
+    cand_local.count = base + (int)__popcll(mask);
+}
+
+template <int BP, int U>
+__global__ __launch_bounds__(64) void k_subseq_dtw32(TaskMap tasks, SubGeo sg,
+                                                     SubCand *__restrict__ cands) {
+    constexpr int W = BP + U - 1;
+    const int lane = threadIdx.x;
+    const SubTask tk = tasks.at(blockIdx.x);
+    const float *__restrict__ S = sg.S + sg.off[tk.shapelet];
+    const int L = (int)sg.len[tk.shapelet];
+    const int w = band_w(L, L, sg.r);
+    const int off = w - 1, B = 2 * w - 1;
+    const int nwin = sg.T - L + 1;
+    const float *__restrict__ Xrow = sg.X + (int64_t)tk.sample * sg.xstride;
+    float pen[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) pen[q] = (BP - 8 + q == B) ? WBX_INF_F : -WBX_INF_F;
+    SubCand *cout = cands + blockIdx.x;
+    SubCand cl;
+    cl.count = 0;
+    float best = WBX_INF_F;
+    for (int b0 = 0; b0 < nwin; b0 += 64) {
+        const int ws = b0 + lane;
+        const bool active = ws < nwin;
+        const float *__restrict__ Y = Xrow + (active ? ws : nwin - 1);
+        float row[BP];
+#pragma unroll
+        for (int k = 0; k < BP; ++k) row[k] = (k == off) ? 0.0f : WBX_INF_F;
+        float yw[W];
+#pragma unroll
+        for (int m = 0; m < W; ++m) yw[m] = ldy(Y, m - off, L);
+        const int rem = L % U;
+        for (int i = 0; i < rem; ++i) {
+            const float xi = S[i];
+            float left = WBX_INF_F;
+#pragma unroll
+            for (int k = 0; k < BP; ++k) {
+                const float up = (k + 1 < BP) ? row[k + 1] : WBX_INF_F;
+                const float mn = fminf(fminf(up, left), row[k]);
+                const float d = xi - yw[k];
+                const float c = __builtin_fmaf(d, d, mn);
+                row[k] = c;
+                left = c;
+            }
+#pragma unroll
+            for (int q = 1; q < 8; ++q) row[BP - 8 + q] = fmaxf(row[BP - 8 + q], pen[q]);
+#pragma unroll
+            for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
+            yw[W - 1] = ldy(Y, i + 1 - off + W - 1, L);
+        }
+        for (int i0 = rem; i0 < L; i0 += U) {
+            float yn[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) yn[u] = ldy(Y, i0 + U - off + W - U + u, L);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const float xi = S[i0 + u];
+                float left = WBX_INF_F;
+#pragma unroll
+                for (int k = 0; k < BP; ++k) {
+                    const float up = (k + 1 < BP) ? row[k + 1] : WBX_INF_F;
+                    const float mn = fminf(fminf(up, left), row[k]);
+                    const float d = xi - yw[k + u];
+                    const float c = __builtin_fmaf(d, d, mn);
+                    row[k] = c;
+                    left = c;
+                }
+#pragma unroll
+                for (int q = 1; q < 8; ++q) row[BP - 8 + q] = fmaxf(row[BP - 8 + q], pen[q]);
+            }
+#pragma unroll
+            for (int m = 0; m < W - U; ++m) yw[m] = yw[m + U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) yw[W - U + u] = yn[u];
+        }
+        float c = WBX_INF_F;
+#pragma unroll
+        for (int k = 0; k < BP; ++k)
+            if (k == off) c = row[k];
+        if (!active) c = WBX_INF_F;
+        best = fminf(best, wave_min(c));
+        const float tau = sg.rho * (best + sg.A) + sg.A;
+        push_candidates(cl, active && c <= tau, ws, cout);
+    }
+    if (lane == 0) {
+        cout->count = cl.count;
+        cout->best32 = best;
+    }
+}
+
+template <int U>
+__global__ __launch_bounds__(64) void k_subseq_euclid32(TaskMap tasks, SubGeo sg,
+                                                        SubCand *__restrict__ cands) {
+    const int lane = threadIdx.x;
+    const SubTask tk = tasks.at(blockIdx.x);
+    const float *__restrict__ S = sg.S + sg.off[tk.shapelet];
+    const int L = (int)sg.len[tk.shapelet];
+    const int nwin = sg.T - L + 1;
+    const float *__restrict__ Xrow = sg.X + (int64_t)tk.sample * sg.xstride;
+    SubCand *cout = cands + blockIdx.x;
+    SubCand cl;
+    cl.count = 0;
+    float best = WBX_INF_F;
+    // each lane scans U consecutive windows per block of 64 U
+    for (int b0 = 0; b0 < nwin; b0 += 64 * U) {
+        const int w0 = b0 + lane * U;
+        float acc[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) acc[q] = 0.0f;
+        float win[U];
+        const int last = sg.T - 1;
+#pragma unroll
+        for (int q = 0; q < U - 1; ++q) {
+            const int e = w0 + q;
+            win[q + 1] = Xrow[e < last ? e : last];
+        }
+        for (int j = 0; j < L; ++j) {
+            const float s = S[j];
+#pragma unroll
+            for (int q = 0; q < U - 1; ++q) win[q] = win[q + 1];
+            const int e = w0 + j + U - 1;
+            win[U - 1] = Xrow[e < last ? e : last];
+#pragma unroll
+            for (int q = 0; q < U; ++q) {
+                const float d = win[q] - s;
+                acc[q] = __builtin_fmaf(d, d, acc[q]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int ws = w0 + q;
+            const bool active = ws < nwin;
+            const float c = active ? acc[q] : WBX_INF_F;
+            best = fminf(best, wave_min(c));
+            const float tau = sg.rho * (best + sg.A) + sg.A;
+            // candidates of window q of every lane: in window order within
+            // this q they are lane-ordered; across q the order is fixed up by
+            // the select pass, which compares (cost, window) pairs.
+            push_candidates(cl, active && c <= tau, ws, cout);
+        }
+    }
+    if (lane == 0) {
+        cout->count = cl.count;
+        cout->best32 = best;
+    }
+}
+
+// ------------------------------------------------------------------ exact
+// Float64 cost of one candidate window per lane; per-lane geometry (all lanes
+// of a launch share BP).  Same arithmetic as dtw_distance: rows of S against
+// the window, cost = min(min(up, left), diag) + v * v.
+template <int BP>
+__global__ __launch_bounds__(64) void k_subseq_cand64(TaskMap tasks,
+                                                      const SubCand *__restrict__ cands,
+                                                      int64_t n_tasks, const double *__restrict__ S64,
+                                                      const int64_t *__restrict__ soff,
+                                                      const int64_t *__restrict__ slen,
+                                                      const double *__restrict__ X64,
+                                                      int64_t xstride, int T, double r, int kind,
+                                                      double *__restrict__ cost64) {
+    const int64_t q = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    const int64_t t = q / kMaxCand;
+    const int slot = (int)(q - t * kMaxCand);
+    if (t >= n_tasks) return;
+    const SubCand &cd = cands[t];
+    if (cd.count > kMaxCand || slot >= cd.count) return;
+    const SubTask tk = tasks.at(t);
+    const double *S = S64 + soff[tk.shapelet];
+    const int L = (int)slen[tk.shapelet];
+    const double *Y = X64 + (int64_t)tk.sample * xstride + cd.win[slot];
+    double res;
+    if (kind == WBX_EUCLIDEAN) {
+        double dist = 0.0;
+        for (int j = 0; j < L; ++j) {
+            double v = Y[j];
+            v -= S[j];
+            dist += v * v;
+        }
+        res = dist;
+    } else {
+        const int w = band_w(L, L, r);
+        const int off = w - 1, B = 2 * w - 1;
+        double row[BP];
+#pragma unroll
+        for (int k = 0; k < BP; ++k) row[k] = (k == off) ? 0.0 : WBX_INF_D;
+        for (int i = 0; i < L; ++i) {
+            const double xi = S[i];
+            double left = WBX_INF_D;
+#pragma unroll
+            for (int k = 0; k < BP; ++k) {
+                const double up = (k + 1 < BP) ? row[k + 1] : WBX_INF_D;
+                const int j = i + k - off;
+                const double yj = ((unsigned)j < (unsigned)L) ? Y[j < 0 ? 0 : (j >= L ? L - 1 : j)]
+                                                              : WBX_INF_D;
+                const double v = xi - yj;
+                const double dg = row[k];
+                const double mn = (left < up ? left : up);
+                const double m = (dg < mn ? dg : mn);
+                double c = m + v * v;
+                if (k >= B) c = WBX_INF_D;
+                row[k] = c;
+                left = c;
+            }
+        }
+        res = WBX_INF_D;
+#pragma unroll
+        for (int k = 0; k < BP; ++k)
+            if (k == off) res = row[k];
+    }
+    cost64[q] = res;
+}
+
+// Per task: first strictly smallest exact cost among the candidates (window
+// order = index order; euclidean candidates are sorted here first).
+__global__ __launch_bounds__(256) void k_subseq_select(TaskMap tasks,
+                                                       const SubCand *__restrict__ cands,
+                                                       int64_t n_tasks,
+                                                       const double *__restrict__ cost64,
+                                                       int64_t n_shapelets,
+                                                       double *__restrict__ dist,
+                                                       int64_t *__restrict__ idx,
+                                                       int32_t *__restrict__ overflow,
+                                                       int *__restrict__ n_overflow) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n_tasks) return;
+    const SubCand &cd = cands[t];
+    const SubTask tk = tasks.at(t);
+    if (cd.count > kMaxCand || cd.count == 0) {
+        const int o = atomicAdd(n_overflow, 1);
+        overflow[o] = (int32_t)t;
+        return;
+    }
+    double best = WBX_INF_D;
+    int bw = 0x7fffffff;
+    for (int s = 0; s < cd.count; ++s) {
+        const double c = cost64[t * kMaxCand + s];
+        const int wv = cd.win[s];
+        if (c < best || (c == best && wv < bw)) {
+            best = c;
+            bw = wv;
+        }
+    }
+    const int64_t o = (int64_t)tk.sample * n_shapelets + tk.shapelet;
+    dist[o] = sqrt(best);
+    idx[o] = bw;
+}
+
+// Exact full scan of the tasks whose candidate list overflowed: one task per
+// 64-lane block, windows in order (the plain kernel's block recursion).
+__global__ __launch_bounds__(64) void k_subseq_rescan(TaskMap tasks, const int32_t *__restrict__ list,
+                                                      const int *__restrict__ n_list, int kind,
+                                                      const double *__restrict__ S64,
+                                                      const int64_t *__restrict__ soff,
+                                                      const int64_t *__restrict__ slen,
+                                                      const double *__restrict__ X64,
+                                                      int64_t xstride, int T, double r, int cap,
+                                                      int64_t n_shapelets, double *__restrict__ dist,
+                                                      int64_t *__restrict__ idx) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double *buf = reinterpret_cast<double *>(smem_raw);
+    if ((int)blockIdx.x >= *n_list) return;
+    const SubTask tk = tasks.at(list[blockIdx.x]);
+    const double *S = S64 + soff[tk.shapelet];
+    const int L = (int)slen[tk.shapelet];
+    const double *x = X64 + (int64_t)tk.sample * xstride;
+    double best = WBX_INF_D;
+    int64_t bi = 0;
+    if (kind == WBX_EUCLIDEAN) {
+        if (threadIdx.x == 0) {
+            for (int s = 0; s < T - L + 1; ++s) {
+                double d = 0.0;
+                for (int k = 0; k < L; ++k) {
+                    if (d >= best) break;
+                    double v = x[s + k];
+                    v -= S[k];
+                    d += v * v;
+                }
+                if (d < best) {
+                    best = d;
+                    bi = s;
+                }
+            }
+        }
+    } else {
+        const int w = band_w(L, L, r);
+        for (int s = 0; s <= T - L; ++s) {
+            const double c = block_dtw<double>(S, L, x + s, L, w, nullptr, buf, buf + cap,
+                                               buf + 2 * cap);
+            if (c < best) {
+                best = c;
+                bi = s;
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        const int64_t o = (int64_t)tk.sample * n_shapelets + tk.shapelet;
+        dist[o] = sqrt(best);
+        idx[o] = bi;
+    }
+}
+
+int launch_subseq_rescan(TaskMap tasks, const int32_t *list, const int *n_list_dev,
+                         int n_list_host, int kind, const double *S64, const int64_t *soff,
+                         const int64_t *slen, const double *X64, int64_t xstride, int T, double r,
+                         int max_len, int64_t n_shapelets, double *dist, int64_t *idx,
+                         hipStream_t st) {
+    if (n_list_host <= 0) return WBX_OK;
+    const size_t lds = (size_t)3 * (size_t)max_len * sizeof(double);
+    if (lds > 64 * 1024) return fail(WBX_EUNSUPPORTED, "shapelet too long for the rescan");
+    hipLaunchKernelGGL(k_subseq_rescan, dim3((unsigned)n_list_host), dim3(64), lds, st, tasks, list,
+                       n_list_dev, kind, S64, soff, slen, X64, xstride, T, r, max_len, n_shapelets,
+                       dist, idx);
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
+
+// ------------------------------------------------------------------ launch
+#define SUB_DTW_CASES(X) X(8, 8) X(16, 8) X(24, 8) X(32, 8) X(40, 8) X(48, 8) X(56, 8) X(64, 8)
+
+int launch_subseq_fast(int kind, int BP, TaskMap tasks, int64_t n_tasks, SubGeo sg,
+                       SubCand *cands, hipStream_t st) {
+    if (n_tasks <= 0) return WBX_OK;
+    if (n_tasks > 0x7fffffff) return fail(WBX_EUNSUPPORTED, "too many subsequence tasks");
+    const dim3 grid((unsigned)n_tasks);
+    if (kind == WBX_EUCLIDEAN) {
+        hipLaunchKernelGGL(k_subseq_euclid32<4>, grid, dim3(64), 0, st, tasks, sg, cands);
+    } else {
+        switch (BP) {
+#define SD_CASE(B_, U_) \
+    case B_: hipLaunchKernelGGL((k_subseq_dtw32<B_, U_>), grid, dim3(64), 0, st, tasks, sg, cands); break;
+            SUB_DTW_CASES(SD_CASE)
+#undef SD_CASE
+            default: return fail(WBX_EUNSUPPORTED, "subsequence band %d", BP);
+        }
+    }
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
+
+int launch_subseq_exact_cands(int kind, int BP, TaskMap tasks, const SubCand *cands,
+                              int64_t n_tasks, const double *S64, const int64_t *soff,
+                              const int64_t *slen, const double *X64, int64_t xstride, int T,
+                              double r, double *cost64, hipStream_t st) {
+    if (n_tasks <= 0) return WBX_OK;
+    const int64_t nl = n_tasks * kMaxCand;
+    const dim3 grid((unsigned)((nl + 63) / 64));
+    switch (kind == WBX_EUCLIDEAN ? 8 : BP) {
+#define C_CASE(B_, U_)                                                                            \
+    case B_:                                                                                      \
+        hipLaunchKernelGGL(k_subseq_cand64<B_>, grid, dim3(64), 0, st, tasks, cands, n_tasks, S64, \
+                           soff, slen, X64, xstride, T, r, kind, cost64);                         \
+        break;
+        SUB_DTW_CASES(C_CASE)
+#undef C_CASE
+        default: return fail(WBX_EUNSUPPORTED, "subsequence band %d", BP);
+    }
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
+
+int launch_subseq_select(TaskMap tasks, const SubCand *cands, int64_t n_tasks,
+                         const double *cost64, int64_t n_shapelets, double *dist, int64_t *idx,
+                         int32_t *overflow, int *n_overflow, hipStream_t st) {
+    if (n_tasks <= 0) return WBX_OK;
+    hipLaunchKernelGGL(k_subseq_select, dim3((unsigned)((n_tasks + 255) / 256)), dim3(256), 0, st,
+                       tasks, cands, n_tasks, cost64, n_shapelets, dist, idx, overflow, n_overflow);
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
+
+}  // namespace wbx

@@ -52,6 +52,21 @@ int launch_fill_f64(double *dst, int64_t n, double value, hipStream_t st) {
     return WBX_OK;
 }
 
+__global__ __launch_bounds__(256) void k_fill_i32(int *__restrict__ dst, int64_t n, int v) {
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * 256)
+        dst[q] = v;
+}
+
+int launch_fill_i32(int *dst, int64_t n, int value, hipStream_t st) {
+    if (n <= 0) return WBX_OK;
+    int64_t nb = (n + 255) / 256;
+    if (nb > 16384) nb = 16384;
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)nb), dim3(256), 0, st, dst, n, value);
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
+
 // Non-negative floats order like their bit patterns, so an unsigned atomicMax
 // on the bits of fabsf(v) is a max reduction.
 template <typename S>

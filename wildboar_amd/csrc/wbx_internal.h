@@ -194,6 +194,7 @@ template <typename S, typename D>
 int launch_convert_rows(const S *src, int64_t n, int64_t len, int64_t sstride, D *dst,
                         int64_t dstride, hipStream_t st);
 int launch_fill_f64(double *dst, int64_t n, double value, hipStream_t st);
+int launch_fill_i32(int *dst, int64_t n, int value, hipStream_t st);
 // *out = max |src[i * stride + c]| as float bits (out must be zeroed first).
 template <typename S>
 int launch_absmax_rows(const S *src, int64_t n, int64_t len, int64_t stride, unsigned *out,
