@@ -116,7 +116,7 @@ def test_batch_graph_replays_match_eager():
         torch.cuda.synchronize()
         assert torch.equal(out, eager), rep
         # eager work between replays
-        Zd = (Xd * 2.0).sum()
+        Zd = (torch.nan_to_num(Xd, nan=0.0) * 2.0).sum()  # padding is EOS (NaN)
         again = plan.run(Xd, Yd, graph=False)
         torch.cuda.synchronize()
         assert torch.isfinite(Zd) and torch.equal(again, eager)

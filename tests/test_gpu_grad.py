@@ -2,10 +2,14 @@
 
 - forward values within 1e-5 relative of the float64 oracle;
 - gradients against the float64 path-backtrack gradient (oracle.dtw_grad,
-  restating _dtw_alignment + dtw_mapping) on tie-free pairs: every step of the
-  float64 path wins by a relative margin > 1e-4, so float32 costs pick the
-  same path; tolerance 1e-4 relative to the gradient's max magnitude;
-- central finite differences (eps = 1e-4) of the float64 distance;
+  restating _dtw_alignment + dtw_mapping).  The inputs are small integers, so
+  every float32 cost is an exact integer: the GPU's path equals the float64
+  path cell for cell, ties included (this also pins the diag > up > left tie
+  rule), and every pair is checked; tolerance 1e-5 relative to the gradient's
+  max magnitude (float32 sums and sqrt);
+- central finite differences (eps = 1e-4) of the float64 distance at points
+  where the float64 path is the same at x, x + eps e_q and x - eps e_q and
+  whose smallest relative gap exceeds the float32 error bound of the cost;
 - bitwise determinism of the backward;
 - DTW barycenter averaging lowers the mean distance.
 """
@@ -55,33 +59,35 @@ def test_forward_values_match_oracle():
     assert np.max(np.abs(P.cpu().numpy() - wantp) / wantp) <= 1e-5
 
 
-@pytest.mark.parametrize("t,r", [(64, 0.1), (128, 0.1), (100, 0.3), (40, 0.0)])
+def ints(seed, n, t):
+    return np.random.default_rng(seed).integers(-4, 5, (n, t)).astype(np.float32)
+
+
+def assert_grad_close(got, want, scale):
+    assert np.max(np.abs(got - want)) <= 1e-5 * scale, float(np.max(np.abs(got - want)) / scale)
+
+
+@pytest.mark.parametrize("t,r", [(64, 0.1), (128, 0.1), (100, 0.3), (40, 0.0), (256, 0.1),
+                                 (33, 1.0)])
 def test_paired_gradient_vs_backtrack_oracle(t, r):
-    X = rw(3, 40, t)
-    Y = rw(4, 40, t)
+    X = ints(3, 40, t)
+    Y = ints(4, 40, t)
     Xd = torch.from_numpy(X).cuda().requires_grad_(True)
     Yd = torch.from_numpy(Y).cuda().requires_grad_(True)
-    G = torch.from_numpy(np.random.default_rng(5).uniform(0.5, 1.5, 40)).cuda()
-    (paired_dtw(Xd, Yd, r) * G).sum().backward()
+    G = np.random.default_rng(5).uniform(0.5, 1.5, 40)
+    (paired_dtw(Xd, Yd, r) * torch.from_numpy(G).cuda()).sum().backward()
     gX = Xd.grad.cpu().numpy()
     gY = Yd.grad.cpu().numpy()
-    checked = 0
     for i in range(40):
-        x64, y64 = X[i].astype(np.float64), Y[i].astype(np.float64)
-        if path_margin(x64, y64, r) <= 1e-4:
-            continue
-        _, ox, oy = oracle.dtw_grad(x64, y64, r)
-        g = float(G[i])
-        scale = max(np.abs(ox).max(), 1e-3)
-        assert np.max(np.abs(gX[i] - g * ox)) <= 1e-4 * scale * max(g, 1.0), i
-        assert np.max(np.abs(gY[i] - g * oy)) <= 1e-4 * scale * max(g, 1.0), i
-        checked += 1
-    assert checked >= 20
+        _, ox, oy = oracle.dtw_grad(X[i].astype(np.float64), Y[i].astype(np.float64), r)
+        scale = G[i] * max(np.abs(ox).max(), np.abs(oy).max(), 1e-3)
+        assert_grad_close(gX[i], G[i] * ox, scale)
+        assert_grad_close(gY[i], G[i] * oy, scale)
 
 
 def test_grid_gradient_sums_pairs():
-    X = rw(6, 12, 64)
-    Y = rw(7, 9, 64)
+    X = ints(6, 12, 64)
+    Y = ints(7, 9, 64)
     Xd = torch.from_numpy(X).cuda().requires_grad_(True)
     Yd = torch.from_numpy(Y).cuda().requires_grad_(True)
     G = np.random.default_rng(8).uniform(0.1, 1.0, (12, 9))
@@ -90,36 +96,46 @@ def test_grid_gradient_sums_pairs():
     wy = np.zeros_like(Y, dtype=np.float64)
     for a in range(12):
         for b in range(9):
-            x64, y64 = X[a].astype(np.float64), Y[b].astype(np.float64)
-            assert path_margin(x64, y64, 0.2) > 1e-5, (a, b)
-            _, ox, oy = oracle.dtw_grad(x64, y64, 0.2)
+            _, ox, oy = oracle.dtw_grad(X[a].astype(np.float64), Y[b].astype(np.float64), 0.2)
             wx[a] += G[a, b] * ox
             wy[b] += G[a, b] * oy
-    np.testing.assert_allclose(Xd.grad.cpu().numpy(), wx, rtol=1e-4, atol=1e-4)
-    np.testing.assert_allclose(Yd.grad.cpu().numpy(), wy, rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(Xd.grad.cpu().numpy(), wx, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(Yd.grad.cpu().numpy(), wy, rtol=1e-5, atol=1e-5)
+
+
+def same_path(x, y, r, band):
+    pi, pj = oracle.mapping(oracle.cost_matrix(x, y, band))
+    return pi, pj
 
 
 def test_gradient_central_differences_eps_1e4():
-    X = rw(9, 8, 48)
-    Y = rw(10, 8, 48)
+    t, r, eps = 40, 0.1, 1e-4
+    X = rw(9, 24, t)
+    Y = rw(10, 24, t)
     Xd = torch.from_numpy(X).cuda().requires_grad_(True)
-    paired_dtw(Xd, torch.from_numpy(Y).cuda(), 0.1).sum().backward()
+    paired_dtw(Xd, torch.from_numpy(Y).cuda(), r).sum().backward()
     g = Xd.grad.cpu().numpy()
-    eps = 1e-4
+    band = oracle.compute_r(t, r)
+    bound = 4 * (2 * t + 2) * 2.0 ** -24  # float32 error of a path cost, with margin
     done = 0
-    for i in range(8):
+    for i in range(24):
         x64, y64 = X[i].astype(np.float64), Y[i].astype(np.float64)
-        if path_margin(x64, y64, 0.1) <= 1e-3:
+        if path_margin(x64, y64, r) <= bound:
             continue
-        for q in range(0, 48, 5):
+        p0 = same_path(x64, y64, r, band)
+        for q in range(0, t, 3):
             xp, xm = x64.copy(), x64.copy()
             xp[q] += eps
             xm[q] -= eps
-            fd = (oracle.metric_distance(xp, y64, "dtw", r=0.1)
-                  - oracle.metric_distance(xm, y64, "dtw", r=0.1)) / (2 * eps)
-            assert abs(fd - g[i, q]) <= 1e-3 * max(1.0, abs(fd)), (i, q, fd, g[i, q])
-        done += 1
-    assert done >= 4
+            pp, pm = same_path(xp, y64, r, band), same_path(xm, y64, r, band)
+            if not (np.array_equal(pp[0], p0[0]) and np.array_equal(pp[1], p0[1])
+                    and np.array_equal(pm[0], p0[0]) and np.array_equal(pm[1], p0[1])):
+                continue
+            fd = (oracle.metric_distance(xp, y64, "dtw", r=r)
+                  - oracle.metric_distance(xm, y64, "dtw", r=r)) / (2 * eps)
+            assert abs(fd - g[i, q]) <= 1e-4 * max(1.0, abs(fd)), (i, q, fd, g[i, q])
+            done += 1
+    assert done >= 20
 
 
 def test_backward_is_deterministic():

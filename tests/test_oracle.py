@@ -239,6 +239,19 @@ def test_subsequence_brute_force(metric):
             assert dist[i, j] == math.sqrt(best)
 
 
+def test_subsequence_match_known_answer():
+    # tests/wildboar/distance/test_distance.py:1047-1060 (inline data): the
+    # euclidean windows of x within threshold 2 of y are [0, 1, 6] and [1, 6].
+    x = np.array([1, 2, 3, 4], float)
+    ys = [np.array([1, 2, 3, 3, 3, 3, 1, 2, 3, 4], float),
+          np.array([10, 2, 3, 3, 3, 3, 1, 2, 3, 4], float)]
+    for y, want in zip(ys, ([0, 1, 6], [1, 6])):
+        prof = [oracle.metric_distance(x, y[s:s + 4], "euclidean") for s in range(7)]
+        assert [s for s, d in enumerate(prof) if d <= 2] == want
+        dist, idx = oracle.subsequence([x], y[None, :], metric="euclidean")
+        assert idx[0, 0] == 6 and dist[0, 0] == 0.0
+
+
 def test_subsequence_euclidean_is_dtw_r0_and_pairwise_transposed():
     rng = np.random.default_rng(10)
     X = rw(rng, 5, 25)

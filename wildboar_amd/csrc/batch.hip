@@ -107,7 +107,6 @@ struct StripState {
     static constexpr int W = S + U - 1;
     float row[S];
     float yw[W];  // yw[q] = y[i + k0 - off + q] at the start of a group
-    float dg0;    // diag input of slot 0: the left input of the previous row
     float f, l;   // first / last real slot of the last computed row
     int i;        // this lane's row at the start of the group
     int lx, ly, k0, kf_local, off;
@@ -135,7 +134,6 @@ __device__ __forceinline__ void strip_init(StripState<R, U> &st, const StripItem
     st.Y = it.swap ? xs.row(it.pair) : ys.row(it.pair);
 #pragma unroll
     for (int m = 0; m < S; ++m) st.row[m] = (!unused && st.k0 + m == it.off) ? 0.0f : WBX_INF_F;
-    st.dg0 = (!unused && st.k0 - 1 == it.off) ? 0.0f : WBX_INF_F;
     st.f = st.row[0];
     st.l = st.shortl ? st.row[R - 1] : st.row[R];
 #pragma unroll
@@ -147,8 +145,6 @@ template <int R, int U>
 __device__ __forceinline__ void strip_row(StripState<R, U> &st, int u, float xi, float Lin,
                                           float Uin) {
     constexpr int S = R + 1;
-    const float dg_first = st.dg0;
-    st.dg0 = Lin;
     float left = Lin;
 #pragma unroll
     for (int m = 0; m < S; ++m) {
@@ -159,7 +155,7 @@ __device__ __forceinline__ void strip_row(StripState<R, U> &st, int u, float xi,
             up = st.shortl ? Uin : st.row[m + 1];
         else
             up = st.row[m + 1];
-        const float dg = (m == 0) ? dg_first : st.row[m];
+        const float dg = st.row[m];  // cell (i-1, j-1) has the same band slot
         const float mn = fminf(fminf(up, left), dg);
         const float d = xi - st.yw[m + u];
         const float c = __builtin_fmaf(d, d, mn);
@@ -190,7 +186,7 @@ __device__ __forceinline__ void strip_park(StripState<R, U> &st, const Series<fl
     st.off = 0;
     st.kf_local = -1;
     st.shortl = true;
-    st.f = st.l = st.dg0 = WBX_INF_F;
+    st.f = st.l = WBX_INF_F;
     st.X = xs.base;
     st.Y = xs.base;
 #pragma unroll
