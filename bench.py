@@ -1,19 +1,28 @@
-"""Benchmark: wildboar's pairwise DTW on one MI355X.
+"""Benchmark: wildboar's DTW distance path on one MI355X.
 
-Workload (one step): pairwise_distance(X, Y, metric="dtw", r=0.1) with X and Y
-2000 x 512 float32 z-normalised Gaussian random walks (seeds 10, 11), inputs
-resident in HBM, FP32 cells, float64 2000 x 2000 output -- the full-matrix
-configuration the metric is quoted on.  The whole step runs through the C ABI
-(wbx_dev_pairwise) on torch's current stream.
+Headline workload (one step): pairwise_distance(X, Y, metric="dtw", r=0.1)
+with X and Y 2000 x 512 float32 z-normalised Gaussian random walks (seeds 10,
+11), inputs resident in HBM, FP32 cells, float64 2000 x 2000 output -- the
+full-matrix configuration the metric is quoted on.  The step runs through the
+C ABI (wbx_dev_pairwise) on torch's current stream.
 
 Printed JSON line (stdout):
-  value     algorithmic band cells per second over the timed steps
-  roofline  the DTW kernel's cells/s (HIP events on its launch stream) against
-            the VALU min-plus peak of 3 instructions per cell
+  value         algorithmic band cells per second over the timed steps
+  roofline      the DTW kernel's cells/s (HIP events on its launch stream)
+                against the VALU min-plus peak of 3 instructions per cell
   cpu_baseline  the float64 C restatement of the reference, all host cores,
-            on a bounded slice of the same workload
-  plain_*   the plain one-pair-per-block kernel on a slice, and the speedup
-  max_rel_err / label_mismatch  FP32 results against the float64 oracle
+                on a bounded slice of the headline workload
+  accuracy      FP32 max relative error against the float64 oracle, 1-NN
+                label mismatches, the plain one-pair-per-block kernel on a
+                slice of the same workload
+  batched       50,000 EOS-padded pairs of uneven length (64..2048) with
+                per-pair r in [0.05, 0.2]: the persistent strip kernels in one
+                hipGraph replay against the plain per-pair kernel
+  train         differentiable DTW, 4096 x 64 pairs of length 256, r = 0.1:
+                forward, forward + backward, and 200 barycenter steps
+  subsequence   pairwise_subsequence_distance, 4096 shapelets (lengths
+                16..128) over random-walk series of length 1024, dtw r = 0.1
+                and euclidean, with exact indices
 """
 
 import argparse
@@ -56,6 +65,192 @@ def band_cells(lx, ly, r):
     return cells
 
 
+def timed(torch, fn, reps):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+# --------------------------------------------------------------------- batched
+def uneven_pairs(n, t_pad, seed_len=30, seed_data=31):
+    """EOS-padded random-walk pairs: lengths U[64, t_pad], r U[0.05, 0.2],
+    z-normalised over each series' own length, plus Laplace(0, 1) spikes with
+    probability 0.05 per timestep."""
+    from wildboar_amd.variable_len import EOS
+
+    rng = np.random.default_rng(seed_len)
+    lx = rng.integers(64, t_pad + 1, n).astype(np.int32)
+    ly = rng.integers(64, t_pad + 1, n).astype(np.int32)
+    r = rng.uniform(0.05, 0.2, n)
+    drng = np.random.default_rng(seed_data)
+    cols = np.arange(t_pad)
+
+    def side(lens):
+        out = np.empty((n, t_pad), np.float32)
+        for c0 in range(0, n, 4096):
+            L = lens[c0:c0 + 4096].astype(np.float64)[:, None]
+            m = cols[None, :] < L
+            wlk = np.cumsum(drng.standard_normal((m.shape[0], t_pad), dtype=np.float32), axis=1,
+                            dtype=np.float64)
+            mu = (wlk * m).sum(1, keepdims=True) / L
+            sd = np.sqrt((((wlk - mu) * m) ** 2).sum(1, keepdims=True) / L)
+            z = (wlk - mu) / sd
+            spike = drng.random(z.shape, dtype=np.float32) < 0.05
+            z[spike] += drng.laplace(0.0, 1.0, int(spike.sum()))
+            out[c0:c0 + 4096] = np.where(m, z, EOS).astype(np.float32)
+        return out
+
+    return side(lx), side(ly), lx, ly, r
+
+
+def run_batched(torch, dev, n_pairs, t_pad, reps):
+    import oracle
+    from wildboar_amd.batch import DtwBatch
+
+    t0 = time.perf_counter()
+    X, Y, lx, ly, r = uneven_pairs(n_pairs, t_pad)
+    log(f"batched: data ready in {time.perf_counter() - t0:.1f} s")
+    Xd = torch.from_numpy(X).to(dev)
+    Yd = torch.from_numpy(Y).to(dev)
+    side = torch.cuda.Stream()
+    res = {}
+    for plain in (False, True):
+        plan = DtwBatch(lx, ly, t_pad, r, precision="fp32", plain=plain)
+        out = torch.empty(n_pairs, dtype=torch.float64, device=dev)
+        with torch.cuda.stream(side):
+            plan.run(Xd, Yd, out=out, graph=True)
+            plan.run(Xd, Yd, out=out, graph=True)
+        def replay():
+            with torch.cuda.stream(side):
+                plan.run(Xd, Yd, out=out, graph=True)
+
+        dt = timed(torch, replay, 1 if plain else reps)
+        res[plain] = (dt, out.cpu().numpy(), plan.cells, plan.n_strip, plan.n_plain)
+        plan.close()
+    dt, got, cells, n_strip, n_plainp = res[False]
+    pdt = res[True][0]
+    pick = np.random.default_rng(0).choice(n_pairs, 200, replace=False)
+    want = np.array([oracle.metric_distance(X[p, :lx[p]].astype(np.float64),
+                                            Y[p, :ly[p]].astype(np.float64), "dtw", r=r[p])
+                     for p in pick])
+    err = float(np.max(np.abs(got[pick] - want) / want))
+    out = {
+        "workload": f"{n_pairs} pairs, lengths U[64,{t_pad}] padded to {t_pad} with EOS, "
+                    "r U[0.05,0.2] per pair, random walks + 5% Laplace spikes, FP32",
+        "cells": int(cells),
+        "ms": dt * 1e3,
+        "value": cells / dt,
+        "unit": "cells/s",
+        "frac_of_peak": cells / dt / PEAK_CELLS,
+        "plain_ms": pdt * 1e3,
+        "plain_value": cells / pdt,
+        "speedup_vs_plain": pdt / dt,
+        "strip_pairs": int(n_strip),
+        "plain_pairs": int(n_plainp),
+        "max_rel_err_200_pairs": err,
+        "graph": True,
+    }
+    log(f"batched: {dt * 1e3:.3f} ms, {cells / dt / 1e12:.2f} Tcells/s; plain {pdt * 1e3:.2f} ms; "
+        f"x{pdt / dt:.1f}; max rel err {err:.2e}")
+    return out
+
+
+# ----------------------------------------------------------------------- train
+def run_train(torch, dev, reps, dba_steps):
+    from wildboar_amd import device
+    from wildboar_amd.dtw import dtw, dtw_barycenter
+
+    X = torch.from_numpy(random_walks(40, 4096, 256).astype(np.float32)).to(dev)
+    P = torch.from_numpy(random_walks(41, 64, 256).astype(np.float32)).to(dev)
+    cells = 4096 * 64 * band_cells(256, 256, 0.1)
+    out = torch.empty((4096, 64), dtype=torch.float64, device=dev)
+    Xg = X.clone().requires_grad_(True)
+    Pg = P.clone().requires_grad_(True)
+
+    def inf():
+        device.pairwise(X, P, metric="dtw", r=0.1, precision="fp32", out=out)
+
+    def fwd():
+        return dtw(Xg, Pg, 0.1)
+
+    def fwd_bwd():
+        dtw(Xg, Pg, 0.1).sum().backward()
+
+    for _ in range(3):
+        inf()
+        fwd_bwd()
+    t_inf = timed(torch, inf, reps)
+    t_fwd = timed(torch, fwd, reps)
+    t_fb = timed(torch, fwd_bwd, reps)
+    D = torch.from_numpy(random_walks(42, 512, 256).astype(np.float32)).to(dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, hist = dtw_barycenter(D, 0.1, steps=dba_steps, lr=1e-2, return_history=True)
+    torch.cuda.synchronize()
+    t_dba = time.perf_counter() - t0
+    res = {
+        "workload": "dtw(X, P, r=0.1) autograd, X 4096x256, P 64x256 float32, grads w.r.t. both",
+        "cells": int(cells),
+        "inference_fwd_ms": t_inf * 1e3,
+        "fwd_ms": t_fwd * 1e3,
+        "fwd_bwd_ms": t_fb * 1e3,
+        "fwd_bwd_over_fwd": t_fb / t_fwd,
+        "fwd_bwd_over_inference_fwd": t_fb / t_inf,
+        "dba": {"series": "512x256 (seed 42)", "steps": dba_steps, "seconds": t_dba,
+                "loss_first": float(hist[0]), "loss_last": float(hist[-1])},
+    }
+    log(f"train: inference {t_inf * 1e3:.3f} ms, fwd {t_fwd * 1e3:.3f} ms, fwd+bwd "
+        f"{t_fb * 1e3:.3f} ms (x{t_fb / t_fwd:.2f}); DBA {dba_steps} steps {t_dba:.2f} s")
+    return res
+
+
+# ----------------------------------------------------------------- subsequence
+def run_subsequence(torch, dev, n_series):
+    import oracle
+    from wildboar_amd import device
+
+    Xall = random_walks(20, 10000, 1024)
+    rng = np.random.default_rng(21)
+    L = rng.integers(16, 129, 4096)
+    src = rng.integers(0, 10000, 4096)
+    start = np.array([rng.integers(0, 1024 - l + 1) for l in L])
+    shp64 = [Xall[s, a:a + l].copy() for s, a, l in zip(src, start, L)]
+    X = torch.from_numpy(Xall[:n_series].astype(np.float32)).to(dev)
+    shp = [torch.from_numpy(s.astype(np.float32)).to(dev) for s in shp64]
+    res = {"workload": f"4096 shapelets, L U[16,128], from 10000 random walks of length 1024; "
+                       f"scan over the first {n_series} series, float32"}
+    for metric in ("dtw", "euclidean"):
+        device.subsequence(shp[:64], X[:64], metric=metric, r=0.1, precision="fp32")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dist, idx = device.subsequence(shp, X, metric=metric, r=0.1, precision="fp32")
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        if metric == "dtw":
+            per = np.array([(1025 - l) * band_cells(int(l), int(l), 0.1) for l in L])
+            unit = "cells/s"
+        else:
+            per = np.array([(1025 - l) * int(l) for l in L])
+            unit = "MAC/s"
+        work = float(per.sum()) * n_series
+        # indices and distances of a block of tasks against the float64 oracle
+        xs = Xall[:8].astype(np.float32).astype(np.float64)
+        od, oi = oracle.subsequence([s.astype(np.float32).astype(np.float64) for s in shp64[:64]],
+                                    xs, metric=metric, r=0.1)
+        gi = idx[:8, :64].cpu().numpy()
+        gd = dist[:8, :64].cpu().numpy()
+        res[metric] = {"seconds": dt, "work": work, "value": work / dt, "unit": unit,
+                       "index_mismatch_8x64": int((gi != oi).sum()),
+                       "dist_mismatch_8x64": int((gd != od).sum())}
+        log(f"subsequence {metric}: {dt:.3f} s, {work / dt / 1e12:.2f} T{unit}; "
+            f"index mismatches {res[metric]['index_mismatch_8x64']}")
+    return res
+
+
+# ------------------------------------------------------------------------ main
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,7 +260,13 @@ def main():
     ap.add_argument("--t", type=int, default=512)
     ap.add_argument("--r", type=float, default=0.1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="headline workload only")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-train", action="store_true")
+    ap.add_argument("--no-subsequence", action="store_true")
+    ap.add_argument("--batched-pairs", type=int, default=50000)
+    ap.add_argument("--subsequence-series", type=int, default=1000)
+    ap.add_argument("--dba-steps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB three times (scales the PMC byte counters)")
@@ -178,18 +379,17 @@ def main():
         Xs = X.cpu().numpy().astype(np.float64)
         Ys = Y.cpu().numpy().astype(np.float64)
         Do = oracle.pairwise(Xs[:rows], Ys, metric="dtw", r=r)
-        rel = np.abs(Dg - Do) / np.abs(Do)
-        result["max_rel_err"] = float(rel.max())
+        acc = {"max_rel_err": float((np.abs(Dg - Do) / np.abs(Do)).max())}
         labels = np.random.default_rng(12).integers(0, 10, n)
         torch.cuda.synchronize()
         ta = time.perf_counter()
         idx, _ = device.argmin(X, Y, 1, metric="dtw", r=r, precision="fp32")
         torch.cuda.synchronize()
-        result["argmin_1nn_ms"] = (time.perf_counter() - ta) * 1e3
+        acc["argmin_1nn_ms"] = (time.perf_counter() - ta) * 1e3
         oi, _ = oracle.argmin(Xs[:rows], Ys, k=1, metric="dtw", r=r)
         got = labels[idx[:rows, 0].cpu().numpy()]
-        result["label_mismatch"] = int((got != labels[oi[:, 0]]).sum())
-        result["label_rows_checked"] = rows
+        acc["label_mismatch"] = int((got != labels[oi[:, 0]]).sum())
+        acc["label_rows_checked"] = rows
 
         # Plain one-pair-per-block kernel on a slice of the same workload.
         prow = 64
@@ -197,18 +397,22 @@ def main():
         try:
             outp = torch.empty((prow, n), dtype=torch.float64, device=dev)
             device.pairwise(X[:prow], Y, metric="dtw", r=r, precision="fp32", out=outp)
-            torch.cuda.synchronize()
-            tp = time.perf_counter()
-            device.pairwise(X[:prow], Y, metric="dtw", r=r, precision="fp32", out=outp)
-            torch.cuda.synchronize()
-            tp = time.perf_counter() - tp
+            tp = timed(torch, lambda: device.pairwise(X[:prow], Y, metric="dtw", r=r,
+                                                      precision="fp32", out=outp), 1)
         finally:
             _lib.set_policy(0)
-        plain = prow * n * cells_pair / tp
-        result["plain_cells_per_s"] = plain
-        result["speedup_vs_plain"] = value / plain
-        log(f"plain {plain / 1e12:.4f} Tcells/s; max rel err {result['max_rel_err']:.2e}; "
-            f"label mismatches {result['label_mismatch']}")
+        acc["plain_cells_per_s"] = prow * n * cells_pair / tp
+        acc["speedup_vs_plain"] = value / acc["plain_cells_per_s"]
+        result["accuracy"] = acc
+        log(f"plain {acc['plain_cells_per_s'] / 1e12:.4f} Tcells/s; max rel err "
+            f"{acc['max_rel_err']:.2e}; label mismatches {acc['label_mismatch']}")
+
+        if not args.no_batched:
+            result["batched"] = run_batched(torch, dev, args.batched_pairs, 2048, 5)
+        if not args.no_train:
+            result["train"] = run_train(torch, dev, 10, args.dba_steps)
+        if not args.no_subsequence:
+            result["subsequence"] = run_subsequence(torch, dev, args.subsequence_series)
 
     if not args.no_cpu:
         import oracle

@@ -41,7 +41,9 @@ extern "C" {
 /* Arithmetic modes. */
 #define WBX_FP64 0        /* float64, bit-identical to the reference            */
 #define WBX_FP32 1        /* float32 cells; argmin/labels certified exact      */
-#define WBX_FP32_ACC64 2  /* float32 differences, float64 accumulated cost     */
+#define WBX_FP32_ACC64 2  /* float32 data, float64 recursion: inputs rounded to
+                             float32 once, then bit-identical to the reference
+                             on those rounded values                            */
 
 /* Input element types for the device-pointer entry points. */
 #define WBX_F64 0

@@ -56,7 +56,9 @@ def test_eos_lengths_match_reference_rule():
 
 
 @pytest.mark.parametrize("seed,n,lo,hi,t_pad", [(2, 300, 20, 300, 320), (3, 120, 1, 40, 48),
-                                                 (4, 60, 200, 700, 700)])
+                                                 (4, 60, 200, 700, 700),
+                                                 # full-width series: loads at both array ends
+                                                 (17, 9, 40, 48, 48)])
 def test_batch_fp32_strip_vs_oracle(seed, n, lo, hi, t_pad):
     X, Y, lx, ly, r = make_pairs(seed, n, lo, hi, t_pad, spikes=0.05)
     want = oracle_batch(X, Y, lx, ly, r)
@@ -88,6 +90,19 @@ def test_batch_fp64_bitwise():
     plan = DtwBatch(lx, ly, 160, r, precision="fp64")
     got = plan.run(torch.from_numpy(X).cuda(), torch.from_numpy(Y).cuda(), graph=False)
     np.testing.assert_array_equal(got.cpu().numpy(), want)
+
+
+def test_batch_fp32_acc64_bitwise_on_rounded_inputs():
+    # float32 data, float64 recursion: the oracle on float32-rounded values,
+    # for float64 inputs (rounded by the library) and float32 inputs
+    X, Y, lx, ly, r = make_pairs(16, 60, 1, 200, 200, spikes=0.05)
+    X32 = X.astype(np.float32)
+    Y32 = Y.astype(np.float32)
+    want = oracle_batch(X32.astype(np.float64), Y32.astype(np.float64), lx, ly, r)
+    plan = DtwBatch(lx, ly, 200, r, precision="fp32_acc64")
+    for Xa, Ya in ((X, Y), (X32, Y32)):
+        got = plan.run(torch.from_numpy(Xa).cuda(), torch.from_numpy(Ya).cuda(), graph=False)
+        np.testing.assert_array_equal(got.cpu().numpy(), want)
 
 
 def test_batch_wdtw_fp64_bitwise_weights_by_array_width():

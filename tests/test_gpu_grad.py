@@ -103,6 +103,27 @@ def test_grid_gradient_sums_pairs():
     np.testing.assert_allclose(Yd.grad.cpu().numpy(), wy, rtol=1e-5, atol=1e-5)
 
 
+@pytest.mark.parametrize("ny", [64, 128])
+def test_grid_gradient_wave_reduced_rows(ny):
+    # ny a multiple of 64: the 64 pairs of a wave share one x row and dX is
+    # summed across the wave (one 64-column chunk, then two chunks)
+    X = ints(20, 5, 48)
+    Y = ints(21, ny, 48)
+    Xd = torch.from_numpy(X).cuda().requires_grad_(True)
+    Yd = torch.from_numpy(Y).cuda().requires_grad_(True)
+    G = np.random.default_rng(22).uniform(0.1, 1.0, (5, ny))
+    (dtw(Xd, Yd, 0.1) * torch.from_numpy(G).cuda()).sum().backward()
+    wx = np.zeros_like(X, dtype=np.float64)
+    wy = np.zeros_like(Y, dtype=np.float64)
+    for a in range(5):
+        for b in range(ny):
+            _, ox, oy = oracle.dtw_grad(X[a].astype(np.float64), Y[b].astype(np.float64), 0.1)
+            wx[a] += G[a, b] * ox
+            wy[b] += G[a, b] * oy
+    np.testing.assert_allclose(Xd.grad.cpu().numpy(), wx, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(Yd.grad.cpu().numpy(), wy, rtol=1e-5, atol=1e-5)
+
+
 def same_path(x, y, r, band):
     pi, pj = oracle.mapping(oracle.cost_matrix(x, y, band))
     return pi, pj

@@ -212,3 +212,34 @@ def test_subsequence_fp32_ties_overflow_and_wide_band():
                                     r=params.get("r", 1.0))
         np.testing.assert_array_equal(i, oi)
         np.testing.assert_array_equal(d, od)
+
+
+# ----------------------------------------------------------------- FP32_ACC64
+def f32(a):
+    return np.asarray(a, np.float64).astype(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("metric,params", [("dtw", {"r": 0.1}), ("dtw", {"r": 1.0}),
+                                           ("wdtw", {"r": 0.2, "g": 0.1}), ("euclidean", {})])
+def test_fp32_acc64_is_fp64_on_rounded_inputs(metric, params):
+    # float32 data, float64 recursion: bitwise the oracle on the rounded values
+    X = rw(40, 30, 90)
+    Y = rw(41, 25, 90)
+    r, g = params.get("r", 1.0), params.get("g", 0.05)
+    got = wd.pairwise_distance(X, Y, metric=metric, metric_params=params, precision="fp32_acc64")
+    np.testing.assert_array_equal(got, oracle.pairwise(f32(X), f32(Y), metric=metric, r=r, g=g))
+    got = wd.paired_distance(X, Y[:25].repeat(2, 0)[:30], metric=metric, metric_params=params,
+                             precision="fp32_acc64")
+    np.testing.assert_array_equal(
+        got, oracle.paired(f32(X), f32(Y[:25].repeat(2, 0)[:30]), metric=metric, r=r, g=g))
+    i, d = wd.argmin_distance(X, Y, k=3, metric=metric, metric_params=params,
+                              return_distance=True, precision="fp32_acc64")
+    oi, od = oracle.argmin(f32(X), f32(Y), k=3, metric=metric, r=r, g=g)
+    np.testing.assert_array_equal(i, oi)
+    np.testing.assert_array_equal(d, od)
+    shp = [X[3, 5:25].copy(), Y[7, 40:47].copy()]
+    d, i = wd.pairwise_subsequence_distance(shp, X, metric=metric, metric_params=params,
+                                            return_index=True, precision="fp32_acc64")
+    od, oi = oracle.subsequence([f32(s) for s in shp], f32(X), metric=metric, r=r, g=g)
+    np.testing.assert_array_equal(i, oi)
+    np.testing.assert_array_equal(d, od)

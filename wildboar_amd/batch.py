@@ -42,7 +42,8 @@ class DtwBatch:
     r : float or array-like of float, window fraction per pair.
     metric : "dtw" or "wdtw" (WDTW weights are sized by ``t_pad``, the array
         width, as the reference sizes them by the array's n_timestep).
-    precision : "fp32" or "fp64".
+    precision : "fp32", "fp64", or "fp32_acc64" (inputs rounded to float32,
+        float64 recursion).
     plain : bool, use the plain one-pair-per-block kernel for every pair.
     """
 

@@ -10,6 +10,7 @@
 //                  and distances equal the reference's.
 #include <algorithm>
 #include <atomic>
+#include <initializer_list>
 #include <math.h>
 #include <mutex>
 #include <stdlib.h>
@@ -52,8 +53,6 @@ int check_metric(const wbx_metric *m) {
         return fail(WBX_EINVAL, "unknown metric kind %d", m->kind);
     if (m->precision < WBX_FP64 || m->precision > WBX_FP32_ACC64)
         return fail(WBX_EINVAL, "unknown precision %d", m->precision);
-    if (m->precision == WBX_FP32_ACC64)
-        return fail(WBX_EUNSUPPORTED, "precision FP32_ACC64 is not available in this build");
     if (m->kind != WBX_EUCLIDEAN && !(m->r >= 0.0 && m->r <= 1.0))
         return fail(WBX_EINVAL, "r == %g, must be in [0, 1]", m->r);
     if (m->kind == WBX_WDTW && !(m->g >= 0.0))
@@ -64,6 +63,39 @@ int check_metric(const wbx_metric *m) {
 hipStream_t pick_stream(DeviceCtx *ctx, void *stream) {
     (void)ctx;
     return (hipStream_t)stream;  // NULL is the default (null) stream
+}
+
+// FP32_ACC64 (float32 data, float64 accumulation): float64 inputs are rounded
+// to float32 once, into the input arena, and every call then runs the float64
+// recursion (float32 inputs already hold float32 values).  The metric handed
+// on has precision FP64.  Call after begin(): the arena follows its ordering.
+struct InArray {
+    const void **ptr;
+    int64_t n, len;
+    int64_t *stride;
+};
+
+int acc64_prepare(DeviceCtx *ctx, wbx_metric *m, int dtype, std::initializer_list<InArray> arrays,
+                  hipStream_t st) {
+    if (m->precision != WBX_FP32_ACC64) return WBX_OK;
+    m->precision = WBX_FP64;
+    if (dtype != WBX_F64) return WBX_OK;
+    size_t total = 0;
+    for (const InArray &a : arrays)
+        if (a.ptr && *a.ptr) total += (size_t)a.n * (size_t)a.len;
+    void *wsp = nullptr;
+    int rc = ws_in_reserve(ctx, total * sizeof(double), &wsp);
+    if (rc) return rc;
+    double *dst = (double *)wsp;
+    for (const InArray &a : arrays) {
+        if (!a.ptr || !*a.ptr) continue;
+        rc = launch_round_f32_rows((const double *)*a.ptr, a.n, a.len, *a.stride, dst, a.len, st);
+        if (rc) return rc;
+        *a.ptr = dst;
+        *a.stride = a.len;
+        dst += (size_t)a.n * (size_t)a.len;
+    }
+    return WBX_OK;
 }
 
 // Orders this call after the arena's previous user (possibly on another stream).
@@ -444,6 +476,10 @@ int subseq_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
         if (hl[j] > max_len) max_len = hl[j];
         if (ho[j] + hl[j] > total) total = ho[j] + hl[j];
     }
+    wbx_metric mm = *m;
+    int64_t s_stride = total;
+    RC(acc64_prepare(ctx, &mm, dtype, {{&S, 1, total, &s_stride}, {&X, n, t, &xstr}}, st));
+    m = &mm;
     const double *w64 = nullptr;
     const float *w32 = nullptr;
     if (m->kind == WBX_WDTW) RC(cached_weights(ctx, t, m->g, &w64, &w32));
@@ -580,11 +616,17 @@ int wbx_dev_pairwise(const wbx_metric *metric, int dtype, const void *X, int64_t
     DevGuard g(ctx->device);
     hipStream_t st = pick_stream(ctx, stream);
     RC(begin(ctx, st));
-    int rc = metric->precision == WBX_FP64
-                 ? pairwise_impl<double>(ctx, metric, dtype, X, nx, x_stride, tx, Y, ny, y_stride,
-                                         ty, singleton, out, st)
-                 : pairwise_impl<float>(ctx, metric, dtype, X, nx, x_stride, tx, Y, ny, y_stride,
-                                        ty, singleton, out, st);
+    wbx_metric mm = *metric;
+    int rc = acc64_prepare(ctx, &mm, dtype,
+                           {{&X, nx, tx, &x_stride}, {singleton ? nullptr : &Y, ny, ty, &y_stride}},
+                           st);
+    if (singleton) y_stride = x_stride;
+    if (!rc)
+        rc = mm.precision == WBX_FP64
+                 ? pairwise_impl<double>(ctx, &mm, dtype, X, nx, x_stride, tx, Y, ny, y_stride, ty,
+                                         singleton, out, st)
+                 : pairwise_impl<float>(ctx, &mm, dtype, X, nx, x_stride, tx, Y, ny, y_stride, ty,
+                                        singleton, out, st);
     int rc2 = end(ctx, st);
     return rc ? rc : rc2;
 }
@@ -601,7 +643,13 @@ int wbx_dev_paired(const wbx_metric *metric, int dtype, const void *X, int64_t x
     RC(begin(ctx, st));
     // _paired_distance binds its rows to the caller's y: out[i] = D(Y[i], X[i]).
     PairSpec ps{PAIR_DIAG, n, n, nullptr, nullptr};
-    int rc;
+    wbx_metric mm = *metric;
+    int rc = acc64_prepare(ctx, &mm, dtype, {{&X, n, t, &x_stride}, {&Y, n, t, &y_stride}}, st);
+    if (rc) {
+        int rc2 = end(ctx, st);
+        return rc ? rc : rc2;
+    }
+    metric = &mm;
     const bool dbl = metric->precision == WBX_FP64;
     Carve cv;
     const size_t o_x = cv.take(stage_bytes(dtype, dbl, n, t));
@@ -655,8 +703,11 @@ int wbx_dev_argmin(const wbx_metric *metric, int dtype, const void *X, int64_t n
     DevGuard g(ctx->device);
     hipStream_t st = pick_stream(ctx, stream);
     RC(begin(ctx, st));
-    int rc = argmin_impl(ctx, metric, dtype, X, nx, x_stride, tx, Y, ny, y_stride, ty, k,
-                         lower_bound, idx, dist, st);
+    wbx_metric mm = *metric;
+    int rc = acc64_prepare(ctx, &mm, dtype, {{&X, nx, tx, &x_stride}, {&Y, ny, ty, &y_stride}}, st);
+    if (!rc)
+        rc = argmin_impl(ctx, &mm, dtype, X, nx, x_stride, tx, Y, ny, y_stride, ty, k, lower_bound,
+                         idx, dist, st);
     int rc2 = end(ctx, st);
     return rc ? rc : rc2;
 }

@@ -95,14 +95,41 @@ __device__ __forceinline__ float dpp_from_right(float v) {  // lane p <- lane p+
                                                       0x130, 0xF, 0xF, false));
 }
 
-__device__ __forceinline__ float ldy_c(const float *__restrict__ Y, int j, int ly) {
-    const int jc = j < 0 ? 0 : (j >= ly ? ly - 1 : j);
-    const float v = Y[jc];
-    return (unsigned)j < (unsigned)ly ? v : WBX_INF_F;
+// Four consecutive floats at linear offset `off` of an array (hi = array
+// length - 4): one 16-byte load when it lies inside the array, otherwise each
+// element clamped on its own.  Elements outside the array are never used
+// (rows outside a series are not computed, columns outside it become +inf).
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+
+__device__ __forceinline__ f4u ld4(const float *__restrict__ base, int off, int hi) {
+    if (off >= 0 && off <= hi) return *(const f4u *)(base + off);
+    f4u v;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int e = off + u;
+        v[u] = base[e < 0 ? 0 : (e > hi + 3 ? hi + 3 : e)];
+    }
+    return v;
 }
+
+// y[j0 .. j0 + 3] of a series of length ly at offset yo, +inf outside [0, ly).
+__device__ __forceinline__ void ld4y(const float *__restrict__ base, int yo, int hi, int j0, int ly,
+                                     float *out) {
+    const f4u v = ld4(base, yo + j0, hi);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) out[u] = (unsigned)(j0 + u) < (unsigned)ly ? v[u] : WBX_INF_F;
+}
+
+// Arrays of one launch: base, row stride and the last offset a 4-float load
+// may start at.
+struct StripArrays {
+    const float *xb, *yb;
+    int xstride, ystride, xhi, yhi;
+};
 
 template <int R, int U>
 struct StripState {
+    static_assert(U == 4, "the strip kernel loads 4 rows / columns at a time");
     static constexpr int S = R + 1;  // slots per lane (slot R is a phantom in short lanes)
     static constexpr int W = S + U - 1;
     float row[S];
@@ -111,14 +138,25 @@ struct StripState {
     int i;        // this lane's row at the start of the group
     int lx, ly, k0, kf_local, off;
     bool shortl;
-    const float *X, *Y;
+    const float *Xa, *Ya;  // arrays holding the row series and the column series
+    int xo, yo, xhi, yhi;  // their offsets in those arrays, last 4-load offsets
 };
+
+template <int R, int U>
+__device__ __forceinline__ void strip_load_window(StripState<R, U> &st) {
+    constexpr int W = StripState<R, U>::W;
+    float tmp[(W + 3) / 4 * 4];
+#pragma unroll
+    for (int q = 0; q < W; q += 4)
+        ld4y(st.Ya, st.yo, st.yhi, st.i + st.k0 - st.off + q, st.ly, tmp + q);
+#pragma unroll
+    for (int q = 0; q < W; ++q) st.yw[q] = tmp[q];
+}
 
 // Row -1 of item `it` at lane p (lag = this lane's row delay).
 template <int R, int U>
 __device__ __forceinline__ void strip_init(StripState<R, U> &st, const StripItem &it, int p,
-                                           int lag, const Series<float> &xs,
-                                           const Series<float> &ys) {
+                                           int lag, const StripArrays &ar) {
     constexpr int S = R + 1;
     const bool unused = p >= it.n;
     st.ly = it.ly;
@@ -130,15 +168,17 @@ __device__ __forceinline__ void strip_init(StripState<R, U> &st, const StripItem
     const int nslots = st.shortl ? R : S;
     const int kf = it.ly - it.lx + it.off;
     st.kf_local = (!unused && kf >= st.k0 && kf < st.k0 + nslots) ? kf - st.k0 : -1;
-    st.X = it.swap ? ys.row(it.pair) : xs.row(it.pair);
-    st.Y = it.swap ? xs.row(it.pair) : ys.row(it.pair);
+    st.Xa = it.swap ? ar.yb : ar.xb;
+    st.Ya = it.swap ? ar.xb : ar.yb;
+    st.xo = it.pair * (it.swap ? ar.ystride : ar.xstride);
+    st.yo = it.pair * (it.swap ? ar.xstride : ar.ystride);
+    st.xhi = it.swap ? ar.yhi : ar.xhi;
+    st.yhi = it.swap ? ar.xhi : ar.yhi;
 #pragma unroll
     for (int m = 0; m < S; ++m) st.row[m] = (!unused && st.k0 + m == it.off) ? 0.0f : WBX_INF_F;
     st.f = st.row[0];
     st.l = st.shortl ? st.row[R - 1] : st.row[R];
-#pragma unroll
-    for (int q = 0; q < StripState<R, U>::W; ++q)
-        st.yw[q] = ldy_c(st.Y, st.i + st.k0 - st.off + q, st.ly);
+    strip_load_window<R, U>(st);
 }
 
 template <int R, int U>
@@ -178,7 +218,7 @@ __device__ __forceinline__ void strip_capture(const StripState<R, U> &st, int pa
 }
 
 template <int R, int U>
-__device__ __forceinline__ void strip_park(StripState<R, U> &st, const Series<float> &xs) {
+__device__ __forceinline__ void strip_park(StripState<R, U> &st, const StripArrays &ar) {
     st.lx = 0;
     st.i = 1;
     st.ly = 1;
@@ -187,8 +227,10 @@ __device__ __forceinline__ void strip_park(StripState<R, U> &st, const Series<fl
     st.kf_local = -1;
     st.shortl = true;
     st.f = st.l = WBX_INF_F;
-    st.X = xs.base;
-    st.Y = xs.base;
+    st.Xa = ar.xb;
+    st.Ya = ar.xb;
+    st.xo = st.yo = 0;
+    st.xhi = st.yhi = ar.xhi;
 #pragma unroll
     for (int m = 0; m < R + 1; ++m) st.row[m] = WBX_INF_F;
 #pragma unroll
@@ -201,17 +243,19 @@ __device__ __forceinline__ void strip_park(StripState<R, U> &st, const Series<fl
 // and a lane's neighbours always hold the same stream in the other set.
 template <int R, int U>
 __global__ __launch_bounds__(64) void k_strip_dtw(const StripItem *__restrict__ items, int n_items,
-                                                  int *__restrict__ counter, Series<float> xs,
-                                                  Series<float> ys, float *__restrict__ sq) {
+                                                  int *__restrict__ counter, StripArrays ar,
+                                                  float *__restrict__ sq) {
     using St = StripState<R, U>;
     constexpr int W = St::W;
     const int p = threadIdx.x;
     St S[2];
-    strip_park<R, U>(S[0], xs);
-    strip_park<R, U>(S[1], xs);
+    strip_park<R, U>(S[0], ar);
+    strip_park<R, U>(S[1], ar);
     int item[2] = {-1, -1};
     int clock[2] = {0, 0}, end[2] = {0, 0};
     bool done[2] = {false, false};
+    bool fresh = true;
+    f4u xv[2];
     while (true) {
 #pragma unroll
         for (int z = 0; z < 2; ++z) {
@@ -231,36 +275,39 @@ __global__ __launch_bounds__(64) void k_strip_dtw(const StripItem *__restrict__ 
                     done[z] = true;
                     item[z] = -1;
                     if (sset == 0)
-                        strip_park<R, U>(S[0], xs);
+                        strip_park<R, U>(S[0], ar);
                     else
-                        strip_park<R, U>(S[1], xs);
+                        strip_park<R, U>(S[1], ar);
                 } else {
                     const StripItem it = items[nxt];
                     item[z] = nxt;
                     const int lag = (p + z) >> 1;
                     if (sset == 0)
-                        strip_init<R, U>(S[0], it, p, lag, xs, ys);
+                        strip_init<R, U>(S[0], it, p, lag, ar);
                     else
-                        strip_init<R, U>(S[1], it, p, lag, xs, ys);
+                        strip_init<R, U>(S[1], it, p, lag, ar);
                     clock[z] = 0;
+                    fresh = true;
                     // lane 63 finishes row lx-1 at clock lx + 30 + z
                     end[z] = ((it.lx + 32 + U - 1) / U) * U;
                 }
             }
         }
         if (done[0] && done[1]) break;
-
-        float xv[2][U], yn[2][U];
+        if (fresh) {  // a stream (re)started: its prefetched x values are stale
 #pragma unroll
-        for (int s = 0; s < 2; ++s)
+            for (int s = 0; s < 2; ++s) xv[s] = ld4(S[s].Xa, S[s].xo + S[s].i, S[s].xhi);
+            fresh = false;
+        }
+        // x and the new y window entries of the next group (used one group later)
+        f4u xn[2];
+        float yn[2][U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int i = S[s].i + u;
-                const int hi = S[s].lx > 0 ? S[s].lx - 1 : 0;
-                const int ic = i < 0 ? 0 : (i > hi ? hi : i);
-                xv[s][u] = S[s].X[ic];
-                yn[s][u] = ldy_c(S[s].Y, S[s].i + U + S[s].k0 - S[s].off + (W - U) + u, S[s].ly);
-            }
+        for (int s = 0; s < 2; ++s) {
+            xn[s] = ld4(S[s].Xa, S[s].xo + S[s].i + U, S[s].xhi);
+            ld4y(S[s].Ya, S[s].yo, S[s].yhi, S[s].i + U + S[s].k0 - S[s].off + (W - U), S[s].ly,
+                 yn[s]);
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
 #pragma unroll
@@ -268,7 +315,7 @@ __global__ __launch_bounds__(64) void k_strip_dtw(const StripItem *__restrict__ 
                 const float Lin = dpp_from_left(S[1 - s].l);
                 const float Uin = dpp_from_right(S[1 - s].f);
                 const int i = S[s].i + u;
-                if (i >= 0 && i < S[s].lx) strip_row<R, U>(S[s], u, xv[s][u], Lin, Uin);
+                if ((unsigned)i < (unsigned)S[s].lx) strip_row<R, U>(S[s], u, xv[s][u], Lin, Uin);
             }
         }
 #pragma unroll
@@ -277,6 +324,7 @@ __global__ __launch_bounds__(64) void k_strip_dtw(const StripItem *__restrict__ 
             for (int q = 0; q < W - U; ++q) S[s].yw[q] = S[s].yw[q + U];
 #pragma unroll
             for (int u = 0; u < U; ++u) S[s].yw[W - U + u] = yn[s][u];
+            xv[s] = xn[s];
             S[s].i += U;
         }
         clock[0] += U;
@@ -285,10 +333,10 @@ __global__ __launch_bounds__(64) void k_strip_dtw(const StripItem *__restrict__ 
 }
 
 template <int R, int U>
-static void launch_strip(const StripItem *items, int n_items, int *counter, Series<float> xs,
-                         Series<float> ys, float *sq, int waves, hipStream_t st) {
+static void launch_strip(const StripItem *items, int n_items, int *counter, StripArrays ar,
+                         float *sq, int waves, hipStream_t st) {
     hipLaunchKernelGGL((k_strip_dtw<R, U>), dim3((unsigned)waves), dim3(64), 0, st, items, n_items,
-                       counter, xs, ys, sq);
+                       counter, ar, sq);
 }
 
 // Strip sizes compiled in.  A pair of band B uses the smallest R with
@@ -297,23 +345,22 @@ static const int kStripR[] = {2, 3, 4, 6, 8, 10, 12, 14, 16, 20, 24, 28, 32, 40}
 constexpr int kNumStripR = sizeof(kStripR) / sizeof(kStripR[0]);
 
 static int launch_strip_r(int R, const StripItem *items, int n_items, int *counter,
-                          Series<float> xs, Series<float> ys, float *sq, int waves,
-                          hipStream_t st) {
+                          StripArrays ar, float *sq, int waves, hipStream_t st) {
     switch (R) {
-        case 2: launch_strip<2, 8>(items, n_items, counter, xs, ys, sq, waves, st); break;
-        case 3: launch_strip<3, 8>(items, n_items, counter, xs, ys, sq, waves, st); break;
-        case 4: launch_strip<4, 8>(items, n_items, counter, xs, ys, sq, waves, st); break;
-        case 6: launch_strip<6, 8>(items, n_items, counter, xs, ys, sq, waves, st); break;
-        case 8: launch_strip<8, 8>(items, n_items, counter, xs, ys, sq, waves, st); break;
-        case 10: launch_strip<10, 8>(items, n_items, counter, xs, ys, sq, waves, st); break;
-        case 12: launch_strip<12, 8>(items, n_items, counter, xs, ys, sq, waves, st); break;
-        case 14: launch_strip<14, 8>(items, n_items, counter, xs, ys, sq, waves, st); break;
-        case 16: launch_strip<16, 8>(items, n_items, counter, xs, ys, sq, waves, st); break;
-        case 20: launch_strip<20, 8>(items, n_items, counter, xs, ys, sq, waves, st); break;
-        case 24: launch_strip<24, 8>(items, n_items, counter, xs, ys, sq, waves, st); break;
-        case 28: launch_strip<28, 4>(items, n_items, counter, xs, ys, sq, waves, st); break;
-        case 32: launch_strip<32, 4>(items, n_items, counter, xs, ys, sq, waves, st); break;
-        case 40: launch_strip<40, 4>(items, n_items, counter, xs, ys, sq, waves, st); break;
+        case 2: launch_strip<2, 4>(items, n_items, counter, ar, sq, waves, st); break;
+        case 3: launch_strip<3, 4>(items, n_items, counter, ar, sq, waves, st); break;
+        case 4: launch_strip<4, 4>(items, n_items, counter, ar, sq, waves, st); break;
+        case 6: launch_strip<6, 4>(items, n_items, counter, ar, sq, waves, st); break;
+        case 8: launch_strip<8, 4>(items, n_items, counter, ar, sq, waves, st); break;
+        case 10: launch_strip<10, 4>(items, n_items, counter, ar, sq, waves, st); break;
+        case 12: launch_strip<12, 4>(items, n_items, counter, ar, sq, waves, st); break;
+        case 14: launch_strip<14, 4>(items, n_items, counter, ar, sq, waves, st); break;
+        case 16: launch_strip<16, 4>(items, n_items, counter, ar, sq, waves, st); break;
+        case 20: launch_strip<20, 4>(items, n_items, counter, ar, sq, waves, st); break;
+        case 24: launch_strip<24, 4>(items, n_items, counter, ar, sq, waves, st); break;
+        case 28: launch_strip<28, 4>(items, n_items, counter, ar, sq, waves, st); break;
+        case 32: launch_strip<32, 4>(items, n_items, counter, ar, sq, waves, st); break;
+        case 40: launch_strip<40, 4>(items, n_items, counter, ar, sq, waves, st); break;
         default: return fail(WBX_EUNSUPPORTED, "strip size %d not compiled", R);
     }
     WBX_HIP_CHECK(hipGetLastError());
@@ -347,7 +394,13 @@ struct BatchPlan {
     wbx_metric metric{};
     int64_t n_pairs = 0, t_pad = 0;
     int policy = 0;
-    std::vector<int> bucket_R, bucket_begin, bucket_count, bucket_waves;
+    std::vector<int> bucket_R, bucket_begin, bucket_count, bucket_waves, bucket_stream;
+    // Buckets run concurrently on up to kAux streams forked from the caller's
+    // stream (default priority) and joined back with events.
+    static constexpr int kAux = 4;
+    int n_aux = 0;
+    hipStream_t aux[kAux] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[kAux] = {};
     int n_strip = 0;
     StripItem *d_items = nullptr;
     int *d_counters = nullptr;
@@ -387,6 +440,11 @@ static void plan_free(BatchPlan *b) {
     if (!b) return;
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (int a = 0; a < b->n_aux; ++a) {
+        if (b->aux[a]) (void)hipStreamDestroy(b->aux[a]);
+        if (b->ev_join[a]) (void)hipEventDestroy(b->ev_join[a]);
+    }
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     void *ptrs[] = {b->d_items, b->d_counters, b->d_plain_list, b->d_lx, b->d_ly,
                     b->d_r,     b->d_sq,       b->d_sq_plain,   b->d_xconv, b->d_yconv};
     for (void *q : ptrs)
@@ -396,12 +454,14 @@ static void plan_free(BatchPlan *b) {
 
 static int plan_build(BatchPlan *b, const int32_t *lx, const int32_t *ly, const double *r) {
     const int64_t n = b->n_pairs;
-    const bool fp32 = b->metric.precision != WBX_FP64;
+    const bool fp32 = b->metric.precision == WBX_FP32;
     const bool dtw = b->metric.kind == WBX_DTW;
     std::vector<StripItem> items;
     std::vector<int> itemR;
     std::vector<int64_t> cost;
     std::vector<int32_t> plain;
+    // the strip kernel addresses a dense copy of the arrays with 32-bit offsets
+    const bool dense_ok = n * b->t_pad >= 4 && n * b->t_pad <= (int64_t)0x7fffffff;
     for (int64_t q = 0; q < n; ++q) {
         const int a = lx[q], c = ly[q];
         const double rq = r ? r[q] : b->metric.r;
@@ -412,7 +472,7 @@ static int plan_build(BatchPlan *b, const int32_t *lx, const int32_t *ly, const 
         const int rx = swap ? c : a, ry = swap ? a : c;
         const Band bd = make_band(rx, ry, rq);
         int R = 0, nl = 0, al = 0;
-        const bool strip = fp32 && dtw && b->policy == 0 && rx >= 1 && ry >= 1 &&
+        const bool strip = fp32 && dtw && b->policy == 0 && rx >= 1 && ry >= 1 && dense_ok &&
                            strip_split(bd.B, 2, &R, &nl, &al);
         if (!strip) {
             plain.push_back((int32_t)q);
@@ -450,10 +510,28 @@ static int plan_build(BatchPlan *b, const int32_t *lx, const int32_t *ly, const 
         b->bucket_count.back()++;
         sorted.push_back(items[order[q]]);
     }
+    // Persistent waves per bucket (two pairs per wave in flight), and the
+    // buckets spread over the auxiliary streams longest-total-cost first.
+    std::vector<int64_t> bcost(b->bucket_R.size(), 0);
     for (size_t k = 0; k < b->bucket_R.size(); ++k) {
         const int waves = (b->bucket_count[k] + 1) / 2;
-        const int cap = b->n_cu * 8;
+        const int cap = b->n_cu * 4;
         b->bucket_waves.push_back(waves < cap ? waves : cap);
+        for (int q = 0; q < b->bucket_count[k]; ++q)
+            bcost[k] += cost[order[b->bucket_begin[k] + q]];
+    }
+    b->n_aux = (int)std::min<size_t>(BatchPlan::kAux, b->bucket_R.size());
+    b->bucket_stream.assign(b->bucket_R.size(), 0);
+    {
+        std::vector<int> by(b->bucket_R.size());
+        std::iota(by.begin(), by.end(), 0);
+        std::sort(by.begin(), by.end(), [&](int u, int v) { return bcost[u] > bcost[v]; });
+        std::vector<int64_t> load((size_t)std::max(b->n_aux, 1), 0);
+        for (int k : by) {
+            const int a = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            b->bucket_stream[k] = a;
+            load[a] += bcost[k];
+        }
     }
     b->n_strip = (int)sorted.size();
     b->n_plain = (int64_t)plain.size();
@@ -485,12 +563,19 @@ static int plan_enqueue(BatchPlan *b, const T *X, int64_t xs, const T *Y, int64_
                         double *out, hipStream_t st, const double *w64, const float *w32) {
     const int64_t n = b->n_pairs, t = b->t_pad;
     PairSpec pl{PAIR_LIST, b->n_plain, 0, b->d_plain_list, b->d_plain_list};
-    if (b->metric.precision == WBX_FP64) {
+    if (b->metric.precision != WBX_FP32) {
+        // FP64, or FP32_ACC64: float64 recursion on inputs rounded to float32
         const double *X64 = (const double *)X, *Y64 = (const double *)Y;
         int64_t xs64 = xs, ys64 = ys;
         if (!std::is_same<T, double>::value) {
             RC((launch_convert_rows<T, double>(X, n, t, xs, (double *)b->d_xconv, t, st)));
             RC((launch_convert_rows<T, double>(Y, n, t, ys, (double *)b->d_yconv, t, st)));
+            X64 = (const double *)b->d_xconv;
+            Y64 = (const double *)b->d_yconv;
+            xs64 = ys64 = t;
+        } else if (b->metric.precision == WBX_FP32_ACC64) {
+            RC(launch_round_f32_rows((const double *)X, n, t, xs, (double *)b->d_xconv, t, st));
+            RC(launch_round_f32_rows((const double *)Y, n, t, ys, (double *)b->d_yconv, t, st));
             X64 = (const double *)b->d_xconv;
             Y64 = (const double *)b->d_yconv;
             xs64 = ys64 = t;
@@ -513,17 +598,41 @@ static int plan_enqueue(BatchPlan *b, const T *X, int64_t xs, const T *Y, int64_
         const int nb = (int)b->bucket_R.size();
         hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((nb + 63) / 64)), dim3(64), 0, st,
                            b->d_counters, nb);
+        WBX_HIP_CHECK(hipEventRecord(b->ev_fork, st));
+        for (int a = 0; a < b->n_aux; ++a) WBX_HIP_CHECK(hipStreamWaitEvent(b->aux[a], b->ev_fork, 0));
+        // 32-bit element offsets; dense copies when the caller's strides need more
+        const float *xb = X32, *yb = Y32;
+        int64_t xst = xs32, yst = ys32;
+        const int64_t lim = (int64_t)0x7fffffff;
+        if ((n - 1) * xst + t > lim) {
+            RC((launch_convert_rows<float, float>(X32, n, t, xst, (float *)b->d_xconv, t, st)));
+            xb = (const float *)b->d_xconv;
+            xst = t;
+        }
+        if ((n - 1) * yst + t > lim) {
+            RC((launch_convert_rows<float, float>(Y32, n, t, yst, (float *)b->d_yconv, t, st)));
+            yb = (const float *)b->d_yconv;
+            yst = t;
+        }
+        StripArrays ar{xb, yb, (int)xst, (int)yst, (int)((n - 1) * xst + t - 4),
+                       (int)((n - 1) * yst + t - 4)};
         for (int k = 0; k < nb; ++k)
             RC(launch_strip_r(b->bucket_R[k], b->d_items + b->bucket_begin[k], b->bucket_count[k],
-                              b->d_counters + k, sx, sy, b->d_sq, b->bucket_waves[k], st));
+                              b->d_counters + k, ar, b->d_sq, b->bucket_waves[k],
+                              b->aux[b->bucket_stream[k]]));
+    }
+    // the plain kernel overlaps the strip buckets on the caller's stream
+    float *sqp = (float *)b->d_sq_plain;
+    if (b->n_plain > 0) RC(launch_plain_dtw<float>(pl, sx, sy, b->metric.r, w32, sqp, st, b->d_r));
+    if (!b->bucket_R.empty()) {
+        for (int a = 0; a < b->n_aux; ++a) {
+            WBX_HIP_CHECK(hipEventRecord(b->ev_join[a], b->aux[a]));
+            WBX_HIP_CHECK(hipStreamWaitEvent(st, b->ev_join[a], 0));
+        }
         PairSpec all{PAIR_DIAG, n, n, nullptr, nullptr};
         RC(launch_finish<float>(all, b->d_sq, out, 0, st));
     }
-    if (b->n_plain > 0) {
-        float *sqp = (float *)b->d_sq_plain;
-        RC(launch_plain_dtw<float>(pl, sx, sy, b->metric.r, w32, sqp, st, b->d_r));
-        RC(launch_finish<float>(pl, sqp, out, 0, st));
-    }
+    if (b->n_plain > 0) RC(launch_finish<float>(pl, sqp, out, 0, st));
     return WBX_OK;
 }
 
@@ -550,8 +659,8 @@ int wbx_batch_create(const wbx_metric *metric, int64_t n_pairs, int64_t t_pad,
     if (!metric || !out_plan || !len_x || !len_y) return fail(WBX_EINVAL, "NULL argument");
     if (metric->kind != WBX_DTW && metric->kind != WBX_WDTW)
         return fail(WBX_EINVAL, "batched DTW needs metric dtw or wdtw");
-    if (metric->precision == WBX_FP32_ACC64)
-        return fail(WBX_EUNSUPPORTED, "precision FP32_ACC64 is not available in this build");
+    if (metric->precision < WBX_FP64 || metric->precision > WBX_FP32_ACC64)
+        return fail(WBX_EINVAL, "unknown precision %d", metric->precision);
     if (n_pairs < 1 || t_pad < 1 || t_pad > (1 << 20)) return fail(WBX_EINVAL, "bad sizes");
     for (int64_t q = 0; q < n_pairs; ++q) {
         if (len_x[q] < 1 || len_x[q] > t_pad || len_y[q] < 1 || len_y[q] > t_pad)
@@ -572,6 +681,13 @@ int wbx_batch_create(const wbx_metric *metric, int64_t n_pairs, int64_t t_pad,
     b->policy = policy;
     b->n_cu = ctx->n_cu > 0 ? ctx->n_cu : 256;
     int rc = plan_build(b, len_x, len_y, r_per_pair);
+    if (!rc && b->n_aux > 0) {
+        bool ok = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess;
+        for (int a = 0; ok && a < b->n_aux; ++a)
+            ok = hipStreamCreateWithFlags(&b->aux[a], hipStreamNonBlocking) == hipSuccess &&
+                 hipEventCreateWithFlags(&b->ev_join[a], hipEventDisableTiming) == hipSuccess;
+        if (!ok) rc = fail(WBX_EDEVICE, "batch: stream or event creation failed");
+    }
     if (!rc) {
         const size_t conv = (size_t)n_pairs * (size_t)t_pad * 8;
         if (hipMalloc(&b->d_xconv, conv) != hipSuccess || hipMalloc(&b->d_yconv, conv) != hipSuccess)

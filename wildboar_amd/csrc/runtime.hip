@@ -60,27 +60,33 @@ DeviceCtx *ctx_current() {
     return g_ctx[dev];
 }
 
-// Grow-only arena.  Callers serialise on the device lock (api layer) and make
+// Grow-only arenas.  Callers serialise on the device lock (api layer) and make
 // their stream wait for the previous user of the arena before touching it.
-int ws_reserve(DeviceCtx *ctx, size_t bytes, void **out) {
+static int arena_reserve(Workspace &a, size_t bytes, void **out) {
     bytes = (bytes + 255) & ~(size_t)255;
     if (bytes == 0) bytes = 256;
-    if (ctx->ws.size < bytes) {
-        if (ctx->ws.ptr) {
+    if (a.size < bytes) {
+        if (a.ptr) {
             if (hipDeviceSynchronize() != hipSuccess)
                 return fail(WBX_EDEVICE, "hipDeviceSynchronize failed");
-            (void)hipFree(ctx->ws.ptr);
-            ctx->ws.ptr = nullptr;
-            ctx->ws.size = 0;
+            (void)hipFree(a.ptr);
+            a.ptr = nullptr;
+            a.size = 0;
         }
         void *p = nullptr;
         if (hipMalloc(&p, bytes) != hipSuccess)
             return fail(WBX_ENOMEM, "device allocation of %zu bytes failed", bytes);
-        ctx->ws.ptr = p;
-        ctx->ws.size = bytes;
+        a.ptr = p;
+        a.size = bytes;
     }
-    *out = ctx->ws.ptr;
+    *out = a.ptr;
     return WBX_OK;
+}
+
+int ws_reserve(DeviceCtx *ctx, size_t bytes, void **out) { return arena_reserve(ctx->ws, bytes, out); }
+
+int ws_in_reserve(DeviceCtx *ctx, size_t bytes, void **out) {
+    return arena_reserve(ctx->ws_in, bytes, out);
 }
 
 // WDTW weights 1 / (1 + exp(-g (i - n/2))) with the host libm exp, as

@@ -1,5 +1,5 @@
 // train.hip -- differentiable DTW: forward that records backtrack decisions,
-// per-pair backtrack, and gradient reductions.
+// row-lockstep backtrack, and gradient reductions.
 //
 // The gradient of D = sqrt(C) along the optimal path P (dtw_mapping's
 // backtrack, distance/dtw.py:395-423: from (n-1, m-1) step to
@@ -12,12 +12,18 @@
 //
 //  k_lane_train      the lane-per-pair kernel of lane.hip plus the two bit
 //                    planes, one 32-bit word per 32 band slots per row, stored
-//                    [pair][row][plane word] so a backtrack reads whole lines.
-//  k_lane_backtrack  one pair per lane: walks the bits from the final cell,
-//                    accumulates (x_i - y_j) * g / D per row and per column
-//                    and writes the per-pair gradients (every row and column
-//                    is on a monotone path, so every entry is written).
-//  k_sum_rows/cols   fixed-order sums of the per-pair gradients into dX, dY.
+//                    [wave][row][plane word][lane] so every store and load of
+//                    a wave is one contiguous 256-byte line.
+//  k_lane_backtrack  one pair per lane, all lanes of a wave on the same row,
+//                    rows from the last to the first.  Within row i the path
+//                    enters at slot k and moves left until the first cell
+//                    whose decision is diag or up; that exit is a bit scan of
+//                    (diag | up) over the row's words, which sit at addresses
+//                    independent of the path and are fetched one row ahead.
+//                    The cells of the segment add (x_i - y_j) G / D to the
+//                    row's and the column's sums.
+//  k_sum_*           fixed-order sums of per-pair gradients into dX, dY
+//                    (the backward is bitwise deterministic).
 #include "lane_common.h"
 
 namespace wbx {
@@ -62,18 +68,22 @@ __device__ __forceinline__ void train_row(float (&row)[BP], const float *yw, flo
     for (int q = 1; q < 8; ++q) row[BP - 8 + q] = fmaxf(row[BP - 8 + q], pen[q]);
 }
 
-// Planes of pair p, row i: dirs[(p * lx + i) * 2 NW + {0..NW-1: diag, NW..: up}],
-// so the backtrack of one pair walks consecutive memory.
+// Plane words of wave v, row i, word w (0..NW-1 diag, NW..2NW-1 up), lane l:
+// dirs[((v * lx + i) * 2 NW + w) * 64 + l].
+__device__ __forceinline__ int64_t plane_index(int64_t wave, int lx, int i, int w, int nw2,
+                                               int lane) {
+    return (((int64_t)wave * lx + i) * nw2 + w) * 64 + lane;
+}
+
 template <int BP>
-__device__ __forceinline__ void store_planes(uint32_t *__restrict__ dirs, int64_t p, int lx, int i,
-                                             uint32_t (&pd)[(BP + 31) / 32],
+__device__ __forceinline__ void store_planes(uint32_t *__restrict__ dirs, int64_t wave, int lane,
+                                             int lx, int i, uint32_t (&pd)[(BP + 31) / 32],
                                              uint32_t (&pu)[(BP + 31) / 32]) {
     constexpr int NW = (BP + 31) / 32;
-    uint32_t *base = dirs + (p * lx + i) * (2 * NW);
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
-        base[w] = pd[w];
-        base[NW + w] = pu[w];
+        dirs[plane_index(wave, lx, i, w, 2 * NW, lane)] = pd[w];
+        dirs[plane_index(wave, lx, i, NW + w, 2 * NW, lane)] = pu[w];
         pd[w] = 0u;
         pu[w] = 0u;
     }
@@ -111,33 +121,90 @@ __global__ __launch_bounds__(64) void k_lane_train(PairSpec ps, Series<float> xs
     const int rem = lx % U;
     for (int i = 0; i < rem; ++i) {
         train_row<BP>(row, yw, X[i], pen, pd, pu);
-        store_planes<BP>(dirs, p, lx, i, pd, pu);
+        store_planes<BP>(dirs, wave, lane, lx, i, pd, pu);
 #pragma unroll
         for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
         yw[W - 1] = ldy(Y, i + 1 - off + W - 1, ly);
     }
+    float xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) xv[u] = X[rem + u < lx ? rem + u : lx - 1];
     for (int i0 = rem; i0 < lx; i0 += U) {
-        float xv[U], yn[U];
+        float xn[U], yn[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            xv[u] = X[i0 + u];
+            const int inext = i0 + U + u;
+            xn[u] = X[inext < lx ? inext : lx - 1];
             yn[u] = ldy(Y, i0 + U - off + W - U + u, ly);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             train_row<BP>(row, yw + u, xv[u], pen, pd, pu);
-            store_planes<BP>(dirs, p, lx, i0 + u, pd, pu);
+            store_planes<BP>(dirs, wave, lane, lx, i0 + u, pd, pu);
         }
 #pragma unroll
         for (int m = 0; m < W - U; ++m) yw[m] = yw[m + U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) yw[W - U + u] = yn[u];
+        for (int u = 0; u < U; ++u) {
+            yw[W - U + u] = yn[u];
+            xv[u] = xn[u];
+        }
     }
     float res = WBX_INF_F;
 #pragma unroll
     for (int k = 0; k < BP; ++k)
         if (k == g.kf) res = row[k];
     if (p0 < ps.n_pairs) sq[p0] = res;
+}
+
+// Sum over the 64 lanes in a fixed order: a running sum within each row of 16
+// lanes (DPP row_shr 1, 2, 4, 8), then rows folded in with row_bcast 15 and
+// 31, which leaves the total in lane 63.  Every lane returns that total.
+#define WBX_DPP(v, ctrl, rmask) \
+    __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, rmask, 0xF, false))
+__device__ __forceinline__ float wave_sum(float v) {
+    v += WBX_DPP(v, 0x111, 0xF);  // row_shr:1
+    v += WBX_DPP(v, 0x112, 0xF);  // row_shr:2
+    v += WBX_DPP(v, 0x114, 0xF);  // row_shr:4
+    v += WBX_DPP(v, 0x118, 0xF);  // row_shr:8
+    v += WBX_DPP(v, 0x142, 0xA);  // row_bcast:15 into rows 1, 3
+    v += WBX_DPP(v, 0x143, 0xC);  // row_bcast:31 into rows 2, 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+#undef WBX_DPP
+
+// Where the per-pair gradients go.
+struct GradOut {
+    int mode;          // 0: GRID pairs, 1: DIAG (paired) pairs
+    int xred;          // GRID with ny % 64 == 0: a wave's lanes share one x row
+    int64_t nx, ny, nbc;  // nbc = ny / 64 when xred
+    float *dX, *dY;    // DIAG: written directly; GRID xred with nbc == 1: dX direct
+    int64_t ldx, ldy;
+    float *xpart;      // GRID xred, nbc > 1: [a][bchunk][i]
+    float *gxp, *gyp;  // GRID: per-pair rows, [wave][i or j][lane]
+    // X and Y transposed (XT[i * nx + a], YT[j * ny + b]): the lanes of a wave
+    // hold neighbouring pairs, so their loads at one row or column coalesce
+    const float *XT, *YT;
+};
+
+// dst[c * n + r] = src[r * stride + c] through a 32 x 32 LDS tile.
+__global__ __launch_bounds__(256) void k_transpose(const float *__restrict__ src, int64_t n,
+                                                   int len, int64_t stride,
+                                                   float *__restrict__ dst) {
+    __shared__ float tile[32][33];
+    const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // ty in [0, 8)
+#pragma unroll
+    for (int q = 0; q < 32; q += 8) {
+        const int64_t r = r0 + ty + q, c = c0 + tx;
+        if (r < n && c < len) tile[ty + q][tx] = src[r * stride + c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 32; q += 8) {
+        const int64_t c = c0 + ty + q, r = r0 + tx;
+        if (r < n && c < len) dst[c * n + r] = tile[tx][ty + q];
+    }
 }
 
 // Bit of slot k in its plane word: slots are shifted in from the low end, so
@@ -148,73 +215,129 @@ __global__ __launch_bounds__(64) void k_lane_backtrack(PairSpec ps, Series<float
                                                        const float *__restrict__ sq,
                                                        const double *__restrict__ G,
                                                        const uint32_t *__restrict__ dirs,
-                                                       float *__restrict__ gxp,
-                                                       float *__restrict__ gyp) {
+                                                       GradOut go) {
     constexpr int NW = (BP + 31) / 32;
     constexpr int LASTC = BP - 32 * (NW - 1);
     const int lane = threadIdx.x;
     const int64_t wave = blockIdx.x;
-    const int64_t p = wave * 64 + lane;
-    if (p >= ps.n_pairs) return;
+    const int64_t p0 = wave * 64 + lane;
+    const bool active = p0 < ps.n_pairs;
+    const int64_t p = active ? p0 : ps.n_pairs - 1;
     int64_t a, b;
     decode_pair(ps, p, a, b);
-    const float *__restrict__ X = xs.row(a);
-    const float *__restrict__ Y = ys.row(b);
-    const int lx = g.lx, ly = g.ly;
+    const float *__restrict__ XT = go.XT + a;  // x_i = XT[i * nx]
+    const float *__restrict__ YT = go.YT + b;  // y_j = YT[j * ny]
+    const int64_t nxs = go.nx, nys = go.ny;
+    (void)xs;
+    (void)ys;
+    const int lx = g.lx, ly = g.ly, off = g.off, B = g.B;
     const double D = sqrt((double)sq[p]);
-    const float coef = D > 0.0 ? (float)(G[p] / D) : 0.0f;
-    float *gx = gxp + p * lx;
-    float *gy = gyp + p * ly;
-    int i = lx - 1, j = ly - 1, k = g.kf;
-    int ci = i, cj = j;
-    float sx = 0.0f, sy = 0.0f;
-    const uint32_t *dbase = dirs + p * (int64_t)lx * (2 * NW);
-    while (true) {
-        const float t = coef * (X[i] - Y[j]);
-        if (i != ci) {
-            gx[ci] = sx;
-            ci = i;
-            sx = 0.0f;
+    const float coef = (active && D > 0.0) ? (float)(G[p] / D) : 0.0f;
+
+    uint32_t wd[NW], wu[NW], nd[NW], nu[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        wd[w] = dirs[plane_index(wave, lx, lx - 1, w, 2 * NW, lane)];
+        wu[w] = dirs[plane_index(wave, lx, lx - 1, NW + w, 2 * NW, lane)];
+    }
+    int k = g.kf;
+    int cj = ly - 1;
+    float sy = 0.0f;
+    float xi = XT[(int64_t)(lx - 1) * nxs];
+    for (int i = lx - 1; i >= 0; --i) {
+        const int klo = off - i > 0 ? off - i : 0;
+        int khi = ly - 1 - i + off;
+        if (khi > B - 1) khi = B - 1;
+        if (k > khi) k = khi;
+        // y of the first two cells of the segment, fetched before the scan
+        float yq[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int j = i + k - off - q;
+            yq[q] = YT[(int64_t)(j < 0 ? 0 : j) * nys];
         }
-        if (j != cj) {
-            gy[cj] = sy;
-            cj = j;
-            sy = 0.0f;
+        // the next row's words and x (addresses do not depend on the path)
+        const int ip = i > 0 ? i - 1 : 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            nd[w] = dirs[plane_index(wave, lx, ip, w, 2 * NW, lane)];
+            nu[w] = dirs[plane_index(wave, lx, ip, NW + w, 2 * NW, lane)];
         }
-        sx += t;
-        sy -= t;
-        if (i == 0 && j == 0) break;
-        if (i == 0) {  // only left is finite (dtw.py:404-406)
-            --j;
-            --k;
-            continue;
+        const float xn = XT[(int64_t)ip * nxs];
+        // exit: the highest slot <= k whose decision is diag or up
+        const int wk = k >> 5;
+        int kx = -1;
+        bool isd = false;
+#pragma unroll
+        for (int w = NW - 1; w >= 0; --w) {
+            const int cnt = (w == NW - 1) ? LASTC : 32;
+            uint32_t msk = wd[w] | wu[w];
+            if (w > wk) msk = 0u;
+            if (w == wk) msk &= ~0u << (cnt - 1 - (k - 32 * w));
+            if (kx < 0 && msk != 0u) {
+                const int q = __builtin_ctz(msk);
+                kx = 32 * w + cnt - 1 - q;
+                isd = ((wd[w] >> q) & 1u) != 0u;
+            }
         }
-        if (j == 0) {  // only up is finite
-            --i;
-            ++k;
-            continue;
+        // Row 0 always ends at column 0 (only left moves there, dtw.py:404-406);
+        // kx < klo is unreachable on a finite path and keeps the walk in bounds.
+        if (i == 0 || kx < klo) {
+            kx = klo;
+            isd = false;
         }
-        const int w = k >> 5;
-        const int cnt = (w == NW - 1) ? LASTC : 32;
-        const int bit = cnt - 1 - (k - 32 * w);
-        const uint32_t wd = dbase[(int64_t)i * (2 * NW) + w];
-        const uint32_t wu = dbase[(int64_t)i * (2 * NW) + NW + w];
-        if ((wd >> bit) & 1u) {  // diag
-            --i;
-            --j;
-        } else if ((wu >> bit) & 1u) {  // up: (i-1, j) is slot k+1 of row i-1
-            --i;
-            ++k;
-        } else {  // left: (i, j-1) is slot k-1
-            --j;
-            --k;
+        float sx = 0.0f;
+        for (int s = k; s >= kx; --s) {
+            const int j = i + s - off;
+            const int q = k - s;
+            const float yv = q == 0 ? yq[0] : q == 1 ? yq[1] : YT[(int64_t)j * nys];
+            const float t = coef * (xi - yv);
+            sx += t;
+            if (j != cj) {
+                if (go.mode == 1) {
+                    if (active) go.dY[p * go.ldy + cj] = sy;
+                } else {
+                    go.gyp[(wave * ly + cj) * 64 + lane] = sy;
+                }
+                cj = j;
+                sy = 0.0f;
+            }
+            sy -= t;
+        }
+        if (go.mode == 1) {
+            if (active) go.dX[p * go.ldx + i] = sx;
+        } else if (go.xred) {
+            const float tot = wave_sum(sx);
+            if (lane == 0) {
+                const int64_t bc = (wave * 64 - a * go.ny) / 64;
+                if (go.nbc == 1)
+                    go.dX[a * go.ldx + i] = tot;
+                else
+                    go.xpart[(a * go.nbc + bc) * lx + i] = tot;
+            }
+        } else {
+            go.gxp[(wave * lx + i) * 64 + lane] = sx;
+        }
+        k = isd ? kx : kx + 1;
+        xi = xn;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            wd[w] = nd[w];
+            wu[w] = nu[w];
         }
     }
-    gx[ci] = sx;
-    gy[cj] = sy;
+    if (go.mode == 1) {
+        if (active) go.dY[p * go.ldy + cj] = sy;
+    } else {
+        go.gyp[(wave * ly + cj) * 64 + lane] = sy;
+    }
 }
 
-// dX[a][i] = sum_b gxp[a * ny + b][i]   (GRID: rows of one a are contiguous)
+__device__ __forceinline__ int64_t pair_slot(int64_t p, int len, int idx) {
+    return ((p >> 6) * len + idx) * 64 + (p & 63);
+}
+
+// dX[a][i] = sum_b gxp[pair (a, b)][i], b in order (GRID without lane sharing)
 __global__ __launch_bounds__(256) void k_sum_rows(const float *__restrict__ gxp, int64_t nx,
                                                   int64_t ny, int lx, float *__restrict__ dX,
                                                   int64_t ldx) {
@@ -223,11 +346,24 @@ __global__ __launch_bounds__(256) void k_sum_rows(const float *__restrict__ gxp,
     const int64_t a = q / lx;
     const int i = (int)(q - a * lx);
     float s = 0.0f;
-    for (int64_t b = 0; b < ny; ++b) s += gxp[(a * ny + b) * lx + i];
+    for (int64_t b = 0; b < ny; ++b) s += gxp[pair_slot(a * ny + b, lx, i)];
     dX[a * ldx + i] = s;
 }
 
-// partial[c][b][j] = sum over a in chunk c of gyp[a * ny + b][j]
+// dX[a][i] = sum over b-chunks c of xpart[a][c][i]
+__global__ __launch_bounds__(256) void k_sum_xpart(const float *__restrict__ xpart, int64_t nx,
+                                                   int64_t nbc, int lx, float *__restrict__ dX,
+                                                   int64_t ldx) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= nx * lx) return;
+    const int64_t a = q / lx;
+    const int i = (int)(q - a * lx);
+    float s = 0.0f;
+    for (int64_t c = 0; c < nbc; ++c) s += xpart[(a * nbc + c) * lx + i];
+    dX[a * ldx + i] = s;
+}
+
+// part[c][j][b] = sum over a in chunk c of gyp[pair (a, b)][j], a in order
 __global__ __launch_bounds__(256) void k_sum_cols_partial(const float *__restrict__ gyp,
                                                           int64_t nx, int64_t ny, int ly,
                                                           int chunk, float *__restrict__ part) {
@@ -235,12 +371,12 @@ __global__ __launch_bounds__(256) void k_sum_cols_partial(const float *__restric
     const int64_t per = ny * ly;
     const int c = blockIdx.y;
     if (q >= per) return;
-    const int64_t b = q / ly;
-    const int j = (int)(q - b * ly);
+    const int j = (int)(q / ny);
+    const int64_t b = q - (int64_t)j * ny;
     const int64_t a0 = (int64_t)c * chunk;
     const int64_t a1 = a0 + chunk < nx ? a0 + chunk : nx;
     float s = 0.0f;
-    for (int64_t a = a0; a < a1; ++a) s += gyp[(a * ny + b) * ly + j];
+    for (int64_t a = a0; a < a1; ++a) s += gyp[pair_slot(a * ny + b, ly, j)];
     part[c * per + q] = s;
 }
 
@@ -252,16 +388,9 @@ __global__ __launch_bounds__(256) void k_sum_cols_final(const float *__restrict_
     if (q >= per) return;
     float s = 0.0f;
     for (int c = 0; c < nc; ++c) s += part[c * per + q];
-    const int64_t b = q / ly;
-    dY[b * ldy_ + (q - b * ly)] = s;
-}
-
-__global__ __launch_bounds__(256) void k_copy_rows(const float *__restrict__ src, int64_t n,
-                                                   int len, float *__restrict__ dst, int64_t ld) {
-    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (q >= n * len) return;
-    const int64_t a = q / len;
-    dst[a * ld + (q - a * len)] = src[q];
+    const int j = (int)(q / ny);
+    const int64_t b = q - (int64_t)j * ny;
+    dY[b * ldy_ + j] = s;
 }
 
 constexpr int TRAIN_MAX_BP = 112;
@@ -278,14 +407,14 @@ static void train_fwd_one(const PairSpec &ps, Series<float> xs, Series<float> ys
 
 template <int BP>
 static void train_bwd_one(const PairSpec &ps, Series<float> xs, Series<float> ys, LaneGeo g,
-                          const float *sq, const double *G, const uint32_t *dirs, float *gxp,
-                          float *gyp, hipStream_t st) {
+                          const float *sq, const double *G, const uint32_t *dirs, GradOut go,
+                          hipStream_t st) {
     hipLaunchKernelGGL((k_lane_backtrack<BP>), dim3((unsigned)((ps.n_pairs + 63) / 64)), dim3(64), 0,
-                       st, ps, xs, ys, g, sq, G, dirs, gxp, gyp);
+                       st, ps, xs, ys, g, sq, G, dirs, go);
 }
 
 #define TRAIN_CASES(X)                                                                          \
-    X(8, 8) X(16, 8) X(24, 8) X(32, 8) X(40, 8) X(48, 8) X(56, 8) X(64, 8) X(72, 4) X(80, 4)    \
+    X(8, 4) X(16, 4) X(24, 4) X(32, 4) X(40, 4) X(48, 4) X(56, 4) X(64, 4) X(72, 4) X(80, 4)    \
         X(88, 4) X(96, 4) X(104, 4) X(112, 4)
 
 struct TrainGeo {
@@ -308,6 +437,42 @@ static int train_geo(const wbx_metric *m, int64_t tx, int64_t ty, TrainGeo *out)
     return WBX_OK;
 }
 
+// Backward scratch (GRID only): per-pair y rows, per-pair x rows when the
+// lanes of a wave do not share an x row, x partials per 64-column chunk, and
+// the column partials.
+struct TrainScratch {
+    bool xred;
+    int64_t nbc, nc, xt, yt, gy, gx, xpart, part, total;  // byte offsets / sizes
+};
+
+static TrainScratch train_scratch(int mode, int64_t nx, int64_t ny, int64_t tx, int64_t ty) {
+    TrainScratch t{};
+    auto al = [](int64_t v) { return (v + 255) / 256 * 256; };
+    int64_t o = 0;
+    t.xt = o;
+    o += al(nx * tx * 4);
+    t.yt = o;
+    o += al(ny * ty * 4);
+    if (mode != 0) {
+        t.total = o;
+        return t;
+    }
+    const int64_t np = nx * ny, waves = (np + 63) / 64;
+    t.xred = ny % 64 == 0;
+    t.nbc = t.xred ? ny / 64 : 0;
+    t.nc = (nx + COL_CHUNK - 1) / COL_CHUNK;
+    t.gy = o;
+    o += al(waves * ty * 64 * 4);
+    t.gx = o;
+    if (!t.xred) o += al(waves * tx * 64 * 4);
+    t.xpart = o;
+    if (t.xred && t.nbc > 1) o += al(nx * t.nbc * tx * 4);
+    t.part = o;
+    o += al(t.nc * ny * ty * 4);
+    t.total = o;
+    return t;
+}
+
 }  // namespace wbx
 
 using namespace wbx;
@@ -326,8 +491,7 @@ int wbx_dtw_train_sizes(const wbx_metric *metric, int mode, int64_t nx, int64_t 
     const int64_t waves = (np + 63) / 64;
     const int64_t planes = waves * tx * 2 * tg.NW * 64 * 4;
     if (state_bytes) *state_bytes = planes + ((np * 4 + 255) / 256) * 256;
-    const int64_t chunks = mode == 0 ? (nx + COL_CHUNK - 1) / COL_CHUNK : 0;
-    if (scratch_bytes) *scratch_bytes = np * (tx + ty) * 4 + chunks * ny * ty * 4 + 1024;
+    if (scratch_bytes) *scratch_bytes = train_scratch(mode, nx, ny, tx, ty).total;
     return WBX_OK;
 }
 
@@ -376,35 +540,54 @@ int wbx_dev_dtw_train_backward(const wbx_metric *metric, int mode, const float *
     hipStream_t st = (hipStream_t)stream;
     PairSpec ps{mode == 0 ? PAIR_GRID : PAIR_DIAG, mode == 0 ? nx * ny : nx, mode == 0 ? ny : nx,
                 nullptr, nullptr};
-    const int64_t np = ps.n_pairs;
-    const int64_t waves = (np + 63) / 64;
+    const int64_t waves = (ps.n_pairs + 63) / 64;
     const uint32_t *dirs = (const uint32_t *)state;
     const float *sq = (const float *)((const char *)state + waves * tx * 2 * tg.NW * 64 * 4);
-    float *gxp = (float *)scratch;
-    float *gyp = gxp + np * tx;
-    float *part = gyp + np * ty;
+    const TrainScratch ts = train_scratch(mode, nx, ny, tx, ty);
+    char *sc = (char *)scratch;
+    GradOut go{};
+    go.mode = mode;
+    go.xred = ts.xred ? 1 : 0;
+    go.nx = nx;
+    go.ny = ny;
+    go.nbc = ts.nbc;
+    go.dX = dX;
+    go.dY = dY;
+    go.ldx = ldx;
+    go.ldy = ldy_;
+    go.gyp = (float *)(sc + ts.gy);
+    go.gxp = (float *)(sc + ts.gx);
+    go.xpart = (float *)(sc + ts.xpart);
+    float *part = (float *)(sc + ts.part);
+    float *xt = (float *)(sc + ts.xt), *yt = (float *)(sc + ts.yt);
+    hipLaunchKernelGGL(k_transpose, dim3((unsigned)((tx + 31) / 32), (unsigned)((nx + 31) / 32)),
+                       dim3(256), 0, st, X, nx, (int)tx, x_stride, xt);
+    hipLaunchKernelGGL(k_transpose, dim3((unsigned)((ty + 31) / 32), (unsigned)((ny + 31) / 32)),
+                       dim3(256), 0, st, Y, ny, (int)ty, y_stride, yt);
+    go.XT = xt;
+    go.YT = yt;
     Series<float> xs{X, x_stride, (int)tx, nullptr}, ys{Y, y_stride, (int)ty, nullptr};
     switch (tg.BP) {
 #define BWD_CASE(BPV, UV) \
-    case BPV: train_bwd_one<BPV>(ps, xs, ys, tg.g, sq, G, dirs, gxp, gyp, st); break;
+    case BPV: train_bwd_one<BPV>(ps, xs, ys, tg.g, sq, G, dirs, go, st); break;
         TRAIN_CASES(BWD_CASE)
 #undef BWD_CASE
         default: return fail(WBX_EUNSUPPORTED, "band %d", tg.g.B);
     }
     WBX_HIP_CHECK(hipGetLastError());
     if (mode == 0) {
-        hipLaunchKernelGGL(k_sum_rows, dim3((unsigned)((nx * tx + 255) / 256)), dim3(256), 0, st, gxp,
-                           nx, ny, (int)tx, dX, ldx);
-        const int nc = (int)((nx + COL_CHUNK - 1) / COL_CHUNK);
-        hipLaunchKernelGGL(k_sum_cols_partial, dim3((unsigned)((ny * ty + 255) / 256), (unsigned)nc),
-                           dim3(256), 0, st, gyp, nx, ny, (int)ty, COL_CHUNK, part);
-        hipLaunchKernelGGL(k_sum_cols_final, dim3((unsigned)((ny * ty + 255) / 256)), dim3(256), 0,
-                           st, part, nc, ny, (int)ty, dY, ldy_);
-    } else {
-        hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((nx * tx + 255) / 256)), dim3(256), 0, st,
-                           gxp, nx, (int)tx, dX, ldx);
-        hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((ny * ty + 255) / 256)), dim3(256), 0, st,
-                           gyp, ny, (int)ty, dY, ldy_);
+        const unsigned gx_blocks = (unsigned)((nx * tx + 255) / 256);
+        if (!ts.xred)
+            hipLaunchKernelGGL(k_sum_rows, dim3(gx_blocks), dim3(256), 0, st, go.gxp, nx, ny,
+                               (int)tx, dX, ldx);
+        else if (ts.nbc > 1)
+            hipLaunchKernelGGL(k_sum_xpart, dim3(gx_blocks), dim3(256), 0, st, go.xpart, nx, ts.nbc,
+                               (int)tx, dX, ldx);
+        const unsigned gy_blocks = (unsigned)((ny * ty + 255) / 256);
+        hipLaunchKernelGGL(k_sum_cols_partial, dim3(gy_blocks, (unsigned)ts.nc), dim3(256), 0, st,
+                           go.gyp, nx, ny, (int)ty, COL_CHUNK, part);
+        hipLaunchKernelGGL(k_sum_cols_final, dim3(gy_blocks), dim3(256), 0, st, part, (int)ts.nc,
+                           ny, (int)ty, dY, ldy_);
     }
     WBX_HIP_CHECK(hipGetLastError());
     return WBX_OK;

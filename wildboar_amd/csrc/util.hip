@@ -37,6 +37,29 @@ template int launch_convert_rows<double, double>(const double *, int64_t, int64_
 template int launch_convert_rows<float, float>(const float *, int64_t, int64_t, int64_t, float *,
                                                int64_t, hipStream_t);
 
+__global__ __launch_bounds__(256) void k_round_f32_rows(const double *__restrict__ src, int64_t n,
+                                                        int64_t len, int64_t sstride,
+                                                        double *__restrict__ dst, int64_t dstride) {
+    const int64_t total = n * len;
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < total;
+         q += (int64_t)gridDim.x * 256) {
+        const int64_t i = q / len, c = q - i * len;
+        dst[i * dstride + c] = (double)(float)src[i * sstride + c];
+    }
+}
+
+int launch_round_f32_rows(const double *src, int64_t n, int64_t len, int64_t sstride, double *dst,
+                          int64_t dstride, hipStream_t st) {
+    const int64_t total = n * len;
+    if (total <= 0) return WBX_OK;
+    int64_t nb = (total + 255) / 256;
+    if (nb > 16384) nb = 16384;
+    hipLaunchKernelGGL(k_round_f32_rows, dim3((unsigned)nb), dim3(256), 0, st, src, n, len, sstride,
+                       dst, dstride);
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
+
 __global__ __launch_bounds__(256) void k_fill_f64(double *__restrict__ dst, int64_t n, double v) {
     for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n;
          q += (int64_t)gridDim.x * 256)

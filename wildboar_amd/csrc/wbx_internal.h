@@ -139,6 +139,7 @@ struct DeviceCtx {
     hipStream_t stream = nullptr;
     hipEvent_t last = nullptr;
     Workspace ws;
+    Workspace ws_in;  // inputs rounded to float32 for the FP32_ACC64 mode
     int n_cu = 0;
     int64_t w_n = -1;
     double w_g = 0.0;
@@ -148,6 +149,7 @@ struct DeviceCtx {
 
 DeviceCtx *ctx_current();              // current device's context (created lazily)
 int ws_reserve(DeviceCtx *ctx, size_t bytes, void **out);
+int ws_in_reserve(DeviceCtx *ctx, size_t bytes, void **out);
 // WDTW weights for width n and penalty g, computed on the host with libm exp
 // and cached on the device (one entry per device).
 int cached_weights(DeviceCtx *ctx, int64_t n, double g, const double **w64, const float **w32);
@@ -195,6 +197,9 @@ int launch_convert_rows(const S *src, int64_t n, int64_t len, int64_t sstride, D
                         int64_t dstride, hipStream_t st);
 int launch_fill_f64(double *dst, int64_t n, double value, hipStream_t st);
 int launch_fill_i32(int *dst, int64_t n, int value, hipStream_t st);
+// dst[i][c] = (double)(float)src[i][c]: float64 rows rounded to float32 values.
+int launch_round_f32_rows(const double *src, int64_t n, int64_t len, int64_t sstride, double *dst,
+                          int64_t dstride, hipStream_t st);
 // *out = max |src[i * stride + c]| as float bits (out must be zeroed first).
 template <typename S>
 int launch_absmax_rows(const S *src, int64_t n, int64_t len, int64_t stride, unsigned *out,

@@ -108,11 +108,10 @@ def dtw_barycenter(X, r=1.0, *, steps=200, lr=1e-2, init=None, return_history=Fa
     mean = (X[0] if init is None else init).detach().clone().to(X.device, torch.float32)
     mean.requires_grad_(True)
     opt = torch.optim.Adam([mean], lr=lr)
-    n = X.shape[0]
     history = []
     for _ in range(steps):
         opt.zero_grad(set_to_none=True)
-        loss = paired_dtw(mean.unsqueeze(0).expand(n, -1).contiguous(), X, r).mean()
+        loss = dtw(mean.unsqueeze(0), X, r).mean()
         loss.backward()
         opt.step()
         if return_history:
