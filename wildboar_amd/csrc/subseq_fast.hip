@@ -10,11 +10,13 @@
 //   window ws = 64 b + l of block b.  For DTW each lane runs the register-band
 //   recursion of lane.hip on its window (all lanes share the shapelet, so the
 //   geometry is uniform); for euclidean each lane sums (x - s)^2 over its
-//   window.  A window is kept as a candidate when its FP32 cost c satisfies
+//   window.  A window is a candidate when its FP32 cost c satisfies
 //   c <= tau(c*) = rho (c* + A) + A, c* the smallest FP32 cost so far: by the
-//   bound in rerank.hip no other window can hold the float64 minimum.  Up to
-//   kMaxCand candidates per task are stored in window order; more sets an
-//   overflow flag.
+//   bound in rerank.hip no other window can hold the float64 minimum.  The
+//   wave keeps up to 64 candidates with their costs in LDS and drops those
+//   that fail tau of the current best when the list fills; at the end the
+//   ones within tau(final best) are stored (at most kMaxCand, else the task
+//   is marked as overflowed).
 // Pass 2 (FP64): one candidate per lane, evaluated with the reference's exact
 //   arithmetic (DTW: min(min(up, left), diag) + v*v, no FMA; euclidean: the
 //   sequential sum), then per task the first strictly smallest candidate wins.
@@ -33,14 +35,71 @@ __device__ __forceinline__ float wave_min(float v) {
     return v;
 }
 
-// Append this block's candidates (lanes with `take`) after `count` entries.
-__device__ __forceinline__ void push_candidates(SubCand &cand_local, bool take, int ws,
-                                                SubCand *__restrict__ out) {
+// Wave-level candidate list in LDS: (window, FP32 cost) pairs, up to kCap.
+// A window enters when its cost is within tau(best so far); when the list
+// would overflow, entries no longer within tau(best now) are dropped (the
+// best only decreases, so a dropped entry could never pass the final test).
+constexpr int kCap = 64;
+
+struct CandList {
+    int count;
+    bool overflow;
+};
+
+__device__ __forceinline__ void cand_push(CandList &cl, int *s_win, float *s_cost, bool take,
+                                          int ws, float c, float tau) {
+    const int lane = threadIdx.x;
+    const uint64_t below = (1ull << lane) - 1ull;
     const uint64_t mask = __ballot(take);
-    const int before = __popcll(mask & ((1ull << threadIdx.x) - 1ull));
-    const int base = cand_local.count;
-    if (take && base + before < kMaxCand) out->win[base + before] = ws;
-    cand_local.count = base + (int)__popcll(mask);
+    const int npush = (int)__popcll(mask);
+    if (npush == 0 || cl.overflow) return;  // uniform
+    if (cl.count + npush > kCap) {
+        __syncthreads();
+        const bool has = lane < cl.count;
+        const int w = has ? s_win[lane] : 0;
+        const float cc = has ? s_cost[lane] : WBX_INF_F;
+        const bool keep = has && cc <= tau;
+        const uint64_t km = __ballot(keep);
+        __syncthreads();
+        if (keep) {
+            const int pos = (int)__popcll(km & below);
+            s_win[pos] = w;
+            s_cost[pos] = cc;
+        }
+        cl.count = (int)__popcll(km);
+        __syncthreads();
+        if (cl.count + npush > kCap) {
+            cl.overflow = true;
+            return;
+        }
+    }
+    if (take) {
+        const int pos = cl.count + (int)__popcll(mask & below);
+        s_win[pos] = ws;
+        s_cost[pos] = c;
+    }
+    cl.count += npush;
+}
+
+// Final list: the entries within tau(final best), at most kMaxCand of them,
+// else the task is marked for the exact rescan (count kMaxCand + 1).
+__device__ __forceinline__ void cand_finish(const CandList &cl, const int *s_win,
+                                            const float *s_cost, float tau, float best,
+                                            SubCand *__restrict__ out) {
+    const int lane = threadIdx.x;
+    __syncthreads();
+    const bool has = !cl.overflow && lane < cl.count;
+    const int w = has ? s_win[lane] : 0;
+    const float cc = has ? s_cost[lane] : WBX_INF_F;
+    const bool keep = has && cc <= tau;
+    const uint64_t km = __ballot(keep);
+    const int n = (int)__popcll(km);
+    const bool ovf = cl.overflow || n > kMaxCand;
+    if (!ovf && keep) out->win[__popcll(km & ((1ull << lane) - 1ull))] = w;
+    if (lane == 0) {
+        out->count = ovf ? kMaxCand + 1 : n;
+        out->best32 = best;
+    }
 }
 
 template <int BP, int U>
@@ -59,8 +118,9 @@ __global__ __launch_bounds__(64) void k_subseq_dtw32(TaskMap tasks, SubGeo sg,
 #pragma unroll
     for (int q = 0; q < 8; ++q) pen[q] = (BP - 8 + q == B) ? WBX_INF_F : -WBX_INF_F;
     SubCand *cout = cands + blockIdx.x;
-    SubCand cl;
-    cl.count = 0;
+    __shared__ int s_win[kCap];
+    __shared__ float s_cost[kCap];
+    CandList cl{0, false};
     float best = WBX_INF_F;
     for (int b0 = 0; b0 < nwin; b0 += 64) {
         const int ws = b0 + lane;
@@ -123,12 +183,9 @@ __global__ __launch_bounds__(64) void k_subseq_dtw32(TaskMap tasks, SubGeo sg,
         if (!active) c = WBX_INF_F;
         best = fminf(best, wave_min(c));
         const float tau = sg.rho * (best + sg.A) + sg.A;
-        push_candidates(cl, active && c <= tau, ws, cout);
+        cand_push(cl, s_win, s_cost, active && c <= tau, ws, c, tau);
     }
-    if (lane == 0) {
-        cout->count = cl.count;
-        cout->best32 = best;
-    }
+    cand_finish(cl, s_win, s_cost, sg.rho * (best + sg.A) + sg.A, best, cout);
 }
 
 template <int U>
@@ -141,8 +198,9 @@ __global__ __launch_bounds__(64) void k_subseq_euclid32(TaskMap tasks, SubGeo sg
     const int nwin = sg.T - L + 1;
     const float *__restrict__ Xrow = sg.X + (int64_t)tk.sample * sg.xstride;
     SubCand *cout = cands + blockIdx.x;
-    SubCand cl;
-    cl.count = 0;
+    __shared__ int s_win[kCap];
+    __shared__ float s_cost[kCap];
+    CandList cl{0, false};
     float best = WBX_INF_F;
     // each lane scans U consecutive windows per block of 64 U
     for (int b0 = 0; b0 < nwin; b0 += 64 * U) {
@@ -179,13 +237,10 @@ __global__ __launch_bounds__(64) void k_subseq_euclid32(TaskMap tasks, SubGeo sg
             // candidates of window q of every lane: in window order within
             // this q they are lane-ordered; across q the order is fixed up by
             // the select pass, which compares (cost, window) pairs.
-            push_candidates(cl, active && c <= tau, ws, cout);
+            cand_push(cl, s_win, s_cost, active && c <= tau, ws, c, tau);
         }
     }
-    if (lane == 0) {
-        cout->count = cl.count;
-        cout->best32 = best;
-    }
+    cand_finish(cl, s_win, s_cost, sg.rho * (best + sg.A) + sg.A, best, cout);
 }
 
 // ------------------------------------------------------------------ exact

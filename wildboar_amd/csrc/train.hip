@@ -68,11 +68,11 @@ __device__ __forceinline__ void train_row(float (&row)[BP], const float *yw, flo
     for (int q = 1; q < 8; ++q) row[BP - 8 + q] = fmaxf(row[BP - 8 + q], pen[q]);
 }
 
-// Plane words of wave v, row i, word w (0..NW-1 diag, NW..2NW-1 up), lane l:
-// dirs[((v * lx + i) * 2 NW + w) * 64 + l].
-__device__ __forceinline__ int64_t plane_index(int64_t wave, int lx, int i, int w, int nw2,
-                                               int lane) {
-    return (((int64_t)wave * lx + i) * nw2 + w) * 64 + lane;
+// Plane words of wave v, row i, lane l: 2 NW words starting at
+// dirs[((v * lx + i) * 64 + l) * 2 NW], as (diag word w, up word w) pairs, so a
+// lane's row is one contiguous piece and a wave's row is 64 of them in a row.
+__device__ __forceinline__ int64_t plane_base(int64_t wave, int lx, int i, int nw2, int lane) {
+    return (((int64_t)wave * lx + i) * 64 + lane) * nw2;
 }
 
 template <int BP>
@@ -80,10 +80,10 @@ __device__ __forceinline__ void store_planes(uint32_t *__restrict__ dirs, int64_
                                              int lx, int i, uint32_t (&pd)[(BP + 31) / 32],
                                              uint32_t (&pu)[(BP + 31) / 32]) {
     constexpr int NW = (BP + 31) / 32;
+    uint2 *d = (uint2 *)(dirs + plane_base(wave, lx, i, 2 * NW, lane));
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
-        dirs[plane_index(wave, lx, i, w, 2 * NW, lane)] = pd[w];
-        dirs[plane_index(wave, lx, i, NW + w, 2 * NW, lane)] = pu[w];
+        d[w] = make_uint2(pd[w], pu[w]);
         pd[w] = 0u;
         pu[w] = 0u;
     }
@@ -173,19 +173,40 @@ __device__ __forceinline__ float wave_sum(float v) {
 }
 #undef WBX_DPP
 
-// Where the per-pair gradients go.
+// Where the per-pair gradients go.  For column j of pair (a, b) the backward
+// writes u = coef * (number of path cells in column j) and v = coef * (sum of
+// x_i over those cells); dD/dy_j = y_j u - v is formed in the reduction, so the
+// path walk itself reads no y.
 struct GradOut {
     int mode;          // 0: GRID pairs, 1: DIAG (paired) pairs
     int xred;          // GRID with ny % 64 == 0: a wave's lanes share one x row
     int64_t nx, ny, nbc;  // nbc = ny / 64 when xred
-    float *dX, *dY;    // DIAG: written directly; GRID xred with nbc == 1: dX direct
-    int64_t ldx, ldy;
+    float *dX;         // DIAG, and GRID xred with nbc == 1: written directly
+    int64_t ldx;
     float *xpart;      // GRID xred, nbc > 1: [a][bchunk][i]
-    float *gxp, *gyp;  // GRID: per-pair rows, [wave][i or j][lane]
-    // X and Y transposed (XT[i * nx + a], YT[j * ny + b]): the lanes of a wave
-    // hold neighbouring pairs, so their loads at one row or column coalesce
-    const float *XT, *YT;
+    float *gxp;        // GRID without lane sharing: per-pair rows [wave][i][lane]
+    float *gu, *gv;    // per-pair column terms, pair-major rows of lyp floats
+    int lyp;           // ly rounded up to a multiple of 8 (rows stay 32-byte aligned)
+    // X transposed (XT[i * nx + a]) and prefix sums of Y transposed
+    // (PT[j * ny + b] = y_b[0] + ... + y_b[j-1], j in [0, ly], float64): the
+    // lanes of a wave hold neighbouring pairs, so their loads coalesce
+    const float *XT;
+    const double *PT;
 };
+
+// PT[j * ny + b] = sum of y_b[0 .. j), one thread per series.
+__global__ __launch_bounds__(256) void k_prefix_t(const float *__restrict__ Y, int64_t ny, int ly,
+                                                  int64_t stride, double *__restrict__ PT) {
+    const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (b >= ny) return;
+    const float *y = Y + b * stride;
+    double acc = 0.0;
+    PT[b] = 0.0;
+    for (int j = 0; j < ly; ++j) {
+        acc += (double)y[j];
+        PT[(int64_t)(j + 1) * ny + b] = acc;
+    }
+}
 
 // dst[c * n + r] = src[r * stride + c] through a 32 x 32 LDS tile.
 __global__ __launch_bounds__(256) void k_transpose(const float *__restrict__ src, int64_t n,
@@ -210,8 +231,7 @@ __global__ __launch_bounds__(256) void k_transpose(const float *__restrict__ src
 // Bit of slot k in its plane word: slots are shifted in from the low end, so
 // slot k of word w sits at bit (count_w - 1 - (k - 32 w)).
 template <int BP>
-__global__ __launch_bounds__(64) void k_lane_backtrack(PairSpec ps, Series<float> xs,
-                                                       Series<float> ys, LaneGeo g,
+__global__ __launch_bounds__(64) void k_lane_backtrack(PairSpec ps, LaneGeo g,
                                                        const float *__restrict__ sq,
                                                        const double *__restrict__ G,
                                                        const uint32_t *__restrict__ dirs,
@@ -225,112 +245,143 @@ __global__ __launch_bounds__(64) void k_lane_backtrack(PairSpec ps, Series<float
     const int64_t p = active ? p0 : ps.n_pairs - 1;
     int64_t a, b;
     decode_pair(ps, p, a, b);
-    const float *__restrict__ XT = go.XT + a;  // x_i = XT[i * nx]
-    const float *__restrict__ YT = go.YT + b;  // y_j = YT[j * ny]
+    const float *__restrict__ XT = go.XT + a;   // x_i = XT[i * nx]
+    const double *__restrict__ PT = go.PT + b;  // prefix of y_b up to j: PT[j * ny]
     const int64_t nxs = go.nx, nys = go.ny;
-    (void)xs;
-    (void)ys;
     const int lx = g.lx, ly = g.ly, off = g.off, B = g.B;
     const double D = sqrt((double)sq[p]);
-    const float coef = (active && D > 0.0) ? (float)(G[p] / D) : 0.0f;
+    const double coefd = (active && D > 0.0) ? G[p] / D : 0.0;
+    const float coef = (float)coefd;
+    // Columns are finished in decreasing order, each exactly once: their
+    // terms go through a CW-entry shift window and out as one vector store per
+    // CW columns (columns >= ly of the top window count as finished zeros).
+    constexpr int CW = 8;
+    typedef float fcw __attribute__((ext_vector_type(CW)));
+    fcw *__restrict__ guw = (fcw *)(go.gu + p * go.lyp);
+    fcw *__restrict__ gvw = (fcw *)(go.gv + p * go.lyp);
+    fcw wu_ = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    fcw wv_ = wu_;
+    int fill = (CW - ly % CW) % CW;
+    auto finish = [&](int col, float u, float v) {
+        wu_ = fcw{u, wu_[0], wu_[1], wu_[2], wu_[3], wu_[4], wu_[5], wu_[6]};
+        wv_ = fcw{v, wv_[0], wv_[1], wv_[2], wv_[3], wv_[4], wv_[5], wv_[6]};
+        if (++fill == CW) {  // col is a multiple of CW: columns col .. col + CW - 1
+            if (active) {
+                guw[col / CW] = wu_;
+                gvw[col / CW] = wv_;
+            }
+            fill = 0;
+        }
+    };
 
-    uint32_t wd[NW], wu[NW], nd[NW], nu[NW];
+    // Rows are taken RB at a time: first the path through the RB rows (bits
+    // only, already in registers) while every x and prefix load those rows need
+    // is issued, then the sums; the next RB rows' bits are fetched in between.
+    constexpr int RB = 5;
+    uint32_t wd[RB][NW], wu[RB][NW];
+    auto load_bits = [&](int ih) {
 #pragma unroll
-    for (int w = 0; w < NW; ++w) {
-        wd[w] = dirs[plane_index(wave, lx, lx - 1, w, 2 * NW, lane)];
-        wu[w] = dirs[plane_index(wave, lx, lx - 1, NW + w, 2 * NW, lane)];
-    }
+        for (int r = 0; r < RB; ++r) {
+            const int i = ih - r < 0 ? 0 : ih - r;
+            const uint2 *src = (const uint2 *)(dirs + plane_base(wave, lx, i, 2 * NW, lane));
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                const uint2 v = src[w];
+                wd[r][w] = v.x;
+                wu[r][w] = v.y;
+            }
+        }
+    };
+    load_bits(lx - 1);
     int k = g.kf;
-    int cj = ly - 1;
-    float sy = 0.0f;
-    float xi = XT[(int64_t)(lx - 1) * nxs];
-    for (int i = lx - 1; i >= 0; --i) {
-        const int klo = off - i > 0 ? off - i : 0;
-        int khi = ly - 1 - i + off;
-        if (khi > B - 1) khi = B - 1;
-        if (k > khi) k = khi;
-        // y of the first two cells of the segment, fetched before the scan
-        float yq[2];
+    int cj = ly - 1;   // column being accumulated
+    float cnt = 0.0f;  // its number of path cells
+    float sxc = 0.0f;  // and the sum of their x
+    for (int ih = lx - 1; ih >= 0; ih -= RB) {
+        int jer[RB], jxr[RB];
+        float xr[RB];
+        double ph[RB], pl[RB];
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int j = i + k - off - q;
-            yq[q] = YT[(int64_t)(j < 0 ? 0 : j) * nys];
-        }
-        // the next row's words and x (addresses do not depend on the path)
-        const int ip = i > 0 ? i - 1 : 0;
+        for (int r = 0; r < RB; ++r) {
+            const int i = ih - r;
+            if (i < 0) break;  // uniform
+            const int klo = off - i > 0 ? off - i : 0;
+            int khi = ly - 1 - i + off;
+            if (khi > B - 1) khi = B - 1;
+            if (k > khi) k = khi;
+            // exit: the highest slot <= k whose decision is diag or up
+            const int wk = k >> 5;
+            int kx = -1;
+            bool isd = false;
 #pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            nd[w] = dirs[plane_index(wave, lx, ip, w, 2 * NW, lane)];
-            nu[w] = dirs[plane_index(wave, lx, ip, NW + w, 2 * NW, lane)];
-        }
-        const float xn = XT[(int64_t)ip * nxs];
-        // exit: the highest slot <= k whose decision is diag or up
-        const int wk = k >> 5;
-        int kx = -1;
-        bool isd = false;
-#pragma unroll
-        for (int w = NW - 1; w >= 0; --w) {
-            const int cnt = (w == NW - 1) ? LASTC : 32;
-            uint32_t msk = wd[w] | wu[w];
-            if (w > wk) msk = 0u;
-            if (w == wk) msk &= ~0u << (cnt - 1 - (k - 32 * w));
-            if (kx < 0 && msk != 0u) {
-                const int q = __builtin_ctz(msk);
-                kx = 32 * w + cnt - 1 - q;
-                isd = ((wd[w] >> q) & 1u) != 0u;
-            }
-        }
-        // Row 0 always ends at column 0 (only left moves there, dtw.py:404-406);
-        // kx < klo is unreachable on a finite path and keeps the walk in bounds.
-        if (i == 0 || kx < klo) {
-            kx = klo;
-            isd = false;
-        }
-        float sx = 0.0f;
-        for (int s = k; s >= kx; --s) {
-            const int j = i + s - off;
-            const int q = k - s;
-            const float yv = q == 0 ? yq[0] : q == 1 ? yq[1] : YT[(int64_t)j * nys];
-            const float t = coef * (xi - yv);
-            sx += t;
-            if (j != cj) {
-                if (go.mode == 1) {
-                    if (active) go.dY[p * go.ldy + cj] = sy;
-                } else {
-                    go.gyp[(wave * ly + cj) * 64 + lane] = sy;
+            for (int w = NW - 1; w >= 0; --w) {
+                const int cnt_w = (w == NW - 1) ? LASTC : 32;
+                uint32_t msk = wd[r][w] | wu[r][w];
+                if (w > wk) msk = 0u;
+                if (w == wk) msk &= ~0u << (cnt_w - 1 - (k - 32 * w));
+                if (kx < 0 && msk != 0u) {
+                    const int q = __builtin_ctz(msk);
+                    kx = 32 * w + cnt_w - 1 - q;
+                    isd = ((wd[r][w] >> q) & 1u) != 0u;
                 }
-                cj = j;
-                sy = 0.0f;
             }
-            sy -= t;
-        }
-        if (go.mode == 1) {
-            if (active) go.dX[p * go.ldx + i] = sx;
-        } else if (go.xred) {
-            const float tot = wave_sum(sx);
-            if (lane == 0) {
-                const int64_t bc = (wave * 64 - a * go.ny) / 64;
-                if (go.nbc == 1)
-                    go.dX[a * go.ldx + i] = tot;
-                else
-                    go.xpart[(a * go.nbc + bc) * lx + i] = tot;
+            // Row 0 always ends at column 0 (only left moves there, dtw.py:404-406);
+            // kx < klo is unreachable on a finite path and keeps the walk in bounds.
+            if (i == 0 || kx < klo) {
+                kx = klo;
+                isd = false;
             }
-        } else {
-            go.gxp[(wave * lx + i) * 64 + lane] = sx;
+            if (k < kx) k = kx;  // likewise unreachable; keeps je >= jx
+            const int je = i + k - off, jx = i + kx - off;  // in [0, ly)
+            jer[r] = je;
+            jxr[r] = jx;
+            xr[r] = XT[(int64_t)i * nxs];
+            ph[r] = PT[(int64_t)(je + 1) * nys];
+            pl[r] = PT[(int64_t)jx * nys];
+            k = isd ? kx : kx + 1;
         }
-        k = isd ? kx : kx + 1;
-        xi = xn;
+        if (ih - RB >= 0) load_bits(ih - RB);
 #pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            wd[w] = nd[w];
-            wu[w] = nu[w];
+        for (int r = 0; r < RB; ++r) {
+            const int i = ih - r;
+            if (i < 0) break;  // uniform
+            const float xi = xr[r];
+            const int je = jer[r], jx = jxr[r];
+            // The open column cj takes this row's cell when the row enters
+            // there (an up step; a diagonal step enters at cj - 1).  Columns
+            // cj down to jx + 1 are then finished: cj with its sums, the rest
+            // with this row's single cell; jx becomes the open column.
+            if (je == cj) {
+                cnt += 1.0f;
+                sxc += xi;
+            }
+            const int nfin = cj - jx;
+            for (int t = 0; t < nfin; ++t)
+                finish(cj - t, t == 0 ? coef * cnt : coef, t == 0 ? coef * sxc : coef * xi);
+            if (nfin > 0) {
+                cj = jx;
+                cnt = 1.0f;
+                sxc = xi;
+            }
+            // dD/dx_i: coef * sum over the row's cells of (x_i - y_j)
+            const float gx = (float)(coefd * ((double)(je - jx + 1) * (double)xi - (ph[r] - pl[r])));
+            if (go.mode == 1) {
+                if (active) go.dX[p * go.ldx + i] = gx;
+            } else if (go.xred) {
+                const float tot = wave_sum(gx);
+                if (lane == 0) {
+                    const int64_t bc = (wave * 64 - a * go.ny) / 64;
+                    if (go.nbc == 1)
+                        go.dX[a * go.ldx + i] = tot;
+                    else
+                        go.xpart[(a * go.nbc + bc) * lx + i] = tot;
+                }
+            } else {
+                go.gxp[(wave * lx + i) * 64 + lane] = gx;
+            }
         }
     }
-    if (go.mode == 1) {
-        if (active) go.dY[p * go.ldy + cj] = sy;
-    } else {
-        go.gyp[(wave * ly + cj) * 64 + lane] = sy;
-    }
+    finish(cj, coef * cnt, coef * sxc);  // column 0 completes the last window
 }
 
 __device__ __forceinline__ int64_t pair_slot(int64_t p, int len, int idx) {
@@ -363,34 +414,60 @@ __global__ __launch_bounds__(256) void k_sum_xpart(const float *__restrict__ xpa
     dX[a * ldx + i] = s;
 }
 
-// part[c][j][b] = sum over a in chunk c of gyp[pair (a, b)][j], a in order
-__global__ __launch_bounds__(256) void k_sum_cols_partial(const float *__restrict__ gyp,
-                                                          int64_t nx, int64_t ny, int ly,
-                                                          int chunk, float *__restrict__ part) {
+// part[c][0][j][b], part[c][1][j][b]: sums over a in chunk c of the column
+// terms u, v of pair (a, b), a in order
+__global__ __launch_bounds__(256) void k_sum_cols_partial(const float *__restrict__ gu,
+                                                          const float *__restrict__ gv, int64_t nx,
+                                                          int64_t ny, int ly, int lyp, int chunk,
+                                                          float *__restrict__ part) {
     const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t per = ny * ly;
     const int c = blockIdx.y;
     if (q >= per) return;
-    const int j = (int)(q / ny);
-    const int64_t b = q - (int64_t)j * ny;
+    const int64_t b = q / ly;
+    const int j = (int)(q - b * ly);
     const int64_t a0 = (int64_t)c * chunk;
     const int64_t a1 = a0 + chunk < nx ? a0 + chunk : nx;
-    float s = 0.0f;
-    for (int64_t a = a0; a < a1; ++a) s += gyp[pair_slot(a * ny + b, ly, j)];
-    part[c * per + q] = s;
+    float su = 0.0f, sv = 0.0f;
+    for (int64_t a = a0; a < a1; ++a) {
+        const int64_t o = (a * ny + b) * lyp + j;
+        su += gu[o];
+        sv += gv[o];
+    }
+    part[(2 * (int64_t)c) * per + q] = su;
+    part[(2 * (int64_t)c + 1) * per + q] = sv;
 }
 
+// dY[b][j] = y_bj * (sum of u) - (sum of v), chunks in order
 __global__ __launch_bounds__(256) void k_sum_cols_final(const float *__restrict__ part, int nc,
-                                                        int64_t ny, int ly, float *__restrict__ dY,
-                                                        int64_t ldy_) {
+                                                        const float *__restrict__ Y,
+                                                        int64_t y_stride, int64_t ny, int ly,
+                                                        float *__restrict__ dY, int64_t ldy_) {
     const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t per = ny * ly;
     if (q >= per) return;
-    float s = 0.0f;
-    for (int c = 0; c < nc; ++c) s += part[c * per + q];
-    const int j = (int)(q / ny);
-    const int64_t b = q - (int64_t)j * ny;
-    dY[b * ldy_ + j] = s;
+    float su = 0.0f, sv = 0.0f;
+    for (int c = 0; c < nc; ++c) {
+        su += part[(2 * (int64_t)c) * per + q];
+        sv += part[(2 * (int64_t)c + 1) * per + q];
+    }
+    const int64_t b = q / ly;
+    const int j = (int)(q - b * ly);
+    dY[b * ldy_ + j] = Y[b * y_stride + j] * su - sv;
+}
+
+// Paired pairs: dY[p][j] = y_pj u_pj - v_pj.
+__global__ __launch_bounds__(256) void k_dy_paired(const float *__restrict__ gu,
+                                                   const float *__restrict__ gv, int64_t n, int ly,
+                                                   int lyp, const float *__restrict__ Y,
+                                                   int64_t y_stride, float *__restrict__ dY,
+                                                   int64_t ldy_) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= n * ly) return;
+    const int64_t p = q / ly;
+    const int j = (int)(q - p * ly);
+    const int64_t o = p * lyp + j;
+    dY[p * ldy_ + j] = Y[p * y_stride + j] * gu[o] - gv[o];
 }
 
 constexpr int TRAIN_MAX_BP = 112;
@@ -406,11 +483,10 @@ static void train_fwd_one(const PairSpec &ps, Series<float> xs, Series<float> ys
 }
 
 template <int BP>
-static void train_bwd_one(const PairSpec &ps, Series<float> xs, Series<float> ys, LaneGeo g,
-                          const float *sq, const double *G, const uint32_t *dirs, GradOut go,
-                          hipStream_t st) {
+static void train_bwd_one(const PairSpec &ps, LaneGeo g, const float *sq, const double *G,
+                          const uint32_t *dirs, GradOut go, hipStream_t st) {
     hipLaunchKernelGGL((k_lane_backtrack<BP>), dim3((unsigned)((ps.n_pairs + 63) / 64)), dim3(64), 0,
-                       st, ps, xs, ys, g, sq, G, dirs, go);
+                       st, ps, g, sq, G, dirs, go);
 }
 
 #define TRAIN_CASES(X)                                                                          \
@@ -442,33 +518,37 @@ static int train_geo(const wbx_metric *m, int64_t tx, int64_t ty, TrainGeo *out)
 // the column partials.
 struct TrainScratch {
     bool xred;
-    int64_t nbc, nc, xt, yt, gy, gx, xpart, part, total;  // byte offsets / sizes
+    int64_t nbc, nc, xt, pt, gu, gv, gx, xpart, part, total;  // byte offsets / sizes
 };
 
 static TrainScratch train_scratch(int mode, int64_t nx, int64_t ny, int64_t tx, int64_t ty) {
     TrainScratch t{};
     auto al = [](int64_t v) { return (v + 255) / 256 * 256; };
+    const int64_t np = mode == 0 ? nx * ny : nx, waves = (np + 63) / 64;
     int64_t o = 0;
     t.xt = o;
     o += al(nx * tx * 4);
-    t.yt = o;
-    o += al(ny * ty * 4);
+    t.pt = o;
+    o += al((ty + 1) * ny * 8);
+    const int64_t lyp = (ty + 7) / 8 * 8;
+    t.gu = o;
+    o += al(np * lyp * 4);
+    t.gv = o;
+    o += al(np * lyp * 4);
+    t.gx = t.xpart = t.part = o;
     if (mode != 0) {
         t.total = o;
         return t;
     }
-    const int64_t np = nx * ny, waves = (np + 63) / 64;
     t.xred = ny % 64 == 0;
     t.nbc = t.xred ? ny / 64 : 0;
     t.nc = (nx + COL_CHUNK - 1) / COL_CHUNK;
-    t.gy = o;
-    o += al(waves * ty * 64 * 4);
     t.gx = o;
     if (!t.xred) o += al(waves * tx * 64 * 4);
     t.xpart = o;
     if (t.xred && t.nbc > 1) o += al(nx * t.nbc * tx * 4);
     t.part = o;
-    o += al(t.nc * ny * ty * 4);
+    o += al(2 * t.nc * ny * ty * 4);
     t.total = o;
     return t;
 }
@@ -552,29 +632,30 @@ int wbx_dev_dtw_train_backward(const wbx_metric *metric, int mode, const float *
     go.ny = ny;
     go.nbc = ts.nbc;
     go.dX = dX;
-    go.dY = dY;
     go.ldx = ldx;
-    go.ldy = ldy_;
-    go.gyp = (float *)(sc + ts.gy);
     go.gxp = (float *)(sc + ts.gx);
     go.xpart = (float *)(sc + ts.xpart);
+    go.gu = (float *)(sc + ts.gu);
+    go.gv = (float *)(sc + ts.gv);
+    go.lyp = (int)((ty + 7) / 8 * 8);
     float *part = (float *)(sc + ts.part);
-    float *xt = (float *)(sc + ts.xt), *yt = (float *)(sc + ts.yt);
+    float *xt = (float *)(sc + ts.xt);
+    double *pt = (double *)(sc + ts.pt);
     hipLaunchKernelGGL(k_transpose, dim3((unsigned)((tx + 31) / 32), (unsigned)((nx + 31) / 32)),
                        dim3(256), 0, st, X, nx, (int)tx, x_stride, xt);
-    hipLaunchKernelGGL(k_transpose, dim3((unsigned)((ty + 31) / 32), (unsigned)((ny + 31) / 32)),
-                       dim3(256), 0, st, Y, ny, (int)ty, y_stride, yt);
+    hipLaunchKernelGGL(k_prefix_t, dim3((unsigned)((ny + 255) / 256)), dim3(256), 0, st, Y, ny,
+                       (int)ty, y_stride, pt);
     go.XT = xt;
-    go.YT = yt;
-    Series<float> xs{X, x_stride, (int)tx, nullptr}, ys{Y, y_stride, (int)ty, nullptr};
+    go.PT = pt;
     switch (tg.BP) {
 #define BWD_CASE(BPV, UV) \
-    case BPV: train_bwd_one<BPV>(ps, xs, ys, tg.g, sq, G, dirs, go, st); break;
+    case BPV: train_bwd_one<BPV>(ps, tg.g, sq, G, dirs, go, st); break;
         TRAIN_CASES(BWD_CASE)
 #undef BWD_CASE
         default: return fail(WBX_EUNSUPPORTED, "band %d", tg.g.B);
     }
     WBX_HIP_CHECK(hipGetLastError());
+    const unsigned gy_blocks = (unsigned)((ny * ty + 255) / 256);
     if (mode == 0) {
         const unsigned gx_blocks = (unsigned)((nx * tx + 255) / 256);
         if (!ts.xred)
@@ -583,11 +664,13 @@ int wbx_dev_dtw_train_backward(const wbx_metric *metric, int mode, const float *
         else if (ts.nbc > 1)
             hipLaunchKernelGGL(k_sum_xpart, dim3(gx_blocks), dim3(256), 0, st, go.xpart, nx, ts.nbc,
                                (int)tx, dX, ldx);
-        const unsigned gy_blocks = (unsigned)((ny * ty + 255) / 256);
         hipLaunchKernelGGL(k_sum_cols_partial, dim3(gy_blocks, (unsigned)ts.nc), dim3(256), 0, st,
-                           go.gyp, nx, ny, (int)ty, COL_CHUNK, part);
-        hipLaunchKernelGGL(k_sum_cols_final, dim3(gy_blocks), dim3(256), 0, st, part, (int)ts.nc,
-                           ny, (int)ty, dY, ldy_);
+                           go.gu, go.gv, nx, ny, (int)ty, go.lyp, COL_CHUNK, part);
+        hipLaunchKernelGGL(k_sum_cols_final, dim3(gy_blocks), dim3(256), 0, st, part, (int)ts.nc, Y,
+                           y_stride, ny, (int)ty, dY, ldy_);
+    } else {
+        hipLaunchKernelGGL(k_dy_paired, dim3(gy_blocks), dim3(256), 0, st, go.gu, go.gv, ny,
+                           (int)ty, go.lyp, Y, y_stride, dY, ldy_);
     }
     WBX_HIP_CHECK(hipGetLastError());
     return WBX_OK;
