@@ -354,13 +354,14 @@ int subseq_fast_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void 
                      int64_t t, int64_t total, int64_t max_len, double *dist, int64_t *idx,
                      hipStream_t st) {
     const bool euclid = m->kind == WBX_EUCLIDEAN;
-    std::vector<std::vector<int32_t>> cls(9);  // BP/8 = 1..8 fast; 0 = exact rescan
+    // classes by band width B = 2 w - 1 (odd, <= 63): one FP32 scan launch
+    // each; class 0 = bands too wide for the scan (exact rescan)
+    std::vector<std::vector<int32_t>> cls(64);
     for (int64_t j = 0; j < n_s; ++j) {
         int c = 1;
         if (!euclid) {
             const int w = band_w((int)hl[j], (int)hl[j], m->r);
-            const int bp = (2 * w - 1 + 7) / 8 * 8;
-            c = bp <= 64 ? bp / 8 : 0;
+            c = 2 * w - 1 <= 63 ? 2 * w - 1 : 0;
         }
         cls[c].push_back((int32_t)j);
     }
@@ -404,7 +405,7 @@ int subseq_fast_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void 
     SubCand *cands = (SubCand *)(ws + o_cand);
     double *cost = (double *)(ws + o_cost);
     int32_t *ovf = (int32_t *)(ws + o_ovf);
-    for (int c = 1; c <= 8; ++c) {
+    for (int c = 1; c < 64; ++c) {
         const std::vector<int32_t> &v = cls[c];
         if (v.empty()) continue;
         int Lmax = 0;
@@ -429,9 +430,9 @@ int subseq_fast_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void 
         const int64_t nt = n * (int64_t)v.size();
         {
             ProfScope pf(st, KERN_SUBSEQ);
-            RC(launch_subseq_fast(m->kind, c * 8, tm, nt, sg, cands, st));
+            RC(launch_subseq_fast(m->kind, c, tm, nt, sg, cands, st));
         }
-        RC(launch_subseq_exact_cands(m->kind, c * 8, tm, cands, nt, s64.ptr, s_off, s_len, x64.ptr,
+        RC(launch_subseq_exact_cands(m->kind, c, tm, cands, nt, s64.ptr, s_off, s_len, x64.ptr,
                                      x64.stride, (int)t, m->r, cost, st));
         RC(launch_fill_i32(cnt, 1, 0, st));
         RC(launch_subseq_select(tm, cands, nt, cost, n_s, dist, idx, ovf, cnt, st));

@@ -43,9 +43,9 @@ struct SubGeo {
     float rho, A;       // certification: tau(c) = rho (c + A) + A
 };
 
-int launch_subseq_fast(int kind, int BP, TaskMap tasks, int64_t n_tasks, SubGeo sg,
+int launch_subseq_fast(int kind, int B, TaskMap tasks, int64_t n_tasks, SubGeo sg,
                        SubCand *cands, hipStream_t st);
-int launch_subseq_exact_cands(int kind, int BP, TaskMap tasks, const SubCand *cands,
+int launch_subseq_exact_cands(int kind, int B, TaskMap tasks, const SubCand *cands,
                               int64_t n_tasks, const double *S64, const int64_t *soff,
                               const int64_t *slen, const double *X64, int64_t xstride, int T,
                               double r, double *cost64, hipStream_t st);

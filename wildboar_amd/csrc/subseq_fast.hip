@@ -110,13 +110,9 @@ __global__ __launch_bounds__(64) void k_subseq_dtw32(TaskMap tasks, SubGeo sg,
     const SubTask tk = tasks.at(blockIdx.x);
     const float *__restrict__ S = sg.S + sg.off[tk.shapelet];
     const int L = (int)sg.len[tk.shapelet];
-    const int w = band_w(L, L, sg.r);
-    const int off = w - 1, B = 2 * w - 1;
+    const int off = band_w(L, L, sg.r) - 1;  // the band has 2 w - 1 == BP slots
     const int nwin = sg.T - L + 1;
     const float *__restrict__ Xrow = sg.X + (int64_t)tk.sample * sg.xstride;
-    float pen[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) pen[q] = (BP - 8 + q == B) ? WBX_INF_F : -WBX_INF_F;
     SubCand *cout = cands + blockIdx.x;
     __shared__ int s_win[kCap];
     __shared__ float s_cost[kCap];
@@ -146,8 +142,6 @@ __global__ __launch_bounds__(64) void k_subseq_dtw32(TaskMap tasks, SubGeo sg,
                 left = c;
             }
 #pragma unroll
-            for (int q = 1; q < 8; ++q) row[BP - 8 + q] = fmaxf(row[BP - 8 + q], pen[q]);
-#pragma unroll
             for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
             yw[W - 1] = ldy(Y, i + 1 - off + W - 1, L);
         }
@@ -168,8 +162,6 @@ __global__ __launch_bounds__(64) void k_subseq_dtw32(TaskMap tasks, SubGeo sg,
                     row[k] = c;
                     left = c;
                 }
-#pragma unroll
-                for (int q = 1; q < 8; ++q) row[BP - 8 + q] = fmaxf(row[BP - 8 + q], pen[q]);
             }
 #pragma unroll
             for (int m = 0; m < W - U; ++m) yw[m] = yw[m + U];
@@ -188,42 +180,79 @@ __global__ __launch_bounds__(64) void k_subseq_dtw32(TaskMap tasks, SubGeo sg,
     cand_finish(cl, s_win, s_cost, sg.rho * (best + sg.A) + sg.A, best, cout);
 }
 
+// Four consecutive floats of a row of length T starting at e (e >= 0): one
+// 16-byte load when it lies inside the row, else each index clamped to T - 1
+// (those values only feed windows past the last one).
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+__device__ __forceinline__ f4u row4(const float *__restrict__ row, int e, int T) {
+    if (e + 3 < T) return *(const f4u *)(row + e);
+    f4u v;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = row[e + q < T ? e + q : T - 1];
+    return v;
+}
+
+// Lane l owns the U = 4 consecutive windows w0 .. w0 + 3 (w0 = b0 + 4 l) and
+// keeps x[w0 + j .. w0 + j + 8) in registers: four steps of j use static
+// indices (window q at step u reads xw[q + u]), then the window moves by four
+// with one 16-byte load.  Each window's sum runs over j in order.
 template <int U>
 __global__ __launch_bounds__(64) void k_subseq_euclid32(TaskMap tasks, SubGeo sg,
                                                         SubCand *__restrict__ cands) {
+    static_assert(U == 4, "four windows per lane");
     const int lane = threadIdx.x;
     const SubTask tk = tasks.at(blockIdx.x);
     const float *__restrict__ S = sg.S + sg.off[tk.shapelet];
     const int L = (int)sg.len[tk.shapelet];
-    const int nwin = sg.T - L + 1;
+    const int T = sg.T;
+    const int nwin = T - L + 1;
     const float *__restrict__ Xrow = sg.X + (int64_t)tk.sample * sg.xstride;
     SubCand *cout = cands + blockIdx.x;
     __shared__ int s_win[kCap];
     __shared__ float s_cost[kCap];
     CandList cl{0, false};
     float best = WBX_INF_F;
-    // each lane scans U consecutive windows per block of 64 U
     for (int b0 = 0; b0 < nwin; b0 += 64 * U) {
         const int w0 = b0 + lane * U;
         float acc[U];
 #pragma unroll
         for (int q = 0; q < U; ++q) acc[q] = 0.0f;
-        float win[U];
-        const int last = sg.T - 1;
-#pragma unroll
-        for (int q = 0; q < U - 1; ++q) {
-            const int e = w0 + q;
-            win[q + 1] = Xrow[e < last ? e : last];
-        }
-        for (int j = 0; j < L; ++j) {
-            const float s = S[j];
-#pragma unroll
-            for (int q = 0; q < U - 1; ++q) win[q] = win[q + 1];
-            const int e = w0 + j + U - 1;
-            win[U - 1] = Xrow[e < last ? e : last];
+        float xw[2 * U];
+        {
+            const f4u lo = row4(Xrow, w0, T), hi = row4(Xrow, w0 + U, T);
 #pragma unroll
             for (int q = 0; q < U; ++q) {
-                const float d = win[q] - s;
+                xw[q] = lo[q];
+                xw[U + q] = hi[q];
+            }
+        }
+        int j0 = 0;
+        for (; j0 + U <= L; j0 += U) {
+            const f4u nx = row4(Xrow, w0 + j0 + 2 * U, T);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const float sj = S[j0 + u];
+#pragma unroll
+                for (int q = 0; q < U; ++q) {
+                    const float d = xw[q + u] - sj;
+                    acc[q] = __builtin_fmaf(d, d, acc[q]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < U; ++q) {
+                xw[q] = xw[U + q];
+                xw[U + q] = nx[q];
+            }
+        }
+        for (int u = 0; j0 + u < L; ++u) {  // the last L % U steps
+            const float sj = S[j0 + u];
+#pragma unroll
+            for (int q = 0; q < U; ++q) {
+                float xv = xw[q];
+#pragma unroll
+                for (int t = 1; t < U; ++t)
+                    if (u == t) xv = xw[q + t];
+                const float d = xv - sj;
                 acc[q] = __builtin_fmaf(d, d, acc[q]);
             }
         }
@@ -234,9 +263,6 @@ __global__ __launch_bounds__(64) void k_subseq_euclid32(TaskMap tasks, SubGeo sg
             const float c = active ? acc[q] : WBX_INF_F;
             best = fminf(best, wave_min(c));
             const float tau = sg.rho * (best + sg.A) + sg.A;
-            // candidates of window q of every lane: in window order within
-            // this q they are lane-ordered; across q the order is fixed up by
-            // the select pass, which compares (cost, window) pairs.
             cand_push(cl, s_win, s_cost, active && c <= tau, ws, c, tau);
         }
     }
@@ -413,9 +439,16 @@ int launch_subseq_rescan(TaskMap tasks, const int32_t *list, const int *n_list_d
 }
 
 // ------------------------------------------------------------------ launch
+// FP32 scans are compiled for every odd band width B = 2 w - 1 <= 63 (equal
+// lengths), so no slot beyond the band is computed.
+#define SUB_SCAN_CASES(X)                                                                        \
+    X(1, 8) X(3, 8) X(5, 8) X(7, 8) X(9, 8) X(11, 8) X(13, 8) X(15, 8) X(17, 8) X(19, 8) X(21, 8) \
+        X(23, 8) X(25, 8) X(27, 8) X(29, 8) X(31, 8) X(33, 4) X(35, 4) X(37, 4) X(39, 4)        \
+            X(41, 4) X(43, 4) X(45, 4) X(47, 4) X(49, 4) X(51, 4) X(53, 4) X(55, 4) X(57, 4)     \
+                X(59, 4) X(61, 4) X(63, 4)
 #define SUB_DTW_CASES(X) X(8, 8) X(16, 8) X(24, 8) X(32, 8) X(40, 8) X(48, 8) X(56, 8) X(64, 8)
 
-int launch_subseq_fast(int kind, int BP, TaskMap tasks, int64_t n_tasks, SubGeo sg,
+int launch_subseq_fast(int kind, int B, TaskMap tasks, int64_t n_tasks, SubGeo sg,
                        SubCand *cands, hipStream_t st) {
     if (n_tasks <= 0) return WBX_OK;
     if (n_tasks > 0x7fffffff) return fail(WBX_EUNSUPPORTED, "too many subsequence tasks");
@@ -423,26 +456,26 @@ int launch_subseq_fast(int kind, int BP, TaskMap tasks, int64_t n_tasks, SubGeo 
     if (kind == WBX_EUCLIDEAN) {
         hipLaunchKernelGGL(k_subseq_euclid32<4>, grid, dim3(64), 0, st, tasks, sg, cands);
     } else {
-        switch (BP) {
+        switch (B) {
 #define SD_CASE(B_, U_) \
     case B_: hipLaunchKernelGGL((k_subseq_dtw32<B_, U_>), grid, dim3(64), 0, st, tasks, sg, cands); break;
-            SUB_DTW_CASES(SD_CASE)
+            SUB_SCAN_CASES(SD_CASE)
 #undef SD_CASE
-            default: return fail(WBX_EUNSUPPORTED, "subsequence band %d", BP);
+            default: return fail(WBX_EUNSUPPORTED, "subsequence band %d", B);
         }
     }
     WBX_HIP_CHECK(hipGetLastError());
     return WBX_OK;
 }
 
-int launch_subseq_exact_cands(int kind, int BP, TaskMap tasks, const SubCand *cands,
+int launch_subseq_exact_cands(int kind, int B, TaskMap tasks, const SubCand *cands,
                               int64_t n_tasks, const double *S64, const int64_t *soff,
                               const int64_t *slen, const double *X64, int64_t xstride, int T,
                               double r, double *cost64, hipStream_t st) {
     if (n_tasks <= 0) return WBX_OK;
     const int64_t nl = n_tasks * kMaxCand;
     const dim3 grid((unsigned)((nl + 63) / 64));
-    switch (kind == WBX_EUCLIDEAN ? 8 : BP) {
+    switch (kind == WBX_EUCLIDEAN ? 8 : (B + 7) / 8 * 8) {
 #define C_CASE(B_, U_)                                                                            \
     case B_:                                                                                      \
         hipLaunchKernelGGL(k_subseq_cand64<B_>, grid, dim3(64), 0, st, tasks, cands, n_tasks, S64, \
@@ -450,7 +483,7 @@ int launch_subseq_exact_cands(int kind, int BP, TaskMap tasks, const SubCand *ca
         break;
         SUB_DTW_CASES(C_CASE)
 #undef C_CASE
-        default: return fail(WBX_EUNSUPPORTED, "subsequence band %d", BP);
+        default: return fail(WBX_EUNSUPPORTED, "subsequence band %d", B);
     }
     WBX_HIP_CHECK(hipGetLastError());
     return WBX_OK;
