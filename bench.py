@@ -219,11 +219,12 @@ def run_subsequence(torch, dev, n_series):
     start = np.array([rng.integers(0, 1024 - l + 1) for l in L])
     shp64 = [Xall[s, a:a + l].copy() for s, a, l in zip(src, start, L)]
     X = torch.from_numpy(Xall[:n_series].astype(np.float32)).to(dev)
-    shp = [torch.from_numpy(s.astype(np.float32)).to(dev) for s in shp64]
+    # shapelets resident on the device, packed once (as X is)
+    shp = device.pack_shapelets([torch.from_numpy(s.astype(np.float32)) for s in shp64], dev)
     res = {"workload": f"4096 shapelets, L U[16,128], from 10000 random walks of length 1024; "
                        f"scan over the first {n_series} series, float32"}
     for metric in ("dtw", "euclidean"):
-        device.subsequence(shp[:64], X[:64], metric=metric, r=0.1, precision="fp32")
+        device.subsequence(shp, X[:8], metric=metric, r=0.1, precision="fp32")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         dist, idx = device.subsequence(shp, X, metric=metric, r=0.1, precision="fp32")

@@ -243,3 +243,23 @@ def test_fp32_acc64_is_fp64_on_rounded_inputs(metric, params):
     od, oi = oracle.subsequence([f32(s) for s in shp], f32(X), metric=metric, r=r, g=g)
     np.testing.assert_array_equal(i, oi)
     np.testing.assert_array_equal(d, od)
+
+
+def test_device_subsequence_packed_matches_list_and_oracle():
+    import torch
+    from wildboar_amd import device
+
+    X = rw(50, 12, 200)
+    rng = np.random.default_rng(51)
+    shp = [X[int(rng.integers(12)), s:s + L].copy() for s, L in [(3, 20), (50, 33), (100, 7), (9, 64)]]
+    Xd = torch.from_numpy(X.astype(np.float32)).cuda()
+    sd = [torch.from_numpy(s.astype(np.float32)) for s in shp]
+    packed = device.pack_shapelets(sd, Xd.device)
+    for metric in ("dtw", "euclidean"):
+        d1, i1 = device.subsequence(sd, Xd, metric=metric, r=0.1)
+        d2, i2 = device.subsequence(packed, Xd, metric=metric, r=0.1)
+        assert torch.equal(d1, d2) and torch.equal(i1, i2)
+        od, oi = oracle.subsequence([s.astype(np.float32).astype(np.float64) for s in shp],
+                                    X.astype(np.float32).astype(np.float64), metric=metric, r=0.1)
+        np.testing.assert_array_equal(i2.cpu().numpy(), oi)
+        np.testing.assert_array_equal(d2.cpu().numpy(), od)

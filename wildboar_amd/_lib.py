@@ -61,6 +61,13 @@ def load():
             raise WbxError(
                 f"{LIB_PATH} is not built; build it with wildboar_amd._lib.build() "
                 "(hipcc --offload-arch=gfx950)")
+        # libwbx.so and torch both need libamdhip64.so.7; whichever loads
+        # first serves both.  Importing torch first keeps one HIP runtime, the
+        # one torch was built with, in the process.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(LIB_PATH)
         p, i64, i32, d = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
         m = ctypes.POINTER(WbxMetric)

@@ -365,8 +365,16 @@ int subseq_fast_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void 
         }
         cls[c].push_back((int32_t)j);
     }
+    // all class lists back to back (one upload); class c's tasks are n * size
     int64_t max_tasks = 0;
-    for (auto &v : cls) max_tasks = std::max<int64_t>(max_tasks, n * (int64_t)v.size());
+    std::vector<int32_t> lists;
+    std::vector<int64_t> cbeg(65, 0);
+    for (int c = 0; c < 64; ++c) {
+        cbeg[c] = (int64_t)lists.size();
+        lists.insert(lists.end(), cls[c].begin(), cls[c].end());
+        max_tasks = std::max<int64_t>(max_tasks, n * (int64_t)cls[c].size());
+    }
+    cbeg[64] = (int64_t)lists.size();
     Carve cv;
     const size_t o_x32 = cv.take(stage_bytes(dtype, false, n, t));
     const size_t o_s32 = cv.take(stage_bytes(dtype, false, 1, total));
@@ -374,9 +382,9 @@ int subseq_fast_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void 
     const size_t o_s64 = cv.take(stage_bytes(dtype, true, 1, total));
     const size_t o_cand = cv.take((size_t)max_tasks * sizeof(SubCand));
     const size_t o_cost = cv.take((size_t)max_tasks * kMaxCand * 8);
-    const size_t o_ovf = cv.take((size_t)max_tasks * 4);
+    const size_t o_ovf = cv.take((size_t)n * (size_t)n_s * 4);
     const size_t o_list = cv.take((size_t)n_s * 4);
-    const size_t o_cnt = cv.take(64);
+    const size_t o_cnt = cv.take(80 * 4);  // 64 class counters, then the |x| max
     void *wsp = nullptr;
     RC(ws_reserve(ctx, cv.total, &wsp));
     char *ws = (char *)wsp;
@@ -386,12 +394,12 @@ int subseq_fast_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void 
     RC(stage<float>(dtype, S, 1, total, total, ws, o_s32, &s32, st));
     RC(stage<double>(dtype, X, n, t, xstr, ws, o_x64, &x64, st));
     RC(stage<double>(dtype, S, 1, total, total, ws, o_s64, &s64, st));
+    int *cnt = (int *)(ws + o_cnt);
+    RC(launch_fill_i32(cnt, 80, 0, st));
     double absmax = 0.0;
     const bool exact_inputs = dtype == WBX_F32;
-    int *cnt = (int *)(ws + o_cnt);
     if (!exact_inputs) {
-        unsigned *mx = (unsigned *)(cnt + 4);
-        RC(launch_fill_i32(cnt, 16, 0, st));
+        unsigned *mx = (unsigned *)(cnt + 64);
         RC(launch_absmax_rows<double>((const double *)X, n, t, xstr, mx, st));
         RC(launch_absmax_rows<double>((const double *)S, 1, total, total, mx, st));
         unsigned hb = 0;
@@ -405,6 +413,11 @@ int subseq_fast_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void 
     SubCand *cands = (SubCand *)(ws + o_cand);
     double *cost = (double *)(ws + o_cost);
     int32_t *ovf = (int32_t *)(ws + o_ovf);
+    WBX_HIP_CHECK(hipMemcpyAsync(list, lists.data(), sizeof(int32_t) * lists.size(),
+                                 hipMemcpyHostToDevice, st));
+    // FP32 scan, FP64 finish of the candidates and selection for every class;
+    // tasks whose candidates overflowed are listed per class and rescanned
+    // exactly after one read-back of the class counters.
     for (int c = 1; c < 64; ++c) {
         const std::vector<int32_t> &v = cls[c];
         if (v.empty()) continue;
@@ -424,9 +437,7 @@ int subseq_fast_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void 
         sg.r = m->r;
         sg.rho = (float)(eb.e_up / eb.e_dn * (1.0 + 3.0 * e64) * (1.0 + 1e-6));
         sg.A = (float)(eb.A * (1.0 + 1e-6));
-        WBX_HIP_CHECK(hipMemcpyAsync(list, v.data(), sizeof(int32_t) * v.size(),
-                                     hipMemcpyHostToDevice, st));
-        TaskMap tm{list, (int32_t)v.size()};
+        TaskMap tm{list + cbeg[c], (int32_t)v.size()};
         const int64_t nt = n * (int64_t)v.size();
         {
             ProfScope pf(st, KERN_SUBSEQ);
@@ -434,30 +445,33 @@ int subseq_fast_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void 
         }
         RC(launch_subseq_exact_cands(m->kind, c, tm, cands, nt, s64.ptr, s_off, s_len, x64.ptr,
                                      x64.stride, (int)t, m->r, cost, st));
-        RC(launch_fill_i32(cnt, 1, 0, st));
-        RC(launch_subseq_select(tm, cands, nt, cost, n_s, dist, idx, ovf, cnt, st));
-        int novf = 0;
-        WBX_HIP_CHECK(hipMemcpyAsync(&novf, cnt, 4, hipMemcpyDeviceToHost, st));
-        WBX_HIP_CHECK(hipStreamSynchronize(st));
-        RC(launch_subseq_rescan(tm, ovf, cnt, novf, m->kind, s64.ptr, s_off, s_len, x64.ptr,
-                                x64.stride, (int)t, m->r, (int)max_len, n_s, dist, idx, st));
+        RC(launch_subseq_select(tm, cands, nt, cost, n_s, dist, idx, ovf + n * cbeg[c], cnt + c,
+                                st));
+    }
+    int novf[64];
+    WBX_HIP_CHECK(hipMemcpyAsync(novf, cnt, sizeof novf, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));
+    for (int c = 1; c < 64; ++c) {
+        if (cls[c].empty() || novf[c] == 0) continue;
+        TaskMap tm{list + cbeg[c], (int32_t)cls[c].size()};
+        RC(launch_subseq_rescan(tm, ovf + n * cbeg[c], cnt + c, novf[c], m->kind, s64.ptr, s_off,
+                                s_len, x64.ptr, x64.stride, (int)t, m->r, (int)max_len, n_s, dist,
+                                idx, st));
     }
     if (!cls[0].empty()) {  // bands too wide for the FP32 kernels: exact rescan
         const std::vector<int32_t> &v = cls[0];
         const int64_t nt = n * (int64_t)v.size();
         std::vector<int32_t> all((size_t)nt);
         for (int64_t q = 0; q < nt; ++q) all[q] = (int32_t)q;
-        WBX_HIP_CHECK(hipMemcpyAsync(list, v.data(), sizeof(int32_t) * v.size(),
-                                     hipMemcpyHostToDevice, st));
         WBX_HIP_CHECK(hipMemcpyAsync(ovf, all.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice,
                                      st));
         const int ntv = (int)nt;
         WBX_HIP_CHECK(hipMemcpyAsync(cnt, &ntv, 4, hipMemcpyHostToDevice, st));
-        TaskMap tm{list, (int32_t)v.size()};
+        TaskMap tm{list + cbeg[0], (int32_t)v.size()};
         RC(launch_subseq_rescan(tm, ovf, cnt, ntv, m->kind, s64.ptr, s_off, s_len, x64.ptr,
                                 x64.stride, (int)t, m->r, (int)max_len, n_s, dist, idx, st));
-        WBX_HIP_CHECK(hipStreamSynchronize(st));
     }
+    WBX_HIP_CHECK(hipStreamSynchronize(st));
     return WBX_OK;
 }
 

@@ -99,19 +99,34 @@ def argmin(X, Y, k=1, *, metric="dtw", r=1.0, g=0.05, precision="fp32", lower_bo
     return idx, dist
 
 
-def subsequence(shapelets, X, *, metric="dtw", r=1.0, g=0.05, precision="fp32"):
-    """(dist (n, n_shapelets) float64, idx (n, n_shapelets) int64)."""
-    X = _rows(X, "X")
-    if isinstance(shapelets, (list, tuple)):
-        lens = torch.tensor([int(s.shape[0]) for s in shapelets], dtype=torch.int64)
-        packed = torch.cat([s.reshape(-1).to(device=X.device, dtype=X.dtype) for s in shapelets])
-    else:
-        raise TypeError("shapelets must be a list of 1-D tensors")
+def pack_shapelets(shapelets, device=None, dtype=None):
+    """Pack a list of 1-D shapelet tensors into (values, offsets, lengths) on
+    the device: the layout the subsequence entry point reads.  Packing once
+    lets repeated scans skip the per-shapelet copies."""
+    if not isinstance(shapelets, (list, tuple)) or not shapelets:
+        raise TypeError("shapelets must be a non-empty list of 1-D tensors")
+    device = device or shapelets[0].device
+    dtype = dtype or shapelets[0].dtype
+    lens = torch.tensor([int(s.shape[0]) for s in shapelets], dtype=torch.int64)
+    packed = torch.cat([s.reshape(-1).to(device=device, dtype=dtype) for s in shapelets])
     offs = torch.zeros_like(lens)
     offs[1:] = torch.cumsum(lens, 0)[:-1]
-    lens = lens.to(X.device)
-    offs = offs.to(X.device)
-    n, ns = X.shape[0], len(shapelets)
+    return packed, offs.to(device), lens.to(device)
+
+
+def subsequence(shapelets, X, *, metric="dtw", r=1.0, g=0.05, precision="fp32"):
+    """(dist (n, n_shapelets) float64, idx (n, n_shapelets) int64).
+
+    shapelets: a list of 1-D tensors, or the (values, offsets, lengths)
+    triple of pack_shapelets."""
+    X = _rows(X, "X")
+    if isinstance(shapelets, tuple) and len(shapelets) == 3 and torch.is_tensor(shapelets[0]):
+        packed, offs, lens = shapelets
+        if packed.dtype != X.dtype or packed.device != X.device:
+            raise TypeError("packed shapelets must have the dtype and device of X")
+    else:
+        packed, offs, lens = pack_shapelets(shapelets, X.device, X.dtype)
+    n, ns = X.shape[0], int(lens.shape[0])
     dist = torch.empty((n, ns), dtype=torch.float64, device=X.device)
     idx = torch.empty((n, ns), dtype=torch.int64, device=X.device)
     m = _metric(metric, r, g, precision)
