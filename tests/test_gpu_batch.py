@@ -137,6 +137,40 @@ def test_batch_graph_replays_match_eager():
         assert torch.isfinite(Zd) and torch.equal(again, eager)
 
 
+def test_batch_launch_shape_and_autotune_keep_results():
+    # the launch shape of the strip buckets (set by hand or by the autotuner)
+    # changes timing only
+    X, Y, lx, ly, r = make_pairs(18, 400, 20, 600, 600, spikes=0.05)
+    Xd = torch.from_numpy(X.astype(np.float32)).cuda()
+    Yd = torch.from_numpy(Y.astype(np.float32)).cuda()
+    plan = DtwBatch(lx, ly, 600, r, precision="fp32")
+    ref = plan.run(Xd, Yd, graph=False).clone()
+    for w, ns in ((1, 1), (2, 4), (12, 2), (32, 3)):
+        plan.set_launch(w, ns)
+        assert torch.equal(plan.run(Xd, Yd, graph=False), ref), (w, ns)
+    with pytest.raises(ValueError):
+        plan.set_launch(0, 4)
+    with pytest.raises(ValueError):
+        plan.set_launch(4, 5)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        ms = plan.autotune(Xd, Yd, reps=1)
+        assert ms is not None and ms > 0
+        got = plan.buckets()
+        assert len(got) >= 2 and sum(bk["pairs"] for bk in got) == plan.n_strip
+        assert all(bk["ms"] is not None and bk["ms"] > 0 and bk["waves"] >= 1 for bk in got)
+        other = DtwBatch(lx, ly, 600, r, precision="fp32")
+        assert other.autotune(Xd, Yd) is None  # same bucket signature: taken from the cache
+        assert [bk["waves"] for bk in other.buckets()] == [bk["waves"] for bk in got]
+        out = plan.run(Xd, Yd, graph=True)
+        out64 = plan.run(Xd.double(), Yd.double(), graph=False)  # float64 inputs, FP32 path
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref) and torch.equal(out64, ref)
+    np.testing.assert_allclose(ref.cpu().numpy(), oracle_batch(X.astype(np.float32).astype(np.float64),
+                               Y.astype(np.float32).astype(np.float64), lx, ly, r), rtol=FP32_RTOL,
+                               atol=1e-6)
+
+
 def test_batch_rejects_bad_lengths():
     with pytest.raises(ValueError):
         DtwBatch([0, 3], [3, 3], 4, 0.1)

@@ -27,7 +27,7 @@ EXPORTS = (
     "wbx_dev_pairwise", "wbx_dev_paired", "wbx_dev_argmin",
     "wbx_dev_subsequence_pairwise", "wbx_set_policy", "wbx_prof_enable", "wbx_prof_take",
     "wbx_prof_valu_peak", "wbx_dev_eos_lengths", "wbx_batch_create", "wbx_batch_free",
-    "wbx_batch_info", "wbx_batch_run", "wbx_dtw_train_sizes", "wbx_dev_dtw_train_forward",
+    "wbx_batch_set_launch", "wbx_batch_autotune", "wbx_batch_bucket_info", "wbx_batch_info", "wbx_batch_run", "wbx_dtw_train_sizes", "wbx_dev_dtw_train_forward",
     "wbx_dev_dtw_train_backward",
 )
 
@@ -83,6 +83,9 @@ def load():
             "wbx_dev_eos_lengths": (i32, [i32, p, i64, i64, i64, p, p]),
             "wbx_batch_create": (i32, [m, i64, i64, p, p, p, i32, p]),
             "wbx_batch_free": (i32, [p]),
+            "wbx_batch_set_launch": (i32, [p, i32, i32]),
+            "wbx_batch_autotune": (i32, [p, i32, p, i64, p, i64, p, i32, p]),
+            "wbx_batch_bucket_info": (i32, [p, i32, p, p, p, p, p, p]),
             "wbx_batch_info": (i32, [p, p, p, p]),
             "wbx_batch_run": (i32, [p, i32, p, i64, p, i64, p, p, i32]),
             "wbx_dtw_train_sizes": (i32, [m, i32, i64, i64, i64, i64, p, p]),

@@ -83,8 +83,40 @@ class DtwBatch:
         self.n_strip = ns.value
         self.n_plain = npl.value
 
-    def run(self, X, Y, out=None, *, graph=True):
-        """Enqueue on torch's current stream; returns the float64 distances."""
+    def set_launch(self, waves_per_cu=8, n_streams=4):
+        """Launch shape of the strip buckets (results do not depend on it)."""
+        _lib.check(self._lib.wbx_batch_set_launch(self._plan, int(waves_per_cu), int(n_streams)))
+
+    def autotune(self, X, Y, reps=2):
+        """Time each strip bucket's launch shapes on (X, Y) and keep the fastest.
+
+        Returns the total of the buckets' best times in ms (None when the
+        choice came from the process-wide cache, or there is nothing to tune).
+        """
+        self._check_inputs(X, Y)
+        ms = ctypes.c_float()
+        dtype = _lib.F32 if X.dtype == torch.float32 else _lib.F64
+        _lib.check(self._lib.wbx_batch_autotune(
+            self._plan, dtype, X.data_ptr(), X.stride(0), Y.data_ptr(), Y.stride(0),
+            torch.cuda.current_stream().cuda_stream, int(reps), ctypes.byref(ms)))
+        return None if ms.value < 0 else float(ms.value)
+
+    def buckets(self):
+        """Per strip bucket: dict(R, pairs, waves, stream, ms)."""
+        nb = ctypes.c_int()
+        _lib.check(self._lib.wbx_batch_bucket_info(self._plan, -1, ctypes.byref(nb), None, None,
+                                                   None, None, None))
+        out = []
+        for k in range(nb.value):
+            v = [ctypes.c_int() for _ in range(4)]
+            ms = ctypes.c_float()
+            _lib.check(self._lib.wbx_batch_bucket_info(self._plan, k, None, *[ctypes.byref(x) for x in v],
+                                                       ctypes.byref(ms)))
+            out.append({"R": v[0].value, "pairs": v[1].value, "waves": v[2].value,
+                        "stream": v[3].value, "ms": None if ms.value < 0 else float(ms.value)})
+        return out
+
+    def _check_inputs(self, X, Y):
         for t, name in ((X, "X"), (Y, "Y")):
             if not t.is_cuda or t.dim() != 2 or t.stride(1) != 1:
                 raise ValueError(f"{name} must be a 2-D device tensor with contiguous rows")
@@ -92,6 +124,10 @@ class DtwBatch:
                 raise ValueError(f"{name} is smaller than the plan ({self.n_pairs}, {self.t_pad})")
         if X.dtype != Y.dtype or X.dtype not in (torch.float32, torch.float64):
             raise TypeError("X and Y must both be float32 or float64")
+
+    def run(self, X, Y, out=None, *, graph=True):
+        """Enqueue on torch's current stream; returns the float64 distances."""
+        self._check_inputs(X, Y)
         if out is None:
             out = torch.empty(self.n_pairs, dtype=torch.float64, device=X.device)
         dtype = _lib.F32 if X.dtype == torch.float32 else _lib.F64
