@@ -194,17 +194,27 @@ struct GradOut {
     const double *PT;
 };
 
-// PT[j * ny + b] = sum of y_b[0 .. j), one thread per series.
-__global__ __launch_bounds__(256) void k_prefix_t(const float *__restrict__ Y, int64_t ny, int ly,
-                                                  int64_t stride, double *__restrict__ PT) {
-    const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (b >= ny) return;
+// PT[j * ny + b] = sum of y_b[0 .. j) in float64; one wave per series, 64
+// columns at a time with a shuffle scan (the sum order is fixed: the result
+// is deterministic).
+__global__ __launch_bounds__(64) void k_prefix_t(const float *__restrict__ Y, int64_t ny, int ly,
+                                                 int64_t stride, double *__restrict__ PT) {
+    const int64_t b = blockIdx.x;
+    const int lane = threadIdx.x;
     const float *y = Y + b * stride;
-    double acc = 0.0;
-    PT[b] = 0.0;
-    for (int j = 0; j < ly; ++j) {
-        acc += (double)y[j];
-        PT[(int64_t)(j + 1) * ny + b] = acc;
+    if (lane == 0) PT[b] = 0.0;
+    double carry = 0.0;
+    for (int j0 = 0; j0 < ly; j0 += 64) {
+        const int j = j0 + lane;
+        double s = j < ly ? (double)y[j] : 0.0;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const double o = __shfl_up(s, d);
+            if (lane >= d) s += o;
+        }
+        s += carry;
+        if (j < ly) PT[(int64_t)(j + 1) * ny + b] = s;
+        carry = __shfl(s, 63);
     }
 }
 
@@ -643,8 +653,8 @@ int wbx_dev_dtw_train_backward(const wbx_metric *metric, int mode, const float *
     double *pt = (double *)(sc + ts.pt);
     hipLaunchKernelGGL(k_transpose, dim3((unsigned)((tx + 31) / 32), (unsigned)((nx + 31) / 32)),
                        dim3(256), 0, st, X, nx, (int)tx, x_stride, xt);
-    hipLaunchKernelGGL(k_prefix_t, dim3((unsigned)((ny + 255) / 256)), dim3(256), 0, st, Y, ny,
-                       (int)ty, y_stride, pt);
+    hipLaunchKernelGGL(k_prefix_t, dim3((unsigned)ny), dim3(64), 0, st, Y, ny, (int)ty, y_stride,
+                       pt);
     go.XT = xt;
     go.PT = pt;
     switch (tg.BP) {
