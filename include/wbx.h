@@ -120,6 +120,28 @@ int wbx_subsequence_pairwise(const wbx_metric *metric, const double *S,
                              int64_t dims, int64_t t, int64_t dim, double *dist,
                              int64_t *idx);
 
+/* Replaces _argmin_subsequence_distance (distance/_cdistance.pyx:1572-1594,
+ * non-scaled _ArgminSubsequenceDistance :1408-1434): pattern i (s_len[i]
+ * values at S + s_off[i]) against series i of X (n series), the k best
+ * windows in the reference's heap order (Heap, utils/_misc.pyx:18-107).  idx
+ * and dist are (n, k).  k <= t - longest pattern + 1. */
+int wbx_argmin_subsequence(const wbx_metric *metric, const double *S,
+                           const int64_t *s_off, const int64_t *s_len,
+                           int64_t n, const double *X, int64_t dims, int64_t t,
+                           int64_t dim, int64_t k, int64_t *idx, double *dist);
+
+/* The per-series window costs behind _subsequence_match and
+ * _paired_subsequence_match (distance/_cdistance.pyx:1065-1141, windows with
+ * cost <= threshold^2 as dtw_subsequence_matches, _elastic.pyx:658-698, and
+ * euclidean_distance_matches, _metric.pyx:762-795): prof (n, nwp), nwp = t -
+ * shortest pattern + 1, holds the squared cost of window s of series i
+ * against pattern (paired ? i : 0), +inf past the pattern's last window. */
+int wbx_subsequence_profile(const wbx_metric *metric, const double *S,
+                            const int64_t *s_off, const int64_t *s_len,
+                            int64_t n_s, int paired, const double *X, int64_t n,
+                            int64_t dims, int64_t t, int64_t dim, int64_t nwp,
+                            double *prof);
+
 /* ------------------------------------------- device-pointer entry points
  * Inputs are device pointers of element type `dtype` (WBX_F64 / WBX_F32);
  * series i of X starts at X + i * x_stride.  Outputs are device memory.
@@ -155,6 +177,23 @@ int wbx_dev_subsequence_pairwise(const wbx_metric *metric, int dtype,
                                  const void *X, int64_t n, int64_t x_stride,
                                  int64_t t, double *dist, int64_t *idx,
                                  void *stream);
+
+/* Device forms of wbx_argmin_subsequence (n patterns, one per series;
+ * idx / dist device (n, k)) and wbx_subsequence_profile (prof device
+ * (n, nwp)); S is packed patterns of element type dtype, s_off / s_len are
+ * device int64 arrays.  Both synchronise `stream` (pattern lengths are read
+ * back). */
+int wbx_dev_argmin_subsequence(const wbx_metric *metric, int dtype,
+                               const void *S, const int64_t *s_off,
+                               const int64_t *s_len, const void *X, int64_t n,
+                               int64_t x_stride, int64_t t, int64_t k,
+                               int64_t *idx, double *dist, void *stream);
+int wbx_dev_subsequence_profile(const wbx_metric *metric, int dtype,
+                                const void *S, const int64_t *s_off,
+                                const int64_t *s_len, int64_t n_s, int paired,
+                                const void *X, int64_t n, int64_t x_stride,
+                                int64_t t, int64_t nwp, double *prof,
+                                void *stream);
 
 /* ------------------------------------------- batched uneven pairs
  * No single reference driver: this is the batch form of DtwMetric._distance

@@ -14,6 +14,11 @@ the reference's driver semantics:
 - ``paired``     : ``_paired_distance``     (``:1632-1652``, D(y[i], x[i]))
 - ``argmin``     : ``_argmin_distance``     (``:1348-1378``, heap order)
 - ``subsequence``: ``_pairwise_subsequence_distance`` (``:1018-1037``)
+- ``argmin_subsequence``: ``_ArgminSubsequenceDistance`` (``:1408-1434``,
+  heap order)
+- ``subsequence_matches``: the per-series window matches behind
+  ``_subsequence_match`` (``:1065-1100``; ``_elastic.pyx:658-698``,
+  ``_metric.pyx:762-795``)
 - ``dtw_grad``   : path-backtrack gradient (``_elastic.pyx:1011-1073`` +
   ``dtw.py:395-423``)
 """
@@ -75,6 +80,10 @@ def _load():
     lib.orc_dtw_mapping.argtypes = [p, i64, i64, p, p]
     lib.orc_dtw_grad.restype = d
     lib.orc_dtw_grad.argtypes = [p, i64, p, i64, d, p, p, p]
+    lib.orc_argmin_subsequence.restype = i32
+    lib.orc_argmin_subsequence.argtypes = [i32, d, d, p, p, p, i64, p, i64, i64, i64, i64, p, p]
+    lib.orc_subsequence_matches.restype = i64
+    lib.orc_subsequence_matches.argtypes = [i32, d, d, p, i64, p, i64, d, p, p]
     _lib = lib
     return lib
 
@@ -221,6 +230,38 @@ def subsequence(shapelets, X, metric="dtw", r=1.0, g=0.05, dim=0, n_threads=None
     if rc != 0:
         raise MemoryError("oracle allocation failed")
     return dist, idx
+
+
+def argmin_subsequence(shapelets, X, k=1, metric="dtw", r=1.0, g=0.05, dim=0):
+    """(idx, dist), each (n_samples, k) in heap order: shapelet i against
+    series i."""
+    X = _ts(X)
+    S, offs, lens = pack_shapelets(shapelets)
+    n = X.shape[0]
+    idx = np.zeros((n, k), dtype=np.int64)
+    dist = np.zeros((n, k))
+    rc = _load().orc_argmin_subsequence(
+        _kind(metric), r, g, S.ctypes.data, offs.ctypes.data, lens.ctypes.data, n,
+        X.ctypes.data, X.shape[1], X.shape[2], dim, k, idx.ctypes.data, dist.ctypes.data)
+    if rc != 0:
+        raise MemoryError("oracle allocation failed")
+    return idx, dist
+
+
+def subsequence_matches(s, x, threshold, metric="dtw", r=1.0, g=0.05):
+    """(indices, distances) of the windows of the 1-D series x whose squared
+    cost against s is <= threshold ** 2, in window order."""
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    nw = x.shape[0] - s.shape[0] + 1
+    idx = np.zeros(max(nw, 1), dtype=np.int64)
+    dist = np.zeros(max(nw, 1))
+    m = _load().orc_subsequence_matches(_kind(metric), r, g, s.ctypes.data, s.shape[0],
+                                        x.ctypes.data, x.shape[0], float(threshold),
+                                        dist.ctypes.data, idx.ctypes.data)
+    if m < 0:
+        raise MemoryError("oracle allocation failed")
+    return idx[:m].copy(), dist[:m].copy()
 
 
 def cost_matrix(x, y, band, weights=None):

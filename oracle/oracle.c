@@ -600,3 +600,96 @@ double orc_dtw_grad(const double *X, int64_t lx, const double *Y, int64_t ly,
     free(A); free(pi); free(pj);
     return D;
 }
+
+/* ------------------------------------------------------------------------
+ * _ArgminSubsequenceDistance.__call__, distance/_cdistance.pyx:1408-1434
+ * (non-scaled): shapelet i (s_len[i] values at s_off[i]) against series i;
+ * windows j in order go through Metric._eadistance against the heap's
+ * current threshold (inf until the heap holds k entries, then its largest
+ * value) and into Heap.push; the result is the heap's array order.
+ * Metric.reset(t, s) sizes the scratch and the WDTW weights by
+ * max(t, longest shapelet) = t.
+ * ---------------------------------------------------------------------- */
+int orc_argmin_subsequence(int kind, double r, double g, const double *S,
+                           const int64_t *s_off, const int64_t *s_len, int64_t n,
+                           const double *X, int64_t dims, int64_t t, int64_t dim,
+                           int64_t k, int64_t *idx, double *dist) {
+    orc_metric m = {kind, r, g, NULL, NULL, NULL};
+    if (metric_reset(&m, t, t) != 0) { metric_free(&m); return -1; }
+    orc_heap_el *heap = (orc_heap_el *)malloc(sizeof(orc_heap_el) * (size_t)k);
+    if (!heap) { metric_free(&m); return -1; }
+    for (int64_t i = 0; i < n; i++) {
+        const double *s = S + s_off[i];
+        const int64_t L = s_len[i];
+        const double *x = ROW(X, i, dims, t, dim);
+        double threshold = INFINITY;
+        int64_t held = 0;
+        for (int64_t j = 0; j + L <= t; j++) {
+            if (metric_eadistance(&m, s, L, x + j, L, &threshold))
+                heap_push(heap, &held, k, j, threshold);
+            threshold = held == k ? heap[0].value : INFINITY;
+        }
+        for (int64_t q = 0; q < k; q++) {
+            idx[i * k + q] = q < held ? heap[q].index : 0;
+            dist[i * k + q] = q < held ? heap[q].value : 0.0;
+        }
+    }
+    free(heap);
+    metric_free(&m);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * Windows of T matching S: dtw_subsequence_matches (_elastic.pyx:658-698,
+ * band _compute_r(L, r), _elastic.pyx:2278-2290) and
+ * euclidean_distance_matches (_metric.pyx:762-795).  A window matches when
+ * its squared cost is <= threshold^2; matches are listed in window order
+ * with distance sqrt(cost).  WDTW weights are sized by the series length t
+ * (WeightedDtwSubsequenceMetric.reset, _elastic.pyx:2368-2378).  Returns the
+ * number of matches, -1 on allocation failure.
+ * ---------------------------------------------------------------------- */
+int64_t orc_subsequence_matches(int kind, double r, double g, const double *S, int64_t L,
+                                const double *T, int64_t t, double threshold,
+                                double *distances, int64_t *matches) {
+    const double thr = threshold * threshold;
+    int64_t n_matches = 0;
+    if (kind == ORC_EUCLIDEAN) {
+        for (int64_t i = 0; i + L <= t; i++) {
+            double dist = 0, x;
+            for (int64_t j = 0; j < L; j++) {
+                if (dist > thr) break;
+                x = T[i + j];
+                x -= S[j];
+                dist += x * x;
+            }
+            if (dist <= thr) {
+                matches[n_matches] = i;
+                distances[n_matches] = sqrt(dist);
+                n_matches++;
+            }
+        }
+        return n_matches;
+    }
+    double *cost = (double *)malloc(sizeof(double) * (size_t)(t + 1));
+    double *cost_prev = (double *)malloc(sizeof(double) * (size_t)(t + 1));
+    double *weights = NULL;
+    if (kind == ORC_WDTW) {
+        weights = (double *)malloc(sizeof(double) * (size_t)t);
+        if (weights) orc_wdtw_weights(t, g, weights);
+    }
+    if (!cost || !cost_prev || (kind == ORC_WDTW && !weights)) {
+        free(cost); free(cost_prev); free(weights);
+        return -1;
+    }
+    const int64_t band = orc_compute_r(L, r);
+    for (int64_t i = 0; i + L <= t; i++) {
+        double dist = orc_dtw_distance(S, L, T + i, L, band, cost, cost_prev, weights, thr);
+        if (dist <= thr) {
+            matches[n_matches] = i;
+            distances[n_matches] = sqrt(dist);
+            n_matches++;
+        }
+    }
+    free(cost); free(cost_prev); free(weights);
+    return n_matches;
+}

@@ -298,3 +298,82 @@ def test_mapping_tie_rule_prefers_diag_then_up():
     pi, pj = oracle.mapping(A)
     # from (1,2): diag (0,1)=0 ties up (0,2)=5? no: diag 0 < up 5, left (1,1)=0 ties diag -> diag
     assert list(zip(pi, pj))[-2] == (0, 1)
+
+
+def _window_costs(s, x, metric, r, wts):
+    """Squared cost of every window of x against s, from the definitions."""
+    L = len(s)
+    out = []
+    for w0 in range(len(x) - L + 1):
+        win = x[w0:w0 + L]
+        if metric == "euclidean":
+            c = 0.0
+            for q in range(L):
+                v = win[q] - s[q]
+                c += v * v
+        else:
+            c = brute_dtw(s, win, r, wts)
+        out.append(c)
+    return out
+
+
+def test_subsequence_matches_known_answer():
+    # tests/wildboar/distance/test_distance.py:1047-1060: windows [0, 1, 6]
+    # and [1, 6]; the distances follow by hand: |(0,0,0,-1)| = 1,
+    # |(1,1,0,-1)| = sqrt(3), the exact copy 0, |(9,0,0,-1)| > 2.
+    s = np.array([1, 2, 3, 4], float)
+    x0 = np.array([1, 2, 3, 3, 3, 3, 1, 2, 3, 4], float)
+    x1 = np.array([10, 2, 3, 3, 3, 3, 1, 2, 3, 4], float)
+    idx, dist = oracle.subsequence_matches(s, x0, 2.0, "euclidean")
+    assert idx.tolist() == [0, 1, 6]
+    assert dist.tolist() == [1.0, math.sqrt(3.0), 0.0]
+    idx, dist = oracle.subsequence_matches(s, x1, 2.0, "euclidean")
+    assert idx.tolist() == [1, 6]
+    # the bound is compared squared: a negative threshold acts as its magnitude
+    assert oracle.subsequence_matches(s, x0, -2.0, "euclidean")[0].tolist() == [0, 1, 6]
+    assert oracle.subsequence_matches(s, x0, float("nan"), "euclidean")[0].size == 0
+
+
+@pytest.mark.parametrize("metric", ["dtw", "euclidean", "wdtw"])
+def test_subsequence_matches_brute_force(metric):
+    rng = np.random.default_rng(21)
+    x = rw(rng, 1, 40)[0]
+    for L in (1, 4, 9, 40):
+        s = rng.standard_normal(L)
+        wts = oracle.wdtw_weights(40, 0.1) if metric == "wdtw" else None
+        costs = _window_costs(s, x, metric, 0.3, wts)
+        for thr in (INF, 0.0, math.sqrt(np.median(costs)), math.sqrt(min(costs))):
+            idx, dist = oracle.subsequence_matches(s, x, thr, metric, r=0.3, g=0.1)
+            want = [w for w, c in enumerate(costs) if c <= thr * thr]
+            assert idx.tolist() == want, (L, thr)
+            assert dist.tolist() == [math.sqrt(costs[w]) for w in want]
+        # threshold inf: every window; its minimum is the subsequence distance
+        idx, dist = oracle.subsequence_matches(s, x, INF, metric, r=0.3, g=0.1)
+        sd, si = oracle.subsequence([s], x[None, :], metric=metric, r=0.3, g=0.1)
+        assert dist.min() == sd[0, 0] and int(np.argmin(dist)) == si[0, 0]
+
+
+@pytest.mark.parametrize("metric", ["dtw", "euclidean", "wdtw"])
+def test_argmin_subsequence_is_heap_of_window_distances(metric):
+    rng = np.random.default_rng(22)
+    X = rw(rng, 5, 33)
+    shapelets = [rng.standard_normal(L) for L in (1, 5, 8, 12, 33)]
+    wts = oracle.wdtw_weights(33, 0.2) if metric == "wdtw" else None
+    for k in (1, 2, 5):
+        idx, dist = oracle.argmin_subsequence(shapelets, X, k=k, metric=metric, r=0.25, g=0.2)
+        for i, s in enumerate(shapelets):
+            d = np.sqrt(_window_costs(s, X[i], metric, 0.25, wts))
+            kk = min(k, d.size)
+            np.testing.assert_array_equal(np.sort(dist[i, :kk]), np.sort(d)[:kk])
+            ri, rd = oracle.argmin_replay(d, kk)
+            assert idx[i, :kk].tolist() == ri.tolist() and dist[i, :kk].tolist() == rd.tolist()
+
+
+def test_argmin_subsequence_euclidean_is_dtw_r0():
+    rng = np.random.default_rng(23)
+    X = rw(rng, 4, 30)
+    shapelets = [rng.standard_normal(L) for L in (3, 6, 10, 2)]
+    a = oracle.argmin_subsequence(shapelets, X, k=4, metric="euclidean")
+    b = oracle.argmin_subsequence(shapelets, X, k=4, metric="dtw", r=0.0)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])

@@ -28,7 +28,8 @@ EXPORTS = (
     "wbx_dev_subsequence_pairwise", "wbx_set_policy", "wbx_prof_enable", "wbx_prof_take",
     "wbx_prof_valu_peak", "wbx_dev_eos_lengths", "wbx_batch_create", "wbx_batch_free",
     "wbx_batch_set_launch", "wbx_batch_autotune", "wbx_batch_bucket_info", "wbx_batch_info", "wbx_batch_run", "wbx_dtw_train_sizes", "wbx_dev_dtw_train_forward",
-    "wbx_dev_dtw_train_backward",
+    "wbx_dev_dtw_train_backward", "wbx_argmin_subsequence", "wbx_subsequence_profile",
+    "wbx_dev_argmin_subsequence", "wbx_dev_subsequence_profile",
 )
 
 
@@ -100,6 +101,11 @@ def load():
             "wbx_dev_paired": (i32, [m, i32, p, i64, p, i64, i64, i64, p, p]),
             "wbx_dev_argmin": (i32, [m, i32, p, i64, i64, i64, p, i64, i64, i64, i64, p, p, p, p]),
             "wbx_dev_subsequence_pairwise": (i32, [m, i32, p, p, p, i64, p, i64, i64, i64, p, p, p]),
+            "wbx_argmin_subsequence": (i32, [m, p, p, p, i64, p, i64, i64, i64, i64, p, p]),
+            "wbx_subsequence_profile": (i32, [m, p, p, p, i64, i32, p, i64, i64, i64, i64, i64, p]),
+            "wbx_dev_argmin_subsequence": (i32, [m, i32, p, p, p, p, i64, i64, i64, i64, p, p, p]),
+            "wbx_dev_subsequence_profile": (i32, [m, i32, p, p, p, i64, i32, p, i64, i64, i64, i64,
+                                                  p, p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)

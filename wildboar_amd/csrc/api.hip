@@ -519,6 +519,56 @@ int subseq_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
                             max_len, dist, idx, st);
 }
 
+// -------------------------------------------------------------- profiles
+// Squared cost of every window of series q of X against pattern
+// (paired ? q : 0) of S: (n, nwp) with nwp = t - shortest pattern + 1.
+// Exact float64 arithmetic for every precision (FP32_ACC64 on the inputs
+// rounded to float32); WDTW weights are sized by t (the series length, as
+// WeightedDtwSubsequenceMetric and Metric.reset(t, s) size them).
+int profile_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
+                 const int64_t *s_off, const int64_t *s_len, int64_t n_s, int paired,
+                 const void *X, int64_t n, int64_t xstr, int64_t t, int64_t nwp, double *prof,
+                 int64_t *max_len_out, hipStream_t st) {
+    if (paired && n_s != n) return fail(WBX_EINVAL, "paired patterns need one per series");
+    std::vector<int64_t> hl((size_t)n_s), ho((size_t)n_s);
+    WBX_HIP_CHECK(hipMemcpyAsync(hl.data(), s_len, sizeof(int64_t) * n_s, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(ho.data(), s_off, sizeof(int64_t) * n_s, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));
+    int64_t max_len = 0, min_len = t, total = 0;
+    for (int64_t j = 0; j < n_s; ++j) {
+        if (hl[j] < 1 || hl[j] > t)
+            return fail(WBX_EINVAL, "Invalid subsequnce shape (%lld > %lld)", (long long)hl[j],
+                        (long long)t);
+        if (ho[j] < 0) return fail(WBX_EINVAL, "negative pattern offset");
+        max_len = hl[j] > max_len ? hl[j] : max_len;
+        min_len = hl[j] < min_len ? hl[j] : min_len;
+        if (ho[j] + hl[j] > total) total = ho[j] + hl[j];
+    }
+    if (nwp != t - min_len + 1)
+        return fail(WBX_EINVAL, "profile width %lld, expected %lld", (long long)nwp,
+                    (long long)(t - min_len + 1));
+    if (max_len_out) *max_len_out = max_len;
+    wbx_metric mm = *m;
+    int64_t s_stride = total;
+    RC(acc64_prepare(ctx, &mm, dtype, {{&S, 1, total, &s_stride}, {&X, n, t, &xstr}}, st));
+    const double *w64 = nullptr;
+    const float *w32 = nullptr;
+    if (mm.kind == WBX_WDTW) RC(cached_weights(ctx, t, mm.g, &w64, &w32));
+    Carve cv;
+    const size_t o_x = cv.take(stage_bytes(dtype, true, n, t));
+    const size_t o_s = cv.take(stage_bytes(dtype, true, 1, total));
+    void *wsp = nullptr;
+    RC(ws_reserve(ctx, cv.total, &wsp));
+    char *ws = (char *)wsp;
+    Staged<double> x64, s64;
+    RC(stage<double>(dtype, X, n, t, xstr, ws, o_x, &x64, st));
+    RC(stage<double>(dtype, S, 1, total, total, ws, o_s, &s64, st));
+    Shapelets sh{s64.ptr, s_off, s_len, n_s};
+    ProfScope pf(st, KERN_SUBSEQ);
+    return launch_profile(mm.kind, mm.r, sh, paired, (int)max_len,
+                          series<double>(x64.ptr, x64.stride, t), n, nwp, w64, prof, st);
+}
+
 // Host-pointer wrapper helpers.
 struct DevBuf {
     void *p = nullptr;
@@ -814,6 +864,138 @@ int wbx_argmin(const wbx_metric *metric, const double *X, int64_t nx, int64_t di
                       (double *)ddist.p, st));
     WBX_HIP_CHECK(hipMemcpyAsync(idx, didx.p, sizeof(int64_t) * nx * k, hipMemcpyDeviceToHost, st));
     WBX_HIP_CHECK(hipMemcpyAsync(dist, ddist.p, sizeof(double) * nx * k, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));
+    return WBX_OK;
+}
+
+int wbx_dev_subsequence_profile(const wbx_metric *metric, int dtype, const void *S,
+                                const int64_t *s_off, const int64_t *s_len, int64_t n_s,
+                                int paired, const void *X, int64_t n, int64_t x_stride, int64_t t,
+                                int64_t nwp, double *prof, void *stream) {
+    RC(check_metric(metric));
+    RC(check_sizes(n, t));
+    if (n_s < 1) return fail(WBX_EINVAL, "Subsequence y cannot be empty.");
+    if (!S || !s_off || !s_len || !X || !prof) return fail(WBX_EINVAL, "NULL argument");
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    DevGuard g(ctx->device);
+    hipStream_t st = pick_stream(ctx, stream);
+    RC(begin(ctx, st));
+    int rc = profile_impl(ctx, metric, dtype, S, s_off, s_len, n_s, paired ? 1 : 0, X, n, x_stride,
+                          t, nwp, prof, nullptr, st);
+    int rc2 = end(ctx, st);
+    return rc ? rc : rc2;
+}
+
+int wbx_dev_argmin_subsequence(const wbx_metric *metric, int dtype, const void *S,
+                               const int64_t *s_off, const int64_t *s_len, const void *X,
+                               int64_t n, int64_t x_stride, int64_t t, int64_t k, int64_t *idx,
+                               double *dist, void *stream) {
+    RC(check_metric(metric));
+    RC(check_sizes(n, t));
+    if (!S || !s_off || !s_len || !X || !idx || !dist) return fail(WBX_EINVAL, "NULL argument");
+    if (k < 1) return fail(WBX_EINVAL, "k == %lld, must be >= 1", (long long)k);
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    DevGuard g(ctx->device);
+    hipStream_t st = pick_stream(ctx, stream);
+    RC(begin(ctx, st));
+    std::vector<int64_t> hl((size_t)n);
+    WBX_HIP_CHECK(hipMemcpyAsync(hl.data(), s_len, sizeof(int64_t) * n, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));
+    int64_t min_len = t, max_len = 0;
+    for (int64_t q = 0; q < n; ++q) {
+        if (hl[q] < 1 || hl[q] > t)
+            return fail(WBX_EINVAL, "the longest subsequence must be shorter than samples.");
+        min_len = hl[q] < min_len ? hl[q] : min_len;
+        max_len = hl[q] > max_len ? hl[q] : max_len;
+    }
+    if (k > t - max_len + 1) return fail(WBX_EINVAL, "k must be less x.shape[-1] - y.shape[-1] + 1.");
+    const int64_t nwp = t - min_len + 1;
+    // the profile and the heaps live in their own allocation: profile_impl
+    // stages its inputs in the shared workspace
+    const size_t prof_bytes = ((size_t)n * nwp * 8 + 255) & ~(size_t)255;
+    void *buf = nullptr;
+    if (hipMalloc(&buf, prof_bytes + (size_t)n * k * 16) != hipSuccess)
+        return fail(WBX_ENOMEM, "profile allocation of %zu bytes failed", prof_bytes);
+    double *prof = (double *)buf;
+    int rc = profile_impl(ctx, metric, dtype, S, s_off, s_len, n, 1, X, n, x_stride, t, nwp, prof,
+                          nullptr, st);
+    // the heap keeps square roots, as Metric._eadistance returns them
+    if (!rc) rc = launch_sqrt_inplace(prof, n * nwp, st);
+    if (!rc) rc = launch_argmin_replay(prof, n, nwp, k, nullptr, (char *)buf + prof_bytes, idx,
+                                       dist, st);
+    const int rc2 = end(ctx, st);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(buf);
+    return rc ? rc : rc2;
+}
+
+int wbx_subsequence_profile(const wbx_metric *metric, const double *S, const int64_t *s_off,
+                            const int64_t *s_len, int64_t n_s, int paired, const double *X,
+                            int64_t n, int64_t dims, int64_t t, int64_t dim, int64_t nwp,
+                            double *prof) {
+    RC(check_metric(metric));
+    RC(check_sizes(n, t));
+    if (n_s < 1) return fail(WBX_EINVAL, "Subsequence y cannot be empty.");
+    if (dims < 1 || dim < 0 || dim >= dims)
+        return fail(WBX_EINVAL, "The parameter dim must be 0 <= dim < n_dims");
+    int64_t total = 0;
+    for (int64_t j = 0; j < n_s; ++j) {
+        if (s_len[j] < 1 || s_len[j] > t)
+            return fail(WBX_EINVAL, "Invalid subsequnce shape (%lld > %lld)", (long long)s_len[j],
+                        (long long)t);
+        if (s_off[j] < 0) return fail(WBX_EINVAL, "negative pattern offset");
+        if (s_off[j] + s_len[j] > total) total = s_off[j] + s_len[j];
+    }
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    hipStream_t st = ctx->stream;
+    DevBuf dx, ds, doff, dlen, dprof;
+    RC(upload(dx, X, sizeof(double) * n * dims * t, st));
+    RC(upload(ds, S, sizeof(double) * total, st));
+    RC(upload(doff, s_off, sizeof(int64_t) * n_s, st));
+    RC(upload(dlen, s_len, sizeof(int64_t) * n_s, st));
+    RC(dprof.alloc(sizeof(double) * n * nwp));
+    RC(wbx_dev_subsequence_profile(metric, WBX_F64, ds.p, (const int64_t *)doff.p,
+                                   (const int64_t *)dlen.p, n_s, paired,
+                                   (const double *)dx.p + dim * t, n, dims * t, t, nwp,
+                                   (double *)dprof.p, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(prof, dprof.p, sizeof(double) * n * nwp, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));
+    return WBX_OK;
+}
+
+int wbx_argmin_subsequence(const wbx_metric *metric, const double *S, const int64_t *s_off,
+                           const int64_t *s_len, int64_t n, const double *X, int64_t dims,
+                           int64_t t, int64_t dim, int64_t k, int64_t *idx, double *dist) {
+    RC(check_metric(metric));
+    RC(check_sizes(n, t));
+    if (dims < 1 || dim < 0 || dim >= dims)
+        return fail(WBX_EINVAL, "The parameter dim must be 0 <= dim < n_dims");
+    if (k < 1) return fail(WBX_EINVAL, "k == %lld, must be >= 1", (long long)k);
+    int64_t total = 0;
+    for (int64_t j = 0; j < n; ++j) {
+        if (s_len[j] < 1 || s_len[j] > t)
+            return fail(WBX_EINVAL, "the longest subsequence must be shorter than samples.");
+        if (s_off[j] < 0) return fail(WBX_EINVAL, "negative pattern offset");
+        if (s_off[j] + s_len[j] > total) total = s_off[j] + s_len[j];
+    }
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    hipStream_t st = ctx->stream;
+    DevBuf dx, ds, doff, dlen, didx, ddist;
+    RC(upload(dx, X, sizeof(double) * n * dims * t, st));
+    RC(upload(ds, S, sizeof(double) * total, st));
+    RC(upload(doff, s_off, sizeof(int64_t) * n, st));
+    RC(upload(dlen, s_len, sizeof(int64_t) * n, st));
+    RC(didx.alloc(sizeof(int64_t) * n * k));
+    RC(ddist.alloc(sizeof(double) * n * k));
+    RC(wbx_dev_argmin_subsequence(metric, WBX_F64, ds.p, (const int64_t *)doff.p,
+                                  (const int64_t *)dlen.p, (const double *)dx.p + dim * t, n,
+                                  dims * t, t, k, (int64_t *)didx.p, (double *)ddist.p, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(idx, didx.p, sizeof(int64_t) * n * k, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(dist, ddist.p, sizeof(double) * n * k, hipMemcpyDeviceToHost, st));
     WBX_HIP_CHECK(hipStreamSynchronize(st));
     return WBX_OK;
 }

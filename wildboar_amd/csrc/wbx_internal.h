@@ -215,6 +215,11 @@ struct Shapelets {
 template <typename T>
 int launch_subseq_exact(int kind, double r, Shapelets sh, int max_len, Series<T> xs, int64_t n,
                         const T *weights, double *dist, int64_t *idx, hipStream_t st);
+// Distance profiles (subseq_match.hip): prof[q * nwp + s] = squared cost of
+// window s of series q against pattern (paired ? q : 0), +inf past its last.
+int launch_profile(int kind, double r, Shapelets sh, int paired, int max_len, Series<double> xs,
+                   int64_t n, int64_t nwp, const double *weights, double *prof, hipStream_t st);
+int launch_sqrt_inplace(double *p, int64_t n, hipStream_t st);
 
 // ---------------------------------------------------------------- re-rank
 struct ErrBound {

@@ -135,3 +135,45 @@ def subsequence(shapelets, X, *, metric="dtw", r=1.0, g=0.05, precision="fp32"):
             m, _dtype(X), packed.data_ptr(), offs.data_ptr(), lens.data_ptr(), ns, X.data_ptr(),
             n, X.stride(0), X.shape[1], dist.data_ptr(), idx.data_ptr(), _stream()))
     return dist, idx
+
+
+def _patterns(shapelets, X):
+    if isinstance(shapelets, tuple) and len(shapelets) == 3 and torch.is_tensor(shapelets[0]):
+        packed, offs, lens = shapelets
+        if packed.dtype != X.dtype or packed.device != X.device:
+            raise TypeError("packed shapelets must have the dtype and device of X")
+        return packed, offs, lens
+    return pack_shapelets(shapelets, X.device, X.dtype)
+
+
+def subsequence_profile(shapelets, X, *, paired=False, metric="dtw", r=1.0, g=0.05):
+    """Squared cost of every window: (n, t - min_len + 1) float64, exact in
+    float64 whatever X's dtype.  Row q scores pattern ``q if paired else 0``
+    against X[q]; windows past a pattern's last are +inf."""
+    X = _rows(X, "X")
+    packed, offs, lens = _patterns(shapelets, X)
+    n, t, ns = X.shape[0], X.shape[1], int(lens.shape[0])
+    nwp = t - int(lens.min().item()) + 1
+    prof = torch.empty((n, max(nwp, 1)), dtype=torch.float64, device=X.device)
+    with torch.cuda.device(X.device):
+        _lib.check(_lib.load().wbx_dev_subsequence_profile(
+            _metric(metric, r, g, "fp64"), _dtype(X), packed.data_ptr(), offs.data_ptr(),
+            lens.data_ptr(), ns, 1 if paired else 0, X.data_ptr(), n, X.stride(0), t, nwp,
+            prof.data_ptr(), _stream()))
+    return prof
+
+
+def argmin_subsequence(shapelets, X, k=1, *, metric="dtw", r=1.0, g=0.05):
+    """(idx (n, k) int64, dist (n, k) float64): the k closest windows of X[q]
+    to pattern q, in the heap order of argmin_subsequence_distance."""
+    X = _rows(X, "X")
+    packed, offs, lens = _patterns(shapelets, X)
+    n, t = X.shape[0], X.shape[1]
+    idx = torch.empty((n, k), dtype=torch.int64, device=X.device)
+    dist = torch.empty((n, k), dtype=torch.float64, device=X.device)
+    with torch.cuda.device(X.device):
+        _lib.check(_lib.load().wbx_dev_argmin_subsequence(
+            _metric(metric, r, g, "fp64"), _dtype(X), packed.data_ptr(), offs.data_ptr(),
+            lens.data_ptr(), X.data_ptr(), n, X.stride(0), t, int(k), idx.data_ptr(),
+            dist.data_ptr(), _stream()))
+    return idx, dist

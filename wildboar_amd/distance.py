@@ -7,6 +7,9 @@ wildboar's ``distance/_distance.py``:
 - ``paired_distance``              (``_distance.py:1082-1175``)
 - ``argmin_distance``              (``_distance.py:1307-1457``)
 - ``pairwise_subsequence_distance`` (``_distance.py:543-636``)
+- ``subsequence_match``            (``_distance.py:732-924``)
+- ``paired_subsequence_match``     (``_distance.py:927-1079``)
+- ``argmin_subsequence_distance``  (``_distance.py:1636-1764``)
 
 Every computation runs in ``libwbx.so`` on the GPU; these functions only
 validate, reshape and dispatch.  One keyword is added: ``precision`` --
@@ -15,6 +18,7 @@ cells in float32 (argmin indices and distances stay exact through a certified
 float64 re-rank).
 """
 
+import math
 import numbers
 import warnings
 
@@ -28,6 +32,9 @@ __all__ = [
     "paired_distance",
     "argmin_distance",
     "pairwise_subsequence_distance",
+    "subsequence_match",
+    "paired_subsequence_match",
+    "argmin_subsequence_distance",
     "check_metric",
     "METRICS",
 ]
@@ -375,3 +382,320 @@ def pairwise_subsequence_distance(y, x, *, dim=0, metric="euclidean", metric_par
     if return_index:
         return _format_return(dist, len(y), x.ndim), _format_return(idx, len(y), x.ndim)
     return _format_return(dist, len(y), x.ndim)
+
+
+# ------------------------------------------------------- subsequence matching
+def _is_arraylike(x):
+    # sklearn.utils.validation._is_arraylike, as the reference uses it
+    return hasattr(x, "__len__") or hasattr(x, "shape") or hasattr(x, "__array__")
+
+
+def _safe_jagged_array(lst):
+    """``utils/__init__.py:147-150``: a 1-D object array of arrays (or None)."""
+    arr = np.empty(len(lst), dtype=object)
+    arr[:] = lst
+    return arr
+
+
+def _std_below_mean(s):
+    """``_distance.py:310-330``."""
+
+    def f(x):
+        return max(np.mean(x) - s * np.std(x), np.min(x))
+
+    return f
+
+
+_THRESHOLD = {"auto": _std_below_mean(2.0)}
+
+
+def _resolve_threshold(threshold, max_matches, n_samples, per_sample):
+    """The threshold branches of ``subsequence_match`` (``_distance.py:836-881``)
+    and ``paired_subsequence_match`` (:1033-1058).  Returns (bound, max_dist,
+    max_matches): ``bound`` is what the GPU scan compares against,
+    ``max_dist(i, d)`` the optional host mask."""
+    if threshold is None:
+        threshold = np.inf
+        if max_matches is None:
+            max_matches = 10
+    if callable(threshold):
+        fn = threshold
+        return np.inf, lambda _, d: d <= fn(d), max_matches
+    if isinstance(threshold, str):
+        if threshold not in _THRESHOLD:
+            raise ValueError("The parameter threshold must be one of {}, got {!r}".format(
+                set(_THRESHOLD), threshold))
+        fn = _THRESHOLD[threshold]
+        return np.inf, lambda _, d: d <= fn(d), max_matches
+    if per_sample and _is_arraylike(threshold):
+        arr = np.asarray(threshold)
+        if len(arr) != n_samples:
+            raise ValueError(f"threshold array length ({len(arr)}) must match "
+                             f"the number of samples ({n_samples})")
+        return np.inf, lambda i, d: d <= arr[i], max_matches
+    if not isinstance(threshold, numbers.Real):
+        kinds = "str, callable, array-like or float" if per_sample else "str, callable or float"
+        raise TypeError("threshold must be %s, not %s" % (kinds, type(threshold).__qualname__))
+    return float(threshold), None, max_matches
+
+
+def _pack(patterns):
+    lens = np.array([p.shape[0] for p in patterns], dtype=np.int64)
+    offs = np.zeros(len(patterns), dtype=np.int64)
+    if len(patterns) > 1:
+        offs[1:] = np.cumsum(lens)[:-1]
+    S = np.ascontiguousarray(np.concatenate(patterns).astype(np.float64))
+    if not np.isfinite(S).all():
+        raise ValueError("Input contains NaN or infinity.")
+    return S, offs, lens
+
+
+def _profiles(patterns, x_, dim, metric, paired):
+    """Squared cost of every window, one row per series: ``(n, t - min_len + 1)``
+    (windows past a pattern's last are +inf)."""
+    S, offs, lens = _pack(patterns)
+    n, dims, t = x_.shape
+    nwp = t - int(lens.min()) + 1
+    prof = np.empty((n, nwp))
+    _lib.check(_lib.load().wbx_subsequence_profile(
+        metric.struct(_lib.FP64), S.ctypes.data, offs.ctypes.data, lens.ctypes.data, len(patterns),
+        1 if paired else 0, x_.ctypes.data, n, dims, t, int(dim), nwp, prof.ctypes.data))
+    return prof
+
+
+def _matches_from_profiles(prof, threshold, n_windows):
+    """The windows with cost <= threshold^2, in order, with their distances
+    (``_metric.pyx:762-795`` / ``_elastic.pyx:658-698``; the early-abandoned
+    sums of the reference never pass that test).  Row i has ``n_windows[i]``
+    windows (``t - len + 1`` of its pattern).  None for no match
+    (``_cdistance.pyx:936-953``)."""
+    bound = threshold * threshold
+    indices, distances = [], []
+    for row, nw in zip(prof, n_windows):
+        row = row[:nw]
+        hit = np.flatnonzero(row <= bound)
+        if hit.size == 0:
+            indices.append(None)
+            distances.append(None)
+        else:
+            indices.append(hit.astype(np.intp))
+            distances.append(np.sqrt(row[hit]))
+    return indices, distances
+
+
+def _any_in_exclude(lst, i, exclude):
+    """``_distance.py:376-397``."""
+    for e in lst:
+        if i - exclude < e < i + exclude:
+            return True
+    return False
+
+
+def _exclude_trivial_matches(indices, distances, exclude):
+    """``_distance.py:400-443``: greedy by distance, dropping matches within
+    ``exclude`` of a kept one."""
+    out_i, out_d = [], []
+    for index, distance in zip(indices, distances):
+        if index is None:
+            out_i.append(None)
+            out_d.append(None)
+            continue
+        order = np.argsort(distance)
+        keep = np.zeros(order.size, dtype=bool)
+        kept = []
+        for j in range(index.size):
+            if not _any_in_exclude(kept, index[order[j]], exclude):
+                kept.append(index[order[j]])
+                keep[order[j]] = True
+        out_i.append(index[keep])
+        out_d.append(distance[keep])
+    return out_i, out_d
+
+
+def _filter_by_max_matches(indices, distances, max_matches):
+    """``_distance.py:446-477``."""
+    out_i, out_d = [], []
+    for index, distance in zip(indices, distances):
+        if index is None:
+            out_i.append(None)
+            out_d.append(None)
+        else:
+            order = np.argsort(distance)[:max_matches]
+            out_i.append(index[order])
+            out_d.append(distance[order])
+    return out_i, out_d
+
+
+def _filter_by_max_dist(indices, distances, max_dist):
+    """``_distance.py:480-511``."""
+    out_i, out_d = [], []
+    for sample, (index, distance) in enumerate(zip(indices, distances)):
+        if index is None:
+            out_i.append(None)
+            out_d.append(None)
+        else:
+            mask = max_dist(sample, distance)
+            out_i.append(index[mask])
+            out_d.append(distance[mask])
+    return out_i, out_d
+
+
+def _subsequence_metric(metric, metric_params, scale):
+    if scale or (isinstance(metric, str) and metric.startswith("scaled_")):
+        raise NotImplementedError("scaled subsequence metrics are not served by this library")
+    if metric_params is not None and not isinstance(metric_params, dict):
+        raise TypeError("The 'metric_params' parameter must be None or a dict.")
+    return check_metric(metric)(**(metric_params if metric_params is not None else {}))
+
+
+def subsequence_match(y, x, threshold=None, *, dim=0, metric="euclidean", metric_params=None,
+                      scale=False, max_matches=None, exclude=None, return_distance=False,
+                      n_jobs=None):
+    """Start indices of the windows of each ``x`` row within ``threshold`` of
+    the subsequence ``y`` (``_distance.py:732-924``).
+
+    Without a threshold the 10 closest windows are returned, ordered by
+    distance; with one, every match in order of occurrence unless
+    ``max_matches`` asks for the closest few.  ``exclude`` drops trivial
+    matches near a closer one.  A sample without matches gives None.
+    """
+    y = _validate_subsequence(y)
+    if len(y) > 1:
+        raise ValueError("A single subsequence expected, got %d" % len(y))
+    y = y[0]
+    x = check_array(x)
+    if y.shape[0] > x.shape[-1]:
+        raise ValueError("Invalid subsequnce shape (%d > %d)" % (y.shape[0], x.shape[-1]))
+    n_dims = x.shape[1] if x.ndim == 3 else 1
+    # the reference's test reads `not 0 >= dim < n_dims` (:821), which lets only
+    # dim <= 0 through; negative dims index from the end there and are
+    # refused here
+    if not (isinstance(dim, numbers.Integral) and 0 <= dim < n_dims):
+        raise ValueError("The parameter dim must be 0 <= dim < n_dims")
+    metric_ = _subsequence_metric(metric, metric_params, scale)
+    if n_jobs is not None:
+        warnings.warn("n_jobs is not yet supported.", UserWarning)
+    n_samples = x.shape[0] if x.ndim > 1 else 1
+    bound, max_dist, max_matches = _resolve_threshold(threshold, max_matches, n_samples, True)
+    if exclude is not None:
+        if isinstance(exclude, numbers.Integral):
+            if exclude < 0:
+                raise ValueError(f"exclude == {exclude}, must be >= 0.")
+        elif isinstance(exclude, numbers.Real):
+            if not exclude >= 0:
+                raise ValueError(f"exclude == {exclude}, must be >= 0.")
+            exclude = math.ceil(y.size * exclude)
+        else:
+            raise TypeError("exclude must be an instance of {int, float}, not %s."
+                            % type(exclude).__qualname__)
+    prof = _profiles([y], _ts(x), dim, metric_, paired=False)
+    indices, distances = _matches_from_profiles(prof, bound,
+                                                [x.shape[-1] - y.shape[0] + 1] * prof.shape[0])
+    if max_dist is not None:
+        indices, distances = _filter_by_max_dist(indices, distances, max_dist)
+    if exclude:
+        indices, distances = _exclude_trivial_matches(indices, distances, exclude)
+    if max_matches:
+        indices, distances = _filter_by_max_matches(indices, distances, max_matches)
+    indices = _format_return(_safe_jagged_array(indices), len(y), x.ndim)
+    if indices.size == 1:
+        indices = indices.item()
+    if return_distance:
+        distances = _format_return(_safe_jagged_array(distances), len(y), x.ndim)
+        if distances.size == 1:
+            distances = distances.item()
+        return indices, distances
+    return indices
+
+
+def paired_subsequence_match(y, x, threshold=None, *, dim=0, metric="euclidean",
+                             metric_params=None, scale=False, max_matches=None,
+                             return_distance=False, n_jobs=None):
+    """The matches of the i-th subsequence in the i-th series
+    (``_distance.py:927-1079``); same threshold rules as ``subsequence_match``
+    except per-sample threshold arrays."""
+    y = _validate_subsequence(y)
+    x = check_array(x, ensure_2d=True)
+    if len(y) != x.shape[0]:
+        raise ValueError("x and y must have the same number of samples")
+    n_dims = x.shape[1] if x.ndim == 3 else 1
+    if not (isinstance(dim, numbers.Integral) and 0 <= dim < n_dims):
+        raise ValueError("The parameter dim must be 0 <= dim < n_dims")
+    for s in y:
+        if s.shape[0] > x.shape[-1]:
+            raise ValueError("invalid subsequnce shape (%d > %d)" % (s.shape[0], x.shape[-1]))
+    metric_ = _subsequence_metric(metric, metric_params, scale)
+    if n_jobs is not None:
+        warnings.warn("n_jobs is not yet supported.", UserWarning)
+    # the reference's one-argument max_dist (:1039) would fail inside
+    # _filter_by_max_dist, which passes (sample, distances); the mask here
+    # takes the distances as intended
+    bound, max_dist, max_matches = _resolve_threshold(threshold, max_matches, len(y), False)
+    if any(s.shape[0] < 1 for s in y):
+        raise ValueError("Subsequence y cannot be empty.")
+    prof = _profiles(y, _ts(x), dim, metric_, paired=True)
+    indices, distances = _matches_from_profiles(prof, bound,
+                                                [x.shape[-1] - s.shape[0] + 1 for s in y])
+    if max_dist is not None:
+        indices, distances = _filter_by_max_dist(indices, distances, max_dist)
+    if max_matches:
+        indices, distances = _filter_by_max_matches(indices, distances, max_matches)
+    indices = _format_return(_safe_jagged_array(indices), len(y), x.ndim)
+    if indices.size == 1:
+        indices = indices.reshape(1)
+    if return_distance:
+        distances = _format_return(_safe_jagged_array(distances), len(y), x.ndim)
+        if distances.size == 1:
+            distances = distances.reshape(1)
+        return indices, distances
+    return indices
+
+
+def argmin_subsequence_distance(y, x, *, dim=0, k=1, metric="euclidean", metric_params=None,
+                                scale=False, return_distance=False, n_jobs=None):
+    """The ``k`` closest windows of ``x[i]`` to subsequence ``y[i]``
+    (``_distance.py:1636-1764``): indices ``(n_samples, k)`` in the
+    reference's heap order (``_cdistance.pyx:1408-1434``) and, with
+    ``return_distance``, their distances."""
+    if isinstance(k, bool) or not isinstance(k, numbers.Integral):
+        raise TypeError(f"The 'k' parameter must be an int, got {k!r}.")
+    if k < 1:
+        raise ValueError(f"The 'k' parameter must be an int in the range [1, inf), got {k!r}.")
+    if isinstance(y, np.ndarray) and y.dtype == float:
+        y2 = check_array(y, allow_3d=False, ensure_2d=True)
+        patterns = list(y2)
+    else:
+        patterns = []
+        for s in y:
+            s = np.asarray(s, dtype=np.float64)
+            if s.ndim > 1:
+                raise ValueError("shapelet must be 1d-array")
+            patterns.append(s)
+    if len(patterns) == 0:
+        raise ValueError("Subsequence y cannot be empty.")
+    lens = np.array([p.shape[0] for p in patterns], dtype=np.int64)
+    if (lens < 1).any():
+        raise ValueError("Subsequence y cannot be empty.")
+    max_len = int(lens.max())
+    x = np.asarray(x)
+    if x.ndim == 1:
+        x = np.broadcast_to(x, shape=(len(patterns), x.shape[0]))
+    x_ = _ts(check_array(x, ensure_2d=True))
+    if not max_len <= x_.shape[2]:
+        raise ValueError("the longest subsequence must be shorter than samples.")
+    if len(patterns) != x_.shape[0]:
+        raise ValueError("both arrays must have the same number of samples.")
+    if not (isinstance(dim, numbers.Integral) and 0 <= dim < x_.shape[1]):
+        raise ValueError(f"The parameter dim must be dim ({dim}) < n_dims ({x_.shape[1]})")
+    if not k <= (x_.shape[2] - max_len + 1):
+        raise ValueError("k must be less x.shape[-1] - y.shape[-1] + 1.")
+    metric_ = _subsequence_metric(metric, metric_params, scale)
+    S, offs, lens = _pack(patterns)
+    n = x_.shape[0]
+    idx = np.empty((n, k), dtype=np.intp)
+    dist = np.empty((n, k))
+    _lib.check(_lib.load().wbx_argmin_subsequence(
+        metric_.struct(_lib.FP64), S.ctypes.data, offs.ctypes.data, lens.ctypes.data, n,
+        x_.ctypes.data, x_.shape[1], x_.shape[2], int(dim), int(k), idx.ctypes.data,
+        dist.ctypes.data))
+    return (idx, dist) if return_distance else idx
