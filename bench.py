@@ -267,6 +267,70 @@ def run_subsequence(torch, dev, n_series):
     return res
 
 
+# -------------------------------------------------------------- float64 paths
+def run_fp64(torch, dev):
+    """The exact float64 paths (the drop-in default precision): window
+    profiles, k best windows and pairwise DTW, each by the register-band
+    float64 kernels and, on the same inputs, by the block kernels (plain
+    policy); a slice of every result against the oracle, bit for bit."""
+    import oracle
+    from wildboar_amd import _lib, device
+
+    Xh = random_walks(40, 2000, 1024)
+    X = torch.from_numpy(Xh).to(dev)
+    rng = np.random.default_rng(41)
+    res = {"workload": "float64 random walks, 2000 series x 1024; dtw r = 0.1 unless noted"}
+
+    def both(name, fn, cells, unit="cells/s"):
+        fn()
+        t_lane = timed(torch, fn, 3)
+        _lib.set_policy(1)
+        try:
+            fn()
+            t_plain = timed(torch, fn, 1)
+        finally:
+            _lib.set_policy(0)
+        res[name] = {"seconds": t_lane, "work": cells, "value": cells / t_lane, "unit": unit,
+                     "plain_seconds": t_plain, "speedup_vs_block": t_plain / t_lane}
+        log(f"fp64 {name}: {t_lane * 1e3:.2f} ms, {cells / t_lane / 1e12:.3f} T{unit}; "
+            f"block kernels {t_plain * 1e3:.1f} ms ({t_plain / t_lane:.1f}x)")
+
+    # 1. profile of one pattern (L = 128) over every series: subsequence_match
+    pat = Xh[7, 300:428].copy()
+    pd = [torch.from_numpy(pat)]
+    nwin = 1024 - 128 + 1
+    both("profile_dtw", lambda: device.subsequence_profile(pd, X, metric="dtw", r=0.1),
+         2000.0 * nwin * band_cells(128, 128, 0.1))
+    prof = device.subsequence_profile(pd, X[:4], metric="dtw", r=0.1).cpu().numpy()
+    band = oracle.compute_r(128, 0.1)
+    want = np.array([[oracle.dtw_distance(pat, Xh[q, s:s + 128], band) for s in range(nwin)]
+                     for q in range(4)])
+    res["profile_dtw"]["mismatch_4_rows"] = int((prof != want).sum())
+    both("profile_euclidean", lambda: device.subsequence_profile(pd, X, metric="euclidean"),
+         2000.0 * nwin * 128, unit="MAC/s")
+    # 2. k = 5 best windows, pattern i (L U[16, 128]) against series i
+    L = rng.integers(16, 129, 2000)
+    pats = [Xh[int(rng.integers(2000)), a:a + l].copy()
+            for a, l in zip(rng.integers(0, 1024 - 128, 2000), L)]
+    pds = device.pack_shapelets([torch.from_numpy(p) for p in pats], dev)
+    cells = float(sum((1025 - l) * band_cells(int(l), int(l), 0.1) for l in L))
+    both("argmin_subsequence_k5",
+         lambda: device.argmin_subsequence(pds, X, k=5, metric="dtw", r=0.1), cells)
+    gi, gd = device.argmin_subsequence(pds, X, k=5, metric="dtw", r=0.1)
+    oi, od = oracle.argmin_subsequence(pats[:8], Xh[:8], k=5, metric="dtw", r=0.1)
+    res["argmin_subsequence_k5"]["mismatch_8_rows"] = int(
+        (gi[:8].cpu().numpy() != oi).sum() + (gd[:8].cpu().numpy() != od).sum())
+    # 3. pairwise DTW 1000 x 1000, t = 256, r = 0.05 (B = 23)
+    A, Bm = X[:1000, :256].contiguous(), X[1000:, :256].contiguous()
+    both("pairwise_dtw_t256_r0.05",
+         lambda: device.pairwise(A, Bm, metric="dtw", r=0.05, precision="fp64"),
+         1e6 * band_cells(256, 256, 0.05))
+    got = device.pairwise(A[:4], Bm, metric="dtw", r=0.05, precision="fp64").cpu().numpy()
+    want = oracle.pairwise(Xh[:4, :256], Xh[1000:, :256], metric="dtw", r=0.05)
+    res["pairwise_dtw_t256_r0.05"]["mismatch_4_rows"] = int((got != want).sum())
+    return res
+
+
 # ------------------------------------------------------------------------ main
 def main():
     ap = argparse.ArgumentParser()
@@ -281,6 +345,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-train", action="store_true")
     ap.add_argument("--no-subsequence", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--batched-pairs", type=int, default=50000)
     ap.add_argument("--subsequence-series", type=int, default=1000)
     ap.add_argument("--dba-steps", type=int, default=200)
@@ -436,6 +501,8 @@ def main():
             result["train"] = run_train(torch, dev, 10, args.dba_steps)
         if not args.no_subsequence:
             result["subsequence"] = run_subsequence(torch, dev, args.subsequence_series)
+        if not args.no_fp64:
+            result["fp64"] = run_fp64(torch, dev)
 
     if not args.no_cpu:
         import oracle

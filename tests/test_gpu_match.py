@@ -201,3 +201,51 @@ def test_profile_entry_validation():
     rc = lib.wbx_argmin_subsequence(m, S.ctypes.data, offs.ctypes.data, lens.ctypes.data, 2,
                                     X.ctypes.data, 1, 10, 0, 9, idx.ctypes.data, dist.ctypes.data)
     assert rc == _lib.WBX_EINVAL  # k > t - max_len + 1 = 7
+
+
+def test_float64_lane_classes_pairs_profiles_and_scans():
+    """Band widths that land in every float64 register class (B = 1 .. 47)
+    and past it (block kernels), weighted and not, equal and unequal
+    lengths; everything bit-identical to the oracle."""
+    X = rw(70, 6, 300)
+    Y = rw(71, 5, 300)
+    for t, r in ((200, 0.1), (240, 0.1), (150, 0.1), (90, 0.05), (300, 0.16)):
+        for metric in ("dtw", "wdtw"):
+            got = wd.pairwise_distance(X[:, :t], Y[:, :t], metric=metric, metric_params={"r": r})
+            want = oracle.pairwise(X[:, :t], Y[:, :t], metric=metric, r=r)
+            np.testing.assert_array_equal(got, want)
+    # unequal lengths: B = |lx - ly| + 2 w - 1 = 41 (class 48) and 23
+    for lx, ly in ((60, 90), (90, 60), (33, 45)):
+        got = wd.pairwise_distance(X[:, :lx], Y[:, :ly], metric="dtw", metric_params={"r": 0.1})
+        np.testing.assert_array_equal(got, oracle.pairwise(X[:, :lx], Y[:, :ly], metric="dtw",
+                                                           r=0.1))
+    pats = [X[0, 5:205].copy(), X[1, 0:240].copy(), X[2, 20:270].copy(), X[3, 1:2].copy(),
+            X[4, 50:90].copy(), X[5, 0:300].copy()]
+    for metric, params in (("dtw", {"r": 0.1}), ("wdtw", {"r": 0.1, "g": 0.05})):
+        prof = host_profile(pats, X, True, metric, params)
+        for q, s in enumerate(pats):
+            want = oracle_costs(s, X[q], metric, params)
+            np.testing.assert_array_equal(prof[q, :want.size], want)
+            assert np.isinf(prof[q, want.size:]).all()
+        d, i = wd.pairwise_subsequence_distance(pats, X, metric=metric, metric_params=params,
+                                                return_index=True)
+        od, oi = oracle.subsequence(pats, X, metric=metric, r=params["r"],
+                                    g=params.get("g", 0.05))
+        np.testing.assert_array_equal(i, oi)
+        np.testing.assert_array_equal(d, od)
+
+
+def test_wdtw_zero_weights_take_the_block_kernels():
+    # g * t / 2 > 709: exp overflows, the first weights are exactly 0 and a
+    # column outside the series would give inf * 0; those weight vectors are
+    # served by the block kernels, still bit-identical
+    X = rw(72, 4, 100)
+    Y = rw(73, 3, 100)
+    got = wd.pairwise_distance(X, Y, metric="wdtw", metric_params={"r": 0.1, "g": 20.0})
+    np.testing.assert_array_equal(got, oracle.pairwise(X, Y, metric="wdtw", r=0.1, g=20.0))
+    assert oracle.wdtw_weights(100, 20.0)[0] == 0.0
+    s = X[1, 10:40].copy()
+    prof = host_profile([s], X, False, "wdtw", {"r": 0.1, "g": 20.0})
+    for q in range(4):
+        np.testing.assert_array_equal(prof[q], oracle_costs(s, X[q], "wdtw", {"r": 0.1,
+                                                                             "g": 20.0}))

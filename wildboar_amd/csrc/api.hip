@@ -151,6 +151,12 @@ struct ProfScope {
 
 enum { KERN_EUCLID = 1, KERN_LANE = 2, KERN_PLAIN = 3, KERN_BATCH = 4, KERN_SUBSEQ = 5 };
 
+// The float64 lane kernels multiply +inf (columns outside a series) by the
+// weights, so they take WDTW only when every cached weight is > 0.
+bool lane64_ok(const DeviceCtx *ctx, const double *weights) {
+    return !weights || (ctx && ctx->w64 == weights && ctx->w_min > 0.0);
+}
+
 template <typename T>
 int run_pairs(const wbx_metric *m, const PairSpec &ps, Series<T> xs, Series<T> ys,
               const T *weights, T *sq, hipStream_t st) {
@@ -163,6 +169,12 @@ int run_pairs(const wbx_metric *m, const PairSpec &ps, Series<T> xs, Series<T> y
             lane_supported(xs.len, ys.len, m->r, 0)) {
             ProfScope pf(st, KERN_LANE);
             return launch_lane_dtw(ps, xs, ys, m->r, nullptr, sq, st);
+        }
+    } else {
+        if (g_policy.load() == 0 && !xs.lens && !ys.lens && lane64_ok(ctx_current(), weights) &&
+            lane64_class(make_band(xs.len, ys.len, m->r).B, weights != nullptr)) {
+            ProfScope pf(st, KERN_LANE);
+            return launch_lane64_pairs(ps, xs, ys, m->r, weights, sq, st);
         }
     }
     ProfScope pf(st, KERN_PLAIN);
@@ -483,12 +495,13 @@ int subseq_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
     WBX_HIP_CHECK(hipMemcpyAsync(hl.data(), s_len, sizeof(int64_t) * n_s, hipMemcpyDeviceToHost, st));
     WBX_HIP_CHECK(hipMemcpyAsync(ho.data(), s_off, sizeof(int64_t) * n_s, hipMemcpyDeviceToHost, st));
     WBX_HIP_CHECK(hipStreamSynchronize(st));
-    int64_t max_len = 0, total = 0;
+    int64_t max_len = 0, min_len = t, total = 0;
     for (int64_t j = 0; j < n_s; ++j) {
         if (hl[j] < 1 || hl[j] > t)
             return fail(WBX_EINVAL, "Invalid subsequnce shape (%lld > %lld)", (long long)hl[j],
                         (long long)t);
         if (hl[j] > max_len) max_len = hl[j];
+        if (hl[j] < min_len) min_len = hl[j];
         if (ho[j] + hl[j] > total) total = ho[j] + hl[j];
     }
     wbx_metric mm = *m;
@@ -511,9 +524,17 @@ int subseq_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
         RC(stage<double>(dtype, S, 1, total, total, ws, o_s, &s64, st));
         Shapelets sh{s64.ptr, s_off, s_len, n_s};
         ProfScope pf(st, KERN_SUBSEQ);
-        return launch_subseq_exact<double>(m->kind, m->r, sh, (int)max_len,
-                                           series<double>(x64.ptr, x64.stride, t), n, w64, dist,
-                                           idx, st);
+        const Series<double> xs = series<double>(x64.ptr, x64.stride, t);
+        int lane_cap = 0;
+        if (m->kind != WBX_EUCLIDEAN && g_policy.load() == 0 && lane64_ok(ctx, w64)) {
+            RC(launch_subseq_lane64(m->r, sh, (int)min_len, (int)max_len, xs, n, w64, dist, idx,
+                                    st));
+            lane_cap = lane64_cap(w64 != nullptr);
+            if (lane64_class(2 * band_w((int)max_len, (int)max_len, m->r) - 1, w64 != nullptr))
+                return WBX_OK;
+        }
+        return launch_subseq_exact<double>(m->kind, m->r, sh, (int)max_len, xs, n, w64, dist, idx,
+                                           st, lane_cap);
     }
     return subseq_fast_impl(ctx, m, dtype, S, s_off, s_len, n_s, hl, X, n, xstr, t, total,
                             max_len, dist, idx, st);
@@ -565,8 +586,9 @@ int profile_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
     RC(stage<double>(dtype, S, 1, total, total, ws, o_s, &s64, st));
     Shapelets sh{s64.ptr, s_off, s_len, n_s};
     ProfScope pf(st, KERN_SUBSEQ);
-    return launch_profile(mm.kind, mm.r, sh, paired, (int)max_len,
-                          series<double>(x64.ptr, x64.stride, t), n, nwp, w64, prof, st);
+    return launch_profile(mm.kind, mm.r, sh, paired, (int)min_len, (int)max_len,
+                          series<double>(x64.ptr, x64.stride, t), n, nwp, w64,
+                          g_policy.load() == 0 && lane64_ok(ctx, w64), prof, st);
 }
 
 // Host-pointer wrapper helpers.

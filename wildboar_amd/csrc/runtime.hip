@@ -101,10 +101,13 @@ int cached_weights(DeviceCtx *ctx, int64_t n, double g, const double **w64, cons
         ctx->w_n = -1;
         std::vector<double> h((size_t)n);
         std::vector<float> hf((size_t)n);
+        double w_min = WBX_INF_D;
         for (int64_t i = 0; i < n; ++i) {
             h[i] = 1.0 / (1.0 + exp(-g * ((double)i - (double)n / 2.0)));
             hf[i] = (float)h[i];
+            w_min = h[i] < w_min ? h[i] : w_min;
         }
+        ctx->w_min = w_min;
         if (hipMalloc(&ctx->w64, sizeof(double) * n) != hipSuccess ||
             hipMalloc(&ctx->w32, sizeof(float) * n) != hipSuccess)
             return fail(WBX_ENOMEM, "weights allocation failed");

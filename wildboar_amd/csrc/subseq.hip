@@ -23,7 +23,8 @@ namespace wbx {
 template <typename T>
 __global__ __launch_bounds__(64) void k_subseq_exact_dtw(Shapelets sh, Series<T> xs, int64_t n,
                                                          double r, const T *__restrict__ weights,
-                                                         int cap, double *__restrict__ dist,
+                                                         int cap, int lane_cap,
+                                                         double *__restrict__ dist,
                                                          int64_t *__restrict__ idx) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *buf = reinterpret_cast<T *>(smem_raw);
@@ -35,6 +36,7 @@ __global__ __launch_bounds__(64) void k_subseq_exact_dtw(Shapelets sh, Series<T>
     const int t = xs.len;
     const T *x = xs.row(i);
     const int w = band_w(L, L, r);
+    if ((2 * w - 1 + 7) / 8 * 8 <= lane_cap) return;  // a lane64 launch takes this task
     T best = (T)WBX_INF_D;
     int64_t best_i = 0;
     for (int s = 0; s <= t - L; ++s) {
@@ -82,7 +84,8 @@ __global__ __launch_bounds__(256) void k_subseq_exact_euclid(Shapelets sh, Serie
 
 template <typename T>
 int launch_subseq_exact(int kind, double r, Shapelets sh, int max_len, Series<T> xs, int64_t n,
-                        const T *weights, double *dist, int64_t *idx, hipStream_t st) {
+                        const T *weights, double *dist, int64_t *idx, hipStream_t st,
+                        int lane_cap) {
     const int64_t nq = n * sh.n;
     if (nq <= 0) return WBX_OK;
     if (kind == WBX_EUCLIDEAN) {
@@ -99,15 +102,15 @@ int launch_subseq_exact(int kind, double r, Shapelets sh, int max_len, Series<T>
         }
         dim3 grid((unsigned)(nq < 65536 ? nq : 65536), (unsigned)((nq + 65535) / 65536));
         hipLaunchKernelGGL(k_subseq_exact_dtw<T>, grid, dim3(64), lds, st, sh, xs, n, r, weights,
-                           max_len, dist, idx);
+                           max_len, lane_cap, dist, idx);
     }
     WBX_HIP_CHECK(hipGetLastError());
     return WBX_OK;
 }
 
 template int launch_subseq_exact<double>(int, double, Shapelets, int, Series<double>, int64_t,
-                                         const double *, double *, int64_t *, hipStream_t);
+                                         const double *, double *, int64_t *, hipStream_t, int);
 template int launch_subseq_exact<float>(int, double, Shapelets, int, Series<float>, int64_t,
-                                        const float *, double *, int64_t *, hipStream_t);
+                                        const float *, double *, int64_t *, hipStream_t, int);
 
 }  // namespace wbx

@@ -145,6 +145,7 @@ struct DeviceCtx {
     double w_g = 0.0;
     double *w64 = nullptr;
     float *w32 = nullptr;
+    double w_min = 0.0;  // smallest cached weight (the lane64 kernels need > 0)
 };
 
 DeviceCtx *ctx_current();              // current device's context (created lazily)
@@ -212,13 +213,33 @@ struct Shapelets {
     const int64_t *len;   // device: length of shapelet j
     int64_t n;
 };
+// Tasks whose band class is at most lane_cap (0: none) are left to the
+// lane64 kernels.
 template <typename T>
 int launch_subseq_exact(int kind, double r, Shapelets sh, int max_len, Series<T> xs, int64_t n,
-                        const T *weights, double *dist, int64_t *idx, hipStream_t st);
+                        const T *weights, double *dist, int64_t *idx, hipStream_t st,
+                        int lane_cap = 0);
+
+// ------------------------------------------------------- float64 lane kernels
+// Register-band FP64 DTW/WDTW, bit-identical to the reference.  lane64_class
+// gives the register class (a multiple of 8) for band width B, or 0 when B is
+// too wide.  Weighted launches need every weight > 0.
+int lane64_class(int B, bool weighted);
+int lane64_cap(bool weighted);
+int launch_lane64_pairs(const PairSpec &ps, Series<double> xs, Series<double> ys, double r,
+                        const double *weights, double *sq, hipStream_t st);
+int launch_profile_lane64(double r, Shapelets sh, int paired, int min_len, int max_len,
+                          Series<double> xs, int64_t n, int64_t nwp, const double *weights,
+                          double *prof, hipStream_t st);
+int launch_subseq_lane64(double r, Shapelets sh, int min_len, int max_len, Series<double> xs,
+                         int64_t n, const double *weights, double *dist, int64_t *idx,
+                         hipStream_t st);
 // Distance profiles (subseq_match.hip): prof[q * nwp + s] = squared cost of
 // window s of series q against pattern (paired ? q : 0), +inf past its last.
-int launch_profile(int kind, double r, Shapelets sh, int paired, int max_len, Series<double> xs,
-                   int64_t n, int64_t nwp, const double *weights, double *prof, hipStream_t st);
+// lane_ok: the lane64 kernels may take the DTW/WDTW rows whose class fits.
+int launch_profile(int kind, double r, Shapelets sh, int paired, int min_len, int max_len,
+                   Series<double> xs, int64_t n, int64_t nwp, const double *weights, bool lane_ok,
+                   double *prof, hipStream_t st);
 int launch_sqrt_inplace(double *p, int64_t n, hipStream_t st);
 
 // ---------------------------------------------------------------- re-rank
