@@ -207,7 +207,9 @@ int wbx_dev_eos_lengths(int dtype, const void *X, int64_t n, int64_t stride,
                         int64_t t, int32_t *len, void *stream);
 
 /* Host-side plan: lengths and per-pair r are host arrays (r_per_pair NULL =
- * metric->r for all).  policy 0 = fastest kernels, 1 = plain kernel only.
+ * metric->r for all).  policy 0 = fastest kernels, 1 = plain kernel only,
+ * 2 = plain kernel walking every pair over the whole padded length (a
+ * measurement baseline; same results).
  * The plan owns its device buffers; free it with wbx_batch_free. */
 int wbx_batch_create(const wbx_metric *metric, int64_t n_pairs, int64_t t_pad,
                      const int32_t *len_x, const int32_t *len_y,

@@ -84,6 +84,22 @@ def test_batch_fp32_plain_policy_matches():
     assert rel_err(plan.run(Xd, Yd, graph=False).cpu().numpy(), want) <= FP32_RTOL
 
 
+def test_batch_padded_iteration_baseline_same_results():
+    # the measurement baseline walks every pair over the padded length; its
+    # results equal the plain kernel's (FP32) and the oracle's (FP64, bitwise)
+    X, Y, lx, ly, r = make_pairs(18, 40, 1, 120, 128, spikes=0.05)
+    Xd32 = torch.from_numpy(X.astype(np.float32)).cuda()
+    Yd32 = torch.from_numpy(Y.astype(np.float32)).cuda()
+    a = DtwBatch(lx, ly, 128, r, precision="fp32", plain=True).run(Xd32, Yd32, graph=False)
+    b = DtwBatch(lx, ly, 128, r, precision="fp32", plain="padded").run(Xd32, Yd32, graph=False)
+    assert torch.equal(a, b)
+    got = DtwBatch(lx, ly, 128, r, precision="fp64", plain="padded").run(
+        torch.from_numpy(X).cuda(), torch.from_numpy(Y).cuda(), graph=False)
+    np.testing.assert_array_equal(got.cpu().numpy(), oracle_batch(X, Y, lx, ly, r))
+    with pytest.raises(ValueError):
+        DtwBatch(lx, ly, 128, r, plain="sometimes")
+
+
 def test_batch_fp64_bitwise():
     X, Y, lx, ly, r = make_pairs(6, 70, 1, 150, 160)
     want = oracle_batch(X, Y, lx, ly, r)

@@ -44,7 +44,10 @@ class DtwBatch:
         width, as the reference sizes them by the array's n_timestep).
     precision : "fp32", "fp64", or "fp32_acc64" (inputs rounded to float32,
         float64 recursion).
-    plain : bool, use the plain one-pair-per-block kernel for every pair.
+    plain : bool or "padded", use the plain one-pair-per-block kernel for
+        every pair; "padded" walks every pair over the whole padded length
+        (the baseline that does not use per-pair lengths in its loops; same
+        results, for measurements).
     """
 
     def __init__(self, len_x, len_y, t_pad, r=1.0, *, metric="dtw", g=0.05,
@@ -68,11 +71,17 @@ class DtwBatch:
             raise ValueError("batched pairs need metric 'dtw' or 'wdtw'")
         m = _lib.metric_struct(kind, _lib.PRECISIONS[precision], r0,
                                _check_g(g) if kind == _lib.WDTW else 0.05)
+        if plain == "padded":
+            policy = 2
+        elif plain in (True, False):
+            policy = 1 if plain else 0
+        else:
+            raise ValueError(f"plain must be a bool or 'padded', got {plain!r}")
         self._lib = _lib.load()
         plan = ctypes.c_void_p()
         _lib.check(self._lib.wbx_batch_create(
             m, self.n_pairs, self.t_pad, lx.ctypes.data, ly.ctypes.data,
-            None if rp is None else rp.ctypes.data, 1 if plain else 0, ctypes.byref(plan)))
+            None if rp is None else rp.ctypes.data, policy, ctypes.byref(plan)))
         self._plan = plan
         cells = ctypes.c_int64()
         ns = ctypes.c_int64()

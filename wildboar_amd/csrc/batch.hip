@@ -772,7 +772,8 @@ static int plan_enqueue(BatchPlan *b, const T *X, int64_t xs, const T *Y, int64_
             xs64 = ys64 = t;
         }
         Series<double> sx{X64, xs64, (int)t, b->d_lx}, sy{Y64, ys64, (int)t, b->d_ly};
-        RC(launch_plain_dtw<double>(pl, sx, sy, b->metric.r, w64, b->d_sq_plain, st, b->d_r));
+        RC(launch_plain_dtw<double>(pl, sx, sy, b->metric.r, w64, b->d_sq_plain, st, b->d_r,
+                                    b->policy == 2));
         return launch_finish<double>(pl, b->d_sq_plain, out, 0, st);
     }
     const float *X32, *Y32;
@@ -795,7 +796,9 @@ static int plan_enqueue(BatchPlan *b, const T *X, int64_t xs, const T *Y, int64_
     }
     // the plain kernel overlaps the strip buckets on the caller's stream
     float *sqp = (float *)b->d_sq_plain;
-    if (b->n_plain > 0) RC(launch_plain_dtw<float>(pl, sx, sy, b->metric.r, w32, sqp, st, b->d_r));
+    if (b->n_plain > 0)
+        RC(launch_plain_dtw<float>(pl, sx, sy, b->metric.r, w32, sqp, st, b->d_r,
+                                   b->policy == 2));
     if (!b->bucket_R.empty()) {
         for (int a = 0; a < b->n_aux; ++a) {
             WBX_HIP_CHECK(hipEventRecord(b->ev_join[a], b->aux[a]));
@@ -834,6 +837,7 @@ int wbx_batch_create(const wbx_metric *metric, int64_t n_pairs, int64_t t_pad,
     if (metric->precision < WBX_FP64 || metric->precision > WBX_FP32_ACC64)
         return fail(WBX_EINVAL, "unknown precision %d", metric->precision);
     if (n_pairs < 1 || t_pad < 1 || t_pad > (1 << 20)) return fail(WBX_EINVAL, "bad sizes");
+    if (policy < 0 || policy > 2) return fail(WBX_EINVAL, "policy must be 0, 1 or 2");
     for (int64_t q = 0; q < n_pairs; ++q) {
         if (len_x[q] < 1 || len_x[q] > t_pad || len_y[q] < 1 || len_y[q] > t_pad)
             return fail(WBX_EINVAL, "pair %lld: lengths (%d, %d) outside [1, %lld]", (long long)q,

@@ -37,6 +37,72 @@ __global__ __launch_bounds__(64) void k_plain_dtw(PairSpec ps, Series<T> xs,
     if (threadIdx.x == 0) sq[p] = res;
 }
 
+// A measurement baseline, selected only by the batch plan's policy 2: the
+// plain kernel as it runs when its loops do not use the per-pair lengths.
+// Every pair walks all 2 t_pad - 1 anti-diagonals of the padded matrix with
+// its own band offsets; cells outside its (lx, ly) matrix are masked to +inf.
+// The results equal k_plain_dtw's.
+template <typename T>
+__global__ __launch_bounds__(64) void k_plain_dtw_padded(PairSpec ps, Series<T> xs,
+                                                         Series<T> ys, double r,
+                                                         const double *__restrict__ rp,
+                                                         const T *__restrict__ weights,
+                                                         int cap, T *__restrict__ sq) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *d0 = reinterpret_cast<T *>(smem_raw);
+    T *d1 = d0 + cap, *d2 = d0 + 2 * cap;
+    const int64_t p = (int64_t)blockIdx.x + (int64_t)blockIdx.y * gridDim.x;
+    if (p >= ps.n_pairs) return;
+    int64_t a, b;
+    decode_pair(ps, p, a, b);
+    const T *__restrict__ X = xs.row(a);
+    const T *__restrict__ Y = ys.row(b);
+    const int lx = xs.length(a), ly = ys.length(b);
+    const int tp = cap;  // the padded length of every row
+    const int w = band_w(lx, ly, rp ? rp[a] : r);
+    const int min_len = lx > ly ? lx - ly : 0;
+    const int max_len = (ly > lx ? ly - lx : 0) + w;
+    const T INF = inf_of<T>();
+    const int tid = threadIdx.x;
+    T res = INF;
+    for (int d = 0; d <= 2 * tp - 2; ++d) {
+        int ilo = d - tp + 1;
+        if (ilo < 0) ilo = 0;
+        const int ilo2 = floordiv2(d - max_len) + 1;
+        if (ilo2 > ilo) ilo = ilo2;
+        int ihi = d < tp - 1 ? d : tp - 1;
+        const int ihi2 = floordiv2(d + min_len + w - 1);
+        if (ihi2 < ihi) ihi = ihi2;
+        const int s = ilo - 1 > 0 ? ilo - 1 : 0;
+        const int e = ihi + 1 < tp - 1 ? ihi + 1 : tp - 1;
+        for (int i = s + tid; i <= e; i += blockDim.x) {
+            T c = INF;
+            const int j = d - i;
+            if (i >= ilo && i <= ihi && i < lx && j < ly) {
+                const T v = X[i] - Y[j];
+                T wt = (T)1.0;
+                if (weights) wt = (i == 0 && j > 0) ? weights[j - 1] : weights[i > j ? i - j : j - i];
+                if (i == 0 && j == 0) {
+                    c = v * v * wt;
+                } else {
+                    const T up = i > 0 ? d1[i - 1] : INF;
+                    const T left = j > 0 ? d1[i] : INF;
+                    const T dg = (i > 0 && j > 0) ? d2[i - 1] : INF;
+                    c = cmin(cmin(up, left), dg) + v * v * wt;
+                }
+            }
+            d0[i] = c;
+        }
+        __syncthreads();
+        if (d == lx + ly - 2) res = d0[lx - 1];
+        T *t = d2;
+        d2 = d1;
+        d1 = d0;
+        d0 = t;
+    }
+    if (tid == 0) sq[p] = res;
+}
+
 template <typename T>
 static int set_lds(const void *fn, size_t lds) {
     if (lds > 64 * 1024) {
@@ -49,23 +115,28 @@ static int set_lds(const void *fn, size_t lds) {
 
 template <typename T>
 int launch_plain_dtw(const PairSpec &ps, Series<T> xs, Series<T> ys, double r,
-                     const T *weights, T *sq, hipStream_t st, const double *rp) {
+                     const T *weights, T *sq, hipStream_t st, const double *rp,
+                     bool padded_iter) {
     if (ps.n_pairs <= 0) return WBX_OK;
     const int cap = xs.len;  // every row length of xs is <= xs.len
     const size_t lds = (size_t)3 * (size_t)cap * sizeof(T);
-    int rc = set_lds<T>((const void *)k_plain_dtw<T>, lds);
+    if (padded_iter && ys.len != xs.len)
+        return fail(WBX_EINVAL, "padded iteration needs one padded length");
+    auto *fn = padded_iter ? k_plain_dtw_padded<T> : k_plain_dtw<T>;
+    int rc = set_lds<T>((const void *)fn, lds);
     if (rc) return rc;
     const int64_t nb = ps.n_pairs;
     dim3 grid((unsigned)(nb < 65536 ? nb : 65536), (unsigned)((nb + 65535) / 65536));
-    hipLaunchKernelGGL(k_plain_dtw<T>, grid, dim3(64), lds, st, ps, xs, ys, r, rp, weights, cap, sq);
+    hipLaunchKernelGGL(fn, grid, dim3(64), lds, st, ps, xs, ys, r, rp, weights, cap, sq);
     WBX_HIP_CHECK(hipGetLastError());
     return WBX_OK;
 }
 
 template int launch_plain_dtw<float>(const PairSpec &, Series<float>, Series<float>, double,
-                                     const float *, float *, hipStream_t, const double *);
+                                     const float *, float *, hipStream_t, const double *, bool);
 template int launch_plain_dtw<double>(const PairSpec &, Series<double>, Series<double>, double,
-                                      const double *, double *, hipStream_t, const double *);
+                                      const double *, double *, hipStream_t, const double *,
+                                      bool);
 
 // ------------------------------------------------------------- euclidean
 // euclidean_distance (distance/_metric.pyx:654-683) with S = x, T = y: windows

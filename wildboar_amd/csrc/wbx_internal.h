@@ -158,9 +158,12 @@ int cached_weights(DeviceCtx *ctx, int64_t n, double g, const double **w64, cons
 // ---------------------------------------------------------------- launchers
 // Plain one-pair-per-block DTW (any band, any lengths, WDTW quirk): writes the
 // squared cost of each pair into sq[p].  rp (optional) gives r per x row.
+// padded_iter: the measurement baseline that walks every pair over the whole
+// padded length (xs.len == ys.len), same results.
 template <typename T>
 int launch_plain_dtw(const PairSpec &ps, Series<T> xs, Series<T> ys, double r,
-                     const T *weights, T *sq, hipStream_t st, const double *rp = nullptr);
+                     const T *weights, T *sq, hipStream_t st, const double *rp = nullptr,
+                     bool padded_iter = false);
 
 // Euclidean: one thread per pair, sequential sum in the reference's order.
 template <typename T>
