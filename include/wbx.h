@@ -37,6 +37,17 @@ extern "C" {
 #define WBX_EUCLIDEAN 0 /* EuclideanMetric,  distance/_metric.pyx:381-409   */
 #define WBX_DTW 1       /* DtwMetric,        distance/_elastic.pyx:3126-3217 */
 #define WBX_WDTW 2      /* WeightedDtwMetric, distance/_elastic.pyx:3322-3341 */
+#define WBX_DDTW 3      /* DerivativeDtwMetric, distance/_elastic.pyx:3228-3320:
+                           DTW on average_slope series (len - 2); pairwise and
+                           paired band by min(x_len, y_len), argmin by
+                           min(x_len - 2, y_len - 2); series shorter than 3 give
+                           0 (argmin: never accepted, zero outputs)            */
+#define WBX_WDDTW 4     /* WeightedDerivativeDtwMetric, _elastic.pyx:3404-3430:
+                           as WBX_DDTW with WDTW weights sized
+                           max(x_len, y_len) - 2                               */
+#define WBX_ADTW 5      /* AmercingDtwMetric, distance/_elastic.pyx:3344-3401,
+                           adtw_distance :943-1008: penalty p on the up and
+                           left moves outside row 0                            */
 
 /* Arithmetic modes. */
 #define WBX_FP64 0        /* float64, bit-identical to the reference            */
@@ -50,10 +61,11 @@ extern "C" {
 #define WBX_F32 1
 
 typedef struct wbx_metric {
-    int32_t kind;      /* WBX_EUCLIDEAN / WBX_DTW / WBX_WDTW                 */
+    int32_t kind;      /* WBX_EUCLIDEAN .. WBX_ADTW                           */
     int32_t precision; /* WBX_FP64 / WBX_FP32 / WBX_FP32_ACC64                */
-    double r;          /* Sakoe-Chiba window fraction in [0, 1] (dtw, wdtw)  */
-    double g;          /* WDTW penalty control, >= 0                          */
+    double r;          /* Sakoe-Chiba window fraction in [0, 1] (elastic)    */
+    double g;          /* WDTW / WDDTW penalty control, >= 0                  */
+    double p;          /* ADTW amercing penalty (not NaN, not -inf)           */
 } wbx_metric;
 
 /* ----------------------------------------------------------------- runtime */

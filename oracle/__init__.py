@@ -32,8 +32,14 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "lib", "liboracle.so")
 
-EUCLIDEAN, DTW, WDTW = 0, 1, 2
-_KINDS = {"euclidean": EUCLIDEAN, "dtw": DTW, "wdtw": WDTW}
+EUCLIDEAN, DTW, WDTW, DDTW, WDDTW, ADTW = 0, 1, 2, 3, 4, 5
+_KINDS = {"euclidean": EUCLIDEAN, "dtw": DTW, "wdtw": WDTW, "ddtw": DDTW, "wddtw": WDDTW,
+          "adtw": ADTW}
+
+
+def _gp(metric, g, p):
+    """The metric's second parameter: the penalty p for adtw, else g."""
+    return float(p) if metric == "adtw" else float(g)
 
 _lib = None
 
@@ -82,6 +88,10 @@ def _load():
     lib.orc_dtw_grad.argtypes = [p, i64, p, i64, d, p, p, p]
     lib.orc_argmin_subsequence.restype = i32
     lib.orc_argmin_subsequence.argtypes = [i32, d, d, p, p, p, i64, p, i64, i64, i64, i64, p, p]
+    lib.orc_average_slope.restype = None
+    lib.orc_average_slope.argtypes = [p, i64, p]
+    lib.orc_adtw_distance.restype = d
+    lib.orc_adtw_distance.argtypes = [p, i64, p, i64, i64, p, p, d, d]
     lib.orc_subsequence_matches.restype = i64
     lib.orc_subsequence_matches.argtypes = [i32, d, d, p, i64, p, i64, d, p, p]
     _lib = lib
@@ -134,16 +144,37 @@ def dtw_distance(x, y, band, weights=None, min_dist=np.inf):
         cost.ctypes.data, cost_prev.ctypes.data, _ptr(weights), float(min_dist))
 
 
-def metric_distance(x, y, metric="dtw", r=1.0, g=0.05, weight_n=0):
+def average_slope(q):
+    """average_slope of a 1-D series: len - 2 values."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    out = np.empty(max(q.shape[0] - 2, 0))
+    _load().orc_average_slope(q.ctypes.data, q.shape[0], out.ctypes.data)
+    return out
+
+
+def adtw_distance(x, y, band, p, min_dist=np.inf):
+    """Raw adtw_distance (squared cost) with an integer band."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.ascontiguousarray(y, dtype=np.float64)
-    return _load().orc_metric_distance(_kind(metric), float(r), float(g),
+    n = max(x.shape[0], y.shape[0]) + 1
+    cost = np.empty(n)
+    cost_prev = np.empty(n)
+    return _load().orc_adtw_distance(x.ctypes.data, x.shape[0], y.ctypes.data, y.shape[0],
+                                     int(band), cost.ctypes.data, cost_prev.ctypes.data,
+                                     float(p), float(min_dist))
+
+
+def metric_distance(x, y, metric="dtw", r=1.0, g=0.05, weight_n=0, p=1.0):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    return _load().orc_metric_distance(_kind(metric), float(r), _gp(metric, g, p),
                                        x.ctypes.data, x.shape[0],
                                        y.ctypes.data, y.shape[0], int(weight_n))
 
 
-def pairwise(X, Y=None, metric="dtw", r=1.0, g=0.05, dim=0, n_threads=None):
+def pairwise(X, Y=None, metric="dtw", r=1.0, g=0.05, dim=0, n_threads=None, p=1.0):
     n_threads = n_threads or default_threads()
+    g = _gp(metric, g, p)
     X = _ts(X)
     lib = _load()
     if Y is None:
@@ -162,9 +193,10 @@ def pairwise(X, Y=None, metric="dtw", r=1.0, g=0.05, dim=0, n_threads=None):
     return out
 
 
-def paired(X, Y, metric="dtw", r=1.0, g=0.05, dim=0, n_threads=None):
+def paired(X, Y, metric="dtw", r=1.0, g=0.05, dim=0, n_threads=None, p=1.0):
     """out[i] = D(Y[i], X[i]) -- the reference's argument swap."""
     n_threads = n_threads or default_threads()
+    g = _gp(metric, g, p)
     X = _ts(X)
     Y = _ts(Y)
     out = np.empty(X.shape[0])
@@ -177,8 +209,9 @@ def paired(X, Y, metric="dtw", r=1.0, g=0.05, dim=0, n_threads=None):
 
 
 def argmin(X, Y, k=1, metric="dtw", r=1.0, g=0.05, dim=0, lower_bound=None,
-           n_threads=None):
+           n_threads=None, p=1.0):
     n_threads = n_threads or default_threads()
+    g = _gp(metric, g, p)
     X = _ts(X)
     Y = _ts(Y)
     k = min(k, Y.shape[0])

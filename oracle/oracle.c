@@ -24,6 +24,11 @@
  *     pairwise == pairwise_subsequence.T at equal lengths.
  *   - alignment/backtrack/gradient: central finite differences at tie-free
  *     points, and the r=0 closed form dD/dx_i = (x_i - y_i) / D.
+ *   - average_slope: hand-computed slopes of short series, linear series give
+ *     a constant slope, quadratics the central difference.
+ *   - adtw_distance: penalty 0 == dtw_distance bitwise, penalty +inf == the
+ *     sequential squared euclidean sum at equal lengths, brute force with an
+ *     explicit penalty on small shapes.
  */
 #include <math.h>
 #include <pthread.h>
@@ -34,6 +39,10 @@
 #define ORC_EUCLIDEAN 0
 #define ORC_DTW 1
 #define ORC_WDTW 2
+#define ORC_DDTW 3
+#define ORC_WDDTW 4
+#define ORC_ADTW 5
+#define ORC_DERIVATIVE(k) ((k) == ORC_DDTW || (k) == ORC_WDDTW)
 
 static inline int64_t imax(int64_t a, int64_t b) { return a > b ? a : b; }
 static inline int64_t imin(int64_t a, int64_t b) { return a < b ? a : b; }
@@ -100,6 +109,61 @@ double orc_dtw_distance(const double *X, int64_t x_length, const double *Y,
     return cost_prev[y_length - 1];
 }
 
+/* average_slope, distance/_elastic.pyx:3220-3225: len - 2 values. */
+void orc_average_slope(const double *q, int64_t len, double *d) {
+    int64_t j = 0;
+    for (int64_t i = 1; i < len - 1; i++) {
+        d[j] = ((q[i] - q[i - 1]) + ((q[i + 1] - q[i - 1]) / 2)) / 2;
+        j++;
+    }
+}
+
+/* adtw_distance, distance/_elastic.pyx:943-1008: the dtw_distance band and
+ * row 0 prefix sum (no penalty); later rows add the penalty to the up and
+ * left moves.  Returns the squared cost. */
+double orc_adtw_distance(const double *X, int64_t x_length, const double *Y,
+                         int64_t y_length, int64_t r, double *cost,
+                         double *cost_prev, double penalty, double min_dist) {
+    int64_t i, j, j_start, j_stop;
+    double x, y, z, v, min_cost, prev_cost, *tmp;
+    int64_t max_len = imax(0, y_length - x_length) + r;
+    int64_t min_len = imax(0, x_length - y_length);
+
+    v = X[0] - Y[0];
+    cost_prev[0] = v * v;
+    for (i = 1; i < imin(y_length, max_len); i++) {
+        v = X[0] - Y[i];
+        cost_prev[i] = cost_prev[i - 1] + v * v;
+    }
+    if (max_len < y_length) cost_prev[max_len] = INFINITY;
+
+    for (i = 1; i < x_length; i++) {
+        j_start = imax(0, i - min_len - r + 1);
+        j_stop = imin(y_length, i + max_len);
+        if (j_start > 0) cost[j_start - 1] = INFINITY;
+        prev_cost = INFINITY;
+        min_cost = INFINITY;
+        for (j = j_start; j < j_stop; j++) {
+            x = cost_prev[j] + penalty;
+            if (j > 0) {
+                y = prev_cost + penalty;
+                z = cost_prev[j - 1];
+            } else {
+                y = INFINITY;
+                z = INFINITY;
+            }
+            v = X[i] - Y[j];
+            cost[j] = dmin(dmin(x, y), z) + v * v;
+            if (cost[j] < min_cost) min_cost = cost[j];
+            prev_cost = cost[j];
+        }
+        if (min_cost > min_dist) return INFINITY;
+        if (j_stop < y_length) cost[j_stop] = INFINITY;
+        tmp = cost; cost = cost_prev; cost_prev = tmp;
+    }
+    return cost_prev[y_length - 1];
+}
+
 /* euclidean_distance, distance/_metric.pyx:654-683 (partial-sum abandon). */
 double orc_euclidean_distance(const double *S, int64_t s_length,
                               const double *T, int64_t t_length,
@@ -144,44 +208,91 @@ double orc_dtw_subsequence_distance(const double *S, int64_t s_length,
  * (:3322-3341), EuclideanMetric (_metric.pyx:381-409).  A Metric owns its
  * scratch rows (reset) and per-array weights.
  * ---------------------------------------------------------------------- */
+/* g is the amercing penalty p for ORC_ADTW (AmercingDtwMetric,
+ * _elastic.pyx:3344-3401). */
 typedef struct {
     int kind;
     double r, g;
     double *cost, *cost_prev, *weights;
+    double *d_x, *d_y; /* derivative kinds: average_slope buffers */
 } orc_metric;
 
+/* DtwMetric.reset (_elastic.pyx:3148-3160), WeightedDtwMetric.reset
+ * (:3334-3341), DerivativeDtwMetric.reset (:3245-3254) and
+ * WeightedDerivativeDtwMetric.reset (:3410-3430, weights sized
+ * max(tx - 2, ty - 2)). */
 static int metric_reset(orc_metric *m, int64_t tx, int64_t ty) {
     int64_t n = imax(tx, ty);
     m->cost = (double *)malloc(sizeof(double) * (size_t)(n + 1));
     m->cost_prev = (double *)malloc(sizeof(double) * (size_t)(n + 1));
     m->weights = NULL;
+    m->d_x = m->d_y = NULL;
+    int ok = m->cost && m->cost_prev;
     if (m->kind == ORC_WDTW) {
         m->weights = (double *)malloc(sizeof(double) * (size_t)n);
         if (m->weights) orc_wdtw_weights(n, m->g, m->weights);
+        ok = ok && m->weights;
     }
-    return (m->cost && m->cost_prev && (m->kind != ORC_WDTW || m->weights)) ? 0 : -1;
+    if (ORC_DERIVATIVE(m->kind)) {
+        m->d_x = (double *)malloc(sizeof(double) * (size_t)imax(tx, 1));
+        m->d_y = (double *)malloc(sizeof(double) * (size_t)imax(ty, 1));
+        ok = ok && m->d_x && m->d_y;
+        if (m->kind == ORC_WDDTW && n - 2 >= 1) {
+            m->weights = (double *)malloc(sizeof(double) * (size_t)(n - 2));
+            if (m->weights) orc_wdtw_weights(n - 2, m->g, m->weights);
+            ok = ok && m->weights;
+        }
+    }
+    return ok ? 0 : -1;
 }
 
 static void metric_free(orc_metric *m) {
-    free(m->cost); free(m->cost_prev); free(m->weights);
-    m->cost = m->cost_prev = m->weights = NULL;
+    free(m->cost); free(m->cost_prev); free(m->weights); free(m->d_x); free(m->d_y);
+    m->cost = m->cost_prev = m->weights = m->d_x = m->d_y = NULL;
 }
 
-/* Metric._distance */
+/* Metric._distance; the derivative kinds: DerivativeDtwMetric._distance
+ * (_elastic.pyx:3256-3281), band from min(x_len, y_len) of the series
+ * themselves; ADTW: AmercingDtwMetric._distance (:3354-3373). */
 static double metric_distance(orc_metric *m, const double *x, int64_t lx,
                               const double *y, int64_t ly) {
     if (m->kind == ORC_EUCLIDEAN)
         return orc_euclidean_distance(x, lx, y, ly, INFINITY, NULL);
+    if (ORC_DERIVATIVE(m->kind)) {
+        if (imin(lx, ly) < 3) return 0;
+        orc_average_slope(x, lx, m->d_x);
+        orc_average_slope(y, ly, m->d_y);
+        return sqrt(orc_dtw_distance(m->d_x, lx - 2, m->d_y, ly - 2,
+                                     orc_compute_r(imin(lx, ly), m->r), m->cost,
+                                     m->cost_prev, m->weights, INFINITY));
+    }
+    if (m->kind == ORC_ADTW)
+        return sqrt(orc_adtw_distance(x, lx, y, ly, orc_compute_r(imin(lx, ly), m->r),
+                                      m->cost, m->cost_prev, m->g, INFINITY));
     return sqrt(orc_dtw_distance(x, lx, y, ly, orc_compute_r(imin(lx, ly), m->r),
                                  m->cost, m->cost_prev, m->weights, INFINITY));
 }
 
-/* Metric._eadistance: True iff d < *min_dist, then *min_dist = d. */
+/* Metric._eadistance: True iff d < *min_dist, then *min_dist = d.  The
+ * derivative kinds band by min(x_len - 2, y_len - 2) here
+ * (DerivativeDtwMetric._eadistance, _elastic.pyx:3283-3320) and never accept
+ * series shorter than 3. */
 static int metric_eadistance(orc_metric *m, const double *x, int64_t lx,
                              const double *y, int64_t ly, double *min_dist) {
     double dist;
     if (m->kind == ORC_EUCLIDEAN)
         dist = orc_euclidean_distance(x, lx, y, ly, min_dist[0] * min_dist[0], NULL);
+    else if (ORC_DERIVATIVE(m->kind)) {
+        if (imin(lx, ly) < 3) return 0;
+        orc_average_slope(x, lx, m->d_x);
+        orc_average_slope(y, ly, m->d_y);
+        dist = sqrt(orc_dtw_distance(m->d_x, lx - 2, m->d_y, ly - 2,
+                                     orc_compute_r(imin(lx - 2, ly - 2), m->r), m->cost,
+                                     m->cost_prev, m->weights, min_dist[0] * min_dist[0]));
+    } else if (m->kind == ORC_ADTW)
+        dist = sqrt(orc_adtw_distance(x, lx, y, ly, orc_compute_r(imin(lx, ly), m->r),
+                                      m->cost, m->cost_prev, m->g,
+                                      min_dist[0] * min_dist[0]));
     else
         dist = sqrt(orc_dtw_distance(x, lx, y, ly, orc_compute_r(imin(lx, ly), m->r),
                                      m->cost, m->cost_prev, m->weights,
@@ -197,7 +308,7 @@ static int metric_eadistance(orc_metric *m, const double *x, int64_t lx,
 double orc_metric_distance(int kind, double r, double g, const double *x,
                            int64_t lx, const double *y, int64_t ly,
                            int64_t weight_n) {
-    orc_metric m = {kind, r, g, NULL, NULL, NULL};
+    orc_metric m = {kind, r, g, NULL, NULL, NULL, NULL, NULL};
     int64_t n = imax(lx, ly);
     if (weight_n > n) n = weight_n;
     if (metric_reset(&m, n, n) != 0) { metric_free(&m); return NAN; }
@@ -286,7 +397,7 @@ typedef struct {
 
 static void *run_job(void *arg) {
     orc_job *jb = (orc_job *)arg;
-    orc_metric m = {jb->kind, jb->r, jb->g, NULL, NULL, NULL};
+    orc_metric m = {jb->kind, jb->r, jb->g, NULL, NULL, NULL, NULL, NULL};
     int64_t i, j;
     jb->status = 0;
     if (jb->op == 4) {
@@ -614,7 +725,7 @@ int orc_argmin_subsequence(int kind, double r, double g, const double *S,
                            const int64_t *s_off, const int64_t *s_len, int64_t n,
                            const double *X, int64_t dims, int64_t t, int64_t dim,
                            int64_t k, int64_t *idx, double *dist) {
-    orc_metric m = {kind, r, g, NULL, NULL, NULL};
+    orc_metric m = {kind, r, g, NULL, NULL, NULL, NULL, NULL};
     if (metric_reset(&m, t, t) != 0) { metric_free(&m); return -1; }
     orc_heap_el *heap = (orc_heap_el *)malloc(sizeof(orc_heap_el) * (size_t)k);
     if (!heap) { metric_free(&m); return -1; }

@@ -1,0 +1,108 @@
+"""GPU parity of the derivative and amercing metrics (ddtw, wddtw, adtw)
+through pairwise_distance / paired_distance / argmin_distance against the
+float64 oracle.  Bars: FP64 bit-identical; FP32 max relative error 1e-5 with
+argmin indices and distances exact; FP32_ACC64 bit-identical on float32-
+rounded inputs."""
+
+import numpy as np
+import pytest
+
+import oracle
+from wildboar_amd import _lib
+from wildboar_amd import distance as wd
+
+pytestmark = pytest.mark.gpu
+
+FP32_RTOL = 1e-5
+CASES = [("ddtw", {"r": 0.1}), ("ddtw", {"r": 1.0}), ("ddtw", {"r": 0.25}),
+         ("wddtw", {"r": 0.1, "g": 0.05}), ("wddtw", {"r": 1.0, "g": 0.5}),
+         ("adtw", {"r": 0.1, "p": 1.0}), ("adtw", {"r": 1.0, "p": 0.1}),
+         ("adtw", {"r": 0.3, "p": 0.0}), ("adtw", {"r": 0.2, "p": float("inf")})]
+
+
+def rw(seed, n, t):
+    rng = np.random.default_rng(seed)
+    x = np.cumsum(rng.standard_normal((n, t)), axis=1)
+    return (x - x.mean(axis=1, keepdims=True)) / x.std(axis=1, keepdims=True)
+
+
+def okw(params):
+    return {"r": params.get("r", 1.0), "g": params.get("g", 0.05), "p": params.get("p", 1.0)}
+
+
+def setup_module(module):
+    _lib.set_policy(0)
+
+
+@pytest.mark.parametrize("metric,params", CASES)
+def test_pairwise_paired_fp64_bitwise(metric, params):
+    X = rw(80, 30, 96)
+    Y = rw(81, 25, 96)
+    got = wd.pairwise_distance(X, Y, metric=metric, metric_params=params)
+    np.testing.assert_array_equal(got, oracle.pairwise(X, Y, metric=metric, **okw(params)))
+    got = wd.pairwise_distance(X, metric=metric, metric_params=params)
+    np.testing.assert_array_equal(got, oracle.pairwise(X, None, metric=metric, **okw(params)))
+    got = wd.paired_distance(X[:25], Y, metric=metric, metric_params=params)
+    np.testing.assert_array_equal(got, oracle.paired(X[:25], Y, metric=metric, **okw(params)))
+
+
+@pytest.mark.parametrize("metric,params", CASES[:7])
+def test_pairwise_fp64_plain_kernel_and_unequal_lengths(metric, params):
+    X = rw(82, 9, 70)
+    Y = rw(83, 8, 110)
+    want = oracle.pairwise(X, Y, metric=metric, **okw(params))
+    np.testing.assert_array_equal(wd.pairwise_distance(X, Y, metric=metric, metric_params=params),
+                                  want)
+    _lib.set_policy(1)
+    try:
+        got = wd.pairwise_distance(X, Y, metric=metric, metric_params=params)
+    finally:
+        _lib.set_policy(0)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("metric,params", CASES)
+@pytest.mark.parametrize("k", [1, 4])
+def test_argmin_fp64_and_fp32_exact(metric, params, k):
+    X = rw(84, 20, 64)
+    Y = rw(85, 40, 64)
+    oi, od = oracle.argmin(X, Y, k=k, metric=metric, **okw(params))
+    for precision in ("fp64", "fp32"):
+        gi, gd = wd.argmin_distance(X, Y, k=k, metric=metric, metric_params=params,
+                                    return_distance=True, precision=precision)
+        np.testing.assert_array_equal(gi, oi)
+        np.testing.assert_array_equal(gd, od)
+
+
+@pytest.mark.parametrize("metric,params", CASES)
+def test_fp32_error_and_acc64_bitwise(metric, params):
+    X = rw(86, 24, 128)
+    Y = rw(87, 20, 128)
+    want = oracle.pairwise(X, Y, metric=metric, **okw(params))
+    got = wd.pairwise_distance(X, Y, metric=metric, metric_params=params, precision="fp32")
+    err = np.abs(got - want) / np.maximum(np.abs(want), 1e-1)
+    assert np.isfinite(got).all() and err.max() <= FP32_RTOL, float(err.max())
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)  # noqa: E731
+    got = wd.pairwise_distance(X, Y, metric=metric, metric_params=params,
+                               precision="fp32_acc64")
+    np.testing.assert_array_equal(got, oracle.pairwise(f32(X), f32(Y), metric=metric,
+                                                       **okw(params)))
+
+
+def test_derivative_short_series_and_validation():
+    X = rw(88, 4, 2)
+    Y = rw(89, 3, 2)
+    assert (wd.pairwise_distance(X, Y, metric="ddtw") == 0).all()
+    assert (wd.paired_distance(X[:3], Y, metric="wddtw") == 0).all()
+    idx, dist = wd.argmin_distance(X, Y, k=2, metric="ddtw", return_distance=True)
+    assert (idx == 0).all() and (dist == 0).all()
+    X = rw(90, 4, 30)
+    with pytest.raises(NotImplementedError):
+        wd.pairwise_subsequence_distance([X[0, :5]], X, metric="adtw")
+    with pytest.raises(ValueError):
+        wd.pairwise_distance(X, metric="adtw", metric_params={"p": float("-inf")})
+    with pytest.raises(TypeError):
+        wd.pairwise_distance(X, metric="adtw", metric_params={"g": 0.1})
+    # WeightedDerivativeDtwMetric takes any float g
+    got = wd.pairwise_distance(X, X[:2], metric="wddtw", metric_params={"g": -0.2})
+    np.testing.assert_array_equal(got, oracle.pairwise(X, X[:2], metric="wddtw", g=-0.2))

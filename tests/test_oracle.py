@@ -377,3 +377,100 @@ def test_argmin_subsequence_euclidean_is_dtw_r0():
     b = oracle.argmin_subsequence(shapelets, X, k=4, metric="dtw", r=0.0)
     np.testing.assert_array_equal(a[0], b[0])
     np.testing.assert_array_equal(a[1], b[1])
+
+
+# ------------------------------------------------------ derivative / amercing
+def brute_adtw(x, y, r, p):
+    """Full-matrix ADTW from the definition: the dtw_distance band, row 0 a
+    plain prefix sum, later cells min(up + p, left + p, diag) + (x_i - y_j)^2."""
+    lx, ly = len(x), len(y)
+    w = band_w(lx, ly, r)
+    hi = max(0, lx - ly) + w - 1
+    lo = -(max(0, ly - lx) + w - 1)
+    C = [[INF] * ly for _ in range(lx)]
+    for i in range(lx):
+        for j in range(ly):
+            if not (lo <= i - j <= hi):
+                continue
+            v = float(x[i]) - float(y[j])
+            if i == 0:
+                C[i][j] = v * v if j == 0 else C[0][j - 1] + v * v
+                continue
+            up = C[i - 1][j] + p
+            left = (C[i][j - 1] if j > 0 else INF) + p
+            diag = C[i - 1][j - 1] if j > 0 else INF
+            m = left if left < up else up
+            m = diag if diag < m else m
+            C[i][j] = m + v * v
+    return C[lx - 1][ly - 1]
+
+
+def test_average_slope_known_answers():
+    # by hand: ((2-1) + (4-1)/2)/2 = 1.25, ((4-2) + (7-2)/2)/2 = 2.25, ...
+    assert oracle.average_slope([1, 2, 4, 7, 11]).tolist() == [1.25, 2.25, 3.25]
+    assert oracle.average_slope(3.0 + 0.5 * np.arange(9)).tolist() == [0.5] * 7
+    i = np.arange(1, 11)
+    assert oracle.average_slope(np.arange(12.0) ** 2).tolist() == (2.0 * i - 0.5).tolist()
+    assert oracle.average_slope([1.0, 2.0]).size == 0
+
+
+def test_adtw_brute_force_and_closed_forms():
+    rng = np.random.default_rng(31)
+    for lx in range(1, 8):
+        for ly in range(1, 8):
+            x = rng.standard_normal(lx)
+            y = rng.standard_normal(ly)
+            for r in (0.0, 0.4, 1.0):
+                band = oracle.compute_r(min(lx, ly), r)
+                for p in (0.0, 0.25, 1.5, INF):
+                    assert oracle.adtw_distance(x, y, band, p) == brute_adtw(x, y, r, p), (lx, ly, r, p)
+                # penalty 0 is DTW, bit for bit
+                assert oracle.adtw_distance(x, y, band, 0.0) == oracle.dtw_distance(x, y, band)
+    # penalty +inf at equal lengths: only the diagonal, the squared euclidean sum
+    for t in (1, 5, 33):
+        x = rng.standard_normal(t)
+        y = rng.standard_normal(t)
+        c = 0.0
+        for a, b in zip(x, y):
+            v = a - b
+            c += v * v
+        for r in (0.0, 0.3, 1.0):
+            assert oracle.adtw_distance(x, y, oracle.compute_r(t, r), INF) == c
+    X = rw(rng, 4, 20)
+    D = oracle.pairwise(X, X[::-1].copy(), metric="adtw", r=0.5, p=0.7)
+    assert D[1, 2] == math.sqrt(brute_adtw(X[1], X[::-1][2], 0.5, 0.7))
+
+
+def test_ddtw_is_dtw_on_slopes_with_the_band_quirk():
+    rng = np.random.default_rng(32)
+    X = rw(rng, 5, 12)
+    Y = rw(rng, 6, 12)
+    r = 0.25  # band 3 by the series' lengths (pairwise), 2 by the slopes' (argmin)
+    assert oracle.compute_r(12, r) == 3 and oracle.compute_r(10, r) == 2
+    D = oracle.pairwise(X, Y, metric="ddtw", r=r)
+    for i in range(5):
+        for j in range(6):
+            want = oracle.dtw_distance(oracle.average_slope(X[i]), oracle.average_slope(Y[j]), 3)
+            assert D[i, j] == math.sqrt(want)
+    idx, dist = oracle.argmin(X, Y, k=3, metric="ddtw", r=r)
+    for i in range(5):
+        row = np.array([math.sqrt(oracle.dtw_distance(oracle.average_slope(X[i]),
+                                                      oracle.average_slope(Y[j]), 2))
+                        for j in range(6)])
+        ri, rd = oracle.argmin_replay(row, 3)
+        assert idx[i].tolist() == ri.tolist() and dist[i].tolist() == rd.tolist()
+    # WDDTW: WDTW on the slopes, weights sized by the slope length
+    W = oracle.pairwise(X, Y, metric="wddtw", r=r, g=0.3)
+    wts = oracle.wdtw_weights(10, 0.3)
+    want = oracle.dtw_distance(oracle.average_slope(X[2]), oracle.average_slope(Y[4]), 3,
+                               weights=wts)
+    assert W[2, 4] == math.sqrt(want)
+    # series shorter than 3 give 0; argmin never accepts them (zero outputs)
+    assert (oracle.pairwise(X[:, :2], Y[:, :2], metric="ddtw") == 0).all()
+    idx, dist = oracle.argmin(X[:, :2], Y[:, :2], k=2, metric="ddtw")
+    assert (idx == 0).all() and (dist == 0).all()
+    # straight lines: constant slopes b and d, the diagonal path of t - 2 cells
+    t = 40
+    i = np.arange(t, dtype=float)
+    d = oracle.metric_distance(1.0 + 0.75 * i, -2.0 + 0.25 * i, "ddtw", r=0.1)
+    assert d == pytest.approx(0.5 * math.sqrt(t - 2), rel=1e-15)

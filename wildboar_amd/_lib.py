@@ -14,11 +14,12 @@ LIB_PATH = os.path.join(_HERE, "lib", "libwbx.so")
 CSRC = os.path.join(_HERE, "csrc")
 
 WBX_OK, WBX_EINVAL, WBX_ENOMEM, WBX_EDEVICE, WBX_EUNSUPPORTED = 0, -1, -2, -3, -4
-EUCLIDEAN, DTW, WDTW = 0, 1, 2
+EUCLIDEAN, DTW, WDTW, DDTW, WDDTW, ADTW = 0, 1, 2, 3, 4, 5
 FP64, FP32, FP32_ACC64 = 0, 1, 2
 F64, F32 = 0, 1
 
-KINDS = {"euclidean": EUCLIDEAN, "dtw": DTW, "wdtw": WDTW}
+KINDS = {"euclidean": EUCLIDEAN, "dtw": DTW, "wdtw": WDTW, "ddtw": DDTW, "wddtw": WDDTW,
+         "adtw": ADTW}
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "fp32_acc64": FP32_ACC64}
 
 EXPORTS = (
@@ -35,7 +36,7 @@ EXPORTS = (
 
 class WbxMetric(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("precision", ctypes.c_int32),
-                ("r", ctypes.c_double), ("g", ctypes.c_double)]
+                ("r", ctypes.c_double), ("g", ctypes.c_double), ("p", ctypes.c_double)]
 
 
 class WbxError(RuntimeError):
@@ -129,8 +130,8 @@ def check(rc):
     raise WbxError(msg or f"wbx call failed with status {rc}")
 
 
-def metric_struct(kind, precision, r=1.0, g=0.05):
-    return WbxMetric(int(kind), int(precision), float(r), float(g))
+def metric_struct(kind, precision, r=1.0, g=0.05, p=1.0):
+    return WbxMetric(int(kind), int(precision), float(r), float(g), float(p))
 
 
 def set_policy(policy):

@@ -49,14 +49,28 @@ struct Carve {
 
 int check_metric(const wbx_metric *m) {
     if (!m) return fail(WBX_EINVAL, "metric is NULL");
-    if (m->kind < WBX_EUCLIDEAN || m->kind > WBX_WDTW)
+    if (m->kind < WBX_EUCLIDEAN || m->kind > WBX_ADTW)
         return fail(WBX_EINVAL, "unknown metric kind %d", m->kind);
     if (m->precision < WBX_FP64 || m->precision > WBX_FP32_ACC64)
         return fail(WBX_EINVAL, "unknown precision %d", m->precision);
     if (m->kind != WBX_EUCLIDEAN && !(m->r >= 0.0 && m->r <= 1.0))
         return fail(WBX_EINVAL, "r == %g, must be in [0, 1]", m->r);
+    // WeightedDtwMetric checks g >= 0 (_elastic.pyx:3326); the derivative
+    // variant takes any float (:3406-3408)
     if (m->kind == WBX_WDTW && !(m->g >= 0.0))
         return fail(WBX_EINVAL, "g == %g, must be >= 0", m->g);
+    // -inf would meet +inf (a cell outside the band) as NaN
+    if (m->kind == WBX_ADTW && !(m->p > -WBX_INF_D))
+        return fail(WBX_EINVAL, "p == %g, must be a number above -inf", m->p);
+    return WBX_OK;
+}
+
+// Subsequence scans and profiles serve euclidean, dtw and wdtw.
+int check_subsequence_metric(const wbx_metric *m) {
+    int rc = check_metric(m);
+    if (rc) return rc;
+    if (m->kind > WBX_WDTW)
+        return fail(WBX_EUNSUPPORTED, "metric kind %d is not served for subsequences", m->kind);
     return WBX_OK;
 }
 
@@ -95,6 +109,48 @@ int acc64_prepare(DeviceCtx *ctx, wbx_metric *m, int dtype, std::initializer_lis
         *a.stride = a.len;
         dst += (size_t)a.n * (size_t)a.len;
     }
+    return WBX_OK;
+}
+
+// DDTW / WDDTW (DerivativeDtwMetric, distance/_elastic.pyx:3228-3320): each
+// input becomes its average slope (float64, from the values as given, or as
+// rounded by acc64_prepare), of length len - 2, and the metric becomes DTW /
+// WDTW on those series (WDDTW's weights are sized by the slope lengths,
+// :3415-3430).  The caller's element type becomes float64.  Needs every
+// length >= 3.  Call after acc64_prepare.
+struct DvArray {
+    const void **ptr;
+    int64_t n;
+    int64_t *len;
+    int64_t *stride;
+};
+
+int deriv_prepare(DeviceCtx *ctx, wbx_metric *m, int *dtype, std::initializer_list<DvArray> arrays,
+                  hipStream_t st) {
+    if (m->kind != WBX_DDTW && m->kind != WBX_WDDTW) return WBX_OK;
+    m->kind = m->kind == WBX_DDTW ? WBX_DTW : WBX_WDTW;
+    size_t total = 0;
+    for (const DvArray &a : arrays)
+        if (a.ptr && *a.ptr) {
+            if (*a.len < 3) return fail(WBX_EINVAL, "derivative metrics need length >= 3");
+            total += (size_t)a.n * (size_t)(*a.len - 2);
+        }
+    void *wsp = nullptr;
+    int rc = ws_dv_reserve(ctx, total * sizeof(double), &wsp);
+    if (rc) return rc;
+    double *dst = (double *)wsp;
+    for (const DvArray &a : arrays) {
+        if (!a.ptr || !*a.ptr) continue;
+        rc = *dtype == WBX_F64
+                 ? launch_slope_rows<double>((const double *)*a.ptr, a.n, *a.len, *a.stride, dst, st)
+                 : launch_slope_rows<float>((const float *)*a.ptr, a.n, *a.len, *a.stride, dst, st);
+        if (rc) return rc;
+        *a.ptr = dst;
+        *a.len -= 2;
+        *a.stride = *a.len;
+        dst += (size_t)a.n * (size_t)*a.len;
+    }
+    *dtype = WBX_F64;
     return WBX_OK;
 }
 
@@ -157,28 +213,32 @@ bool lane64_ok(const DeviceCtx *ctx, const double *weights) {
     return !weights || (ctx && ctx->w64 == weights && ctx->w_min > 0.0);
 }
 
+// Squared costs of the pairs of ps.  m->kind is EUCLIDEAN, DTW, WDTW or ADTW
+// (the derivative kinds arrive here as DTW / WDTW on slopes); w_fixed > 0
+// replaces every pair's band width.
 template <typename T>
 int run_pairs(const wbx_metric *m, const PairSpec &ps, Series<T> xs, Series<T> ys,
-              const T *weights, T *sq, hipStream_t st) {
+              const T *weights, T *sq, hipStream_t st, int w_fixed = 0) {
     if (m->kind == WBX_EUCLIDEAN) {
         ProfScope pf(st, KERN_EUCLID);
         return launch_euclid_pairs<T>(ps, xs, ys, sq, st);
     }
+    const double *pen = m->kind == WBX_ADTW ? &m->p : nullptr;
     if constexpr (std::is_same<T, float>::value) {
         if (g_policy.load() == 0 && !weights && !xs.lens && !ys.lens &&
-            lane_supported(xs.len, ys.len, m->r, 0)) {
+            lane_supported(xs.len, ys.len, m->r, 0, w_fixed)) {
             ProfScope pf(st, KERN_LANE);
-            return launch_lane_dtw(ps, xs, ys, m->r, nullptr, sq, st);
+            return launch_lane_dtw(ps, xs, ys, m->r, nullptr, sq, st, w_fixed, pen);
         }
     } else {
         if (g_policy.load() == 0 && !xs.lens && !ys.lens && lane64_ok(ctx_current(), weights) &&
-            lane64_class(make_band(xs.len, ys.len, m->r).B, weights != nullptr)) {
+            lane64_class(make_band(xs.len, ys.len, m->r, w_fixed).B, weights != nullptr)) {
             ProfScope pf(st, KERN_LANE);
-            return launch_lane64_pairs(ps, xs, ys, m->r, weights, sq, st);
+            return launch_lane64_pairs(ps, xs, ys, m->r, weights, sq, st, w_fixed, pen);
         }
     }
     ProfScope pf(st, KERN_PLAIN);
-    return launch_plain_dtw<T>(ps, xs, ys, m->r, weights, sq, st);
+    return launch_plain_dtw<T>(ps, xs, ys, m->r, weights, sq, st, nullptr, false, w_fixed, pen);
 }
 
 // Inputs of element type `dtype` brought to the compute type T (dense copy
@@ -222,7 +282,7 @@ size_t stage_bytes(int dtype, bool want_double, int64_t n, int64_t len) {
 template <typename T>
 int pairwise_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *X, int64_t nx,
                   int64_t xstr, int64_t tx, const void *Y, int64_t ny, int64_t ystr, int64_t ty,
-                  bool singleton, double *out, hipStream_t st) {
+                  bool singleton, double *out, hipStream_t st, int w_fixed) {
     PairSpec ps{};
     if (singleton) {
         ps.mode = PAIR_TRIANGLE;
@@ -256,7 +316,7 @@ int pairwise_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *X,
     }
     T *sq = (T *)(ws + o_sq);
     RC(run_pairs<T>(m, ps, series<T>(sx.ptr, sx.stride, tx), series<T>(sy.ptr, sy.stride, ty), w,
-                    sq, st));
+                    sq, st, w_fixed));
     if (singleton) RC(launch_fill_f64(out, nx * nx, 0.0, st));
     return launch_finish<T>(ps, sq, out, singleton ? nx : ny, st);
 }
@@ -266,7 +326,9 @@ int argmin_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *X, i
                 int64_t xstr, int64_t tx, const void *Y, int64_t ny, int64_t ystr, int64_t ty,
                 int64_t k, const double *lb, int64_t *idx, double *dist, hipStream_t st) {
     const int64_t np = nx * ny;
-    const bool certified = m->precision == WBX_FP32 && k <= 32;
+    // the certified FP32 candidate bound covers the DTW recursion; ADTW runs
+    // in float64 throughout
+    const bool certified = m->precision == WBX_FP32 && k <= 32 && m->kind != WBX_ADTW;
     Carve cv;
     const size_t o_x64 = cv.take(stage_bytes(dtype, true, nx, tx));
     const size_t o_y64 = cv.take(stage_bytes(dtype, true, ny, ty));
@@ -707,13 +769,28 @@ int wbx_dev_pairwise(const wbx_metric *metric, int dtype, const void *X, int64_t
     int rc = acc64_prepare(ctx, &mm, dtype,
                            {{&X, nx, tx, &x_stride}, {singleton ? nullptr : &Y, ny, ty, &y_stride}},
                            st);
+    int w_fixed = 0;
+    if (!rc && (mm.kind == WBX_DDTW || mm.kind == WBX_WDDTW)) {
+        if ((tx < ty ? tx : ty) < 3) {
+            // DerivativeDtwMetric._distance returns 0 below length 3
+            rc = launch_fill_f64(out, singleton ? nx * nx : nx * ny, 0.0, st);
+            int rc2 = end(ctx, st);
+            return rc ? rc : rc2;
+        }
+        // the pairwise band comes from the series' own lengths (_elastic.pyx:3276)
+        w_fixed = band_w((int)tx, (int)ty, mm.r);
+        rc = deriv_prepare(ctx, &mm, &dtype,
+                           {{&X, nx, &tx, &x_stride}, {singleton ? nullptr : &Y, ny, &ty, &y_stride}},
+                           st);
+        if (singleton) ty = tx;
+    }
     if (singleton) y_stride = x_stride;
     if (!rc)
         rc = mm.precision == WBX_FP64
                  ? pairwise_impl<double>(ctx, &mm, dtype, X, nx, x_stride, tx, Y, ny, y_stride, ty,
-                                         singleton, out, st)
+                                         singleton, out, st, w_fixed)
                  : pairwise_impl<float>(ctx, &mm, dtype, X, nx, x_stride, tx, Y, ny, y_stride, ty,
-                                        singleton, out, st);
+                                        singleton, out, st, w_fixed);
     int rc2 = end(ctx, st);
     return rc ? rc : rc2;
 }
@@ -732,6 +809,17 @@ int wbx_dev_paired(const wbx_metric *metric, int dtype, const void *X, int64_t x
     PairSpec ps{PAIR_DIAG, n, n, nullptr, nullptr};
     wbx_metric mm = *metric;
     int rc = acc64_prepare(ctx, &mm, dtype, {{&X, n, t, &x_stride}, {&Y, n, t, &y_stride}}, st);
+    int w_fixed = 0;
+    if (!rc && (mm.kind == WBX_DDTW || mm.kind == WBX_WDDTW)) {
+        if (t < 3) {
+            rc = launch_fill_f64(out, n, 0.0, st);
+            int rc2 = end(ctx, st);
+            return rc ? rc : rc2;
+        }
+        w_fixed = band_w((int)t, (int)t, mm.r);
+        int64_t ty = t;
+        rc = deriv_prepare(ctx, &mm, &dtype, {{&X, n, &t, &x_stride}, {&Y, n, &ty, &y_stride}}, st);
+    }
     if (rc) {
         int rc2 = end(ctx, st);
         return rc ? rc : rc2;
@@ -756,7 +844,7 @@ int wbx_dev_paired(const wbx_metric *metric, int dtype, const void *X, int64_t x
             if (!rc)
                 rc = run_pairs<double>(metric, ps, series<double>(sy.ptr, sy.stride, t),
                                        series<double>(sx.ptr, sx.stride, t), w64,
-                                       (double *)(ws + o_sq), st);
+                                       (double *)(ws + o_sq), st, w_fixed);
             if (!rc) rc = launch_finish<double>(ps, (double *)(ws + o_sq), out, n, st);
         } else if (!rc) {
             Staged<float> sx, sy;
@@ -765,7 +853,7 @@ int wbx_dev_paired(const wbx_metric *metric, int dtype, const void *X, int64_t x
             if (!rc)
                 rc = run_pairs<float>(metric, ps, series<float>(sy.ptr, sy.stride, t),
                                       series<float>(sx.ptr, sx.stride, t), w32,
-                                      (float *)(ws + o_sq), st);
+                                      (float *)(ws + o_sq), st, w_fixed);
             if (!rc) rc = launch_finish<float>(ps, (float *)(ws + o_sq), out, n, st);
         }
     }
@@ -792,6 +880,20 @@ int wbx_dev_argmin(const wbx_metric *metric, int dtype, const void *X, int64_t n
     RC(begin(ctx, st));
     wbx_metric mm = *metric;
     int rc = acc64_prepare(ctx, &mm, dtype, {{&X, nx, tx, &x_stride}, {&Y, ny, ty, &y_stride}}, st);
+    if (!rc && (mm.kind == WBX_DDTW || mm.kind == WBX_WDDTW)) {
+        if ((tx < ty ? tx : ty) < 3) {
+            // DerivativeDtwMetric._eadistance never accepts a series shorter
+            // than 3: the driver's zero-initialised outputs stay
+            // (_cdistance.pyx:1360-1361)
+            rc = launch_fill_f64(dist, nx * k, 0.0, st);
+            if (!rc) rc = launch_fill_f64((double *)idx, nx * k, 0.0, st);
+            int rc2 = end(ctx, st);
+            return rc ? rc : rc2;
+        }
+        // argmin bands by the slope lengths (_elastic.pyx:3308): no override
+        rc = deriv_prepare(ctx, &mm, &dtype, {{&X, nx, &tx, &x_stride}, {&Y, ny, &ty, &y_stride}},
+                           st);
+    }
     if (!rc)
         rc = argmin_impl(ctx, &mm, dtype, X, nx, x_stride, tx, Y, ny, y_stride, ty, k, lower_bound,
                          idx, dist, st);
@@ -803,7 +905,7 @@ int wbx_dev_subsequence_pairwise(const wbx_metric *metric, int dtype, const void
                                  const int64_t *s_off, const int64_t *s_len, int64_t n_s,
                                  const void *X, int64_t n, int64_t x_stride, int64_t t,
                                  double *dist, int64_t *idx, void *stream) {
-    RC(check_metric(metric));
+    RC(check_subsequence_metric(metric));
     RC(check_sizes(n, t));
     if (n_s < 1) return fail(WBX_EINVAL, "Subsequence y cannot be empty.");
     DeviceCtx *ctx = ctx_current();
@@ -894,7 +996,7 @@ int wbx_dev_subsequence_profile(const wbx_metric *metric, int dtype, const void 
                                 const int64_t *s_off, const int64_t *s_len, int64_t n_s,
                                 int paired, const void *X, int64_t n, int64_t x_stride, int64_t t,
                                 int64_t nwp, double *prof, void *stream) {
-    RC(check_metric(metric));
+    RC(check_subsequence_metric(metric));
     RC(check_sizes(n, t));
     if (n_s < 1) return fail(WBX_EINVAL, "Subsequence y cannot be empty.");
     if (!S || !s_off || !s_len || !X || !prof) return fail(WBX_EINVAL, "NULL argument");
@@ -913,7 +1015,7 @@ int wbx_dev_argmin_subsequence(const wbx_metric *metric, int dtype, const void *
                                const int64_t *s_off, const int64_t *s_len, const void *X,
                                int64_t n, int64_t x_stride, int64_t t, int64_t k, int64_t *idx,
                                double *dist, void *stream) {
-    RC(check_metric(metric));
+    RC(check_subsequence_metric(metric));
     RC(check_sizes(n, t));
     if (!S || !s_off || !s_len || !X || !idx || !dist) return fail(WBX_EINVAL, "NULL argument");
     if (k < 1) return fail(WBX_EINVAL, "k == %lld, must be >= 1", (long long)k);
@@ -957,7 +1059,7 @@ int wbx_subsequence_profile(const wbx_metric *metric, const double *S, const int
                             const int64_t *s_len, int64_t n_s, int paired, const double *X,
                             int64_t n, int64_t dims, int64_t t, int64_t dim, int64_t nwp,
                             double *prof) {
-    RC(check_metric(metric));
+    RC(check_subsequence_metric(metric));
     RC(check_sizes(n, t));
     if (n_s < 1) return fail(WBX_EINVAL, "Subsequence y cannot be empty.");
     if (dims < 1 || dim < 0 || dim >= dims)
@@ -991,7 +1093,7 @@ int wbx_subsequence_profile(const wbx_metric *metric, const double *S, const int
 int wbx_argmin_subsequence(const wbx_metric *metric, const double *S, const int64_t *s_off,
                            const int64_t *s_len, int64_t n, const double *X, int64_t dims,
                            int64_t t, int64_t dim, int64_t k, int64_t *idx, double *dist) {
-    RC(check_metric(metric));
+    RC(check_subsequence_metric(metric));
     RC(check_sizes(n, t));
     if (dims < 1 || dim < 0 || dim >= dims)
         return fail(WBX_EINVAL, "The parameter dim must be 0 <= dim < n_dims");
@@ -1025,7 +1127,7 @@ int wbx_argmin_subsequence(const wbx_metric *metric, const double *S, const int6
 int wbx_subsequence_pairwise(const wbx_metric *metric, const double *S, const int64_t *s_off,
                              const int64_t *s_len, int64_t n_s, const double *X, int64_t n,
                              int64_t dims, int64_t t, int64_t dim, double *dist, int64_t *idx) {
-    RC(check_metric(metric));
+    RC(check_subsequence_metric(metric));
     RC(check_sizes(n, t));
     if (n_s < 1) return fail(WBX_EINVAL, "Subsequence y cannot be empty.");
     if (dims < 1 || dim < 0 || dim >= dims)

@@ -27,9 +27,12 @@ __device__ __forceinline__ T cmin(T a, T b) { return b < a ? b : a; }
 // clipped).  Each diagonal also writes +inf just outside that range, so every
 // neighbour read of the next two diagonals sees either a band cell or +inf.
 // Returns the squared cost of cell (lx-1, ly-1) in every lane.
-template <typename T>
+// AD: the amercing recursion of adtw_distance (distance/_elastic.pyx:943-1008):
+//     cost = min(min(up + p, left + p), diag) + v * v   (rows i >= 1)
+// with row 0 the plain prefix sum (no penalty, no weights).
+template <typename T, bool AD = false>
 __device__ T block_dtw(const T *__restrict__ X, int lx, const T *__restrict__ Y, int ly, int w,
-                       const T *__restrict__ weights, T *d0, T *d1, T *d2) {
+                       const T *__restrict__ weights, T *d0, T *d1, T *d2, T pen = (T)0) {
     const int min_len = lx > ly ? lx - ly : 0;
     const int max_len = (ly > lx ? ly - lx : 0) + w;
     const T INF = inf_of<T>();
@@ -53,6 +56,11 @@ __device__ T block_dtw(const T *__restrict__ X, int lx, const T *__restrict__ Y,
                 if (weights) wt = (i == 0 && j > 0) ? weights[j - 1] : weights[i > j ? i - j : j - i];
                 if (i == 0 && j == 0) {
                     c = v * v * wt;
+                } else if constexpr (AD) {
+                    const T up = i > 0 ? d1[i - 1] : INF;
+                    const T left = j > 0 ? d1[i] : INF;
+                    const T dg = (i > 0 && j > 0) ? d2[i - 1] : INF;
+                    c = i > 0 ? cmin(cmin(up + pen, left + pen), dg) + v * v : left + v * v;
                 } else {
                     const T up = i > 0 ? d1[i - 1] : INF;
                     const T left = j > 0 ? d1[i] : INF;

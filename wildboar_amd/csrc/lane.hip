@@ -25,14 +25,21 @@ namespace wbx {
 
 // pen[q] is +inf for slot BP-8+q == B and -inf otherwise (uniform values), so
 // max(row, pen) resets slot B without a branch.
-template <int BP>
+// AD: the amercing recursion (adtw_distance, distance/_elastic.pyx:943-1008),
+// m = min3(up + p, left + p, diag); row 0 runs with p = 0 (its prefix sum
+// carries no penalty).
+template <int BP, bool AD = false>
 __device__ __forceinline__ void lane_row(float (&row)[BP], const float *yw, float xi,
-                                         const float (&pen)[8]) {
+                                         const float (&pen)[8], float p = 0.0f) {
     float left = WBX_INF_F;
 #pragma unroll
     for (int k = 0; k < BP; ++k) {
         const float up = (k + 1 < BP) ? row[k + 1] : WBX_INF_F;
-        const float m = fminf(fminf(up, left), row[k]);
+        float m;
+        if constexpr (AD)
+            m = fminf(fminf(up + p, left + p), row[k]);
+        else
+            m = fminf(fminf(up, left), row[k]);
         const float d = xi - yw[k];
         const float c = __builtin_fmaf(d, d, m);
         row[k] = c;
@@ -42,9 +49,9 @@ __device__ __forceinline__ void lane_row(float (&row)[BP], const float *yw, floa
     for (int q = 1; q < 8; ++q) row[BP - 8 + q] = fmaxf(row[BP - 8 + q], pen[q]);
 }
 
-template <int BP, int U>
+template <int BP, int U, bool AD = false>
 __global__ __launch_bounds__(64) void k_lane_dtw(PairSpec ps, Series<float> xs,
-                                                 Series<float> ys, LaneGeo g,
+                                                 Series<float> ys, LaneGeo g, float pen_ad,
                                                  float *__restrict__ sq) {
     constexpr int W = BP + U - 1;
     const int64_t p0 = (int64_t)blockIdx.x * 64 + threadIdx.x;
@@ -66,15 +73,23 @@ __global__ __launch_bounds__(64) void k_lane_dtw(PairSpec ps, Series<float> xs,
 #pragma unroll
     for (int m = 0; m < W; ++m) yw[m] = ldy(Y, m - off, ly);
 
+    int i_begin = 0;
+    if constexpr (AD) {
+        lane_row<BP, true>(row, yw, X[0], pen, 0.0f);
+#pragma unroll
+        for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
+        yw[W - 1] = ldy(Y, 1 - off + W - 1, ly);
+        i_begin = 1;
+    }
     // Leading rows one at a time so the rest is a whole number of U-groups.
-    const int rem = lx % U;
-    for (int i = 0; i < rem; ++i) {
-        lane_row<BP>(row, yw, X[i], pen);
+    const int rem = (lx - i_begin) % U;
+    for (int i = i_begin; i < i_begin + rem; ++i) {
+        lane_row<BP, AD>(row, yw, X[i], pen, pen_ad);
 #pragma unroll
         for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
         yw[W - 1] = ldy(Y, i + 1 - off + W - 1, ly);
     }
-    for (int i0 = rem; i0 < lx; i0 += U) {
+    for (int i0 = i_begin + rem; i0 < lx; i0 += U) {
         float xv[U], yn[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -82,7 +97,7 @@ __global__ __launch_bounds__(64) void k_lane_dtw(PairSpec ps, Series<float> xs,
             yn[u] = ldy(Y, i0 + U - off + W - U + u, ly);
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) lane_row<BP>(row, yw + u, xv[u], pen);
+        for (int u = 0; u < U; ++u) lane_row<BP, AD>(row, yw + u, xv[u], pen, pen_ad);
 #pragma unroll
         for (int m = 0; m < W - U; ++m) yw[m] = yw[m + U];
 #pragma unroll
@@ -101,43 +116,49 @@ __global__ __launch_bounds__(64) void k_lane_dtw(PairSpec ps, Series<float> xs,
 
 template <int BP, int U>
 static void launch_one(const PairSpec &ps, Series<float> xs, Series<float> ys, LaneGeo g,
-                       float *sq, hipStream_t st) {
+                       const double *penalty, float *sq, hipStream_t st) {
     dim3 grid((unsigned)((ps.n_pairs + 63) / 64));
-    hipLaunchKernelGGL((k_lane_dtw<BP, U>), grid, dim3(64), 0, st, ps, xs, ys, g, sq);
+    if (penalty)
+        hipLaunchKernelGGL((k_lane_dtw<BP, U, true>), grid, dim3(64), 0, st, ps, xs, ys, g,
+                           (float)*penalty, sq);
+    else
+        hipLaunchKernelGGL((k_lane_dtw<BP, U, false>), grid, dim3(64), 0, st, ps, xs, ys, g, 0.0f,
+                           sq);
 }
 
 constexpr int LANE_MAX_BP = 112;
 
-bool lane_supported(int lx, int ly, double r, int weighted) {
+bool lane_supported(int lx, int ly, double r, int weighted, int w_fixed) {
     if (weighted) return false;
-    Band bd = make_band(lx, ly, r);
+    Band bd = make_band(lx, ly, r, w_fixed);
     return bd.B <= LANE_MAX_BP && lx >= 1 && ly >= 1;
 }
 
 int launch_lane_dtw(const PairSpec &ps, Series<float> xs, Series<float> ys, double r,
-                    const float *weights, float *sq, hipStream_t st) {
+                    const float *weights, float *sq, hipStream_t st, int w_fixed,
+                    const double *penalty) {
     if (ps.n_pairs <= 0) return WBX_OK;
     if (weights) return fail(WBX_EUNSUPPORTED, "lane kernel: weighted DTW not supported");
     if (xs.lens || ys.lens) return fail(WBX_EUNSUPPORTED, "lane kernel: needs one geometry");
-    Band bd = make_band(xs.len, ys.len, r);
+    Band bd = make_band(xs.len, ys.len, r, w_fixed);
     if (bd.B > LANE_MAX_BP) return fail(WBX_EUNSUPPORTED, "lane kernel: band %d too wide", bd.B);
     LaneGeo g{bd.lx, bd.ly, bd.off, bd.B, bd.ly - bd.lx + bd.off};
     const int BP = (bd.B + 7) / 8 * 8;
     switch (BP) {
-        case 8: launch_one<8, 8>(ps, xs, ys, g, sq, st); break;
-        case 16: launch_one<16, 8>(ps, xs, ys, g, sq, st); break;
-        case 24: launch_one<24, 8>(ps, xs, ys, g, sq, st); break;
-        case 32: launch_one<32, 8>(ps, xs, ys, g, sq, st); break;
-        case 40: launch_one<40, 8>(ps, xs, ys, g, sq, st); break;
-        case 48: launch_one<48, 8>(ps, xs, ys, g, sq, st); break;
-        case 56: launch_one<56, 8>(ps, xs, ys, g, sq, st); break;
-        case 64: launch_one<64, 8>(ps, xs, ys, g, sq, st); break;
-        case 72: launch_one<72, 8>(ps, xs, ys, g, sq, st); break;
-        case 80: launch_one<80, 8>(ps, xs, ys, g, sq, st); break;
-        case 88: launch_one<88, 8>(ps, xs, ys, g, sq, st); break;
-        case 96: launch_one<96, 4>(ps, xs, ys, g, sq, st); break;
-        case 104: launch_one<104, 4>(ps, xs, ys, g, sq, st); break;
-        case 112: launch_one<112, 4>(ps, xs, ys, g, sq, st); break;
+        case 8: launch_one<8, 8>(ps, xs, ys, g, penalty, sq, st); break;
+        case 16: launch_one<16, 8>(ps, xs, ys, g, penalty, sq, st); break;
+        case 24: launch_one<24, 8>(ps, xs, ys, g, penalty, sq, st); break;
+        case 32: launch_one<32, 8>(ps, xs, ys, g, penalty, sq, st); break;
+        case 40: launch_one<40, 8>(ps, xs, ys, g, penalty, sq, st); break;
+        case 48: launch_one<48, 8>(ps, xs, ys, g, penalty, sq, st); break;
+        case 56: launch_one<56, 8>(ps, xs, ys, g, penalty, sq, st); break;
+        case 64: launch_one<64, 8>(ps, xs, ys, g, penalty, sq, st); break;
+        case 72: launch_one<72, 8>(ps, xs, ys, g, penalty, sq, st); break;
+        case 80: launch_one<80, 8>(ps, xs, ys, g, penalty, sq, st); break;
+        case 88: launch_one<88, 8>(ps, xs, ys, g, penalty, sq, st); break;
+        case 96: launch_one<96, 4>(ps, xs, ys, g, penalty, sq, st); break;
+        case 104: launch_one<104, 4>(ps, xs, ys, g, penalty, sq, st); break;
+        case 112: launch_one<112, 4>(ps, xs, ys, g, penalty, sq, st); break;
         default: return fail(WBX_EUNSUPPORTED, "lane kernel: band %d", bd.B);
     }
     WBX_HIP_CHECK(hipGetLastError());

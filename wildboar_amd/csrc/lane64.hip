@@ -41,14 +41,21 @@ __device__ __forceinline__ double ldy64(const double *__restrict__ Y, int j, int
 __device__ __forceinline__ double dmin(double a, double b) { return b < a ? b : a; }
 
 // One row of BP cells.  wk[k] is the weight of slot k (unused when !WT).
-template <int BP, bool WT>
+// AD: the amercing recursion, min(min(up + p, left + p), diag) + v * v
+// (adtw_distance, distance/_elastic.pyx:943-1008); row 0 passes p = 0, which
+// leaves the prefix sum exact (x + 0 == x for the costs, all >= 0 or +inf).
+template <int BP, bool WT, bool AD>
 __device__ __forceinline__ void row64(double (&row)[BP], const double *yw, double xi,
-                                      const double (&pen)[8], const double *wk) {
+                                      const double (&pen)[8], const double *wk, double p) {
     double left = WBX_INF_D;
 #pragma unroll
     for (int k = 0; k < BP; ++k) {
         const double up = (k + 1 < BP) ? row[k + 1] : WBX_INF_D;
-        const double m = dmin(dmin(up, left), row[k]);
+        double m;
+        if constexpr (AD)
+            m = dmin(dmin(up + p, left + p), row[k]);
+        else
+            m = dmin(dmin(up, left), row[k]);
         const double v = xi - yw[k];
         double term = v * v;
         if constexpr (WT) term = term * wk[k];
@@ -62,11 +69,12 @@ __device__ __forceinline__ void row64(double (&row)[BP], const double *yw, doubl
 
 // Squared DTW cost of X[0, lx) against Y[0, ly) with band offset `off`,
 // width B <= BP, final slot kf.  weights: wn >= max(lx, ly) values, all > 0.
-template <int BP, int U, bool WT>
+template <int BP, int U, bool WT, bool AD = false>
 __device__ __forceinline__ double lane64_cost(const double *__restrict__ X, int lx,
                                               const double *__restrict__ Y, int ly, int off,
                                               int B, int kf,
-                                              const double *__restrict__ weights, int wn) {
+                                              const double *__restrict__ weights, int wn,
+                                              double p = 0.0) {
     constexpr int W = BP + U - 1;
     double row[BP];
 #pragma unroll
@@ -95,13 +103,13 @@ __device__ __forceinline__ double lane64_cost(const double *__restrict__ X, int 
 
     // row 0 on its own (its weights differ), then single rows until the rest
     // is a whole number of U-groups
-    row64<BP, WT>(row, yw, X[0], pen, w0);
+    row64<BP, WT, AD>(row, yw, X[0], pen, w0, 0.0);
 #pragma unroll
     for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
     yw[W - 1] = ldy64(Y, 1 - off + W - 1, ly);
     const int rem = (lx - 1) % U;
     for (int i = 1; i <= rem; ++i) {
-        row64<BP, WT>(row, yw, X[i], pen, wk);
+        row64<BP, WT, AD>(row, yw, X[i], pen, wk, p);
 #pragma unroll
         for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
         yw[W - 1] = ldy64(Y, i + 1 - off + W - 1, ly);
@@ -114,7 +122,7 @@ __device__ __forceinline__ double lane64_cost(const double *__restrict__ X, int 
             yn[u] = ldy64(Y, i0 + U - off + W - U + u, ly);
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) row64<BP, WT>(row, yw + u, xv[u], pen, wk);
+        for (int u = 0; u < U; ++u) row64<BP, WT, AD>(row, yw + u, xv[u], pen, wk, p);
 #pragma unroll
         for (int m = 0; m < W - U; ++m) yw[m] = yw[m + U];
 #pragma unroll
@@ -127,17 +135,17 @@ __device__ __forceinline__ double lane64_cost(const double *__restrict__ X, int 
     return res;
 }
 
-template <int BP, int U, bool WT>
+template <int BP, int U, bool WT, bool AD>
 __global__ __launch_bounds__(64) void k_lane64_pairs(PairSpec ps, Series<double> xs,
                                                      Series<double> ys, int off, int B, int kf,
                                                      const double *__restrict__ weights, int wn,
-                                                     double *__restrict__ sq) {
+                                                     double pen, double *__restrict__ sq) {
     const int64_t p0 = (int64_t)blockIdx.x * 64 + threadIdx.x;
     const int64_t p = p0 < ps.n_pairs ? p0 : ps.n_pairs - 1;
     int64_t a, b;
     decode_pair(ps, p, a, b);
-    const double c = lane64_cost<BP, U, WT>(xs.row(a), xs.len, ys.row(b), ys.len, off, B, kf,
-                                            weights, wn);
+    const double c = lane64_cost<BP, U, WT, AD>(xs.row(a), xs.len, ys.row(b), ys.len, off, B,
+                                                kf, weights, wn, pen);
     if (p0 < ps.n_pairs) sq[p0] = c;
 }
 
@@ -227,22 +235,28 @@ int lane64_class(int B, bool weighted) {
 }
 
 int launch_lane64_pairs(const PairSpec &ps, Series<double> xs, Series<double> ys, double r,
-                        const double *weights, double *sq, hipStream_t st) {
+                        const double *weights, double *sq, hipStream_t st, int w_fixed,
+                        const double *penalty) {
     if (ps.n_pairs <= 0) return WBX_OK;
-    const Band bd = make_band(xs.len, ys.len, r);
+    const Band bd = make_band(xs.len, ys.len, r, w_fixed);
     const int BP = lane64_class(bd.B, weights != nullptr);
     if (!BP || xs.lens || ys.lens) return fail(WBX_EUNSUPPORTED, "lane64: unsupported geometry");
+    if (penalty && weights) return fail(WBX_EUNSUPPORTED, "adtw takes no weights");
     const int kf = bd.ly - bd.lx + bd.off;
     const int wn = xs.len > ys.len ? xs.len : ys.len;
+    const double pen = penalty ? *penalty : 0.0;
     const dim3 grid((unsigned)((ps.n_pairs + 63) / 64));
 #define WBX_L64_PAIRS(C)                                                                        \
     case C:                                                                                     \
         if (weights)                                                                            \
-            hipLaunchKernelGGL((k_lane64_pairs<C, 4, true>), grid, dim3(64), 0, st, ps, xs, ys, \
-                               bd.off, bd.B, kf, weights, wn, sq);                              \
+            hipLaunchKernelGGL((k_lane64_pairs<C, 4, true, false>), grid, dim3(64), 0, st, ps,  \
+                               xs, ys, bd.off, bd.B, kf, weights, wn, 0.0, sq);                 \
+        else if (penalty)                                                                       \
+            hipLaunchKernelGGL((k_lane64_pairs<C, 4, false, true>), grid, dim3(64), 0, st, ps,  \
+                               xs, ys, bd.off, bd.B, kf, weights, wn, pen, sq);                 \
         else                                                                                    \
-            hipLaunchKernelGGL((k_lane64_pairs<C, 4, false>), grid, dim3(64), 0, st, ps, xs,   \
-                               ys, bd.off, bd.B, kf, weights, wn, sq);                          \
+            hipLaunchKernelGGL((k_lane64_pairs<C, 4, false, false>), grid, dim3(64), 0, st, ps, \
+                               xs, ys, bd.off, bd.B, kf, weights, wn, 0.0, sq);                 \
         break;
     switch (BP) {
         WBX_LANE64_CLASSES(WBX_L64_PAIRS)

@@ -18,12 +18,15 @@
 
 namespace wbx {
 
-template <typename T>
+// w_fixed > 0 replaces the band width of every pair (the derivative
+// metrics band by the lengths of the series before the slope transform).
+template <typename T, bool AD>
 __global__ __launch_bounds__(64) void k_plain_dtw(PairSpec ps, Series<T> xs,
                                                   Series<T> ys, double r,
                                                   const double *__restrict__ rp,
                                                   const T *__restrict__ weights,
-                                                  int cap, T *__restrict__ sq) {
+                                                  int cap, int w_fixed, T pen,
+                                                  T *__restrict__ sq) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *buf = reinterpret_cast<T *>(smem_raw);
     const int64_t p = (int64_t)blockIdx.x + (int64_t)blockIdx.y * gridDim.x;
@@ -32,8 +35,9 @@ __global__ __launch_bounds__(64) void k_plain_dtw(PairSpec ps, Series<T> xs,
     decode_pair(ps, p, a, b);
     const int lx = xs.length(a), ly = ys.length(b);
     const double rr = rp ? rp[a] : r;
-    const T res = block_dtw<T>(xs.row(a), lx, ys.row(b), ly, band_w(lx, ly, rr), weights, buf,
-                               buf + cap, buf + 2 * cap);
+    const int w = w_fixed > 0 ? w_fixed : band_w(lx, ly, rr);
+    const T res = block_dtw<T, AD>(xs.row(a), lx, ys.row(b), ly, w, weights, buf, buf + cap,
+                                   buf + 2 * cap, pen);
     if (threadIdx.x == 0) sq[p] = res;
 }
 
@@ -116,27 +120,41 @@ static int set_lds(const void *fn, size_t lds) {
 template <typename T>
 int launch_plain_dtw(const PairSpec &ps, Series<T> xs, Series<T> ys, double r,
                      const T *weights, T *sq, hipStream_t st, const double *rp,
-                     bool padded_iter) {
+                     bool padded_iter, int w_fixed, const double *penalty) {
     if (ps.n_pairs <= 0) return WBX_OK;
     const int cap = xs.len;  // every row length of xs is <= xs.len
     const size_t lds = (size_t)3 * (size_t)cap * sizeof(T);
-    if (padded_iter && ys.len != xs.len)
-        return fail(WBX_EINVAL, "padded iteration needs one padded length");
-    auto *fn = padded_iter ? k_plain_dtw_padded<T> : k_plain_dtw<T>;
-    int rc = set_lds<T>((const void *)fn, lds);
-    if (rc) return rc;
     const int64_t nb = ps.n_pairs;
     dim3 grid((unsigned)(nb < 65536 ? nb : 65536), (unsigned)((nb + 65535) / 65536));
-    hipLaunchKernelGGL(fn, grid, dim3(64), lds, st, ps, xs, ys, r, rp, weights, cap, sq);
+    if (padded_iter) {
+        if (ys.len != xs.len) return fail(WBX_EINVAL, "padded iteration needs one padded length");
+        if (w_fixed > 0 || penalty) return fail(WBX_EUNSUPPORTED, "padded iteration: dtw only");
+        int rc = set_lds<T>((const void *)k_plain_dtw_padded<T>, lds);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_plain_dtw_padded<T>, grid, dim3(64), lds, st, ps, xs, ys, r, rp,
+                           weights, cap, sq);
+    } else if (penalty) {
+        if (weights) return fail(WBX_EUNSUPPORTED, "adtw takes no weights");
+        int rc = set_lds<T>((const void *)k_plain_dtw<T, true>, lds);
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_plain_dtw<T, true>), grid, dim3(64), lds, st, ps, xs, ys, r, rp,
+                           weights, cap, w_fixed, (T)*penalty, sq);
+    } else {
+        int rc = set_lds<T>((const void *)k_plain_dtw<T, false>, lds);
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_plain_dtw<T, false>), grid, dim3(64), lds, st, ps, xs, ys, r, rp,
+                           weights, cap, w_fixed, (T)0, sq);
+    }
     WBX_HIP_CHECK(hipGetLastError());
     return WBX_OK;
 }
 
 template int launch_plain_dtw<float>(const PairSpec &, Series<float>, Series<float>, double,
-                                     const float *, float *, hipStream_t, const double *, bool);
+                                     const float *, float *, hipStream_t, const double *, bool,
+                                     int, const double *);
 template int launch_plain_dtw<double>(const PairSpec &, Series<double>, Series<double>, double,
                                       const double *, double *, hipStream_t, const double *,
-                                      bool);
+                                      bool, int, const double *);
 
 // ------------------------------------------------------------- euclidean
 // euclidean_distance (distance/_metric.pyx:654-683) with S = x, T = y: windows

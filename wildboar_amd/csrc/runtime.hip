@@ -89,6 +89,10 @@ int ws_in_reserve(DeviceCtx *ctx, size_t bytes, void **out) {
     return arena_reserve(ctx->ws_in, bytes, out);
 }
 
+int ws_dv_reserve(DeviceCtx *ctx, size_t bytes, void **out) {
+    return arena_reserve(ctx->ws_dv, bytes, out);
+}
+
 // WDTW weights 1 / (1 + exp(-g (i - n/2))) with the host libm exp, as
 // WeightedDtwMetric.reset (distance/_elastic.pyx:3334-3341) computes them.
 int cached_weights(DeviceCtx *ctx, int64_t n, double g, const double **w64, const float **w32) {

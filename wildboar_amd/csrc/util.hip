@@ -60,6 +60,40 @@ int launch_round_f32_rows(const double *src, int64_t n, int64_t len, int64_t sst
     return WBX_OK;
 }
 
+// average_slope (distance/_elastic.pyx:3220-3225) of every row, in float64:
+// dst[i][c] = ((q[c+1] - q[c]) + ((q[c+2] - q[c]) / 2)) / 2, c < len - 2.
+template <typename S>
+__global__ __launch_bounds__(256) void k_slope_rows(const S *__restrict__ src, int64_t n,
+                                                    int64_t len, int64_t sstride,
+                                                    double *__restrict__ dst) {
+    const int64_t dl = len - 2;
+    const int64_t total = n * dl;
+    for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * 256) {
+        const int64_t i = g / dl, c = g - i * dl;
+        const S *q = src + i * sstride + c;
+        const double q0 = (double)q[0], q1 = (double)q[1], q2 = (double)q[2];
+        dst[g] = ((q1 - q0) + ((q2 - q0) / 2)) / 2;
+    }
+}
+
+template <typename S>
+int launch_slope_rows(const S *src, int64_t n, int64_t len, int64_t sstride, double *dst,
+                      hipStream_t st) {
+    const int64_t total = n * (len - 2);
+    if (total <= 0) return WBX_OK;
+    int64_t nb = (total + 255) / 256;
+    if (nb > 16384) nb = 16384;
+    hipLaunchKernelGGL((k_slope_rows<S>), dim3((unsigned)nb), dim3(256), 0, st, src, n, len,
+                       sstride, dst);
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
+template int launch_slope_rows<double>(const double *, int64_t, int64_t, int64_t, double *,
+                                       hipStream_t);
+template int launch_slope_rows<float>(const float *, int64_t, int64_t, int64_t, double *,
+                                      hipStream_t);
+
 __global__ __launch_bounds__(256) void k_fill_f64(double *__restrict__ dst, int64_t n, double v) {
     for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n;
          q += (int64_t)gridDim.x * 256)

@@ -41,11 +41,12 @@ __host__ __device__ inline int band_w(int lx, int ly, double r) {
     return (int)(v > 1.0 ? v : 1.0);
 }
 
-__host__ __device__ inline Band make_band(int lx, int ly, double r) {
+// The band of a pair for width w (w_fixed > 0), else for fraction r.
+__host__ __device__ inline Band make_band(int lx, int ly, double r, int w_fixed = 0) {
     Band b;
     b.lx = lx;
     b.ly = ly;
-    b.w = band_w(lx, ly, r);
+    b.w = w_fixed > 0 ? w_fixed : band_w(lx, ly, r);
     int min_len = lx > ly ? lx - ly : 0;
     int max_len = (ly > lx ? ly - lx : 0) + b.w;
     b.off = min_len + b.w - 1;
@@ -140,6 +141,7 @@ struct DeviceCtx {
     hipEvent_t last = nullptr;
     Workspace ws;
     Workspace ws_in;  // inputs rounded to float32 for the FP32_ACC64 mode
+    Workspace ws_dv;  // average slopes of the inputs (derivative metrics)
     int n_cu = 0;
     int64_t w_n = -1;
     double w_g = 0.0;
@@ -151,6 +153,7 @@ struct DeviceCtx {
 DeviceCtx *ctx_current();              // current device's context (created lazily)
 int ws_reserve(DeviceCtx *ctx, size_t bytes, void **out);
 int ws_in_reserve(DeviceCtx *ctx, size_t bytes, void **out);
+int ws_dv_reserve(DeviceCtx *ctx, size_t bytes, void **out);
 // WDTW weights for width n and penalty g, computed on the host with libm exp
 // and cached on the device (one entry per device).
 int cached_weights(DeviceCtx *ctx, int64_t n, double g, const double **w64, const float **w32);
@@ -159,11 +162,13 @@ int cached_weights(DeviceCtx *ctx, int64_t n, double g, const double **w64, cons
 // Plain one-pair-per-block DTW (any band, any lengths, WDTW quirk): writes the
 // squared cost of each pair into sq[p].  rp (optional) gives r per x row.
 // padded_iter: the measurement baseline that walks every pair over the whole
-// padded length (xs.len == ys.len), same results.
+// padded length (xs.len == ys.len), same results.  w_fixed > 0: the band
+// width of every pair.  penalty != NULL: the ADTW recursion with *penalty.
 template <typename T>
 int launch_plain_dtw(const PairSpec &ps, Series<T> xs, Series<T> ys, double r,
                      const T *weights, T *sq, hipStream_t st, const double *rp = nullptr,
-                     bool padded_iter = false);
+                     bool padded_iter = false, int w_fixed = 0,
+                     const double *penalty = nullptr);
 
 // Euclidean: one thread per pair, sequential sum in the reference's order.
 template <typename T>
@@ -172,9 +177,11 @@ int launch_euclid_pairs(const PairSpec &ps, Series<T> xs, Series<T> ys, T *sq,
 
 // Fast FP32 lane-per-pair kernel for pairs that share one geometry.
 // Returns WBX_EUNSUPPORTED when the band is too wide for it.
+// w_fixed > 0: band width of every pair; penalty != NULL: ADTW.
 int launch_lane_dtw(const PairSpec &ps, Series<float> xs, Series<float> ys,
-                    double r, const float *weights, float *sq, hipStream_t st);
-bool lane_supported(int lx, int ly, double r, int weighted);
+                    double r, const float *weights, float *sq, hipStream_t st, int w_fixed = 0,
+                    const double *penalty = nullptr);
+bool lane_supported(int lx, int ly, double r, int weighted, int w_fixed = 0);
 
 // sqrt + scatter of per-pair squared costs into the driver's output layout.
 template <typename T>
@@ -204,6 +211,10 @@ int launch_fill_i32(int *dst, int64_t n, int value, hipStream_t st);
 // dst[i][c] = (double)(float)src[i][c]: float64 rows rounded to float32 values.
 int launch_round_f32_rows(const double *src, int64_t n, int64_t len, int64_t sstride, double *dst,
                           int64_t dstride, hipStream_t st);
+// dst (n rows of len - 2, packed) = average_slope of every row, in float64.
+template <typename S>
+int launch_slope_rows(const S *src, int64_t n, int64_t len, int64_t sstride, double *dst,
+                      hipStream_t st);
 // *out = max |src[i * stride + c]| as float bits (out must be zeroed first).
 template <typename S>
 int launch_absmax_rows(const S *src, int64_t n, int64_t len, int64_t stride, unsigned *out,
@@ -230,7 +241,8 @@ int launch_subseq_exact(int kind, double r, Shapelets sh, int max_len, Series<T>
 int lane64_class(int B, bool weighted);
 int lane64_cap(bool weighted);
 int launch_lane64_pairs(const PairSpec &ps, Series<double> xs, Series<double> ys, double r,
-                        const double *weights, double *sq, hipStream_t st);
+                        const double *weights, double *sq, hipStream_t st, int w_fixed = 0,
+                        const double *penalty = nullptr);
 int launch_profile_lane64(double r, Shapelets sh, int paired, int min_len, int max_len,
                           Series<double> xs, int64_t n, int64_t nwp, const double *weights,
                           double *prof, hipStream_t st);

@@ -1,4 +1,5 @@
-"""Drop-in distance API for metric = "euclidean", "dtw" and "wdtw".
+"""Drop-in distance API for metric = "euclidean", "dtw", "wdtw", "ddtw",
+"wddtw" and "adtw" (subsequence functions: euclidean, dtw, wdtw).
 
 Same names, signatures, metric names, ``metric_params`` and return shapes as
 wildboar's ``distance/_distance.py``:
@@ -65,22 +66,22 @@ def _check_g(g):
 class _Metric:
     """A resolved metric: the kind, its parameters and elasticity."""
 
-    def __init__(self, name, kind, r=1.0, g=0.05):
-        self.name, self.kind, self.r, self.g = name, kind, r, g
+    def __init__(self, name, kind, r=1.0, g=0.05, p=1.0):
+        self.name, self.kind, self.r, self.g, self.p = name, kind, r, g, p
 
     @property
     def is_elastic(self):
         return self.kind != _lib.EUCLIDEAN
 
     def struct(self, precision):
-        return _lib.metric_struct(self.kind, precision, self.r, self.g)
+        return _lib.metric_struct(self.kind, precision, self.r, self.g, self.p)
 
     def __reduce__(self):
-        return _make_metric, (self.name, self.r, self.g)
+        return _make_metric, (self.name, self.r, self.g, self.p)
 
 
-def _make_metric(name, r, g):
-    return _Metric(name, _lib.KINDS[name], r, g)
+def _make_metric(name, r, g, p=1.0):
+    return _Metric(name, _lib.KINDS[name], r, g, p)
 
 
 def _euclidean(**params):
@@ -101,9 +102,44 @@ def _wdtw(r=1.0, g=0.05, **params):
     return _Metric("wdtw", _lib.WDTW, _check_r(r), _check_g(g))
 
 
+def _real(v, name):
+    # the Cython signatures take `double` (_elastic.pyx:3236, 3346, 3406)
+    if isinstance(v, bool) or not isinstance(v, numbers.Real):
+        raise TypeError(f"{name} must be an instance of float, not {type(v).__name__}.")
+    return float(v)
+
+
+def _ddtw(r=1.0, **params):
+    # DerivativeDtwMetric.__init__ (_elastic.pyx:3234-3235): DtwMetric's r check
+    if params:
+        raise TypeError(f"__init__() got an unexpected keyword argument '{next(iter(params))}'")
+    return _Metric("ddtw", _lib.DDTW, _check_r(r))
+
+
+def _wddtw(r=1.0, g=0.05, **params):
+    # WeightedDerivativeDtwMetric.__init__ (_elastic.pyx:3406-3408) checks r
+    # only; g is any float
+    if params:
+        raise TypeError(f"__init__() got an unexpected keyword argument '{next(iter(params))}'")
+    return _Metric("wddtw", _lib.WDDTW, _check_r(r), _real(g, "g"))
+
+
+def _adtw(r=1.0, p=1.0, **params):
+    # AmercingDtwMetric.__init__ (_elastic.pyx:3344-3348): check_scalar(p, float)
+    if params:
+        raise TypeError(f"__init__() got an unexpected keyword argument '{next(iter(params))}'")
+    p = _real(p, "p")
+    if not p > -np.inf:
+        # -inf + inf (a move from outside the band) would be NaN
+        raise ValueError(f"p == {p}, must be a number above -inf.")
+    return _Metric("adtw", _lib.ADTW, _check_r(r), p=p)
+
+
 # The subset of the reference's _METRICS registry (_distance.py:185-204)
-# served here; the same names resolve for subsequence metrics (:148-182).
-METRICS = {"euclidean": _euclidean, "dtw": _dtw, "wdtw": _wdtw}
+# served here.  Subsequence functions serve euclidean, dtw and wdtw; the
+# other names raise NotImplementedError there.
+METRICS = {"euclidean": _euclidean, "dtw": _dtw, "wdtw": _wdtw, "ddtw": _ddtw,
+           "wddtw": _wddtw, "adtw": _adtw}
 
 
 def check_metric(metric):
