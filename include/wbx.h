@@ -207,6 +207,27 @@ int wbx_dev_subsequence_profile(const wbx_metric *metric, int dtype,
                                 int64_t t, int64_t nwp, double *prof,
                                 void *stream);
 
+/* ------------------------------------------------------- warping paths */
+
+/* dtw_mapping's optimal path (distance/dtw.py:362-427, cells of
+ * _dtw_alignment, _elastic.pyx:1011-1073) for the pairs (X row p, Y row p),
+ * p < n, float64 on the device, warp size w (dtw.py:_compute_warp_size:
+ * max(floor(max(tx, ty) * r), 1)).  A row-major path covers one contiguous
+ * column range per row: rows[p * tx * 2 + 2 i] and [.. + 2 i + 1] are the
+ * first and last column of row i.  x_stride 0 aligns one X row with every Y
+ * row. */
+int wbx_dev_dtw_path(const double *X, int64_t x_stride, int64_t tx, const double *Y,
+                     int64_t y_stride, int64_t ty, int64_t n, int64_t w, int32_t *rows,
+                     void *stream);
+/* One majorize-minimize step of _mm_dtw_average (distance/dtw.py:640-666):
+ * aligns `mean` (tm values) with each X row (n rows of tx), then
+ * mean_out[m] = sum over rows i and aligned columns x (in order) of
+ * X[i, x] * w_i, divided by the summed weights w_i; sample_weight may be
+ * NULL (all 1).  All pointers are device pointers. */
+int wbx_dev_dtw_mm_update(const double *mean, int64_t tm, const double *X, int64_t x_stride,
+                          int64_t tx, int64_t n, int64_t w, const double *sample_weight,
+                          double *mean_out, void *stream);
+
 /* ------------------------------------------- batched uneven pairs
  * No single reference driver: this is the batch form of DtwMetric._distance
  * (distance/_elastic.pyx:3167-3186) over pairs of different lengths, each with

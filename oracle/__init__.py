@@ -331,3 +331,37 @@ def dtw_grad(x, y, r, weights=None):
                              y.shape[0], float(r), _ptr(weights),
                              gx.ctypes.data, gy.ctypes.data)
     return D, gx, gy
+
+
+def warp_size(x_size, r, y_size=0):
+    """_compute_warp_size, distance/dtw.py:38-40."""
+    return max(int(np.floor(max(x_size, y_size) * r)), 1)
+
+
+def mm_dtw_average(X, mean, r=1.0, tol=1e-5, max_epoch=50, sample_weight=None):
+    """_mm_dtw_average, distance/dtw.py:640-666, with dtw_alignment's warp
+    size (dtw.py:246-290) over the cost matrix and backtrack above; the cost
+    is the mean DTW distance (dtw.py:520-525).  Returns (mean, cost)."""
+    X = np.asarray(X, dtype=np.float64)
+    mean = np.array(mean, dtype=np.float64)
+
+    def costfn(m):
+        d = pairwise(m[None, :], X, metric="dtw", r=r)[0]
+        return np.mean(d) if sample_weight is None else np.average(d, weights=sample_weight)
+
+    cost = costfn(mean)
+    for _ in range(max_epoch):
+        z = np.zeros(mean.shape[0])
+        V = np.zeros(mean.shape[0])
+        w_band = warp_size(mean.shape[0], r, X.shape[1])
+        for i in range(X.shape[0]):
+            pm, px = mapping(cost_matrix(mean, X[i], w_band))
+            wgt = 1.0 if sample_weight is None else sample_weight[i]
+            for m, x in zip(pm, px):
+                V[m] += wgt
+                z[m] += X[i, x] * wgt
+        mean = z / V
+        prev, cost = cost, costfn(mean)
+        if abs(prev - cost) < tol:
+            break
+    return mean, cost

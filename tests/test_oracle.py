@@ -474,3 +474,24 @@ def test_ddtw_is_dtw_on_slopes_with_the_band_quirk():
     i = np.arange(t, dtype=float)
     d = oracle.metric_distance(1.0 + 0.75 * i, -2.0 + 0.25 * i, "ddtw", r=0.1)
     assert d == pytest.approx(0.5 * math.sqrt(t - 2), rel=1e-15)
+
+
+def test_mm_dtw_average_fixed_points():
+    rng = np.random.default_rng(33)
+    s = rw(rng, 1, 30)[0]
+    # four identical series are a fixed point: every path is the diagonal and
+    # 4 s / 4 == s
+    mean, cost = oracle.mm_dtw_average(np.stack([s] * 4), s, r=0.2)
+    np.testing.assert_array_equal(mean, s)
+    assert cost == 0.0
+    # r = 0 leaves only the diagonal: one step from any start averages the
+    # series themselves
+    X = np.stack([s, s, s, s])
+    mean, _ = oracle.mm_dtw_average(X, s + 1e-3, r=0.0, max_epoch=1)
+    np.testing.assert_array_equal(mean, s)
+    # the mapping of identical series is the diagonal; a one-step shift puts
+    # two cells on the first and last rows
+    pi, pj = oracle.mapping(oracle.cost_matrix(s, s, oracle.warp_size(30, 0.2)))
+    assert pi.tolist() == list(range(30)) and pj.tolist() == list(range(30))
+    pi, pj = oracle.mapping(oracle.cost_matrix(s, np.r_[s[0], s[:-1]], 30))
+    assert (pi[0], pj[0], pi[-1], pj[-1]) == (0, 0, 29, 29) and len(pi) == 31

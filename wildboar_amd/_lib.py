@@ -30,7 +30,8 @@ EXPORTS = (
     "wbx_prof_valu_peak", "wbx_dev_eos_lengths", "wbx_batch_create", "wbx_batch_free",
     "wbx_batch_set_launch", "wbx_batch_autotune", "wbx_batch_bucket_info", "wbx_batch_info", "wbx_batch_run", "wbx_dtw_train_sizes", "wbx_dev_dtw_train_forward",
     "wbx_dev_dtw_train_backward", "wbx_argmin_subsequence", "wbx_subsequence_profile",
-    "wbx_dev_argmin_subsequence", "wbx_dev_subsequence_profile",
+    "wbx_dev_argmin_subsequence", "wbx_dev_subsequence_profile", "wbx_dev_dtw_path",
+    "wbx_dev_dtw_mm_update",
 )
 
 
@@ -84,6 +85,8 @@ def load():
             "wbx_prof_valu_peak": (i32, [p]),
             "wbx_dev_eos_lengths": (i32, [i32, p, i64, i64, i64, p, p]),
             "wbx_batch_create": (i32, [m, i64, i64, p, p, p, i32, p]),
+            "wbx_dev_dtw_path": (i32, [p, i64, i64, p, i64, i64, i64, i64, p, p]),
+            "wbx_dev_dtw_mm_update": (i32, [p, i64, p, i64, i64, i64, i64, p, p, p]),
             "wbx_batch_free": (i32, [p]),
             "wbx_batch_set_launch": (i32, [p, i32, i32]),
             "wbx_batch_autotune": (i32, [p, i32, p, i64, p, i64, p, i32, p]),

@@ -919,6 +919,74 @@ int wbx_dev_subsequence_pairwise(const wbx_metric *metric, int dtype, const void
     return rc ? rc : rc2;
 }
 
+// ------------------------------------------------------- warping paths
+// Pairs per launch so that the move bytes stay within 512 MiB.
+static int64_t path_chunk(int64_t n, int64_t lx, int64_t B) {
+    const int64_t per = lx * B;
+    int64_t c = ((int64_t)512 << 20) / (per > 0 ? per : 1);
+    if (c < 1) c = 1;
+    return c < n ? c : n;
+}
+
+int wbx_dev_dtw_path(const double *X, int64_t x_stride, int64_t tx, const double *Y,
+                     int64_t y_stride, int64_t ty, int64_t n, int64_t w, int32_t *rows,
+                     void *stream) {
+    if (!X || !Y || !rows) return fail(WBX_EINVAL, "NULL argument");
+    RC(check_sizes(n, tx));
+    RC(check_sizes(n, ty));
+    if (w < 1) return fail(WBX_EINVAL, "warp size must be >= 1, got %lld", (long long)w);
+    if (w > (tx > ty ? tx : ty)) w = tx > ty ? tx : ty;
+    if (x_stride < 0 || y_stride < 0) return fail(WBX_EINVAL, "negative stride");
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    DevGuard g(ctx->device);
+    hipStream_t st = pick_stream(ctx, stream);
+    RC(begin(ctx, st));
+    const Band bd = make_band((int)tx, (int)ty, 0.0, (int)w);
+    const int64_t chunk = path_chunk(n, tx, bd.B);
+    void *wsp = nullptr;
+    int rc = ws_reserve(ctx, (size_t)chunk * (size_t)tx * (size_t)bd.B, &wsp);
+    for (int64_t p0 = 0; !rc && p0 < n; p0 += chunk) {
+        const int64_t c = n - p0 < chunk ? n - p0 : chunk;
+        rc = launch_path_rows(X + p0 * x_stride, x_stride, Y + p0 * y_stride, y_stride, c, (int)tx,
+                              (int)ty, (int)w, (unsigned char *)wsp, rows + p0 * tx * 2, st);
+    }
+    int rc2 = end(ctx, st);
+    return rc ? rc : rc2;
+}
+
+int wbx_dev_dtw_mm_update(const double *mean, int64_t tm, const double *X, int64_t x_stride,
+                          int64_t tx, int64_t n, int64_t w, const double *sample_weight,
+                          double *mean_out, void *stream) {
+    if (!mean || !X || !mean_out) return fail(WBX_EINVAL, "NULL argument");
+    RC(check_sizes(n, tx));
+    RC(check_sizes(1, tm));
+    if (w < 1) return fail(WBX_EINVAL, "warp size must be >= 1, got %lld", (long long)w);
+    if (w > (tm > tx ? tm : tx)) w = tm > tx ? tm : tx;
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    DevGuard g(ctx->device);
+    hipStream_t st = pick_stream(ctx, stream);
+    RC(begin(ctx, st));
+    const Band bd = make_band((int)tm, (int)tx, 0.0, (int)w);
+    const int64_t chunk = path_chunk(n, tm, bd.B);
+    Carve cv;
+    const size_t o_rows = cv.take((size_t)n * (size_t)tm * 8);
+    const size_t o_mv = cv.take((size_t)chunk * (size_t)tm * (size_t)bd.B);
+    void *wsp = nullptr;
+    int rc = ws_reserve(ctx, cv.total, &wsp);
+    int32_t *rows = wsp ? (int32_t *)((char *)wsp + o_rows) : nullptr;
+    // every pair aligns the same mean (stride 0) with its series
+    for (int64_t p0 = 0; !rc && p0 < n; p0 += chunk) {
+        const int64_t c = n - p0 < chunk ? n - p0 : chunk;
+        rc = launch_path_rows(mean, 0, X + p0 * x_stride, x_stride, c, (int)tm, (int)tx, (int)w,
+                              (unsigned char *)wsp + o_mv, rows + p0 * tm * 2, st);
+    }
+    if (!rc) rc = launch_mm_update(rows, X, x_stride, n, (int)tm, sample_weight, mean_out, st);
+    int rc2 = end(ctx, st);
+    return rc ? rc : rc2;
+}
+
 // ------------------------------------------------------------ host drop-ins
 int wbx_pairwise(const wbx_metric *metric, const double *X, int64_t nx, int64_t dims, int64_t tx,
                  const double *Y, int64_t ny, int64_t ty, int64_t dim, double *out) {

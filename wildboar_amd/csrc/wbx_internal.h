@@ -234,6 +234,17 @@ int launch_subseq_exact(int kind, double r, Shapelets sh, int max_len, Series<T>
                         const T *weights, double *dist, int64_t *idx, hipStream_t st,
                         int lane_cap = 0);
 
+// --------------------------------------------------------- warping paths
+// dtw_mapping's path of each pair (X row p, Y row p), band width w, as
+// per-row column ranges rows[p][2 i .. 2 i + 1]; moves: n * lx * B bytes.
+int launch_path_rows(const double *X, int64_t xstride, const double *Y, int64_t ystride,
+                     int64_t n, int lx, int ly, int w, unsigned char *moves, int32_t *rows,
+                     hipStream_t st);
+// mean[m] = sum of the X values aligned to m / their count (rows from
+// launch_path_rows with the mean as X), in the reference's order.
+int launch_mm_update(const int32_t *rows, const double *X, int64_t xstride, int64_t n, int lm,
+                     const double *sample_weight, double *mean, hipStream_t st);
+
 // ------------------------------------------------------- float64 lane kernels
 // Register-band FP64 DTW/WDTW, bit-identical to the reference.  lane64_class
 // gives the register class (a multiple of 8) for band width B, or 0 when B is

@@ -15,6 +15,9 @@ cell) and the backward walks them; nothing is recomputed on the CPU.
 ``dtw_barycenter`` averages series by gradient descent on the mean DTW
 distance (Adam), the gradient-based counterpart of ``dtw_average``
 (``distance/dtw.py:429-690``), which uses alignment (MM/SSG) updates.
+
+``dtw_mapping`` and ``dtw_average`` (majorize-minimize) mirror the
+reference's functions of the same names with float64 paths on the GPU.
 """
 
 import ctypes
@@ -120,3 +123,128 @@ def dtw_barycenter(X, r=1.0, *, steps=200, lr=1e-2, init=None, return_history=Fa
     if return_history:
         return mean, torch.stack(history).cpu().numpy()
     return mean
+
+
+# ------------------------------------------------- alignment paths and MM DBA
+def _warp_size(x_size, r, y_size=0):
+    """_compute_warp_size (distance/dtw.py:38-40)."""
+    import math
+    import numbers
+
+    if isinstance(r, bool) or not isinstance(r, numbers.Real) or not (0 <= r <= 1):
+        raise ValueError(f"r == {r!r}, must be a float in [0, 1].")
+    return max(math.floor(max(x_size, y_size) * r), 1)
+
+
+def _series_1d(a, name):
+    import numpy as np
+
+    a = np.asarray(a, dtype=np.float64)
+    if a.ndim > 1 and a.size == max(a.shape):
+        a = a.ravel()
+    if a.ndim != 1 or a.shape[0] < 1:
+        raise ValueError(f"{name} must be a non-empty 1-D array, got shape {a.shape}")
+    if not np.isfinite(a).all():
+        raise ValueError(f"Input {name} contains NaN or infinity.")
+    return np.ascontiguousarray(a)
+
+
+def _path_rows(x_d, y_d, w):
+    """Device (lx, 2) int32 column ranges of the optimal path, per row of x."""
+    rows = torch.empty((x_d.shape[-1], 2), dtype=torch.int32, device=x_d.device)
+    with torch.cuda.device(x_d.device):
+        _lib.check(_lib.load().wbx_dev_dtw_path(
+            x_d.data_ptr(), 0, x_d.shape[-1], y_d.data_ptr(), 0, y_d.shape[-1], 1, int(w),
+            rows.data_ptr(), torch.cuda.current_stream().cuda_stream))
+    return rows
+
+
+def dtw_mapping(x=None, y=None, *, alignment=None, r=1, return_index=False):
+    """Boolean (len(x), len(y)) indicator of the optimal warping path
+    (``distance/dtw.py:362-427``), computed on the GPU in float64: the cost
+    cells of ``dtw_alignment`` and the backtrack that prefers the diagonal,
+    then up, then left move.  With ``return_index`` also ``indicator.nonzero()``.
+    ``alignment`` (a precomputed matrix) is not served here."""
+    import numpy as np
+
+    if alignment is not None:
+        raise NotImplementedError("dtw_mapping from a precomputed alignment is not served")
+    if x is None or y is None:
+        raise ValueError("if alignment=None, neither x or y can be None")
+    x = _series_1d(x, "x")
+    y = _series_1d(y, "y")
+    w = _warp_size(x.shape[0], r, y_size=y.shape[0])
+    dev = torch.device("cuda", torch.cuda.current_device())
+    rows = _path_rows(torch.from_numpy(x).to(dev), torch.from_numpy(y).to(dev), w).cpu().numpy()
+    indicator = np.zeros((x.shape[0], y.shape[0]), dtype=bool)
+    for i, (lo, hi) in enumerate(rows):
+        indicator[i, lo:hi + 1] = True
+    if return_index:
+        return indicator, indicator.nonzero()
+    return indicator
+
+
+def dtw_average(X, *, r=1.0, g=None, sample_weight=None, init="random", method="mm",
+                max_stable=5, learning_rate=0.1, decay=0.9, tol=1e-5, max_epoch=50,
+                return_cost=False, verbose=False, random_state=None):
+    """DTW barycenter by majorize-minimize (``distance/dtw.py:429-690``,
+    ``_mm_dtw_average`` :640-666): each epoch aligns the mean with every
+    series on the GPU (float64 paths), sets each mean value to the average of
+    the values aligned to it -- in the reference's order of additions -- and
+    stops when the mean DTW distance changes by less than ``tol``.  Results
+    equal the reference's for ``g=None``; WDTW alignments (``g``) and the
+    ``"ssg"`` method are not served."""
+    import numpy as np
+    from sklearn.utils import check_random_state
+    from sklearn.utils.validation import _check_sample_weight
+
+    from . import distance as wd
+
+    X = wd.check_array(X, ensure_2d=True, allow_3d=False)
+    if X.shape[0] < 2:
+        raise ValueError(f"Found array with {X.shape[0]} sample(s) (shape={X.shape}) while a "
+                         "minimum of 2 is required.")
+    _warp_size(X.shape[1], r)
+    random_state = check_random_state(random_state)
+    if isinstance(init, str) and init == "random":
+        mean = X[random_state.randint(X.shape[0])].copy()
+    elif hasattr(init, "__len__") or hasattr(init, "__array__"):
+        mean = _series_1d(np.array(init, copy=True), "init").copy()
+    else:
+        raise ValueError("init must be array-like or 'random', not %r" % type(init).__qualname__)
+    if sample_weight is not None:
+        sample_weight = np.ascontiguousarray(_check_sample_weight(sample_weight, X),
+                                             dtype=np.float64)
+    if g is not None:
+        raise NotImplementedError("dtw_average with WDTW alignments (g) is not served")
+    if method == "ssg":
+        raise NotImplementedError("dtw_average(method='ssg') is not served")
+    if method != "mm":
+        raise ValueError("method must be 'mm' or 'ssg', got %r" % method)
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    Xd = torch.from_numpy(np.ascontiguousarray(X)).to(dev)
+    wd_ = None if sample_weight is None else torch.from_numpy(sample_weight).to(dev)
+    lib = _lib.load()
+
+    def costfn(mean):
+        d = wd.pairwise_distance(mean, X, metric="dtw", metric_params={"r": r})
+        return np.mean(d) if sample_weight is None else np.average(d, weights=sample_weight)
+
+    w = _warp_size(mean.shape[0], r, y_size=X.shape[1])
+    cost = costfn(mean)
+    for epoch in range(max_epoch):
+        md = torch.from_numpy(mean).to(dev)
+        out = torch.empty_like(md)
+        with torch.cuda.device(dev):
+            _lib.check(lib.wbx_dev_dtw_mm_update(
+                md.data_ptr(), md.shape[0], Xd.data_ptr(), Xd.stride(0), Xd.shape[1],
+                Xd.shape[0], int(w), None if wd_ is None else wd_.data_ptr(), out.data_ptr(),
+                torch.cuda.current_stream().cuda_stream))
+        mean = out.cpu().numpy()
+        prev_cost, cost = cost, costfn(mean)
+        if abs(prev_cost - cost) < tol:
+            if verbose:
+                print(f"Complete at epoch={epoch} with cost={cost}.")
+            break
+    return (mean, cost) if return_cost else mean
