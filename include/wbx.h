@@ -207,6 +207,22 @@ int wbx_dev_subsequence_profile(const wbx_metric *metric, int dtype,
                                 int64_t t, int64_t nwp, double *prof,
                                 void *stream);
 
+/* ----------------------------------------------------------- lower bounds */
+
+/* DTW envelope of n float64 device rows (dtw_envelop, distance/dtw.py:155-184;
+ * _dtw_envelop + find_min_max, _elastic.pyx:1076-1092, 98-150): lower/upper
+ * (n, t) packed = min/max over [k - r, k + r]; 0 <= r < t. */
+int wbx_dev_dtw_envelope(const double *X, int64_t n, int64_t stride, int64_t t, int64_t r,
+                         double *lower, double *upper, void *stream);
+/* LB_Keogh of every X row against every envelope row (_dtw_lb_keogh,
+ * _elastic.pyx:1095-1116; cumulative_bound :229-260): out (nx, ny). */
+int wbx_dev_lb_keogh(const double *X, int64_t nx, int64_t x_stride, const double *lower,
+                     const double *upper, int64_t ny, int64_t t, double *out, void *stream);
+/* DtwKimLowerBound.transform (distance/lb.py:206-312): out[i, j] for query
+ * Y[i] and fitted X[j], (ny, nx). */
+int wbx_dev_lb_kim(const double *Y, int64_t ny, int64_t y_stride, const double *X, int64_t nx,
+                   int64_t x_stride, int64_t t, double *out, void *stream);
+
 /* ------------------------------------------------------- warping paths */
 
 /* dtw_mapping's optimal path (distance/dtw.py:362-427, cells of

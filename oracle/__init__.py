@@ -92,6 +92,10 @@ def _load():
     lib.orc_average_slope.argtypes = [p, i64, p]
     lib.orc_adtw_distance.restype = d
     lib.orc_adtw_distance.argtypes = [p, i64, p, i64, i64, p, p, d, d]
+    lib.orc_dtw_envelop.restype = i32
+    lib.orc_dtw_envelop.argtypes = [p, i64, i64, p, p]
+    lib.orc_lb_keogh.restype = d
+    lib.orc_lb_keogh.argtypes = [p, i64, p, p]
     lib.orc_subsequence_matches.restype = i64
     lib.orc_subsequence_matches.argtypes = [i32, d, d, p, i64, p, i64, d, p, p]
     _lib = lib
@@ -365,3 +369,50 @@ def mm_dtw_average(X, mean, r=1.0, tol=1e-5, max_epoch=50, sample_weight=None):
         if abs(prev - cost) < tol:
             break
     return mean, cost
+
+
+def dtw_envelop(x, r):
+    """(lower, upper) of a 1-D series for an integer warp r (the deque pass)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    lower = np.empty(x.shape[0])
+    upper = np.empty(x.shape[0])
+    if _load().orc_dtw_envelop(x.ctypes.data, x.shape[0], int(r), lower.ctypes.data,
+                               upper.ctypes.data) != 0:
+        raise MemoryError("oracle allocation failed")
+    return lower, upper
+
+
+def lb_keogh(x, lower, upper):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    lower = np.ascontiguousarray(lower, dtype=np.float64)
+    upper = np.ascontiguousarray(upper, dtype=np.float64)
+    return _load().orc_lb_keogh(x.ctypes.data, x.shape[0], lower.ctypes.data, upper.ctypes.data)
+
+
+def lb_kim(X_fit, Y):
+    """DtwKimLowerBound.transform, distance/lb.py:206-312 (numpy, as the
+    reference): (len(Y), len(X_fit))."""
+    X_fit = np.asarray(X_fit, dtype=np.float64)
+    Y = np.asarray(Y, dtype=np.float64)
+
+    def dist(a, b):
+        v = a - b
+        return v * v
+
+    out = np.empty((Y.shape[0], X_fit.shape[0]))
+    t = X_fit.shape[-1]
+    for i in range(Y.shape[0]):
+        x0, x0_, y0, y0_ = X_fit[:, 0], X_fit[:, -1], Y[i, 0], Y[i, -1]
+        d = dist(x0, y0) + dist(x0_, y0_)
+        if t > 1:
+            x1, x1_, y1, y1_ = X_fit[:, 1], X_fit[:, -2], Y[i, 1], Y[i, -2]
+            d += np.minimum(dist(x1, y0), np.minimum(dist(x0, y1), dist(x1, y1)))
+            d += np.minimum(dist(x1_, y1), np.minimum(dist(x0_, y1_), dist(x1_, y1_)))
+            if t > 2:
+                x2, x2_, y2, y2_ = X_fit[:, 2], X_fit[:, -3], Y[i, 2], Y[i, -3]
+                d += np.minimum(dist(x0, y2), np.minimum(dist(x1, y2), np.minimum(
+                    dist(x2, y2), np.minimum(dist(x2, y1), dist(x2, y0)))))
+                d += np.minimum(dist(x0_, y2_), np.minimum(dist(x1_, y2_), np.minimum(
+                    dist(x2_, y2_), np.minimum(dist(x2_, y1_), dist(x2_, y0_)))))
+        out[i, :] = d
+    return out

@@ -164,6 +164,70 @@ double orc_adtw_distance(const double *X, int64_t x_length, const double *Y,
     return cost_prev[y_length - 1];
 }
 
+/* find_min_max, distance/_elastic.pyx:98-150: the deque pass behind
+ * _dtw_envelop (:1076-1092).  The deques hold at most 2 r + 2 indices. */
+int orc_dtw_envelop(const double *T, int64_t length, int64_t r, double *lower,
+                    double *upper) {
+    int64_t cap = 2 * r + 2 + length, *du = malloc(sizeof(int64_t) * (size_t)cap),
+            *dl = malloc(sizeof(int64_t) * (size_t)cap);
+    if (!du || !dl) { free(du); free(dl); return -1; }
+    /* deque as [front, back) over a buffer that only grows at the back */
+    int64_t uf = 0, ub = 0, lf = 0, lb = 0, i, k;
+    du[ub++] = 0;
+    dl[lb++] = 0;
+    for (i = 1; i < length; i++) {
+        if (i > r) {
+            k = i - r - 1;
+            upper[k] = T[du[uf]];
+            lower[k] = T[dl[lf]];
+        }
+        double current = T[i], prev = T[i - 1];
+        if (current > prev) {
+            ub--;
+            while (ub > uf && current > T[du[ub - 1]]) ub--;
+        } else {
+            lb--;
+            while (lb > lf && current < T[dl[lb - 1]]) lb--;
+        }
+        du[ub++] = i;
+        dl[lb++] = i;
+        if (i == 2 * r + 1 + du[uf])
+            uf++;
+        else if (i == 2 * r + 1 + dl[lf])
+            lf++;
+    }
+    for (i = length; i < length + r + 1; i++) {
+        k = i - r - 1;
+        upper[k] = T[du[uf]];
+        lower[k] = T[dl[lf]];
+        if (i - du[uf] >= 2 * r + 1) uf++;
+        if (i - dl[lf] >= 2 * r + 1) lf++;
+    }
+    free(du);
+    free(dl);
+    return 0;
+}
+
+/* cumulative_bound with mean 0, std 1 (distance/_elastic.pyx:229-260), as
+ * _dtw_lb_keogh calls it (:1095-1116): returns sqrt of the sum. */
+double orc_lb_keogh(const double *T, int64_t length, const double *lower,
+                    const double *upper) {
+    double min_dist = 0, x, d, us, ls;
+    for (int64_t i = 0; i < length; i++) {
+        x = (T[i] - 0.0) / 1.0;
+        us = (upper[i] - 0.0) / 1.0;
+        ls = (lower[i] - 0.0) / 1.0;
+        if (x > us)
+            d = (x - us) * (x - us);
+        else if (x < ls)
+            d = (x - ls) * (x - ls);
+        else
+            d = 0;
+        min_dist += d;
+    }
+    return sqrt(min_dist);
+}
+
 /* euclidean_distance, distance/_metric.pyx:654-683 (partial-sum abandon). */
 double orc_euclidean_distance(const double *S, int64_t s_length,
                               const double *T, int64_t t_length,

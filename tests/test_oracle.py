@@ -495,3 +495,34 @@ def test_mm_dtw_average_fixed_points():
     assert pi.tolist() == list(range(30)) and pj.tolist() == list(range(30))
     pi, pj = oracle.mapping(oracle.cost_matrix(s, np.r_[s[0], s[:-1]], 30))
     assert (pi[0], pj[0], pi[-1], pj[-1]) == (0, 0, 29, 29) and len(pi) == 31
+
+
+# ------------------------------------------------------------- lower bounds
+def test_envelope_deque_is_the_windowed_min_max():
+    rng = np.random.default_rng(34)
+    for trial in range(600):
+        n = int(rng.integers(1, 40))
+        r = int(rng.integers(0, n))
+        x = rng.integers(0, 4, n).astype(float) if trial % 2 else rng.standard_normal(n)
+        lo, up = oracle.dtw_envelop(x, r)
+        assert lo.tolist() == [x[max(0, k - r):k + r + 1].min() for k in range(n)], (n, r)
+        assert up.tolist() == [x[max(0, k - r):k + r + 1].max() for k in range(n)], (n, r)
+
+
+def test_lb_keogh_known_answer_and_bound():
+    assert oracle.lb_keogh([0.0, 3.0, 0.0], [1.0] * 3, [2.0] * 3) == math.sqrt(3.0)
+    assert oracle.lb_keogh([1.5, 1.0, 2.0], [1.0] * 3, [2.0] * 3) == 0.0
+    rng = np.random.default_rng(35)
+    X = rw(rng, 20, 50)
+    for r in (0.05, 0.2, 1.0):
+        warp = max(math.floor(50 * r), 1)
+        warp = warp - 1 if warp == 50 else warp
+        for i in range(0, 20, 2):
+            lo, up = oracle.dtw_envelop(X[i + 1], warp)
+            lb = oracle.lb_keogh(X[i], lo, up)
+            assert lb <= oracle.metric_distance(X[i], X[i + 1], "dtw", r=r) + 1e-12
+    # LB_Kim (squared) bounds the squared unconstrained DTW cost
+    K = oracle.lb_kim(X[:10], X[10:])
+    for i in range(10):
+        for j in range(10):
+            assert K[i, j] <= oracle.metric_distance(X[j], X[10 + i], "dtw", r=1.0) ** 2 + 1e-12

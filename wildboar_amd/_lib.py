@@ -31,7 +31,7 @@ EXPORTS = (
     "wbx_batch_set_launch", "wbx_batch_autotune", "wbx_batch_bucket_info", "wbx_batch_info", "wbx_batch_run", "wbx_dtw_train_sizes", "wbx_dev_dtw_train_forward",
     "wbx_dev_dtw_train_backward", "wbx_argmin_subsequence", "wbx_subsequence_profile",
     "wbx_dev_argmin_subsequence", "wbx_dev_subsequence_profile", "wbx_dev_dtw_path",
-    "wbx_dev_dtw_mm_update",
+    "wbx_dev_dtw_mm_update", "wbx_dev_dtw_envelope", "wbx_dev_lb_keogh", "wbx_dev_lb_kim",
 )
 
 
@@ -87,6 +87,9 @@ def load():
             "wbx_batch_create": (i32, [m, i64, i64, p, p, p, i32, p]),
             "wbx_dev_dtw_path": (i32, [p, i64, i64, p, i64, i64, i64, i64, p, p]),
             "wbx_dev_dtw_mm_update": (i32, [p, i64, p, i64, i64, i64, i64, p, p, p]),
+            "wbx_dev_dtw_envelope": (i32, [p, i64, i64, i64, i64, p, p, p]),
+            "wbx_dev_lb_keogh": (i32, [p, i64, i64, p, p, i64, i64, p, p]),
+            "wbx_dev_lb_kim": (i32, [p, i64, i64, p, i64, i64, i64, p, p]),
             "wbx_batch_free": (i32, [p]),
             "wbx_batch_set_launch": (i32, [p, i32, i32]),
             "wbx_batch_autotune": (i32, [p, i32, p, i64, p, i64, p, i32, p]),

@@ -987,6 +987,48 @@ int wbx_dev_dtw_mm_update(const double *mean, int64_t tm, const double *X, int64
     return rc ? rc : rc2;
 }
 
+// ----------------------------------------------------------- lower bounds
+int wbx_dev_dtw_envelope(const double *X, int64_t n, int64_t stride, int64_t t, int64_t r,
+                         double *lower, double *upper, void *stream) {
+    if (!X || !lower || !upper) return fail(WBX_EINVAL, "NULL argument");
+    RC(check_sizes(n, t));
+    if (r < 0 || r >= t) return fail(WBX_EINVAL, "invalid r");
+    if (stride < t) return fail(WBX_EINVAL, "stride %lld < length %lld", (long long)stride,
+                                (long long)t);
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    DevGuard g(ctx->device);
+    hipStream_t st = pick_stream(ctx, stream);
+    return launch_envelope(X, n, stride, (int)t, (int)r, lower, upper, st);
+}
+
+int wbx_dev_lb_keogh(const double *X, int64_t nx, int64_t x_stride, const double *lower,
+                     const double *upper, int64_t ny, int64_t t, double *out, void *stream) {
+    if (!X || !lower || !upper || !out) return fail(WBX_EINVAL, "NULL argument");
+    RC(check_sizes(nx, t));
+    RC(check_sizes(ny, t));
+    if (x_stride < t) return fail(WBX_EINVAL, "stride %lld < length %lld", (long long)x_stride,
+                                  (long long)t);
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    DevGuard g(ctx->device);
+    hipStream_t st = pick_stream(ctx, stream);
+    return launch_lb_keogh(X, nx, x_stride, lower, upper, ny, (int)t, out, st);
+}
+
+int wbx_dev_lb_kim(const double *Y, int64_t ny, int64_t y_stride, const double *X, int64_t nx,
+                   int64_t x_stride, int64_t t, double *out, void *stream) {
+    if (!X || !Y || !out) return fail(WBX_EINVAL, "NULL argument");
+    RC(check_sizes(nx, t));
+    RC(check_sizes(ny, t));
+    if (x_stride < t || y_stride < t) return fail(WBX_EINVAL, "stride below the length");
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    DevGuard g(ctx->device);
+    hipStream_t st = pick_stream(ctx, stream);
+    return launch_lb_kim(Y, ny, y_stride, X, nx, x_stride, (int)t, out, st);
+}
+
 // ------------------------------------------------------------ host drop-ins
 int wbx_pairwise(const wbx_metric *metric, const double *X, int64_t nx, int64_t dims, int64_t tx,
                  const double *Y, int64_t ny, int64_t ty, int64_t dim, double *out) {

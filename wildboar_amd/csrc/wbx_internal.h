@@ -245,6 +245,14 @@ int launch_path_rows(const double *X, int64_t xstride, const double *Y, int64_t 
 int launch_mm_update(const int32_t *rows, const double *X, int64_t xstride, int64_t n, int lm,
                      const double *sample_weight, double *mean, hipStream_t st);
 
+// ----------------------------------------------------------- lower bounds
+int launch_envelope(const double *X, int64_t n, int64_t stride, int t, int r, double *lower,
+                    double *upper, hipStream_t st);
+int launch_lb_keogh(const double *X, int64_t nx, int64_t x_stride, const double *lower,
+                    const double *upper, int64_t ny, int t, double *out, hipStream_t st);
+int launch_lb_kim(const double *Y, int64_t ny, int64_t y_stride, const double *X, int64_t nx,
+                  int64_t x_stride, int t, double *out, hipStream_t st);
+
 // ------------------------------------------------------- float64 lane kernels
 // Register-band FP64 DTW/WDTW, bit-identical to the reference.  lane64_class
 // gives the register class (a multiple of 8) for band width B, or 0 when B is

@@ -248,3 +248,43 @@ def dtw_average(X, *, r=1.0, g=None, sample_weight=None, init="random", method="
                 print(f"Complete at epoch={epoch} with cost={cost}.")
             break
     return (mean, cost) if return_cost else mean
+
+
+def dtw_envelop(x, *, r=1.0):
+    """(lower, upper) warping envelope of a series (``distance/dtw.py:155-184``),
+    on the GPU."""
+    from .lb import _envelope, envelope_warp
+
+    x = _series_1d(x, "x")
+    dev = torch.device("cuda", torch.cuda.current_device())
+    lower, upper = _envelope(torch.from_numpy(x).to(dev)[None, :], envelope_warp(x.shape[0], r))
+    return lower[0].cpu().numpy(), upper[0].cpu().numpy()
+
+
+def dtw_lb_keogh(x, y=None, *, lower=None, upper=None, r=1.0):
+    """(LB_Keogh, per-timestep terms) of x against the envelope of y or the
+    given one (``distance/dtw.py:187-243``)."""
+    import numpy as np
+
+    x = _series_1d(x, "x")
+    if y is not None:
+        y = _series_1d(y, "y")
+        if y.shape[0] != x.shape[0]:
+            raise ValueError("x (%d) and y (%d) must have the same number of timesteps"
+                             % (x.shape[0], y.shape[0]))
+        lower, upper = dtw_envelop(y, r=r)
+    elif lower is None or upper is None:
+        raise ValueError("both y, lower and upper can't be None")
+    lower = _series_1d(lower, "lower")
+    upper = _series_1d(upper, "upper")
+    if lower.shape[0] != upper.shape[0] or lower.shape[0] != x.shape[0]:
+        raise ValueError("lower (%d), upper (%d) and x (%d) have the same number of timesteps"
+                         % (lower.shape[0], upper.shape[0], x.shape[0]))
+    from .lb import _lb_keogh
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    d = _lb_keogh(torch.from_numpy(x).to(dev)[None, :], torch.from_numpy(lower).to(dev)[None, :],
+                  torch.from_numpy(upper).to(dev)[None, :])
+    # the per-timestep terms of the bound, as cumulative_bound's cb
+    cb = np.where(x > upper, (x - upper) ** 2, np.where(x < lower, (x - lower) ** 2, 0.0))
+    return float(d[0, 0].item()), cb
