@@ -335,6 +335,13 @@ def run_fp64(torch, dev):
     got = device.pairwise(A[:4], Bm, metric="dtw", r=0.05, precision="fp64").cpu().numpy()
     want = oracle.pairwise(Xh[:4, :256], Xh[1000:, :256], metric="dtw", r=0.05)
     res["pairwise_dtw_t256_r0.05"]["mismatch_4_rows"] = int((got != want).sum())
+    # 4. the same at r = 0.1 (B = 49, the widest register class)
+    both("pairwise_dtw_t256_r0.1",
+         lambda: device.pairwise(A, Bm, metric="dtw", r=0.1, precision="fp64"),
+         1e6 * band_cells(256, 256, 0.1))
+    got = device.pairwise(A[:4], Bm, metric="dtw", r=0.1, precision="fp64").cpu().numpy()
+    want = oracle.pairwise(Xh[:4, :256], Xh[1000:, :256], metric="dtw", r=0.1)
+    res["pairwise_dtw_t256_r0.1"]["mismatch_4_rows"] = int((got != want).sum())
     return res
 
 

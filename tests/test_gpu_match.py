@@ -204,12 +204,13 @@ def test_profile_entry_validation():
 
 
 def test_float64_lane_classes_pairs_profiles_and_scans():
-    """Band widths that land in every float64 register class (B = 1 .. 47)
+    """Band widths that land in every float64 register class (B = 1 .. 55)
     and past it (block kernels), weighted and not, equal and unequal
     lengths; everything bit-identical to the oracle."""
     X = rw(70, 6, 300)
     Y = rw(71, 5, 300)
-    for t, r in ((200, 0.1), (240, 0.1), (150, 0.1), (90, 0.05), (300, 0.16)):
+    for t, r in ((200, 0.1), (240, 0.1), (256, 0.1), (280, 0.1), (150, 0.1), (90, 0.05),
+                 (300, 0.16)):
         for metric in ("dtw", "wdtw"):
             got = wd.pairwise_distance(X[:, :t], Y[:, :t], metric=metric, metric_params={"r": r})
             want = oracle.pairwise(X[:, :t], Y[:, :t], metric=metric, r=r)

@@ -16,6 +16,8 @@
 // routes other weight vectors to the block kernels).  Slots past the band
 // width B (padding up to BP) are reset to +inf after every row.
 //
+// Register classes BP = 8 .. 56 slots (WDTW: .. 32, the per-slot weights
+// take registers too).
 // Kernels:
 //   k_lane64_pairs   -- pairwise / paired / argmin rows with one geometry
 //   k_profile_lane64 -- window profiles: row q, windows 64 b .. 64 b + 63 of
@@ -220,10 +222,10 @@ __global__ __launch_bounds__(64) void k_subseq_lane64(Shapelets sh, Series<doubl
     }
 }
 
-constexpr int kMaxBP = 48;    // DTW
+constexpr int kMaxBP = 56;    // DTW (56: 256 VGPRs plus AGPR spill slots, no scratch)
 constexpr int kMaxBPW = 32;   // WDTW (per-slot weights in registers too)
 
-#define WBX_LANE64_CLASSES(X) X(8) X(16) X(24) X(32) X(40) X(48)
+#define WBX_LANE64_CLASSES(X) X(8) X(16) X(24) X(32) X(40) X(48) X(56)
 
 }  // namespace
 
