@@ -233,8 +233,8 @@ int wbx_dev_lb_kim(const double *Y, int64_t ny, int64_t y_stride, const double *
  * first and last column of row i.  x_stride 0 aligns one X row with every Y
  * row. */
 int wbx_dev_dtw_path(const double *X, int64_t x_stride, int64_t tx, const double *Y,
-                     int64_t y_stride, int64_t ty, int64_t n, int64_t w, int32_t *rows,
-                     void *stream);
+                     int64_t y_stride, int64_t ty, int64_t n, int64_t w, const double *weights,
+                     int32_t *rows, void *stream);
 /* One majorize-minimize step of _mm_dtw_average (distance/dtw.py:640-666):
  * aligns `mean` (tm values) with each X row (n rows of tx), then
  * mean_out[m] = sum over rows i and aligned columns x (in order) of

@@ -88,6 +88,8 @@ def _load():
     lib.orc_dtw_grad.argtypes = [p, i64, p, i64, d, p, p, p]
     lib.orc_argmin_subsequence.restype = i32
     lib.orc_argmin_subsequence.argtypes = [i32, d, d, p, p, p, i64, p, i64, i64, i64, i64, p, p]
+    lib.orc_dtw_alignment.restype = i32
+    lib.orc_dtw_alignment.argtypes = [p, i64, p, i64, i64, p, p]
     lib.orc_average_slope.restype = None
     lib.orc_average_slope.argtypes = [p, i64, p]
     lib.orc_adtw_distance.restype = d
@@ -415,4 +417,17 @@ def lb_kim(X_fit, Y):
                 d += np.minimum(dist(x0_, y2_), np.minimum(dist(x1_, y2_), np.minimum(
                     dist(x2_, y2_), np.minimum(dist(x2_, y1_), dist(x2_, y0_)))))
         out[i, :] = d
+    return out
+
+
+def dtw_alignment(x, y, band, weights=None):
+    """_dtw_alignment's matrix (+inf off the band), weights without the row-0
+    quirk."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.empty((x.shape[0], y.shape[0]))
+    if weights is not None:
+        weights = np.ascontiguousarray(weights, dtype=np.float64)
+    _load().orc_dtw_alignment(x.ctypes.data, x.shape[0], y.ctypes.data, y.shape[0], int(band),
+                              _ptr(weights), out.ctypes.data)
     return out

@@ -700,6 +700,34 @@ void orc_argmin_replay(const double *row, int64_t ny, int64_t k,
  *   dD/dx_i = sum_{(i,j) in P} w_ij (x_i - y_j) / D,  dD/dy_j = -sum ... / D,
  * with w_ij the weight the forward applied to that cell; 0 when D == 0.
  * ---------------------------------------------------------------------- */
+/* _dtw_alignment (distance/_elastic.pyx:1011-1073) itself: as
+ * orc_dtw_cost_matrix, but row 0 weighs column j with w[j] (no row-0 quirk),
+ * as wdtw_alignment (distance/dtw.py:292-327) sees its weights. */
+int orc_dtw_alignment(const double *X, int64_t lx, const double *Y, int64_t ly,
+                      int64_t r, const double *weights, double *out) {
+    int64_t max_len = imax(0, ly - lx) + r;
+    int64_t min_len = imax(0, lx - ly);
+    double v, w = 1.0;
+    for (int64_t q = 0; q < lx * ly; q++) out[q] = INFINITY;
+    for (int64_t i = 0; i < lx; i++) {
+        int64_t j_start = i == 0 ? 0 : imax(0, i - min_len - r + 1);
+        int64_t j_stop = i == 0 ? imin(ly, max_len) : imin(ly, i + max_len);
+        for (int64_t j = j_start; j < j_stop; j++) {
+            v = X[i] - Y[j];
+            if (weights) w = weights[labs(i - j)];
+            if (i == 0 && j == 0) {
+                out[0] = v * v * w;
+                continue;
+            }
+            double up = i > 0 ? out[(i - 1) * ly + j] : INFINITY;
+            double left = j > 0 ? out[i * ly + j - 1] : INFINITY;
+            double diag = (i > 0 && j > 0) ? out[(i - 1) * ly + j - 1] : INFINITY;
+            out[i * ly + j] = dmin(dmin(up, left), diag) + v * v * w;
+        }
+    }
+    return 0;
+}
+
 int orc_dtw_cost_matrix(const double *X, int64_t lx, const double *Y,
                         int64_t ly, int64_t r, const double *weights,
                         double *out) {

@@ -526,3 +526,31 @@ def test_lb_keogh_known_answer_and_bound():
     for i in range(10):
         for j in range(10):
             assert K[i, j] <= oracle.metric_distance(X[j], X[10 + i], "dtw", r=1.0) ** 2 + 1e-12
+
+
+def test_alignment_matrix():
+    # unweighted: the alignment is the distance cost matrix; weighted: a
+    # brute-force matrix with w[|i - j|] in every row (row 0 included)
+    rng = np.random.default_rng(36)
+    for lx, ly in ((1, 1), (5, 9), (9, 5), (12, 12)):
+        x = rng.standard_normal(lx)
+        y = rng.standard_normal(ly)
+        for band in (1, 2, 4, max(lx, ly)):
+            np.testing.assert_array_equal(oracle.dtw_alignment(x, y, band),
+                                          oracle.cost_matrix(x, y, band))
+            w = rng.uniform(0.2, 1.5, max(lx, ly))
+            A = oracle.dtw_alignment(x, y, band, w)
+            hi = max(0, lx - ly) + band - 1
+            lo = -(max(0, ly - lx) + band - 1)
+            B = np.full((lx, ly), np.inf)
+            for i in range(lx):
+                for j in range(ly):
+                    if not lo <= i - j <= hi:
+                        continue
+                    c = (x[i] - y[j]) * (x[i] - y[j]) * w[abs(i - j)]
+                    if i or j:
+                        m = min(B[i - 1, j] if i else np.inf, B[i, j - 1] if j else np.inf,
+                                B[i - 1, j - 1] if i and j else np.inf)
+                        c = m + c
+                    B[i, j] = c
+            np.testing.assert_array_equal(A, B)

@@ -929,8 +929,8 @@ static int64_t path_chunk(int64_t n, int64_t lx, int64_t B) {
 }
 
 int wbx_dev_dtw_path(const double *X, int64_t x_stride, int64_t tx, const double *Y,
-                     int64_t y_stride, int64_t ty, int64_t n, int64_t w, int32_t *rows,
-                     void *stream) {
+                     int64_t y_stride, int64_t ty, int64_t n, int64_t w, const double *weights,
+                     int32_t *rows, void *stream) {
     if (!X || !Y || !rows) return fail(WBX_EINVAL, "NULL argument");
     RC(check_sizes(n, tx));
     RC(check_sizes(n, ty));
@@ -949,7 +949,8 @@ int wbx_dev_dtw_path(const double *X, int64_t x_stride, int64_t tx, const double
     for (int64_t p0 = 0; !rc && p0 < n; p0 += chunk) {
         const int64_t c = n - p0 < chunk ? n - p0 : chunk;
         rc = launch_path_rows(X + p0 * x_stride, x_stride, Y + p0 * y_stride, y_stride, c, (int)tx,
-                              (int)ty, (int)w, (unsigned char *)wsp, rows + p0 * tx * 2, st);
+                              (int)ty, (int)w, weights, (unsigned char *)wsp, rows + p0 * tx * 2,
+                              st);
     }
     int rc2 = end(ctx, st);
     return rc ? rc : rc2;
@@ -980,7 +981,7 @@ int wbx_dev_dtw_mm_update(const double *mean, int64_t tm, const double *X, int64
     for (int64_t p0 = 0; !rc && p0 < n; p0 += chunk) {
         const int64_t c = n - p0 < chunk ? n - p0 : chunk;
         rc = launch_path_rows(mean, 0, X + p0 * x_stride, x_stride, c, (int)tm, (int)tx, (int)w,
-                              (unsigned char *)wsp + o_mv, rows + p0 * tm * 2, st);
+                              nullptr, (unsigned char *)wsp + o_mv, rows + p0 * tm * 2, st);
     }
     if (!rc) rc = launch_mm_update(rows, X, x_stride, n, (int)tm, sample_weight, mean_out, st);
     int rc2 = end(ctx, st);

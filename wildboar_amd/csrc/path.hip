@@ -4,7 +4,8 @@
 // k_path_moves: one pair per 64-lane block.  The anti-diagonal sweep of
 //   block_dtw computes the cells of _dtw_alignment (distance/_elastic.pyx:
 //   1011-1073; for DTW the same cell arithmetic as dtw_distance) with band
-//   width w, and stores for every band cell the move dtw_mapping
+//   width w (and, for wdtw_alignment, weights w[|i - j|]), and stores for
+//   every band cell the move dtw_mapping
 //   (distance/dtw.py:362-427) takes out of it: argmin over (diag, up, left)
 //   of the neighbours' costs, first wins, +inf outside the band or matrix.
 // k_path_walk: one pair per thread walks the moves from (lx-1, ly-1) to
@@ -33,6 +34,7 @@ __global__ __launch_bounds__(64) void k_path_moves(const double *__restrict__ X,
                                                    const double *__restrict__ Y, int64_t ystride,
                                                    int64_t n, int lx, int ly, int w, int B,
                                                    int off, int cap,
+                                                   const double *__restrict__ weights,
                                                    unsigned char *__restrict__ moves) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double *d0 = reinterpret_cast<double *>(smem_raw);
@@ -61,13 +63,16 @@ __global__ __launch_bounds__(64) void k_path_moves(const double *__restrict__ X,
             if (i >= ilo && i <= ihi) {
                 const int j = d - i;
                 const double v = x[i] - y[j];
+                // _dtw_alignment weighs every cell, row 0 included, by
+                // w[|i - j|] (1 without weights: v * v * 1 == v * v)
+                const double wt = weights ? weights[i > j ? i - j : j - i] : 1.0;
                 if (i == 0 && j == 0) {
-                    c = v * v;
+                    c = v * v * wt;
                 } else {
                     const double up = i > 0 ? d1[i - 1] : INF;
                     const double left = j > 0 ? d1[i] : INF;
                     const double dg = (i > 0 && j > 0) ? d2[i - 1] : INF;
-                    c = cmin(cmin(up, left), dg) + v * v;
+                    c = cmin(cmin(up, left), dg) + v * v * wt;
                     mv[(int64_t)i * B + (j - i + off)] = move_of(dg, up, left);
                 }
             }
@@ -133,8 +138,8 @@ __global__ __launch_bounds__(256) void k_mm_update(const int32_t *__restrict__ r
 }  // namespace
 
 int launch_path_rows(const double *X, int64_t xstride, const double *Y, int64_t ystride,
-                     int64_t n, int lx, int ly, int w, unsigned char *moves, int32_t *rows,
-                     hipStream_t st) {
+                     int64_t n, int lx, int ly, int w, const double *weights,
+                     unsigned char *moves, int32_t *rows, hipStream_t st) {
     if (n <= 0) return WBX_OK;
     const Band bd = make_band(lx, ly, 0.0, w);
     const int cap = lx;
@@ -146,7 +151,7 @@ int launch_path_rows(const double *X, int64_t xstride, const double *Y, int64_t 
     }
     if (n > 0x7fffffff) return fail(WBX_EUNSUPPORTED, "path: too many pairs");
     hipLaunchKernelGGL(k_path_moves, dim3((unsigned)n), dim3(64), lds, st, X, xstride, Y, ystride,
-                       n, lx, ly, w, bd.B, bd.off, cap, moves);
+                       n, lx, ly, w, bd.B, bd.off, cap, weights, moves);
     WBX_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_path_walk, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, lx, ly,
                        bd.B, bd.off, moves, rows);

@@ -237,9 +237,10 @@ int launch_subseq_exact(int kind, double r, Shapelets sh, int max_len, Series<T>
 // --------------------------------------------------------- warping paths
 // dtw_mapping's path of each pair (X row p, Y row p), band width w, as
 // per-row column ranges rows[p][2 i .. 2 i + 1]; moves: n * lx * B bytes.
+// weights (nullable): w[|i - j|] for every cell, max(lx, ly) values.
 int launch_path_rows(const double *X, int64_t xstride, const double *Y, int64_t ystride,
-                     int64_t n, int lx, int ly, int w, unsigned char *moves, int32_t *rows,
-                     hipStream_t st);
+                     int64_t n, int lx, int ly, int w, const double *weights,
+                     unsigned char *moves, int32_t *rows, hipStream_t st);
 // mean[m] = sum of the X values aligned to m / their count (rows from
 // launch_path_rows with the mean as X), in the reference's order.
 int launch_mm_update(const int32_t *rows, const double *X, int64_t xstride, int64_t n, int lm,

@@ -149,13 +149,15 @@ def _series_1d(a, name):
     return np.ascontiguousarray(a)
 
 
-def _path_rows(x_d, y_d, w):
-    """Device (lx, 2) int32 column ranges of the optimal path, per row of x."""
+def _path_rows(x_d, y_d, w, weights_d=None):
+    """Device (lx, 2) int32 column ranges of the optimal path, per row of x;
+    weights_d: w[|i - j|] per cell (wdtw_alignment), or None."""
     rows = torch.empty((x_d.shape[-1], 2), dtype=torch.int32, device=x_d.device)
     with torch.cuda.device(x_d.device):
         _lib.check(_lib.load().wbx_dev_dtw_path(
             x_d.data_ptr(), 0, x_d.shape[-1], y_d.data_ptr(), 0, y_d.shape[-1], 1, int(w),
-            rows.data_ptr(), torch.cuda.current_stream().cuda_stream))
+            None if weights_d is None else weights_d.data_ptr(), rows.data_ptr(),
+            torch.cuda.current_stream().cuda_stream))
     return rows
 
 
