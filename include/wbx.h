@@ -239,10 +239,11 @@ int wbx_dev_dtw_path(const double *X, int64_t x_stride, int64_t tx, const double
  * aligns `mean` (tm values) with each X row (n rows of tx), then
  * mean_out[m] = sum over rows i and aligned columns x (in order) of
  * X[i, x] * w_i, divided by the summed weights w_i; sample_weight may be
- * NULL (all 1).  All pointers are device pointers. */
+ * NULL (all 1).  weights: the wdtw_alignment cell weights as in
+ * wbx_dev_dtw_path, or NULL.  All pointers are device pointers. */
 int wbx_dev_dtw_mm_update(const double *mean, int64_t tm, const double *X, int64_t x_stride,
-                          int64_t tx, int64_t n, int64_t w, const double *sample_weight,
-                          double *mean_out, void *stream);
+                          int64_t tx, int64_t n, int64_t w, const double *weights,
+                          const double *sample_weight, double *mean_out, void *stream);
 
 /* ------------------------------------------- batched uneven pairs
  * No single reference driver: this is the batch form of DtwMetric._distance

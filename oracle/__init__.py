@@ -344,15 +344,24 @@ def warp_size(x_size, r, y_size=0):
     return max(int(np.floor(max(x_size, y_size) * r)), 1)
 
 
-def mm_dtw_average(X, mean, r=1.0, tol=1e-5, max_epoch=50, sample_weight=None):
+def jeong_weight(n, g):
+    """distance/dtw.py:330-359 (numpy's exp, as the reference)."""
+    return 1.0 / (1.0 + np.exp(-g * (np.arange(n, dtype=float) - n / 2.0)))
+
+
+def mm_dtw_average(X, mean, r=1.0, tol=1e-5, max_epoch=50, sample_weight=None, g=None):
     """_mm_dtw_average, distance/dtw.py:640-666, with dtw_alignment's warp
-    size (dtw.py:246-290) over the cost matrix and backtrack above; the cost
-    is the mean DTW distance (dtw.py:520-525).  Returns (mean, cost)."""
+    size (dtw.py:246-290) over the alignment matrix and backtrack above
+    (wdtw_alignment's weights when g is given); the cost is the mean DTW or
+    WDTW distance (dtw.py:505-525).  Returns (mean, cost)."""
     X = np.asarray(X, dtype=np.float64)
     mean = np.array(mean, dtype=np.float64)
 
     def costfn(m):
-        d = pairwise(m[None, :], X, metric="dtw", r=r)[0]
+        if g is None:
+            d = pairwise(m[None, :], X, metric="dtw", r=r)[0]
+        else:
+            d = pairwise(m[None, :], X, metric="wdtw", r=r, g=g)[0]
         return np.mean(d) if sample_weight is None else np.average(d, weights=sample_weight)
 
     cost = costfn(mean)
@@ -360,8 +369,9 @@ def mm_dtw_average(X, mean, r=1.0, tol=1e-5, max_epoch=50, sample_weight=None):
         z = np.zeros(mean.shape[0])
         V = np.zeros(mean.shape[0])
         w_band = warp_size(mean.shape[0], r, X.shape[1])
+        cw = None if g is None else jeong_weight(max(mean.shape[0], X.shape[1]), g)
         for i in range(X.shape[0]):
-            pm, px = mapping(cost_matrix(mean, X[i], w_band))
+            pm, px = mapping(dtw_alignment(mean, X[i], w_band, cw))
             wgt = 1.0 if sample_weight is None else sample_weight[i]
             for m, x in zip(pm, px):
                 V[m] += wgt

@@ -56,6 +56,11 @@ def test_dtw_average_mm_bitwise(r):
     a = dtw_average(X, r=r, random_state=7, max_epoch=2)
     b, _ = oracle.mm_dtw_average(X, X[np.random.RandomState(7).randint(12)], r=r, max_epoch=2)
     np.testing.assert_array_equal(a, b)
+    # WDTW barycenters: wdtw_alignment paths, mean WDTW cost
+    mean, cost = dtw_average(X, r=r, g=0.1, init=init, return_cost=True, max_epoch=6)
+    om, oc = oracle.mm_dtw_average(X, init, r=r, g=0.1, max_epoch=6)
+    np.testing.assert_array_equal(mean, om)
+    assert cost == oc
     with pytest.raises(NotImplementedError):
         dtw_average(X, method="ssg")
     with pytest.raises(ValueError):
@@ -86,43 +91,47 @@ def test_knn_and_nearest_neighbors_match_the_oracle():
         assert i.shape == (80, 3) and not (i == np.arange(80)[:, None]).any()
 
 
-def _kmeans_restated(x, k, metric, r, random_state, max_iter=300, tol=1e-3):
+def _kmeans_restated(x, k, metric, r, random_state, max_iter=300, tol=1e-3, g=None):
     """_neighbors.py:483-564 over the oracle (one init)."""
     rs = np.random.RandomState(random_state)
+    if g is not None:
+        metric = "wdtw"
     C = x[rs.choice(x.shape[0], size=k, replace=False)].copy()
     cost, reassign = -np.inf, True
     for it in range(max_iter):
-        a, dist = oracle.argmin(x, C, k=1, metric=metric, r=r)
+        a, dist = oracle.argmin(x, C, k=1, metric=metric, r=r, g=g or 0.05)
         a = a.ravel()
-        prev, cost = cost, oracle.paired(x, C[a], metric=metric, r=r).sum() / x.shape[0]
+        prev, cost = cost, oracle.paired(x, C[a], metric=metric, r=r,
+                                         g=g or 0.05).sum() / x.shape[0]
         if math.isclose(cost, prev, rel_tol=tol):
             reassign = False
             break
         for c in range(k):
-            g = x[a == c]
-            if g.shape[0] == 0:
+            grp = x[a == c]
+            if grp.shape[0] == 0:
                 far = dist.min(axis=1).argmax()
                 a[far] = c
                 C[c] = x[far]
-            elif g.shape[0] == 1:
-                C[c] = g
+            elif grp.shape[0] == 1:
+                C[c] = grp
             elif metric == "euclidean":
-                C[c] = g.mean(axis=0)
+                C[c] = grp.mean(axis=0)
             else:
                 rs.randint(np.iinfo(np.int32).max)
-                C[c] = oracle.mm_dtw_average(g, C[c], r=r)[0]
+                C[c] = oracle.mm_dtw_average(grp, C[c], r=r, g=g)[0]
     if reassign:
-        a = oracle.argmin(x, C, k=1, metric=metric, r=r)[0].ravel()
+        a = oracle.argmin(x, C, k=1, metric=metric, r=r, g=g or 0.05)[0].ravel()
     return a, cost, C, it
 
 
-@pytest.mark.parametrize("metric", ["euclidean", "dtw"])
-def test_kmeans_matches_the_restated_loop(metric):
+@pytest.mark.parametrize("metric,g", [("euclidean", None), ("dtw", None), ("dtw", 0.2)])
+def test_kmeans_matches_the_restated_loop(metric, g):
     X = np.concatenate([rw(95, 20, 48), rw(96, 20, 48) + 2.0, rw(97, 20, 48)[:, ::-1] - 1.0])
-    km = KMeans(3, metric=metric, r=0.2, random_state=11, max_iter=10).fit(X)
-    a, cost, C, it = _kmeans_restated(X, 3, metric, 0.2, 11, max_iter=10)
+    km = KMeans(3, metric=metric, r=0.2, g=g, random_state=11, max_iter=10).fit(X)
+    a, cost, C, it = _kmeans_restated(X, 3, metric, 0.2, 11, max_iter=10, g=g)
     np.testing.assert_array_equal(km.labels_, a)
     np.testing.assert_array_equal(km.cluster_centers_, C)
     assert km.inertia_ == cost and km.n_iter_ == it
-    np.testing.assert_array_equal(km.predict(X[:7]),
-                                  oracle.pairwise(X[:7], C, metric=metric, r=0.2).argmin(axis=1))
+    pm = "wdtw" if g is not None else metric
+    np.testing.assert_array_equal(km.predict(X[:7]), oracle.pairwise(
+        X[:7], C, metric=pm, r=0.2, g=g or 0.05).argmin(axis=1))

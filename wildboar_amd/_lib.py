@@ -86,7 +86,7 @@ def load():
             "wbx_dev_eos_lengths": (i32, [i32, p, i64, i64, i64, p, p]),
             "wbx_batch_create": (i32, [m, i64, i64, p, p, p, i32, p]),
             "wbx_dev_dtw_path": (i32, [p, i64, i64, p, i64, i64, i64, i64, p, p, p]),
-            "wbx_dev_dtw_mm_update": (i32, [p, i64, p, i64, i64, i64, i64, p, p, p]),
+            "wbx_dev_dtw_mm_update": (i32, [p, i64, p, i64, i64, i64, i64, p, p, p, p]),
             "wbx_dev_dtw_envelope": (i32, [p, i64, i64, i64, i64, p, p, p]),
             "wbx_dev_lb_keogh": (i32, [p, i64, i64, p, p, i64, i64, p, p]),
             "wbx_dev_lb_kim": (i32, [p, i64, i64, p, i64, i64, i64, p, p]),

@@ -957,8 +957,8 @@ int wbx_dev_dtw_path(const double *X, int64_t x_stride, int64_t tx, const double
 }
 
 int wbx_dev_dtw_mm_update(const double *mean, int64_t tm, const double *X, int64_t x_stride,
-                          int64_t tx, int64_t n, int64_t w, const double *sample_weight,
-                          double *mean_out, void *stream) {
+                          int64_t tx, int64_t n, int64_t w, const double *weights,
+                          const double *sample_weight, double *mean_out, void *stream) {
     if (!mean || !X || !mean_out) return fail(WBX_EINVAL, "NULL argument");
     RC(check_sizes(n, tx));
     RC(check_sizes(1, tm));
@@ -981,7 +981,7 @@ int wbx_dev_dtw_mm_update(const double *mean, int64_t tm, const double *X, int64
     for (int64_t p0 = 0; !rc && p0 < n; p0 += chunk) {
         const int64_t c = n - p0 < chunk ? n - p0 : chunk;
         rc = launch_path_rows(mean, 0, X + p0 * x_stride, x_stride, c, (int)tm, (int)tx, (int)w,
-                              nullptr, (unsigned char *)wsp + o_mv, rows + p0 * tm * 2, st);
+                              weights, (unsigned char *)wsp + o_mv, rows + p0 * tm * 2, st);
     }
     if (!rc) rc = launch_mm_update(rows, X, x_stride, n, (int)tm, sample_weight, mean_out, st);
     int rc2 = end(ctx, st);

@@ -194,8 +194,9 @@ def dtw_average(X, *, r=1.0, g=None, sample_weight=None, init="random", method="
     series on the GPU (float64 paths), sets each mean value to the average of
     the values aligned to it -- in the reference's order of additions -- and
     stops when the mean DTW distance changes by less than ``tol``.  Results
-    equal the reference's for ``g=None``; WDTW alignments (``g``) and the
-    ``"ssg"`` method are not served."""
+    equal the reference's.  With ``g`` the paths are those of
+    ``wdtw_alignment`` and the cost is the mean WDTW distance.  The ``"ssg"``
+    method is not served."""
     import numpy as np
     from sklearn.utils import check_random_state
     from sklearn.utils.validation import _check_sample_weight
@@ -218,7 +219,10 @@ def dtw_average(X, *, r=1.0, g=None, sample_weight=None, init="random", method="
         sample_weight = np.ascontiguousarray(_check_sample_weight(sample_weight, X),
                                              dtype=np.float64)
     if g is not None:
-        raise NotImplementedError("dtw_average with WDTW alignments (g) is not served")
+        import numbers
+
+        if isinstance(g, bool) or not isinstance(g, numbers.Real) or not g > 0:
+            raise ValueError(f"g == {g!r}, must be a float in (0, inf).")
     if method == "ssg":
         raise NotImplementedError("dtw_average(method='ssg') is not served")
     if method != "mm":
@@ -229,11 +233,18 @@ def dtw_average(X, *, r=1.0, g=None, sample_weight=None, init="random", method="
     wd_ = None if sample_weight is None else torch.from_numpy(sample_weight).to(dev)
     lib = _lib.load()
 
+    metric, params = ("dtw", {"r": r}) if g is None else ("wdtw", {"r": r, "g": g})
+
     def costfn(mean):
-        d = wd.pairwise_distance(mean, X, metric="dtw", metric_params={"r": r})
+        d = wd.pairwise_distance(mean, X, metric=metric, metric_params=params)
         return np.mean(d) if sample_weight is None else np.average(d, weights=sample_weight)
 
     w = _warp_size(mean.shape[0], r, y_size=X.shape[1])
+    cell_w = None
+    if g is not None:
+        from .counterfactual import jeong_weight
+
+        cell_w = torch.from_numpy(jeong_weight(max(mean.shape[0], X.shape[1]), g)).to(dev)
     cost = costfn(mean)
     for epoch in range(max_epoch):
         md = torch.from_numpy(mean).to(dev)
@@ -241,7 +252,8 @@ def dtw_average(X, *, r=1.0, g=None, sample_weight=None, init="random", method="
         with torch.cuda.device(dev):
             _lib.check(lib.wbx_dev_dtw_mm_update(
                 md.data_ptr(), md.shape[0], Xd.data_ptr(), Xd.stride(0), Xd.shape[1],
-                Xd.shape[0], int(w), None if wd_ is None else wd_.data_ptr(), out.data_ptr(),
+                Xd.shape[0], int(w), None if cell_w is None else cell_w.data_ptr(),
+                None if wd_ is None else wd_.data_ptr(), out.data_ptr(),
                 torch.cuda.current_stream().cuda_stream))
         mean = out.cpu().numpy()
         prev_cost, cost = cost, costfn(mean)

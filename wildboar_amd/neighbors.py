@@ -209,10 +209,23 @@ class _DtwCluster(_KMeansCluster):
                            random_state=self.random_state.randint(np.iinfo(np.int32).max))
 
 
+class _WDtwCluster(_KMeansCluster):
+    def __init__(self, centroids, r, g, random_state):
+        super().__init__(centroids, "wdtw", {"r": r, "g": g}, random_state)
+        self.r = r
+        self.g = g
+
+    def _update_centroid(self, group, current_center):
+        from .dtw import dtw_average
+
+        return dtw_average(group, r=self.r, g=self.g, init=current_center, method="mm",
+                           random_state=self.random_state.randint(np.iinfo(np.int32).max))
+
+
 class KMeans(ClusterMixin, TransformerMixin, BaseEstimator):
-    """k-means with euclidean or DTW distances (``_neighbors.py:404-612``);
-    DTW centroids are majorize-minimize barycenters.  WDTW (``g``) is not
-    served."""
+    """k-means with euclidean, DTW or (``g`` given) WDTW distances
+    (``_neighbors.py:404-612``); the elastic centroids are majorize-minimize
+    barycenters."""
 
     def __init__(self, n_clusters=8, *, metric="euclidean", r=1.0, g=None, init="random",
                  n_init="auto", max_iter=300, tol=1e-3, verbose=0, random_state=None):
@@ -238,7 +251,6 @@ class KMeans(ClusterMixin, TransformerMixin, BaseEstimator):
             if isinstance(self.g, bool) or not isinstance(self.g, numbers.Real) or not self.g > 0:
                 raise ValueError(f"The 'g' parameter must be None or a float in the range "
                                  f"(0, inf), got {self.g!r}.")
-            raise NotImplementedError("KMeans with WDTW centroids (g) is not served")
         if self.init != "random":
             raise ValueError(f"The 'init' parameter must be 'random', got {self.init!r}.")
         if not (self.n_init == "auto" or (isinstance(self.n_init, numbers.Integral)
@@ -278,8 +290,10 @@ class KMeans(ClusterMixin, TransformerMixin, BaseEstimator):
         centroids = x[random_state.choice(x.shape[0], size=self.n_clusters, replace=False)]
         if self.metric == "euclidean":
             cluster = _EuclideanCluster(centroids, random_state=random_state)
-        else:
+        elif self.g is None:
             cluster = _DtwCluster(centroids, r=self.r, random_state=random_state)
+        else:
+            cluster = _WDtwCluster(centroids, r=self.r, g=self.g, random_state=random_state)
         prev_cost = np.inf
         cost = -np.inf
         reassign = True
@@ -302,7 +316,11 @@ class KMeans(ClusterMixin, TransformerMixin, BaseEstimator):
         check_is_fitted(self, "cluster_centers_")
         x = _validate_x(x, allow_3d=False)
         metric_params = {}
+        metric = self.metric
         if self.metric == "dtw":
             metric_params["r"] = self.r
-        return wd.pairwise_distance(x, self.cluster_centers_, dim="mean", metric=self.metric,
+            if self.g is not None:
+                metric_params["g"] = self.g
+                metric = "wdtw"
+        return wd.pairwise_distance(x, self.cluster_centers_, dim="mean", metric=metric,
                                     metric_params=metric_params)
