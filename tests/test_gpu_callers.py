@@ -135,3 +135,24 @@ def test_kmeans_matches_the_restated_loop(metric, g):
     pm = "wdtw" if g is not None else metric
     np.testing.assert_array_equal(km.predict(X[:7]), oracle.pairwise(
         X[:7], C, metric=pm, r=0.2, g=g or 0.05).argmin(axis=1))
+
+
+@pytest.mark.parametrize("algorithm,init", [("fast", "random"), ("fast", "auto"), ("pam", "auto"),
+                                            ("pam", "random"), ("fast", "min")])
+def test_kmedoids_on_gpu_distances_equals_precomputed_oracle(algorithm, init):
+    from wildboar_amd.neighbors import KMedoids
+
+    X = np.concatenate([rw(98, 15, 40), rw(99, 15, 40) + 3.0, rw(100, 15, 40) - 3.0])
+    D = oracle.pairwise(X, None, metric="dtw", r=0.3)
+    a = KMedoids(3, metric="dtw", metric_params={"r": 0.3}, algorithm=algorithm, init=init,
+                 random_state=5).fit(X)
+    b = KMedoids(3, metric="precomputed", algorithm=algorithm, init=init,
+                 random_state=5).fit(D)
+    np.testing.assert_array_equal(a.labels_, b.labels_)
+    np.testing.assert_array_equal(a.medoid_indices_, b.medoid_indices_)
+    assert a.inertia_ == b.inertia_
+    np.testing.assert_array_equal(a.predict(X[:5]),
+                                  oracle.pairwise(X[:5], X[a.medoid_indices_], metric="dtw",
+                                                  r=0.3).argmin(axis=1))
+    if algorithm == "pam":
+        assert sorted(m // 15 for m in a.medoid_indices_) == [0, 1, 2]

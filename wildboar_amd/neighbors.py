@@ -5,6 +5,7 @@ Mirrors of wildboar's ``distance/_neighbors.py``:
 - ``NearestNeighbors``      (``_neighbors.py:18-169``)
 - ``KNeighborsClassifier``  (``_neighbors.py:172-305``)
 - ``KMeans``                (``_neighbors.py:308-612``)
+- ``KMedoids``              (``_neighbors.py:615-930``, PAM helpers ``_cneighbors.pyx``)
 
 Same parameters, fitted attributes and results; the distances come from
 ``wildboar_amd.distance`` (``argmin_distance`` / ``pairwise_distance`` /
@@ -25,7 +26,7 @@ from sklearn.utils.validation import check_is_fitted
 
 from . import distance as wd
 
-__all__ = ["NearestNeighbors", "KNeighborsClassifier", "KMeans"]
+__all__ = ["NearestNeighbors", "KNeighborsClassifier", "KMeans", "KMedoids"]
 
 
 def _check_int(v, name, low):
@@ -324,3 +325,241 @@ class KMeans(ClusterMixin, TransformerMixin, BaseEstimator):
                 metric = "wdtw"
         return wd.pairwise_distance(x, self.cluster_centers_, dim="mean", metric=metric,
                                     metric_params=metric_params)
+
+
+# ------------------------------------------------------------------ k-medoids
+def _pam_optimal_swap(D, medoid_idxs, not_medoid_idxs, Djs, Ejs, block=128):
+    """The best (medoid, non-medoid) swap of PAM (``_cneighbors.pyx:13-67``):
+    the change in cost of every swap, each summed over the non-medoids in
+    order (a running sum, as the reference's loop), the first smallest change
+    if it is negative, else None.  Candidates are taken ``block`` at a time."""
+    idx_j = not_medoid_idxs
+    k, m = medoid_idxs.size, idx_j.size
+    Dij = D[np.ix_(medoid_idxs, idx_j)]                  # (k, m): D[id_i, id_j]
+    dj, ej = Djs[idx_j], Ejs[idx_j]
+    in_i = Dij == dj                                     # cluster_i_bool
+    ei = Ejs[medoid_idxs][None, :]
+    change = np.empty((m, k))
+    for h0 in range(0, m, block):
+        hs = not_medoid_idxs[h0:h0 + block]
+        Dhj = D[np.ix_(hs, idx_j)]                       # D[id_h, id_j]
+        Djh = D[np.ix_(idx_j, hs)].T                     # D[id_j, id_h]
+        second = Dhj < ej                                # second_best_medoid
+        t_a = Djh - dj
+        t_b = np.broadcast_to(ej - dj, t_a.shape)
+        t_c = np.where(Djh < dj, Djh - dj, 0.0)
+        terms = np.where(in_i[None, :, :], np.where(second[:, None, :], t_a[:, None, :],
+                                                    t_b[:, None, :]), t_c[:, None, :])
+        run = np.cumsum(terms, axis=2)[:, :, -1] if m else np.zeros((hs.size, k))
+        Dhi = D[np.ix_(hs, medoid_idxs)]                 # D[id_h, id_i]
+        Dih = D[np.ix_(medoid_idxs, hs)].T               # D[id_i, id_h]
+        change[h0:h0 + block] = run + np.where(Dhi < ei, Dih, ei)
+    flat = change.ravel()                                # h outer, i inner
+    if flat.size == 0:
+        return None
+    best = int(np.argmin(flat))
+    if flat[best] < 0.0:
+        h, i = divmod(best, k)
+        return medoid_idxs[i], not_medoid_idxs[h], flat[best]
+    return None
+
+
+def _pam_build(D, n_clusters):
+    """PAM's greedy BUILD initialisation (``_cneighbors.pyx:70-100``)."""
+    n = len(D)
+    medoids = np.zeros(n_clusters, dtype=int)
+    medoids[0] = np.argmin(np.sum(D, axis=0))
+    not_medoids = np.delete(np.arange(n, dtype=int), medoids[0])
+    Dj = D[medoids[0]].copy()
+    for c in range(1, n_clusters):
+        gains = np.maximum(0.0, Dj[not_medoids][None, :] - D[np.ix_(not_medoids, not_medoids)])
+        cost = np.cumsum(gains, axis=1)[:, -1]
+        # `>=` keeps the last of equal gains; a gain of 0 beats the start of 0
+        best = int(np.flatnonzero(cost >= max(0.0, cost.max()))[-1])
+        medoids[c] = not_medoids[best]
+        not_medoids = np.delete(not_medoids, best)
+        Dj = np.minimum(Dj, D[:, medoids[c]])
+    return medoids
+
+
+class _KMedoidsCluster:
+    """``_neighbors.py:615-632``."""
+
+    def __init__(self, dist, cluster_idx):
+        self._dist = dist
+        self._cluster_idx = cluster_idx
+        self._prev_cluster_idx = cluster_idx.copy()
+
+    def assign(self):
+        self._prev_cluster_idx[:] = self._cluster_idx
+        self.labels_ = self._dist[self._cluster_idx].argmin(axis=0)
+
+    def cost(self):
+        # flat indices into the distance matrix, as the reference takes them
+        self.cost_ = (np.take(self._dist, self._cluster_idx[self.labels_]).sum()
+                      / self._dist.shape[0])
+
+    def update(self):
+        pass
+
+
+class _FastKMedoidsCluster(_KMedoidsCluster):
+    """``_neighbors.py:635-647``."""
+
+    def update(self):
+        for idx in range(self._cluster_idx.shape[0]):
+            cluster_idx = np.where(self.labels_ == idx)[0]
+            if cluster_idx.shape[0] == 0:
+                continue
+            cost = self._dist[cluster_idx, cluster_idx.reshape(-1, 1)].sum(axis=1)
+            min_idx = cost.argmin()
+            min_cost = cost[min_idx]
+            curr_cost = cost[(cluster_idx == self._cluster_idx[idx]).argmax()]
+            if min_cost < curr_cost:
+                self._cluster_idx[idx] = cluster_idx[min_idx]
+
+
+class _PamKMedoidsCluster(_KMedoidsCluster):
+    """``_neighbors.py:650-674``."""
+
+    def __init__(self, dist, cluster_idx):
+        super().__init__(dist, cluster_idx)
+        self._djs, self._ejs = np.sort(self._dist[self._cluster_idx], axis=0)[[0, 1]]
+
+    def update(self):
+        not_cluster_idx = np.delete(np.arange(self._dist.shape[0]), self._cluster_idx)
+        swap = _pam_optimal_swap(self._dist, self._cluster_idx, not_cluster_idx, self._djs,
+                                 self._ejs)
+        if swap is not None:
+            i, j, _ = swap
+            self._cluster_idx[self._cluster_idx == i] = j
+            self._djs, self._ejs = np.sort(self._dist[self._cluster_idx], axis=0)[[0, 1]]
+
+
+class KMedoids(ClusterMixin, TransformerMixin, BaseEstimator):
+    """k-medoids over the GPU distance matrix (``_neighbors.py:677-930``):
+    ``"fast"`` (alternate) or ``"pam"`` (BUILD + SWAP) updates."""
+
+    def __init__(self, n_clusters=8, metric="euclidean", metric_params=None, init="random",
+                 n_init="auto", algorithm="fast", max_iter=30, tol=1e-4, verbose=0,
+                 n_jobs=None, random_state=None):
+        self.n_clusters = n_clusters
+        self.metric = metric
+        self.metric_params = metric_params
+        self.init = init
+        self.n_init = n_init
+        self.algorithm = algorithm
+        self.max_iter = max_iter
+        self.tol = tol
+        self.verbose = verbose
+        self.n_jobs = n_jobs
+        self.random_state = random_state
+
+    def _validate_params(self):
+        _check_int(self.n_clusters, "n_clusters", 1)
+        _check_metric_name(self.metric, set(wd.METRICS) | {"precomputed"})
+        if self.metric_params is not None and not isinstance(self.metric_params, dict):
+            raise ValueError("The 'metric_params' parameter must be None or a dict.")
+        if self.init not in ("random", "auto", "min"):
+            raise ValueError(f"The 'init' parameter must be a str among "
+                             f"{{'auto', 'min', 'random'}}, got {self.init!r}.")
+        if not (self.n_init == "auto" or (isinstance(self.n_init, numbers.Integral)
+                                          and not isinstance(self.n_init, bool)
+                                          and self.n_init >= 1)):
+            raise ValueError(f"The 'n_init' parameter must be 'auto' or an int >= 1, "
+                             f"got {self.n_init!r}.")
+        if self.algorithm not in ("fast", "pam"):
+            raise ValueError(f"The 'algorithm' parameter must be a str among "
+                             f"{{'fast', 'pam'}}, got {self.algorithm!r}.")
+        _check_int(self.max_iter, "max_iter", 1)
+        if isinstance(self.tol, bool) or not isinstance(self.tol, numbers.Real) or self.tol < 0:
+            raise ValueError(f"The 'tol' parameter must be a float >= 0, got {self.tol!r}.")
+
+    def fit(self, x, y=None):
+        import warnings
+
+        from sklearn.exceptions import ConvergenceWarning
+
+        x = (np.asarray(x, dtype=float) if self.metric == "precomputed"
+             else _validate_x(x, allow_3d=True))
+        self._validate_params()
+        if self.n_init == "auto":
+            n_init = 1 if self.init == "auto" else 10
+        else:
+            n_init = self.n_init
+        if n_init > 1 and self.init != "random":
+            n_init = 1
+        max_iter = self.max_iter
+        if self.algorithm == "pam" and self.n_clusters == 1 and self.max_iter != 0:
+            warnings.warn("n_clusters must be larger than 1 if max_iter larger than 0")
+            max_iter = 0
+        random_state = check_random_state(self.random_state)
+        if self.metric == "precomputed":
+            dist = x
+        else:
+            dist = wd.pairwise_distance(x, dim="mean", metric=self.metric,
+                                        metric_params=self.metric_params)
+        best_iter, best_cost, best_clusterer, best_reassign = 0, np.inf, None, None
+        clusterer = None
+        for i in range(n_init):
+            if self.verbose:
+                print(f"Running initialization {i}/{n_init}")
+            it, clusterer, reassign = self._fit_one_init(dist, random_state=random_state,
+                                                         max_iter=max_iter,
+                                                         warn=warnings.warn,
+                                                         warning=ConvergenceWarning)
+            if clusterer.cost_ < best_cost:
+                best_cost = clusterer.cost_
+                best_clusterer = clusterer
+                best_iter = it
+                best_reassign = reassign
+        if best_reassign:
+            clusterer.assign()  # the reference reassigns the last run's clusterer
+        self.cluster_centers_ = (None if self.metric == "precomputed"
+                                 else x[best_clusterer._cluster_idx])
+        self.inertia_ = best_clusterer.cost_
+        self.medoid_indices_ = best_clusterer._cluster_idx
+        self.n_iter_ = best_iter
+        self.labels_ = best_clusterer.labels_
+        return self
+
+    def _fit_one_init(self, dist, *, max_iter, random_state, warn, warning):
+        centers = self._init_centers(dist, self.n_clusters, random_state=random_state)
+        cls = _PamKMedoidsCluster if self.algorithm == "pam" else _FastKMedoidsCluster
+        clusterer = cls(dist, centers)
+        clusterer.assign()
+        reassign = False
+        prev_cost = np.inf
+        it = 0
+        for it in range(max_iter):
+            clusterer.update()
+            clusterer.cost()
+            if self.verbose:
+                print(f"Iteration {it}/{self.max_iter}: current_cost={clusterer.cost_}")
+            if math.isclose(clusterer.cost_, prev_cost, rel_tol=self.tol):
+                reassign = True
+                break
+            prev_cost = clusterer.cost_
+            clusterer.assign()
+        if it + 1 == self.max_iter:
+            warn("Maximum number of iterations reached before convergence. Consider "
+                 "increasing max_iter to improve the fit", warning)
+        return it, clusterer, reassign
+
+    def _init_centers(self, dist, n_clusters, random_state):
+        if self.init == "random":
+            return random_state.choice(dist.shape[0], n_clusters, replace=False)
+        if self.init == "min" or (self.algorithm == "fast" and self.init == "auto"):
+            return np.argpartition(dist.sum(axis=1), n_clusters)[:n_clusters]
+        return _pam_build(dist, n_clusters)
+
+    def predict(self, x):
+        return self.transform(x).argmin(axis=1)
+
+    def transform(self, x):
+        check_is_fitted(self, "medoid_indices_")
+        if self.metric == "precomputed":
+            return np.asarray(x, dtype=float)[:, self.medoid_indices_]
+        x = _validate_x(x, allow_3d=True)
+        return wd.pairwise_distance(x, self.cluster_centers_, dim="mean", metric=self.metric,
+                                    metric_params=self.metric_params)
