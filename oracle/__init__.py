@@ -254,9 +254,10 @@ def pack_shapelets(shapelets):
     return np.ascontiguousarray(packed), offs, lens
 
 
-def subsequence(shapelets, X, metric="dtw", r=1.0, g=0.05, dim=0, n_threads=None):
+def subsequence(shapelets, X, metric="dtw", r=1.0, g=0.05, dim=0, n_threads=None, p=1.0):
     """(dist, idx), each (n_samples, n_shapelets)."""
     n_threads = n_threads or default_threads()
+    g = _gp(metric, g, p)
     X = _ts(X)
     S, offs, lens = pack_shapelets(shapelets)
     n = X.shape[0]

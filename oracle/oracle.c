@@ -249,6 +249,43 @@ double orc_euclidean_distance(const double *S, int64_t s_length,
     return sqrt(min_dist);
 }
 
+/* ddtw_subsequence_distance, distance/_elastic.pyx:780-818: S is already the
+ * pattern's average slope (s_length - 2 values); each window's slope goes
+ * into T_buffer.  0 below length 3. */
+double orc_ddtw_subsequence_distance(const double *S, int64_t s_length, const double *T,
+                                     int64_t t_length, int64_t r, double *cost,
+                                     double *cost_prev, const double *weight_vector,
+                                     double *T_buffer, int64_t *index) {
+    if (s_length < 3) return 0;
+    double dist, min_dist = INFINITY;
+    for (int64_t i = 0; i < t_length - s_length + 1; i++) {
+        orc_average_slope(T + i, s_length, T_buffer);
+        dist = orc_dtw_distance(S, s_length - 2, T_buffer, s_length - 2, r, cost, cost_prev,
+                                weight_vector, min_dist);
+        if (dist < min_dist) {
+            if (index != NULL) *index = i;
+            min_dist = dist;
+        }
+    }
+    return sqrt(min_dist);
+}
+
+/* adtw_subsequence_distance, distance/_elastic.pyx:701-734. */
+double orc_adtw_subsequence_distance(const double *S, int64_t s_length, const double *T,
+                                     int64_t t_length, int64_t r, double *cost,
+                                     double *cost_prev, double penalty, int64_t *index) {
+    double dist, min_dist = INFINITY;
+    for (int64_t i = 0; i < t_length - s_length + 1; i++) {
+        dist = orc_adtw_distance(S, s_length, T + i, s_length, r, cost, cost_prev, penalty,
+                                 min_dist);
+        if (dist < min_dist) {
+            if (index != NULL) *index = i;
+            min_dist = dist;
+        }
+    }
+    return sqrt(min_dist);
+}
+
 /* dtw_subsequence_distance, distance/_elastic.pyx:622-655. */
 double orc_dtw_subsequence_distance(const double *S, int64_t s_length,
                                     const double *T, int64_t t_length,
@@ -473,7 +510,19 @@ static void *run_job(void *arg) {
             m.weights = (double *)malloc(sizeof(double) * (size_t)jb->tx);
             if (m.weights) orc_wdtw_weights(jb->tx, jb->g, m.weights);
         }
-        if (!m.cost || !m.cost_prev || (jb->kind == ORC_WDTW && !m.weights)) {
+        /* WeightedDerivativeDtwSubsequenceMetric.reset (_elastic.pyx:2530-2540):
+         * weights for n_timestep = X.shape[2] - 2 */
+        if (jb->kind == ORC_WDDTW && jb->tx >= 3) {
+            m.weights = (double *)malloc(sizeof(double) * (size_t)jb->tx);
+            if (m.weights) orc_wdtw_weights(jb->tx - 2, jb->g, m.weights);
+        }
+        if (ORC_DERIVATIVE(jb->kind)) {
+            m.d_x = (double *)malloc(sizeof(double) * (size_t)(jb->tx + 1));
+            m.d_y = (double *)malloc(sizeof(double) * (size_t)(jb->tx + 1));
+        }
+        if (!m.cost || !m.cost_prev || ((jb->kind == ORC_WDTW || (jb->kind == ORC_WDDTW &&
+                                                                   jb->tx >= 3)) && !m.weights) ||
+            (ORC_DERIVATIVE(jb->kind) && (!m.d_x || !m.d_y))) {
             jb->status = -1;
             metric_free(&m);
             return NULL;
@@ -487,6 +536,19 @@ static void *run_job(void *arg) {
                 if (jb->kind == ORC_EUCLIDEAN)
                     /* EuclideanSubsequenceMetric._distance, _metric.pyx:30-41 */
                     d = orc_euclidean_distance(s, sl, x, jb->tx, INFINITY, &min_index);
+                else if (ORC_DERIVATIVE(jb->kind)) {
+                    /* DerivativeDtwSubsequenceMetric._distance, _elastic.pyx:2470-2493;
+                     * the reference leaves the index unset below length 3: 0 here */
+                    min_index = 0;
+                    if (sl >= 3) orc_average_slope(s, sl, m.d_x);
+                    d = orc_ddtw_subsequence_distance(m.d_x, sl, x, jb->tx,
+                                                      orc_compute_r(sl, jb->r), m.cost,
+                                                      m.cost_prev, m.weights, m.d_y,
+                                                      &min_index);
+                } else if (jb->kind == ORC_ADTW)
+                    /* AmercingDtwSubsequenceMetric._distance, _elastic.pyx:2303-2322 */
+                    d = orc_adtw_subsequence_distance(s, sl, x, jb->tx, orc_compute_r(sl, jb->r),
+                                                      m.cost, m.cost_prev, jb->g, &min_index);
                 else
                     /* DtwSubsequenceMetric._distance, _elastic.pyx:2242-2263 */
                     d = orc_dtw_subsequence_distance(s, sl, x, jb->tx,

@@ -98,7 +98,7 @@ def test_derivative_short_series_and_validation():
     assert (idx == 0).all() and (dist == 0).all()
     X = rw(90, 4, 30)
     with pytest.raises(NotImplementedError):
-        wd.pairwise_subsequence_distance([X[0, :5]], X, metric="adtw")
+        wd.argmin_subsequence_distance([X[0, :5]] * 4, X, metric="adtw")
     with pytest.raises(ValueError):
         wd.pairwise_distance(X, metric="adtw", metric_params={"p": float("-inf")})
     with pytest.raises(TypeError):
@@ -106,3 +106,22 @@ def test_derivative_short_series_and_validation():
     # WeightedDerivativeDtwMetric takes any float g
     got = wd.pairwise_distance(X, X[:2], metric="wddtw", metric_params={"g": -0.2})
     np.testing.assert_array_equal(got, oracle.pairwise(X, X[:2], metric="wddtw", g=-0.2))
+
+
+@pytest.mark.parametrize("metric,params", [("ddtw", {"r": 0.1}), ("ddtw", {"r": 1.0}),
+                                           ("wddtw", {"r": 0.2, "g": 0.1}),
+                                           ("adtw", {"r": 0.1, "p": 0.5}),
+                                           ("adtw", {"r": 1.0, "p": 2.0})])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_subsequence_derivative_and_amercing_bitwise(metric, params, precision):
+    X = rw(84, 9, 120)
+    rng = np.random.default_rng(85)
+    pats = [X[2, 10:40].copy(), rng.standard_normal(7), X[5, 0:2].copy(), X[0, 3:120].copy(),
+            X[8, 50:53].copy(), rng.standard_normal(64)]
+    d, i = wd.pairwise_subsequence_distance(pats, X, metric=metric, metric_params=params,
+                                            return_index=True, precision=precision)
+    od, oi = oracle.subsequence(pats, X, metric=metric, **okw(params))
+    np.testing.assert_array_equal(d, od)
+    np.testing.assert_array_equal(i, oi)
+    with pytest.raises(NotImplementedError):
+        wd.subsequence_match(pats[0], X, metric=metric, metric_params=params)

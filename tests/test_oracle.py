@@ -554,3 +554,33 @@ def test_alignment_matrix():
                         c = m + c
                     B[i, j] = c
             np.testing.assert_array_equal(A, B)
+
+
+def test_derivative_and_amercing_subsequence():
+    rng = np.random.default_rng(37)
+    X = rw(rng, 4, 50)
+    pats = [X[1, 7:19].copy(), rng.standard_normal(6), X[2, 0:2].copy(), X[3, 10:50].copy()]
+    dd, di = oracle.subsequence(pats, X, metric="ddtw", r=0.25)
+    for i in range(4):
+        for j, s in enumerate(pats):
+            L = len(s)
+            if L < 3:
+                assert dd[i, j] == 0.0
+                continue
+            # DTW of the slopes, band of the pattern's own length (the window's
+            # slope is the series' slope over the same span)
+            band = oracle.compute_r(L, 0.25)
+            xs = oracle.average_slope(X[i])
+            ss = oracle.average_slope(s)
+            costs = [oracle.dtw_distance(ss, xs[w:w + L - 2], band) for w in range(50 - L + 1)]
+            assert dd[i, j] == math.sqrt(min(costs)) and di[i, j] == int(np.argmin(costs))
+    assert dd[1, 0] == 0.0 and di[1, 0] == 7
+    # adtw: penalty 0 is dtw, penalty inf at equal lengths the euclidean scan
+    a0 = oracle.subsequence(pats, X, metric="adtw", r=0.3, p=0.0)
+    dt = oracle.subsequence(pats, X, metric="dtw", r=0.3)
+    np.testing.assert_array_equal(a0[0], dt[0])
+    np.testing.assert_array_equal(a0[1], dt[1])
+    ai = oracle.subsequence(pats, X, metric="adtw", r=0.3, p=math.inf)
+    eu = oracle.subsequence(pats, X, metric="euclidean")
+    np.testing.assert_array_equal(ai[0], eu[0])
+    np.testing.assert_array_equal(ai[1], eu[1])

@@ -65,11 +65,12 @@ int check_metric(const wbx_metric *m) {
     return WBX_OK;
 }
 
-// Subsequence scans and profiles serve euclidean, dtw and wdtw.
-int check_subsequence_metric(const wbx_metric *m) {
+// Subsequence profiles serve euclidean, dtw and wdtw; the subsequence scan
+// also ddtw, wddtw and adtw.
+int check_subsequence_metric(const wbx_metric *m, bool scan = false) {
     int rc = check_metric(m);
     if (rc) return rc;
-    if (m->kind > WBX_WDTW)
+    if (m->kind > WBX_WDTW && !scan)
         return fail(WBX_EUNSUPPORTED, "metric kind %d is not served for subsequences", m->kind);
     return WBX_OK;
 }
@@ -549,6 +550,83 @@ int subseq_fast_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void 
     return WBX_OK;
 }
 
+// ddtw / wddtw / adtw subsequence scans, float64.  DerivativeDtwSubsequenceMetric
+// (distance/_elastic.pyx:2443-2540, ddtw_subsequence_distance :780-818):
+// DTW / WDTW of the pattern's average slope against the slopes of each window
+// -- a slice of the series' slopes -- with the band of the pattern's own
+// length, 0 below length 3; WDDTW's weights are sized by the series' slope
+// length (:2530-2540).  AmercingDtwSubsequenceMetric (:2293-2350,
+// adtw_subsequence_distance :701-734).
+int subseq_other_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
+                      const int64_t *s_off, const int64_t *s_len, int64_t n_s,
+                      const std::vector<int64_t> &hl, const void *X, int64_t n, int64_t xstr,
+                      int64_t t, int64_t total, int64_t max_len, double *dist, int64_t *idx,
+                      hipStream_t st) {
+    const bool deriv = m->kind != WBX_ADTW;
+    std::vector<int64_t> nl((size_t)n_s), no((size_t)n_s);
+    int64_t ntot = 0, nmin = t, nmax = 0;
+    for (int64_t j = 0; j < n_s; ++j) {
+        nl[j] = deriv ? (hl[j] >= 3 ? hl[j] - 2 : 0) : hl[j];
+        no[j] = ntot;
+        ntot += nl[j];
+        if (nl[j] > 0) {
+            nmin = nl[j] < nmin ? nl[j] : nmin;
+            nmax = nl[j] > nmax ? nl[j] : nmax;
+        }
+    }
+    const int64_t tt = deriv ? t - 2 : t;
+    Carve cv;
+    const size_t o_x = cv.take(stage_bytes(dtype, true, n, t));
+    const size_t o_s = cv.take(stage_bytes(dtype, true, 1, total));
+    const size_t o_xs = deriv ? cv.take((size_t)n * (size_t)(tt > 0 ? tt : 0) * 8) : 0;
+    const size_t o_ss = deriv ? cv.take((size_t)ntot * 8) : 0;
+    const size_t o_meta = cv.take((size_t)n_s * 16);
+    void *wsp = nullptr;
+    RC(ws_reserve(ctx, cv.total, &wsp));
+    char *ws = (char *)wsp;
+    Staged<double> x64, s64;
+    RC(stage<double>(dtype, X, n, t, xstr, ws, o_x, &x64, st));
+    RC(stage<double>(dtype, S, 1, total, total, ws, o_s, &s64, st));
+    int64_t *d_len = (int64_t *)(ws + o_meta), *d_off = d_len + n_s;
+    WBX_HIP_CHECK(hipMemcpyAsync(d_len, nl.data(), sizeof(int64_t) * n_s, hipMemcpyHostToDevice, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(d_off, no.data(), sizeof(int64_t) * n_s, hipMemcpyHostToDevice, st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));  // the host vectors end with this call
+    RC(launch_subseq_short(d_len, n_s, n, dist, idx, st));
+    if (nmax == 0) return WBX_OK;  // every pattern shorter than 3
+    const double *xp = x64.ptr, *sp = s64.ptr;
+    int64_t xs_stride = x64.stride;
+    const int64_t *soff = s_off;
+    if (deriv) {
+        double *xsl = (double *)(ws + o_xs), *ssl = (double *)(ws + o_ss);
+        RC(launch_slope_rows<double>(x64.ptr, n, t, x64.stride, xsl, st));
+        RC(launch_slope_packed(s64.ptr, s_off, s_len, n_s, d_off, max_len, ssl, st));
+        xp = xsl;
+        sp = ssl;
+        xs_stride = tt;
+        soff = d_off;
+    }
+    const double *w64 = nullptr;
+    const float *w32 = nullptr;
+    if (m->kind == WBX_WDDTW) RC(cached_weights(ctx, tt, m->g, &w64, &w32));
+    const int kind = m->kind == WBX_DDTW ? WBX_DTW : (m->kind == WBX_WDDTW ? WBX_WDTW : WBX_ADTW);
+    const double *pen = m->kind == WBX_ADTW ? &m->p : nullptr;
+    const int band_add = deriv ? 2 : 0;
+    Shapelets sh{sp, soff, d_len, n_s};
+    const Series<double> xs = series<double>(xp, xs_stride, tt);
+    ProfScope pf(st, KERN_SUBSEQ);
+    int lane_cap = 0;
+    if (g_policy.load() == 0 && lane64_ok(ctx, w64)) {
+        RC(launch_subseq_lane64(m->r, sh, (int)nmin, (int)nmax, xs, n, w64, dist, idx, st, band_add,
+                                pen));
+        lane_cap = lane64_cap(w64 != nullptr);
+        if (lane64_class(2 * band_w((int)nmax + band_add, (int)nmax + band_add, m->r) - 1,
+                         w64 != nullptr))
+            return WBX_OK;
+    }
+    return launch_subseq_exact<double>(kind, m->r, sh, (int)nmax, xs, n, w64, dist, idx, st,
+                                       lane_cap, band_add, pen);
+}
+
 int subseq_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
                 const int64_t *s_off, const int64_t *s_len, int64_t n_s, const void *X,
                 int64_t n, int64_t xstr, int64_t t, double *dist, int64_t *idx, hipStream_t st) {
@@ -570,6 +648,9 @@ int subseq_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
     int64_t s_stride = total;
     RC(acc64_prepare(ctx, &mm, dtype, {{&S, 1, total, &s_stride}, {&X, n, t, &xstr}}, st));
     m = &mm;
+    if (m->kind > WBX_WDTW)
+        return subseq_other_impl(ctx, m, dtype, S, s_off, s_len, n_s, hl, X, n, xstr, t, total,
+                                 max_len, dist, idx, st);
     const double *w64 = nullptr;
     const float *w32 = nullptr;
     if (m->kind == WBX_WDTW) RC(cached_weights(ctx, t, m->g, &w64, &w32));
@@ -905,7 +986,7 @@ int wbx_dev_subsequence_pairwise(const wbx_metric *metric, int dtype, const void
                                  const int64_t *s_off, const int64_t *s_len, int64_t n_s,
                                  const void *X, int64_t n, int64_t x_stride, int64_t t,
                                  double *dist, int64_t *idx, void *stream) {
-    RC(check_subsequence_metric(metric));
+    RC(check_subsequence_metric(metric, true));
     RC(check_sizes(n, t));
     if (n_s < 1) return fail(WBX_EINVAL, "Subsequence y cannot be empty.");
     DeviceCtx *ctx = ctx_current();
@@ -1238,7 +1319,7 @@ int wbx_argmin_subsequence(const wbx_metric *metric, const double *S, const int6
 int wbx_subsequence_pairwise(const wbx_metric *metric, const double *S, const int64_t *s_off,
                              const int64_t *s_len, int64_t n_s, const double *X, int64_t n,
                              int64_t dims, int64_t t, int64_t dim, double *dist, int64_t *idx) {
-    RC(check_subsequence_metric(metric));
+    RC(check_subsequence_metric(metric, true));
     RC(check_sizes(n, t));
     if (n_s < 1) return fail(WBX_EINVAL, "Subsequence y cannot be empty.");
     if (dims < 1 || dim < 0 || dim >= dims)

@@ -176,16 +176,22 @@ __global__ __launch_bounds__(64) void k_profile_lane64(Shapelets sh, int paired,
     if (s < nwp) prof[q * nwp + s] = s < nwin ? c : WBX_INF_D;
 }
 
-template <int BP, int U, bool WT>
+// band_add: the band comes from pattern length L + band_add (2 for the
+// derivative metrics, whose patterns are average slopes of length L + 2 series,
+// distance/_elastic.pyx:2490-2493).  AD: the amercing recursion with penalty
+// pen (adtw_subsequence_distance, :701-734).
+template <int BP, int U, bool WT, bool AD>
 __global__ __launch_bounds__(64) void k_subseq_lane64(Shapelets sh, Series<double> xs, int64_t n,
                                                       double r, const double *__restrict__ weights,
+                                                      int band_add, double pen,
                                                       double *__restrict__ dist,
                                                       int64_t *__restrict__ idx) {
     const int64_t q = (int64_t)blockIdx.x + (int64_t)blockIdx.y * gridDim.x;
     if (q >= n * sh.n) return;
     const int64_t i = q / sh.n, j = q - i * sh.n;
     const int L = (int)sh.len[j];
-    const int B = pattern_B(L, r);
+    if (L < 1) return;  // handled by the caller
+    const int B = pattern_B(L + band_add, r);
     if ((B + 7) / 8 * 8 != BP) return;
     const double *__restrict__ S = (const double *)sh.S + sh.off[j];
     const int t = xs.len;
@@ -200,7 +206,7 @@ __global__ __launch_bounds__(64) void k_subseq_lane64(Shapelets sh, Series<doubl
     for (int b0 = 0; b0 < nwin; b0 += 64) {
         const int s = b0 + threadIdx.x;
         const int sc = s < nwin ? s : nwin - 1;
-        const double c = lane64_cost<BP, U, WT>(S, L, x + sc, L, off, B, off, weights, t);
+        const double c = lane64_cost<BP, U, WT, AD>(S, L, x + sc, L, off, B, off, weights, t, pen);
         if (s < nwin && c < best) {
             best = c;
             best_s = s;
@@ -308,9 +314,10 @@ int launch_profile_lane64(double r, Shapelets sh, int paired, int min_len, int m
 
 int launch_subseq_lane64(double r, Shapelets sh, int min_len, int max_len, Series<double> xs,
                          int64_t n, const double *weights, double *dist, int64_t *idx,
-                         hipStream_t st) {
+                         hipStream_t st, int band_add, const double *penalty) {
     int lo, hi;
-    class_range(min_len, max_len, r, weights != nullptr, &lo, &hi);
+    class_range(min_len + band_add, max_len + band_add, r, weights != nullptr, &lo, &hi);
+    const double pen = penalty ? *penalty : 0.0;
     const int64_t nq = n * sh.n;
     if (!lo || nq <= 0) return WBX_OK;
     const dim3 grid((unsigned)(nq < 65536 ? nq : 65536), (unsigned)((nq + 65535) / 65536));
@@ -318,11 +325,14 @@ int launch_subseq_lane64(double r, Shapelets sh, int min_len, int max_len, Serie
 #define WBX_L64_SUB(C)                                                                          \
     case C:                                                                                     \
         if (weights)                                                                            \
-            hipLaunchKernelGGL((k_subseq_lane64<C, 4, true>), grid, dim3(64), 0, st, sh, xs, n, \
-                               r, weights, dist, idx);                                          \
+            hipLaunchKernelGGL((k_subseq_lane64<C, 4, true, false>), grid, dim3(64), 0, st, sh, \
+                               xs, n, r, weights, band_add, 0.0, dist, idx);                    \
+        else if (penalty)                                                                       \
+            hipLaunchKernelGGL((k_subseq_lane64<C, 4, false, true>), grid, dim3(64), 0, st, sh, \
+                               xs, n, r, weights, band_add, pen, dist, idx);                    \
         else                                                                                    \
-            hipLaunchKernelGGL((k_subseq_lane64<C, 4, false>), grid, dim3(64), 0, st, sh, xs,   \
-                               n, r, weights, dist, idx);                                       \
+            hipLaunchKernelGGL((k_subseq_lane64<C, 4, false, false>), grid, dim3(64), 0, st,    \
+                               sh, xs, n, r, weights, band_add, 0.0, dist, idx);                \
         break;
         switch (BP) {
             WBX_LANE64_CLASSES(WBX_L64_SUB)

@@ -20,11 +20,12 @@
 
 namespace wbx {
 
-template <typename T>
+// band_add / AD: as k_subseq_lane64 (derivative patterns, amercing).
+template <typename T, bool AD>
 __global__ __launch_bounds__(64) void k_subseq_exact_dtw(Shapelets sh, Series<T> xs, int64_t n,
                                                          double r, const T *__restrict__ weights,
-                                                         int cap, int lane_cap,
-                                                         double *__restrict__ dist,
+                                                         int cap, int lane_cap, int band_add,
+                                                         T pen, double *__restrict__ dist,
                                                          int64_t *__restrict__ idx) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *buf = reinterpret_cast<T *>(smem_raw);
@@ -35,12 +36,14 @@ __global__ __launch_bounds__(64) void k_subseq_exact_dtw(Shapelets sh, Series<T>
     const int L = (int)sh.len[j];
     const int t = xs.len;
     const T *x = xs.row(i);
-    const int w = band_w(L, L, r);
+    if (L < 1) return;  // handled by the caller
+    const int w = band_w(L + band_add, L + band_add, r);
     if ((2 * w - 1 + 7) / 8 * 8 <= lane_cap) return;  // a lane64 launch takes this task
     T best = (T)WBX_INF_D;
     int64_t best_i = 0;
     for (int s = 0; s <= t - L; ++s) {
-        const T c = block_dtw<T>(S, L, x + s, L, w, weights, buf, buf + cap, buf + 2 * cap);
+        const T c = block_dtw<T, AD>(S, L, x + s, L, w, weights, buf, buf + cap, buf + 2 * cap,
+                                     pen);
         if (c < best) {
             best = c;
             best_i = s;
@@ -82,10 +85,36 @@ __global__ __launch_bounds__(256) void k_subseq_exact_euclid(Shapelets sh, Serie
     idx[q] = best_i;
 }
 
+// dist = 0, idx = 0 for the patterns of length < 1 (derivative patterns of
+// series shorter than 3: DerivativeDtwSubsequenceMetric returns 0 there,
+// distance/_elastic.pyx:792-793, and leaves the index unset).
+__global__ __launch_bounds__(256) void k_subseq_short(const int64_t *__restrict__ len,
+                                                      int64_t n_s, int64_t n,
+                                                      double *__restrict__ dist,
+                                                      int64_t *__restrict__ idx) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= n * n_s) return;
+    if (len[q % n_s] < 1) {
+        dist[q] = 0.0;
+        idx[q] = 0;
+    }
+}
+
+int launch_subseq_short(const int64_t *len, int64_t n_s, int64_t n, double *dist, int64_t *idx,
+                        hipStream_t st) {
+    const int64_t total = n * n_s;
+    if (total <= 0) return WBX_OK;
+    if ((total + 255) / 256 > 0x7fffffff) return fail(WBX_EUNSUPPORTED, "too many tasks");
+    hipLaunchKernelGGL(k_subseq_short, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                       len, n_s, n, dist, idx);
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
+
 template <typename T>
 int launch_subseq_exact(int kind, double r, Shapelets sh, int max_len, Series<T> xs, int64_t n,
                         const T *weights, double *dist, int64_t *idx, hipStream_t st,
-                        int lane_cap) {
+                        int lane_cap, int band_add, const double *penalty) {
     const int64_t nq = n * sh.n;
     if (nq <= 0) return WBX_OK;
     if (kind == WBX_EUCLIDEAN) {
@@ -96,21 +125,30 @@ int launch_subseq_exact(int kind, double r, Shapelets sh, int max_len, Series<T>
         if (lds > 64 * 1024) {
             if (lds > 160 * 1024)
                 return fail(WBX_EUNSUPPORTED, "shapelet of length %d exceeds LDS", max_len);
-            WBX_HIP_CHECK(hipFuncSetAttribute((const void *)k_subseq_exact_dtw<T>,
+            WBX_HIP_CHECK(hipFuncSetAttribute((const void *)k_subseq_exact_dtw<T, false>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)lds));
+            WBX_HIP_CHECK(hipFuncSetAttribute((const void *)k_subseq_exact_dtw<T, true>,
                                               hipFuncAttributeMaxDynamicSharedMemorySize,
                                               (int)lds));
         }
         dim3 grid((unsigned)(nq < 65536 ? nq : 65536), (unsigned)((nq + 65535) / 65536));
-        hipLaunchKernelGGL(k_subseq_exact_dtw<T>, grid, dim3(64), lds, st, sh, xs, n, r, weights,
-                           max_len, lane_cap, dist, idx);
+        if (penalty)
+            hipLaunchKernelGGL((k_subseq_exact_dtw<T, true>), grid, dim3(64), lds, st, sh, xs, n,
+                               r, weights, max_len, lane_cap, band_add, (T)*penalty, dist, idx);
+        else
+            hipLaunchKernelGGL((k_subseq_exact_dtw<T, false>), grid, dim3(64), lds, st, sh, xs, n,
+                               r, weights, max_len, lane_cap, band_add, (T)0, dist, idx);
     }
     WBX_HIP_CHECK(hipGetLastError());
     return WBX_OK;
 }
 
 template int launch_subseq_exact<double>(int, double, Shapelets, int, Series<double>, int64_t,
-                                         const double *, double *, int64_t *, hipStream_t, int);
+                                         const double *, double *, int64_t *, hipStream_t, int,
+                                         int, const double *);
 template int launch_subseq_exact<float>(int, double, Shapelets, int, Series<float>, int64_t,
-                                        const float *, double *, int64_t *, hipStream_t, int);
+                                        const float *, double *, int64_t *, hipStream_t, int, int,
+                                        const double *);
 
 }  // namespace wbx

@@ -94,6 +94,30 @@ template int launch_slope_rows<double>(const double *, int64_t, int64_t, int64_t
 template int launch_slope_rows<float>(const float *, int64_t, int64_t, int64_t, double *,
                                       hipStream_t);
 
+__global__ __launch_bounds__(256) void k_slope_packed(const double *__restrict__ S,
+                                                      const int64_t *__restrict__ off,
+                                                      const int64_t *__restrict__ len, int64_t n_s,
+                                                      const int64_t *__restrict__ out_off,
+                                                      int64_t max_len, double *__restrict__ out) {
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= n_s * max_len) return;
+    const int64_t j = g / max_len, c = g - j * max_len;
+    if (c >= len[j] - 2) return;
+    const double *q = S + off[j] + c;
+    out[out_off[j] + c] = ((q[1] - q[0]) + ((q[2] - q[0]) / 2)) / 2;
+}
+
+int launch_slope_packed(const double *S, const int64_t *off, const int64_t *len, int64_t n_s,
+                        const int64_t *out_off, int64_t max_len, double *out, hipStream_t st) {
+    const int64_t total = n_s * max_len;
+    if (total <= 0) return WBX_OK;
+    if ((total + 255) / 256 > 0x7fffffff) return fail(WBX_EUNSUPPORTED, "too many patterns");
+    hipLaunchKernelGGL(k_slope_packed, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, S,
+                       off, len, n_s, out_off, max_len, out);
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
+
 __global__ __launch_bounds__(256) void k_fill_f64(double *__restrict__ dst, int64_t n, double v) {
     for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n;
          q += (int64_t)gridDim.x * 256)

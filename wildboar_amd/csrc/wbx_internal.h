@@ -229,10 +229,13 @@ struct Shapelets {
 };
 // Tasks whose band class is at most lane_cap (0: none) are left to the
 // lane64 kernels.
+// band_add: the band of a pattern of length L is that of L + band_add
+// (derivative metrics); penalty != NULL: ADTW.  Patterns of length < 1 are
+// left to the caller.
 template <typename T>
 int launch_subseq_exact(int kind, double r, Shapelets sh, int max_len, Series<T> xs, int64_t n,
                         const T *weights, double *dist, int64_t *idx, hipStream_t st,
-                        int lane_cap = 0);
+                        int lane_cap = 0, int band_add = 0, const double *penalty = nullptr);
 
 // --------------------------------------------------------- warping paths
 // dtw_mapping's path of each pair (X row p, Y row p), band width w, as
@@ -268,7 +271,13 @@ int launch_profile_lane64(double r, Shapelets sh, int paired, int min_len, int m
                           double *prof, hipStream_t st);
 int launch_subseq_lane64(double r, Shapelets sh, int min_len, int max_len, Series<double> xs,
                          int64_t n, const double *weights, double *dist, int64_t *idx,
-                         hipStream_t st);
+                         hipStream_t st, int band_add = 0, const double *penalty = nullptr);
+int launch_subseq_short(const int64_t *len, int64_t n_s, int64_t n, double *dist, int64_t *idx,
+                        hipStream_t st);
+// Average slopes of packed patterns: pattern j (offset off[j], length len[j])
+// becomes len[j] - 2 values at out_off[j] (nothing for len[j] < 3).
+int launch_slope_packed(const double *S, const int64_t *off, const int64_t *len, int64_t n_s,
+                        const int64_t *out_off, int64_t max_len, double *out, hipStream_t st);
 // Distance profiles (subseq_match.hip): prof[q * nwp + s] = squared cost of
 // window s of series q against pattern (paired ? q : 0), +inf past its last.
 // lane_ok: the lane64 kernels may take the DTW/WDTW rows whose class fits.
