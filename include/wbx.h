@@ -142,6 +142,27 @@ int wbx_argmin_subsequence(const wbx_metric *metric, const double *S,
                            int64_t n, const double *X, int64_t dims, int64_t t,
                            int64_t dim, int64_t k, int64_t *idx, double *dist);
 
+/* pairwise_subsequence_distance with scale=True (distance/_distance.py:
+ * 543-636; "scaled_" + metric, :280-300) through
+ * _pairwise_subsequence_distance (_cdistance.pyx:1018-1037), replacing
+ * ScaledEuclideanSubsequenceMetric (metric->kind WBX_EUCLIDEAN,
+ * distance/_metric.pyx:96-160, scaled_euclidean_distance :561-651) and
+ * ScaledDtwSubsequenceMetric (WBX_DTW with band fraction r,
+ * distance/_elastic.pyx:1928-2203, scaled_dtw_subsequence_distance :343-482,
+ * its LB_Kim / LB_Keogh cascade included).  Every window is z-normalised with
+ * the reference's running mean and std; s_mean / s_std are each pattern's
+ * mean and std as the reference's Subsequence holds them (from_array,
+ * _cdistance.pyx:453-467: np.mean, np.std, std <= 1e-13 stored as 0; a 0 std
+ * is used as 1.0, persistent_distance :283-299).  Float64 arithmetic for
+ * every precision.  Outputs as wbx_subsequence_pairwise; a series with no
+ * accepted window gets index 0 (the reference leaves it unset). */
+int wbx_scaled_subsequence_pairwise(const wbx_metric *metric, const double *S,
+                                    const int64_t *s_off, const int64_t *s_len,
+                                    const double *s_mean, const double *s_std,
+                                    int64_t n_s, const double *X, int64_t n,
+                                    int64_t dims, int64_t t, int64_t dim,
+                                    double *dist, int64_t *idx);
+
 /* The per-series window costs behind _subsequence_match and
  * _paired_subsequence_match (distance/_cdistance.pyx:1065-1141, windows with
  * cost <= threshold^2 as dtw_subsequence_matches, _elastic.pyx:658-698, and
@@ -189,6 +210,19 @@ int wbx_dev_subsequence_pairwise(const wbx_metric *metric, int dtype,
                                  const void *X, int64_t n, int64_t x_stride,
                                  int64_t t, double *dist, int64_t *idx,
                                  void *stream);
+
+/* Device form of wbx_scaled_subsequence_pairwise: S packed patterns of
+ * element type dtype; s_off / s_len (int64) and s_mean / s_std (float64) are
+ * device arrays.  Synchronises `stream` (pattern lengths are read back). */
+int wbx_dev_scaled_subsequence_pairwise(const wbx_metric *metric, int dtype,
+                                        const void *S, const int64_t *s_off,
+                                        const int64_t *s_len,
+                                        const double *s_mean,
+                                        const double *s_std, int64_t n_s,
+                                        const void *X, int64_t n,
+                                        int64_t x_stride, int64_t t,
+                                        double *dist, int64_t *idx,
+                                        void *stream);
 
 /* Device forms of wbx_argmin_subsequence (n patterns, one per series;
  * idx / dist device (n, k)) and wbx_subsequence_profile (prof device

@@ -78,6 +78,13 @@ def _load():
     lib.orc_argmin.argtypes = [i32, d, d, p, i64, i64, i64, p, i64, i64, i64, i64, p, p, p, i32]
     lib.orc_subsequence_pairwise.restype = i32
     lib.orc_subsequence_pairwise.argtypes = [i32, d, d, p, p, p, i64, p, i64, i64, i64, i64, p, p, i32]
+    lib.orc_window_stats.restype = None
+    lib.orc_window_stats.argtypes = [p, i64, i64, p, p]
+    lib.orc_warp_width.restype = i64
+    lib.orc_warp_width.argtypes = [i64, d]
+    lib.orc_scaled_subsequence_pairwise.restype = i32
+    lib.orc_scaled_subsequence_pairwise.argtypes = [i32, d, p, p, p, p, p, i64, p, i64, i64, i32,
+                                                    p, p]
     lib.orc_argmin_replay.restype = None
     lib.orc_argmin_replay.argtypes = [p, i64, i64, p, p, p]
     lib.orc_dtw_cost_matrix.restype = i32
@@ -267,6 +274,48 @@ def subsequence(shapelets, X, metric="dtw", r=1.0, g=0.05, dim=0, n_threads=None
         _kind(metric), r, g, S.ctypes.data, offs.ctypes.data, lens.ctypes.data,
         len(shapelets), X.ctypes.data, n, X.shape[1], X.shape[2], dim,
         dist.ctypes.data, idx.ctypes.data, n_threads)
+    if rc != 0:
+        raise MemoryError("oracle allocation failed")
+    return dist, idx
+
+
+def pattern_stats(shapelets):
+    """(mean, std) of each pattern as ScaledSubsequenceMetric.from_array
+    (distance/_cdistance.pyx:453-467): np.mean, np.std, std <= 1e-13 -> 0."""
+    mean = np.array([np.mean(np.asarray(s, dtype=np.float64)) for s in shapelets])
+    std = np.array([np.std(np.asarray(s, dtype=np.float64)) for s in shapelets])
+    std[std <= 1e-13] = 0.0
+    return mean, std
+
+
+def window_stats(x, L):
+    """Running (mean, std) of every length-L window of the 1-D series x."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    nw = x.shape[0] - L + 1
+    mean, std = np.empty(nw), np.empty(nw)
+    _load().orc_window_stats(x.ctypes.data, x.shape[0], L, mean.ctypes.data, std.ctypes.data)
+    return mean, std
+
+
+def warp_width(L, r):
+    return int(_load().orc_warp_width(L, r))
+
+
+def scaled_subsequence(shapelets, X, metric="dtw", r=1.0, dim=0, cascade=True):
+    """(dist, idx) of pairwise_subsequence_distance(scale=True) for
+    metric "euclidean" (ScaledEuclideanSubsequenceMetric) or "dtw"
+    (ScaledDtwSubsequenceMetric, the UCR-suite cascade).  cascade=False:
+    every window's full scaled DTW (the exhaustive minimum)."""
+    X = np.ascontiguousarray(_ts(X)[:, dim, :])
+    S, offs, lens = pack_shapelets(shapelets)
+    mean, std = pattern_stats(shapelets)
+    n, t = X.shape
+    dist = np.empty((n, len(shapelets)))
+    idx = np.empty((n, len(shapelets)), dtype=np.int64)
+    rc = _load().orc_scaled_subsequence_pairwise(
+        {"euclidean": 0, "dtw": 1}[metric], r, S.ctypes.data, offs.ctypes.data,
+        lens.ctypes.data, mean.ctypes.data, std.ctypes.data, len(shapelets), X.ctypes.data, n,
+        t, int(cascade), dist.ctypes.data, idx.ctypes.data)
     if rc != 0:
         raise MemoryError("oracle allocation failed")
     return dist, idx

@@ -958,3 +958,252 @@ int64_t orc_subsequence_matches(int kind, double r, double g, const double *S, i
     free(cost); free(cost_prev); free(weights);
     return n_matches;
 }
+
+/* ------------------------------------------------------------------------
+ * z-normalised subsequence scans (scale=True).  Every window is normalised
+ * with the running mean and standard deviation of the reference's scan
+ * (sums ex, ex2 updated by + x[i] / - x[i - L] in that order); the pattern
+ * with its own mean and std (std 0 replaced by 1.0, persistent_distance,
+ * distance/_cdistance.pyx:283-299).
+ * ---------------------------------------------------------------------- */
+
+/* Running window statistics of scaled_euclidean_distance
+ * (distance/_metric.pyx:584-598) and scaled_dtw_subsequence_distance
+ * (distance/_elastic.pyx:387-404): mean[I], std[I] for I in [0, t - L]. */
+void orc_window_stats(const double *T, int64_t t, int64_t L, double *mean, double *std) {
+    double ex = 0, ex2 = 0, mu, tmp, v;
+    for (int64_t i = 0; i < t; i++) {
+        v = T[i];
+        ex += v;
+        ex2 += v * v;
+        if (i >= L - 1) {
+            int64_t I = i - (L - 1);
+            mu = ex / (double)L;
+            tmp = ex2 / (double)L - mu * mu;
+            mean[I] = mu;
+            std[I] = tmp > 0 ? sqrt(tmp) : 1.0;
+            v = T[I];
+            ex -= v;
+            ex2 -= v * v;
+        }
+    }
+}
+
+/* scaled_euclidean_distance + inner_scaled_euclidean_distance,
+ * distance/_metric.pyx:561-651 (partial sums abandoned at >= best). */
+double orc_scaled_euclidean_subsequence(const double *S, int64_t L, double s_mean, double s_std,
+                                        const double *T, int64_t t, int64_t *index) {
+    int64_t nwin = t - L + 1;
+    double *mean = malloc(sizeof(double) * (size_t)nwin), *std = malloc(sizeof(double) * (size_t)nwin);
+    if (!mean || !std) { free(mean); free(std); return NAN; }
+    if (s_std == 0) s_std = 1.0;
+    orc_window_stats(T, t, L, mean, std);
+    double min_dist = INFINITY;
+    for (int64_t I = 0; I < nwin; I++) {
+        double dist = 0, x;
+        for (int64_t i = 0; i < L; i++) {
+            if (dist >= min_dist) break;
+            x = (S[i] - s_mean) / s_std;
+            x -= (T[I + i] - mean[I]) / std[I];
+            dist += x * x;
+        }
+        if (dist < min_dist) {
+            min_dist = dist;
+            if (index) *index = I;
+        }
+    }
+    free(mean);
+    free(std);
+    return sqrt(min_dist);
+}
+
+/* _compute_warp_width, distance/_elastic.pyx:1917-1921. */
+int64_t orc_warp_width(int64_t length, double r) {
+    if (r == 1) return length - 1;
+    return (int64_t)floor((double)length * r);
+}
+
+static inline double sqd(double a, double b) {
+    double s = a - b;
+    return s * s;
+}
+
+/* constant_lower_bound, distance/_elastic.pyx:158-226 (LB_Kim on the first
+ * and last three points; the last-but-one term compares T[L-2] with S[L-2]
+ * twice and T[L-1] with S[L-2], as the reference does).  The reference reads
+ * outside both buffers below length 3; there the bound is not applied. */
+static double scaled_lb_kim(const double *S, double s_mean, double s_std, const double *T,
+                            double t_mean, double t_std, int64_t L, double best) {
+    if (L < 3) return 0;
+    if (t_std == 0) return 0;
+    double t_x0 = (T[0] - t_mean) / t_std, t_y0 = (T[L - 1] - t_mean) / t_std;
+    double s_x0 = (S[0] - s_mean) / s_std, s_y0 = (S[L - 1] - s_mean) / s_std;
+    double d = sqd(t_x0, s_x0) + sqd(t_y0, s_y0);
+    if (d >= best) return d;
+    double t_x1 = (T[1] - t_mean) / t_std, s_x1 = (S[1] - s_mean) / s_std;
+    d += dmin(dmin(sqd(t_x1, s_x0), sqd(t_x0, s_x1)), sqd(t_x1, s_x1));
+    if (d >= best) return d;
+    double t_y1 = (T[L - 2] - t_mean) / t_std, s_y1 = (S[L - 2] - s_mean) / s_std;
+    d += dmin(dmin(sqd(t_y1, s_y1), sqd(t_y0, s_y1)), sqd(t_y1, s_y1));
+    if (d >= best) return d;
+    double t_x2 = (T[2] - t_mean) / t_std, s_x2 = (S[2] - s_mean) / s_std;
+    d += dmin(dmin(dmin(sqd(t_x0, s_x2), dmin(sqd(t_x1, s_x2), sqd(t_x2, s_x2))), sqd(t_x2, s_x1)),
+              sqd(t_x2, s_x0));
+    if (d >= best) return d;
+    double t_y2 = (T[L - 3] - t_mean) / t_std, s_y2 = (S[L - 3] - s_mean) / s_std;
+    d += dmin(dmin(dmin(sqd(t_y0, s_y2), dmin(sqd(t_y1, s_y2), sqd(t_y2, s_y2))), sqd(t_y2, s_y1)),
+              sqd(t_y2, s_y0));
+    return d;
+}
+
+/* cumulative_bound, distance/_elastic.pyx:229-260: LB_Keogh of T
+ * (normalised by mean/std) against an envelope normalised by lu_mean/lu_std;
+ * cb[i] receives each term; stops once the sum reaches best. */
+static double scaled_cumulative_bound(const double *T, int64_t L, double mean, double std,
+                                      double lu_mean, double lu_std, const double *lower,
+                                      const double *upper, double *cb, double best) {
+    double sum = 0, x, d, us, ls;
+    for (int64_t i = 0; i < L; i++) {
+        if (sum >= best) break;
+        x = (T[i] - mean) / std;
+        us = (upper[i] - lu_mean) / lu_std;
+        ls = (lower[i] - lu_mean) / lu_std;
+        if (x > us)
+            d = sqd(x, us);
+        else if (x < ls)
+            d = sqd(x, ls);
+        else
+            d = 0;
+        sum += d;
+        cb[i] = d;
+    }
+    return sum;
+}
+
+/* inner_scaled_dtw_subsequence_distance, distance/_elastic.pyx:263-339:
+ * banded (|i - j| <= r) DTW of the normalised pattern (rows) against the
+ * normalised window (columns) on 2 r + 1 slots, abandoned when a row's
+ * minimum plus the remaining bound cb[i + r + 1] reaches best. */
+static double scaled_dtw_inner(const double *S, int64_t L, double s_mean, double s_std,
+                               const double *X, double mean, double std, int64_t r,
+                               const double *cb, double *cost, double *cost_prev, double best) {
+    int64_t i, j, k = 0;
+    double x, y, z, v, min_cost, *tmp;
+    if (std == 0 && s_std == 0) return 0;
+    for (i = 0; i < 2 * r + 1; i++) {
+        cost[i] = INFINITY;
+        cost_prev[i] = INFINITY;
+    }
+    for (i = 0; i < L; i++) {
+        k = imax(0, r - i);
+        min_cost = INFINITY;
+        for (j = imax(0, i - r); j < imin(L, i + r + 1); j++) {
+            if (i == 0 && j == 0) {
+                v = s_std != 0 ? (S[0] - s_mean) / s_std : 0;
+                if (std != 0) v -= (X[0] - mean) / std;
+                cost[k] = v * v;
+            } else {
+                y = (j - 1 < 0 || k - 1 < 0) ? INFINITY : cost[k - 1];
+                x = (i - 1 < 0 || k + 1 > 2 * r) ? INFINITY : cost_prev[k + 1];
+                z = (i - 1 < 0 || j - 1 < 0) ? INFINITY : cost_prev[k];
+                v = (S[i] - s_mean) / s_std;
+                if (std != 0) v -= (X[j] - mean) / std;
+                cost[k] = dmin(dmin(x, y), z) + v * v;
+            }
+            if (cost[k] < min_cost) min_cost = cost[k];
+            k++;
+        }
+        if (i + r < L - 1 && min_cost + cb[i + r + 1] >= best) return min_cost + cb[i + r + 1];
+        tmp = cost;
+        cost = cost_prev;
+        cost_prev = tmp;
+    }
+    return cost_prev[k - 1];
+}
+
+/* ScaledDtwSubsequenceMetric._distance (distance/_elastic.pyx:2074-2110) with
+ * scaled_dtw_subsequence_distance (:343-482): the UCR-suite cascade LB_Kim ->
+ * LB_Keogh(window vs pattern envelope) -> LB_Keogh(pattern vs series
+ * envelope) -> early-abandoned DTW, windows in increasing order, strict <.
+ * The warp width is _compute_warp_width(L, r); both envelopes come from
+ * find_min_max with it.  cascade == 0 computes every window's full DTW
+ * instead (the exhaustive minimum; a test property, not the reference). */
+double orc_scaled_dtw_subsequence(const double *S, int64_t L, double s_mean, double s_std,
+                                  const double *T, int64_t t, double rr, int cascade,
+                                  int64_t *index) {
+    int64_t r = orc_warp_width(L, rr), nwin = t - L + 1, I, k;
+    size_t nb = sizeof(double);
+    double *mean = malloc(nb * (size_t)nwin), *std = malloc(nb * (size_t)nwin),
+           *s_lo = malloc(nb * (size_t)L), *s_up = malloc(nb * (size_t)L),
+           *t_lo = malloc(nb * (size_t)t), *t_up = malloc(nb * (size_t)t),
+           *cb = malloc(nb * (size_t)L), *cb_1 = malloc(nb * (size_t)L),
+           *cb_2 = malloc(nb * (size_t)L), *cost = malloc(nb * (size_t)(2 * r + 1)),
+           *cost_prev = malloc(nb * (size_t)(2 * r + 1));
+    double result = NAN;
+    if (!mean || !std || !s_lo || !s_up || !t_lo || !t_up || !cb || !cb_1 || !cb_2 || !cost ||
+        !cost_prev)
+        goto out;
+    if (s_std == 0) s_std = 1.0;
+    orc_window_stats(T, t, L, mean, std);
+    if (orc_dtw_envelop(S, L, r, s_lo, s_up) || orc_dtw_envelop(T, t, r, t_lo, t_up)) goto out;
+    double min_dist = INFINITY, dist;
+    for (I = 0; I < nwin; I++) {
+        const double *X = T + I;
+        if (!cascade) {
+            for (k = 0; k < L; k++) cb[k] = 0;
+            dist = scaled_dtw_inner(S, L, s_mean, s_std, X, mean[I], std[I], r, cb, cost,
+                                    cost_prev, INFINITY);
+            if (dist < min_dist) {
+                if (index) *index = I;
+                min_dist = dist;
+            }
+            continue;
+        }
+        double lb_kim = scaled_lb_kim(S, s_mean, s_std, X, mean[I], std[I], L, min_dist);
+        if (!(lb_kim < min_dist)) continue;
+        double lb_k = scaled_cumulative_bound(X, L, mean[I], std[I], s_mean, s_std, s_lo, s_up,
+                                              cb_1, min_dist);
+        if (!(lb_k < min_dist)) continue;
+        double lb_k2 = scaled_cumulative_bound(S, L, s_mean, s_std, mean[I], std[I], t_lo + I,
+                                               t_up + I, cb_2, min_dist);
+        if (!(lb_k2 < min_dist)) continue;
+        const double *src = lb_k > lb_k2 ? cb_1 : cb_2;
+        cb[L - 1] = src[L - 1];
+        for (k = L - 2; k >= 0; k--) cb[k] = cb[k + 1] + src[k];
+        dist = scaled_dtw_inner(S, L, s_mean, s_std, X, mean[I], std[I], r, cb, cost, cost_prev,
+                                min_dist);
+        if (dist < min_dist) {
+            if (index) *index = I;
+            min_dist = dist;
+        }
+    }
+    result = sqrt(min_dist);
+out:
+    free(mean); free(std); free(s_lo); free(s_up); free(t_lo); free(t_up);
+    free(cb); free(cb_1); free(cb_2); free(cost); free(cost_prev);
+    return result;
+}
+
+/* _pairwise_subsequence_distance (_cdistance.pyx:1018-1037) for
+ * ScaledEuclideanSubsequenceMetric (kind 0) and ScaledDtwSubsequenceMetric
+ * (kind 1); s_mean / s_std as the Subsequence holds them (from_array,
+ * _cdistance.pyx:453-467).  A series with no accepted window keeps index 0. */
+int orc_scaled_subsequence_pairwise(int kind, double r, const double *S, const int64_t *s_off,
+                                    const int64_t *s_len, const double *s_mean,
+                                    const double *s_std, int64_t n_s, const double *X, int64_t n,
+                                    int64_t t, int cascade, double *dist, int64_t *idx) {
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = 0; j < n_s; j++) {
+            int64_t index = 0;
+            double d = kind == 0
+                           ? orc_scaled_euclidean_subsequence(S + s_off[j], s_len[j], s_mean[j],
+                                                              s_std[j], X + i * t, t, &index)
+                           : orc_scaled_dtw_subsequence(S + s_off[j], s_len[j], s_mean[j],
+                                                        s_std[j], X + i * t, t, r, cascade,
+                                                        &index);
+            if (d != d) return -1;
+            dist[i * n_s + j] = d;
+            idx[i * n_s + j] = index;
+        }
+    return 0;
+}

@@ -1,0 +1,114 @@
+"""GPU parity of the z-normalised subsequence scans (pairwise_subsequence_distance
+with scale=True: ScaledEuclideanSubsequenceMetric, ScaledDtwSubsequenceMetric
+and its LB_Kim / LB_Keogh cascade) against the float64 oracle.  Bar:
+distances and indices bit-identical, for the register-band kernel (policy 0)
+and the scratch-row kernel (policy 1)."""
+
+import numpy as np
+import pytest
+
+import oracle
+from wildboar_amd import _lib
+from wildboar_amd import distance as wd
+
+pytestmark = pytest.mark.gpu
+
+
+def rw(seed, n, t):
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.standard_normal((n, t)), axis=1)
+
+
+def patterns(X, seed):
+    rng = np.random.default_rng(seed)
+    t = X.shape[1]
+    out = []
+    for L in (1, 2, 3, 5, 6, 7, 13, 30, 64, t):
+        s = int(rng.integers(0, t - L + 1))
+        out.append(X[int(rng.integers(0, X.shape[0])), s:s + L] * rng.uniform(0.5, 2.0)
+                   + rng.normal())
+    out += [rng.standard_normal(9), np.full(8, 1.5), rng.standard_normal(40)]
+    return out
+
+
+def setup_module(module):
+    _lib.set_policy(0)
+
+
+def teardown_module(module):
+    _lib.set_policy(0)
+
+
+def run(pats, X, metric, params, policy):
+    _lib.set_policy(policy)
+    try:
+        return wd.pairwise_subsequence_distance(pats, X, metric=metric, metric_params=params,
+                                                scale=True, return_index=True)
+    finally:
+        _lib.set_policy(0)
+
+
+@pytest.mark.parametrize("r", [0.0, 0.05, 0.1, 0.3, 1.0])
+@pytest.mark.parametrize("policy", [0, 1])
+def test_scaled_dtw_bitwise(r, policy):
+    X = rw(900, 11, 150)
+    pats = patterns(X, 901)
+    d, i = run(pats, X, "dtw", {"r": r}, policy)
+    od, oi = oracle.scaled_subsequence(pats, X, "dtw", r=r)
+    np.testing.assert_array_equal(d, od)
+    np.testing.assert_array_equal(i, oi)
+
+
+def test_scaled_euclidean_bitwise_and_names():
+    X = rw(902, 13, 200)
+    pats = patterns(X, 903)
+    d, i = run(pats, X, "euclidean", None, 0)
+    od, oi = oracle.scaled_subsequence(pats, X, "euclidean")
+    np.testing.assert_array_equal(d, od)
+    np.testing.assert_array_equal(i, oi)
+    d2 = wd.pairwise_subsequence_distance(pats, X, metric="scaled_euclidean")
+    np.testing.assert_array_equal(d2, od)
+    d3 = wd.pairwise_subsequence_distance(pats[6], X, metric="scaled_dtw",
+                                          metric_params={"r": 0.2})
+    np.testing.assert_array_equal(d3, oracle.scaled_subsequence([pats[6]], X, "dtw", r=0.2)[0][:, 0])
+
+
+def test_scaled_dtw_where_lb_kim_prunes_the_best_window():
+    # shapes where the reference's LB_Kim exceeds the DTW of the best window
+    # (the oracle's cascade differs from the exhaustive minimum there)
+    rng = np.random.default_rng(1)
+    X = np.cumsum(rng.standard_normal((40, 300)), axis=1)
+    pats = []
+    for _ in range(30):
+        L = int(rng.integers(6, 60))
+        s = int(rng.integers(0, 300 - L))
+        pats.append(X[int(rng.integers(0, 40)), s:s + L] * rng.uniform(0.5, 2) + rng.normal())
+    od, oi = oracle.scaled_subsequence(pats, X, "dtw", r=0.3)
+    ed, _ = oracle.scaled_subsequence(pats, X, "dtw", r=0.3, cascade=False)
+    assert (od != ed).sum() > 0
+    for policy in (0, 1):
+        d, i = run(pats, X, "dtw", {"r": 0.3}, policy)
+        np.testing.assert_array_equal(d, od)
+        np.testing.assert_array_equal(i, oi)
+
+
+def test_scaled_float32_inputs_and_validation():
+    X = rw(904, 6, 90).astype(np.float32)
+    pats = [X[1, 10:30].astype(np.float64), np.arange(12.0)]
+    d, i = wd.pairwise_subsequence_distance(pats, X, metric="dtw", metric_params={"r": 0.1},
+                                            scale=True, return_index=True)
+    od, oi = oracle.scaled_subsequence(pats, X.astype(np.float64), "dtw", r=0.1)
+    np.testing.assert_array_equal(d, od)
+    np.testing.assert_array_equal(i, oi)
+    with pytest.raises(NotImplementedError):
+        wd.pairwise_subsequence_distance(pats, X, metric="wdtw", scale=True)
+    with pytest.raises(NotImplementedError):
+        wd.pairwise_subsequence_distance(pats, X, metric="scaled_lcss")
+    with pytest.raises(ValueError):
+        wd.pairwise_subsequence_distance(pats, X, metric="scaled_nope")
+    with pytest.raises(ValueError):
+        wd.pairwise_subsequence_distance(pats, X, metric="dtw", metric_params={"r": 1.5},
+                                         scale=True)
+    with pytest.raises(TypeError):
+        wd.pairwise_subsequence_distance(pats, X, metric="euclidean", metric_params={"r": 0.1},
+                                         scale=True)

@@ -1,0 +1,619 @@
+// scaled.hip -- z-normalised subsequence scans, float64.
+//
+// ScaledEuclideanSubsequenceMetric (distance/_metric.pyx:96-160,
+// scaled_euclidean_distance :561-651) and ScaledDtwSubsequenceMetric
+// (distance/_elastic.pyx:1928-2203, scaled_dtw_subsequence_distance
+// :343-482, inner_scaled_dtw_subsequence_distance :263-339) as
+// _pairwise_subsequence_distance drives them (_cdistance.pyx:1018-1037).
+//
+// Window statistics.  The reference normalises window I with the mean and
+// std of a running scan, ex += x[k], ex2 += x[k]^2 and, after window I,
+// ex -= x[I], ex2 -= x[I]^2; mean = ex / L, std = sqrt(ex2 / L - mean^2)
+// (1.0 when that is <= 0).  The rounding of those sums depends on every
+// earlier step, so k_window_stats replays the scan, one thread per series.
+//
+// Cascade.  Per window the reference computes, in this order, LB_Kim
+// (constant_lower_bound :158-226, partial sums returned once they reach the
+// best so far), LB_Keogh of the window against the pattern's envelope and
+// of the pattern against the series' envelope (cumulative_bound :229-260,
+// same early exit), then the DTW, abandoned at row i when the row minimum
+// plus the remaining bound cb[i + r + 1] reaches the best.  A window is
+// accepted (best := its DTW, index := I) iff every one of those values is
+// below the best.  Each is a sum of non-negative terms, so its partial value
+// reaches the best iff its full value does; hence the window is accepted iff
+//     T(I) = max(K, LK, LK2, max_i(rowmin_i + cb[i + r + 1]), D) < best,
+// with every term computed in full.  The kernels compute T and D for every
+// window in parallel and replay the acceptance rule over the windows in
+// increasing order: the reference's result bit for bit, including windows
+// its LB_Kim prunes although their DTW is the smallest (its last-but-one
+// term omits the (T[L-2], S[L-1]) cell, and below length 6 the first and
+// last three points overlap).  Below length 3 the reference's LB_Kim reads
+// outside both buffers; there it is not applied (K = 0).
+#pragma clang fp contract(off)
+
+#include <algorithm>
+#include <vector>
+
+#include "wbx_internal.h"
+
+namespace wbx {
+
+namespace {
+
+__host__ __device__ inline int warp_width(int L, double r) {
+    // _compute_warp_width, distance/_elastic.pyx:1917-1921
+    if (r == 1.0) return L - 1;
+    return (int)floor((double)L * r);
+}
+
+__device__ __forceinline__ double dmin(double a, double b) { return b < a ? b : a; }
+__device__ __forceinline__ double dmax(double a, double b) { return b > a ? b : a; }
+__device__ __forceinline__ double sqd(double a, double b) {
+    const double s = a - b;
+    return s * s;
+}
+
+// One thread per series: mean / std of windows [0, t - L], row stride t-L+1.
+__global__ __launch_bounds__(64) void k_window_stats(const double *__restrict__ X, int64_t n,
+                                                     int64_t stride, int t, int L,
+                                                     double *__restrict__ mean,
+                                                     double *__restrict__ sdev) {
+    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    const double *__restrict__ x = X + i * stride;
+    const int nwin = t - L + 1;
+    double *__restrict__ mo = mean + i * nwin;
+    double *__restrict__ so = sdev + i * nwin;
+    const double dl = (double)L;
+    double ex = 0.0, ex2 = 0.0;
+    for (int k = 0; k < t; ++k) {
+        const double v = x[k];
+        ex += v;
+        ex2 += v * v;
+        if (k >= L - 1) {
+            const int I = k - (L - 1);
+            const double mu = ex / dl;
+            const double tmp = ex2 / dl - mu * mu;
+            mo[I] = mu;
+            so[I] = tmp > 0.0 ? sqrt(tmp) : 1.0;
+            const double u = x[I];
+            ex -= u;
+            ex2 -= u * u;
+        }
+    }
+}
+
+// One block per pattern: zs = (S - mean) / std (std 0 read as 1.0,
+// persistent_distance, _cdistance.pyx:283-299) and, for DTW, the pattern's
+// envelope (find_min_max with the warp width, min / max over [k - w, k + w])
+// normalised the same way, as cumulative_bound normalises it.
+__global__ __launch_bounds__(256) void k_scaled_patterns(
+    const double *__restrict__ S, const int64_t *__restrict__ off, const int64_t *__restrict__ len,
+    const double *__restrict__ s_mean, const double *__restrict__ s_std, double r, int with_env,
+    double *__restrict__ zs, double *__restrict__ zl, double *__restrict__ zu) {
+    const int64_t j = blockIdx.x;
+    const int L = (int)len[j];
+    const int64_t o = off[j];
+    const double *__restrict__ s = S + o;
+    const double mu = s_mean[j];
+    const double sd = s_std[j] != 0.0 ? s_std[j] : 1.0;
+    const int w = warp_width(L, r);
+    for (int k = threadIdx.x; k < L; k += 256) {
+        zs[o + k] = (s[k] - mu) / sd;
+        if (with_env) {
+            const int a = k - w > 0 ? k - w : 0;
+            const int b = k + w < L - 1 ? k + w : L - 1;
+            double lo = s[a], hi = s[a];
+            for (int q = a + 1; q <= b; ++q) {
+                const double v = s[q];
+                lo = v < lo ? v : lo;
+                hi = v > hi ? v : hi;
+            }
+            zl[o + k] = (lo - mu) / sd;
+            zu[o + k] = (hi - mu) / sd;
+        }
+    }
+}
+
+struct ScaledTask {
+    const double *X;  // rows of the current chunk
+    int64_t xstride;
+    int t, L, rw, B;
+    int64_t c0, cn;   // first series of the chunk, series in the chunk
+    const double *wmean, *wstd;  // (cn, t - L + 1)
+    const double *elo, *eup;     // series envelopes (cn, t), DTW only
+    const double *zs, *zl, *zu;  // normalised patterns (packed as S)
+    const int64_t *s_off;
+    const int64_t *order;        // pattern indices of this length group
+    int64_t gc, n_s;
+    double *dist;
+    int64_t *idx;
+};
+
+__device__ __forceinline__ int64_t task_id() {
+    return (int64_t)blockIdx.x + (int64_t)blockIdx.y * gridDim.x;
+}
+
+// Keeps the first strict minimum of each lane, then the wave's smallest
+// value, smallest window among equal values.
+__device__ __forceinline__ void wave_first_min(double &best, int &best_s) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o);
+        const int os = __shfl_xor(best_s, o);
+        if (ob < best || (ob == best && os < best_s)) {
+            best = ob;
+            best_s = os;
+        }
+    }
+}
+
+// scaled_euclidean_distance: one wave per (series, pattern), one window per
+// lane; the full sum of ((S[k] - s_mean) / s_std - (x - mean) / std)^2 in k
+// order (the partial-sum abandon only drops windows whose full sum is not
+// below the best).
+__global__ __launch_bounds__(64) void k_scaled_euclid(ScaledTask a) {
+    const int64_t q = task_id();
+    if (q >= a.cn * a.gc) return;
+    const int64_t ii = q / a.gc, jj = q - ii * a.gc;
+    const int64_t pj = a.order[jj];
+    const int L = a.L, nwin = a.t - L + 1;
+    const double *__restrict__ x = a.X + ii * a.xstride;
+    const double *__restrict__ z = a.zs + a.s_off[pj];
+    const double *__restrict__ mrow = a.wmean + ii * nwin;
+    const double *__restrict__ srow = a.wstd + ii * nwin;
+    double best = WBX_INF_D;
+    int best_s = nwin;
+    for (int b0 = 0; b0 < nwin; b0 += 64) {
+        const int s = b0 + (int)threadIdx.x;
+        const int sc = s < nwin ? s : nwin - 1;
+        const double mu = mrow[sc], sd = srow[sc];
+        const double *__restrict__ xw = x + sc;
+        double d = 0.0;
+        for (int k = 0; k < L; ++k) {
+            double v = z[k];
+            v -= (xw[k] - mu) / sd;
+            d += v * v;
+        }
+        if (s < nwin && d < best) {
+            best = d;
+            best_s = s;
+        }
+    }
+    wave_first_min(best, best_s);
+    if (threadIdx.x == 0) {
+        const int64_t o = (a.c0 + ii) * a.n_s + pj;
+        a.dist[o] = sqrt(best);
+        a.idx[o] = best_s < nwin ? best_s : 0;
+    }
+}
+
+// LB_Kim of the reference (constant_lower_bound), in full; L >= 3.
+__device__ double scaled_kim(const double *__restrict__ z, const double *__restrict__ xw,
+                             double mu, double sd, int L) {
+    const double tx0 = (xw[0] - mu) / sd, ty0 = (xw[L - 1] - mu) / sd;
+    const double sx0 = z[0], sy0 = z[L - 1];
+    double d = sqd(tx0, sx0) + sqd(ty0, sy0);
+    const double tx1 = (xw[1] - mu) / sd, sx1 = z[1];
+    d += dmin(dmin(sqd(tx1, sx0), sqd(tx0, sx1)), sqd(tx1, sx1));
+    const double ty1 = (xw[L - 2] - mu) / sd, sy1 = z[L - 2];
+    d += dmin(dmin(sqd(ty1, sy1), sqd(ty0, sy1)), sqd(ty1, sy1));
+    const double tx2 = (xw[2] - mu) / sd, sx2 = z[2];
+    d += dmin(dmin(dmin(sqd(tx0, sx2), dmin(sqd(tx1, sx2), sqd(tx2, sx2))), sqd(tx2, sx1)),
+              sqd(tx2, sx0));
+    const double ty2 = (xw[L - 3] - mu) / sd, sy2 = z[L - 3];
+    d += dmin(dmin(dmin(sqd(ty0, sy2), dmin(sqd(ty1, sy2), sqd(ty2, sy2))), sqd(ty2, sy1)),
+              sqd(ty2, sy0));
+    return d;
+}
+
+__device__ __forceinline__ double keogh_term(double x, double ls, double us) {
+    if (x > us) return sqd(x, us);
+    if (x < ls) return sqd(x, ls);
+    return 0.0;
+}
+
+// The bounds of one window: returns max(K, LK, LK2) and leaves, at
+// cb[m * 64] for m in [0, nc), nc = L - rw - 1, the suffix sum
+// cb[rw + 1 + m] of the chosen LB_Keogh terms (cb_1 when LK > LK2, else
+// cb_2), built from the last position backwards as the reference does.
+// zx (nullable): receives the normalised window at zx[k * 64].
+__device__ double scaled_bounds(const ScaledTask &a, const double *__restrict__ z,
+                                const double *__restrict__ zl, const double *__restrict__ zu,
+                                const double *__restrict__ xw, const double *__restrict__ el,
+                                const double *__restrict__ eu, double mu, double sd,
+                                double *__restrict__ cb, double *__restrict__ zx) {
+    const int L = a.L, rw = a.rw, nc = L - rw - 1;
+    double T = L >= 3 ? scaled_kim(z, xw, mu, sd, L) : 0.0;
+    double lk = 0.0, lk2 = 0.0;
+    for (int k = 0; k < L; ++k) {
+        const double x = (xw[k] - mu) / sd;
+        if (zx) zx[(int64_t)k * 64] = x;
+        const double d = keogh_term(x, zl[k], zu[k]);
+        lk += d;
+        if (k > rw) cb[(int64_t)(k - rw - 1) * 64] = d;
+    }
+    for (int k = 0; k < L; ++k) {
+        const double us = (eu[k] - mu) / sd;
+        const double ls = (el[k] - mu) / sd;
+        const double d = keogh_term(z[k], ls, us);
+        lk2 += d;
+        if (k > rw) cb[(int64_t)(nc + k - rw - 1) * 64] = d;
+    }
+    const bool one = lk > lk2;
+    double acc = 0.0;
+    for (int k = L - 1; k > rw; --k) {
+        const int m = k - rw - 1;
+        const double v = one ? cb[(int64_t)m * 64] : cb[(int64_t)(nc + m) * 64];
+        acc = k == L - 1 ? v : acc + v;
+        cb[(int64_t)m * 64] = acc;
+    }
+    T = dmax(T, lk);
+    return dmax(T, lk2);
+}
+
+// Replays the reference's acceptance over the 64 windows b0 + l in order.
+__device__ __forceinline__ void accept_block(double T, double D, int b0, int nwin, double &best,
+                                             int &bi) {
+    for (int l = 0; l < 64; ++l) {
+        const double Tl = __shfl(T, l), Dl = __shfl(D, l);
+        if (b0 + l < nwin && Tl < best) {
+            best = Dl;
+            bi = b0 + l;
+        }
+    }
+}
+
+__device__ __forceinline__ double ldz(const double *__restrict__ xw, int j, int L, double mu,
+                                      double sd) {
+    const int jc = j < 0 ? 0 : (j >= L ? L - 1 : j);
+    const double v = (xw[jc] - mu) / sd;
+    return (unsigned)j < (unsigned)L ? v : WBX_INF_D;
+}
+
+// One row of the register band: slot k holds column j = i + k - rw.  Slots
+// >= B are reset to +inf after the row, so the row minimum over all BP slots
+// is the minimum over the band cells (cells outside the window are +inf).
+template <int BP>
+__device__ __forceinline__ double srow(double (&row)[BP], const double *yw, double xi,
+                                       const double (&pen)[8]) {
+    double left = WBX_INF_D;
+#pragma unroll
+    for (int k = 0; k < BP; ++k) {
+        const double up = (k + 1 < BP) ? row[k + 1] : WBX_INF_D;
+        const double m = dmin(dmin(up, left), row[k]);
+        const double v = xi - yw[k];
+        const double c = m + v * v;
+        row[k] = c;
+        left = c;
+    }
+#pragma unroll
+    for (int q = 1; q < 8; ++q) row[BP - 8 + q] = fmax(row[BP - 8 + q], pen[q]);
+    double rm = row[0];
+#pragma unroll
+    for (int k = 1; k < BP; ++k) rm = dmin(rm, row[k]);
+    return rm;
+}
+
+// Banded DTW (|i - j| <= rw) of z (rows) against the normalised window
+// (columns) in registers; D = cell (L-1, L-1), M = max over rows i with
+// i + rw < L - 1 of rowmin_i + cb[i * 64].
+template <int BP, int U>
+__device__ __forceinline__ void scaled_cost(const double *__restrict__ z, int L,
+                                            const double *__restrict__ xw, double mu, double sd,
+                                            int rw, int B, const double *__restrict__ cb,
+                                            double &D, double &M) {
+    constexpr int W = BP + U - 1;
+    const int off = rw;
+    double row[BP];
+#pragma unroll
+    for (int k = 0; k < BP; ++k) row[k] = (k == off) ? 0.0 : WBX_INF_D;
+    double pen[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) pen[q] = (BP - 8 + q >= B) ? WBX_INF_D : -WBX_INF_D;
+    double yw[W];
+#pragma unroll
+    for (int m = 0; m < W; ++m) yw[m] = ldz(xw, m - off, L, mu, sd);
+    const int last = L - 1 - rw;  // rows i < last are checked against cb
+    M = -WBX_INF_D;
+    double rm = srow<BP>(row, yw, z[0], pen);
+    if (0 < last) M = dmax(M, rm + cb[0]);
+#pragma unroll
+    for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
+    yw[W - 1] = ldz(xw, 1 - off + W - 1, L, mu, sd);
+    const int rem = (L - 1) % U;
+    for (int i = 1; i <= rem; ++i) {
+        rm = srow<BP>(row, yw, z[i], pen);
+        if (i < last) M = dmax(M, rm + cb[(int64_t)i * 64]);
+#pragma unroll
+        for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
+        yw[W - 1] = ldz(xw, i + 1 - off + W - 1, L, mu, sd);
+    }
+    for (int i0 = 1 + rem; i0 < L; i0 += U) {
+        double zv[U], yn[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            zv[u] = z[i0 + u];
+            yn[u] = ldz(xw, i0 + U - off + W - U + u, L, mu, sd);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            rm = srow<BP>(row, yw + u, zv[u], pen);
+            if (i0 + u < last) M = dmax(M, rm + cb[(int64_t)(i0 + u) * 64]);
+        }
+#pragma unroll
+        for (int m = 0; m < W - U; ++m) yw[m] = yw[m + U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) yw[W - U + u] = yn[u];
+    }
+    double res = WBX_INF_D;
+#pragma unroll
+    for (int k = 0; k < BP; ++k)
+        if (k == off) res = row[k];
+    D = res;
+}
+
+// Scaled DTW, band in registers (2 rw + 1 <= BP).  One wave per (series,
+// pattern) task, tasks strided over a fixed grid; block b owns the scratch
+// columns scratch[b * per_lane * 64 + m * 64 + lane].
+template <int BP, int U>
+__global__ __launch_bounds__(64) void k_scaled_dtw(ScaledTask a, double *__restrict__ scratch,
+                                                   int64_t per_lane) {
+    double *__restrict__ cb = scratch + (int64_t)blockIdx.x * per_lane * 64 + threadIdx.x;
+    const int L = a.L, nwin = a.t - L + 1;
+    for (int64_t q = blockIdx.x; q < a.cn * a.gc; q += gridDim.x) {
+        const int64_t ii = q / a.gc, jj = q - ii * a.gc;
+        const int64_t pj = a.order[jj];
+        const int64_t so = a.s_off[pj];
+        const double *__restrict__ x = a.X + ii * a.xstride;
+        const double *__restrict__ el = a.elo + ii * a.t;
+        const double *__restrict__ eu = a.eup + ii * a.t;
+        const double *__restrict__ mrow = a.wmean + ii * nwin;
+        const double *__restrict__ srw = a.wstd + ii * nwin;
+        double best = WBX_INF_D;
+        int bi = -1;
+        for (int b0 = 0; b0 < nwin; b0 += 64) {
+            const int s = b0 + (int)threadIdx.x;
+            const int sc = s < nwin ? s : nwin - 1;
+            const double mu = mrow[sc], sd = srw[sc];
+            double T = scaled_bounds(a, a.zs + so, a.zl + so, a.zu + so, x + sc, el + sc, eu + sc,
+                                     mu, sd, cb, nullptr);
+            double D, M;
+            scaled_cost<BP, U>(a.zs + so, L, x + sc, mu, sd, a.rw, a.B, cb, D, M);
+            T = dmax(dmax(T, M), D);
+            accept_block(T, D, b0, nwin, best, bi);
+        }
+        if (threadIdx.x == 0) {
+            const int64_t o = (a.c0 + ii) * a.n_s + pj;
+            a.dist[o] = sqrt(best);
+            a.idx[o] = bi >= 0 ? bi : 0;
+        }
+    }
+}
+
+// Scaled DTW for any band: the normalised window and one cost row per lane
+// live in scratch (columns j at zx[j * 64], row[j * 64]); the row is updated
+// in place left to right with the previous row's value of column j - 1 kept
+// as the diagonal.  Columns a row has not reached yet hold +inf.
+__global__ __launch_bounds__(64) void k_scaled_dtw_wide(ScaledTask a, double *__restrict__ scratch,
+                                                        int64_t per_lane) {
+    double *__restrict__ base = scratch + (int64_t)blockIdx.x * per_lane * 64 + threadIdx.x;
+    const int L = a.L, rw = a.rw, nwin = a.t - L + 1;
+    const int nc = L - rw - 1;
+    double *__restrict__ cb = base;
+    double *__restrict__ zx = base + (int64_t)2 * nc * 64;
+    double *__restrict__ row = zx + (int64_t)L * 64;
+    for (int64_t q = blockIdx.x; q < a.cn * a.gc; q += gridDim.x) {
+        const int64_t ii = q / a.gc, jj = q - ii * a.gc;
+        const int64_t pj = a.order[jj];
+        const int64_t so = a.s_off[pj];
+        const double *__restrict__ z = a.zs + so;
+        const double *__restrict__ x = a.X + ii * a.xstride;
+        const double *__restrict__ el = a.elo + ii * a.t;
+        const double *__restrict__ eu = a.eup + ii * a.t;
+        const double *__restrict__ mrow = a.wmean + ii * nwin;
+        const double *__restrict__ srw = a.wstd + ii * nwin;
+        double best = WBX_INF_D;
+        int bi = -1;
+        for (int b0 = 0; b0 < nwin; b0 += 64) {
+            const int s = b0 + (int)threadIdx.x;
+            const int sc = s < nwin ? s : nwin - 1;
+            const double mu = mrow[sc], sd = srw[sc];
+            double T = scaled_bounds(a, z, a.zl + so, a.zu + so, x + sc, el + sc, eu + sc, mu, sd,
+                                     cb, zx);
+            for (int j = 0; j < L; ++j) row[(int64_t)j * 64] = WBX_INF_D;
+            double M = -WBX_INF_D;
+            for (int i = 0; i < L; ++i) {
+                const double xi = z[i];
+                const int j0 = i - rw > 0 ? i - rw : 0;
+                const int j1 = i + rw < L - 1 ? i + rw : L - 1;
+                double left = WBX_INF_D;
+                double diag = i == 0 ? 0.0 : (j0 > 0 ? row[(int64_t)(j0 - 1) * 64] : WBX_INF_D);
+                double rm = WBX_INF_D;
+                for (int j = j0; j <= j1; ++j) {
+                    const double up = row[(int64_t)j * 64];
+                    const double v = xi - zx[(int64_t)j * 64];
+                    const double c = dmin(dmin(up, left), diag) + v * v;
+                    diag = up;
+                    row[(int64_t)j * 64] = c;
+                    left = c;
+                    rm = dmin(rm, c);
+                }
+                if (i < nc) M = dmax(M, rm + cb[(int64_t)i * 64]);
+            }
+            const double D = row[(int64_t)(L - 1) * 64];
+            T = dmax(dmax(T, M), D);
+            accept_block(T, D, b0, nwin, best, bi);
+        }
+        if (threadIdx.x == 0) {
+            const int64_t o = (a.c0 + ii) * a.n_s + pj;
+            a.dist[o] = sqrt(best);
+            a.idx[o] = bi >= 0 ? bi : 0;
+        }
+    }
+}
+
+constexpr int kMaxBPS = 56;
+
+#define WBX_SCALED_CLASSES(X) X(8) X(16) X(24) X(32) X(40) X(48) X(56)
+
+inline dim3 task_grid(int64_t nq) {
+    const int64_t gx = nq < 65535 ? nq : 65535;
+    return dim3((unsigned)gx, (unsigned)((nq + gx - 1) / gx));
+}
+
+// Scratch doubles per lane (wide kernel; the register kernel uses the first
+// 2 (L - 1)).
+inline int64_t per_lane_doubles(int64_t max_len) { return 4 * max_len; }
+constexpr size_t kScratchCap = (size_t)256 << 20;
+constexpr size_t kChunkCap = (size_t)512 << 20;
+
+int64_t scratch_waves(int64_t max_len) {
+    const int64_t w = (int64_t)(kScratchCap / ((size_t)per_lane_doubles(max_len) * 64 * 8));
+    return w < 1 ? 1 : (w > 2048 ? 2048 : w);
+}
+
+int64_t chunk_rows(int kind, int64_t n, int64_t t, int64_t min_len) {
+    const int64_t per = (t - min_len + 1) * 2 * 8 + (kind == WBX_DTW ? t * 2 * 8 : 0);
+    int64_t c = (int64_t)(kChunkCap / (size_t)per);
+    if (c < 1) c = 1;
+    return c < n ? c : n;
+}
+
+}  // namespace
+
+size_t scaled_ws_bytes(int kind, const int64_t *hl, int64_t n_s, int64_t total, int64_t n,
+                       int64_t t) {
+    int64_t min_len = t, max_len = 1;
+    for (int64_t j = 0; j < n_s; ++j) {
+        min_len = hl[j] < min_len ? hl[j] : min_len;
+        max_len = hl[j] > max_len ? hl[j] : max_len;
+    }
+    const int64_t cn = chunk_rows(kind, n, t, min_len);
+    size_t b = 0;
+    auto take = [&](size_t bytes) { b += (bytes + 255) & ~(size_t)255; };
+    take((size_t)total * 8 * 3);                         // zs, zl, zu
+    take((size_t)n_s * 8);                               // order
+    take((size_t)cn * (size_t)(t - min_len + 1) * 16);   // window mean / std
+    if (kind == WBX_DTW) {
+        take((size_t)cn * (size_t)t * 16);               // series envelopes
+        take((size_t)scratch_waves(max_len) * 64 * (size_t)per_lane_doubles(max_len) * 8);
+    }
+    return b;
+}
+
+int launch_scaled_subseq(int kind, double r, const double *S, const int64_t *s_off,
+                         const int64_t *s_len, const int64_t *hl, int64_t n_s, int64_t total,
+                         const double *s_mean, const double *s_std, const double *X,
+                         int64_t xstride, int64_t n, int64_t t, double *dist, int64_t *idx,
+                         char *ws, bool wide_only, hipStream_t st) {
+    if (n <= 0 || n_s <= 0) return WBX_OK;
+    if (t > 0x7fffffff || total > 0x7fffffffLL * 4)
+        return fail(WBX_EUNSUPPORTED, "scaled scan: series too long");
+    const bool dtw = kind == WBX_DTW;
+    // patterns sorted by length (stable): groups share window statistics
+    std::vector<int64_t> order((size_t)n_s);
+    for (int64_t j = 0; j < n_s; ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return hl[a] < hl[b]; });
+    int64_t min_len = t, max_len = 1;
+    for (int64_t j = 0; j < n_s; ++j) {
+        min_len = hl[j] < min_len ? hl[j] : min_len;
+        max_len = hl[j] > max_len ? hl[j] : max_len;
+    }
+    const int64_t cn_max = chunk_rows(kind, n, t, min_len);
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        char *p = ws + o;
+        o += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    double *zs = (double *)take((size_t)total * 8);
+    double *zl = (double *)take((size_t)total * 8);
+    double *zu = (double *)take((size_t)total * 8);
+    int64_t *d_order = (int64_t *)take((size_t)n_s * 8);
+    double *wmean = (double *)take((size_t)cn_max * (size_t)(t - min_len + 1) * 16);
+    double *wstd = wmean + cn_max * (t - min_len + 1);
+    double *elo = nullptr, *eup = nullptr, *scratch = nullptr;
+    const int64_t waves = scratch_waves(max_len);
+    if (dtw) {
+        elo = (double *)take((size_t)cn_max * (size_t)t * 16);
+        eup = elo + cn_max * t;
+        scratch = (double *)take((size_t)waves * 64 * (size_t)per_lane_doubles(max_len) * 8);
+    }
+    WBX_HIP_CHECK(hipMemcpyAsync(d_order, order.data(), sizeof(int64_t) * n_s,
+                                 hipMemcpyHostToDevice, st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));  // `order` ends with this call
+    hipLaunchKernelGGL(k_scaled_patterns, dim3((unsigned)n_s), dim3(256), 0, st, S, s_off, s_len,
+                       s_mean, s_std, r, dtw ? 1 : 0, zs, zl, zu);
+    WBX_HIP_CHECK(hipGetLastError());
+    const int64_t per_lane = per_lane_doubles(max_len);
+    for (int64_t c0 = 0; c0 < n; c0 += cn_max) {
+        const int64_t cn = n - c0 < cn_max ? n - c0 : cn_max;
+        const double *xc = X + c0 * xstride;
+        int env_w = -1;
+        for (int64_t g0 = 0; g0 < n_s;) {
+            const int L = (int)hl[order[g0]];
+            int64_t g1 = g0 + 1;
+            while (g1 < n_s && hl[order[g1]] == L) ++g1;
+            hipLaunchKernelGGL(k_window_stats, dim3((unsigned)((cn + 63) / 64)), dim3(64), 0, st,
+                               xc, cn, xstride, (int)t, L, wmean, wstd);
+            WBX_HIP_CHECK(hipGetLastError());
+            ScaledTask a{};
+            a.X = xc;
+            a.xstride = xstride;
+            a.t = (int)t;
+            a.L = L;
+            a.c0 = c0;
+            a.cn = cn;
+            a.wmean = wmean;
+            a.wstd = wstd;
+            a.zs = zs;
+            a.zl = zl;
+            a.zu = zu;
+            a.s_off = s_off;
+            a.order = d_order + g0;
+            a.gc = g1 - g0;
+            a.n_s = n_s;
+            a.dist = dist;
+            a.idx = idx;
+            const int64_t nq = cn * a.gc;
+            if (!dtw) {
+                hipLaunchKernelGGL(k_scaled_euclid, task_grid(nq), dim3(64), 0, st, a);
+                WBX_HIP_CHECK(hipGetLastError());
+            } else {
+                const int rw = warp_width(L, r);
+                if (rw != env_w) {
+                    const int rc = launch_envelope(xc, cn, xstride, (int)t, rw, elo, eup, st);
+                    if (rc) return rc;
+                    env_w = rw;
+                }
+                a.elo = elo;
+                a.eup = eup;
+                a.rw = rw;
+                a.B = 2 * rw + 1;
+                const int BP = (a.B + 7) / 8 * 8;
+                const dim3 grid((unsigned)(nq < waves ? nq : waves));
+                if (!wide_only && BP <= kMaxBPS) {
+#define WBX_SCALED_LAUNCH(C)                                                                  \
+    case C:                                                                                   \
+        hipLaunchKernelGGL((k_scaled_dtw<C, 4>), grid, dim3(64), 0, st, a, scratch, per_lane); \
+        break;
+                    switch (BP) {
+                        WBX_SCALED_CLASSES(WBX_SCALED_LAUNCH)
+                        default: return fail(WBX_EUNSUPPORTED, "scaled dtw: band %d", a.B);
+                    }
+#undef WBX_SCALED_LAUNCH
+                } else {
+                    hipLaunchKernelGGL(k_scaled_dtw_wide, grid, dim3(64), 0, st, a, scratch,
+                                       per_lane);
+                }
+                WBX_HIP_CHECK(hipGetLastError());
+            }
+            g0 = g1;
+        }
+    }
+    return WBX_OK;
+}
+
+}  // namespace wbx

@@ -237,6 +237,21 @@ int launch_subseq_exact(int kind, double r, Shapelets sh, int max_len, Series<T>
                         const T *weights, double *dist, int64_t *idx, hipStream_t st,
                         int lane_cap = 0, int band_add = 0, const double *penalty = nullptr);
 
+// ------------------------------------------------ z-normalised scans
+// pairwise_subsequence_distance with scale=True for kind WBX_EUCLIDEAN
+// (ScaledEuclideanSubsequenceMetric) or WBX_DTW (ScaledDtwSubsequenceMetric,
+// band fraction r): float64 patterns S (device, s_off / s_len device, hl the
+// host lengths, total = packed length), pattern mean / std (device), float64
+// series X (n rows of length t, stride xstride).  ws holds scaled_ws_bytes.
+// wide_only: the DTW band in scratch memory for every width.
+size_t scaled_ws_bytes(int kind, const int64_t *hl, int64_t n_s, int64_t total, int64_t n,
+                       int64_t t);
+int launch_scaled_subseq(int kind, double r, const double *S, const int64_t *s_off,
+                         const int64_t *s_len, const int64_t *hl, int64_t n_s, int64_t total,
+                         const double *s_mean, const double *s_std, const double *X,
+                         int64_t xstride, int64_t n, int64_t t, double *dist, int64_t *idx,
+                         char *ws, bool wide_only, hipStream_t st);
+
 // --------------------------------------------------------- warping paths
 // dtw_mapping's path of each pair (X row p, Y row p), band width w, as
 // per-row column ranges rows[p][2 i .. 2 i + 1]; moves: n * lx * B bytes.

@@ -136,8 +136,9 @@ def _adtw(r=1.0, p=1.0, **params):
 
 
 # The subset of the reference's _METRICS registry (_distance.py:185-204)
-# served here.  Subsequence functions serve euclidean, dtw and wdtw; the
-# other names raise NotImplementedError there.
+# served here.  pairwise_subsequence_distance serves all of them, and with
+# scale=True scaled_euclidean and scaled_dtw; the window-profile functions
+# serve euclidean, dtw and wdtw and raise NotImplementedError for the rest.
 METRICS = {"euclidean": _euclidean, "dtw": _dtw, "wdtw": _wdtw, "ddtw": _ddtw,
            "wddtw": _wddtw, "adtw": _adtw}
 
@@ -394,8 +395,10 @@ def pairwise_subsequence_distance(y, x, *, dim=0, metric="euclidean", metric_par
     for s in y:
         if s.shape[0] > x.shape[-1]:
             raise ValueError("Invalid subsequnce shape (%d > %d)" % (s.shape[0], x.shape[-1]))
-    if scale:
-        raise NotImplementedError("scaled subsequence metrics are not served by this library")
+    scaled = _scaled_name(metric, scale)
+    if scaled is not None:
+        return _scaled_pairwise_subsequence(y, x, scaled, metric_params, dim, return_index,
+                                            precision)
     metric_ = check_metric(metric)(**(metric_params if metric_params is not None else {}))
     prec = _precision(precision)
     x_ = _ts(x)
@@ -415,6 +418,73 @@ def pairwise_subsequence_distance(y, x, *, dim=0, metric="euclidean", metric_par
         metric_.struct(prec), S.ctypes.data, offs.ctypes.data, lens.ctypes.data, len(y),
         x_.ctypes.data, x_.shape[0], x_.shape[1], x_.shape[2], int(dim), dist.ctypes.data,
         idx.ctypes.data))
+    if return_index:
+        return _format_return(dist, len(y), x.ndim), _format_return(idx, len(y), x.ndim)
+    return _format_return(dist, len(y), x.ndim)
+
+
+# Scaled subsequence metrics served: "scaled_" + these (_distance.py:151-183);
+# the registry's other scaled names raise NotImplementedError.
+_SCALED_SERVED = ("euclidean", "dtw")
+_REFERENCE_ONLY = ("lcss", "edr", "twe", "msm", "erp", "manhattan", "minkowski", "chebyshev",
+                   "cosine", "angular")
+
+
+def _scaled_name(metric, scale):
+    """The base metric of a scaled request (``_infer_scaled_metric``,
+    ``_distance.py:208-228``, applied when ``scale`` is set or the name starts
+    with ``scaled_``), or None for an unscaled one."""
+    if callable(metric):
+        if scale:
+            raise NotImplementedError("scaled callable metrics are not served by this library")
+        return None
+    if not isinstance(metric, str) or not (scale or metric.startswith("scaled_")):
+        return None
+    base = metric[len("scaled_"):] if metric.startswith("scaled_") else metric
+    if base not in _SCALED_SERVED:
+        if base in METRICS or base in _REFERENCE_ONLY:
+            raise NotImplementedError("metric 'scaled_%s' is not served by this library" % base)
+        raise ValueError("unsupported metric 'scaled_{}'".format(base))
+    return base
+
+
+def _pattern_stats(y):
+    """Mean and std of each pattern as ``ScaledSubsequenceMetric.from_array``
+    computes them (``_cdistance.pyx:453-467``): ``np.mean``, ``np.std``, and a
+    std of at most 1e-13 stored as 0."""
+    mean = np.array([np.mean(s) for s in y], dtype=np.float64)
+    std = np.array([np.std(s) for s in y], dtype=np.float64)
+    std[std <= 1e-13] = 0.0
+    return mean, std
+
+
+def _scaled_pairwise_subsequence(y, x, base, metric_params, dim, return_index, precision):
+    """``pairwise_subsequence_distance`` with ``ScaledEuclideanSubsequenceMetric``
+    (``_metric.pyx:96-160``) or ``ScaledDtwSubsequenceMetric``
+    (``_elastic.pyx:1928-2203``)."""
+    params = metric_params if metric_params is not None else {}
+    if metric_params is not None and not isinstance(metric_params, dict):
+        raise TypeError("The 'metric_params' parameter must be None or a dict.")
+    metric_ = check_metric(base)(**params)
+    prec = _precision(precision)
+    x_ = _ts(x)
+    if not (isinstance(dim, numbers.Integral) and 0 <= dim < x_.shape[1]):
+        raise ValueError("The parameter dim must be 0 <= dim < n_dims")
+    for s in y:
+        if np.isnan(s).any() or np.isinf(s).any():
+            raise ValueError("Input contains NaN or infinity.")
+    lens = np.array([s.shape[0] for s in y], dtype=np.int64)
+    offs = np.zeros(len(y), dtype=np.int64)
+    if len(y) > 1:
+        offs[1:] = np.cumsum(lens)[:-1]
+    S = np.ascontiguousarray(np.concatenate(y).astype(np.float64))
+    mean, std = _pattern_stats(y)
+    dist = np.empty((x_.shape[0], len(y)))
+    idx = np.empty((x_.shape[0], len(y)), dtype=np.intp)
+    _lib.check(_lib.load().wbx_scaled_subsequence_pairwise(
+        metric_.struct(prec), S.ctypes.data, offs.ctypes.data, lens.ctypes.data,
+        mean.ctypes.data, std.ctypes.data, len(y), x_.ctypes.data, x_.shape[0], x_.shape[1],
+        x_.shape[2], int(dim), dist.ctypes.data, idx.ctypes.data))
     if return_index:
         return _format_return(dist, len(y), x.ndim), _format_return(idx, len(y), x.ndim)
     return _format_return(dist, len(y), x.ndim)
