@@ -342,7 +342,29 @@ def run_fp64(torch, dev):
     got = device.pairwise(A[:4], Bm, metric="dtw", r=0.1, precision="fp64").cpu().numpy()
     want = oracle.pairwise(Xh[:4, :256], Xh[1000:, :256], metric="dtw", r=0.1)
     res["pairwise_dtw_t256_r0.1"]["mismatch_4_rows"] = int((got != want).sum())
+    # 5. z-normalised scan (scale=True, the UCR-suite cascade reproduced):
+    # 64 patterns, L U[16, 128], against 1000 series; the block baseline is
+    # the scratch-row kernel
+    Ls = rng.integers(16, 129, 64)
+    spats = [Xh[int(rng.integers(2000)), a:a + l] * 1.7 + 0.3
+             for a, l in zip(rng.integers(0, 1024 - 128, 64), Ls)]
+    sp = [torch.from_numpy(p.copy()) for p in spats]
+    Xs = X[:1000]
+    cells = 1000.0 * sum((1025 - int(l)) * square_band_cells(int(l), 0.1) for l in Ls)
+    both("scaled_dtw_subsequence_64x1000",
+         lambda: device.scaled_subsequence(sp, Xs, metric="dtw", r=0.1), cells)
+    gd, gi = device.scaled_subsequence(sp, Xs[:3], metric="dtw", r=0.1)
+    od, oi = oracle.scaled_subsequence(spats, Xh[:3], "dtw", r=0.1)
+    res["scaled_dtw_subsequence_64x1000"]["mismatch_3_rows"] = int(
+        (gd.cpu().numpy() != od).sum() + (gi.cpu().numpy() != oi).sum())
     return res
+
+
+def square_band_cells(L, r):
+    """Band cells of the scaled scan's DTW: |i - j| <= w on an L x L matrix,
+    w = _compute_warp_width(L, r) (distance/_elastic.pyx:1917-1921)."""
+    w = L - 1 if r == 1 else int(np.floor(L * r))
+    return sum(min(L - 1, i + w) - max(0, i - w) + 1 for i in range(L))
 
 
 # ------------------------------------------------------------------------ main

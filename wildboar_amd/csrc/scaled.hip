@@ -10,7 +10,8 @@
 // std of a running scan, ex += x[k], ex2 += x[k]^2 and, after window I,
 // ex -= x[I], ex2 -= x[I]^2; mean = ex / L, std = sqrt(ex2 / L - mean^2)
 // (1.0 when that is <= 0).  The rounding of those sums depends on every
-// earlier step, so k_window_stats replays the scan, one thread per series.
+// earlier step, so k_window_stats_all replays the scan, one thread per
+// (series, pattern length).
 //
 // Cascade.  Per window the reference computes, in this order, LB_Kim
 // (constant_lower_bound :158-226, partial sums returned once they reach the
@@ -51,36 +52,6 @@ __device__ __forceinline__ double dmax(double a, double b) { return b > a ? b : 
 __device__ __forceinline__ double sqd(double a, double b) {
     const double s = a - b;
     return s * s;
-}
-
-// One thread per series: mean / std of windows [0, t - L], row stride t-L+1.
-__global__ __launch_bounds__(64) void k_window_stats(const double *__restrict__ X, int64_t n,
-                                                     int64_t stride, int t, int L,
-                                                     double *__restrict__ mean,
-                                                     double *__restrict__ sdev) {
-    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (i >= n) return;
-    const double *__restrict__ x = X + i * stride;
-    const int nwin = t - L + 1;
-    double *__restrict__ mo = mean + i * nwin;
-    double *__restrict__ so = sdev + i * nwin;
-    const double dl = (double)L;
-    double ex = 0.0, ex2 = 0.0;
-    for (int k = 0; k < t; ++k) {
-        const double v = x[k];
-        ex += v;
-        ex2 += v * v;
-        if (k >= L - 1) {
-            const int I = k - (L - 1);
-            const double mu = ex / dl;
-            const double tmp = ex2 / dl - mu * mu;
-            mo[I] = mu;
-            so[I] = tmp > 0.0 ? sqrt(tmp) : 1.0;
-            const double u = x[I];
-            ex -= u;
-            ex2 -= u * u;
-        }
-    }
 }
 
 // One block per pattern: zs = (S - mean) / std (std 0 read as 1.0,
@@ -188,20 +159,21 @@ __global__ __launch_bounds__(64) void k_scaled_euclid(ScaledTask a) {
     }
 }
 
-// LB_Kim of the reference (constant_lower_bound), in full; L >= 3.
-__device__ double scaled_kim(const double *__restrict__ z, const double *__restrict__ xw,
-                             double mu, double sd, int L) {
-    const double tx0 = (xw[0] - mu) / sd, ty0 = (xw[L - 1] - mu) / sd;
+// LB_Kim of the reference (constant_lower_bound), in full, on the
+// normalised window zx[k * 64] (the same quotients the reference forms);
+// L >= 3.
+__device__ double scaled_kim(const double *__restrict__ z, const double *__restrict__ zx, int L) {
+    const double tx0 = zx[0], ty0 = zx[(int64_t)(L - 1) * 64];
     const double sx0 = z[0], sy0 = z[L - 1];
     double d = sqd(tx0, sx0) + sqd(ty0, sy0);
-    const double tx1 = (xw[1] - mu) / sd, sx1 = z[1];
+    const double tx1 = zx[64], sx1 = z[1];
     d += dmin(dmin(sqd(tx1, sx0), sqd(tx0, sx1)), sqd(tx1, sx1));
-    const double ty1 = (xw[L - 2] - mu) / sd, sy1 = z[L - 2];
+    const double ty1 = zx[(int64_t)(L - 2) * 64], sy1 = z[L - 2];
     d += dmin(dmin(sqd(ty1, sy1), sqd(ty0, sy1)), sqd(ty1, sy1));
-    const double tx2 = (xw[2] - mu) / sd, sx2 = z[2];
+    const double tx2 = zx[128], sx2 = z[2];
     d += dmin(dmin(dmin(sqd(tx0, sx2), dmin(sqd(tx1, sx2), sqd(tx2, sx2))), sqd(tx2, sx1)),
               sqd(tx2, sx0));
-    const double ty2 = (xw[L - 3] - mu) / sd, sy2 = z[L - 3];
+    const double ty2 = zx[(int64_t)(L - 3) * 64], sy2 = z[L - 3];
     d += dmin(dmin(dmin(sqd(ty0, sy2), dmin(sqd(ty1, sy2), sqd(ty2, sy2))), sqd(ty2, sy1)),
               sqd(ty2, sy0));
     return d;
@@ -217,18 +189,17 @@ __device__ __forceinline__ double keogh_term(double x, double ls, double us) {
 // cb[m * 64] for m in [0, nc), nc = L - rw - 1, the suffix sum
 // cb[rw + 1 + m] of the chosen LB_Keogh terms (cb_1 when LK > LK2, else
 // cb_2), built from the last position backwards as the reference does.
-// zx (nullable): receives the normalised window at zx[k * 64].
+// zx receives the normalised window at zx[k * 64].
 __device__ double scaled_bounds(const ScaledTask &a, const double *__restrict__ z,
                                 const double *__restrict__ zl, const double *__restrict__ zu,
                                 const double *__restrict__ xw, const double *__restrict__ el,
                                 const double *__restrict__ eu, double mu, double sd,
                                 double *__restrict__ cb, double *__restrict__ zx) {
     const int L = a.L, rw = a.rw, nc = L - rw - 1;
-    double T = L >= 3 ? scaled_kim(z, xw, mu, sd, L) : 0.0;
     double lk = 0.0, lk2 = 0.0;
     for (int k = 0; k < L; ++k) {
         const double x = (xw[k] - mu) / sd;
-        if (zx) zx[(int64_t)k * 64] = x;
+        zx[(int64_t)k * 64] = x;
         const double d = keogh_term(x, zl[k], zu[k]);
         lk += d;
         if (k > rw) cb[(int64_t)(k - rw - 1) * 64] = d;
@@ -248,8 +219,8 @@ __device__ double scaled_bounds(const ScaledTask &a, const double *__restrict__ 
         acc = k == L - 1 ? v : acc + v;
         cb[(int64_t)m * 64] = acc;
     }
-    T = dmax(T, lk);
-    return dmax(T, lk2);
+    const double T = L >= 3 ? scaled_kim(z, zx, L) : 0.0;
+    return dmax(dmax(T, lk), lk2);
 }
 
 // Replays the reference's acceptance over the 64 windows b0 + l in order.
@@ -264,10 +235,10 @@ __device__ __forceinline__ void accept_block(double T, double D, int b0, int nwi
     }
 }
 
-__device__ __forceinline__ double ldz(const double *__restrict__ xw, int j, int L, double mu,
-                                      double sd) {
+// Normalised window value of column j (+inf outside the window).
+__device__ __forceinline__ double ldz(const double *__restrict__ zx, int j, int L) {
     const int jc = j < 0 ? 0 : (j >= L ? L - 1 : j);
-    const double v = (xw[jc] - mu) / sd;
+    const double v = zx[(int64_t)jc * 64];
     return (unsigned)j < (unsigned)L ? v : WBX_INF_D;
 }
 
@@ -300,9 +271,9 @@ __device__ __forceinline__ double srow(double (&row)[BP], const double *yw, doub
 // i + rw < L - 1 of rowmin_i + cb[i * 64].
 template <int BP, int U>
 __device__ __forceinline__ void scaled_cost(const double *__restrict__ z, int L,
-                                            const double *__restrict__ xw, double mu, double sd,
-                                            int rw, int B, const double *__restrict__ cb,
-                                            double &D, double &M) {
+                                            const double *__restrict__ zx, int rw, int B,
+                                            const double *__restrict__ cb, double &D,
+                                            double &M) {
     constexpr int W = BP + U - 1;
     const int off = rw;
     double row[BP];
@@ -313,28 +284,28 @@ __device__ __forceinline__ void scaled_cost(const double *__restrict__ z, int L,
     for (int q = 0; q < 8; ++q) pen[q] = (BP - 8 + q >= B) ? WBX_INF_D : -WBX_INF_D;
     double yw[W];
 #pragma unroll
-    for (int m = 0; m < W; ++m) yw[m] = ldz(xw, m - off, L, mu, sd);
+    for (int m = 0; m < W; ++m) yw[m] = ldz(zx, m - off, L);
     const int last = L - 1 - rw;  // rows i < last are checked against cb
     M = -WBX_INF_D;
     double rm = srow<BP>(row, yw, z[0], pen);
     if (0 < last) M = dmax(M, rm + cb[0]);
 #pragma unroll
     for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
-    yw[W - 1] = ldz(xw, 1 - off + W - 1, L, mu, sd);
+    yw[W - 1] = ldz(zx, 1 - off + W - 1, L);
     const int rem = (L - 1) % U;
     for (int i = 1; i <= rem; ++i) {
         rm = srow<BP>(row, yw, z[i], pen);
         if (i < last) M = dmax(M, rm + cb[(int64_t)i * 64]);
 #pragma unroll
         for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
-        yw[W - 1] = ldz(xw, i + 1 - off + W - 1, L, mu, sd);
+        yw[W - 1] = ldz(zx, i + 1 - off + W - 1, L);
     }
     for (int i0 = 1 + rem; i0 < L; i0 += U) {
         double zv[U], yn[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             zv[u] = z[i0 + u];
-            yn[u] = ldz(xw, i0 + U - off + W - U + u, L, mu, sd);
+            yn[u] = ldz(zx, i0 + U - off + W - U + u, L);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -353,49 +324,58 @@ __device__ __forceinline__ void scaled_cost(const double *__restrict__ z, int L,
     D = res;
 }
 
-// Scaled DTW, band in registers (2 rw + 1 <= BP).  One wave per (series,
-// pattern) task, tasks strided over a fixed grid; block b owns the scratch
-// columns scratch[b * per_lane * 64 + m * 64 + lane].
+// Scaled DTW work items: (task q, block of 64 windows) flattened over a
+// grid of persistent waves; block b owns the scratch columns
+// scratch[b * per_lane * 64 + m * 64 + lane].  Each lane writes T and D of
+// its window to Tb / Db [q * nwin + window]; k_scaled_accept replays the
+// acceptance rule per task afterwards.
+struct ScaledItems {
+    int64_t n_items, nblk;
+    double *Tb, *Db;
+};
+
+__device__ __forceinline__ void item_task(const ScaledTask &a, const ScaledItems &it, int64_t item,
+                                          int64_t &q, int &b0, int64_t &ii, int64_t &pj) {
+    q = item / it.nblk;
+    b0 = (int)(item - q * it.nblk) * 64;
+    ii = q / a.gc;
+    pj = a.order[q - ii * a.gc];
+}
+
+// Band in registers (2 rw + 1 <= BP).  Scratch per lane: the suffix bounds
+// (2 nc values) and the normalised window (L values).
 template <int BP, int U>
-__global__ __launch_bounds__(64) void k_scaled_dtw(ScaledTask a, double *__restrict__ scratch,
-                                                   int64_t per_lane) {
+__global__ __launch_bounds__(64) void k_scaled_dtw(ScaledTask a, ScaledItems it,
+                                                   double *__restrict__ scratch, int64_t per_lane) {
     double *__restrict__ cb = scratch + (int64_t)blockIdx.x * per_lane * 64 + threadIdx.x;
     const int L = a.L, nwin = a.t - L + 1;
-    for (int64_t q = blockIdx.x; q < a.cn * a.gc; q += gridDim.x) {
-        const int64_t ii = q / a.gc, jj = q - ii * a.gc;
-        const int64_t pj = a.order[jj];
+    double *__restrict__ zx = cb + (int64_t)2 * (L - a.rw - 1) * 64;
+    for (int64_t item = blockIdx.x; item < it.n_items; item += gridDim.x) {
+        int64_t q, ii, pj;
+        int b0;
+        item_task(a, it, item, q, b0, ii, pj);
         const int64_t so = a.s_off[pj];
-        const double *__restrict__ x = a.X + ii * a.xstride;
-        const double *__restrict__ el = a.elo + ii * a.t;
-        const double *__restrict__ eu = a.eup + ii * a.t;
-        const double *__restrict__ mrow = a.wmean + ii * nwin;
-        const double *__restrict__ srw = a.wstd + ii * nwin;
-        double best = WBX_INF_D;
-        int bi = -1;
-        for (int b0 = 0; b0 < nwin; b0 += 64) {
-            const int s = b0 + (int)threadIdx.x;
-            const int sc = s < nwin ? s : nwin - 1;
-            const double mu = mrow[sc], sd = srw[sc];
-            double T = scaled_bounds(a, a.zs + so, a.zl + so, a.zu + so, x + sc, el + sc, eu + sc,
-                                     mu, sd, cb, nullptr);
-            double D, M;
-            scaled_cost<BP, U>(a.zs + so, L, x + sc, mu, sd, a.rw, a.B, cb, D, M);
-            T = dmax(dmax(T, M), D);
-            accept_block(T, D, b0, nwin, best, bi);
-        }
-        if (threadIdx.x == 0) {
-            const int64_t o = (a.c0 + ii) * a.n_s + pj;
-            a.dist[o] = sqrt(best);
-            a.idx[o] = bi >= 0 ? bi : 0;
+        const int s = b0 + (int)threadIdx.x;
+        const int sc = s < nwin ? s : nwin - 1;
+        const double mu = a.wmean[ii * nwin + sc], sd = a.wstd[ii * nwin + sc];
+        double T = scaled_bounds(a, a.zs + so, a.zl + so, a.zu + so, a.X + ii * a.xstride + sc,
+                                 a.elo + ii * a.t + sc, a.eup + ii * a.t + sc, mu, sd, cb, zx);
+        double D, M;
+        scaled_cost<BP, U>(a.zs + so, L, zx, a.rw, a.B, cb, D, M);
+        T = dmax(dmax(T, M), D);
+        if (s < nwin) {
+            it.Tb[q * nwin + s] = T;
+            it.Db[q * nwin + s] = D;
         }
     }
 }
 
-// Scaled DTW for any band: the normalised window and one cost row per lane
-// live in scratch (columns j at zx[j * 64], row[j * 64]); the row is updated
-// in place left to right with the previous row's value of column j - 1 kept
-// as the diagonal.  Columns a row has not reached yet hold +inf.
-__global__ __launch_bounds__(64) void k_scaled_dtw_wide(ScaledTask a, double *__restrict__ scratch,
+// Any band: the normalised window and one cost row per lane live in scratch
+// (columns j at zx[j * 64], row[j * 64]); the row is updated in place left
+// to right with the previous row's value of column j - 1 kept as the
+// diagonal.  Columns a row has not reached yet hold +inf.
+__global__ __launch_bounds__(64) void k_scaled_dtw_wide(ScaledTask a, ScaledItems it,
+                                                        double *__restrict__ scratch,
                                                         int64_t per_lane) {
     double *__restrict__ base = scratch + (int64_t)blockIdx.x * per_lane * 64 + threadIdx.x;
     const int L = a.L, rw = a.rw, nwin = a.t - L + 1;
@@ -403,52 +383,100 @@ __global__ __launch_bounds__(64) void k_scaled_dtw_wide(ScaledTask a, double *__
     double *__restrict__ cb = base;
     double *__restrict__ zx = base + (int64_t)2 * nc * 64;
     double *__restrict__ row = zx + (int64_t)L * 64;
-    for (int64_t q = blockIdx.x; q < a.cn * a.gc; q += gridDim.x) {
-        const int64_t ii = q / a.gc, jj = q - ii * a.gc;
-        const int64_t pj = a.order[jj];
+    for (int64_t item = blockIdx.x; item < it.n_items; item += gridDim.x) {
+        int64_t q, ii, pj;
+        int b0;
+        item_task(a, it, item, q, b0, ii, pj);
         const int64_t so = a.s_off[pj];
         const double *__restrict__ z = a.zs + so;
-        const double *__restrict__ x = a.X + ii * a.xstride;
-        const double *__restrict__ el = a.elo + ii * a.t;
-        const double *__restrict__ eu = a.eup + ii * a.t;
-        const double *__restrict__ mrow = a.wmean + ii * nwin;
-        const double *__restrict__ srw = a.wstd + ii * nwin;
-        double best = WBX_INF_D;
-        int bi = -1;
-        for (int b0 = 0; b0 < nwin; b0 += 64) {
-            const int s = b0 + (int)threadIdx.x;
-            const int sc = s < nwin ? s : nwin - 1;
-            const double mu = mrow[sc], sd = srw[sc];
-            double T = scaled_bounds(a, z, a.zl + so, a.zu + so, x + sc, el + sc, eu + sc, mu, sd,
-                                     cb, zx);
-            for (int j = 0; j < L; ++j) row[(int64_t)j * 64] = WBX_INF_D;
-            double M = -WBX_INF_D;
-            for (int i = 0; i < L; ++i) {
-                const double xi = z[i];
-                const int j0 = i - rw > 0 ? i - rw : 0;
-                const int j1 = i + rw < L - 1 ? i + rw : L - 1;
-                double left = WBX_INF_D;
-                double diag = i == 0 ? 0.0 : (j0 > 0 ? row[(int64_t)(j0 - 1) * 64] : WBX_INF_D);
-                double rm = WBX_INF_D;
-                for (int j = j0; j <= j1; ++j) {
-                    const double up = row[(int64_t)j * 64];
-                    const double v = xi - zx[(int64_t)j * 64];
-                    const double c = dmin(dmin(up, left), diag) + v * v;
-                    diag = up;
-                    row[(int64_t)j * 64] = c;
-                    left = c;
-                    rm = dmin(rm, c);
-                }
-                if (i < nc) M = dmax(M, rm + cb[(int64_t)i * 64]);
+        const int s = b0 + (int)threadIdx.x;
+        const int sc = s < nwin ? s : nwin - 1;
+        const double mu = a.wmean[ii * nwin + sc], sd = a.wstd[ii * nwin + sc];
+        double T = scaled_bounds(a, z, a.zl + so, a.zu + so, a.X + ii * a.xstride + sc,
+                                 a.elo + ii * a.t + sc, a.eup + ii * a.t + sc, mu, sd, cb, zx);
+        for (int j = 0; j < L; ++j) row[(int64_t)j * 64] = WBX_INF_D;
+        double M = -WBX_INF_D;
+        for (int i = 0; i < L; ++i) {
+            const double xi = z[i];
+            const int j0 = i - rw > 0 ? i - rw : 0;
+            const int j1 = i + rw < L - 1 ? i + rw : L - 1;
+            double left = WBX_INF_D;
+            double diag = i == 0 ? 0.0 : (j0 > 0 ? row[(int64_t)(j0 - 1) * 64] : WBX_INF_D);
+            double rm = WBX_INF_D;
+            for (int j = j0; j <= j1; ++j) {
+                const double up = row[(int64_t)j * 64];
+                const double v = xi - zx[(int64_t)j * 64];
+                const double c = dmin(dmin(up, left), diag) + v * v;
+                diag = up;
+                row[(int64_t)j * 64] = c;
+                left = c;
+                rm = dmin(rm, c);
             }
-            const double D = row[(int64_t)(L - 1) * 64];
-            T = dmax(dmax(T, M), D);
-            accept_block(T, D, b0, nwin, best, bi);
+            if (i < nc) M = dmax(M, rm + cb[(int64_t)i * 64]);
         }
-        if (threadIdx.x == 0) {
-            const int64_t o = (a.c0 + ii) * a.n_s + pj;
-            a.dist[o] = sqrt(best);
-            a.idx[o] = bi >= 0 ? bi : 0;
+        const double D = row[(int64_t)(L - 1) * 64];
+        T = dmax(dmax(T, M), D);
+        if (s < nwin) {
+            it.Tb[q * nwin + s] = T;
+            it.Db[q * nwin + s] = D;
+        }
+    }
+}
+
+// One wave per task: the windows' (T, D) in increasing order, 64 at a time.
+__global__ __launch_bounds__(64) void k_scaled_accept(ScaledTask a, ScaledItems it) {
+    const int64_t q = task_id();
+    if (q >= a.cn * a.gc) return;
+    const int64_t ii = q / a.gc;
+    const int64_t pj = a.order[q - ii * a.gc];
+    const int nwin = a.t - a.L + 1;
+    double best = WBX_INF_D;
+    int bi = -1;
+    for (int b0 = 0; b0 < nwin; b0 += 64) {
+        const int s = b0 + (int)threadIdx.x;
+        const int sc = s < nwin ? s : nwin - 1;
+        const double T = it.Tb[q * nwin + sc], D = it.Db[q * nwin + sc];
+        accept_block(T, D, b0, nwin, best, bi);
+    }
+    if (threadIdx.x == 0) {
+        const int64_t o = (a.c0 + ii) * a.n_s + pj;
+        a.dist[o] = sqrt(best);
+        a.idx[o] = bi >= 0 ? bi : 0;
+    }
+}
+
+// Window statistics of every pattern length at once: thread (g, i) replays
+// the scan of series i for length lens[g] into mean / sdev + goff[g]
+// (rows of t - lens[g] + 1).
+__global__ __launch_bounds__(64) void k_window_stats_all(const double *__restrict__ X, int64_t cn,
+                                                         int64_t stride, int t, int n_groups,
+                                                         const int *__restrict__ lens,
+                                                         const int64_t *__restrict__ goff,
+                                                         double *__restrict__ mean,
+                                                         double *__restrict__ sdev) {
+    const int64_t id = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (id >= cn * n_groups) return;
+    const int64_t g = id / cn, i = id - g * cn;
+    const int L = lens[g];
+    const int nwin = t - L + 1;
+    const double *__restrict__ x = X + i * stride;
+    double *__restrict__ mo = mean + goff[g] + i * nwin;
+    double *__restrict__ so = sdev + goff[g] + i * nwin;
+    const double dl = (double)L;
+    double ex = 0.0, ex2 = 0.0;
+    for (int k = 0; k < t; ++k) {
+        const double v = x[k];
+        ex += v;
+        ex2 += v * v;
+        if (k >= L - 1) {
+            const int I = k - (L - 1);
+            const double mu = ex / dl;
+            const double tmp = ex2 / dl - mu * mu;
+            mo[I] = mu;
+            so[I] = tmp > 0.0 ? sqrt(tmp) : 1.0;
+            const double u = x[I];
+            ex -= u;
+            ex2 -= u * u;
         }
     }
 }
@@ -462,19 +490,56 @@ inline dim3 task_grid(int64_t nq) {
     return dim3((unsigned)gx, (unsigned)((nq + gx - 1) / gx));
 }
 
-// Scratch doubles per lane (wide kernel; the register kernel uses the first
-// 2 (L - 1)).
-inline int64_t per_lane_doubles(int64_t max_len) { return 4 * max_len; }
-constexpr size_t kScratchCap = (size_t)256 << 20;
-constexpr size_t kChunkCap = (size_t)512 << 20;
+// Scratch doubles per lane for a pattern length: 2 (L - w - 1) suffix-bound
+// values and L normalised window values, plus L row values for the wide
+// kernel.  At most 4 L.
+inline int64_t per_lane_doubles(int64_t L, int rw, bool wide) {
+    return 2 * (L - rw - 1) + L + (wide ? L : 0);
+}
+constexpr size_t kScratchCap = (size_t)512 << 20;  // DTW scratch rows
+constexpr size_t kChunkCap = (size_t)512 << 20;    // window statistics + envelopes
+constexpr size_t kTDCap = (size_t)256 << 20;       // per-window (T, D)
+constexpr int64_t kMaxWaves = 8192;
 
-int64_t scratch_waves(int64_t max_len) {
-    const int64_t w = (int64_t)(kScratchCap / ((size_t)per_lane_doubles(max_len) * 64 * 8));
-    return w < 1 ? 1 : (w > 2048 ? 2048 : w);
+size_t scratch_bytes(int64_t max_len) {
+    const size_t one = (size_t)64 * 8 * (size_t)(4 * max_len);  // at least one wave
+    const size_t want = (size_t)kMaxWaves * one;
+    return want < kScratchCap ? want : (one > kScratchCap ? one : kScratchCap);
 }
 
-int64_t chunk_rows(int kind, int64_t n, int64_t t, int64_t min_len) {
-    const int64_t per = (t - min_len + 1) * 2 * 8 + (kind == WBX_DTW ? t * 2 * 8 : 0);
+int64_t group_waves(int64_t n_items, size_t bytes, int64_t per_lane) {
+    int64_t w = (int64_t)(bytes / ((size_t)per_lane * 64 * 8));
+    w = w < 1 ? 1 : (w > kMaxWaves ? kMaxWaves : w);
+    return n_items < w ? n_items : w;
+}
+
+struct Groups {
+    std::vector<int64_t> order;             // pattern indices sorted by length
+    std::vector<int64_t> start;             // group g = order[start[g] .. start[g + 1])
+    std::vector<int> len;                   // length of group g
+    int64_t win_total = 0;                  // sum over groups of t - len + 1
+};
+
+Groups make_groups(const int64_t *hl, int64_t n_s, int64_t t) {
+    Groups G;
+    G.order.resize((size_t)n_s);
+    for (int64_t j = 0; j < n_s; ++j) G.order[j] = j;
+    std::stable_sort(G.order.begin(), G.order.end(),
+                     [&](int64_t a, int64_t b) { return hl[a] < hl[b]; });
+    for (int64_t k = 0; k < n_s; ++k) {
+        const int L = (int)hl[G.order[k]];
+        if (k == 0 || L != G.len.back()) {
+            G.start.push_back(k);
+            G.len.push_back(L);
+            G.win_total += t - L + 1;
+        }
+    }
+    G.start.push_back(n_s);
+    return G;
+}
+
+int64_t chunk_rows(int kind, int64_t n, int64_t t, int64_t win_total) {
+    const int64_t per = win_total * 16 + (kind == WBX_DTW ? t * 16 : 0);
     int64_t c = (int64_t)(kChunkCap / (size_t)per);
     if (c < 1) c = 1;
     return c < n ? c : n;
@@ -484,20 +549,21 @@ int64_t chunk_rows(int kind, int64_t n, int64_t t, int64_t min_len) {
 
 size_t scaled_ws_bytes(int kind, const int64_t *hl, int64_t n_s, int64_t total, int64_t n,
                        int64_t t) {
-    int64_t min_len = t, max_len = 1;
-    for (int64_t j = 0; j < n_s; ++j) {
-        min_len = hl[j] < min_len ? hl[j] : min_len;
-        max_len = hl[j] > max_len ? hl[j] : max_len;
-    }
-    const int64_t cn = chunk_rows(kind, n, t, min_len);
+    const Groups G = make_groups(hl, n_s, t);
+    int64_t max_len = G.len.back();
+    const int64_t cn = chunk_rows(kind, n, t, G.win_total);
     size_t b = 0;
     auto take = [&](size_t bytes) { b += (bytes + 255) & ~(size_t)255; };
-    take((size_t)total * 8 * 3);                         // zs, zl, zu
-    take((size_t)n_s * 8);                               // order
-    take((size_t)cn * (size_t)(t - min_len + 1) * 16);   // window mean / std
+    take((size_t)total * 8);                                // zs
+    take((size_t)total * 8);                                // zl
+    take((size_t)total * 8);                                // zu
+    take((size_t)n_s * 8);                                  // order
+    take(G.len.size() * 16);                                // group offsets, lengths
+    take((size_t)cn * (size_t)G.win_total * 16);            // window mean / std
     if (kind == WBX_DTW) {
-        take((size_t)cn * (size_t)t * 16);               // series envelopes
-        take((size_t)scratch_waves(max_len) * 64 * (size_t)per_lane_doubles(max_len) * 8);
+        take((size_t)cn * (size_t)t * 16);                  // series envelopes
+        take(scratch_bytes(max_len));
+        take(kTDCap > (size_t)cn * (size_t)t * 16 ? kTDCap : (size_t)cn * (size_t)t * 16);
     }
     return b;
 }
@@ -508,20 +574,17 @@ int launch_scaled_subseq(int kind, double r, const double *S, const int64_t *s_o
                          int64_t xstride, int64_t n, int64_t t, double *dist, int64_t *idx,
                          char *ws, bool wide_only, hipStream_t st) {
     if (n <= 0 || n_s <= 0) return WBX_OK;
-    if (t > 0x7fffffff || total > 0x7fffffffLL * 4)
-        return fail(WBX_EUNSUPPORTED, "scaled scan: series too long");
+    if (t > 0x7fffffff) return fail(WBX_EUNSUPPORTED, "scaled scan: series too long");
     const bool dtw = kind == WBX_DTW;
-    // patterns sorted by length (stable): groups share window statistics
-    std::vector<int64_t> order((size_t)n_s);
-    for (int64_t j = 0; j < n_s; ++j) order[j] = j;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int64_t a, int64_t b) { return hl[a] < hl[b]; });
-    int64_t min_len = t, max_len = 1;
-    for (int64_t j = 0; j < n_s; ++j) {
-        min_len = hl[j] < min_len ? hl[j] : min_len;
-        max_len = hl[j] > max_len ? hl[j] : max_len;
+    const Groups G = make_groups(hl, n_s, t);
+    const int n_groups = (int)G.len.size();
+    const int64_t max_len = G.len.back();
+    const int64_t cn_max = chunk_rows(kind, n, t, G.win_total);
+    std::vector<int64_t> goff((size_t)n_groups);
+    for (int64_t g = 0, w = 0; g < n_groups; ++g) {
+        goff[g] = w * cn_max;
+        w += t - G.len[g] + 1;
     }
-    const int64_t cn_max = chunk_rows(kind, n, t, min_len);
     size_t o = 0;
     auto take = [&](size_t bytes) {
         char *p = ws + o;
@@ -532,33 +595,42 @@ int launch_scaled_subseq(int kind, double r, const double *S, const int64_t *s_o
     double *zl = (double *)take((size_t)total * 8);
     double *zu = (double *)take((size_t)total * 8);
     int64_t *d_order = (int64_t *)take((size_t)n_s * 8);
-    double *wmean = (double *)take((size_t)cn_max * (size_t)(t - min_len + 1) * 16);
-    double *wstd = wmean + cn_max * (t - min_len + 1);
-    double *elo = nullptr, *eup = nullptr, *scratch = nullptr;
-    const int64_t waves = scratch_waves(max_len);
+    char *gmeta = take((size_t)n_groups * 16);
+    int64_t *d_goff = (int64_t *)gmeta;
+    int *d_glen = (int *)(gmeta + (size_t)n_groups * 8);
+    double *wmean = (double *)take((size_t)cn_max * (size_t)G.win_total * 16);
+    double *wstd = wmean + cn_max * G.win_total;
+    double *elo = nullptr, *eup = nullptr, *scratch = nullptr, *Tb = nullptr;
+    const size_t sbytes = scratch_bytes(max_len);
+    size_t td_bytes = 0;
     if (dtw) {
         elo = (double *)take((size_t)cn_max * (size_t)t * 16);
         eup = elo + cn_max * t;
-        scratch = (double *)take((size_t)waves * 64 * (size_t)per_lane_doubles(max_len) * 8);
+        scratch = (double *)take(sbytes);
+        td_bytes = kTDCap > (size_t)cn_max * (size_t)t * 16 ? kTDCap : (size_t)cn_max * (size_t)t * 16;
+        Tb = (double *)take(td_bytes);
     }
-    WBX_HIP_CHECK(hipMemcpyAsync(d_order, order.data(), sizeof(int64_t) * n_s,
+    WBX_HIP_CHECK(hipMemcpyAsync(d_order, G.order.data(), sizeof(int64_t) * n_s,
                                  hipMemcpyHostToDevice, st));
-    WBX_HIP_CHECK(hipStreamSynchronize(st));  // `order` ends with this call
+    WBX_HIP_CHECK(hipMemcpyAsync(d_glen, G.len.data(), sizeof(int) * n_groups,
+                                 hipMemcpyHostToDevice, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(d_goff, goff.data(), sizeof(int64_t) * n_groups,
+                                 hipMemcpyHostToDevice, st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));  // the host vectors end with this call
     hipLaunchKernelGGL(k_scaled_patterns, dim3((unsigned)n_s), dim3(256), 0, st, S, s_off, s_len,
                        s_mean, s_std, r, dtw ? 1 : 0, zs, zl, zu);
     WBX_HIP_CHECK(hipGetLastError());
-    const int64_t per_lane = per_lane_doubles(max_len);
     for (int64_t c0 = 0; c0 < n; c0 += cn_max) {
         const int64_t cn = n - c0 < cn_max ? n - c0 : cn_max;
         const double *xc = X + c0 * xstride;
+        const int64_t nst = cn * n_groups;
+        hipLaunchKernelGGL(k_window_stats_all, dim3((unsigned)((nst + 63) / 64)), dim3(64), 0, st,
+                           xc, cn, xstride, (int)t, n_groups, d_glen, d_goff, wmean, wstd);
+        WBX_HIP_CHECK(hipGetLastError());
         int env_w = -1;
-        for (int64_t g0 = 0; g0 < n_s;) {
-            const int L = (int)hl[order[g0]];
-            int64_t g1 = g0 + 1;
-            while (g1 < n_s && hl[order[g1]] == L) ++g1;
-            hipLaunchKernelGGL(k_window_stats, dim3((unsigned)((cn + 63) / 64)), dim3(64), 0, st,
-                               xc, cn, xstride, (int)t, L, wmean, wstd);
-            WBX_HIP_CHECK(hipGetLastError());
+        for (int g = 0; g < n_groups; ++g) {
+            const int L = G.len[g];
+            const int nwin = (int)(t - L + 1);
             ScaledTask a{};
             a.X = xc;
             a.xstride = xstride;
@@ -566,38 +638,53 @@ int launch_scaled_subseq(int kind, double r, const double *S, const int64_t *s_o
             a.L = L;
             a.c0 = c0;
             a.cn = cn;
-            a.wmean = wmean;
-            a.wstd = wstd;
+            a.wmean = wmean + goff[g];
+            a.wstd = wstd + goff[g];
             a.zs = zs;
             a.zl = zl;
             a.zu = zu;
             a.s_off = s_off;
-            a.order = d_order + g0;
-            a.gc = g1 - g0;
             a.n_s = n_s;
             a.dist = dist;
             a.idx = idx;
-            const int64_t nq = cn * a.gc;
+            const int64_t g0 = G.start[g], g1 = G.start[g + 1];
             if (!dtw) {
-                hipLaunchKernelGGL(k_scaled_euclid, task_grid(nq), dim3(64), 0, st, a);
+                a.order = d_order + g0;
+                a.gc = g1 - g0;
+                hipLaunchKernelGGL(k_scaled_euclid, task_grid(cn * a.gc), dim3(64), 0, st, a);
                 WBX_HIP_CHECK(hipGetLastError());
-            } else {
-                const int rw = warp_width(L, r);
-                if (rw != env_w) {
-                    const int rc = launch_envelope(xc, cn, xstride, (int)t, rw, elo, eup, st);
-                    if (rc) return rc;
-                    env_w = rw;
-                }
-                a.elo = elo;
-                a.eup = eup;
-                a.rw = rw;
-                a.B = 2 * rw + 1;
-                const int BP = (a.B + 7) / 8 * 8;
-                const dim3 grid((unsigned)(nq < waves ? nq : waves));
-                if (!wide_only && BP <= kMaxBPS) {
-#define WBX_SCALED_LAUNCH(C)                                                                  \
-    case C:                                                                                   \
-        hipLaunchKernelGGL((k_scaled_dtw<C, 4>), grid, dim3(64), 0, st, a, scratch, per_lane); \
+                continue;
+            }
+            const int rw = warp_width(L, r);
+            if (rw != env_w) {
+                const int rc = launch_envelope(xc, cn, xstride, (int)t, rw, elo, eup, st);
+                if (rc) return rc;
+                env_w = rw;
+            }
+            a.elo = elo;
+            a.eup = eup;
+            a.rw = rw;
+            a.B = 2 * rw + 1;
+            const int BP = (a.B + 7) / 8 * 8;
+            const bool wide = wide_only || BP > kMaxBPS;
+            const int64_t per_lane = per_lane_doubles(L, rw, wide);
+            // patterns of the group in batches whose (T, D) fit the buffer
+            int64_t bsz = (int64_t)(td_bytes / ((size_t)cn * (size_t)nwin * 16));
+            if (bsz < 1) bsz = 1;
+            for (int64_t b0 = g0; b0 < g1; b0 += bsz) {
+                a.order = d_order + b0;
+                a.gc = g1 - b0 < bsz ? g1 - b0 : bsz;
+                ScaledItems it;
+                it.nblk = (nwin + 63) / 64;
+                it.n_items = cn * a.gc * it.nblk;
+                it.Tb = Tb;
+                it.Db = Tb + cn * a.gc * nwin;
+                const dim3 grid((unsigned)group_waves(it.n_items, sbytes, per_lane));
+                if (!wide) {
+#define WBX_SCALED_LAUNCH(C)                                                              \
+    case C:                                                                               \
+        hipLaunchKernelGGL((k_scaled_dtw<C, 4>), grid, dim3(64), 0, st, a, it, scratch, \
+                           per_lane);                                                     \
         break;
                     switch (BP) {
                         WBX_SCALED_CLASSES(WBX_SCALED_LAUNCH)
@@ -605,12 +692,13 @@ int launch_scaled_subseq(int kind, double r, const double *S, const int64_t *s_o
                     }
 #undef WBX_SCALED_LAUNCH
                 } else {
-                    hipLaunchKernelGGL(k_scaled_dtw_wide, grid, dim3(64), 0, st, a, scratch,
+                    hipLaunchKernelGGL(k_scaled_dtw_wide, grid, dim3(64), 0, st, a, it, scratch,
                                        per_lane);
                 }
                 WBX_HIP_CHECK(hipGetLastError());
+                hipLaunchKernelGGL(k_scaled_accept, task_grid(cn * a.gc), dim3(64), 0, st, a, it);
+                WBX_HIP_CHECK(hipGetLastError());
             }
-            g0 = g1;
         }
     }
     return WBX_OK;

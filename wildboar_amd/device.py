@@ -6,6 +6,7 @@ are ordered with surrounding torch work.  Inputs are (n, t) float32 or float64
 tensors whose last dimension is contiguous.  No host copies, no CPU fallback.
 """
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -134,6 +135,33 @@ def subsequence(shapelets, X, *, metric="dtw", r=1.0, g=0.05, precision="fp32"):
         _lib.check(_lib.load().wbx_dev_subsequence_pairwise(
             m, _dtype(X), packed.data_ptr(), offs.data_ptr(), lens.data_ptr(), ns, X.data_ptr(),
             n, X.stride(0), X.shape[1], dist.data_ptr(), idx.data_ptr(), _stream()))
+    return dist, idx
+
+
+def scaled_subsequence(shapelets, X, *, metric="dtw", r=1.0, precision="fp64"):
+    """pairwise_subsequence_distance(scale=True) on device tensors: (dist, idx)
+    of ScaledEuclideanSubsequenceMetric (metric="euclidean") or
+    ScaledDtwSubsequenceMetric (metric="dtw", band fraction r).  shapelets: a
+    list of 1-D tensors; their mean / std are taken on the host with numpy as
+    the reference's from_array takes them (std <= 1e-13 -> 0)."""
+    X = _rows(X, "X")
+    if not isinstance(shapelets, (list, tuple)) or not shapelets:
+        raise TypeError("shapelets must be a non-empty list of 1-D tensors")
+    host = [s.detach().reshape(-1).cpu().double().numpy() for s in shapelets]
+    mean = torch.tensor([float(np.mean(h)) for h in host], dtype=torch.float64)
+    std = torch.tensor([float(np.std(h)) for h in host], dtype=torch.float64)
+    std[std <= 1e-13] = 0.0
+    packed, offs, lens = pack_shapelets(shapelets, X.device, X.dtype)
+    mean, std = mean.to(X.device), std.to(X.device)
+    n, ns = X.shape[0], len(shapelets)
+    dist = torch.empty((n, ns), dtype=torch.float64, device=X.device)
+    idx = torch.empty((n, ns), dtype=torch.int64, device=X.device)
+    m = _metric(metric, r, 0.05, precision)
+    with torch.cuda.device(X.device):
+        _lib.check(_lib.load().wbx_dev_scaled_subsequence_pairwise(
+            m, _dtype(X), packed.data_ptr(), offs.data_ptr(), lens.data_ptr(), mean.data_ptr(),
+            std.data_ptr(), ns, X.data_ptr(), n, X.stride(0), X.shape[1], dist.data_ptr(),
+            idx.data_ptr(), _stream()))
     return dist, idx
 
 
