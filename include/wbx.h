@@ -163,6 +163,26 @@ int wbx_scaled_subsequence_pairwise(const wbx_metric *metric, const double *S,
                                     int64_t dims, int64_t t, int64_t dim,
                                     double *dist, int64_t *idx);
 
+/* The window profiles behind subsequence_match / paired_subsequence_match
+ * with scale=True (distance/_distance.py:732-1079): the windows the
+ * reference's scaled matching records at `threshold` --
+ * scaled_euclidean_distance_matches (distance/_metric.pyx:798-866) for
+ * metric->kind WBX_EUCLIDEAN, scaled_dtw_matches (distance/_elastic.pyx:
+ * 484-619, its LB_Kim / LB_Keogh cascade against threshold^2 included) for
+ * WBX_DTW.  prof (n, nwp), nwp = t - shortest pattern + 1 (pattern 0 alone
+ * when not paired), holds the squared distance recorded for window s of
+ * series i against pattern (paired ? i : 0) -- sqrt of it is the reported
+ * distance -- and +inf where nothing is recorded or past the pattern's last
+ * window.  threshold = +inf records every window (the reference's distance
+ * profile).  s_mean / s_std as for wbx_scaled_subsequence_pairwise. */
+int wbx_scaled_subsequence_profile(const wbx_metric *metric, const double *S,
+                                   const int64_t *s_off, const int64_t *s_len,
+                                   const double *s_mean, const double *s_std,
+                                   int64_t n_s, int paired, const double *X,
+                                   int64_t n, int64_t dims, int64_t t,
+                                   int64_t dim, int64_t nwp, double threshold,
+                                   double *prof);
+
 /* The per-series window costs behind _subsequence_match and
  * _paired_subsequence_match (distance/_cdistance.pyx:1065-1141, windows with
  * cost <= threshold^2 as dtw_subsequence_matches, _elastic.pyx:658-698, and
@@ -223,6 +243,18 @@ int wbx_dev_scaled_subsequence_pairwise(const wbx_metric *metric, int dtype,
                                         int64_t x_stride, int64_t t,
                                         double *dist, int64_t *idx,
                                         void *stream);
+
+/* Device form of wbx_scaled_subsequence_profile (prof device (n, nwp);
+ * synchronises `stream`). */
+int wbx_dev_scaled_subsequence_profile(const wbx_metric *metric, int dtype,
+                                       const void *S, const int64_t *s_off,
+                                       const int64_t *s_len,
+                                       const double *s_mean,
+                                       const double *s_std, int64_t n_s,
+                                       int paired, const void *X, int64_t n,
+                                       int64_t x_stride, int64_t t,
+                                       int64_t nwp, double threshold,
+                                       double *prof, void *stream);
 
 /* Device forms of wbx_argmin_subsequence (n patterns, one per series;
  * idx / dist device (n, k)) and wbx_subsequence_profile (prof device

@@ -82,6 +82,10 @@ def _load():
     lib.orc_window_stats.argtypes = [p, i64, i64, p, p]
     lib.orc_warp_width.restype = i64
     lib.orc_warp_width.argtypes = [i64, d]
+    lib.orc_scaled_euclidean_matches.restype = i64
+    lib.orc_scaled_euclidean_matches.argtypes = [p, i64, d, d, p, i64, d, p, p]
+    lib.orc_scaled_dtw_matches.restype = i64
+    lib.orc_scaled_dtw_matches.argtypes = [p, i64, d, d, p, i64, d, d, p, p]
     lib.orc_scaled_subsequence_pairwise.restype = i32
     lib.orc_scaled_subsequence_pairwise.argtypes = [i32, d, p, p, p, p, p, i64, p, i64, i64, i32,
                                                     p, p]
@@ -319,6 +323,30 @@ def scaled_subsequence(shapelets, X, metric="dtw", r=1.0, dim=0, cascade=True):
     if rc != 0:
         raise MemoryError("oracle allocation failed")
     return dist, idx
+
+
+def scaled_subsequence_matches(s, x, threshold, metric="dtw", r=1.0):
+    """(indices, distances) of the windows of the 1-D series x that the
+    reference's scaled matching records for pattern s (scaled_euclidean /
+    scaled_dtw ``_matches``), in window order."""
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    mean, std = pattern_stats([s])
+    nw = x.shape[0] - s.shape[0] + 1
+    idx = np.zeros(max(nw, 1), dtype=np.int64)
+    dist = np.zeros(max(nw, 1))
+    lib = _load()
+    if metric == "euclidean":
+        n = lib.orc_scaled_euclidean_matches(s.ctypes.data, s.shape[0], mean[0], std[0],
+                                             x.ctypes.data, x.shape[0], float(threshold),
+                                             dist.ctypes.data, idx.ctypes.data)
+    else:
+        n = lib.orc_scaled_dtw_matches(s.ctypes.data, s.shape[0], mean[0], std[0], x.ctypes.data,
+                                       x.shape[0], float(r), float(threshold), dist.ctypes.data,
+                                       idx.ctypes.data)
+    if n < 0:
+        raise MemoryError("oracle allocation failed")
+    return idx[:n].copy(), dist[:n].copy()
 
 
 def argmin_subsequence(shapelets, X, k=1, metric="dtw", r=1.0, g=0.05, dim=0):

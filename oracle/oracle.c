@@ -1207,3 +1207,84 @@ int orc_scaled_subsequence_pairwise(int kind, double r, const double *S, const i
         }
     return 0;
 }
+
+/* scaled_euclidean_distance_matches, distance/_metric.pyx:798-866: windows
+ * whose (partial-sum abandoned) squared distance is <= threshold^2, in
+ * window order; distances[k] = sqrt of it.  Returns the count. */
+int64_t orc_scaled_euclidean_matches(const double *S, int64_t L, double s_mean, double s_std,
+                                     const double *T, int64_t t, double threshold,
+                                     double *distances, int64_t *matches) {
+    int64_t nwin = t - L + 1, n_matches = 0;
+    double *mean = malloc(sizeof(double) * (size_t)nwin), *std = malloc(sizeof(double) * (size_t)nwin);
+    if (!mean || !std) { free(mean); free(std); return -1; }
+    if (s_std == 0) s_std = 1.0;
+    orc_window_stats(T, t, L, mean, std);
+    threshold = threshold * threshold;
+    for (int64_t I = 0; I < nwin; I++) {
+        double dist = 0, x;
+        for (int64_t i = 0; i < L; i++) {
+            if (dist >= threshold) break;
+            x = (S[i] - s_mean) / s_std;
+            x -= (T[I + i] - mean[I]) / std[I];
+            dist += x * x;
+        }
+        if (dist <= threshold) {
+            matches[n_matches] = I;
+            distances[n_matches] = sqrt(dist);
+            n_matches++;
+        }
+    }
+    free(mean);
+    free(std);
+    return n_matches;
+}
+
+/* scaled_dtw_matches, distance/_elastic.pyx:484-619 (as ScaledDtwSubsequence-
+ * Metric._matches, :2112-2149, calls it): the cascade against the fixed
+ * threshold^2; a window is recorded when its (possibly abandoned) DTW value
+ * is <= threshold^2. */
+int64_t orc_scaled_dtw_matches(const double *S, int64_t L, double s_mean, double s_std,
+                               const double *T, int64_t t, double rr, double threshold,
+                               double *distances, int64_t *matches) {
+    int64_t r = orc_warp_width(L, rr), nwin = t - L + 1, I, k, n_matches = -1;
+    size_t nb = sizeof(double);
+    double *mean = malloc(nb * (size_t)nwin), *std = malloc(nb * (size_t)nwin),
+           *s_lo = malloc(nb * (size_t)L), *s_up = malloc(nb * (size_t)L),
+           *t_lo = malloc(nb * (size_t)t), *t_up = malloc(nb * (size_t)t),
+           *cb = malloc(nb * (size_t)L), *cb_1 = malloc(nb * (size_t)L),
+           *cb_2 = malloc(nb * (size_t)L), *cost = malloc(nb * (size_t)(2 * r + 1)),
+           *cost_prev = malloc(nb * (size_t)(2 * r + 1));
+    if (!mean || !std || !s_lo || !s_up || !t_lo || !t_up || !cb || !cb_1 || !cb_2 || !cost ||
+        !cost_prev)
+        goto out;
+    if (s_std == 0) s_std = 1.0;
+    orc_window_stats(T, t, L, mean, std);
+    if (orc_dtw_envelop(S, L, r, s_lo, s_up) || orc_dtw_envelop(T, t, r, t_lo, t_up)) goto out;
+    threshold = threshold * threshold;
+    n_matches = 0;
+    for (I = 0; I < nwin; I++) {
+        const double *X = T + I;
+        double lb_kim = scaled_lb_kim(S, s_mean, s_std, X, mean[I], std[I], L, threshold);
+        if (!(lb_kim < threshold)) continue;
+        double lb_k = scaled_cumulative_bound(X, L, mean[I], std[I], s_mean, s_std, s_lo, s_up,
+                                              cb_1, threshold);
+        if (!(lb_k < threshold)) continue;
+        double lb_k2 = scaled_cumulative_bound(S, L, s_mean, s_std, mean[I], std[I], t_lo + I,
+                                               t_up + I, cb_2, threshold);
+        if (!(lb_k2 < threshold)) continue;
+        const double *src = lb_k > lb_k2 ? cb_1 : cb_2;
+        cb[L - 1] = src[L - 1];
+        for (k = L - 2; k >= 0; k--) cb[k] = cb[k + 1] + src[k];
+        double dist = scaled_dtw_inner(S, L, s_mean, s_std, X, mean[I], std[I], r, cb, cost,
+                                       cost_prev, threshold);
+        if (dist <= threshold) {
+            matches[n_matches] = I;
+            distances[n_matches] = sqrt(dist);
+            n_matches++;
+        }
+    }
+out:
+    free(mean); free(std); free(s_lo); free(s_up); free(t_lo); free(t_up);
+    free(cb); free(cb_1); free(cb_2); free(cost); free(cost_prev);
+    return n_matches;
+}

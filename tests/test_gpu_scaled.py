@@ -112,3 +112,49 @@ def test_scaled_float32_inputs_and_validation():
     with pytest.raises(TypeError):
         wd.pairwise_subsequence_distance(pats, X, metric="euclidean", metric_params={"r": 0.1},
                                          scale=True)
+
+
+@pytest.mark.parametrize("metric,params", [("euclidean", None), ("dtw", {"r": 0.1}),
+                                           ("dtw", {"r": 0.5}), ("dtw", {"r": 1.0})])
+@pytest.mark.parametrize("policy", [0, 1])
+def test_scaled_subsequence_match_bitwise(metric, params, policy):
+    X = rw(905, 7, 160)
+    s = X[3, 40:72] * 2.5 - 1.0
+    r = (params or {}).get("r", 1.0)
+    _lib.set_policy(policy)
+    try:
+        for threshold in (0.5, 2.0, 4.0, np.inf):
+            idx, dist = wd.subsequence_match(s, X, threshold, metric=metric, metric_params=params,
+                                             scale=True, return_distance=True)
+            for q in range(X.shape[0]):
+                oi, od = oracle.scaled_subsequence_matches(s, X[q], threshold, metric, r)
+                if oi.size == 0:
+                    assert idx[q] is None and dist[q] is None
+                else:
+                    np.testing.assert_array_equal(idx[q], oi)
+                    np.testing.assert_array_equal(dist[q], od)
+        # default: the 10 closest windows of the full profile
+        idx = wd.subsequence_match(s, X, metric=metric, metric_params=params, scale=True)
+        oi, od = oracle.scaled_subsequence_matches(s, X[0], np.inf, metric, r)
+        np.testing.assert_array_equal(idx[0], oi[np.argsort(od)[:10]])
+    finally:
+        _lib.set_policy(0)
+
+
+@pytest.mark.parametrize("metric,r", [("euclidean", 1.0), ("dtw", 0.2)])
+def test_scaled_paired_subsequence_match_bitwise(metric, r):
+    X = rw(906, 6, 120)
+    rng = np.random.default_rng(907)
+    pats = [X[q, 10 + 5 * q:10 + 5 * q + L] * 1.5 + 0.2
+            for q, L in enumerate((8, 20, 20, 33, 5, 64))]
+    pats[1] = rng.standard_normal(20)
+    params = None if metric == "euclidean" else {"r": r}
+    idx, dist = wd.paired_subsequence_match(pats, X, 3.0, metric=metric, metric_params=params,
+                                            scale=True, return_distance=True)
+    for q in range(X.shape[0]):
+        oi, od = oracle.scaled_subsequence_matches(pats[q], X[q], 3.0, metric, r)
+        if oi.size == 0:
+            assert idx[q] is None
+        else:
+            np.testing.assert_array_equal(idx[q], oi)
+            np.testing.assert_array_equal(dist[q], od)

@@ -14,7 +14,7 @@ import oracle
 from wildboar_amd import distance as wd
 
 
-def _oracle_profiles(patterns, x_, dim, metric, paired):
+def _oracle_profiles(patterns, x_, dim, metric, paired, threshold=np.inf):
     n, _, t = x_.shape
     nwp = t - min(p.shape[0] for p in patterns) + 1
     prof = np.full((n, nwp), np.inf)
@@ -157,7 +157,9 @@ def test_match_validation_matches_reference():
     with pytest.raises(TypeError):
         wd.subsequence_match(np.array([1, 2]), np.array([1, 2, 3, 4]), exclude="2")
     with pytest.raises(NotImplementedError):
-        wd.subsequence_match(np.array([1, 2]), np.array([1, 2, 3, 4]), scale=True)
+        wd.subsequence_match(np.array([1, 2]), np.array([1, 2, 3, 4]), metric="wdtw", scale=True)
+    with pytest.raises(ValueError):
+        wd.subsequence_match(np.array([1, 2]), np.array([1, 2, 3, 4]), metric="scaled_nope")
     with pytest.warns(UserWarning, match="n_jobs"):
         with pytest.raises(TypeError):
             wd.subsequence_match(np.array([1, 2]), np.array([1, 2, 3, 4]), threshold=object(),

@@ -139,3 +139,40 @@ def test_scaled_scans_are_affine_invariant_in_the_pattern():
         d2, i2 = oracle.scaled_subsequence([p * 7.5 + 3.0 for p in pats], X, metric, r=r)
         np.testing.assert_allclose(d2, d, rtol=1e-7)
         np.testing.assert_array_equal(i2, i)
+
+
+@pytest.mark.parametrize("metric,r", [("euclidean", 1.0), ("dtw", 0.1), ("dtw", 1.0)])
+def test_scaled_matches_at_infinity_are_the_full_profile(metric, r):
+    X = rw(9, 3, 150)
+    s = X[1, 30:60] * 2.0 + 5.0
+    d, i = oracle.scaled_subsequence([s], X, metric, r=r, cascade=False) if metric == "dtw" \
+        else oracle.scaled_subsequence([s], X, metric)
+    for q in range(X.shape[0]):
+        idx, dist = oracle.scaled_subsequence_matches(s, X[q], np.inf, metric, r)
+        np.testing.assert_array_equal(idx, np.arange(150 - 30 + 1))
+        # every window recorded with its full distance: the minimum is the
+        # exhaustive scan's, at its index
+        assert dist.min() == d[q, 0] and np.argmin(dist) == i[q, 0]
+        zs = zpat(s)
+        w = oracle.warp_width(30, r)
+        full = [np.sqrt(((z - zs) ** 2).sum()) if metric == "euclidean"
+                else np.sqrt(band_dtw(zs, z, w)) for z in znorm_windows(X[q], 30)]
+        np.testing.assert_allclose(dist, full, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("metric,r", [("euclidean", 1.0), ("dtw", 0.3)])
+def test_scaled_matches_at_a_threshold_are_recorded_full_distances(metric, r):
+    X = rw(10, 4, 200)
+    s = X[2, 70:100].copy()
+    for q in range(X.shape[0]):
+        all_i, all_d = oracle.scaled_subsequence_matches(s, X[q], np.inf, metric, r)
+        for th in (1.0, 2.5, 4.0):
+            idx, dist = oracle.scaled_subsequence_matches(s, X[q], th, metric, r)
+            assert (dist <= th).all()
+            np.testing.assert_array_equal(dist, all_d[idx])
+            if metric == "euclidean":
+                # no bound: exactly the windows within the threshold
+                np.testing.assert_array_equal(idx, all_i[all_d <= th])
+            else:
+                # the cascade only removes windows
+                assert set(idx.tolist()) <= set(all_i[all_d <= th].tolist())

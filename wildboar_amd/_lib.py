@@ -33,6 +33,7 @@ EXPORTS = (
     "wbx_dev_argmin_subsequence", "wbx_dev_subsequence_profile", "wbx_dev_dtw_path",
     "wbx_dev_dtw_mm_update", "wbx_dev_dtw_envelope", "wbx_dev_lb_keogh", "wbx_dev_lb_kim",
     "wbx_scaled_subsequence_pairwise", "wbx_dev_scaled_subsequence_pairwise",
+    "wbx_scaled_subsequence_profile", "wbx_dev_scaled_subsequence_profile",
 )
 
 
@@ -118,6 +119,10 @@ def load():
                                                       p, p]),
             "wbx_dev_scaled_subsequence_pairwise": (i32, [m, i32, p, p, p, p, p, i64, p, i64, i64,
                                                           i64, p, p, p]),
+            "wbx_scaled_subsequence_profile": (i32, [m, p, p, p, p, p, i64, i32, p, i64, i64, i64,
+                                                     i64, i64, d, p]),
+            "wbx_dev_scaled_subsequence_profile": (i32, [m, i32, p, p, p, p, p, i64, i32, p, i64,
+                                                         i64, i64, i64, d, p, p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)

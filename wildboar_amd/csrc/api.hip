@@ -801,6 +801,53 @@ int scaled_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
                                 g_policy.load() != 0, st);
 }
 
+// Window profiles of the reference's scaled matching (subsequence_match /
+// paired_subsequence_match with scale=True, _distance.py:732-1079 through
+// ScaledEuclideanSubsequenceMetric._matches, _metric.pyx:141-167, and
+// ScaledDtwSubsequenceMetric._matches, _elastic.pyx:2112-2149).
+int scaled_profile_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
+                        const int64_t *s_off, const int64_t *s_len, const double *s_mean,
+                        const double *s_std, int64_t n_s, int paired, const void *X, int64_t n,
+                        int64_t xstr, int64_t t, int64_t nwp, double threshold, double *prof,
+                        hipStream_t st) {
+    if (paired && n_s != n) return fail(WBX_EINVAL, "paired patterns need one per series");
+    std::vector<int64_t> hl((size_t)n_s), ho((size_t)n_s);
+    WBX_HIP_CHECK(hipMemcpyAsync(hl.data(), s_len, sizeof(int64_t) * n_s, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(ho.data(), s_off, sizeof(int64_t) * n_s, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));
+    int64_t min_len = t, total = 0;
+    for (int64_t j = 0; j < n_s; ++j) {
+        if (hl[j] < 1 || hl[j] > t)
+            return fail(WBX_EINVAL, "Invalid subsequnce shape (%lld > %lld)", (long long)hl[j],
+                        (long long)t);
+        if (ho[j] < 0) return fail(WBX_EINVAL, "negative pattern offset");
+        if (paired || j == 0) min_len = hl[j] < min_len ? hl[j] : min_len;
+        if (ho[j] + hl[j] > total) total = ho[j] + hl[j];
+    }
+    if (nwp != t - min_len + 1)
+        return fail(WBX_EINVAL, "profile width %lld, expected %lld", (long long)nwp,
+                    (long long)(t - min_len + 1));
+    if (threshold != threshold) return fail(WBX_EINVAL, "threshold is NaN");
+    wbx_metric mm = *m;
+    int64_t s_stride = total;
+    RC(acc64_prepare(ctx, &mm, dtype, {{&S, 1, total, &s_stride}, {&X, n, t, &xstr}}, st));
+    Carve cv;
+    const size_t o_x = cv.take(stage_bytes(dtype, true, n, t));
+    const size_t o_s = cv.take(stage_bytes(dtype, true, 1, total));
+    const size_t o_k = cv.take(scaled_profile_ws_bytes(mm.kind, hl.data(), n_s, total,
+                                                       paired ? 1 : 0, n, t));
+    void *wsp = nullptr;
+    RC(ws_reserve(ctx, cv.total, &wsp));
+    char *ws = (char *)wsp;
+    Staged<double> x64, s64;
+    RC(stage<double>(dtype, X, n, t, xstr, ws, o_x, &x64, st));
+    RC(stage<double>(dtype, S, 1, total, total, ws, o_s, &s64, st));
+    ProfScope pf(st, KERN_SUBSEQ);
+    return launch_scaled_profile(mm.kind, mm.r, s64.ptr, s_off, s_len, hl.data(), n_s, total,
+                                 s_mean, s_std, paired ? 1 : 0, x64.ptr, x64.stride, n, t, nwp,
+                                 threshold, prof, ws + o_k, g_policy.load() != 0, st);
+}
+
 }  // namespace
 
 // ===================================================================== C ABI
@@ -1018,6 +1065,32 @@ int wbx_dev_argmin(const wbx_metric *metric, int dtype, const void *X, int64_t n
     if (!rc)
         rc = argmin_impl(ctx, &mm, dtype, X, nx, x_stride, tx, Y, ny, y_stride, ty, k, lower_bound,
                          idx, dist, st);
+    int rc2 = end(ctx, st);
+    return rc ? rc : rc2;
+}
+
+int wbx_dev_scaled_subsequence_profile(const wbx_metric *metric, int dtype, const void *S,
+                                       const int64_t *s_off, const int64_t *s_len,
+                                       const double *s_mean, const double *s_std, int64_t n_s,
+                                       int paired, const void *X, int64_t n, int64_t x_stride,
+                                       int64_t t, int64_t nwp, double threshold, double *prof,
+                                       void *stream) {
+    RC(check_metric(metric));
+    if (metric->kind != WBX_EUCLIDEAN && metric->kind != WBX_DTW)
+        return fail(WBX_EUNSUPPORTED, "scaled subsequence metric kind %d is not served",
+                    metric->kind);
+    RC(check_sizes(n, t));
+    if (n_s < 1) return fail(WBX_EINVAL, "Subsequence y cannot be empty.");
+    if (n_s > 0x7fffffff) return fail(WBX_EUNSUPPORTED, "too many patterns");
+    if (!S || !s_off || !s_len || !s_mean || !s_std || !X || !prof)
+        return fail(WBX_EINVAL, "NULL argument");
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    DevGuard g(ctx->device);
+    hipStream_t st = pick_stream(ctx, stream);
+    RC(begin(ctx, st));
+    int rc = scaled_profile_impl(ctx, metric, dtype, S, s_off, s_len, s_mean, s_std, n_s,
+                                 paired ? 1 : 0, X, n, x_stride, t, nwp, threshold, prof, st);
     int rc2 = end(ctx, st);
     return rc ? rc : rc2;
 }
@@ -1375,6 +1448,45 @@ int wbx_argmin_subsequence(const wbx_metric *metric, const double *S, const int6
                                   dims * t, t, k, (int64_t *)didx.p, (double *)ddist.p, st));
     WBX_HIP_CHECK(hipMemcpyAsync(idx, didx.p, sizeof(int64_t) * n * k, hipMemcpyDeviceToHost, st));
     WBX_HIP_CHECK(hipMemcpyAsync(dist, ddist.p, sizeof(double) * n * k, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));
+    return WBX_OK;
+}
+
+int wbx_scaled_subsequence_profile(const wbx_metric *metric, const double *S,
+                                   const int64_t *s_off, const int64_t *s_len,
+                                   const double *s_mean, const double *s_std, int64_t n_s,
+                                   int paired, const double *X, int64_t n, int64_t dims,
+                                   int64_t t, int64_t dim, int64_t nwp, double threshold,
+                                   double *prof) {
+    RC(check_metric(metric));
+    RC(check_sizes(n, t));
+    if (n_s < 1) return fail(WBX_EINVAL, "Subsequence y cannot be empty.");
+    if (dims < 1 || dim < 0 || dim >= dims)
+        return fail(WBX_EINVAL, "The parameter dim must be 0 <= dim < n_dims");
+    int64_t total = 0;
+    for (int64_t j = 0; j < n_s; ++j) {
+        if (s_len[j] < 1 || s_len[j] > t)
+            return fail(WBX_EINVAL, "Invalid subsequnce shape (%lld > %lld)", (long long)s_len[j],
+                        (long long)t);
+        if (s_off[j] < 0) return fail(WBX_EINVAL, "negative pattern offset");
+        if (s_off[j] + s_len[j] > total) total = s_off[j] + s_len[j];
+    }
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    hipStream_t st = ctx->stream;
+    DevBuf dx, ds, doff, dlen, dmean, dstd, dprof;
+    RC(upload(dx, X, sizeof(double) * n * dims * t, st));
+    RC(upload(ds, S, sizeof(double) * total, st));
+    RC(upload(doff, s_off, sizeof(int64_t) * n_s, st));
+    RC(upload(dlen, s_len, sizeof(int64_t) * n_s, st));
+    RC(upload(dmean, s_mean, sizeof(double) * n_s, st));
+    RC(upload(dstd, s_std, sizeof(double) * n_s, st));
+    RC(dprof.alloc(sizeof(double) * n * nwp));
+    RC(wbx_dev_scaled_subsequence_profile(
+        metric, WBX_F64, ds.p, (const int64_t *)doff.p, (const int64_t *)dlen.p,
+        (const double *)dmean.p, (const double *)dstd.p, n_s, paired,
+        (const double *)dx.p + dim * t, n, dims * t, t, nwp, threshold, (double *)dprof.p, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(prof, dprof.p, sizeof(double) * n * nwp, hipMemcpyDeviceToHost, st));
     WBX_HIP_CHECK(hipStreamSynchronize(st));
     return WBX_OK;
 }

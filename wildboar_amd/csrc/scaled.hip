@@ -95,10 +95,16 @@ struct ScaledTask {
     const double *elo, *eup;     // series envelopes (cn, t), DTW only
     const double *zs, *zl, *zu;  // normalised patterns (packed as S)
     const int64_t *s_off;
-    const int64_t *order;        // pattern indices of this length group
+    const int64_t *order;        // scans: pattern of each task column; profiles: of each
+                                 // row (NULL: pattern 0)
     int64_t gc, n_s;
     double *dist;
     int64_t *idx;
+    // window profiles (prof != NULL): task row ii writes prof[out_row[ii] * nwp + window]
+    const int64_t *out_row;
+    double th;                   // threshold^2 of the reference's matching
+    int64_t nwp;
+    double *prof;
 };
 
 __device__ __forceinline__ int64_t task_id() {
@@ -268,12 +274,13 @@ __device__ __forceinline__ double srow(double (&row)[BP], const double *yw, doub
 
 // Banded DTW (|i - j| <= rw) of z (rows) against the normalised window
 // (columns) in registers; D = cell (L-1, L-1), M = max over rows i with
-// i + rw < L - 1 of rowmin_i + cb[i * 64].
+// i + rw < L - 1 of v_i = rowmin_i + cb[i * 64], A = the first v_i >= th
+// (the value the reference's abandoned DTW returns), else D.
 template <int BP, int U>
 __device__ __forceinline__ void scaled_cost(const double *__restrict__ z, int L,
                                             const double *__restrict__ zx, int rw, int B,
-                                            const double *__restrict__ cb, double &D,
-                                            double &M) {
+                                            const double *__restrict__ cb, double th, double &D,
+                                            double &M, double &A) {
     constexpr int W = BP + U - 1;
     const int off = rw;
     double row[BP];
@@ -287,15 +294,24 @@ __device__ __forceinline__ void scaled_cost(const double *__restrict__ z, int L,
     for (int m = 0; m < W; ++m) yw[m] = ldz(zx, m - off, L);
     const int last = L - 1 - rw;  // rows i < last are checked against cb
     M = -WBX_INF_D;
+    bool ab = false;
+    A = WBX_INF_D;
+    auto check = [&](double v) {
+        M = dmax(M, v);
+        if (!ab && v >= th) {
+            ab = true;
+            A = v;
+        }
+    };
     double rm = srow<BP>(row, yw, z[0], pen);
-    if (0 < last) M = dmax(M, rm + cb[0]);
+    if (0 < last) check(rm + cb[0]);
 #pragma unroll
     for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
     yw[W - 1] = ldz(zx, 1 - off + W - 1, L);
     const int rem = (L - 1) % U;
     for (int i = 1; i <= rem; ++i) {
         rm = srow<BP>(row, yw, z[i], pen);
-        if (i < last) M = dmax(M, rm + cb[(int64_t)i * 64]);
+        if (i < last) check(rm + cb[(int64_t)i * 64]);
 #pragma unroll
         for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
         yw[W - 1] = ldz(zx, i + 1 - off + W - 1, L);
@@ -310,7 +326,7 @@ __device__ __forceinline__ void scaled_cost(const double *__restrict__ z, int L,
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             rm = srow<BP>(row, yw + u, zv[u], pen);
-            if (i0 + u < last) M = dmax(M, rm + cb[(int64_t)(i0 + u) * 64]);
+            if (i0 + u < last) check(rm + cb[(int64_t)(i0 + u) * 64]);
         }
 #pragma unroll
         for (int m = 0; m < W - U; ++m) yw[m] = yw[m + U];
@@ -322,6 +338,7 @@ __device__ __forceinline__ void scaled_cost(const double *__restrict__ z, int L,
     for (int k = 0; k < BP; ++k)
         if (k == off) res = row[k];
     D = res;
+    if (!ab) A = res;
 }
 
 // Scaled DTW work items: (task q, block of 64 windows) flattened over a
@@ -339,7 +356,25 @@ __device__ __forceinline__ void item_task(const ScaledTask &a, const ScaledItems
     q = item / it.nblk;
     b0 = (int)(item - q * it.nblk) * 64;
     ii = q / a.gc;
-    pj = a.order[q - ii * a.gc];
+    if (a.prof)  // profiles: one task per row, the row's own pattern
+        pj = a.order ? a.order[ii] : 0;
+    else
+        pj = a.order[q - ii * a.gc];
+}
+
+// The reference's matching records a window when every bound is below th
+// and the (possibly abandoned) DTW value A is <= th; the profile holds A
+// then, +inf otherwise.
+__device__ __forceinline__ void scaled_store(const ScaledTask &a, const ScaledItems &it,
+                                             int64_t q, int64_t ii, int s, int nwin, double T,
+                                             double M, double D, double A, double Q) {
+    if (s >= nwin) return;
+    if (a.prof) {
+        a.prof[a.out_row[ii] * a.nwp + s] = (Q < a.th && A <= a.th) ? A : WBX_INF_D;
+    } else {
+        it.Tb[q * nwin + s] = dmax(dmax(T, M), D);
+        it.Db[q * nwin + s] = D;
+    }
 }
 
 // Band in registers (2 rw + 1 <= BP).  Scratch per lane: the suffix bounds
@@ -358,15 +393,12 @@ __global__ __launch_bounds__(64) void k_scaled_dtw(ScaledTask a, ScaledItems it,
         const int s = b0 + (int)threadIdx.x;
         const int sc = s < nwin ? s : nwin - 1;
         const double mu = a.wmean[ii * nwin + sc], sd = a.wstd[ii * nwin + sc];
-        double T = scaled_bounds(a, a.zs + so, a.zl + so, a.zu + so, a.X + ii * a.xstride + sc,
-                                 a.elo + ii * a.t + sc, a.eup + ii * a.t + sc, mu, sd, cb, zx);
-        double D, M;
-        scaled_cost<BP, U>(a.zs + so, L, zx, a.rw, a.B, cb, D, M);
-        T = dmax(dmax(T, M), D);
-        if (s < nwin) {
-            it.Tb[q * nwin + s] = T;
-            it.Db[q * nwin + s] = D;
-        }
+        const double T = scaled_bounds(a, a.zs + so, a.zl + so, a.zu + so,
+                                       a.X + ii * a.xstride + sc, a.elo + ii * a.t + sc,
+                                       a.eup + ii * a.t + sc, mu, sd, cb, zx);
+        double D, M, A;
+        scaled_cost<BP, U>(a.zs + so, L, zx, a.rw, a.B, cb, a.th, D, M, A);
+        scaled_store(a, it, q, ii, s, nwin, T, M, D, A, T);
     }
 }
 
@@ -392,10 +424,12 @@ __global__ __launch_bounds__(64) void k_scaled_dtw_wide(ScaledTask a, ScaledItem
         const int s = b0 + (int)threadIdx.x;
         const int sc = s < nwin ? s : nwin - 1;
         const double mu = a.wmean[ii * nwin + sc], sd = a.wstd[ii * nwin + sc];
-        double T = scaled_bounds(a, z, a.zl + so, a.zu + so, a.X + ii * a.xstride + sc,
-                                 a.elo + ii * a.t + sc, a.eup + ii * a.t + sc, mu, sd, cb, zx);
+        const double T = scaled_bounds(a, z, a.zl + so, a.zu + so, a.X + ii * a.xstride + sc,
+                                       a.elo + ii * a.t + sc, a.eup + ii * a.t + sc, mu, sd, cb,
+                                       zx);
         for (int j = 0; j < L; ++j) row[(int64_t)j * 64] = WBX_INF_D;
-        double M = -WBX_INF_D;
+        double M = -WBX_INF_D, A = WBX_INF_D;
+        bool ab = false;
         for (int i = 0; i < L; ++i) {
             const double xi = z[i];
             const int j0 = i - rw > 0 ? i - rw : 0;
@@ -412,14 +446,18 @@ __global__ __launch_bounds__(64) void k_scaled_dtw_wide(ScaledTask a, ScaledItem
                 left = c;
                 rm = dmin(rm, c);
             }
-            if (i < nc) M = dmax(M, rm + cb[(int64_t)i * 64]);
+            if (i < nc) {
+                const double v = rm + cb[(int64_t)i * 64];
+                M = dmax(M, v);
+                if (!ab && v >= a.th) {
+                    ab = true;
+                    A = v;
+                }
+            }
         }
         const double D = row[(int64_t)(L - 1) * 64];
-        T = dmax(dmax(T, M), D);
-        if (s < nwin) {
-            it.Tb[q * nwin + s] = T;
-            it.Db[q * nwin + s] = D;
-        }
+        if (!ab) A = D;
+        scaled_store(a, it, q, ii, s, nwin, T, M, D, A, T);
     }
 }
 
@@ -479,6 +517,72 @@ __global__ __launch_bounds__(64) void k_window_stats_all(const double *__restric
             ex2 -= u * u;
         }
     }
+}
+
+// Scaled euclidean profile: one wave per (task row, 64 windows); the
+// reference's partial sum stops once it reaches th (scaled_euclidean_distance_
+// matches, _metric.pyx:798-866); the window is recorded when the sum is <= th.
+__global__ __launch_bounds__(64) void k_scaled_euclid_prof(ScaledTask a, int64_t nblk) {
+    const int64_t item = task_id();
+    if (item >= a.cn * nblk) return;
+    const int64_t ii = item / nblk;
+    const int b0 = (int)(item - ii * nblk) * 64;
+    const int64_t pj = a.order ? a.order[ii] : 0;
+    const int L = a.L, nwin = a.t - L + 1;
+    const int s = b0 + (int)threadIdx.x;
+    if (s >= nwin) return;
+    const double *__restrict__ z = a.zs + a.s_off[pj];
+    const double *__restrict__ xw = a.X + ii * a.xstride + s;
+    const double mu = a.wmean[ii * nwin + s], sd = a.wstd[ii * nwin + s];
+    double d = 0.0;
+    for (int k = 0; k < L; ++k) {
+        if (d >= a.th) break;
+        double v = z[k];
+        v -= (xw[k] - mu) / sd;
+        d += v * v;
+    }
+    a.prof[a.out_row[ii] * a.nwp + s] = d <= a.th ? d : WBX_INF_D;
+}
+
+// Window statistics of one pattern length for cn rows (rows of t - L + 1).
+__global__ __launch_bounds__(64) void k_window_stats(const double *__restrict__ X, int64_t cn,
+                                                     int64_t stride, int t, int L,
+                                                     double *__restrict__ mean,
+                                                     double *__restrict__ sdev) {
+    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (i >= cn) return;
+    const int nwin = t - L + 1;
+    const double *__restrict__ x = X + i * stride;
+    double *__restrict__ mo = mean + i * nwin;
+    double *__restrict__ so = sdev + i * nwin;
+    const double dl = (double)L;
+    double ex = 0.0, ex2 = 0.0;
+    for (int k = 0; k < t; ++k) {
+        const double v = x[k];
+        ex += v;
+        ex2 += v * v;
+        if (k >= L - 1) {
+            const int I = k - (L - 1);
+            const double mu = ex / dl;
+            const double tmp = ex2 / dl - mu * mu;
+            mo[I] = mu;
+            so[I] = tmp > 0.0 ? sqrt(tmp) : 1.0;
+            const double u = x[I];
+            ex -= u;
+            ex2 -= u * u;
+        }
+    }
+}
+
+// dst row i = X row sel[i] (t values).
+__global__ __launch_bounds__(256) void k_gather_rows(const double *__restrict__ X, int64_t stride,
+                                                     const int64_t *__restrict__ sel, int64_t cn,
+                                                     int t, double *__restrict__ dst) {
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= cn * t) return;
+    const int64_t i = g / t;
+    const int k = (int)(g - i * t);
+    dst[g] = X[sel[i] * stride + k];
 }
 
 constexpr int kMaxBPS = 56;
@@ -647,6 +751,7 @@ int launch_scaled_subseq(int kind, double r, const double *S, const int64_t *s_o
             a.n_s = n_s;
             a.dist = dist;
             a.idx = idx;
+            a.th = WBX_INF_D;
             const int64_t g0 = G.start[g], g1 = G.start[g + 1];
             if (!dtw) {
                 a.order = d_order + g0;
@@ -699,6 +804,175 @@ int launch_scaled_subseq(int kind, double r, const double *S, const int64_t *s_o
                 hipLaunchKernelGGL(k_scaled_accept, task_grid(cn * a.gc), dim3(64), 0, st, a, it);
                 WBX_HIP_CHECK(hipGetLastError());
             }
+        }
+    }
+    return WBX_OK;
+}
+
+// ------------------------------------------------------------ profiles
+// Task rows are series; paired: pattern i for series i, else pattern 0.
+// Rows are grouped by their pattern's length, gathered into a contiguous
+// chunk, and scanned with the threshold of the reference's matching.
+struct ProfPlan {
+    std::vector<int64_t> sel;    // series, grouped by pattern length
+    std::vector<int64_t> start;  // group g = sel[start[g] .. start[g + 1])
+    std::vector<int> len;
+    int64_t cn_max = 0;
+};
+
+static ProfPlan prof_plan(int kind, const int64_t *hl, int paired, int64_t n, int64_t t) {
+    ProfPlan P;
+    P.sel.resize((size_t)n);
+    for (int64_t i = 0; i < n; ++i) P.sel[i] = i;
+    auto len_of = [&](int64_t i) { return paired ? hl[i] : hl[0]; };
+    std::stable_sort(P.sel.begin(), P.sel.end(),
+                     [&](int64_t a, int64_t b) { return len_of(a) < len_of(b); });
+    for (int64_t k = 0; k < n; ++k) {
+        const int L = (int)len_of(P.sel[k]);
+        if (k == 0 || L != P.len.back()) {
+            P.start.push_back(k);
+            P.len.push_back(L);
+        }
+    }
+    P.start.push_back(n);
+    // per row: the gathered series, its window statistics, its envelope
+    const int64_t per = t * 8 + t * 16 + (kind == WBX_DTW ? t * 16 : 0);
+    int64_t c = (int64_t)(kChunkCap / (size_t)per);
+    c = c < 1 ? 1 : c;
+    P.cn_max = c < n ? c : n;
+    return P;
+}
+
+size_t scaled_profile_ws_bytes(int kind, const int64_t *hl, int64_t n_s, int64_t total,
+                               int paired, int64_t n, int64_t t) {
+    const ProfPlan P = prof_plan(kind, hl, paired, n, t);
+    int64_t max_len = 1;
+    for (int L : P.len) max_len = L > max_len ? L : max_len;
+    size_t b = 0;
+    auto take = [&](size_t bytes) { b += (bytes + 255) & ~(size_t)255; };
+    take((size_t)total * 8);
+    take((size_t)total * 8);
+    take((size_t)total * 8);
+    take((size_t)n * 8);                              // sel
+    take((size_t)P.cn_max * (size_t)t * 8);           // gathered rows
+    take((size_t)P.cn_max * (size_t)t * 16);          // window mean / std
+    if (kind == WBX_DTW) {
+        take((size_t)P.cn_max * (size_t)t * 16);      // envelopes
+        take(scratch_bytes(max_len));
+    }
+    (void)n_s;
+    return b;
+}
+
+int launch_scaled_profile(int kind, double r, const double *S, const int64_t *s_off,
+                          const int64_t *s_len, const int64_t *hl, int64_t n_s, int64_t total,
+                          const double *s_mean, const double *s_std, int paired, const double *X,
+                          int64_t xstride, int64_t n, int64_t t, int64_t nwp, double threshold,
+                          double *prof, char *ws, bool wide_only, hipStream_t st) {
+    if (n <= 0) return WBX_OK;
+    if (t > 0x7fffffff) return fail(WBX_EUNSUPPORTED, "scaled profile: series too long");
+    const bool dtw = kind == WBX_DTW;
+    const ProfPlan P = prof_plan(kind, hl, paired, n, t);
+    int64_t max_len = 1;
+    for (int L : P.len) max_len = L > max_len ? L : max_len;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        char *p = ws + o;
+        o += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    double *zs = (double *)take((size_t)total * 8);
+    double *zl = (double *)take((size_t)total * 8);
+    double *zu = (double *)take((size_t)total * 8);
+    int64_t *d_sel = (int64_t *)take((size_t)n * 8);
+    const int64_t cn_max = P.cn_max;
+    double *rows = (double *)take((size_t)cn_max * (size_t)t * 8);
+    double *wmean = (double *)take((size_t)cn_max * (size_t)t * 16);
+    double *wstd = wmean + cn_max * t;
+    double *elo = nullptr, *eup = nullptr, *scratch = nullptr;
+    const size_t sbytes = scratch_bytes(max_len);
+    if (dtw) {
+        elo = (double *)take((size_t)cn_max * (size_t)t * 16);
+        eup = elo + cn_max * t;
+        scratch = (double *)take(sbytes);
+    }
+    WBX_HIP_CHECK(hipMemcpyAsync(d_sel, P.sel.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
+                                 st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));  // P.sel ends with this call
+    int rc = launch_fill_f64(prof, n * nwp, WBX_INF_D, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_scaled_patterns, dim3((unsigned)n_s), dim3(256), 0, st, S, s_off, s_len,
+                       s_mean, s_std, r, dtw ? 1 : 0, zs, zl, zu);
+    WBX_HIP_CHECK(hipGetLastError());
+    const double th = threshold * threshold;
+    for (size_t g = 0; g < P.len.size(); ++g) {
+        const int L = P.len[g];
+        const int nwin = (int)(t - L + 1);
+        const int rw = warp_width(L, r);
+        for (int64_t c0 = P.start[g]; c0 < P.start[g + 1]; c0 += cn_max) {
+            const int64_t cn = P.start[g + 1] - c0 < cn_max ? P.start[g + 1] - c0 : cn_max;
+            hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((cn * t + 255) / 256)), dim3(256), 0,
+                               st, X, xstride, d_sel + c0, cn, (int)t, rows);
+            WBX_HIP_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(k_window_stats, dim3((unsigned)((cn + 63) / 64)), dim3(64), 0, st,
+                               rows, cn, t, (int)t, L, wmean, wstd);
+            WBX_HIP_CHECK(hipGetLastError());
+            ScaledTask a{};
+            a.X = rows;
+            a.xstride = t;
+            a.t = (int)t;
+            a.L = L;
+            a.c0 = 0;
+            a.cn = cn;
+            a.wmean = wmean;
+            a.wstd = wstd;
+            a.zs = zs;
+            a.zl = zl;
+            a.zu = zu;
+            a.s_off = s_off;
+            a.order = paired ? d_sel + c0 : nullptr;
+            a.gc = 1;
+            a.n_s = n_s;
+            a.out_row = d_sel + c0;
+            a.th = th;
+            a.nwp = nwp;
+            a.prof = prof;
+            const int64_t nblk = (nwin + 63) / 64;
+            if (!dtw) {
+                hipLaunchKernelGGL(k_scaled_euclid_prof, task_grid(cn * nblk), dim3(64), 0, st, a,
+                                   nblk);
+                WBX_HIP_CHECK(hipGetLastError());
+                continue;
+            }
+            rc = launch_envelope(rows, cn, t, (int)t, rw, elo, eup, st);
+            if (rc) return rc;
+            a.elo = elo;
+            a.eup = eup;
+            a.rw = rw;
+            a.B = 2 * rw + 1;
+            const int BP = (a.B + 7) / 8 * 8;
+            const bool wide = wide_only || BP > kMaxBPS;
+            const int64_t per_lane = per_lane_doubles(L, rw, wide);
+            ScaledItems it{};
+            it.nblk = nblk;
+            it.n_items = cn * nblk;
+            const dim3 grid((unsigned)group_waves(it.n_items, sbytes, per_lane));
+            if (!wide) {
+#define WBX_SCALED_LAUNCH(C)                                                              \
+    case C:                                                                               \
+        hipLaunchKernelGGL((k_scaled_dtw<C, 4>), grid, dim3(64), 0, st, a, it, scratch, \
+                           per_lane);                                                     \
+        break;
+                switch (BP) {
+                    WBX_SCALED_CLASSES(WBX_SCALED_LAUNCH)
+                    default: return fail(WBX_EUNSUPPORTED, "scaled dtw: band %d", a.B);
+                }
+#undef WBX_SCALED_LAUNCH
+            } else {
+                hipLaunchKernelGGL(k_scaled_dtw_wide, grid, dim3(64), 0, st, a, it, scratch,
+                                   per_lane);
+            }
+            WBX_HIP_CHECK(hipGetLastError());
         }
     }
     return WBX_OK;

@@ -251,6 +251,17 @@ int launch_scaled_subseq(int kind, double r, const double *S, const int64_t *s_o
                          const double *s_mean, const double *s_std, const double *X,
                          int64_t xstride, int64_t n, int64_t t, double *dist, int64_t *idx,
                          char *ws, bool wide_only, hipStream_t st);
+// Window profiles of the reference's scaled matching at `threshold`:
+// prof (n, nwp) holds the squared distance the matching records for window
+// s of series i against pattern (paired ? i : 0), +inf where it records
+// nothing or past the pattern's last window.
+size_t scaled_profile_ws_bytes(int kind, const int64_t *hl, int64_t n_s, int64_t total,
+                               int paired, int64_t n, int64_t t);
+int launch_scaled_profile(int kind, double r, const double *S, const int64_t *s_off,
+                          const int64_t *s_len, const int64_t *hl, int64_t n_s, int64_t total,
+                          const double *s_mean, const double *s_std, int paired, const double *X,
+                          int64_t xstride, int64_t n, int64_t t, int64_t nwp, double threshold,
+                          double *prof, char *ws, bool wide_only, hipStream_t st);
 
 // --------------------------------------------------------- warping paths
 // dtw_mapping's path of each pair (X row p, Y row p), band width w, as

@@ -556,13 +556,29 @@ def _pack(patterns):
     return S, offs, lens
 
 
-def _profiles(patterns, x_, dim, metric, paired):
+class _ScaledMetric:
+    """A scaled subsequence metric (``scaled_euclidean`` / ``scaled_dtw``)."""
+
+    def __init__(self, base):
+        self.base = base
+
+
+def _profiles(patterns, x_, dim, metric, paired, threshold=np.inf):
     """Squared cost of every window, one row per series: ``(n, t - min_len + 1)``
-    (windows past a pattern's last are +inf)."""
+    (windows past a pattern's last are +inf).  Scaled metrics: the squared
+    distance the reference's scaled matching records at ``threshold``, +inf
+    for windows it does not record."""
     S, offs, lens = _pack(patterns)
     n, dims, t = x_.shape
     nwp = t - int(lens.min()) + 1
     prof = np.empty((n, nwp))
+    if isinstance(metric, _ScaledMetric):
+        mean, std = _pattern_stats(patterns)
+        _lib.check(_lib.load().wbx_scaled_subsequence_profile(
+            metric.base.struct(_lib.FP64), S.ctypes.data, offs.ctypes.data, lens.ctypes.data,
+            mean.ctypes.data, std.ctypes.data, len(patterns), 1 if paired else 0, x_.ctypes.data,
+            n, dims, t, int(dim), nwp, float(threshold), prof.ctypes.data))
+        return prof
     _lib.check(_lib.load().wbx_subsequence_profile(
         metric.struct(_lib.FP64), S.ctypes.data, offs.ctypes.data, lens.ctypes.data, len(patterns),
         1 if paired else 0, x_.ctypes.data, n, dims, t, int(dim), nwp, prof.ctypes.data))
@@ -646,12 +662,15 @@ def _filter_by_max_dist(indices, distances, max_dist):
     return out_i, out_d
 
 
-def _subsequence_metric(metric, metric_params, scale):
-    if scale or (isinstance(metric, str) and metric.startswith("scaled_")):
-        raise NotImplementedError("scaled subsequence metrics are not served by this library")
+def _subsequence_metric(metric, metric_params, scale, scaled_ok=False):
     if metric_params is not None and not isinstance(metric_params, dict):
         raise TypeError("The 'metric_params' parameter must be None or a dict.")
-    return check_metric(metric)(**(metric_params if metric_params is not None else {}))
+    params = metric_params if metric_params is not None else {}
+    if scale or (isinstance(metric, str) and metric.startswith("scaled_")):
+        if not scaled_ok:
+            raise NotImplementedError("scaled subsequence metrics are not served here")
+        return _ScaledMetric(check_metric(_scaled_name(metric, scale))(**params))
+    return check_metric(metric)(**params)
 
 
 def subsequence_match(y, x, threshold=None, *, dim=0, metric="euclidean", metric_params=None,
@@ -678,7 +697,7 @@ def subsequence_match(y, x, threshold=None, *, dim=0, metric="euclidean", metric
     # refused here
     if not (isinstance(dim, numbers.Integral) and 0 <= dim < n_dims):
         raise ValueError("The parameter dim must be 0 <= dim < n_dims")
-    metric_ = _subsequence_metric(metric, metric_params, scale)
+    metric_ = _subsequence_metric(metric, metric_params, scale, scaled_ok=True)
     if n_jobs is not None:
         warnings.warn("n_jobs is not yet supported.", UserWarning)
     n_samples = x.shape[0] if x.ndim > 1 else 1
@@ -694,7 +713,7 @@ def subsequence_match(y, x, threshold=None, *, dim=0, metric="euclidean", metric
         else:
             raise TypeError("exclude must be an instance of {int, float}, not %s."
                             % type(exclude).__qualname__)
-    prof = _profiles([y], _ts(x), dim, metric_, paired=False)
+    prof = _profiles([y], _ts(x), dim, metric_, paired=False, threshold=bound)
     indices, distances = _matches_from_profiles(prof, bound,
                                                 [x.shape[-1] - y.shape[0] + 1] * prof.shape[0])
     if max_dist is not None:
@@ -730,7 +749,7 @@ def paired_subsequence_match(y, x, threshold=None, *, dim=0, metric="euclidean",
     for s in y:
         if s.shape[0] > x.shape[-1]:
             raise ValueError("invalid subsequnce shape (%d > %d)" % (s.shape[0], x.shape[-1]))
-    metric_ = _subsequence_metric(metric, metric_params, scale)
+    metric_ = _subsequence_metric(metric, metric_params, scale, scaled_ok=True)
     if n_jobs is not None:
         warnings.warn("n_jobs is not yet supported.", UserWarning)
     # the reference's one-argument max_dist (:1039) would fail inside
@@ -739,7 +758,7 @@ def paired_subsequence_match(y, x, threshold=None, *, dim=0, metric="euclidean",
     bound, max_dist, max_matches = _resolve_threshold(threshold, max_matches, len(y), False)
     if any(s.shape[0] < 1 for s in y):
         raise ValueError("Subsequence y cannot be empty.")
-    prof = _profiles(y, _ts(x), dim, metric_, paired=True)
+    prof = _profiles(y, _ts(x), dim, metric_, paired=True, threshold=bound)
     indices, distances = _matches_from_profiles(prof, bound,
                                                 [x.shape[-1] - s.shape[0] + 1 for s in y])
     if max_dist is not None:
