@@ -146,10 +146,13 @@ int wbx_argmin_subsequence(const wbx_metric *metric, const double *S,
  * 543-636; "scaled_" + metric, :280-300) through
  * _pairwise_subsequence_distance (_cdistance.pyx:1018-1037), replacing
  * ScaledEuclideanSubsequenceMetric (metric->kind WBX_EUCLIDEAN,
- * distance/_metric.pyx:96-160, scaled_euclidean_distance :561-651) and
+ * distance/_metric.pyx:96-160, scaled_euclidean_distance :561-651),
  * ScaledDtwSubsequenceMetric (WBX_DTW with band fraction r,
  * distance/_elastic.pyx:1928-2203, scaled_dtw_subsequence_distance :343-482,
- * its LB_Kim / LB_Keogh cascade included).  Every window is z-normalised with
+ * its LB_Kim / LB_Keogh cascade included) and ScaledSubsequenceMetricWrap
+ * (_cdistance.pyx:470-567) around WBX_WDTW, WBX_DDTW, WBX_WDDTW or WBX_ADTW
+ * (penalty >= 0): windows normalised with incremental statistics, the first
+ * window of smallest wrapped distance.  Every window is z-normalised with
  * the reference's running mean and std; s_mean / s_std are each pattern's
  * mean and std as the reference's Subsequence holds them (from_array,
  * _cdistance.pyx:453-467: np.mean, np.std, std <= 1e-13 stored as 0; a 0 std

@@ -86,6 +86,8 @@ def _load():
     lib.orc_scaled_euclidean_matches.argtypes = [p, i64, d, d, p, i64, d, p, p]
     lib.orc_scaled_dtw_matches.restype = i64
     lib.orc_scaled_dtw_matches.argtypes = [p, i64, d, d, p, i64, d, d, p, p]
+    lib.orc_scaled_wrap_pairwise.restype = i32
+    lib.orc_scaled_wrap_pairwise.argtypes = [i32, d, d, p, p, p, p, p, i64, p, i64, i64, p, p]
     lib.orc_scaled_subsequence_pairwise.restype = i32
     lib.orc_scaled_subsequence_pairwise.argtypes = [i32, d, p, p, p, p, p, i64, p, i64, i64, i32,
                                                     p, p]
@@ -305,17 +307,27 @@ def warp_width(L, r):
     return int(_load().orc_warp_width(L, r))
 
 
-def scaled_subsequence(shapelets, X, metric="dtw", r=1.0, dim=0, cascade=True):
+def scaled_subsequence(shapelets, X, metric="dtw", r=1.0, dim=0, cascade=True, g=0.05, p=1.0):
     """(dist, idx) of pairwise_subsequence_distance(scale=True) for
-    metric "euclidean" (ScaledEuclideanSubsequenceMetric) or "dtw"
-    (ScaledDtwSubsequenceMetric, the UCR-suite cascade).  cascade=False:
-    every window's full scaled DTW (the exhaustive minimum)."""
+    metric "euclidean" (ScaledEuclideanSubsequenceMetric), "dtw"
+    (ScaledDtwSubsequenceMetric, the UCR-suite cascade; cascade=False:
+    every window's full scaled DTW, the exhaustive minimum) or "wdtw",
+    "ddtw", "wddtw", "adtw" (ScaledSubsequenceMetricWrap)."""
     X = np.ascontiguousarray(_ts(X)[:, dim, :])
     S, offs, lens = pack_shapelets(shapelets)
     mean, std = pattern_stats(shapelets)
     n, t = X.shape
     dist = np.empty((n, len(shapelets)))
     idx = np.empty((n, len(shapelets)), dtype=np.int64)
+    if metric in ("wdtw", "ddtw", "wddtw", "adtw"):
+        # ScaledSubsequenceMetricWrap around the metric (incremental statistics)
+        rc = _load().orc_scaled_wrap_pairwise(
+            _kind(metric), r, _gp(metric, g, p), S.ctypes.data, offs.ctypes.data,
+            lens.ctypes.data, mean.ctypes.data, std.ctypes.data, len(shapelets), X.ctypes.data,
+            n, t, dist.ctypes.data, idx.ctypes.data)
+        if rc != 0:
+            raise MemoryError("oracle allocation failed")
+        return dist, idx
     rc = _load().orc_scaled_subsequence_pairwise(
         {"euclidean": 0, "dtw": 1}[metric], r, S.ctypes.data, offs.ctypes.data,
         lens.ctypes.data, mean.ctypes.data, std.ctypes.data, len(shapelets), X.ctypes.data, n,

@@ -1288,3 +1288,94 @@ out:
     free(cb); free(cb_1); free(cb_2); free(cost); free(cost_prev);
     return n_matches;
 }
+
+/* IncStats, utils/_stats.pyx:45-92 (weight 1.0). */
+typedef struct {
+    double mean, n_samples, sum_square, sum;
+} orc_incstats;
+
+static void incstats_add(orc_incstats *s, double value) {
+    double next_m;
+    s->n_samples += 1.0;
+    next_m = s->mean + (value - s->mean) / s->n_samples;
+    s->sum_square += (value - s->mean) * (value - next_m);
+    s->mean = next_m;
+    s->sum += 1.0 * value;
+}
+
+static void incstats_remove(orc_incstats *s, double value) {
+    double old_m;
+    if (s->n_samples == 1.0) {
+        s->n_samples = 0.0;
+        s->mean = 0.0;
+        s->sum_square = 0.0;
+    } else {
+        old_m = (s->n_samples * s->mean - value) / (s->n_samples - 1.0);
+        s->sum_square -= (value - s->mean) * (value - old_m);
+        s->mean = old_m;
+        s->n_samples -= 1.0;
+    }
+    s->sum -= 1.0 * value;
+}
+
+static double incstats_variance(const orc_incstats *s) {
+    double var;
+    if (s->n_samples <= 1) return 0;
+    var = s->sum_square / s->n_samples;
+    if (var < 1e-13) var = 0.0;
+    return var;
+}
+
+/* ScaledSubsequenceMetricWrap._distance, distance/_cdistance.pyx:495-545,
+ * around the metric `kind` (WDTW, DDTW, WDDTW, ADTW; g = the penalty for
+ * ADTW), reset with the series array (wrap.reset(X, X), :489-493). */
+double orc_scaled_wrap_subsequence(int kind, double r, double g, const double *S, int64_t L,
+                                   double s_mean, double s_std, const double *T, int64_t t,
+                                   int64_t *index) {
+    orc_metric m = {kind, r, g, NULL, NULL, NULL, NULL, NULL};
+    double *s_buffer = malloc(sizeof(double) * (size_t)t), *x_buffer = malloc(sizeof(double) * (size_t)t);
+    double min_dist = INFINITY, mean, std;
+    int64_t i, j;
+    orc_incstats stats = {0.0, 0.0, 0.0, 0.0};
+    if (!s_buffer || !x_buffer || metric_reset(&m, t, t) != 0) {
+        free(s_buffer); free(x_buffer); metric_free(&m);
+        return NAN;
+    }
+    if (s_std == 0) s_std = 1.0;
+    for (i = 0; i < L - 1; i++) {
+        s_buffer[i] = (S[i] - s_mean) / s_std;
+        incstats_add(&stats, T[i]);
+    }
+    s_buffer[L - 1] = (S[L - 1] - s_mean) / s_std;
+    for (i = 0; i < t - L + 1; i++) {
+        incstats_add(&stats, T[i + L - 1]);
+        std = incstats_variance(&stats);
+        std = std == 0.0 ? 1.0 : sqrt(std);
+        mean = stats.mean;
+        for (j = 0; j < L; j++) x_buffer[j] = (T[i + j] - mean) / std;
+        incstats_remove(&stats, T[i]);
+        if (metric_eadistance(&m, s_buffer, L, x_buffer, L, &min_dist)) {
+            if (index) *index = i;
+        }
+    }
+    free(s_buffer);
+    free(x_buffer);
+    metric_free(&m);
+    return min_dist;
+}
+
+int orc_scaled_wrap_pairwise(int kind, double r, double g, const double *S, const int64_t *s_off,
+                             const int64_t *s_len, const double *s_mean, const double *s_std,
+                             int64_t n_s, const double *X, int64_t n, int64_t t, double *dist,
+                             int64_t *idx) {
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = 0; j < n_s; j++) {
+            int64_t index = 0;
+            double d = orc_scaled_wrap_subsequence(kind, r, g, S + s_off[j], s_len[j], s_mean[j],
+                                                   s_std[j], X + i * t, t, &index);
+            if (d != d) return -1;
+            dist[i * n_s + j] = d;
+            idx[i * n_s + j] = index;
+        }
+    return 0;
+}

@@ -101,8 +101,6 @@ def test_scaled_float32_inputs_and_validation():
     np.testing.assert_array_equal(d, od)
     np.testing.assert_array_equal(i, oi)
     with pytest.raises(NotImplementedError):
-        wd.pairwise_subsequence_distance(pats, X, metric="wdtw", scale=True)
-    with pytest.raises(NotImplementedError):
         wd.pairwise_subsequence_distance(pats, X, metric="scaled_lcss")
     with pytest.raises(ValueError):
         wd.pairwise_subsequence_distance(pats, X, metric="scaled_nope")
@@ -158,3 +156,32 @@ def test_scaled_paired_subsequence_match_bitwise(metric, r):
         else:
             np.testing.assert_array_equal(idx[q], oi)
             np.testing.assert_array_equal(dist[q], od)
+
+
+@pytest.mark.parametrize("metric,params", [("wdtw", {"r": 0.2, "g": 0.1}),
+                                           ("wdtw", {"r": 1.0, "g": 0.05}),
+                                           ("ddtw", {"r": 0.1}), ("ddtw", {"r": 1.0}),
+                                           ("wddtw", {"r": 0.3, "g": -0.2}),
+                                           ("adtw", {"r": 0.2, "p": 0.5}),
+                                           ("adtw", {"r": 1.0, "p": 0.0})])
+def test_scaled_wrapped_metrics_bitwise(metric, params):
+    X = rw(908, 9, 130)
+    pats = patterns(X, 909)
+    d, i = wd.pairwise_subsequence_distance(pats, X, metric=metric, metric_params=params,
+                                            scale=True, return_index=True)
+    od, oi = oracle.scaled_subsequence(pats, X, metric, r=params.get("r", 1.0),
+                                       g=params.get("g", 0.05), p=params.get("p", 1.0))
+    np.testing.assert_array_equal(d, od)
+    np.testing.assert_array_equal(i, oi)
+    d2 = wd.pairwise_subsequence_distance(pats[:4], X, metric="scaled_" + metric,
+                                          metric_params=params)
+    np.testing.assert_array_equal(d2, od[:, :4])
+
+
+def test_scaled_wrapped_refusals():
+    X = rw(910, 3, 40)
+    with pytest.raises(NotImplementedError):
+        wd.pairwise_subsequence_distance([X[0, :5]], X, metric="adtw",
+                                         metric_params={"p": -1.0}, scale=True)
+    with pytest.raises(NotImplementedError):
+        wd.subsequence_match(X[0, :5], X, 1.0, metric="wdtw", scale=True)

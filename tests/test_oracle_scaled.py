@@ -176,3 +176,39 @@ def test_scaled_matches_at_a_threshold_are_recorded_full_distances(metric, r):
             else:
                 # the cascade only removes windows
                 assert set(idx.tolist()) <= set(all_i[all_d <= th].tolist())
+
+
+def test_scaled_wrap_is_the_first_smallest_znormalised_distance():
+    # ScaledSubsequenceMetricWrap: windows normalised by incremental statistics
+    # (population std), the wrapped metric's _eadistance on the normalised
+    # buffers (derivative kinds: slopes with the band of length L - 2; weights
+    # sized by the series, t and t - 2), the window of smallest distance
+    X = rw(11, 3, 90)
+    t = X.shape[1]
+    rng = np.random.default_rng(12)
+    pats = [X[1, 20:45] * 3.0 - 2.0, rng.standard_normal(9)]
+    r = 0.3
+    for metric, kw in (("wdtw", {"g": 0.1}), ("ddtw", {}), ("wddtw", {"g": 0.2}),
+                       ("adtw", {"p": 0.3})):
+        d, i = oracle.scaled_subsequence(pats, X, metric, r=r, **kw)
+        for a in range(X.shape[0]):
+            for b, s in enumerate(pats):
+                L = len(s)
+                zs = zpat(s)
+                costs = []
+                for z in znorm_windows(X[a], L):
+                    if metric == "wdtw":
+                        c = oracle.dtw_distance(zs, z, oracle.compute_r(L, r),
+                                                weights=oracle.wdtw_weights(t, kw["g"]))
+                    elif metric == "adtw":
+                        c = oracle.adtw_distance(zs, z, oracle.compute_r(L, r), kw["p"])
+                    else:
+                        wts = oracle.wdtw_weights(t - 2, kw["g"]) if metric == "wddtw" else None
+                        c = oracle.dtw_distance(oracle.average_slope(zs), oracle.average_slope(z),
+                                                oracle.compute_r(L - 2, r), weights=wts)
+                    costs.append(np.sqrt(c))
+                costs = np.array(costs)
+                np.testing.assert_allclose(d[a, b], costs.min(), rtol=1e-8, atol=1e-12)
+                assert costs[i[a, b]] <= costs.min() * (1 + 1e-8) + 1e-12
+        # the pattern's own (affinely changed) window is found, distance ~0
+        assert i[1, 0] == 20 and d[1, 0] < 1e-6

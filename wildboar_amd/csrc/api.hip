@@ -801,6 +801,51 @@ int scaled_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
                                 g_policy.load() != 0, st);
 }
 
+// The wrapped metrics of the same scan (ScaledSubsequenceMetricWrap,
+// _cdistance.pyx:470-567, "scaled_" + wdtw / ddtw / wddtw / adtw,
+// _distance.py:151-160): the wrapped metric is reset with the series array
+// (wrap.reset(X, X)), so WDTW weights are sized t and WDDTW weights t - 2.
+int scaled_wrap_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
+                     const int64_t *s_off, const int64_t *s_len, const double *s_mean,
+                     const double *s_std, int64_t n_s, const void *X, int64_t n, int64_t xstr,
+                     int64_t t, double *dist, int64_t *idx, hipStream_t st) {
+    if (m->kind == WBX_ADTW && !(m->p >= 0.0))
+        return fail(WBX_EUNSUPPORTED, "scaled adtw with a negative penalty is not served");
+    std::vector<int64_t> hl((size_t)n_s), ho((size_t)n_s);
+    WBX_HIP_CHECK(hipMemcpyAsync(hl.data(), s_len, sizeof(int64_t) * n_s, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(ho.data(), s_off, sizeof(int64_t) * n_s, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));
+    int64_t total = 0;
+    for (int64_t j = 0; j < n_s; ++j) {
+        if (hl[j] < 1 || hl[j] > t)
+            return fail(WBX_EINVAL, "Invalid subsequnce shape (%lld > %lld)", (long long)hl[j],
+                        (long long)t);
+        if (ho[j] < 0) return fail(WBX_EINVAL, "negative shapelet offset");
+        if (ho[j] + hl[j] > total) total = ho[j] + hl[j];
+    }
+    wbx_metric mm = *m;
+    int64_t s_stride = total;
+    RC(acc64_prepare(ctx, &mm, dtype, {{&S, 1, total, &s_stride}, {&X, n, t, &xstr}}, st));
+    const double *w64 = nullptr;
+    const float *w32 = nullptr;
+    if (mm.kind == WBX_WDTW) RC(cached_weights(ctx, t, mm.g, &w64, &w32));
+    if (mm.kind == WBX_WDDTW && t > 2) RC(cached_weights(ctx, t - 2, mm.g, &w64, &w32));
+    Carve cv;
+    const size_t o_x = cv.take(stage_bytes(dtype, true, n, t));
+    const size_t o_s = cv.take(stage_bytes(dtype, true, 1, total));
+    const size_t o_k = cv.take(scaled_wrap_ws_bytes(hl.data(), n_s, total, n, t));
+    void *wsp = nullptr;
+    RC(ws_reserve(ctx, cv.total, &wsp));
+    char *ws = (char *)wsp;
+    Staged<double> x64, s64;
+    RC(stage<double>(dtype, X, n, t, xstr, ws, o_x, &x64, st));
+    RC(stage<double>(dtype, S, 1, total, total, ws, o_s, &s64, st));
+    ProfScope pf(st, KERN_SUBSEQ);
+    return launch_scaled_wrap(mm.kind, mm.r, w64, mm.kind == WBX_ADTW ? mm.p : 0.0, s64.ptr,
+                              s_off, s_len, hl.data(), n_s, total, s_mean, s_std, x64.ptr,
+                              x64.stride, n, t, dist, idx, ws + o_k, st);
+}
+
 // Window profiles of the reference's scaled matching (subsequence_match /
 // paired_subsequence_match with scale=True, _distance.py:732-1079 through
 // ScaledEuclideanSubsequenceMetric._matches, _metric.pyx:141-167, and
@@ -1101,9 +1146,6 @@ int wbx_dev_scaled_subsequence_pairwise(const wbx_metric *metric, int dtype, con
                                         const void *X, int64_t n, int64_t x_stride, int64_t t,
                                         double *dist, int64_t *idx, void *stream) {
     RC(check_metric(metric));
-    if (metric->kind != WBX_EUCLIDEAN && metric->kind != WBX_DTW)
-        return fail(WBX_EUNSUPPORTED, "scaled subsequence metric kind %d is not served",
-                    metric->kind);
     RC(check_sizes(n, t));
     if (n_s < 1) return fail(WBX_EINVAL, "Subsequence y cannot be empty.");
     if (n_s > 0x7fffffff) return fail(WBX_EUNSUPPORTED, "too many shapelets");
@@ -1112,8 +1154,11 @@ int wbx_dev_scaled_subsequence_pairwise(const wbx_metric *metric, int dtype, con
     DevGuard g(ctx->device);
     hipStream_t st = pick_stream(ctx, stream);
     RC(begin(ctx, st));
-    int rc = scaled_impl(ctx, metric, dtype, S, s_off, s_len, s_mean, s_std, n_s, X, n, x_stride,
-                         t, dist, idx, st);
+    const bool direct = metric->kind == WBX_EUCLIDEAN || metric->kind == WBX_DTW;
+    int rc = direct ? scaled_impl(ctx, metric, dtype, S, s_off, s_len, s_mean, s_std, n_s, X, n,
+                                  x_stride, t, dist, idx, st)
+                    : scaled_wrap_impl(ctx, metric, dtype, S, s_off, s_len, s_mean, s_std, n_s,
+                                       X, n, x_stride, t, dist, idx, st);
     int rc2 = end(ctx, st);
     return rc ? rc : rc2;
 }

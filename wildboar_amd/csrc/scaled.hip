@@ -105,6 +105,7 @@ struct ScaledTask {
     double th;                   // threshold^2 of the reference's matching
     int64_t nwp;
     double *prof;
+    bool values_are_distances;   // accept: the per-window values are already sqrt'd
 };
 
 __device__ __forceinline__ int64_t task_id() {
@@ -478,8 +479,140 @@ __global__ __launch_bounds__(64) void k_scaled_accept(ScaledTask a, ScaledItems 
     }
     if (threadIdx.x == 0) {
         const int64_t o = (a.c0 + ii) * a.n_s + pj;
-        a.dist[o] = sqrt(best);
+        a.dist[o] = a.values_are_distances ? best : sqrt(best);
         a.idx[o] = bi >= 0 ? bi : 0;
+    }
+}
+
+// ------------------------------------------------ wrapped metrics
+// ScaledSubsequenceMetricWrap (_cdistance.pyx:470-567) around WeightedDtw-,
+// DerivativeDtw-, WeightedDerivativeDtw- and AmercingDtwMetric: every window
+// is normalised with incremental statistics (IncStats, utils/_stats.pyx:
+// 45-92: Welford add, the matching remove, population variance, < 1e-13 ->
+// 0, then std 0 -> 1.0) and the wrapped metric's _eadistance compares
+// sqrt(cost) with the best so far (strict <).  Its early abandon (a row
+// minimum above best^2) only drops windows whose full sqrt(cost) is not
+// below the best, so the result is the first window of smallest sqrt(cost).
+
+struct WrapArgs {
+    const double *weights;  // WDTW / WDDTW weights (NULL: none)
+    double pen;             // ADTW penalty (0 otherwise)
+    int deriv;              // average slopes of both buffers
+    int band;               // _compute_r(Lw, r)
+};
+
+// Replays the wrap's running statistics for one pattern length L, one thread
+// per (group, series) as k_window_stats_all lays them out.
+__global__ __launch_bounds__(64) void k_incstats_all(const double *__restrict__ X, int64_t cn,
+                                                     int64_t stride, int t, int n_groups,
+                                                     const int *__restrict__ lens,
+                                                     const int64_t *__restrict__ goff,
+                                                     double *__restrict__ mean,
+                                                     double *__restrict__ sdev) {
+    const int64_t id = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (id >= cn * n_groups) return;
+    const int64_t g = id / cn, i = id - g * cn;
+    const int L = lens[g];
+    const int nwin = t - L + 1;
+    const double *__restrict__ x = X + i * stride;
+    double *__restrict__ mo = mean + goff[g] + i * nwin;
+    double *__restrict__ so = sdev + goff[g] + i * nwin;
+    double n = 0.0, m = 0.0, ss = 0.0;
+    auto add = [&](double v) {
+        n += 1.0;
+        const double next_m = m + (v - m) / n;
+        ss += (v - m) * (v - next_m);
+        m = next_m;
+    };
+    for (int k = 0; k < L - 1; ++k) add(x[k]);
+    for (int I = 0; I < nwin; ++I) {
+        add(x[I + L - 1]);
+        double var = 0.0;
+        if (!(n <= 1.0)) {
+            var = ss / n;
+            if (var < 1e-13) var = 0.0;
+        }
+        so[I] = var == 0.0 ? 1.0 : sqrt(var);
+        mo[I] = m;
+        const double v = x[I];
+        if (n == 1.0) {
+            n = 0.0;
+            m = 0.0;
+            ss = 0.0;
+        } else {
+            const double old_m = (n * m - v) / (n - 1.0);
+            ss -= (v - m) * (v - old_m);
+            m = old_m;
+            n -= 1.0;
+        }
+    }
+}
+
+// One lane per window: the normalised window (or its average slopes) and one
+// cost row in scratch; the reference's dtw_distance / adtw_distance band and
+// row 0 (weights w[j - 1] there, w[|i - j|] below) on Lw x Lw, updated in
+// place.  Writes sqrt(cost) per window to it.Tb (= it.Db).
+__global__ __launch_bounds__(64) void k_scaled_wrap(ScaledTask a, ScaledItems it, WrapArgs w,
+                                                    double *__restrict__ scratch,
+                                                    int64_t per_lane) {
+    double *__restrict__ zx = scratch + (int64_t)blockIdx.x * per_lane * 64 + threadIdx.x;
+    const int L = a.L, nwin = a.t - L + 1;
+    const int Lw = w.deriv ? L - 2 : L;
+    double *__restrict__ row = zx + (int64_t)L * 64;
+    const int r = w.band;
+    for (int64_t item = blockIdx.x; item < it.n_items; item += gridDim.x) {
+        int64_t q, ii, pj;
+        int b0;
+        item_task(a, it, item, q, b0, ii, pj);
+        const double *__restrict__ z = a.zs + a.s_off[pj];
+        const int s = b0 + (int)threadIdx.x;
+        const int sc = s < nwin ? s : nwin - 1;
+        const double mu = a.wmean[ii * nwin + sc], sd = a.wstd[ii * nwin + sc];
+        const double *__restrict__ xw = a.X + ii * a.xstride + sc;
+        for (int k = 0; k < L; ++k) zx[(int64_t)k * 64] = (xw[k] - mu) / sd;
+        if (w.deriv)  // average_slope (_elastic.pyx:3220-3225), in place
+            for (int k = 0; k < Lw; ++k) {
+                const double q0 = zx[(int64_t)k * 64], q1 = zx[(int64_t)(k + 1) * 64],
+                             q2 = zx[(int64_t)(k + 2) * 64];
+                zx[(int64_t)k * 64] = ((q1 - q0) + ((q2 - q0) / 2)) / 2;
+            }
+        auto xrow = [&](int i) {
+            if (!w.deriv) return z[i];
+            return ((z[i + 1] - z[i]) + ((z[i + 2] - z[i]) / 2)) / 2;
+        };
+        // row 0: prefix sum over [0, min(Lw, r))
+        const int r0 = Lw < r ? Lw : r;
+        {
+            const double x0 = xrow(0);
+            double acc = 0.0;
+            for (int j = 0; j < Lw; ++j) {
+                double c = WBX_INF_D;
+                if (j < r0) {
+                    const double v = x0 - zx[(int64_t)j * 64];
+                    const double wt = w.weights ? w.weights[j > 0 ? j - 1 : 0] : 1.0;
+                    c = j == 0 ? v * v * wt : acc + v * v * wt;
+                    acc = c;
+                }
+                row[(int64_t)j * 64] = c;
+            }
+        }
+        for (int i = 1; i < Lw; ++i) {
+            const double xi = xrow(i);
+            const int j0 = i - r + 1 > 0 ? i - r + 1 : 0;
+            const int j1 = i + r < Lw ? i + r : Lw;
+            double left = WBX_INF_D;
+            double diag = j0 > 0 ? row[(int64_t)(j0 - 1) * 64] : WBX_INF_D;
+            for (int j = j0; j < j1; ++j) {
+                const double up = row[(int64_t)j * 64];
+                const double v = xi - zx[(int64_t)j * 64];
+                const double wt = w.weights ? w.weights[i > j ? i - j : j - i] : 1.0;
+                const double c = dmin(dmin(up + w.pen, left + w.pen), diag) + v * v * wt;
+                diag = up;
+                row[(int64_t)j * 64] = c;
+                left = c;
+            }
+        }
+        if (s < nwin) it.Tb[q * nwin + s] = sqrt(row[(int64_t)(Lw - 1) * 64]);
     }
 }
 
@@ -801,6 +934,135 @@ int launch_scaled_subseq(int kind, double r, const double *S, const int64_t *s_o
                                        per_lane);
                 }
                 WBX_HIP_CHECK(hipGetLastError());
+                hipLaunchKernelGGL(k_scaled_accept, task_grid(cn * a.gc), dim3(64), 0, st, a, it);
+                WBX_HIP_CHECK(hipGetLastError());
+            }
+        }
+    }
+    return WBX_OK;
+}
+
+// ------------------------------------------------------- wrapped metrics
+size_t scaled_wrap_ws_bytes(const int64_t *hl, int64_t n_s, int64_t total, int64_t n,
+                            int64_t t) {
+    const Groups G = make_groups(hl, n_s, t);
+    const int64_t max_len = G.len.back();
+    const int64_t cn = chunk_rows(WBX_EUCLIDEAN, n, t, G.win_total);
+    size_t b = 0;
+    auto take = [&](size_t bytes) { b += (bytes + 255) & ~(size_t)255; };
+    take((size_t)total * 8);                      // zs
+    take((size_t)n_s * 8);                        // order
+    take(G.len.size() * 16);                      // group offsets, lengths
+    take((size_t)cn * (size_t)G.win_total * 16);  // window mean / std
+    take(scratch_bytes(max_len));
+    take(kTDCap > (size_t)cn * (size_t)t * 8 ? kTDCap : (size_t)cn * (size_t)t * 8);
+    return b;
+}
+
+int launch_scaled_wrap(int kind, double r, const double *weights, double penalty,
+                       const double *S, const int64_t *s_off, const int64_t *s_len,
+                       const int64_t *hl, int64_t n_s, int64_t total, const double *s_mean,
+                       const double *s_std, const double *X, int64_t xstride, int64_t n,
+                       int64_t t, double *dist, int64_t *idx, char *ws, hipStream_t st) {
+    if (n <= 0 || n_s <= 0) return WBX_OK;
+    if (t > 0x7fffffff) return fail(WBX_EUNSUPPORTED, "scaled scan: series too long");
+    const bool deriv = kind == WBX_DDTW || kind == WBX_WDDTW;
+    const Groups G = make_groups(hl, n_s, t);
+    const int n_groups = (int)G.len.size();
+    const int64_t max_len = G.len.back();
+    const int64_t cn_max = chunk_rows(WBX_EUCLIDEAN, n, t, G.win_total);
+    std::vector<int64_t> goff((size_t)n_groups);
+    for (int64_t g = 0, w = 0; g < n_groups; ++g) {
+        goff[g] = w * cn_max;
+        w += t - G.len[g] + 1;
+    }
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        char *p = ws + o;
+        o += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    double *zs = (double *)take((size_t)total * 8);
+    int64_t *d_order = (int64_t *)take((size_t)n_s * 8);
+    char *gmeta = take((size_t)n_groups * 16);
+    int64_t *d_goff = (int64_t *)gmeta;
+    int *d_glen = (int *)(gmeta + (size_t)n_groups * 8);
+    double *wmean = (double *)take((size_t)cn_max * (size_t)G.win_total * 16);
+    double *wstd = wmean + cn_max * G.win_total;
+    const size_t sbytes = scratch_bytes(max_len);
+    double *scratch = (double *)take(sbytes);
+    const size_t td_bytes =
+        kTDCap > (size_t)cn_max * (size_t)t * 8 ? kTDCap : (size_t)cn_max * (size_t)t * 8;
+    double *Tb = (double *)take(td_bytes);
+    WBX_HIP_CHECK(hipMemcpyAsync(d_order, G.order.data(), sizeof(int64_t) * n_s,
+                                 hipMemcpyHostToDevice, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(d_glen, G.len.data(), sizeof(int) * n_groups,
+                                 hipMemcpyHostToDevice, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(d_goff, goff.data(), sizeof(int64_t) * n_groups,
+                                 hipMemcpyHostToDevice, st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));  // the host vectors end with this call
+    hipLaunchKernelGGL(k_scaled_patterns, dim3((unsigned)n_s), dim3(256), 0, st, S, s_off, s_len,
+                       s_mean, s_std, r, 0, zs, (double *)nullptr, (double *)nullptr);
+    WBX_HIP_CHECK(hipGetLastError());
+    for (int64_t c0 = 0; c0 < n; c0 += cn_max) {
+        const int64_t cn = n - c0 < cn_max ? n - c0 : cn_max;
+        const double *xc = X + c0 * xstride;
+        const int64_t nst = cn * n_groups;
+        hipLaunchKernelGGL(k_incstats_all, dim3((unsigned)((nst + 63) / 64)), dim3(64), 0, st, xc,
+                           cn, xstride, (int)t, n_groups, d_glen, d_goff, wmean, wstd);
+        WBX_HIP_CHECK(hipGetLastError());
+        for (int g = 0; g < n_groups; ++g) {
+            const int L = G.len[g];
+            const int nwin = (int)(t - L + 1);
+            const int64_t g0 = G.start[g], g1 = G.start[g + 1];
+            ScaledTask a{};
+            a.X = xc;
+            a.xstride = xstride;
+            a.t = (int)t;
+            a.L = L;
+            a.c0 = c0;
+            a.cn = cn;
+            a.wmean = wmean + goff[g];
+            a.wstd = wstd + goff[g];
+            a.zs = zs;
+            a.s_off = s_off;
+            a.n_s = n_s;
+            a.dist = dist;
+            a.idx = idx;
+            a.th = WBX_INF_D;
+            a.values_are_distances = true;
+            const int Lw = deriv ? L - 2 : L;
+            WrapArgs w;
+            w.weights = weights;
+            w.pen = penalty;
+            w.deriv = deriv ? 1 : 0;
+            {
+                const double v = floor((double)(Lw > 0 ? Lw : 0) * r);
+                w.band = (int)(v > 1.0 ? v : 1.0);  // _compute_r, _elastic.pyx:1924-1925
+            }
+            const int64_t per_lane = 2 * (int64_t)L;
+            int64_t bsz = (int64_t)(td_bytes / ((size_t)cn * (size_t)nwin * 8));
+            if (bsz < 1) bsz = 1;
+            for (int64_t b0 = g0; b0 < g1; b0 += bsz) {
+                a.order = d_order + b0;
+                a.gc = g1 - b0 < bsz ? g1 - b0 : bsz;
+                ScaledItems it;
+                it.nblk = (nwin + 63) / 64;
+                it.n_items = cn * a.gc * it.nblk;
+                it.Tb = Tb;
+                it.Db = Tb;
+                if (deriv && L < 3) {
+                    // DerivativeDtwMetric._eadistance never accepts below
+                    // length 3 (_elastic.pyx:3291-3292): the wrap returns
+                    // +inf with the index unset (0 here)
+                    const int rc = launch_fill_f64(Tb, cn * a.gc * nwin, WBX_INF_D, st);
+                    if (rc) return rc;
+                } else {
+                    const dim3 grid((unsigned)group_waves(it.n_items, sbytes, per_lane));
+                    hipLaunchKernelGGL(k_scaled_wrap, grid, dim3(64), 0, st, a, it, w, scratch,
+                                       per_lane);
+                    WBX_HIP_CHECK(hipGetLastError());
+                }
                 hipLaunchKernelGGL(k_scaled_accept, task_grid(cn * a.gc), dim3(64), 0, st, a, it);
                 WBX_HIP_CHECK(hipGetLastError());
             }

@@ -251,6 +251,16 @@ int launch_scaled_subseq(int kind, double r, const double *S, const int64_t *s_o
                          const double *s_mean, const double *s_std, const double *X,
                          int64_t xstride, int64_t n, int64_t t, double *dist, int64_t *idx,
                          char *ws, bool wide_only, hipStream_t st);
+// pairwise_subsequence_distance with scale=True for the wrapped metrics
+// (ScaledSubsequenceMetricWrap): kind WBX_WDTW (weights sized t), WBX_DDTW,
+// WBX_WDDTW (weights sized t - 2) or WBX_ADTW (penalty >= 0).
+size_t scaled_wrap_ws_bytes(const int64_t *hl, int64_t n_s, int64_t total, int64_t n,
+                            int64_t t);
+int launch_scaled_wrap(int kind, double r, const double *weights, double penalty,
+                       const double *S, const int64_t *s_off, const int64_t *s_len,
+                       const int64_t *hl, int64_t n_s, int64_t total, const double *s_mean,
+                       const double *s_std, const double *X, int64_t xstride, int64_t n,
+                       int64_t t, double *dist, int64_t *idx, char *ws, hipStream_t st);
 // Window profiles of the reference's scaled matching at `threshold`:
 // prof (n, nwp) holds the squared distance the matching records for window
 // s of series i against pattern (paired ? i : 0), +inf where it records

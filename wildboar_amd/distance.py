@@ -424,8 +424,11 @@ def pairwise_subsequence_distance(y, x, *, dim=0, metric="euclidean", metric_par
 
 
 # Scaled subsequence metrics served: "scaled_" + these (_distance.py:151-183);
-# the registry's other scaled names raise NotImplementedError.
-_SCALED_SERVED = ("euclidean", "dtw")
+# the registry's other scaled names raise NotImplementedError.  The wrapped
+# ones (ScaledSubsequenceMetricWrap) are served by pairwise_subsequence_distance;
+# matching serves scaled_euclidean and scaled_dtw.
+_SCALED_SERVED = ("euclidean", "dtw", "wdtw", "ddtw", "wddtw", "adtw")
+_SCALED_MATCH_SERVED = ("euclidean", "dtw")
 _REFERENCE_ONLY = ("lcss", "edr", "twe", "msm", "erp", "manhattan", "minkowski", "chebyshev",
                    "cosine", "angular")
 
@@ -667,9 +670,10 @@ def _subsequence_metric(metric, metric_params, scale, scaled_ok=False):
         raise TypeError("The 'metric_params' parameter must be None or a dict.")
     params = metric_params if metric_params is not None else {}
     if scale or (isinstance(metric, str) and metric.startswith("scaled_")):
-        if not scaled_ok:
+        base = _scaled_name(metric, scale)
+        if not scaled_ok or base not in _SCALED_MATCH_SERVED:
             raise NotImplementedError("scaled subsequence metrics are not served here")
-        return _ScaledMetric(check_metric(_scaled_name(metric, scale))(**params))
+        return _ScaledMetric(check_metric(base)(**params))
     return check_metric(metric)(**params)
 
 
