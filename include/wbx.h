@@ -186,6 +186,19 @@ int wbx_scaled_subsequence_profile(const wbx_metric *metric, const double *S,
                                    int64_t dim, int64_t nwp, double threshold,
                                    double *prof);
 
+/* argmin_subsequence_distance with scale=True (distance/_distance.py:
+ * 1738-1756) through _ScaledArgminSubsequenceDistance
+ * (distance/_cdistance.pyx:1451-1549): pattern i, normalised with its
+ * fast_mean_std (utils/_stats.pyx:22-42; std 0 read as 1.0), against the
+ * windows of series i normalised with incremental statistics, through the
+ * metric's _eadistance (any kind; adtw with penalty >= 0) and the k-Heap; idx
+ * and dist (n, k) in heap order.  k <= t - longest pattern + 1. */
+int wbx_scaled_argmin_subsequence(const wbx_metric *metric, const double *S,
+                                  const int64_t *s_off, const int64_t *s_len,
+                                  int64_t n, const double *X, int64_t dims,
+                                  int64_t t, int64_t dim, int64_t k,
+                                  int64_t *idx, double *dist);
+
 /* The per-series window costs behind _subsequence_match and
  * _paired_subsequence_match (distance/_cdistance.pyx:1065-1141, windows with
  * cost <= threshold^2 as dtw_subsequence_matches, _elastic.pyx:658-698, and
@@ -258,6 +271,15 @@ int wbx_dev_scaled_subsequence_profile(const wbx_metric *metric, int dtype,
                                        int64_t x_stride, int64_t t,
                                        int64_t nwp, double threshold,
                                        double *prof, void *stream);
+
+/* Device form of wbx_scaled_argmin_subsequence (idx / dist device (n, k));
+ * synchronises `stream`. */
+int wbx_dev_scaled_argmin_subsequence(const wbx_metric *metric, int dtype,
+                                      const void *S, const int64_t *s_off,
+                                      const int64_t *s_len, const void *X,
+                                      int64_t n, int64_t x_stride, int64_t t,
+                                      int64_t k, int64_t *idx, double *dist,
+                                      void *stream);
 
 /* Device forms of wbx_argmin_subsequence (n patterns, one per series;
  * idx / dist device (n, k)) and wbx_subsequence_profile (prof device

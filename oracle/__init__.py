@@ -88,6 +88,8 @@ def _load():
     lib.orc_scaled_dtw_matches.argtypes = [p, i64, d, d, p, i64, d, d, p, p]
     lib.orc_scaled_wrap_pairwise.restype = i32
     lib.orc_scaled_wrap_pairwise.argtypes = [i32, d, d, p, p, p, p, p, i64, p, i64, i64, p, p]
+    lib.orc_scaled_argmin_subsequence.restype = i32
+    lib.orc_scaled_argmin_subsequence.argtypes = [i32, d, d, p, p, p, i64, p, i64, i64, p, p]
     lib.orc_scaled_subsequence_pairwise.restype = i32
     lib.orc_scaled_subsequence_pairwise.argtypes = [i32, d, p, p, p, p, p, i64, p, i64, i64, i32,
                                                     p, p]
@@ -359,6 +361,22 @@ def scaled_subsequence_matches(s, x, threshold, metric="dtw", r=1.0):
     if n < 0:
         raise MemoryError("oracle allocation failed")
     return idx[:n].copy(), dist[:n].copy()
+
+
+def scaled_argmin_subsequence(shapelets, X, k=1, metric="dtw", r=1.0, g=0.05, p=1.0):
+    """(idx, dist), each (n_samples, k) in heap order: argmin_subsequence_distance
+    with scale=True (pattern i against series i, both z-normalised)."""
+    X = np.ascontiguousarray(_ts(X)[:, 0, :])
+    S, offs, lens = pack_shapelets(shapelets)
+    n, t = X.shape
+    idx = np.zeros((n, k), dtype=np.int64)
+    dist = np.zeros((n, k))
+    rc = _load().orc_scaled_argmin_subsequence(
+        _kind(metric), r, _gp(metric, g, p), S.ctypes.data, offs.ctypes.data, lens.ctypes.data,
+        n, X.ctypes.data, t, k, idx.ctypes.data, dist.ctypes.data)
+    if rc != 0:
+        raise MemoryError("oracle allocation failed")
+    return idx, dist
 
 
 def argmin_subsequence(shapelets, X, k=1, metric="dtw", r=1.0, g=0.05, dim=0):

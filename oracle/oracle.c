@@ -1379,3 +1379,67 @@ int orc_scaled_wrap_pairwise(int kind, double r, double g, const double *S, cons
         }
     return 0;
 }
+
+/* fast_mean_std, utils/_stats.pyx:22-42. */
+static void fast_mean_std(const double *data, int64_t length, double *mean, double *std) {
+    double ex = 0, ex2 = 0, v;
+    for (int64_t i = 0; i < length; i++) {
+        v = data[i];
+        ex += v;
+        ex2 += v * v;
+    }
+    mean[0] = ex / (double)length;
+    ex2 = ex2 / (double)length - mean[0] * mean[0];
+    std[0] = ex2 > 1e-13 ? sqrt(ex2) : 0.0;
+}
+
+/* _ScaledArgminSubsequenceDistance.__call__, distance/_cdistance.pyx:1451-
+ * 1549: pattern i (fast_mean_std, std 0 -> 1.0) against the windows of
+ * series i normalised with incremental statistics, through the metric's
+ * _eadistance and the k-Heap; outputs in heap order. */
+int orc_scaled_argmin_subsequence(int kind, double r, double g, const double *S,
+                                  const int64_t *s_off, const int64_t *s_len, int64_t n,
+                                  const double *X, int64_t t, int64_t k, int64_t *idx,
+                                  double *dist) {
+    orc_metric m = {kind, r, g, NULL, NULL, NULL, NULL, NULL};
+    double *s_buffer = malloc(sizeof(double) * (size_t)t), *t_buffer = malloc(sizeof(double) * (size_t)t);
+    orc_heap_el *heap = (orc_heap_el *)malloc(sizeof(orc_heap_el) * (size_t)k);
+    if (!s_buffer || !t_buffer || !heap || metric_reset(&m, t, t) != 0) {
+        free(s_buffer); free(t_buffer); free(heap); metric_free(&m);
+        return -1;
+    }
+    for (int64_t i = 0; i < n; i++) {
+        const double *s = S + s_off[i], *x = X + i * t;
+        const int64_t L = s_len[i];
+        double distance = INFINITY, mean, std;
+        int64_t held = 0, q, j;
+        orc_incstats stats = {0.0, 0.0, 0.0, 0.0};
+        fast_mean_std(s, L, &mean, &std);
+        if (std == 0.0) std = 1.0;
+        for (q = 0; q < L - 1; q++) {
+            s_buffer[q] = (s[q] - mean) / std;
+            incstats_add(&stats, x[q]);
+        }
+        s_buffer[L - 1] = (s[L - 1] - mean) / std;
+        for (j = 0; j < t - L + 1; j++) {
+            incstats_add(&stats, x[j + L - 1]);
+            std = incstats_variance(&stats);
+            std = std == 0.0 ? 1.0 : sqrt(std);
+            mean = stats.mean;
+            for (q = 0; q < L; q++) t_buffer[q] = (x[j + q] - mean) / std;
+            incstats_remove(&stats, x[j]);
+            if (metric_eadistance(&m, s_buffer, L, t_buffer, L, &distance))
+                heap_push(heap, &held, k, j, distance);
+            distance = held == k ? heap[0].value : INFINITY;
+        }
+        for (q = 0; q < k; q++) {
+            idx[i * k + q] = q < held ? heap[q].index : 0;
+            dist[i * k + q] = q < held ? heap[q].value : 0.0;
+        }
+    }
+    free(s_buffer);
+    free(t_buffer);
+    free(heap);
+    metric_free(&m);
+    return 0;
+}

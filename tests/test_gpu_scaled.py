@@ -185,3 +185,26 @@ def test_scaled_wrapped_refusals():
                                          metric_params={"p": -1.0}, scale=True)
     with pytest.raises(NotImplementedError):
         wd.subsequence_match(X[0, :5], X, 1.0, metric="wdtw", scale=True)
+
+
+@pytest.mark.parametrize("metric,params", [("euclidean", None), ("dtw", {"r": 0.2}),
+                                           ("dtw", {"r": 1.0}), ("wdtw", {"r": 0.5, "g": 0.1}),
+                                           ("ddtw", {"r": 0.3}), ("wddtw", {"r": 0.3, "g": 0.3}),
+                                           ("adtw", {"r": 0.4, "p": 0.2})])
+@pytest.mark.parametrize("k", [1, 4])
+def test_scaled_argmin_subsequence_bitwise(metric, params, k):
+    X = rw(911, 8, 100)
+    rng = np.random.default_rng(912)
+    pats = [X[q, 3 * q:3 * q + L] * 2.0 + 1.0 for q, L in enumerate((5, 12, 30, 2, 17, 40, 8, 25))]
+    pats[3] = rng.standard_normal(2)
+    pats[6] = np.full(8, 3.0)  # constant: fast_mean_std gives std 0 -> 1.0
+    p = params or {}
+    idx, dist = wd.argmin_subsequence_distance(pats, X, k=k, metric=metric, metric_params=params,
+                                               scale=True, return_distance=True)
+    oi, od = oracle.scaled_argmin_subsequence(pats, X, k=k, metric=metric, r=p.get("r", 1.0),
+                                              g=p.get("g", 0.05), p=p.get("p", 1.0))
+    np.testing.assert_array_equal(idx, oi)
+    np.testing.assert_array_equal(dist, od)
+    idx2 = wd.argmin_subsequence_distance(pats, X, k=k, metric="scaled_" + metric,
+                                          metric_params=params)
+    np.testing.assert_array_equal(idx2, oi)

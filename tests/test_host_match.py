@@ -183,5 +183,5 @@ def test_argmin_subsequence_validation():
         wd.argmin_subsequence_distance(S, X, k=0)
     with pytest.raises(ValueError):
         wd.argmin_subsequence_distance(S, X, metric="lcss_not_here")
-    with pytest.raises(NotImplementedError):
-        wd.argmin_subsequence_distance(S, X, metric="scaled_euclidean")
+    with pytest.raises(ValueError):
+        wd.argmin_subsequence_distance(S, X, metric="scaled_lcss_not_here")

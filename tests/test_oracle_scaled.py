@@ -212,3 +212,31 @@ def test_scaled_wrap_is_the_first_smallest_znormalised_distance():
                 assert costs[i[a, b]] <= costs.min() * (1 + 1e-8) + 1e-12
         # the pattern's own (affinely changed) window is found, distance ~0
         assert i[1, 0] == 20 and d[1, 0] < 1e-6
+
+
+@pytest.mark.parametrize("metric", ["euclidean", "dtw", "ddtw"])
+def test_scaled_argmin_k1_matches_the_znormalised_minimum(metric):
+    # argmin with k = 1 keeps the first window of smallest distance over the
+    # z-normalised windows: the pattern's own affine copy at distance ~0, and
+    # the distance of the best window equals brute force on numpy-normalised
+    # windows
+    X = rw(13, 4, 80)
+    pats = [X[q, 10:10 + L] * 1.5 - 0.5 for q, L in enumerate((9, 16, 24, 31))]
+    i, d = oracle.scaled_argmin_subsequence(pats, X, k=1, metric=metric, r=0.25)
+    for q, s in enumerate(pats):
+        assert i[q, 0] == 10 and d[q, 0] < 1e-6
+    i, d = oracle.scaled_argmin_subsequence([np.sin(np.arange(12.0))] * 4, X, k=1,
+                                            metric=metric, r=0.25)
+    for q in range(4):
+        zs = zpat(np.sin(np.arange(12.0)))
+        costs = []
+        for z in znorm_windows(X[q], 12):
+            if metric == "euclidean":
+                costs.append(np.sqrt(((z - zs) ** 2).sum()))
+            elif metric == "dtw":
+                costs.append(np.sqrt(oracle.dtw_distance(zs, z, oracle.compute_r(12, 0.25))))
+            else:
+                costs.append(np.sqrt(oracle.dtw_distance(oracle.average_slope(zs),
+                                                         oracle.average_slope(z),
+                                                         oracle.compute_r(10, 0.25))))
+        np.testing.assert_allclose(d[q, 0], min(costs), rtol=1e-8)

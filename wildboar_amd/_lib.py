@@ -34,6 +34,7 @@ EXPORTS = (
     "wbx_dev_dtw_mm_update", "wbx_dev_dtw_envelope", "wbx_dev_lb_keogh", "wbx_dev_lb_kim",
     "wbx_scaled_subsequence_pairwise", "wbx_dev_scaled_subsequence_pairwise",
     "wbx_scaled_subsequence_profile", "wbx_dev_scaled_subsequence_profile",
+    "wbx_scaled_argmin_subsequence", "wbx_dev_scaled_argmin_subsequence",
 )
 
 
@@ -123,6 +124,9 @@ def load():
                                                      i64, i64, d, p]),
             "wbx_dev_scaled_subsequence_profile": (i32, [m, i32, p, p, p, p, p, i64, i32, p, i64,
                                                          i64, i64, i64, d, p, p]),
+            "wbx_scaled_argmin_subsequence": (i32, [m, p, p, p, i64, p, i64, i64, i64, i64, p, p]),
+            "wbx_dev_scaled_argmin_subsequence": (i32, [m, i32, p, p, p, p, i64, i64, i64, i64, p,
+                                                        p, p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)

@@ -1140,6 +1140,79 @@ int wbx_dev_scaled_subsequence_profile(const wbx_metric *metric, int dtype, cons
     return rc ? rc : rc2;
 }
 
+int wbx_dev_scaled_argmin_subsequence(const wbx_metric *metric, int dtype, const void *S,
+                                      const int64_t *s_off, const int64_t *s_len, const void *X,
+                                      int64_t n, int64_t x_stride, int64_t t, int64_t k,
+                                      int64_t *idx, double *dist, void *stream) {
+    RC(check_metric(metric));
+    RC(check_sizes(n, t));
+    if (!S || !s_off || !s_len || !X || !idx || !dist) return fail(WBX_EINVAL, "NULL argument");
+    if (k < 1) return fail(WBX_EINVAL, "k == %lld, must be >= 1", (long long)k);
+    if (metric->kind == WBX_ADTW && !(metric->p >= 0.0))
+        return fail(WBX_EUNSUPPORTED, "scaled adtw with a negative penalty is not served");
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    DevGuard g(ctx->device);
+    hipStream_t st = pick_stream(ctx, stream);
+    RC(begin(ctx, st));
+    std::vector<int64_t> hl((size_t)n), ho((size_t)n);
+    WBX_HIP_CHECK(hipMemcpyAsync(hl.data(), s_len, sizeof(int64_t) * n, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(ho.data(), s_off, sizeof(int64_t) * n, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));
+    int64_t min_len = t, max_len = 0, total = 0;
+    for (int64_t q = 0; q < n; ++q) {
+        if (hl[q] < 1 || hl[q] > t)
+            return fail(WBX_EINVAL, "the longest subsequence must be shorter than samples.");
+        if (ho[q] < 0) return fail(WBX_EINVAL, "negative pattern offset");
+        min_len = hl[q] < min_len ? hl[q] : min_len;
+        max_len = hl[q] > max_len ? hl[q] : max_len;
+        if (ho[q] + hl[q] > total) total = ho[q] + hl[q];
+    }
+    if (k > t - max_len + 1) return fail(WBX_EINVAL, "k must be less x.shape[-1] - y.shape[-1] + 1.");
+    const int64_t nwp = t - min_len + 1;
+    wbx_metric mm = *metric;
+    int64_t s_stride = total;
+    RC(acc64_prepare(ctx, &mm, dtype, {{&S, 1, total, &s_stride}, {&X, n, t, &x_stride}}, st));
+    // Metric.reset(t, s) sizes the weights by the wider array: t
+    const double *w64 = nullptr;
+    const float *w32 = nullptr;
+    if (mm.kind == WBX_WDTW) RC(cached_weights(ctx, t, mm.g, &w64, &w32));
+    if (mm.kind == WBX_WDDTW && t > 2) RC(cached_weights(ctx, t - 2, mm.g, &w64, &w32));
+    // the profile and the heaps live in their own allocation; the workspace
+    // holds the staged inputs and the scan's buffers
+    const size_t prof_bytes = ((size_t)n * nwp * 8 + 255) & ~(size_t)255;
+    void *buf = nullptr;
+    if (hipMalloc(&buf, prof_bytes + (size_t)n * k * 16) != hipSuccess)
+        return fail(WBX_ENOMEM, "profile allocation of %zu bytes failed", prof_bytes);
+    double *prof = (double *)buf;
+    int rc = WBX_OK;
+    {
+        Carve cv;
+        const size_t o_x = cv.take(stage_bytes(dtype, true, n, t));
+        const size_t o_s = cv.take(stage_bytes(dtype, true, 1, total));
+        const size_t o_k = cv.take(scaled_argmin_ws_bytes(hl.data(), n, total, t));
+        void *wsp = nullptr;
+        rc = ws_reserve(ctx, cv.total, &wsp);
+        char *ws = (char *)wsp;
+        Staged<double> x64{}, s64{};
+        if (!rc) rc = stage<double>(dtype, X, n, t, x_stride, ws, o_x, &x64, st);
+        if (!rc) rc = stage<double>(dtype, S, 1, total, total, ws, o_s, &s64, st);
+        if (!rc) {
+            ProfScope pf(st, KERN_SUBSEQ);
+            rc = launch_scaled_argmin_profile(mm.kind, mm.r, w64,
+                                              mm.kind == WBX_ADTW ? mm.p : 0.0, s64.ptr, s_off,
+                                              s_len, hl.data(), total, x64.ptr, x64.stride, n, t,
+                                              nwp, prof, ws + o_k, st);
+        }
+    }
+    if (!rc) rc = launch_argmin_replay(prof, n, nwp, k, nullptr, (char *)buf + prof_bytes, idx,
+                                       dist, st);
+    const int rc2 = end(ctx, st);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(buf);
+    return rc ? rc : rc2;
+}
+
 int wbx_dev_scaled_subsequence_pairwise(const wbx_metric *metric, int dtype, const void *S,
                                         const int64_t *s_off, const int64_t *s_len,
                                         const double *s_mean, const double *s_std, int64_t n_s,
@@ -1532,6 +1605,41 @@ int wbx_scaled_subsequence_profile(const wbx_metric *metric, const double *S,
         (const double *)dmean.p, (const double *)dstd.p, n_s, paired,
         (const double *)dx.p + dim * t, n, dims * t, t, nwp, threshold, (double *)dprof.p, st));
     WBX_HIP_CHECK(hipMemcpyAsync(prof, dprof.p, sizeof(double) * n * nwp, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));
+    return WBX_OK;
+}
+
+int wbx_scaled_argmin_subsequence(const wbx_metric *metric, const double *S,
+                                  const int64_t *s_off, const int64_t *s_len, int64_t n,
+                                  const double *X, int64_t dims, int64_t t, int64_t dim,
+                                  int64_t k, int64_t *idx, double *dist) {
+    RC(check_metric(metric));
+    RC(check_sizes(n, t));
+    if (dims < 1 || dim < 0 || dim >= dims)
+        return fail(WBX_EINVAL, "The parameter dim must be 0 <= dim < n_dims");
+    int64_t total = 0;
+    for (int64_t j = 0; j < n; ++j) {
+        if (s_len[j] < 1 || s_len[j] > t)
+            return fail(WBX_EINVAL, "the longest subsequence must be shorter than samples.");
+        if (s_off[j] < 0) return fail(WBX_EINVAL, "negative pattern offset");
+        if (s_off[j] + s_len[j] > total) total = s_off[j] + s_len[j];
+    }
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    hipStream_t st = ctx->stream;
+    DevBuf dx, ds, doff, dlen, didx, ddist;
+    RC(upload(dx, X, sizeof(double) * n * dims * t, st));
+    RC(upload(ds, S, sizeof(double) * total, st));
+    RC(upload(doff, s_off, sizeof(int64_t) * n, st));
+    RC(upload(dlen, s_len, sizeof(int64_t) * n, st));
+    RC(didx.alloc(sizeof(int64_t) * n * k));
+    RC(ddist.alloc(sizeof(double) * n * k));
+    RC(wbx_dev_scaled_argmin_subsequence(metric, WBX_F64, ds.p, (const int64_t *)doff.p,
+                                         (const int64_t *)dlen.p, (const double *)dx.p + dim * t,
+                                         n, dims * t, t, k, (int64_t *)didx.p, (double *)ddist.p,
+                                         st));
+    WBX_HIP_CHECK(hipMemcpyAsync(idx, didx.p, sizeof(int64_t) * n * k, hipMemcpyDeviceToHost, st));
+    WBX_HIP_CHECK(hipMemcpyAsync(dist, ddist.p, sizeof(double) * n * k, hipMemcpyDeviceToHost, st));
     WBX_HIP_CHECK(hipStreamSynchronize(st));
     return WBX_OK;
 }

@@ -501,22 +501,11 @@ struct WrapArgs {
     int band;               // _compute_r(Lw, r)
 };
 
-// Replays the wrap's running statistics for one pattern length L, one thread
-// per (group, series) as k_window_stats_all lays them out.
-__global__ __launch_bounds__(64) void k_incstats_all(const double *__restrict__ X, int64_t cn,
-                                                     int64_t stride, int t, int n_groups,
-                                                     const int *__restrict__ lens,
-                                                     const int64_t *__restrict__ goff,
-                                                     double *__restrict__ mean,
-                                                     double *__restrict__ sdev) {
-    const int64_t id = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (id >= cn * n_groups) return;
-    const int64_t g = id / cn, i = id - g * cn;
-    const int L = lens[g];
+// The wrap's running statistics of one series for pattern length L: mean /
+// std of windows [0, t - L].
+__device__ void incstats_row(const double *__restrict__ x, int t, int L, double *__restrict__ mo,
+                             double *__restrict__ so) {
     const int nwin = t - L + 1;
-    const double *__restrict__ x = X + i * stride;
-    double *__restrict__ mo = mean + goff[g] + i * nwin;
-    double *__restrict__ so = sdev + goff[g] + i * nwin;
     double n = 0.0, m = 0.0, ss = 0.0;
     auto add = [&](double v) {
         n += 1.0;
@@ -546,6 +535,55 @@ __global__ __launch_bounds__(64) void k_incstats_all(const double *__restrict__ 
             n -= 1.0;
         }
     }
+}
+
+// One thread per (group, series), laid out as k_window_stats_all.
+__global__ __launch_bounds__(64) void k_incstats_all(const double *__restrict__ X, int64_t cn,
+                                                     int64_t stride, int t, int n_groups,
+                                                     const int *__restrict__ lens,
+                                                     const int64_t *__restrict__ goff,
+                                                     double *__restrict__ mean,
+                                                     double *__restrict__ sdev) {
+    const int64_t id = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (id >= cn * n_groups) return;
+    const int64_t g = id / cn, i = id - g * cn;
+    const int L = lens[g];
+    const int nwin = t - L + 1;
+    incstats_row(X + i * stride, t, L, mean + goff[g] + i * nwin, sdev + goff[g] + i * nwin);
+}
+
+// One thread per row, one length (rows of t - L + 1).
+__global__ __launch_bounds__(64) void k_incstats_rows(const double *__restrict__ X, int64_t cn,
+                                                      int64_t stride, int t, int L,
+                                                      double *__restrict__ mean,
+                                                      double *__restrict__ sdev) {
+    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (i >= cn) return;
+    const int nwin = t - L + 1;
+    incstats_row(X + i * stride, t, L, mean + i * nwin, sdev + i * nwin);
+}
+
+// fast_mean_std (utils/_stats.pyx:22-42) of each pattern, one thread each;
+// the scaled argmin then reads a 0 std as 1.0.
+__global__ __launch_bounds__(64) void k_fast_mean_std(const double *__restrict__ S,
+                                                      const int64_t *__restrict__ off,
+                                                      const int64_t *__restrict__ len, int64_t n,
+                                                      double *__restrict__ mean,
+                                                      double *__restrict__ sdev) {
+    const int64_t j = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (j >= n) return;
+    const double *__restrict__ s = S + off[j];
+    const int64_t L = len[j];
+    double ex = 0.0, ex2 = 0.0;
+    for (int64_t i = 0; i < L; ++i) {
+        const double v = s[i];
+        ex += v;
+        ex2 += v * v;
+    }
+    const double mu = ex / (double)L;
+    const double v2 = ex2 / (double)L - mu * mu;
+    mean[j] = mu;
+    sdev[j] = v2 > 1e-13 ? sqrt(v2) : 0.0;
 }
 
 // One lane per window: the normalised window (or its average slopes) and one
@@ -612,7 +650,13 @@ __global__ __launch_bounds__(64) void k_scaled_wrap(ScaledTask a, ScaledItems it
                 left = c;
             }
         }
-        if (s < nwin) it.Tb[q * nwin + s] = sqrt(row[(int64_t)(Lw - 1) * 64]);
+        if (s < nwin) {
+            const double d = sqrt(row[(int64_t)(Lw - 1) * 64]);
+            if (a.prof)
+                a.prof[a.out_row[ii] * a.nwp + s] = d;
+            else
+                it.Tb[q * nwin + s] = d;
+        }
     }
 }
 
@@ -1066,6 +1110,130 @@ int launch_scaled_wrap(int kind, double r, const double *weights, double penalty
                 hipLaunchKernelGGL(k_scaled_accept, task_grid(cn * a.gc), dim3(64), 0, st, a, it);
                 WBX_HIP_CHECK(hipGetLastError());
             }
+        }
+    }
+    return WBX_OK;
+}
+
+// argmin_subsequence_distance(scale=True): prof[i * nwp + s] = the metric's
+// distance of window s of series i against pattern i (both normalised), +inf
+// past the pattern's last window.  Kinds: euclidean (the diagonal, band 1),
+// dtw, wdtw, ddtw, wddtw, adtw.
+size_t scaled_argmin_ws_bytes(const int64_t *hl, int64_t n, int64_t total, int64_t t) {
+    int64_t max_len = 1;
+    for (int64_t i = 0; i < n; ++i) max_len = hl[i] > max_len ? hl[i] : max_len;
+    const int64_t per = t * 8 + t * 16;
+    int64_t cn = (int64_t)(kChunkCap / (size_t)per);
+    cn = cn < 1 ? 1 : (cn < n ? cn : n);
+    size_t b = 0;
+    auto take = [&](size_t bytes) { b += (bytes + 255) & ~(size_t)255; };
+    take((size_t)total * 8);                 // zs
+    take((size_t)n * 16);                    // pattern mean / std
+    take((size_t)n * 8);                     // sel
+    take((size_t)cn * (size_t)t * 8);        // gathered rows
+    take((size_t)cn * (size_t)t * 16);       // window mean / std
+    take(scratch_bytes(max_len));
+    return b;
+}
+
+int launch_scaled_argmin_profile(int kind, double r, const double *weights, double penalty,
+                                 const double *S, const int64_t *s_off, const int64_t *s_len,
+                                 const int64_t *hl, int64_t total, const double *X,
+                                 int64_t xstride, int64_t n, int64_t t, int64_t nwp, double *prof,
+                                 char *ws, hipStream_t st) {
+    if (n <= 0) return WBX_OK;
+    if (t > 0x7fffffff) return fail(WBX_EUNSUPPORTED, "scaled argmin: series too long");
+    const bool deriv = kind == WBX_DDTW || kind == WBX_WDDTW;
+    std::vector<int64_t> sel((size_t)n), start;
+    std::vector<int> lens;
+    for (int64_t i = 0; i < n; ++i) sel[i] = i;
+    std::stable_sort(sel.begin(), sel.end(), [&](int64_t a, int64_t b) { return hl[a] < hl[b]; });
+    for (int64_t k = 0; k < n; ++k) {
+        const int L = (int)hl[sel[k]];
+        if (k == 0 || L != lens.back()) {
+            start.push_back(k);
+            lens.push_back(L);
+        }
+    }
+    start.push_back(n);
+    int64_t max_len = 1;
+    for (int L : lens) max_len = L > max_len ? L : max_len;
+    const int64_t per = t * 8 + t * 16;
+    int64_t cn_max = (int64_t)(kChunkCap / (size_t)per);
+    cn_max = cn_max < 1 ? 1 : (cn_max < n ? cn_max : n);
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        char *p = ws + o;
+        o += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    double *zs = (double *)take((size_t)total * 8);
+    double *pmean = (double *)take((size_t)n * 16);
+    double *pstd = pmean + n;
+    int64_t *d_sel = (int64_t *)take((size_t)n * 8);
+    double *rows = (double *)take((size_t)cn_max * (size_t)t * 8);
+    double *wmean = (double *)take((size_t)cn_max * (size_t)t * 16);
+    double *wstd = wmean + cn_max * t;
+    const size_t sbytes = scratch_bytes(max_len);
+    double *scratch = (double *)take(sbytes);
+    WBX_HIP_CHECK(hipMemcpyAsync(d_sel, sel.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
+                                 st));
+    WBX_HIP_CHECK(hipStreamSynchronize(st));  // sel ends with this call
+    int rc = launch_fill_f64(prof, n * nwp, WBX_INF_D, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fast_mean_std, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, S, s_off,
+                       s_len, n, pmean, pstd);
+    WBX_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_scaled_patterns, dim3((unsigned)n), dim3(256), 0, st, S, s_off, s_len,
+                       pmean, pstd, r, 0, zs, (double *)nullptr, (double *)nullptr);
+    WBX_HIP_CHECK(hipGetLastError());
+    for (size_t g = 0; g < lens.size(); ++g) {
+        const int L = lens[g];
+        const int nwin = (int)(t - L + 1);
+        const int Lw = deriv ? L - 2 : L;
+        if (deriv && L < 3) continue;  // never accepted: the profile stays +inf
+        WrapArgs w;
+        w.weights = weights;
+        w.pen = penalty;
+        w.deriv = deriv ? 1 : 0;
+        if (kind == WBX_EUCLIDEAN) {
+            w.band = 1;  // the diagonal: euclidean_distance's sequential sum
+        } else {
+            const double v = floor((double)Lw * r);
+            w.band = (int)(v > 1.0 ? v : 1.0);  // _compute_r, _elastic.pyx:1924-1925
+        }
+        for (int64_t c0 = start[g]; c0 < start[g + 1]; c0 += cn_max) {
+            const int64_t cn = start[g + 1] - c0 < cn_max ? start[g + 1] - c0 : cn_max;
+            hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((cn * t + 255) / 256)), dim3(256), 0,
+                               st, X, xstride, d_sel + c0, cn, (int)t, rows);
+            WBX_HIP_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(k_incstats_rows, dim3((unsigned)((cn + 63) / 64)), dim3(64), 0, st,
+                               rows, cn, t, (int)t, L, wmean, wstd);
+            WBX_HIP_CHECK(hipGetLastError());
+            ScaledTask a{};
+            a.X = rows;
+            a.xstride = t;
+            a.t = (int)t;
+            a.L = L;
+            a.cn = cn;
+            a.wmean = wmean;
+            a.wstd = wstd;
+            a.zs = zs;
+            a.s_off = s_off;
+            a.order = d_sel + c0;
+            a.gc = 1;
+            a.n_s = n;
+            a.out_row = d_sel + c0;
+            a.th = WBX_INF_D;
+            a.nwp = nwp;
+            a.prof = prof;
+            ScaledItems it{};
+            it.nblk = (nwin + 63) / 64;
+            it.n_items = cn * it.nblk;
+            const int64_t per_lane = 2 * (int64_t)L;
+            const dim3 grid((unsigned)group_waves(it.n_items, sbytes, per_lane));
+            hipLaunchKernelGGL(k_scaled_wrap, grid, dim3(64), 0, st, a, it, w, scratch, per_lane);
+            WBX_HIP_CHECK(hipGetLastError());
         }
     }
     return WBX_OK;

@@ -818,6 +818,22 @@ def argmin_subsequence_distance(y, x, *, dim=0, k=1, metric="euclidean", metric_
         raise ValueError(f"The parameter dim must be dim ({dim}) < n_dims ({x_.shape[1]})")
     if not k <= (x_.shape[2] - max_len + 1):
         raise ValueError("k must be less x.shape[-1] - y.shape[-1] + 1.")
+    if isinstance(metric, str) and (scale or metric.startswith("scaled_")):
+        # _distance.py:1738-1756: "scaled_" + metric or scale=True runs the
+        # plain metric on z-normalised buffers (_ScaledArgminSubsequenceDistance)
+        if metric_params is not None and not isinstance(metric_params, dict):
+            raise TypeError("The 'metric_params' parameter must be None or a dict.")
+        base = metric[len("scaled_"):] if metric.startswith("scaled_") else metric
+        metric_ = check_metric(base)(**(metric_params if metric_params is not None else {}))
+        S, offs, lens = _pack(patterns)
+        n = x_.shape[0]
+        idx = np.empty((n, k), dtype=np.intp)
+        dist = np.empty((n, k))
+        _lib.check(_lib.load().wbx_scaled_argmin_subsequence(
+            metric_.struct(_lib.FP64), S.ctypes.data, offs.ctypes.data, lens.ctypes.data, n,
+            x_.ctypes.data, x_.shape[1], x_.shape[2], int(dim), int(k), idx.ctypes.data,
+            dist.ctypes.data))
+        return (idx, dist) if return_distance else idx
     metric_ = _subsequence_metric(metric, metric_params, scale)
     S, offs, lens = _pack(patterns)
     n = x_.shape[0]
