@@ -172,7 +172,9 @@ int wbx_scaled_subsequence_pairwise(const wbx_metric *metric, const double *S,
  * scaled_euclidean_distance_matches (distance/_metric.pyx:798-866) for
  * metric->kind WBX_EUCLIDEAN, scaled_dtw_matches (distance/_elastic.pyx:
  * 484-619, its LB_Kim / LB_Keogh cascade against threshold^2 included) for
- * WBX_DTW.  prof (n, nwp), nwp = t - shortest pattern + 1 (pattern 0 alone
+ * WBX_DTW, ScaledSubsequenceMetricWrap._matches (_cdistance.pyx:547-615:
+ * windows whose distance is below threshold, strict) for WBX_WDTW,
+ * WBX_DDTW, WBX_WDDTW and WBX_ADTW (penalty >= 0).  prof (n, nwp), nwp = t - shortest pattern + 1 (pattern 0 alone
  * when not paired), holds the squared distance recorded for window s of
  * series i against pattern (paired ? i : 0) -- sqrt of it is the reported
  * distance -- and +inf where nothing is recorded or past the pattern's last

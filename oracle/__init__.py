@@ -90,6 +90,8 @@ def _load():
     lib.orc_scaled_wrap_pairwise.argtypes = [i32, d, d, p, p, p, p, p, i64, p, i64, i64, p, p]
     lib.orc_scaled_argmin_subsequence.restype = i32
     lib.orc_scaled_argmin_subsequence.argtypes = [i32, d, d, p, p, p, i64, p, i64, i64, p, p]
+    lib.orc_scaled_wrap_matches.restype = i64
+    lib.orc_scaled_wrap_matches.argtypes = [i32, d, d, p, i64, d, d, p, i64, d, p, p]
     lib.orc_scaled_subsequence_pairwise.restype = i32
     lib.orc_scaled_subsequence_pairwise.argtypes = [i32, d, p, p, p, p, p, i64, p, i64, i64, i32,
                                                     p, p]
@@ -339,7 +341,7 @@ def scaled_subsequence(shapelets, X, metric="dtw", r=1.0, dim=0, cascade=True, g
     return dist, idx
 
 
-def scaled_subsequence_matches(s, x, threshold, metric="dtw", r=1.0):
+def scaled_subsequence_matches(s, x, threshold, metric="dtw", r=1.0, g=0.05, p=1.0):
     """(indices, distances) of the windows of the 1-D series x that the
     reference's scaled matching records for pattern s (scaled_euclidean /
     scaled_dtw ``_matches``), in window order."""
@@ -350,7 +352,11 @@ def scaled_subsequence_matches(s, x, threshold, metric="dtw", r=1.0):
     idx = np.zeros(max(nw, 1), dtype=np.int64)
     dist = np.zeros(max(nw, 1))
     lib = _load()
-    if metric == "euclidean":
+    if metric in ("wdtw", "ddtw", "wddtw", "adtw"):
+        n = lib.orc_scaled_wrap_matches(_kind(metric), float(r), _gp(metric, g, p), s.ctypes.data,
+                                        s.shape[0], mean[0], std[0], x.ctypes.data, x.shape[0],
+                                        float(threshold), dist.ctypes.data, idx.ctypes.data)
+    elif metric == "euclidean":
         n = lib.orc_scaled_euclidean_matches(s.ctypes.data, s.shape[0], mean[0], std[0],
                                              x.ctypes.data, x.shape[0], float(threshold),
                                              dist.ctypes.data, idx.ctypes.data)

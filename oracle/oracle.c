@@ -1443,3 +1443,44 @@ int orc_scaled_argmin_subsequence(int kind, double r, double g, const double *S,
     metric_free(&m);
     return 0;
 }
+
+/* ScaledSubsequenceMetricWrap._matches, distance/_cdistance.pyx:547-615: a
+ * window is recorded (with the metric's distance) when _eadistance accepts
+ * it against the threshold itself, i.e. distance < threshold. */
+int64_t orc_scaled_wrap_matches(int kind, double r, double g, const double *S, int64_t L,
+                                double s_mean, double s_std, const double *T, int64_t t,
+                                double threshold, double *distances, int64_t *matches) {
+    orc_metric m = {kind, r, g, NULL, NULL, NULL, NULL, NULL};
+    double *s_buffer = malloc(sizeof(double) * (size_t)t), *x_buffer = malloc(sizeof(double) * (size_t)t);
+    double mean, std, tmp_dist;
+    int64_t i, j, n = 0;
+    orc_incstats stats = {0.0, 0.0, 0.0, 0.0};
+    if (!s_buffer || !x_buffer || metric_reset(&m, t, t) != 0) {
+        free(s_buffer); free(x_buffer); metric_free(&m);
+        return -1;
+    }
+    if (s_std == 0) s_std = 1.0;
+    for (i = 0; i < L - 1; i++) {
+        s_buffer[i] = (S[i] - s_mean) / s_std;
+        incstats_add(&stats, T[i]);
+    }
+    s_buffer[L - 1] = (S[L - 1] - s_mean) / s_std;
+    for (i = 0; i < t - L + 1; i++) {
+        incstats_add(&stats, T[i + L - 1]);
+        std = incstats_variance(&stats);
+        std = std == 0.0 ? 1.0 : sqrt(std);
+        mean = stats.mean;
+        for (j = 0; j < L; j++) x_buffer[j] = (T[i + j] - mean) / std;
+        incstats_remove(&stats, T[i]);
+        tmp_dist = threshold;
+        if (metric_eadistance(&m, s_buffer, L, x_buffer, L, &tmp_dist)) {
+            matches[n] = i;
+            distances[n] = tmp_dist;
+            n++;
+        }
+    }
+    free(s_buffer);
+    free(x_buffer);
+    metric_free(&m);
+    return n;
+}

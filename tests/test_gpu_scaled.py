@@ -184,7 +184,8 @@ def test_scaled_wrapped_refusals():
         wd.pairwise_subsequence_distance([X[0, :5]], X, metric="adtw",
                                          metric_params={"p": -1.0}, scale=True)
     with pytest.raises(NotImplementedError):
-        wd.subsequence_match(X[0, :5], X, 1.0, metric="wdtw", scale=True)
+        wd.subsequence_match(X[0, :5], X, 1.0, metric="adtw", metric_params={"p": -0.5},
+                             scale=True)
 
 
 @pytest.mark.parametrize("metric,params", [("euclidean", None), ("dtw", {"r": 0.2}),
@@ -208,3 +209,35 @@ def test_scaled_argmin_subsequence_bitwise(metric, params, k):
     idx2 = wd.argmin_subsequence_distance(pats, X, k=k, metric="scaled_" + metric,
                                           metric_params=params)
     np.testing.assert_array_equal(idx2, oi)
+
+
+@pytest.mark.parametrize("metric,params", [("wdtw", {"r": 0.3, "g": 0.1}), ("ddtw", {"r": 0.2}),
+                                           ("wddtw", {"r": 1.0, "g": -0.1}),
+                                           ("adtw", {"r": 0.5, "p": 0.3})])
+def test_scaled_wrapped_matching_bitwise(metric, params):
+    X = rw(913, 5, 110)
+    s = X[2, 30:55] * 0.7 + 4.0
+    p = params
+    for threshold in (0.5, 1.5, np.inf):
+        idx, dist = wd.subsequence_match(s, X, threshold, metric=metric, metric_params=params,
+                                         scale=True, return_distance=True)
+        for q in range(X.shape[0]):
+            oi, od = oracle.scaled_subsequence_matches(s, X[q], threshold, metric,
+                                                       p.get("r", 1.0), p.get("g", 0.05),
+                                                       p.get("p", 1.0))
+            if oi.size == 0:
+                assert idx[q] is None
+            else:
+                np.testing.assert_array_equal(idx[q], oi)
+                np.testing.assert_array_equal(dist[q], od)
+    pats = [X[q, 5:5 + L] + q for q, L in enumerate((10, 2, 20, 33, 7))]
+    idx, dist = wd.paired_subsequence_match(pats, X, 2.0, metric=metric, metric_params=params,
+                                            scale=True, return_distance=True)
+    for q in range(X.shape[0]):
+        oi, od = oracle.scaled_subsequence_matches(pats[q], X[q], 2.0, metric, p.get("r", 1.0),
+                                                   p.get("g", 0.05), p.get("p", 1.0))
+        if oi.size == 0:
+            assert idx[q] is None
+        else:
+            np.testing.assert_array_equal(idx[q], oi)
+            np.testing.assert_array_equal(dist[q], od)

@@ -157,7 +157,7 @@ def test_match_validation_matches_reference():
     with pytest.raises(TypeError):
         wd.subsequence_match(np.array([1, 2]), np.array([1, 2, 3, 4]), exclude="2")
     with pytest.raises(NotImplementedError):
-        wd.subsequence_match(np.array([1, 2]), np.array([1, 2, 3, 4]), metric="wdtw", scale=True)
+        wd.subsequence_match(np.array([1, 2]), np.array([1, 2, 3, 4]), metric="scaled_lcss")
     with pytest.raises(ValueError):
         wd.subsequence_match(np.array([1, 2]), np.array([1, 2, 3, 4]), metric="scaled_nope")
     with pytest.warns(UserWarning, match="n_jobs"):

@@ -240,3 +240,21 @@ def test_scaled_argmin_k1_matches_the_znormalised_minimum(metric):
                                                          oracle.average_slope(z),
                                                          oracle.compute_r(10, 0.25))))
         np.testing.assert_allclose(d[q, 0], min(costs), rtol=1e-8)
+
+
+@pytest.mark.parametrize("metric,kw", [("wdtw", {"g": 0.1}), ("ddtw", {}), ("adtw", {"p": 0.4})])
+def test_scaled_wrap_matches_are_the_strictly_closer_windows(metric, kw):
+    # the wrap records a window when its distance is below the threshold
+    # (strict); at infinity every window, with the distances whose minimum
+    # the wrap's scan reports
+    X = rw(14, 3, 70)
+    s = X[0, 10:30] * 2.0
+    d, i = oracle.scaled_subsequence([s], X, metric, r=0.3, **kw)
+    for q in range(X.shape[0]):
+        all_i, all_d = oracle.scaled_subsequence_matches(s, X[q], np.inf, metric, 0.3, **kw)
+        np.testing.assert_array_equal(all_i, np.arange(70 - 20 + 1))
+        assert all_d.min() == d[q, 0] and np.argmin(all_d) == i[q, 0]
+        th = float(np.median(all_d))
+        idx, dist = oracle.scaled_subsequence_matches(s, X[q], th, metric, 0.3, **kw)
+        np.testing.assert_array_equal(idx, all_i[all_d < th])
+        np.testing.assert_array_equal(dist, all_d[all_d < th])

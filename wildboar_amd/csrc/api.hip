@@ -876,11 +876,20 @@ int scaled_profile_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const vo
     wbx_metric mm = *m;
     int64_t s_stride = total;
     RC(acc64_prepare(ctx, &mm, dtype, {{&S, 1, total, &s_stride}, {&X, n, t, &xstr}}, st));
+    const bool wrap = mm.kind != WBX_EUCLIDEAN && mm.kind != WBX_DTW;
+    if (wrap && mm.kind == WBX_ADTW && !(mm.p >= 0.0))
+        return fail(WBX_EUNSUPPORTED, "scaled adtw with a negative penalty is not served");
+    const double *w64 = nullptr;
+    const float *w32 = nullptr;
+    if (mm.kind == WBX_WDTW) RC(cached_weights(ctx, t, mm.g, &w64, &w32));
+    if (mm.kind == WBX_WDDTW && t > 2) RC(cached_weights(ctx, t - 2, mm.g, &w64, &w32));
     Carve cv;
     const size_t o_x = cv.take(stage_bytes(dtype, true, n, t));
     const size_t o_s = cv.take(stage_bytes(dtype, true, 1, total));
-    const size_t o_k = cv.take(scaled_profile_ws_bytes(mm.kind, hl.data(), n_s, total,
-                                                       paired ? 1 : 0, n, t));
+    const size_t o_k =
+        cv.take(wrap ? scaled_wrap_profile_ws_bytes(hl.data(), n_s, n, total, t)
+                     : scaled_profile_ws_bytes(mm.kind, hl.data(), n_s, total, paired ? 1 : 0,
+                                               n, t));
     void *wsp = nullptr;
     RC(ws_reserve(ctx, cv.total, &wsp));
     char *ws = (char *)wsp;
@@ -888,6 +897,11 @@ int scaled_profile_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const vo
     RC(stage<double>(dtype, X, n, t, xstr, ws, o_x, &x64, st));
     RC(stage<double>(dtype, S, 1, total, total, ws, o_s, &s64, st));
     ProfScope pf(st, KERN_SUBSEQ);
+    if (wrap)  // ScaledSubsequenceMetricWrap._matches: distance < threshold
+        return launch_scaled_wrap_profile(mm.kind, mm.r, w64, mm.kind == WBX_ADTW ? mm.p : 0.0,
+                                          s64.ptr, s_off, s_len, hl.data(), n_s, total, s_mean,
+                                          s_std, paired ? 1 : 0, x64.ptr, x64.stride, n, t, nwp,
+                                          true, threshold, prof, ws + o_k, st);
     return launch_scaled_profile(mm.kind, mm.r, s64.ptr, s_off, s_len, hl.data(), n_s, total,
                                  s_mean, s_std, paired ? 1 : 0, x64.ptr, x64.stride, n, t, nwp,
                                  threshold, prof, ws + o_k, g_policy.load() != 0, st);
@@ -1121,9 +1135,6 @@ int wbx_dev_scaled_subsequence_profile(const wbx_metric *metric, int dtype, cons
                                        int64_t t, int64_t nwp, double threshold, double *prof,
                                        void *stream) {
     RC(check_metric(metric));
-    if (metric->kind != WBX_EUCLIDEAN && metric->kind != WBX_DTW)
-        return fail(WBX_EUNSUPPORTED, "scaled subsequence metric kind %d is not served",
-                    metric->kind);
     RC(check_sizes(n, t));
     if (n_s < 1) return fail(WBX_EINVAL, "Subsequence y cannot be empty.");
     if (n_s > 0x7fffffff) return fail(WBX_EUNSUPPORTED, "too many patterns");
@@ -1190,7 +1201,7 @@ int wbx_dev_scaled_argmin_subsequence(const wbx_metric *metric, int dtype, const
         Carve cv;
         const size_t o_x = cv.take(stage_bytes(dtype, true, n, t));
         const size_t o_s = cv.take(stage_bytes(dtype, true, 1, total));
-        const size_t o_k = cv.take(scaled_argmin_ws_bytes(hl.data(), n, total, t));
+        const size_t o_k = cv.take(scaled_wrap_profile_ws_bytes(hl.data(), n, n, total, t));
         void *wsp = nullptr;
         rc = ws_reserve(ctx, cv.total, &wsp);
         char *ws = (char *)wsp;
@@ -1199,10 +1210,10 @@ int wbx_dev_scaled_argmin_subsequence(const wbx_metric *metric, int dtype, const
         if (!rc) rc = stage<double>(dtype, S, 1, total, total, ws, o_s, &s64, st);
         if (!rc) {
             ProfScope pf(st, KERN_SUBSEQ);
-            rc = launch_scaled_argmin_profile(mm.kind, mm.r, w64,
-                                              mm.kind == WBX_ADTW ? mm.p : 0.0, s64.ptr, s_off,
-                                              s_len, hl.data(), total, x64.ptr, x64.stride, n, t,
-                                              nwp, prof, ws + o_k, st);
+            rc = launch_scaled_wrap_profile(mm.kind, mm.r, w64, mm.kind == WBX_ADTW ? mm.p : 0.0,
+                                            s64.ptr, s_off, s_len, hl.data(), n, total, nullptr,
+                                            nullptr, 1, x64.ptr, x64.stride, n, t, nwp, false,
+                                            WBX_INF_D, prof, ws + o_k, st);
         }
     }
     if (!rc) rc = launch_argmin_replay(prof, n, nwp, k, nullptr, (char *)buf + prof_bytes, idx,

@@ -652,8 +652,9 @@ __global__ __launch_bounds__(64) void k_scaled_wrap(ScaledTask a, ScaledItems it
         }
         if (s < nwin) {
             const double d = sqrt(row[(int64_t)(Lw - 1) * 64]);
-            if (a.prof)
-                a.prof[a.out_row[ii] * a.nwp + s] = d;
+            if (a.prof)  // matching: accepted iff d < threshold, kept squared
+                a.prof[a.out_row[ii] * a.nwp + s] =
+                    a.values_are_distances ? d : (d < a.th ? d * d : WBX_INF_D);
             else
                 it.Tb[q * nwin + s] = d;
         }
@@ -1115,20 +1116,24 @@ int launch_scaled_wrap(int kind, double r, const double *weights, double penalty
     return WBX_OK;
 }
 
-// argmin_subsequence_distance(scale=True): prof[i * nwp + s] = the metric's
-// distance of window s of series i against pattern i (both normalised), +inf
-// past the pattern's last window.  Kinds: euclidean (the diagonal, band 1),
-// dtw, wdtw, ddtw, wddtw, adtw.
-size_t scaled_argmin_ws_bytes(const int64_t *hl, int64_t n, int64_t total, int64_t t) {
+// Window profiles of the wrapped / plain metrics on normalised buffers, for
+// series i against pattern (paired ? i : 0), +inf past the pattern's last
+// window: argmin_subsequence_distance(scale=True) (match = false: the
+// distance; patterns by fast_mean_std when s_mean is NULL) and the wrap's
+// _matches (ScaledSubsequenceMetricWrap._matches, _cdistance.pyx:547-615;
+// match = true: the squared distance where distance < threshold, else +inf).
+// Kinds: euclidean (the diagonal, band 1), dtw, wdtw, ddtw, wddtw, adtw.
+size_t scaled_wrap_profile_ws_bytes(const int64_t *hl, int64_t n_s, int64_t n, int64_t total,
+                                    int64_t t) {
     int64_t max_len = 1;
-    for (int64_t i = 0; i < n; ++i) max_len = hl[i] > max_len ? hl[i] : max_len;
+    for (int64_t i = 0; i < n_s; ++i) max_len = hl[i] > max_len ? hl[i] : max_len;
     const int64_t per = t * 8 + t * 16;
     int64_t cn = (int64_t)(kChunkCap / (size_t)per);
     cn = cn < 1 ? 1 : (cn < n ? cn : n);
     size_t b = 0;
     auto take = [&](size_t bytes) { b += (bytes + 255) & ~(size_t)255; };
     take((size_t)total * 8);                 // zs
-    take((size_t)n * 16);                    // pattern mean / std
+    take((size_t)n_s * 16);                  // pattern mean / std
     take((size_t)n * 8);                     // sel
     take((size_t)cn * (size_t)t * 8);        // gathered rows
     take((size_t)cn * (size_t)t * 16);       // window mean / std
@@ -1136,20 +1141,24 @@ size_t scaled_argmin_ws_bytes(const int64_t *hl, int64_t n, int64_t total, int64
     return b;
 }
 
-int launch_scaled_argmin_profile(int kind, double r, const double *weights, double penalty,
-                                 const double *S, const int64_t *s_off, const int64_t *s_len,
-                                 const int64_t *hl, int64_t total, const double *X,
-                                 int64_t xstride, int64_t n, int64_t t, int64_t nwp, double *prof,
-                                 char *ws, hipStream_t st) {
+int launch_scaled_wrap_profile(int kind, double r, const double *weights, double penalty,
+                               const double *S, const int64_t *s_off, const int64_t *s_len,
+                               const int64_t *hl, int64_t n_s, int64_t total,
+                               const double *s_mean, const double *s_std, int paired,
+                               const double *X, int64_t xstride, int64_t n, int64_t t,
+                               int64_t nwp, bool match, double threshold, double *prof, char *ws,
+                               hipStream_t st) {
     if (n <= 0) return WBX_OK;
-    if (t > 0x7fffffff) return fail(WBX_EUNSUPPORTED, "scaled argmin: series too long");
+    if (t > 0x7fffffff) return fail(WBX_EUNSUPPORTED, "scaled profile: series too long");
     const bool deriv = kind == WBX_DDTW || kind == WBX_WDDTW;
     std::vector<int64_t> sel((size_t)n), start;
     std::vector<int> lens;
+    auto len_of = [&](int64_t i) { return paired ? hl[i] : hl[0]; };
     for (int64_t i = 0; i < n; ++i) sel[i] = i;
-    std::stable_sort(sel.begin(), sel.end(), [&](int64_t a, int64_t b) { return hl[a] < hl[b]; });
+    std::stable_sort(sel.begin(), sel.end(),
+                     [&](int64_t a, int64_t b) { return len_of(a) < len_of(b); });
     for (int64_t k = 0; k < n; ++k) {
-        const int L = (int)hl[sel[k]];
+        const int L = (int)len_of(sel[k]);
         if (k == 0 || L != lens.back()) {
             start.push_back(k);
             lens.push_back(L);
@@ -1168,8 +1177,8 @@ int launch_scaled_argmin_profile(int kind, double r, const double *weights, doub
         return p;
     };
     double *zs = (double *)take((size_t)total * 8);
-    double *pmean = (double *)take((size_t)n * 16);
-    double *pstd = pmean + n;
+    double *pmean = (double *)take((size_t)n_s * 16);
+    double *pstd = pmean + n_s;
     int64_t *d_sel = (int64_t *)take((size_t)n * 8);
     double *rows = (double *)take((size_t)cn_max * (size_t)t * 8);
     double *wmean = (double *)take((size_t)cn_max * (size_t)t * 16);
@@ -1181,11 +1190,16 @@ int launch_scaled_argmin_profile(int kind, double r, const double *weights, doub
     WBX_HIP_CHECK(hipStreamSynchronize(st));  // sel ends with this call
     int rc = launch_fill_f64(prof, n * nwp, WBX_INF_D, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_fast_mean_std, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, S, s_off,
-                       s_len, n, pmean, pstd);
-    WBX_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_scaled_patterns, dim3((unsigned)n), dim3(256), 0, st, S, s_off, s_len,
-                       pmean, pstd, r, 0, zs, (double *)nullptr, (double *)nullptr);
+    if (!s_mean) {
+        // the scaled argmin normalises its patterns with fast_mean_std
+        hipLaunchKernelGGL(k_fast_mean_std, dim3((unsigned)((n_s + 63) / 64)), dim3(64), 0, st, S,
+                           s_off, s_len, n_s, pmean, pstd);
+        WBX_HIP_CHECK(hipGetLastError());
+        s_mean = pmean;
+        s_std = pstd;
+    }
+    hipLaunchKernelGGL(k_scaled_patterns, dim3((unsigned)n_s), dim3(256), 0, st, S, s_off, s_len,
+                       s_mean, s_std, r, 0, zs, (double *)nullptr, (double *)nullptr);
     WBX_HIP_CHECK(hipGetLastError());
     for (size_t g = 0; g < lens.size(); ++g) {
         const int L = lens[g];
@@ -1220,11 +1234,12 @@ int launch_scaled_argmin_profile(int kind, double r, const double *weights, doub
             a.wstd = wstd;
             a.zs = zs;
             a.s_off = s_off;
-            a.order = d_sel + c0;
+            a.order = paired ? d_sel + c0 : nullptr;
             a.gc = 1;
-            a.n_s = n;
+            a.n_s = n_s;
             a.out_row = d_sel + c0;
-            a.th = WBX_INF_D;
+            a.th = threshold;
+            a.values_are_distances = !match;
             a.nwp = nwp;
             a.prof = prof;
             ScaledItems it{};

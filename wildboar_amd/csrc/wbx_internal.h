@@ -264,12 +264,18 @@ int launch_scaled_wrap(int kind, double r, const double *weights, double penalty
 // argmin_subsequence_distance(scale=True): the distance of every window of
 // series i against pattern i, both normalised (pattern by fast_mean_std,
 // windows by incremental statistics), +inf past the pattern's last window.
-size_t scaled_argmin_ws_bytes(const int64_t *hl, int64_t n, int64_t total, int64_t t);
-int launch_scaled_argmin_profile(int kind, double r, const double *weights, double penalty,
-                                 const double *S, const int64_t *s_off, const int64_t *s_len,
-                                 const int64_t *hl, int64_t total, const double *X,
-                                 int64_t xstride, int64_t n, int64_t t, int64_t nwp, double *prof,
-                                 char *ws, hipStream_t st);
+// The same kernels give the wrap's matching profile (match = true: the
+// squared distance where the distance is below threshold, else +inf;
+// patterns i or 0 by `paired`, their mean / std given).
+size_t scaled_wrap_profile_ws_bytes(const int64_t *hl, int64_t n_s, int64_t n, int64_t total,
+                                    int64_t t);
+int launch_scaled_wrap_profile(int kind, double r, const double *weights, double penalty,
+                               const double *S, const int64_t *s_off, const int64_t *s_len,
+                               const int64_t *hl, int64_t n_s, int64_t total,
+                               const double *s_mean, const double *s_std, int paired,
+                               const double *X, int64_t xstride, int64_t n, int64_t t,
+                               int64_t nwp, bool match, double threshold, double *prof, char *ws,
+                               hipStream_t st);
 // Window profiles of the reference's scaled matching at `threshold`:
 // prof (n, nwp) holds the squared distance the matching records for window
 // s of series i against pattern (paired ? i : 0), +inf where it records

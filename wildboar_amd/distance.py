@@ -424,11 +424,9 @@ def pairwise_subsequence_distance(y, x, *, dim=0, metric="euclidean", metric_par
 
 
 # Scaled subsequence metrics served: "scaled_" + these (_distance.py:151-183);
-# the registry's other scaled names raise NotImplementedError.  The wrapped
-# ones (ScaledSubsequenceMetricWrap) are served by pairwise_subsequence_distance;
-# matching serves scaled_euclidean and scaled_dtw.
+# the registry's other scaled names raise NotImplementedError.
 _SCALED_SERVED = ("euclidean", "dtw", "wdtw", "ddtw", "wddtw", "adtw")
-_SCALED_MATCH_SERVED = ("euclidean", "dtw")
+_SCALED_MATCH_SERVED = _SCALED_SERVED
 _REFERENCE_ONLY = ("lcss", "edr", "twe", "msm", "erp", "manhattan", "minkowski", "chebyshev",
                    "cosine", "angular")
 
