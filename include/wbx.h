@@ -133,7 +133,8 @@ int wbx_subsequence_pairwise(const wbx_metric *metric, const double *S,
                              int64_t *idx);
 
 /* Replaces _argmin_subsequence_distance (distance/_cdistance.pyx:1572-1594,
- * non-scaled _ArgminSubsequenceDistance :1408-1434): pattern i (s_len[i]
+ * non-scaled _ArgminSubsequenceDistance :1408-1434; any metric kind, adtw
+ * with penalty >= 0): pattern i (s_len[i]
  * values at S + s_off[i]) against series i of X (n series), the k best
  * windows in the reference's heap order (Heap, utils/_misc.pyx:18-107).  idx
  * and dist are (n, k).  k <= t - longest pattern + 1. */
@@ -203,10 +204,13 @@ int wbx_scaled_argmin_subsequence(const wbx_metric *metric, const double *S,
 
 /* The per-series window costs behind _subsequence_match and
  * _paired_subsequence_match (distance/_cdistance.pyx:1065-1141, windows with
- * cost <= threshold^2 as dtw_subsequence_matches, _elastic.pyx:658-698, and
- * euclidean_distance_matches, _metric.pyx:762-795): prof (n, nwp), nwp = t -
- * shortest pattern + 1, holds the squared cost of window s of series i
- * against pattern (paired ? i : 0), +inf past the pattern's last window. */
+ * cost <= threshold^2 as dtw_subsequence_matches, _elastic.pyx:658-698,
+ * euclidean_distance_matches, _metric.pyx:762-795, ddtw_ / adtw_subsequence_
+ * matches, _elastic.pyx:737-866 -- band by the pattern's own length, nothing
+ * below length 3 for the derivative kinds, adtw penalty >= 0): prof (n, nwp),
+ * nwp = t - shortest pattern + 1, holds the squared cost of window s of
+ * series i against pattern (paired ? i : 0), +inf past the pattern's last
+ * window. */
 int wbx_subsequence_profile(const wbx_metric *metric, const double *S,
                             const int64_t *s_off, const int64_t *s_len,
                             int64_t n_s, int paired, const double *X, int64_t n,

@@ -401,9 +401,10 @@ def argmin_subsequence(shapelets, X, k=1, metric="dtw", r=1.0, g=0.05, dim=0):
     return idx, dist
 
 
-def subsequence_matches(s, x, threshold, metric="dtw", r=1.0, g=0.05):
+def subsequence_matches(s, x, threshold, metric="dtw", r=1.0, g=0.05, p=1.0):
     """(indices, distances) of the windows of the 1-D series x whose squared
     cost against s is <= threshold ** 2, in window order."""
+    g = _gp(metric, g, p)
     s = np.ascontiguousarray(s, dtype=np.float64)
     x = np.ascontiguousarray(x, dtype=np.float64)
     nw = x.shape[0] - s.shape[0] + 1

@@ -935,27 +935,49 @@ int64_t orc_subsequence_matches(int kind, double r, double g, const double *S, i
         }
         return n_matches;
     }
+    if (ORC_DERIVATIVE(kind) && L < 3) return 0; /* ddtw_subsequence_matches, :843-844 */
     double *cost = (double *)malloc(sizeof(double) * (size_t)(t + 1));
     double *cost_prev = (double *)malloc(sizeof(double) * (size_t)(t + 1));
-    double *weights = NULL;
-    if (kind == ORC_WDTW) {
-        weights = (double *)malloc(sizeof(double) * (size_t)t);
-        if (weights) orc_wdtw_weights(t, g, weights);
+    double *weights = NULL, *s_slope = NULL, *t_buffer = NULL;
+    int64_t nw = kind == ORC_WDTW ? t : (kind == ORC_WDDTW ? t - 2 : 0);
+    if (nw > 0) {
+        weights = (double *)malloc(sizeof(double) * (size_t)nw);
+        if (weights) orc_wdtw_weights(nw, g, weights);
     }
-    if (!cost || !cost_prev || (kind == ORC_WDTW && !weights)) {
-        free(cost); free(cost_prev); free(weights);
+    if (ORC_DERIVATIVE(kind)) {
+        s_slope = (double *)malloc(sizeof(double) * (size_t)L);
+        t_buffer = (double *)malloc(sizeof(double) * (size_t)L);
+    }
+    if (!cost || !cost_prev || (nw > 0 && !weights) ||
+        (ORC_DERIVATIVE(kind) && (!s_slope || !t_buffer))) {
+        free(cost); free(cost_prev); free(weights); free(s_slope); free(t_buffer);
         return -1;
     }
+    /* DtwSubsequenceMetric, DerivativeDtwSubsequenceMetric and
+     * AmercingDtwSubsequenceMetric._matches (_elastic.pyx:2493-2520,
+     * 2330-2350) band by the pattern's own length */
     const int64_t band = orc_compute_r(L, r);
+    if (ORC_DERIVATIVE(kind)) orc_average_slope(S, L, s_slope);
     for (int64_t i = 0; i + L <= t; i++) {
-        double dist = orc_dtw_distance(S, L, T + i, L, band, cost, cost_prev, weights, thr);
+        double dist;
+        if (ORC_DERIVATIVE(kind)) {
+            /* ddtw_subsequence_matches, _elastic.pyx:821-866 */
+            orc_average_slope(T + i, L, t_buffer);
+            dist = orc_dtw_distance(s_slope, L - 2, t_buffer, L - 2, band, cost, cost_prev,
+                                    weights, thr);
+        } else if (kind == ORC_ADTW) {
+            /* adtw_subsequence_matches, _elastic.pyx:737-777 */
+            dist = orc_adtw_distance(S, L, T + i, L, band, cost, cost_prev, g, thr);
+        } else {
+            dist = orc_dtw_distance(S, L, T + i, L, band, cost, cost_prev, weights, thr);
+        }
         if (dist <= thr) {
             matches[n_matches] = i;
             distances[n_matches] = sqrt(dist);
             n_matches++;
         }
     }
-    free(cost); free(cost_prev); free(weights);
+    free(cost); free(cost_prev); free(weights); free(s_slope); free(t_buffer);
     return n_matches;
 }
 

@@ -98,7 +98,8 @@ def test_derivative_short_series_and_validation():
     assert (idx == 0).all() and (dist == 0).all()
     X = rw(90, 4, 30)
     with pytest.raises(NotImplementedError):
-        wd.argmin_subsequence_distance([X[0, :5]] * 4, X, metric="adtw")
+        wd.argmin_subsequence_distance([X[0, :5]] * 4, X, metric="adtw",
+                                       metric_params={"p": -1.0})
     with pytest.raises(ValueError):
         wd.pairwise_distance(X, metric="adtw", metric_params={"p": float("-inf")})
     with pytest.raises(TypeError):
@@ -123,5 +124,41 @@ def test_subsequence_derivative_and_amercing_bitwise(metric, params, precision):
     od, oi = oracle.subsequence(pats, X, metric=metric, **okw(params))
     np.testing.assert_array_equal(d, od)
     np.testing.assert_array_equal(i, oi)
-    with pytest.raises(NotImplementedError):
-        wd.subsequence_match(pats[0], X, metric=metric, metric_params=params)
+
+
+@pytest.mark.parametrize("metric,params", [("ddtw", {"r": 0.1}), ("ddtw", {"r": 1.0}),
+                                           ("wddtw", {"r": 0.2, "g": 0.1}),
+                                           ("adtw", {"r": 0.1, "p": 0.5}),
+                                           ("adtw", {"r": 1.0, "p": 0.0})])
+def test_derivative_and_amercing_matching_and_argmin_bitwise(metric, params):
+    X = rw(92, 6, 100)
+    s = X[2, 20:50].copy()
+    p = okw(params)
+    for threshold in (1.0, 3.0, np.inf):
+        idx, dist = wd.subsequence_match(s, X, threshold, metric=metric, metric_params=params,
+                                         return_distance=True)
+        for q in range(X.shape[0]):
+            oi, od = oracle.subsequence_matches(s, X[q], threshold, metric, p["r"], p["g"],
+                                                p["p"])
+            if oi.size == 0:
+                assert idx[q] is None
+            else:
+                np.testing.assert_array_equal(idx[q], oi)
+                np.testing.assert_array_equal(dist[q], od)
+    pats = [X[q, 3 + q:3 + q + L].copy() for q, L in enumerate((2, 9, 30, 17, 44, 6))]
+    idx, dist = wd.paired_subsequence_match(pats, X, 2.0, metric=metric, metric_params=params,
+                                            return_distance=True)
+    for q in range(X.shape[0]):
+        oi, od = oracle.subsequence_matches(pats[q], X[q], 2.0, metric, p["r"], p["g"], p["p"])
+        if oi.size == 0:
+            assert idx[q] is None
+        else:
+            np.testing.assert_array_equal(idx[q], oi)
+            np.testing.assert_array_equal(dist[q], od)
+    for k in (1, 3):
+        gi, gd = wd.argmin_subsequence_distance(pats, X, k=k, metric=metric,
+                                                metric_params=params, return_distance=True)
+        oi, od = oracle.argmin_subsequence(pats, X, k=k, metric=metric, r=p["r"],
+                                           g=p["p"] if metric == "adtw" else p["g"])
+        np.testing.assert_array_equal(gi, oi)
+        np.testing.assert_array_equal(gd, od)

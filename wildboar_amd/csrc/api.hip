@@ -65,14 +65,10 @@ int check_metric(const wbx_metric *m) {
     return WBX_OK;
 }
 
-// Subsequence profiles serve euclidean, dtw and wdtw; the subsequence scan
-// also ddtw, wddtw and adtw.
+// Every metric kind is served for subsequences.
 int check_subsequence_metric(const wbx_metric *m, bool scan = false) {
-    int rc = check_metric(m);
-    if (rc) return rc;
-    if (m->kind > WBX_WDTW && !scan)
-        return fail(WBX_EUNSUPPORTED, "metric kind %d is not served for subsequences", m->kind);
-    return WBX_OK;
+    (void)scan;
+    return check_metric(m);
 }
 
 hipStream_t pick_stream(DeviceCtx *ctx, void *stream) {
@@ -718,6 +714,33 @@ int profile_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
     const double *w64 = nullptr;
     const float *w32 = nullptr;
     if (mm.kind == WBX_WDTW) RC(cached_weights(ctx, t, mm.g, &w64, &w32));
+    if (mm.kind > WBX_WDTW) {
+        // DerivativeDtw- / WeightedDerivativeDtw- / AmercingDtwSubsequenceMetric
+        // ._matches (_elastic.pyx:2493-2520, 2330-2350; ddtw_ / adtw_subsequence_
+        // matches :821-866, :737-777): the full squared cost of every window,
+        // band by the pattern's own length, WDDTW weights sized t - 2; their
+        // row-minimum abandon never drops a window whose cost is within the
+        // threshold (penalty >= 0)
+        if (mm.kind == WBX_ADTW && !(mm.p >= 0.0))
+            return fail(WBX_EUNSUPPORTED, "adtw window profiles need a penalty >= 0");
+        if (mm.kind == WBX_WDDTW && t > 2) RC(cached_weights(ctx, t - 2, mm.g, &w64, &w32));
+        Carve cv;
+        const size_t o_x = cv.take(stage_bytes(dtype, true, n, t));
+        const size_t o_s = cv.take(stage_bytes(dtype, true, 1, total));
+        const size_t o_k = cv.take(scaled_wrap_profile_ws_bytes(hl.data(), n_s, n, total, t));
+        void *wsp = nullptr;
+        RC(ws_reserve(ctx, cv.total, &wsp));
+        char *ws = (char *)wsp;
+        Staged<double> x64, s64;
+        RC(stage<double>(dtype, X, n, t, xstr, ws, o_x, &x64, st));
+        RC(stage<double>(dtype, S, 1, total, total, ws, o_s, &s64, st));
+        ProfScope pf(st, KERN_SUBSEQ);
+        const bool deriv = mm.kind != WBX_ADTW;
+        return launch_scaled_wrap_profile(mm.kind, mm.r, w64, deriv ? 0.0 : mm.p, s64.ptr, s_off,
+                                          s_len, hl.data(), n_s, total, NORM_NONE, nullptr,
+                                          nullptr, paired, x64.ptr, x64.stride, n, t, nwp, 2,
+                                          deriv ? 2 : 0, WBX_INF_D, prof, ws + o_k, st);
+    }
     Carve cv;
     const size_t o_x = cv.take(stage_bytes(dtype, true, n, t));
     const size_t o_s = cv.take(stage_bytes(dtype, true, 1, total));
@@ -899,9 +922,10 @@ int scaled_profile_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const vo
     ProfScope pf(st, KERN_SUBSEQ);
     if (wrap)  // ScaledSubsequenceMetricWrap._matches: distance < threshold
         return launch_scaled_wrap_profile(mm.kind, mm.r, w64, mm.kind == WBX_ADTW ? mm.p : 0.0,
-                                          s64.ptr, s_off, s_len, hl.data(), n_s, total, s_mean,
-                                          s_std, paired ? 1 : 0, x64.ptr, x64.stride, n, t, nwp,
-                                          true, threshold, prof, ws + o_k, st);
+                                          s64.ptr, s_off, s_len, hl.data(), n_s, total,
+                                          NORM_GIVEN, s_mean, s_std, paired ? 1 : 0, x64.ptr,
+                                          x64.stride, n, t, nwp, 1, 0, threshold, prof, ws + o_k,
+                                          st);
     return launch_scaled_profile(mm.kind, mm.r, s64.ptr, s_off, s_len, hl.data(), n_s, total,
                                  s_mean, s_std, paired ? 1 : 0, x64.ptr, x64.stride, n, t, nwp,
                                  threshold, prof, ws + o_k, g_policy.load() != 0, st);
@@ -1151,10 +1175,17 @@ int wbx_dev_scaled_subsequence_profile(const wbx_metric *metric, int dtype, cons
     return rc ? rc : rc2;
 }
 
-int wbx_dev_scaled_argmin_subsequence(const wbx_metric *metric, int dtype, const void *S,
-                                      const int64_t *s_off, const int64_t *s_len, const void *X,
-                                      int64_t n, int64_t x_stride, int64_t t, int64_t k,
-                                      int64_t *idx, double *dist, void *stream) {
+}  // extern "C"
+
+namespace {
+// argmin_subsequence_distance through the wrapped-metric kernels: norm
+// NORM_FAST for scale=True (_ScaledArgminSubsequenceDistance), NORM_NONE for
+// the unscaled ddtw / wddtw / adtw (_ArgminSubsequenceDistance,
+// _cdistance.pyx:1408-1434, the metric's _eadistance: derivative band by
+// min(x_len - 2, y_len - 2)).
+int wrap_argmin(const wbx_metric *metric, int dtype, const void *S, const int64_t *s_off,
+                const int64_t *s_len, const void *X, int64_t n, int64_t x_stride, int64_t t,
+                int64_t k, int64_t *idx, double *dist, void *stream, int norm) {
     RC(check_metric(metric));
     RC(check_sizes(n, t));
     if (!S || !s_off || !s_len || !X || !idx || !dist) return fail(WBX_EINVAL, "NULL argument");
@@ -1211,9 +1242,9 @@ int wbx_dev_scaled_argmin_subsequence(const wbx_metric *metric, int dtype, const
         if (!rc) {
             ProfScope pf(st, KERN_SUBSEQ);
             rc = launch_scaled_wrap_profile(mm.kind, mm.r, w64, mm.kind == WBX_ADTW ? mm.p : 0.0,
-                                            s64.ptr, s_off, s_len, hl.data(), n, total, nullptr,
-                                            nullptr, 1, x64.ptr, x64.stride, n, t, nwp, false,
-                                            WBX_INF_D, prof, ws + o_k, st);
+                                            s64.ptr, s_off, s_len, hl.data(), n, total, norm,
+                                            nullptr, nullptr, 1, x64.ptr, x64.stride, n, t, nwp,
+                                            0, 0, WBX_INF_D, prof, ws + o_k, st);
         }
     }
     if (!rc) rc = launch_argmin_replay(prof, n, nwp, k, nullptr, (char *)buf + prof_bytes, idx,
@@ -1222,6 +1253,17 @@ int wbx_dev_scaled_argmin_subsequence(const wbx_metric *metric, int dtype, const
     (void)hipStreamSynchronize(st);
     (void)hipFree(buf);
     return rc ? rc : rc2;
+}
+}  // namespace
+
+extern "C" {
+
+int wbx_dev_scaled_argmin_subsequence(const wbx_metric *metric, int dtype, const void *S,
+                                      const int64_t *s_off, const int64_t *s_len, const void *X,
+                                      int64_t n, int64_t x_stride, int64_t t, int64_t k,
+                                      int64_t *idx, double *dist, void *stream) {
+    return wrap_argmin(metric, dtype, S, s_off, s_len, X, n, x_stride, t, k, idx, dist, stream,
+                       NORM_FAST);
 }
 
 int wbx_dev_scaled_subsequence_pairwise(const wbx_metric *metric, int dtype, const void *S,
@@ -1473,6 +1515,9 @@ int wbx_dev_argmin_subsequence(const wbx_metric *metric, int dtype, const void *
                                int64_t n, int64_t x_stride, int64_t t, int64_t k, int64_t *idx,
                                double *dist, void *stream) {
     RC(check_subsequence_metric(metric));
+    if (metric->kind > WBX_WDTW)
+        return wrap_argmin(metric, dtype, S, s_off, s_len, X, n, x_stride, t, k, idx, dist,
+                           stream, NORM_NONE);
     RC(check_sizes(n, t));
     if (!S || !s_off || !s_len || !X || !idx || !dist) return fail(WBX_EINVAL, "NULL argument");
     if (k < 1) return fail(WBX_EINVAL, "k == %lld, must be >= 1", (long long)k);

@@ -66,8 +66,9 @@ __global__ __launch_bounds__(256) void k_scaled_patterns(
     const int L = (int)len[j];
     const int64_t o = off[j];
     const double *__restrict__ s = S + o;
-    const double mu = s_mean[j];
-    const double sd = s_std[j] != 0.0 ? s_std[j] : 1.0;
+    // no statistics: the pattern itself ((s - 0) / 1 == s)
+    const double mu = s_mean ? s_mean[j] : 0.0;
+    const double sd = s_std ? (s_std[j] != 0.0 ? s_std[j] : 1.0) : 1.0;
     const int w = warp_width(L, r);
     for (int k = threadIdx.x; k < L; k += 256) {
         zs[o + k] = (s[k] - mu) / sd;
@@ -106,7 +107,10 @@ struct ScaledTask {
     int64_t nwp;
     double *prof;
     bool values_are_distances;   // accept: the per-window values are already sqrt'd
+    int prof_mode;               // k_scaled_wrap profiles: PROF_DIST / PROF_WRAP_MATCH / PROF_COST
 };
+
+enum { PROF_DIST = 0, PROF_WRAP_MATCH = 1, PROF_COST = 2 };
 
 __device__ __forceinline__ int64_t task_id() {
     return (int64_t)blockIdx.x + (int64_t)blockIdx.y * gridDim.x;
@@ -605,7 +609,9 @@ __global__ __launch_bounds__(64) void k_scaled_wrap(ScaledTask a, ScaledItems it
         const double *__restrict__ z = a.zs + a.s_off[pj];
         const int s = b0 + (int)threadIdx.x;
         const int sc = s < nwin ? s : nwin - 1;
-        const double mu = a.wmean[ii * nwin + sc], sd = a.wstd[ii * nwin + sc];
+        // no statistics: the window itself ((x - 0) / 1 == x)
+        const double mu = a.wmean ? a.wmean[ii * nwin + sc] : 0.0;
+        const double sd = a.wstd ? a.wstd[ii * nwin + sc] : 1.0;
         const double *__restrict__ xw = a.X + ii * a.xstride + sc;
         for (int k = 0; k < L; ++k) zx[(int64_t)k * 64] = (xw[k] - mu) / sd;
         if (w.deriv)  // average_slope (_elastic.pyx:3220-3225), in place
@@ -651,12 +657,16 @@ __global__ __launch_bounds__(64) void k_scaled_wrap(ScaledTask a, ScaledItems it
             }
         }
         if (s < nwin) {
-            const double d = sqrt(row[(int64_t)(Lw - 1) * 64]);
-            if (a.prof)  // matching: accepted iff d < threshold, kept squared
-                a.prof[a.out_row[ii] * a.nwp + s] =
-                    a.values_are_distances ? d : (d < a.th ? d * d : WBX_INF_D);
-            else
+            const double c = row[(int64_t)(Lw - 1) * 64];
+            const double d = sqrt(c);
+            if (a.prof) {
+                double v = d;                                            // PROF_DIST
+                if (a.prof_mode == PROF_WRAP_MATCH) v = d < a.th ? d * d : WBX_INF_D;
+                if (a.prof_mode == PROF_COST) v = c;
+                a.prof[a.out_row[ii] * a.nwp + s] = v;
+            } else {
                 it.Tb[q * nwin + s] = d;
+            }
         }
     }
 }
@@ -1143,11 +1153,11 @@ size_t scaled_wrap_profile_ws_bytes(const int64_t *hl, int64_t n_s, int64_t n, i
 
 int launch_scaled_wrap_profile(int kind, double r, const double *weights, double penalty,
                                const double *S, const int64_t *s_off, const int64_t *s_len,
-                               const int64_t *hl, int64_t n_s, int64_t total,
+                               const int64_t *hl, int64_t n_s, int64_t total, int norm,
                                const double *s_mean, const double *s_std, int paired,
                                const double *X, int64_t xstride, int64_t n, int64_t t,
-                               int64_t nwp, bool match, double threshold, double *prof, char *ws,
-                               hipStream_t st) {
+                               int64_t nwp, int mode, int band_add, double threshold,
+                               double *prof, char *ws, hipStream_t st) {
     if (n <= 0) return WBX_OK;
     if (t > 0x7fffffff) return fail(WBX_EUNSUPPORTED, "scaled profile: series too long");
     const bool deriv = kind == WBX_DDTW || kind == WBX_WDDTW;
@@ -1190,13 +1200,16 @@ int launch_scaled_wrap_profile(int kind, double r, const double *weights, double
     WBX_HIP_CHECK(hipStreamSynchronize(st));  // sel ends with this call
     int rc = launch_fill_f64(prof, n * nwp, WBX_INF_D, st);
     if (rc) return rc;
-    if (!s_mean) {
+    if (norm == NORM_FAST) {
         // the scaled argmin normalises its patterns with fast_mean_std
         hipLaunchKernelGGL(k_fast_mean_std, dim3((unsigned)((n_s + 63) / 64)), dim3(64), 0, st, S,
                            s_off, s_len, n_s, pmean, pstd);
         WBX_HIP_CHECK(hipGetLastError());
         s_mean = pmean;
         s_std = pstd;
+    } else if (norm == NORM_NONE) {
+        s_mean = nullptr;
+        s_std = nullptr;
     }
     hipLaunchKernelGGL(k_scaled_patterns, dim3((unsigned)n_s), dim3(256), 0, st, S, s_off, s_len,
                        s_mean, s_std, r, 0, zs, (double *)nullptr, (double *)nullptr);
@@ -1213,7 +1226,7 @@ int launch_scaled_wrap_profile(int kind, double r, const double *weights, double
         if (kind == WBX_EUCLIDEAN) {
             w.band = 1;  // the diagonal: euclidean_distance's sequential sum
         } else {
-            const double v = floor((double)Lw * r);
+            const double v = floor((double)(Lw + band_add) * r);
             w.band = (int)(v > 1.0 ? v : 1.0);  // _compute_r, _elastic.pyx:1924-1925
         }
         for (int64_t c0 = start[g]; c0 < start[g + 1]; c0 += cn_max) {
@@ -1221,17 +1234,19 @@ int launch_scaled_wrap_profile(int kind, double r, const double *weights, double
             hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((cn * t + 255) / 256)), dim3(256), 0,
                                st, X, xstride, d_sel + c0, cn, (int)t, rows);
             WBX_HIP_CHECK(hipGetLastError());
-            hipLaunchKernelGGL(k_incstats_rows, dim3((unsigned)((cn + 63) / 64)), dim3(64), 0, st,
-                               rows, cn, t, (int)t, L, wmean, wstd);
-            WBX_HIP_CHECK(hipGetLastError());
+            if (norm != NORM_NONE) {
+                hipLaunchKernelGGL(k_incstats_rows, dim3((unsigned)((cn + 63) / 64)), dim3(64), 0,
+                                   st, rows, cn, t, (int)t, L, wmean, wstd);
+                WBX_HIP_CHECK(hipGetLastError());
+            }
             ScaledTask a{};
             a.X = rows;
             a.xstride = t;
             a.t = (int)t;
             a.L = L;
             a.cn = cn;
-            a.wmean = wmean;
-            a.wstd = wstd;
+            a.wmean = norm != NORM_NONE ? wmean : nullptr;
+            a.wstd = norm != NORM_NONE ? wstd : nullptr;
             a.zs = zs;
             a.s_off = s_off;
             a.order = paired ? d_sel + c0 : nullptr;
@@ -1239,7 +1254,7 @@ int launch_scaled_wrap_profile(int kind, double r, const double *weights, double
             a.n_s = n_s;
             a.out_row = d_sel + c0;
             a.th = threshold;
-            a.values_are_distances = !match;
+            a.prof_mode = mode;
             a.nwp = nwp;
             a.prof = prof;
             ScaledItems it{};

@@ -269,13 +269,19 @@ int launch_scaled_wrap(int kind, double r, const double *weights, double penalty
 // patterns i or 0 by `paired`, their mean / std given).
 size_t scaled_wrap_profile_ws_bytes(const int64_t *hl, int64_t n_s, int64_t n, int64_t total,
                                     int64_t t);
+// norm: NORM_NONE (the raw series and patterns), NORM_GIVEN (s_mean / s_std,
+// windows by incremental statistics), NORM_FAST (patterns by fast_mean_std).
+// mode: 0 the distance sqrt(cost), 1 the wrap's match (distance^2 where the
+// distance < threshold, else +inf), 2 the squared cost.  The band is
+// _compute_r(Lw + band_add, r) with Lw the compared length.
+enum { NORM_NONE = 0, NORM_GIVEN = 1, NORM_FAST = 2 };
 int launch_scaled_wrap_profile(int kind, double r, const double *weights, double penalty,
                                const double *S, const int64_t *s_off, const int64_t *s_len,
-                               const int64_t *hl, int64_t n_s, int64_t total,
+                               const int64_t *hl, int64_t n_s, int64_t total, int norm,
                                const double *s_mean, const double *s_std, int paired,
                                const double *X, int64_t xstride, int64_t n, int64_t t,
-                               int64_t nwp, bool match, double threshold, double *prof, char *ws,
-                               hipStream_t st);
+                               int64_t nwp, int mode, int band_add, double threshold,
+                               double *prof, char *ws, hipStream_t st);
 // Window profiles of the reference's scaled matching at `threshold`:
 // prof (n, nwp) holds the squared distance the matching records for window
 // s of series i against pattern (paired ? i : 0), +inf where it records
