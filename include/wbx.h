@@ -332,6 +332,17 @@ int wbx_dev_lb_kim(const double *Y, int64_t ny, int64_t y_stride, const double *
 int wbx_dev_dtw_path(const double *X, int64_t x_stride, int64_t tx, const double *Y,
                      int64_t y_stride, int64_t ty, int64_t n, int64_t w, const double *weights,
                      int32_t *rows, void *stream);
+/* One step of the stochastic subgradient mean, _ssg_dtw_average
+ * (distance/dtw.py:597-647, the body of its inner loop, :618-633): with
+ * rows the path of (mean, x) from wbx_dev_dtw_path (per-row column ranges),
+ * z[m] = sum over the path columns c of row m, in order, of
+ * mean[m] - x[c] * weight, then mean[m] -= learning_rate * z[m], in place.
+ * mean (tm values), rows (tm x 2 int32) and x (tx values) are device
+ * pointers. */
+int wbx_dev_dtw_ssg_update(double *mean, int64_t tm, const int32_t *rows,
+                           const double *x, int64_t tx, double weight,
+                           double learning_rate, void *stream);
+
 /* One majorize-minimize step of _mm_dtw_average (distance/dtw.py:640-666):
  * aligns `mean` (tm values) with each X row (n rows of tx), then
  * mean_out[m] = sum over rows i and aligned columns x (in order) of

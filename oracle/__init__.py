@@ -498,6 +498,50 @@ def mm_dtw_average(X, mean, r=1.0, tol=1e-5, max_epoch=50, sample_weight=None, g
     return mean, cost
 
 
+def ssg_dtw_average(X, mean, r=1.0, max_epoch=50, max_stable=5, learning_rate=0.1, decay=0.9,
+                    sample_weight=None, g=None, random_state=0):
+    """_ssg_dtw_average, distance/dtw.py:597-647, over the alignment matrix
+    and backtrack above (dtw_alignment's warp size; wdtw_alignment's weights
+    when g is given) with numpy's RandomState shuffling the series order.
+    Returns (best mean, its cost)."""
+    X = np.asarray(X, dtype=np.float64)
+    mean = np.array(mean, dtype=np.float64)
+    rs = np.random.RandomState(random_state) if not hasattr(random_state, "shuffle") \
+        else random_state
+
+    def costfn(m):
+        if g is None:
+            d = pairwise(m[None, :], X, metric="dtw", r=r)[0]
+        else:
+            d = pairwise(m[None, :], X, metric="wdtw", r=r, g=g)[0]
+        return np.mean(d) if sample_weight is None else np.average(d, weights=sample_weight)
+
+    best_mean, min_cost, n_stable = None, np.inf, 0
+    order = np.arange(X.shape[0])
+    z = np.empty(mean.shape[0])
+    w_band = warp_size(mean.shape[0], r, X.shape[1])
+    cw = None if g is None else jeong_weight(max(mean.shape[0], X.shape[1]), g)
+    for epoch in range(max_epoch):
+        if n_stable > max_stable:
+            break
+        rs.shuffle(order)
+        for i, o in enumerate(order):
+            z.fill(0)
+            pm, px = mapping(dtw_alignment(mean, X[o], w_band, cw))
+            w = 1.0 if sample_weight is None else sample_weight[o]
+            for m, x in zip(pm, px):
+                z[m] += mean[m] - X[o, x] * w
+            mean -= learning_rate * z
+            if epoch == 0:
+                learning_rate = decay**i * learning_rate
+        cost = costfn(mean)
+        if cost < min_cost:
+            min_cost, n_stable, best_mean = cost, 0, mean.copy()
+        else:
+            n_stable += 1
+    return best_mean, min_cost
+
+
 def dtw_envelop(x, r):
     """(lower, upper) of a 1-D series for an integer warp r (the deque pass)."""
     x = np.ascontiguousarray(x, dtype=np.float64)

@@ -61,8 +61,6 @@ def test_dtw_average_mm_bitwise(r):
     om, oc = oracle.mm_dtw_average(X, init, r=r, g=0.1, max_epoch=6)
     np.testing.assert_array_equal(mean, om)
     assert cost == oc
-    with pytest.raises(NotImplementedError):
-        dtw_average(X, method="ssg")
     with pytest.raises(ValueError):
         dtw_average(X[:1])
 
@@ -156,3 +154,21 @@ def test_kmedoids_on_gpu_distances_equals_precomputed_oracle(algorithm, init):
                                                   r=0.3).argmin(axis=1))
     if algorithm == "pam":
         assert sorted(m // 15 for m in a.medoid_indices_) == [0, 1, 2]
+
+
+@pytest.mark.parametrize("r,g,weighted", [(0.2, None, False), (1.0, None, True), (0.3, 0.1, False)])
+def test_ssg_average_bitwise(r, g, weighted):
+    from wildboar_amd.dtw import dtw_average
+
+    rng = np.random.default_rng(31)
+    X = np.cumsum(rng.standard_normal((7, 40)), axis=1)
+    sw = rng.uniform(0.5, 2.0, 7) if weighted else None
+    init = X[3].copy()
+    got, cost = dtw_average(X, r=r, g=g, sample_weight=sw, init=init, method="ssg",
+                            max_epoch=6, max_stable=2, learning_rate=0.2, decay=0.85,
+                            return_cost=True, random_state=5)
+    want, wcost = oracle.ssg_dtw_average(X, init, r=r, max_epoch=6, max_stable=2,
+                                         learning_rate=0.2, decay=0.85, sample_weight=sw, g=g,
+                                         random_state=5)
+    np.testing.assert_array_equal(got, want)
+    assert cost == wcost

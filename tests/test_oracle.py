@@ -584,3 +584,28 @@ def test_derivative_and_amercing_subsequence():
     eu = oracle.subsequence(pats, X, metric="euclidean")
     np.testing.assert_array_equal(ai[0], eu[0])
     np.testing.assert_array_equal(ai[1], eu[1])
+
+
+def test_ssg_average_on_the_diagonal_is_the_convex_step():
+    # r = 0 gives a warp size of 1, the diagonal path at equal lengths: each
+    # step is mean <- mean - lr * (mean - w * x_o) (distance/dtw.py:618-633),
+    # series in RandomState's shuffled order, the epoch-0 rate compounded by
+    # decay**i
+    X = np.cumsum(np.random.default_rng(15).standard_normal((6, 20)), axis=1)
+    mean0 = X[2].copy()
+    got, cost = oracle.ssg_dtw_average(X, mean0, r=0.0, max_epoch=4, max_stable=10,
+                                       learning_rate=0.3, decay=0.8, random_state=7)
+    rs = np.random.RandomState(7)
+    mean, lr, order = mean0.copy(), 0.3, np.arange(6)
+    best, best_cost = None, np.inf
+    for epoch in range(4):
+        rs.shuffle(order)
+        for i, o in enumerate(order):
+            mean = mean - lr * (mean - X[o] * 1.0)
+            if epoch == 0:
+                lr = 0.8**i * lr
+        c = np.mean(oracle.pairwise(mean[None], X, metric="dtw", r=0.0)[0])
+        if c < best_cost:
+            best, best_cost = mean.copy(), c
+    np.testing.assert_array_equal(got, best)
+    assert cost == best_cost

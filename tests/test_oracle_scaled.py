@@ -258,3 +258,4 @@ def test_scaled_wrap_matches_are_the_strictly_closer_windows(metric, kw):
         idx, dist = oracle.scaled_subsequence_matches(s, X[q], th, metric, 0.3, **kw)
         np.testing.assert_array_equal(idx, all_i[all_d < th])
         np.testing.assert_array_equal(dist, all_d[all_d < th])
+

@@ -35,6 +35,7 @@ EXPORTS = (
     "wbx_scaled_subsequence_pairwise", "wbx_dev_scaled_subsequence_pairwise",
     "wbx_scaled_subsequence_profile", "wbx_dev_scaled_subsequence_profile",
     "wbx_scaled_argmin_subsequence", "wbx_dev_scaled_argmin_subsequence",
+    "wbx_dev_dtw_ssg_update",
 )
 
 
@@ -125,6 +126,7 @@ def load():
             "wbx_dev_scaled_subsequence_profile": (i32, [m, i32, p, p, p, p, p, i64, i32, p, i64,
                                                          i64, i64, i64, d, p, p]),
             "wbx_scaled_argmin_subsequence": (i32, [m, p, p, p, i64, p, i64, i64, i64, i64, p, p]),
+            "wbx_dev_dtw_ssg_update": (i32, [p, i64, p, p, i64, d, d, p]),
             "wbx_dev_scaled_argmin_subsequence": (i32, [m, i32, p, p, p, p, i64, i64, i64, i64, p,
                                                         p, p]),
         }

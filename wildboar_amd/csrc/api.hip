@@ -1344,6 +1344,21 @@ int wbx_dev_dtw_path(const double *X, int64_t x_stride, int64_t tx, const double
     return rc ? rc : rc2;
 }
 
+int wbx_dev_dtw_ssg_update(double *mean, int64_t tm, const int32_t *rows, const double *x,
+                           int64_t tx, double weight, double learning_rate, void *stream) {
+    if (!mean || !rows || !x) return fail(WBX_EINVAL, "NULL argument");
+    RC(check_sizes(1, tm));
+    RC(check_sizes(1, tx));
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    DevGuard g(ctx->device);
+    hipStream_t st = pick_stream(ctx, stream);
+    RC(begin(ctx, st));
+    int rc = launch_ssg_update(rows, x, weight, learning_rate, (int)tm, mean, st);
+    int rc2 = end(ctx, st);
+    return rc ? rc : rc2;
+}
+
 int wbx_dev_dtw_mm_update(const double *mean, int64_t tm, const double *X, int64_t x_stride,
                           int64_t tx, int64_t n, int64_t w, const double *weights,
                           const double *sample_weight, double *mean_out, void *stream) {

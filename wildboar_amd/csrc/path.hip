@@ -16,6 +16,8 @@
 //   thread m adds, for series i = 0 .. n-1 in order and the path columns x of
 //   row m in increasing order, V[m] += w and z[m] += X[i, x] * w, then
 //   mean[m] = z[m] / V[m] -- the reference's order of additions exactly.
+// k_ssg_update: one step of _ssg_dtw_average (distance/dtw.py:597-647) for
+//   one series along its path.
 #pragma clang fp contract(off)
 
 #include "block_dtw.h"
@@ -135,7 +137,31 @@ __global__ __launch_bounds__(256) void k_mm_update(const int32_t *__restrict__ r
     mean[m] = z / V;
 }
 
+// _ssg_dtw_average's step (distance/dtw.py:618-633): thread m sums, over the
+// path columns x of row m in increasing order, z += mean[m] - x[x] * w, then
+// mean[m] -= lr * z (every z from the mean before the step).
+__global__ __launch_bounds__(256) void k_ssg_update(const int32_t *__restrict__ rows,
+                                                    const double *__restrict__ x, double w,
+                                                    double lr, int lm, double *__restrict__ mean) {
+    const int m = blockIdx.x * 256 + threadIdx.x;
+    if (m >= lm) return;
+    const int32_t lo = rows[2 * m], hi = rows[2 * m + 1];
+    const double mm = mean[m];
+    double z = 0.0;
+    for (int32_t c = lo; c <= hi; ++c) z += mm - x[c] * w;
+    mean[m] = mm - lr * z;
+}
+
 }  // namespace
+
+int launch_ssg_update(const int32_t *rows, const double *x, double w, double lr, int lm,
+                      double *mean, hipStream_t st) {
+    if (lm <= 0) return WBX_OK;
+    hipLaunchKernelGGL(k_ssg_update, dim3((unsigned)((lm + 255) / 256)), dim3(256), 0, st, rows, x,
+                       w, lr, lm, mean);
+    WBX_HIP_CHECK(hipGetLastError());
+    return WBX_OK;
+}
 
 int launch_path_rows(const double *X, int64_t xstride, const double *Y, int64_t ystride,
                      int64_t n, int lx, int ly, int w, const double *weights,

@@ -305,6 +305,9 @@ int launch_path_rows(const double *X, int64_t xstride, const double *Y, int64_t 
 // launch_path_rows with the mean as X), in the reference's order.
 int launch_mm_update(const int32_t *rows, const double *X, int64_t xstride, int64_t n, int lm,
                      const double *sample_weight, double *mean, hipStream_t st);
+// One stochastic-subgradient step along the path rows of (mean, x), in place.
+int launch_ssg_update(const int32_t *rows, const double *x, double w, double lr, int lm,
+                      double *mean, hipStream_t st);
 
 // ----------------------------------------------------------- lower bounds
 int launch_envelope(const double *X, int64_t n, int64_t stride, int t, int r, double *lower,

@@ -195,8 +195,11 @@ def dtw_average(X, *, r=1.0, g=None, sample_weight=None, init="random", method="
     the values aligned to it -- in the reference's order of additions -- and
     stops when the mean DTW distance changes by less than ``tol``.  Results
     equal the reference's.  With ``g`` the paths are those of
-    ``wdtw_alignment`` and the cost is the mean WDTW distance.  The ``"ssg"``
-    method is not served."""
+    ``wdtw_alignment`` and the cost is the mean WDTW distance.  ``"ssg"`` is
+    the stochastic subgradient mean (``_ssg_dtw_average`` :597-647): series in
+    the order ``random_state`` shuffles them, one GPU path and in-place step
+    per series, the epoch-0 learning-rate decay as the reference compounds it,
+    and the lowest-cost mean kept."""
     import numpy as np
     from sklearn.utils import check_random_state
     from sklearn.utils.validation import _check_sample_weight
@@ -223,9 +226,7 @@ def dtw_average(X, *, r=1.0, g=None, sample_weight=None, init="random", method="
 
         if isinstance(g, bool) or not isinstance(g, numbers.Real) or not g > 0:
             raise ValueError(f"g == {g!r}, must be a float in (0, inf).")
-    if method == "ssg":
-        raise NotImplementedError("dtw_average(method='ssg') is not served")
-    if method != "mm":
+    if method not in ("mm", "ssg"):
         raise ValueError("method must be 'mm' or 'ssg', got %r" % method)
 
     dev = torch.device("cuda", torch.cuda.current_device())
@@ -245,6 +246,36 @@ def dtw_average(X, *, r=1.0, g=None, sample_weight=None, init="random", method="
         from .counterfactual import jeong_weight
 
         cell_w = torch.from_numpy(jeong_weight(max(mean.shape[0], X.shape[1]), g)).to(dev)
+    if method == "ssg":
+        md = torch.from_numpy(np.ascontiguousarray(mean, dtype=np.float64)).to(dev)
+        order = np.arange(X.shape[0])
+        best_mean, min_cost, n_stable = None, np.inf, 0
+        stream = torch.cuda.current_stream().cuda_stream
+        for epoch in range(max_epoch):
+            if n_stable > max_stable:
+                if verbose:
+                    print(f"Completed at epoch={epoch} with cost={min_cost}.")
+                break
+            random_state.shuffle(order)
+            for i, o in enumerate(order):
+                rows = _path_rows(md, Xd[o], w, cell_w)
+                wgt = 1.0 if sample_weight is None else float(sample_weight[o])
+                with torch.cuda.device(dev):
+                    _lib.check(lib.wbx_dev_dtw_ssg_update(
+                        md.data_ptr(), md.shape[0], rows.data_ptr(), Xd[o].data_ptr(),
+                        Xd.shape[1], wgt, float(learning_rate), stream))
+                if epoch == 0:
+                    learning_rate = decay**i * learning_rate
+            mean = md.cpu().numpy()
+            cost = costfn(mean)
+            if cost < min_cost:
+                if verbose:
+                    print(f"New min cost={cost} at epoch={epoch} with "
+                          f"learning_rate={learning_rate}.")
+                min_cost, n_stable, best_mean = cost, 0, mean.copy()
+            else:
+                n_stable += 1
+        return (best_mean, min_cost) if return_cost else best_mean
     cost = costfn(mean)
     for epoch in range(max_epoch):
         md = torch.from_numpy(mean).to(dev)
