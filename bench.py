@@ -342,7 +342,20 @@ def run_fp64(torch, dev):
     got = device.pairwise(A[:4], Bm, metric="dtw", r=0.1, precision="fp64").cpu().numpy()
     want = oracle.pairwise(Xh[:4, :256], Xh[1000:, :256], metric="dtw", r=0.1)
     res["pairwise_dtw_t256_r0.1"]["mismatch_4_rows"] = int((got != want).sum())
-    # 5. z-normalised scan (scale=True, the UCR-suite cascade reproduced):
+    # 5. derivative and amercing DTW, pairwise 1000 x 1000, t = 256, r = 0.1
+    both("pairwise_ddtw_t256_r0.1",
+         lambda: device.pairwise(A, Bm, metric="ddtw", r=0.1, precision="fp64"),
+         1e6 * square_band_w_cells(254, band_w_of(256, 0.1)))
+    got = device.pairwise(A[:4], Bm, metric="ddtw", r=0.1, precision="fp64").cpu().numpy()
+    want = oracle.pairwise(Xh[:4, :256], Xh[1000:, :256], metric="ddtw", r=0.1)
+    res["pairwise_ddtw_t256_r0.1"]["mismatch_4_rows"] = int((got != want).sum())
+    both("pairwise_adtw_t256_r0.1_p0.5",
+         lambda: device.pairwise(A, Bm, metric="adtw", r=0.1, p=0.5, precision="fp64"),
+         1e6 * band_cells(256, 256, 0.1))
+    got = device.pairwise(A[:4], Bm, metric="adtw", r=0.1, p=0.5, precision="fp64").cpu().numpy()
+    want = oracle.pairwise(Xh[:4, :256], Xh[1000:, :256], metric="adtw", r=0.1, p=0.5)
+    res["pairwise_adtw_t256_r0.1_p0.5"]["mismatch_4_rows"] = int((got != want).sum())
+    # 6. z-normalised scan (scale=True, the UCR-suite cascade reproduced):
     # 64 patterns, L U[16, 128], against 1000 series; the block baseline is
     # the scratch-row kernel
     Ls = rng.integers(16, 129, 64)
@@ -358,6 +371,16 @@ def run_fp64(torch, dev):
     res["scaled_dtw_subsequence_64x1000"]["mismatch_3_rows"] = int(
         (gd.cpu().numpy() != od).sum() + (gi.cpu().numpy() != oi).sum())
     return res
+
+
+def band_w_of(n, r):
+    """_compute_r (distance/_elastic.pyx:1924-1925): max(floor(n * r), 1)."""
+    return max(int(np.floor(n * r)), 1)
+
+
+def square_band_w_cells(n, w):
+    """Cells of dtw_distance's band on n x n with width w: |i - j| < w."""
+    return sum(min(n, i + w) - max(0, i - w + 1) for i in range(n))
 
 
 def square_band_cells(L, r):

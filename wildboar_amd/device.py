@@ -31,22 +31,23 @@ def _rows(t, name):
     return t
 
 
-def _metric(metric, r, g, precision):
+def _metric(metric, r, g, precision, p=1.0):
     kind = _lib.KINDS[metric]
     prec = _lib.PRECISIONS[precision]
     if kind != _lib.EUCLIDEAN:
         r = _check_r(r)
     if kind == _lib.WDTW:
         g = _check_g(g)
-    return _lib.metric_struct(kind, prec, r, g)
+    return _lib.metric_struct(kind, prec, r, g, p)
 
 
 def _stream():
     return torch.cuda.current_stream().cuda_stream
 
 
-def pairwise(X, Y=None, *, metric="dtw", r=1.0, g=0.05, precision="fp32", out=None):
-    """(nx, ny) float64 distances; ``Y=None`` gives the symmetric singleton matrix."""
+def pairwise(X, Y=None, *, metric="dtw", r=1.0, g=0.05, precision="fp32", out=None, p=1.0):
+    """(nx, ny) float64 distances; ``Y=None`` gives the symmetric singleton matrix.
+    ``p``: the amercing penalty of metric="adtw"."""
     X = _rows(X, "X")
     if Y is not None:
         Y = _rows(Y, "Y")
@@ -56,7 +57,7 @@ def pairwise(X, Y=None, *, metric="dtw", r=1.0, g=0.05, precision="fp32", out=No
     ny = nx if Y is None else Y.shape[0]
     if out is None:
         out = torch.empty((nx, ny), dtype=torch.float64, device=X.device)
-    m = _metric(metric, r, g, precision)
+    m = _metric(metric, r, g, precision, p)
     with torch.cuda.device(X.device):
         _lib.check(_lib.load().wbx_dev_pairwise(
             m, _dtype(X), X.data_ptr(), nx, X.stride(0), X.shape[1],
