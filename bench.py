@@ -342,6 +342,15 @@ def run_fp64(torch, dev):
     got = device.pairwise(A[:4], Bm, metric="dtw", r=0.1, precision="fp64").cpu().numpy()
     want = oracle.pairwise(Xh[:4, :256], Xh[1000:, :256], metric="dtw", r=0.1)
     res["pairwise_dtw_t256_r0.1"]["mismatch_4_rows"] = int((got != want).sum())
+    # the default float64 precision on a 512-long band of 101 slots (the widest
+    # register class, 104), 500 x 500
+    A5, B5 = X[:500, :512].contiguous(), X[1000:1500, :512].contiguous()
+    both("pairwise_dtw_t512_r0.1",
+         lambda: device.pairwise(A5, B5, metric="dtw", r=0.1, precision="fp64"),
+         2.5e5 * band_cells(512, 512, 0.1))
+    got = device.pairwise(A5[:2], B5, metric="dtw", r=0.1, precision="fp64").cpu().numpy()
+    want = oracle.pairwise(Xh[:2, :512], Xh[1000:1500, :512], metric="dtw", r=0.1)
+    res["pairwise_dtw_t512_r0.1"]["mismatch_2_rows"] = int((got != want).sum())
     # 5. derivative and amercing DTW, pairwise 1000 x 1000, t = 256, r = 0.1
     both("pairwise_ddtw_t256_r0.1",
          lambda: device.pairwise(A, Bm, metric="ddtw", r=0.1, precision="fp64"),

@@ -162,3 +162,18 @@ def test_derivative_and_amercing_matching_and_argmin_bitwise(metric, params):
                                            g=p["p"] if metric == "adtw" else p["g"])
         np.testing.assert_array_equal(gi, oi)
         np.testing.assert_array_equal(gd, od)
+
+
+@pytest.mark.parametrize("tx,ty,r,metric", [(300, 300, 0.1, "dtw"), (400, 400, 0.12, "dtw"),
+                                            (512, 512, 0.1, "dtw"), (300, 340, 0.1, "dtw"),
+                                            (512, 512, 0.1, "adtw"), (300, 300, 0.2, "ddtw")])
+def test_fp64_wide_register_classes_bitwise(tx, ty, r, metric):
+    # bands of 57..104 slots run the wide float64 register classes
+    X = rw(93, 3, tx)
+    Y = rw(94, 7, ty)
+    params = {"r": r, "p": 0.5} if metric == "adtw" else {"r": r}
+    got = wd.pairwise_distance(X, Y, metric=metric, metric_params=params)
+    np.testing.assert_array_equal(got, oracle.pairwise(X, Y, metric=metric, **okw(params)))
+    if tx == ty:
+        got = wd.paired_distance(X, Y[:3], metric=metric, metric_params=params)
+        np.testing.assert_array_equal(got, oracle.paired(X, Y[:3], metric=metric, **okw(params)))

@@ -229,7 +229,7 @@ int run_pairs(const wbx_metric *m, const PairSpec &ps, Series<T> xs, Series<T> y
         }
     } else {
         if (g_policy.load() == 0 && !xs.lens && !ys.lens && lane64_ok(ctx_current(), weights) &&
-            lane64_class(make_band(xs.len, ys.len, m->r, w_fixed).B, weights != nullptr)) {
+            lane64_pairs_class(make_band(xs.len, ys.len, m->r, w_fixed).B, weights != nullptr)) {
             ProfScope pf(st, KERN_LANE);
             return launch_lane64_pairs(ps, xs, ys, m->r, weights, sq, st, w_fixed, pen);
         }

@@ -230,8 +230,13 @@ __global__ __launch_bounds__(64) void k_subseq_lane64(Shapelets sh, Series<doubl
 
 constexpr int kMaxBP = 56;    // DTW (56: 256 VGPRs plus AGPR spill slots, no scratch)
 constexpr int kMaxBPW = 32;   // WDTW (per-slot weights in registers too)
+// Pairwise / paired DTW and ADTW only: wider rows live in VGPRs + AGPRs
+// (64..80 without scratch; 88..104 spill a few slots to scratch), one wave
+// per SIMD like class 56 -- still far ahead of the block kernel.
+constexpr int kMaxBPWide = 104;
 
 #define WBX_LANE64_CLASSES(X) X(8) X(16) X(24) X(32) X(40) X(48) X(56)
+#define WBX_LANE64_WIDE(X) X(64) X(72) X(80) X(88) X(96) X(104)
 
 }  // namespace
 
@@ -242,12 +247,17 @@ int lane64_class(int B, bool weighted) {
     return BP <= (weighted ? kMaxBPW : kMaxBP) ? BP : 0;
 }
 
+int lane64_pairs_class(int B, bool weighted) {
+    const int BP = (B + 7) / 8 * 8;
+    return BP <= (weighted ? kMaxBPW : kMaxBPWide) ? BP : 0;
+}
+
 int launch_lane64_pairs(const PairSpec &ps, Series<double> xs, Series<double> ys, double r,
                         const double *weights, double *sq, hipStream_t st, int w_fixed,
                         const double *penalty) {
     if (ps.n_pairs <= 0) return WBX_OK;
     const Band bd = make_band(xs.len, ys.len, r, w_fixed);
-    const int BP = lane64_class(bd.B, weights != nullptr);
+    const int BP = lane64_pairs_class(bd.B, weights != nullptr);
     if (!BP || xs.lens || ys.lens) return fail(WBX_EUNSUPPORTED, "lane64: unsupported geometry");
     if (penalty && weights) return fail(WBX_EUNSUPPORTED, "adtw takes no weights");
     const int kf = bd.ly - bd.lx + bd.off;
@@ -266,10 +276,21 @@ int launch_lane64_pairs(const PairSpec &ps, Series<double> xs, Series<double> ys
             hipLaunchKernelGGL((k_lane64_pairs<C, 4, false, false>), grid, dim3(64), 0, st, ps, \
                                xs, ys, bd.off, bd.B, kf, weights, wn, 0.0, sq);                 \
         break;
+#define WBX_L64_PAIRS_WIDE(C)                                                                   \
+    case C:                                                                                     \
+        if (penalty)                                                                            \
+            hipLaunchKernelGGL((k_lane64_pairs<C, 4, false, true>), grid, dim3(64), 0, st, ps,  \
+                               xs, ys, bd.off, bd.B, kf, weights, wn, pen, sq);                 \
+        else                                                                                    \
+            hipLaunchKernelGGL((k_lane64_pairs<C, 4, false, false>), grid, dim3(64), 0, st, ps, \
+                               xs, ys, bd.off, bd.B, kf, weights, wn, 0.0, sq);                 \
+        break;
     switch (BP) {
         WBX_LANE64_CLASSES(WBX_L64_PAIRS)
+        WBX_LANE64_WIDE(WBX_L64_PAIRS_WIDE)
         default: return fail(WBX_EUNSUPPORTED, "lane64: band %d", bd.B);
     }
+#undef WBX_L64_PAIRS_WIDE
 #undef WBX_L64_PAIRS
     WBX_HIP_CHECK(hipGetLastError());
     return WBX_OK;

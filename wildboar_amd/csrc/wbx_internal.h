@@ -322,6 +322,8 @@ int launch_lb_kim(const double *Y, int64_t ny, int64_t y_stride, const double *X
 // gives the register class (a multiple of 8) for band width B, or 0 when B is
 // too wide.  Weighted launches need every weight > 0.
 int lane64_class(int B, bool weighted);
+// The class of a pairwise / paired band (unweighted rows up to 104 slots).
+int lane64_pairs_class(int B, bool weighted);
 int lane64_cap(bool weighted);
 int launch_lane64_pairs(const PairSpec &ps, Series<double> xs, Series<double> ys, double r,
                         const double *weights, double *sq, hipStream_t st, int w_fixed = 0,
