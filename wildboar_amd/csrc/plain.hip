@@ -7,7 +7,7 @@
 //  k_euclid_tile  16x16 pairs per block, LDS tiles, each lane sums its pair
 //                 sequentially in the reference's order (exact in FP64).
 //  k_finish       sqrt + scatter into the driver's output layout.
-//  k_argmin_replay  _ArgMinBatch's heap acceptance rule, one lane per x row.
+//  k_argmin_replay_wave  _ArgMinBatch's heap acceptance rule, one wave per x row.
 //
 // Arithmetic order per cell is the reference's (distance/_elastic.pyx:925-928):
 //     cost = min(min(up, left), diag) + (v * v) * w,   v = x_i - y_j
@@ -318,49 +318,69 @@ __device__ void heap_shift_up(HeapEl *h, int64_t pos, int64_t endpos) {
     heap_shift_down(h, startpos, pos);
 }
 
-__global__ __launch_bounds__(64) void k_argmin_replay(const double *__restrict__ D, int64_t nx,
-                                                      int64_t ny, int64_t k,
-                                                      const double *__restrict__ lb,
-                                                      HeapEl *__restrict__ heaps,
-                                                      int64_t *__restrict__ idx,
-                                                      double *__restrict__ dist) {
-    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (i >= nx) return;
+// One wave per row: 64 values are read together and only those below the
+// threshold at the start of the group (and not skipped by the bound) are
+// replayed, in index order, by lane 0; the threshold only falls, so every
+// value the sequential rule accepts is among them and the heap sees exactly
+// the reference's pushes.
+__global__ __launch_bounds__(64) void k_argmin_replay_wave(const double *__restrict__ D,
+                                                           int64_t nx, int64_t ny, int64_t k,
+                                                           const double *__restrict__ lb,
+                                                           HeapEl *__restrict__ heaps,
+                                                           int64_t *__restrict__ idx,
+                                                           double *__restrict__ dist) {
+    const int64_t i = (int64_t)blockIdx.x + (int64_t)blockIdx.y * gridDim.x;
+    if (i >= nx) return;  // uniform: one row per wave
+    const int lane = threadIdx.x;
     HeapEl *h = heaps + i * k;
     const double *row = D + i * ny;
     double threshold = WBX_INF_D;
     int64_t n = 0;
-    for (int64_t j = 0; j < ny; ++j) {
-        if (lb && lb[i * ny + j] >= threshold) continue;
-        const double v = row[j];
-        if (v < threshold) {
-            HeapEl e{v, j};
-            if (n == 0) {
-                h[0] = e;
-                n = 1;
-            } else if (n < k) {
-                h[n] = e;
-                n += 1;
-                heap_shift_down(h, 0, n - 1);
-            } else if (h[0].value > v) {
-                h[0] = e;
-                heap_shift_up(h, 0, k);
+    for (int64_t b0 = 0; b0 < ny; b0 += 64) {
+        const int64_t j = b0 + lane;
+        const bool in = j < ny;
+        const double v = in ? row[j] : WBX_INF_D;
+        const double l = (lb && in) ? lb[i * ny + j] : -WBX_INF_D;
+        const bool cand = in && v < threshold && !(l >= threshold);
+        uint64_t mask = __ballot(cand);
+        while (mask) {
+            const int q = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            const double vq = __shfl(v, q), lq = __shfl(l, q);
+            if (lq >= threshold || !(vq < threshold)) continue;
+            double top = 0.0;
+            if (lane == 0) {
+                HeapEl e{vq, b0 + q};
+                if (n == 0) {
+                    h[0] = e;
+                } else if (n < k) {
+                    h[n] = e;
+                    heap_shift_down(h, 0, n);
+                } else if (h[0].value > vq) {
+                    h[0] = e;
+                    heap_shift_up(h, 0, k);
+                }
+                top = h[0].value;
             }
-            threshold = (n == k) ? h[0].value : WBX_INF_D;
+            n = n < k ? n + 1 : n;
+            top = __shfl(top, 0);
+            threshold = (n == k) ? top : WBX_INF_D;
         }
     }
-    for (int64_t q = 0; q < k; ++q) {
-        idx[i * k + q] = q < n ? h[q].index : 0;
-        dist[i * k + q] = q < n ? h[q].value : 0.0;
-    }
+    if (lane == 0)
+        for (int64_t q = 0; q < k; ++q) {
+            idx[i * k + q] = q < n ? h[q].index : 0;
+            dist[i * k + q] = q < n ? h[q].value : 0.0;
+        }
 }
 
 int launch_argmin_replay(const double *D, int64_t nx, int64_t ny, int64_t k,
                          const double *lower_bound, void *heaps, int64_t *idx, double *dist,
                          hipStream_t st) {
     if (nx <= 0) return WBX_OK;
-    hipLaunchKernelGGL(k_argmin_replay, dim3((unsigned)((nx + 63) / 64)), dim3(64), 0, st, D, nx,
-                       ny, k, lower_bound, (HeapEl *)heaps, idx, dist);
+    const int64_t gx = nx < 65535 ? nx : 65535;
+    hipLaunchKernelGGL(k_argmin_replay_wave, dim3((unsigned)gx, (unsigned)((nx + gx - 1) / gx)),
+                       dim3(64), 0, st, D, nx, ny, k, lower_bound, (HeapEl *)heaps, idx, dist);
     WBX_HIP_CHECK(hipGetLastError());
     return WBX_OK;
 }

@@ -3,7 +3,7 @@
 // contraction off).  They serve
 //   argmin_subsequence_distance: _ArgminSubsequenceDistance
 //       (distance/_cdistance.pyx:1408-1434), a heap over the windows in
-//       order, replayed by k_argmin_replay on the square roots;
+//       order, replayed by k_argmin_replay_wave on the square roots;
 //   subsequence_match: dtw_subsequence_matches (distance/_elastic.pyx:658-698)
 //       and euclidean_distance_matches (distance/_metric.pyx:762-795), the
 //       windows whose cost is <= threshold^2.
