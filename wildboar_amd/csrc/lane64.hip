@@ -40,7 +40,24 @@ __device__ __forceinline__ double ldy64(const double *__restrict__ Y, int j, int
     return (unsigned)j < (unsigned)ly ? v : WBX_INF_D;
 }
 
-__device__ __forceinline__ double dmin(double a, double b) { return b < a ? b : a; }
+// The reference's min, b < a ? b : a.  F: as one v_min_f64, equal here --
+// the operands are never NaN (finite inputs; +inf only meets finite values
+// or +inf, the penalty is > -inf) and never -0 (costs are sums of squares,
+// +0 + -0 is +0), where the two could differ.  The wide rows keep the
+// compare-and-select form: with v_min_f64 the scheduler keeps more values
+// live and spills them to scratch.
+template <bool F>
+__device__ __forceinline__ double dmin(double a, double b) {
+    if constexpr (F)
+        return __builtin_fmin(a, b);
+    else
+        return b < a ? b : a;
+}
+
+template <int BP, bool AD>
+constexpr bool fast_min() {
+    return AD ? BP <= 56 : BP <= 80;
+}
 
 // One row of BP cells.  wk[k] is the weight of slot k (unused when !WT).
 // AD: the amercing recursion, min(min(up + p, left + p), diag) + v * v
@@ -55,9 +72,9 @@ __device__ __forceinline__ void row64(double (&row)[BP], const double *yw, doubl
         const double up = (k + 1 < BP) ? row[k + 1] : WBX_INF_D;
         double m;
         if constexpr (AD)
-            m = dmin(dmin(up + p, left + p), row[k]);
+            m = dmin<fast_min<BP, AD>()>(dmin<fast_min<BP, AD>()>(up + p, left + p), row[k]);
         else
-            m = dmin(dmin(up, left), row[k]);
+            m = dmin<fast_min<BP, AD>()>(dmin<fast_min<BP, AD>()>(up, left), row[k]);
         const double v = xi - yw[k];
         double term = v * v;
         if constexpr (WT) term = term * wk[k];

@@ -47,8 +47,12 @@ __host__ __device__ inline int warp_width(int L, double r) {
     return (int)floor((double)L * r);
 }
 
-__device__ __forceinline__ double dmin(double a, double b) { return b < a ? b : a; }
-__device__ __forceinline__ double dmax(double a, double b) { return b > a ? b : a; }
+// min / max as v_min_f64 / v_max_f64: equal to the reference's
+// b < a ? b : a here -- no operand is NaN (finite inputs, window std > 0,
+// pattern std 0 read as 1.0) and none is -0 (costs and bounds are sums of
+// squares).
+__device__ __forceinline__ double dmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ double dmax(double a, double b) { return __builtin_fmax(a, b); }
 __device__ __forceinline__ double sqd(double a, double b) {
     const double s = a - b;
     return s * s;
@@ -223,12 +227,21 @@ __device__ double scaled_bounds(const ScaledTask &a, const double *__restrict__ 
         if (k > rw) cb[(int64_t)(nc + k - rw - 1) * 64] = d;
     }
     const bool one = lk > lk2;
+    const double *src = one ? cb : cb + (int64_t)nc * 64;  // may alias cb
+    // suffix sums from the last position down, 8 loads issued before their
+    // sums (each slot is read before it is rewritten)
     double acc = 0.0;
-    for (int k = L - 1; k > rw; --k) {
-        const int m = k - rw - 1;
-        const double v = one ? cb[(int64_t)m * 64] : cb[(int64_t)(nc + m) * 64];
-        acc = k == L - 1 ? v : acc + v;
-        cb[(int64_t)m * 64] = acc;
+    constexpr int KB = 8;
+    for (int m0 = nc - 1; m0 >= 0; m0 -= KB) {
+        double v[KB];
+#pragma unroll
+        for (int u = 0; u < KB; ++u) v[u] = m0 - u >= 0 ? src[(int64_t)(m0 - u) * 64] : 0.0;
+#pragma unroll
+        for (int u = 0; u < KB; ++u) {
+            if (m0 - u < 0) break;
+            acc = m0 - u == nc - 1 ? v[u] : acc + v[u];
+            cb[(int64_t)(m0 - u) * 64] = acc;
+        }
     }
     const double T = L >= 3 ? scaled_kim(z, zx, L) : 0.0;
     return dmax(dmax(T, lk), lk2);
@@ -322,16 +335,17 @@ __device__ __forceinline__ void scaled_cost(const double *__restrict__ z, int L,
         yw[W - 1] = ldz(zx, i + 1 - off + W - 1, L);
     }
     for (int i0 = 1 + rem; i0 < L; i0 += U) {
-        double zv[U], yn[U];
+        double zv[U], yn[U], cv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             zv[u] = z[i0 + u];
             yn[u] = ldz(zx, i0 + U - off + W - U + u, L);
+            cv[u] = i0 + u < last ? cb[(int64_t)(i0 + u) * 64] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             rm = srow<BP>(row, yw + u, zv[u], pen);
-            if (i0 + u < last) check(rm + cb[(int64_t)(i0 + u) * 64]);
+            if (i0 + u < last) check(rm + cv[u]);
         }
 #pragma unroll
         for (int m = 0; m < W - U; ++m) yw[m] = yw[m + U];
