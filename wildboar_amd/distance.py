@@ -136,9 +136,9 @@ def _adtw(r=1.0, p=1.0, **params):
 
 
 # The subset of the reference's _METRICS registry (_distance.py:185-204)
-# served here.  pairwise_subsequence_distance serves all of them, and with
-# scale=True scaled_euclidean and scaled_dtw; the window-profile functions
-# serve euclidean, dtw and wdtw and raise NotImplementedError for the rest.
+# served here, by every distance and subsequence function; with scale=True
+# (or a "scaled_" name) the subsequence functions serve the scaled forms of
+# all six.
 METRICS = {"euclidean": _euclidean, "dtw": _dtw, "wdtw": _wdtw, "ddtw": _ddtw,
            "wddtw": _wddtw, "adtw": _adtw}
 
