@@ -177,3 +177,32 @@ def test_fp64_wide_register_classes_bitwise(tx, ty, r, metric):
     if tx == ty:
         got = wd.paired_distance(X, Y[:3], metric=metric, metric_params=params)
         np.testing.assert_array_equal(got, oracle.paired(X, Y[:3], metric=metric, **okw(params)))
+
+
+@pytest.mark.parametrize("metric,params", [("dtw", {"r": 1.0}), ("adtw", {"r": 1.0, "p": 0.5}),
+                                           ("ddtw", {"r": 1.0})])
+def test_fp64_wide_classes_in_subsequence_scans_and_profiles(metric, params):
+    # the default r = 1 with patterns of length 29..52 gives 57..104 slots
+    X = rw(95, 5, 160)
+    pats = [X[q, 7:7 + L].copy() for q, L in enumerate((30, 41, 52, 36, 45))]
+    d, i = wd.pairwise_subsequence_distance(pats, X, metric=metric, metric_params=params,
+                                            return_index=True)
+    od, oi = oracle.subsequence(pats, X, metric=metric, **okw(params))
+    np.testing.assert_array_equal(d, od)
+    np.testing.assert_array_equal(i, oi)
+    p = okw(params)
+    idx, dist = wd.subsequence_match(pats[1], X, 4.0, metric=metric, metric_params=params,
+                                     return_distance=True)
+    for q in range(X.shape[0]):
+        qi, qd = oracle.subsequence_matches(pats[1], X[q], 4.0, metric, p["r"], p["g"], p["p"])
+        if qi.size == 0:
+            assert idx[q] is None
+        else:
+            np.testing.assert_array_equal(idx[q], qi)
+            np.testing.assert_array_equal(dist[q], qd)
+    gi, gd = wd.argmin_subsequence_distance(pats, X, k=3, metric=metric, metric_params=params,
+                                            return_distance=True)
+    oi, od = oracle.argmin_subsequence(pats, X, k=3, metric=metric, r=p["r"],
+                                       g=p["p"] if metric == "adtw" else p["g"])
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gd, od)

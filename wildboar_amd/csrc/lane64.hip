@@ -245,11 +245,11 @@ __global__ __launch_bounds__(64) void k_subseq_lane64(Shapelets sh, Series<doubl
     }
 }
 
-constexpr int kMaxBP = 56;    // DTW (56: 256 VGPRs plus AGPR spill slots, no scratch)
 constexpr int kMaxBPW = 32;   // WDTW (per-slot weights in registers too)
-// Pairwise / paired DTW and ADTW only: wider rows live in VGPRs + AGPRs
-// (64..80 without scratch; 88..104 spill a few slots to scratch), one wave
-// per SIMD like class 56 -- still far ahead of the block kernel.
+// Unweighted DTW and ADTW (pairs, windows, profiles): wider rows live in
+// VGPRs + AGPRs (64..80 without scratch; 88..104 spill a few slots to
+// scratch), one wave per SIMD like class 56 -- still far ahead of the block
+// kernel.
 constexpr int kMaxBPWide = 104;
 
 #define WBX_LANE64_CLASSES(X) X(8) X(16) X(24) X(32) X(40) X(48) X(56)
@@ -257,17 +257,14 @@ constexpr int kMaxBPWide = 104;
 
 }  // namespace
 
-int lane64_cap(bool weighted) { return weighted ? kMaxBPW : kMaxBP; }
+int lane64_cap(bool weighted) { return weighted ? kMaxBPW : kMaxBPWide; }
 
 int lane64_class(int B, bool weighted) {
     const int BP = (B + 7) / 8 * 8;
-    return BP <= (weighted ? kMaxBPW : kMaxBP) ? BP : 0;
-}
-
-int lane64_pairs_class(int B, bool weighted) {
-    const int BP = (B + 7) / 8 * 8;
     return BP <= (weighted ? kMaxBPW : kMaxBPWide) ? BP : 0;
 }
+
+int lane64_pairs_class(int B, bool weighted) { return lane64_class(B, weighted); }
 
 int launch_lane64_pairs(const PairSpec &ps, Series<double> xs, Series<double> ys, double r,
                         const double *weights, double *sq, hipStream_t st, int w_fixed,
@@ -317,7 +314,7 @@ int launch_lane64_pairs(const PairSpec &ps, Series<double> xs, Series<double> ys
 static void class_range(int min_len, int max_len, double r, bool weighted, int *lo, int *hi) {
     *lo = lane64_class(2 * band_w(min_len, min_len, r) - 1, weighted);
     *hi = lane64_class(2 * band_w(max_len, max_len, r) - 1, weighted);
-    if (*lo && !*hi) *hi = weighted ? kMaxBPW : kMaxBP;
+    if (*lo && !*hi) *hi = weighted ? kMaxBPW : kMaxBPWide;
 }
 
 int launch_profile_lane64(double r, Shapelets sh, int paired, int min_len, int max_len,
@@ -340,10 +337,17 @@ int launch_profile_lane64(double r, Shapelets sh, int paired, int min_len, int m
             hipLaunchKernelGGL((k_profile_lane64<C, 4, false>), grid, dim3(64), 0, st, sh,      \
                                paired, xs, n, nwp, nblk, r, weights, prof);                     \
         break;
+#define WBX_L64_PROF_WIDE(C)                                                                    \
+    case C:                                                                                     \
+        hipLaunchKernelGGL((k_profile_lane64<C, 4, false>), grid, dim3(64), 0, st, sh, paired,  \
+                           xs, n, nwp, nblk, r, weights, prof);                                 \
+        break;
         switch (BP) {
             WBX_LANE64_CLASSES(WBX_L64_PROF)
+            WBX_LANE64_WIDE(WBX_L64_PROF_WIDE)
             default: return fail(WBX_EUNSUPPORTED, "lane64: class %d", BP);
         }
+#undef WBX_L64_PROF_WIDE
 #undef WBX_L64_PROF
         WBX_HIP_CHECK(hipGetLastError());
     }
@@ -372,10 +376,21 @@ int launch_subseq_lane64(double r, Shapelets sh, int min_len, int max_len, Serie
             hipLaunchKernelGGL((k_subseq_lane64<C, 4, false, false>), grid, dim3(64), 0, st,    \
                                sh, xs, n, r, weights, band_add, 0.0, dist, idx);                \
         break;
+#define WBX_L64_SUB_WIDE(C)                                                                     \
+    case C:                                                                                     \
+        if (penalty)                                                                            \
+            hipLaunchKernelGGL((k_subseq_lane64<C, 4, false, true>), grid, dim3(64), 0, st, sh, \
+                               xs, n, r, weights, band_add, pen, dist, idx);                    \
+        else                                                                                    \
+            hipLaunchKernelGGL((k_subseq_lane64<C, 4, false, false>), grid, dim3(64), 0, st,    \
+                               sh, xs, n, r, weights, band_add, 0.0, dist, idx);                \
+        break;
         switch (BP) {
             WBX_LANE64_CLASSES(WBX_L64_SUB)
+            WBX_LANE64_WIDE(WBX_L64_SUB_WIDE)
             default: return fail(WBX_EUNSUPPORTED, "lane64: class %d", BP);
         }
+#undef WBX_L64_SUB_WIDE
 #undef WBX_L64_SUB
         WBX_HIP_CHECK(hipGetLastError());
     }
