@@ -358,6 +358,12 @@ def run_fp64(torch, dev):
     got = device.pairwise(A[:4], Bm, metric="ddtw", r=0.1, precision="fp64").cpu().numpy()
     want = oracle.pairwise(Xh[:4, :256], Xh[1000:, :256], metric="ddtw", r=0.1)
     res["pairwise_ddtw_t256_r0.1"]["mismatch_4_rows"] = int((got != want).sum())
+    both("pairwise_wdtw_t256_r0.1_g0.05",
+         lambda: device.pairwise(A, Bm, metric="wdtw", r=0.1, g=0.05, precision="fp64"),
+         1e6 * band_cells(256, 256, 0.1))
+    got = device.pairwise(A[:4], Bm, metric="wdtw", r=0.1, g=0.05, precision="fp64").cpu().numpy()
+    want = oracle.pairwise(Xh[:4, :256], Xh[1000:, :256], metric="wdtw", r=0.1, g=0.05)
+    res["pairwise_wdtw_t256_r0.1_g0.05"]["mismatch_4_rows"] = int((got != want).sum())
     both("pairwise_adtw_t256_r0.1_p0.5",
          lambda: device.pairwise(A, Bm, metric="adtw", r=0.1, p=0.5, precision="fp64"),
          1e6 * band_cells(256, 256, 0.1))

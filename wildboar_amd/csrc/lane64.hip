@@ -245,7 +245,7 @@ __global__ __launch_bounds__(64) void k_subseq_lane64(Shapelets sh, Series<doubl
     }
 }
 
-constexpr int kMaxBPW = 32;   // WDTW (per-slot weights in registers too)
+constexpr int kMaxBPW = 56;   // WDTW (per-slot weights in registers too; 40..56 in AGPRs)
 // Unweighted DTW and ADTW (pairs, windows, profiles): wider rows live in
 // VGPRs + AGPRs (64..80 without scratch; 88..104 spill a few slots to
 // scratch), one wave per SIMD like class 56 -- still far ahead of the block
