@@ -16,8 +16,9 @@
 // routes other weight vectors to the block kernels).  Slots past the band
 // width B (padding up to BP) are reset to +inf after every row.
 //
-// Register classes BP = 8 .. 56 slots (WDTW: .. 32, the per-slot weights
-// take registers too).
+// Register classes BP = 8 .. 104 slots for DTW and ADTW (rows past 56 live
+// in VGPRs plus AGPRs, 88..104 with a few scratch slots), 8 .. 56 for WDTW
+// (the per-slot weights take registers too).
 // Kernels:
 //   k_lane64_pairs   -- pairwise / paired / argmin rows with one geometry
 //   k_profile_lane64 -- window profiles: row q, windows 64 b .. 64 b + 63 of
