@@ -173,10 +173,11 @@ __device__ __forceinline__ float wave_sum(float v) {
 }
 #undef WBX_DPP
 
-// Where the per-pair gradients go.  For column j of pair (a, b) the backward
-// writes u = coef * (number of path cells in column j) and v = coef * (sum of
-// x_i over those cells); dD/dy_j = y_j u - v is formed in the reduction, so the
-// path walk itself reads no y.
+// Where the per-pair gradients go.  For column j of pair (a, b) the walk keeps
+// u = coef * (number of path cells in column j) and v = coef * (sum of x_i
+// over those cells) and writes the one term dD/dy_j = y_j u - v when the
+// column's 8-column window is complete; the window's 8 y values are fetched
+// one window ahead from YP, so the walk never waits on them.
 struct GradOut {
     int mode;          // 0: GRID pairs, 1: DIAG (paired) pairs
     int xred;          // GRID with ny % 64 == 0: a wave's lanes share one x row
@@ -185,8 +186,9 @@ struct GradOut {
     int64_t ldx;
     float *xpart;      // GRID xred, nbc > 1: [a][bchunk][i]
     float *gxp;        // GRID without lane sharing: per-pair rows [wave][i][lane]
-    float *gu, *gv;    // per-pair column terms, pair-major rows of lyp floats
+    float *gw;         // per-pair column terms, pair-major rows of lyp floats
     int lyp;           // ly rounded up to a multiple of 8 (rows stay 32-byte aligned)
+    const float *YP;   // y_b in rows of lyp floats, zero past ly
     // X transposed (XT[i * nx + a]) and prefix sums of Y transposed
     // (PT[j * ny + b] = y_b[0] + ... + y_b[j-1], j in [0, ly], float64): the
     // lanes of a wave hold neighbouring pairs, so their loads coalesce
@@ -197,12 +199,15 @@ struct GradOut {
 // PT[j * ny + b] = sum of y_b[0 .. j) in float64; one wave per series, 64
 // columns at a time with a shuffle scan (the sum order is fixed: the result
 // is deterministic).
+// YP[b * lyp + j] = y_b[j], zero for ly <= j < lyp.
 __global__ __launch_bounds__(64) void k_prefix_t(const float *__restrict__ Y, int64_t ny, int ly,
-                                                 int64_t stride, double *__restrict__ PT) {
+                                                 int64_t stride, double *__restrict__ PT,
+                                                 float *__restrict__ YP, int lyp) {
     const int64_t b = blockIdx.x;
     const int lane = threadIdx.x;
     const float *y = Y + b * stride;
     if (lane == 0) PT[b] = 0.0;
+    for (int j = lane; j < lyp; j += 64) YP[b * lyp + j] = j < ly ? y[j] : 0.0f;
     double carry = 0.0;
     for (int j0 = 0; j0 < ly; j0 += 64) {
         const int j = j0 + lane;
@@ -267,19 +272,18 @@ __global__ __launch_bounds__(64) void k_lane_backtrack(PairSpec ps, LaneGeo g,
     // CW columns (columns >= ly of the top window count as finished zeros).
     constexpr int CW = 8;
     typedef float fcw __attribute__((ext_vector_type(CW)));
-    fcw *__restrict__ guw = (fcw *)(go.gu + p * go.lyp);
-    fcw *__restrict__ gvw = (fcw *)(go.gv + p * go.lyp);
+    fcw *__restrict__ gww = (fcw *)(go.gw + p * go.lyp);
+    const fcw *__restrict__ ypw = (const fcw *)(go.YP + b * go.lyp);
     fcw wu_ = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     fcw wv_ = wu_;
+    fcw yv_ = ypw[(ly - 1) / CW];  // y of the first window to complete
     int fill = (CW - ly % CW) % CW;
     auto finish = [&](int col, float u, float v) {
         wu_ = fcw{u, wu_[0], wu_[1], wu_[2], wu_[3], wu_[4], wu_[5], wu_[6]};
         wv_ = fcw{v, wv_[0], wv_[1], wv_[2], wv_[3], wv_[4], wv_[5], wv_[6]};
         if (++fill == CW) {  // col is a multiple of CW: columns col .. col + CW - 1
-            if (active) {
-                guw[col / CW] = wu_;
-                gvw[col / CW] = wv_;
-            }
+            if (active) gww[col / CW] = wu_ * yv_ - wv_;
+            yv_ = ypw[col >= CW ? col / CW - 1 : 0];  // the next window down
             fill = 0;
         }
     };
@@ -424,10 +428,9 @@ __global__ __launch_bounds__(256) void k_sum_xpart(const float *__restrict__ xpa
     dX[a * ldx + i] = s;
 }
 
-// part[c][0][j][b], part[c][1][j][b]: sums over a in chunk c of the column
-// terms u, v of pair (a, b), a in order
-__global__ __launch_bounds__(256) void k_sum_cols_partial(const float *__restrict__ gu,
-                                                          const float *__restrict__ gv, int64_t nx,
+// part[c][b][j]: sums over a in chunk c of the column terms of pair (a, b),
+// a in order
+__global__ __launch_bounds__(256) void k_sum_cols_partial(const float *__restrict__ gw, int64_t nx,
                                                           int64_t ny, int ly, int lyp, int chunk,
                                                           float *__restrict__ part) {
     const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -438,46 +441,34 @@ __global__ __launch_bounds__(256) void k_sum_cols_partial(const float *__restric
     const int j = (int)(q - b * ly);
     const int64_t a0 = (int64_t)c * chunk;
     const int64_t a1 = a0 + chunk < nx ? a0 + chunk : nx;
-    float su = 0.0f, sv = 0.0f;
-    for (int64_t a = a0; a < a1; ++a) {
-        const int64_t o = (a * ny + b) * lyp + j;
-        su += gu[o];
-        sv += gv[o];
-    }
-    part[(2 * (int64_t)c) * per + q] = su;
-    part[(2 * (int64_t)c + 1) * per + q] = sv;
+    float s = 0.0f;
+    for (int64_t a = a0; a < a1; ++a) s += gw[(a * ny + b) * lyp + j];
+    part[(int64_t)c * per + q] = s;
 }
 
-// dY[b][j] = y_bj * (sum of u) - (sum of v), chunks in order
+// dY[b][j] = sum of the chunk partials, chunks in order
 __global__ __launch_bounds__(256) void k_sum_cols_final(const float *__restrict__ part, int nc,
-                                                        const float *__restrict__ Y,
-                                                        int64_t y_stride, int64_t ny, int ly,
-                                                        float *__restrict__ dY, int64_t ldy_) {
+                                                        int64_t ny, int ly, float *__restrict__ dY,
+                                                        int64_t ldy_) {
     const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t per = ny * ly;
     if (q >= per) return;
-    float su = 0.0f, sv = 0.0f;
-    for (int c = 0; c < nc; ++c) {
-        su += part[(2 * (int64_t)c) * per + q];
-        sv += part[(2 * (int64_t)c + 1) * per + q];
-    }
+    float s = 0.0f;
+    for (int c = 0; c < nc; ++c) s += part[(int64_t)c * per + q];
     const int64_t b = q / ly;
     const int j = (int)(q - b * ly);
-    dY[b * ldy_ + j] = Y[b * y_stride + j] * su - sv;
+    dY[b * ldy_ + j] = s;
 }
 
-// Paired pairs: dY[p][j] = y_pj u_pj - v_pj.
-__global__ __launch_bounds__(256) void k_dy_paired(const float *__restrict__ gu,
-                                                   const float *__restrict__ gv, int64_t n, int ly,
-                                                   int lyp, const float *__restrict__ Y,
-                                                   int64_t y_stride, float *__restrict__ dY,
-                                                   int64_t ldy_) {
+// Paired pairs: dY[p][j] is pair p's column term.
+__global__ __launch_bounds__(256) void k_dy_paired(const float *__restrict__ gw, int64_t n, int ly,
+                                                   int lyp, float *__restrict__ dY, int64_t ldy_) {
     const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (q >= n * ly) return;
     const int64_t p = q / ly;
     const int j = (int)(q - p * ly);
     const int64_t o = p * lyp + j;
-    dY[p * ldy_ + j] = Y[p * y_stride + j] * gu[o] - gv[o];
+    dY[p * ldy_ + j] = gw[o];
 }
 
 constexpr int TRAIN_MAX_BP = 112;
@@ -528,7 +519,7 @@ static int train_geo(const wbx_metric *m, int64_t tx, int64_t ty, TrainGeo *out)
 // the column partials.
 struct TrainScratch {
     bool xred;
-    int64_t nbc, nc, xt, pt, gu, gv, gx, xpart, part, total;  // byte offsets / sizes
+    int64_t nbc, nc, xt, pt, gw, yp, gx, xpart, part, total;  // byte offsets / sizes
 };
 
 static TrainScratch train_scratch(int mode, int64_t nx, int64_t ny, int64_t tx, int64_t ty) {
@@ -541,10 +532,10 @@ static TrainScratch train_scratch(int mode, int64_t nx, int64_t ny, int64_t tx, 
     t.pt = o;
     o += al((ty + 1) * ny * 8);
     const int64_t lyp = (ty + 7) / 8 * 8;
-    t.gu = o;
+    t.gw = o;
     o += al(np * lyp * 4);
-    t.gv = o;
-    o += al(np * lyp * 4);
+    t.yp = o;
+    o += al(ny * lyp * 4);
     t.gx = t.xpart = t.part = o;
     if (mode != 0) {
         t.total = o;
@@ -558,7 +549,7 @@ static TrainScratch train_scratch(int mode, int64_t nx, int64_t ny, int64_t tx, 
     t.xpart = o;
     if (t.xred && t.nbc > 1) o += al(nx * t.nbc * tx * 4);
     t.part = o;
-    o += al(2 * t.nc * ny * ty * 4);
+    o += al(t.nc * ny * ty * 4);
     t.total = o;
     return t;
 }
@@ -645,16 +636,17 @@ int wbx_dev_dtw_train_backward(const wbx_metric *metric, int mode, const float *
     go.ldx = ldx;
     go.gxp = (float *)(sc + ts.gx);
     go.xpart = (float *)(sc + ts.xpart);
-    go.gu = (float *)(sc + ts.gu);
-    go.gv = (float *)(sc + ts.gv);
+    go.gw = (float *)(sc + ts.gw);
     go.lyp = (int)((ty + 7) / 8 * 8);
+    float *yp = (float *)(sc + ts.yp);
+    go.YP = yp;
     float *part = (float *)(sc + ts.part);
     float *xt = (float *)(sc + ts.xt);
     double *pt = (double *)(sc + ts.pt);
     hipLaunchKernelGGL(k_transpose, dim3((unsigned)((tx + 31) / 32), (unsigned)((nx + 31) / 32)),
                        dim3(256), 0, st, X, nx, (int)tx, x_stride, xt);
     hipLaunchKernelGGL(k_prefix_t, dim3((unsigned)ny), dim3(64), 0, st, Y, ny, (int)ty, y_stride,
-                       pt);
+                       pt, yp, go.lyp);
     go.XT = xt;
     go.PT = pt;
     switch (tg.BP) {
@@ -675,12 +667,12 @@ int wbx_dev_dtw_train_backward(const wbx_metric *metric, int mode, const float *
             hipLaunchKernelGGL(k_sum_xpart, dim3(gx_blocks), dim3(256), 0, st, go.xpart, nx, ts.nbc,
                                (int)tx, dX, ldx);
         hipLaunchKernelGGL(k_sum_cols_partial, dim3(gy_blocks, (unsigned)ts.nc), dim3(256), 0, st,
-                           go.gu, go.gv, nx, ny, (int)ty, go.lyp, COL_CHUNK, part);
-        hipLaunchKernelGGL(k_sum_cols_final, dim3(gy_blocks), dim3(256), 0, st, part, (int)ts.nc, Y,
-                           y_stride, ny, (int)ty, dY, ldy_);
+                           go.gw, nx, ny, (int)ty, go.lyp, COL_CHUNK, part);
+        hipLaunchKernelGGL(k_sum_cols_final, dim3(gy_blocks), dim3(256), 0, st, part, (int)ts.nc,
+                           ny, (int)ty, dY, ldy_);
     } else {
-        hipLaunchKernelGGL(k_dy_paired, dim3(gy_blocks), dim3(256), 0, st, go.gu, go.gv, ny,
-                           (int)ty, go.lyp, Y, y_stride, dY, ldy_);
+        hipLaunchKernelGGL(k_dy_paired, dim3(gy_blocks), dim3(256), 0, st, go.gw, ny, (int)ty,
+                           go.lyp, dY, ldy_);
     }
     WBX_HIP_CHECK(hipGetLastError());
     return WBX_OK;
