@@ -442,6 +442,7 @@ __global__ __launch_bounds__(256) void k_sum_cols_partial(const float *__restric
     const int64_t a0 = (int64_t)c * chunk;
     const int64_t a1 = a0 + chunk < nx ? a0 + chunk : nx;
     float s = 0.0f;
+#pragma unroll 8
     for (int64_t a = a0; a < a1; ++a) s += gw[(a * ny + b) * lyp + j];
     part[(int64_t)c * per + q] = s;
 }
@@ -472,7 +473,7 @@ __global__ __launch_bounds__(256) void k_dy_paired(const float *__restrict__ gw,
 }
 
 constexpr int TRAIN_MAX_BP = 112;
-constexpr int COL_CHUNK = 64;
+constexpr int COL_CHUNK = 256;  // x series per column partial: 16 partials for 4096
 
 static int train_bp(int B) { return (B + 7) / 8 * 8; }
 
