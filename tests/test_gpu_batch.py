@@ -194,3 +194,58 @@ def test_batch_rejects_bad_lengths():
         DtwBatch([5], [3], 4, 0.1)
     with pytest.raises(ValueError):
         DtwBatch([2], [3], 4, [1.5])
+
+
+def test_batch_temporary_plan_and_recapture_with_new_pointers():
+    # A plan dropped right after a graph run waits for that run before its
+    # graph, streams and buffers go; a live plan re-captures when the output
+    # or the inputs move, after its previous replay has finished.
+    X, Y, lx, ly, r = make_pairs(21, 150, 30, 500, 500, spikes=0.05)
+    Xd = torch.from_numpy(X.astype(np.float32)).cuda()
+    Yd = torch.from_numpy(Y.astype(np.float32)).cuda()
+    want = oracle_batch(X.astype(np.float32).astype(np.float64),
+                        Y.astype(np.float32).astype(np.float64), lx, ly, r)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        tmp = DtwBatch(lx, ly, 500, r, precision="fp32").run(Xd, Yd, graph=True)
+        plan = DtwBatch(lx, ly, 500, r, precision="fp32")
+        outs = [torch.full((len(lx),), -1.0, dtype=torch.float64, device="cuda") for _ in range(3)]
+        for o in outs:  # each new `out` re-captures while the last replay may run
+            plan.run(Xd, Yd, out=o, graph=True)
+        X2, Y2 = Xd.clone(), Yd.clone()
+        moved = plan.run(X2, Y2, graph=True)  # new inputs: another capture
+        again = plan.run(X2, Y2, graph=True)  # same pointers: a replay
+    torch.cuda.synchronize()
+    for got in [tmp, moved, again] + outs:
+        assert rel_err(got.cpu().numpy(), want) <= FP32_RTOL
+    del plan
+    torch.cuda.synchronize()
+
+
+def test_batch_wdtw_plans_with_different_g_replay_in_turn():
+    # Each WDTW plan owns its weight vector: replays of one plan's graph are
+    # not affected by another plan (or a pairwise call) using another g.
+    from wildboar_amd import distance as wd
+
+    X, Y, lx, ly, r = make_pairs(22, 120, 20, 300, 300, spikes=0.05)
+    Xd = torch.from_numpy(X.astype(np.float32)).cuda()
+    Yd = torch.from_numpy(Y.astype(np.float32)).cuda()
+    X32 = X.astype(np.float32).astype(np.float64)
+    Y32 = Y.astype(np.float32).astype(np.float64)
+    plans = {g: DtwBatch(lx, ly, 300, r, metric="wdtw", g=g, precision="fp32") for g in (0.05, 0.3)}
+    eager = {g: p.run(Xd, Yd, graph=False).clone() for g, p in plans.items()}
+    torch.cuda.synchronize()
+    for g in plans:
+        want = oracle_batch(X32, Y32, lx, ly, r, metric="wdtw", g=g, weight_n=300)
+        assert rel_err(eager[g].cpu().numpy(), want) <= FP32_RTOL
+    side = torch.cuda.Stream()
+    Z = np.random.default_rng(3).standard_normal((3, 41))
+    for rep in range(3):
+        for g, p in plans.items():
+            with torch.cuda.stream(side):
+                got = p.run(Xd, Yd, graph=True)
+            wd.pairwise_distance(Z, Z, metric="wdtw", metric_params={"g": 0.7 + rep})
+            torch.cuda.synchronize()
+            assert torch.equal(got, eager[g]), (rep, g)
+    for p in plans.values():
+        p.close()

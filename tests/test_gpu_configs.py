@@ -1,0 +1,169 @@
+"""GPU parity at the full series length of the stated workloads, on reduced rows.
+
+Pairwise 2000 x 2000 x 512 (seeds 10 / 11, labels from seed 12) is checked on
+its first 16 x 40 rows for every metric leg of that workload: dtw with
+r in {0.05, 0.1, 1.0}, wdtw g = 0.05 (r = 0.1 and the default r = 1.0),
+euclidean, and the 1-NN labels, in float64 (bitwise) and float32 (relative
+error <= 1e-5; argmin indices, heap order and returned distances exact).
+
+The 50,000 uneven pairs padded to 2048 are checked on a subset of the same
+generator chosen so that every strip size of the plan receives pairs.
+"""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from wildboar_amd import _lib
+from wildboar_amd import distance as wd
+from wildboar_amd.batch import DtwBatch
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+FP32_RTOL = 1e-5
+T = 512
+NX, NY = 16, 40
+
+
+def _rw(seed, n, t):
+    rng = np.random.default_rng(seed)
+    x = np.cumsum(rng.standard_normal((n, t)), axis=1)
+    return (x - x.mean(axis=1, keepdims=True)) / x.std(axis=1, keepdims=True)
+
+
+_DATA = {}
+
+
+def cfg2_rows():
+    if not _DATA:
+        _DATA["X"] = _rw(10, 2000, T)[:NX].copy()
+        _DATA["Y"] = _rw(11, 2000, T)[:NY].copy()
+        _DATA["y"] = np.random.default_rng(12).integers(0, 10, 2000)[:NY]
+    return _DATA["X"], _DATA["Y"], _DATA["y"]
+
+
+def setup_module(module):
+    _lib.set_policy(0)
+
+
+def rel_err(got, want):
+    got = np.asarray(got)
+    want = np.asarray(want)
+    assert np.isfinite(got).all()
+    return float(np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-1)))
+
+
+LEGS = [("dtw", {"r": 0.05}), ("dtw", {"r": 0.1}), ("dtw", {"r": 1.0}),
+        ("wdtw", {"r": 0.1, "g": 0.05}), ("wdtw", {"r": 1.0, "g": 0.05}), ("euclidean", {})]
+
+
+def _oracle_kw(params):
+    return {"r": params.get("r", 1.0), "g": params.get("g", 0.05)}
+
+
+@pytest.mark.parametrize("metric,params", LEGS)
+def test_cfg2_pairwise_fp64_bitwise(metric, params):
+    X, Y, _ = cfg2_rows()
+    want = oracle.pairwise(X, Y, metric=metric, **_oracle_kw(params))
+    got = wd.pairwise_distance(X, Y, metric=metric, metric_params=params)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("metric,params", LEGS)
+def test_cfg2_pairwise_fp32(metric, params):
+    X, Y, _ = cfg2_rows()
+    want = oracle.pairwise(X, Y, metric=metric, **_oracle_kw(params))
+    got = wd.pairwise_distance(X.astype(np.float32), Y.astype(np.float32), metric=metric,
+                               metric_params=params, precision="fp32")
+    assert rel_err(got, want) <= FP32_RTOL
+
+
+@pytest.mark.parametrize("metric,params", [("dtw", {"r": 0.1}), ("dtw", {"r": 1.0}),
+                                           ("wdtw", {"r": 0.1, "g": 0.05})])
+def test_cfg2_singleton_and_paired(metric, params):
+    X, Y, _ = cfg2_rows()
+    kw = _oracle_kw(params)
+    for prec in ("fp64", "fp32"):
+        got = wd.pairwise_distance(X, metric=metric, metric_params=params, precision=prec)
+        want = oracle.pairwise(X, None, metric=metric, **kw)
+        if prec == "fp64":
+            np.testing.assert_array_equal(got, want)
+        else:
+            assert rel_err(got, want) <= FP32_RTOL
+        got = wd.paired_distance(X, Y[:NX], metric=metric, metric_params=params, precision=prec)
+        want = oracle.paired(X, Y[:NX], metric=metric, **kw)
+        if prec == "fp64":
+            np.testing.assert_array_equal(got, want)
+        else:
+            assert rel_err(got, want) <= FP32_RTOL
+
+
+@pytest.mark.parametrize("metric,params", LEGS)
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_cfg2_nearest_neighbour_labels_exact(metric, params, precision):
+    X, Y, y = cfg2_rows()
+    kw = _oracle_kw(params)
+    for k in (1, 5):
+        oi, od = oracle.argmin(X, Y, k=k, metric=metric, **kw)
+        gi, gd = wd.argmin_distance(X, Y, k=k, metric=metric, metric_params=params,
+                                    return_distance=True, precision=precision)
+        np.testing.assert_array_equal(gi, oi)  # heap order included
+        np.testing.assert_array_equal(gd, od)
+        if k == 1:
+            np.testing.assert_array_equal(y[gi[:, 0]], y[oi[:, 0]])
+
+
+# ----------------------------------------------------------------- batched
+def _plan_R(lx, ly, r):
+    """Strip size the plan gives a pair: the smallest compiled R whose 64
+    lanes cover the band (shorter series on the rows)."""
+    import math
+
+    table = (2, 3, 4, 6, 8, 10, 12, 14, 16, 20, 24, 28, 32, 40)
+    a, c = min(lx, ly), max(lx, ly)
+    w = max(math.floor(a * r), 1)
+    B = (c - a) + 2 * w - 1
+    for R in table:
+        if 64 * (R + 1) < B:
+            continue
+        n = (B + R) // (R + 1)
+        if n * R > B or n > 64:
+            continue
+        return R
+    return None
+
+
+def test_uneven_pairs_2048_every_strip_size():
+    from bench import uneven_pairs
+
+    X, Y, lx, ly, r = uneven_pairs(3000, 2048)
+    Rs = np.array([_plan_R(int(a), int(b), float(q)) for a, b, q in zip(lx, ly, r)])
+    pick = []
+    for R in np.unique(Rs):
+        pick.extend(np.flatnonzero(Rs == R)[:24].tolist())
+    pick = np.array(sorted(pick))
+    Xs, Ys, lxs, lys, rs = X[pick], Y[pick], lx[pick], ly[pick], r[pick]
+    plan = DtwBatch(lxs, lys, 2048, rs, precision="fp32")
+    got_R = sorted(b["R"] for b in plan.buckets() if b["pairs"] > 0)
+    assert set(range(2, 33)) & set(got_R) == {2, 3, 4, 6, 8, 10, 12, 14, 16, 20, 24, 28, 32}
+    assert plan.n_plain == 0
+    Xd = torch.from_numpy(Xs).cuda()
+    Yd = torch.from_numpy(Ys).cuda()
+    want = np.array([oracle.metric_distance(Xs[p, :lxs[p]].astype(np.float64),
+                                            Ys[p, :lys[p]].astype(np.float64), "dtw", r=rs[p])
+                     for p in range(len(pick))])
+    side = torch.cuda.Stream()  # graph capture needs a stream other than the null stream
+    for graph in (False, True, True):
+        with torch.cuda.stream(side):
+            got = plan.run(Xd, Yd, graph=graph)
+        side.synchronize()
+        assert rel_err(got.cpu().numpy(), want) <= FP32_RTOL
+    plan.close()

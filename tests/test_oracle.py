@@ -8,6 +8,10 @@ Independent checks, each chosen so a plausible slip in the restatement fails:
 - r = 1 is the textbook unconstrained DTW;
 - D(x, x) = 0, symmetry of DTW, monotone non-increasing in r;
 - WDTW with g = 0 has every weight exactly 0.5, so WDTW^2 == 0.5 * DTW^2;
+- the WDTW weight vector by its closed forms: w[n/2] = 1/2 exactly, point
+  symmetry w[i] + w[n-i] = 1, strictly increasing for g > 0, agreement with an
+  independent logistic (scipy expit), and the size rule n = max(widths)
+  through a brute force with expit weights;
 - the argmin_distance docstring known answer (distance/_distance.py:1380-1387);
 - argmin equals a brute-force "k smallest, first index on ties" selection;
 - subsequence scans against brute force over windows, euclidean == dtw(r=0),
@@ -113,11 +117,51 @@ def test_brute_force_weighted_with_row0_quirk():
                 assert got == brute_dtw(x, y, r, wts), (lx, ly, r)
 
 
-def test_weights_are_libm_formula():
-    for n, g in [(1, 0.05), (7, 0.5), (128, 0.05), (513, 1.3)]:
-        w = oracle.wdtw_weights(n, g)
-        ref = [1.0 / (1.0 + math.exp(-g * (i - n / 2.0))) for i in range(n)]
-        assert w.tolist() == ref
+def _ulps(a, b):
+    """Distance in units in the last place of the larger magnitude."""
+    return np.abs(a - b) / np.spacing(np.maximum(np.abs(a), np.abs(b)))
+
+
+@pytest.mark.parametrize("n,g", [(2, 0.05), (8, 0.5), (128, 0.05), (512, 0.05), (1024, 0.01),
+                                 (300, 1.3)])
+def test_wdtw_weights_closed_forms_even_n(n, g):
+    # The logistic weight 1 / (1 + exp(-g (i - n/2))) (distance/_elastic.pyx:3334-3341)
+    # is pinned by properties the formula fixes, not by retyping it:
+    #  - centre: i = n/2 gives exp(0) = 1, so w = 1/2 exactly;
+    #  - point symmetry about the centre: w[i] + w[n - i] = 1 (logistic);
+    #  - strictly increasing in i for g > 0 (a sign slip reverses it);
+    #  - an independent logistic (scipy's expit) agrees to a few ulp.
+    # A centring on n instead of n/2, or i - 1 for i, fails the first two.
+    from scipy.special import expit
+
+    w = oracle.wdtw_weights(n, g)
+    assert w.shape == (n,)
+    assert w[n // 2] == 0.5
+    i = np.arange(1, n)
+    s = w[i] + w[n - i]
+    assert np.all(np.abs(s - 1.0) <= 2 * np.spacing(1.0)), float(np.max(np.abs(s - 1.0)))
+    d = np.diff(w)
+    unsat = (w[:-1] < 1.0) & (w[1:] < 1.0)  # exp(-g (i - n/2)) below 2^-53: w rounds to 1
+    assert np.all(d[unsat] > 0.0)
+    assert np.all(d >= 0.0)
+    assert np.max(_ulps(w, expit(g * (np.arange(n) - n / 2.0)))) <= 4
+
+
+@pytest.mark.parametrize("n", [1, 7, 33, 513])
+def test_wdtw_weights_closed_forms_odd_n(n):
+    # odd n: the centre n/2 falls between two samples, so w[(n-1)/2] and
+    # w[(n+1)/2] are symmetric about 1/2: w[i] + w[n - i] = 1 still holds
+    # for i in 1..n-1, and g = 0 makes every weight exactly 1/2.
+    from scipy.special import expit
+
+    g = 0.2
+    w = oracle.wdtw_weights(n, g)
+    if n > 1:
+        i = np.arange(1, n)
+        assert np.all(np.abs(w[i] + w[n - i] - 1.0) <= 2 * np.spacing(1.0))
+        assert w[(n - 1) // 2] < 0.5 < w[(n + 1) // 2]
+    assert np.max(_ulps(w, expit(g * (np.arange(n) - n / 2.0)))) <= 4
+    assert np.all(oracle.wdtw_weights(n, 0.0) == 0.5)
 
 
 def test_r0_is_euclidean_bitwise():
@@ -165,6 +209,27 @@ def test_wdtw_g0_is_half_dtw_and_asymmetric():
                 - oracle.metric_distance(X[b], X[a], "wdtw", r=0.4, g=0.3))
             for a in range(4) for b in range(4)]
     assert max(asym) > 0.0
+
+
+def test_wdtw_pairwise_weights_sized_by_wider_array():
+    # WeightedDtwMetric.reset sizes the weights by max(X.shape[2], Y.shape[2])
+    # (distance/_elastic.pyx:3336); the brute force uses expit weights of that
+    # size, so the vector is pinned end to end (value, centre and size).
+    from scipy.special import expit
+
+    rng = np.random.default_rng(61)
+    X = rw(rng, 3, 30)
+    Y = rw(rng, 4, 41)
+    g, r = 0.15, 0.3
+    D = oracle.pairwise(X, Y, metric="wdtw", r=r, g=g)
+    wts = expit(g * (np.arange(41) - 41 / 2.0))
+    for a in range(3):
+        for b in range(4):
+            want = math.sqrt(brute_dtw(X[a], Y[b], r, wts))
+            assert D[a, b] == pytest.approx(want, rel=1e-13)
+    # the smaller width would centre the weights elsewhere: values differ
+    wts30 = expit(g * (np.arange(41) - 30 / 2.0))
+    assert abs(D[0, 0] - math.sqrt(brute_dtw(X[0], Y[0], r, wts30))) > 1e-6 * D[0, 0]
 
 
 def test_argmin_docstring_known_answer():

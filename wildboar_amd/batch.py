@@ -135,20 +135,32 @@ class DtwBatch:
             raise TypeError("X and Y must both be float32 or float64")
 
     def run(self, X, Y, out=None, *, graph=True):
-        """Enqueue on torch's current stream; returns the float64 distances."""
+        """Enqueue on torch's current stream; returns the float64 distances.
+
+        With ``graph=True`` and no ``out``, the captured graph writes into a
+        buffer the plan owns (so replays keep one capture) and the result is
+        a copy of it, made on the same stream.
+        """
         self._check_inputs(X, Y)
-        if out is None:
-            out = torch.empty(self.n_pairs, dtype=torch.float64, device=X.device)
         dtype = _lib.F32 if X.dtype == torch.float32 else _lib.F64
+        owned = out is None and graph
+        if owned:
+            if getattr(self, "_out", None) is None or self._out.device != X.device:
+                self._out = torch.empty(self.n_pairs, dtype=torch.float64, device=X.device)
+            out = self._out
+        elif out is None:
+            out = torch.empty(self.n_pairs, dtype=torch.float64, device=X.device)
         _lib.check(self._lib.wbx_batch_run(
             self._plan, dtype, X.data_ptr(), X.stride(0), Y.data_ptr(), Y.stride(0),
             out.data_ptr(), torch.cuda.current_stream().cuda_stream, 1 if graph else 0))
-        return out
+        return out.clone() if owned else out
 
     def close(self):
+        """Releases the plan; waits for its last run to finish first."""
         if getattr(self, "_plan", None):
             self._lib.wbx_batch_free(self._plan)
             self._plan = None
+        self._out = None
 
     def __del__(self):
         try:

@@ -207,7 +207,7 @@ enum { KERN_EUCLID = 1, KERN_LANE = 2, KERN_PLAIN = 3, KERN_BATCH = 4, KERN_SUBS
 // The float64 lane kernels multiply +inf (columns outside a series) by the
 // weights, so they take WDTW only when every cached weight is > 0.
 bool lane64_ok(const DeviceCtx *ctx, const double *weights) {
-    return !weights || (ctx && ctx->w64 == weights && ctx->w_min > 0.0);
+    return !weights || cached_weights_min(ctx, weights) > 0.0;
 }
 
 // Squared costs of the pairs of ps.  m->kind is EUCLIDEAN, DTW, WDTW or ADTW

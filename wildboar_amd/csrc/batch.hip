@@ -500,6 +500,15 @@ struct BatchPlan {
     float *d_sq = nullptr;
     double *d_sq_plain = nullptr;
     void *d_xconv = nullptr, *d_yconv = nullptr;
+    // WDTW weights sized by t_pad, owned by the plan (a captured graph keeps
+    // these pointers, so they live exactly as long as the plan)
+    double *d_w64 = nullptr;
+    float *d_w32 = nullptr;
+    // completion of the last enqueued run or replay: recorded after every
+    // run outside stream capture, waited on before anything a run reads or
+    // writes (graph, streams, buffers) is destroyed
+    hipEvent_t ev_done = nullptr;
+    bool pending = false;
     int64_t cells = 0, slots = 0;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -525,7 +534,15 @@ static int dmalloc(T **p, size_t n) {
     return WBX_OK;
 }
 
+// Waits for the plan's last recorded run (it may still be executing the
+// graph, or reading the plan's buffers from the forked streams).
+static void plan_wait(BatchPlan *b) {
+    if (b->pending && b->ev_done) (void)hipEventSynchronize(b->ev_done);
+    b->pending = false;
+}
+
 static void plan_drop_graph(BatchPlan *b) {
+    plan_wait(b);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
@@ -540,10 +557,11 @@ static void plan_free(BatchPlan *b) {
         if (b->ev_join[a]) (void)hipEventDestroy(b->ev_join[a]);
     }
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_done) (void)hipEventDestroy(b->ev_done);
     void *ptrs[] = {b->d_items, b->d_counters, b->d_plain_list, b->d_lx,
                     b->d_ly,    b->d_r,        b->d_sq,         b->d_sq_plain,
                     b->d_xconv, b->d_yconv,    b->d_stage_pairs, b->d_stage,
-                    b->d_zero};
+                    b->d_zero,  b->d_w64,      b->d_w32};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     delete b;
@@ -864,6 +882,15 @@ int wbx_batch_create(const wbx_metric *metric, int64_t n_pairs, int64_t t_pad,
                  hipEventCreateWithFlags(&b->ev_join[a], hipEventDisableTiming) == hipSuccess;
         if (!ok) rc = fail(WBX_EDEVICE, "batch: stream or event creation failed");
     }
+    if (!rc && hipEventCreateWithFlags(&b->ev_done, hipEventDisableTiming) != hipSuccess)
+        rc = fail(WBX_EDEVICE, "batch: event creation failed");
+    if (!rc && metric->kind == WBX_WDTW) {
+        // weights sized by the padded array width, as the reference sizes
+        // them by the array's n_timestep
+        rc = dmalloc(&b->d_w64, (size_t)t_pad);
+        if (!rc) rc = dmalloc(&b->d_w32, (size_t)t_pad);
+        if (!rc) rc = fill_weights(t_pad, metric->g, b->d_w64, b->d_w32, nullptr);
+    }
     if (!rc) {
         const size_t conv = (size_t)n_pairs * (size_t)t_pad * 8;
         if (hipMalloc(&b->d_xconv, conv) != hipSuccess || hipMalloc(&b->d_yconv, conv) != hipSuccess)
@@ -1079,14 +1106,18 @@ int wbx_batch_run(void *plan, int dtype, const void *X, int64_t x_stride, const 
     DeviceCtx *ctx = ctx_current();
     if (!ctx) return WBX_EDEVICE;
     hipStream_t st = (hipStream_t)stream;
-    const double *w64 = nullptr;
-    const float *w32 = nullptr;
-    if (b->metric.kind == WBX_WDTW) {
-        // weights sized by the padded array width, as the reference sizes
-        // them by the array's n_timestep
-        const int rc = cached_weights(ctx, b->t_pad, b->metric.g, &w64, &w32);
-        if (rc) return rc;
-    }
+    const double *w64 = b->d_w64;
+    const float *w32 = b->d_w32;
+    // Marks the run for plan_wait; a run enqueued into a caller's capture
+    // has no completion this plan can wait on.
+    auto mark_done = [&]() -> int {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        WBX_HIP_CHECK(hipStreamIsCapturing(st, &cs));
+        if (cs != hipStreamCaptureStatusNone) return WBX_OK;
+        WBX_HIP_CHECK(hipEventRecord(b->ev_done, st));
+        b->pending = true;
+        return WBX_OK;
+    };
     auto enqueue = [&](hipStream_t s) -> int {
         return dtype == WBX_F64
                    ? plan_enqueue<double>(b, (const double *)X, x_stride, (const double *)Y,
@@ -1094,7 +1125,10 @@ int wbx_batch_run(void *plan, int dtype, const void *X, int64_t x_stride, const 
                    : plan_enqueue<float>(b, (const float *)X, x_stride, (const float *)Y, y_stride,
                                          out, s, w64, w32);
     };
-    if (!use_graph) return enqueue(st);
+    if (!use_graph) {
+        RC(enqueue(st));
+        return mark_done();
+    }
     if (st == nullptr)
         return fail(WBX_EINVAL, "graph capture needs a stream other than the null stream");
     const bool same = b->exec && b->g_x == X && b->g_y == Y && b->g_out == out &&
@@ -1122,7 +1156,7 @@ int wbx_batch_run(void *plan, int dtype, const void *X, int64_t x_stride, const 
         b->g_ys = y_stride;
     }
     WBX_HIP_CHECK(hipGraphLaunch(b->exec, st));
-    return WBX_OK;
+    return mark_done();
 }
 
 }  // extern "C"

@@ -94,35 +94,67 @@ int ws_dv_reserve(DeviceCtx *ctx, size_t bytes, void **out) {
 }
 
 // WDTW weights 1 / (1 + exp(-g (i - n/2))) with the host libm exp, as
-// WeightedDtwMetric.reset (distance/_elastic.pyx:3334-3341) computes them.
-int cached_weights(DeviceCtx *ctx, int64_t n, double g, const double **w64, const float **w32) {
-    if (ctx->w_n != n || ctx->w_g != g) {
-        WBX_HIP_CHECK(hipDeviceSynchronize());
-        if (ctx->w64) (void)hipFree(ctx->w64);
-        if (ctx->w32) (void)hipFree(ctx->w32);
-        ctx->w64 = nullptr;
-        ctx->w32 = nullptr;
-        ctx->w_n = -1;
-        std::vector<double> h((size_t)n);
-        std::vector<float> hf((size_t)n);
-        double w_min = WBX_INF_D;
-        for (int64_t i = 0; i < n; ++i) {
-            h[i] = 1.0 / (1.0 + exp(-g * ((double)i - (double)n / 2.0)));
-            hf[i] = (float)h[i];
-            w_min = h[i] < w_min ? h[i] : w_min;
-        }
-        ctx->w_min = w_min;
-        if (hipMalloc(&ctx->w64, sizeof(double) * n) != hipSuccess ||
-            hipMalloc(&ctx->w32, sizeof(float) * n) != hipSuccess)
-            return fail(WBX_ENOMEM, "weights allocation failed");
-        WBX_HIP_CHECK(hipMemcpy(ctx->w64, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-        WBX_HIP_CHECK(hipMemcpy(ctx->w32, hf.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-        ctx->w_n = n;
-        ctx->w_g = g;
+// WeightedDtwMetric.reset (distance/_elastic.pyx:3334-3341) computes them,
+// copied into the device vectors w64 / w32 (n elements each).
+int fill_weights(int64_t n, double g, double *w64, float *w32, double *w_min) {
+    std::vector<double> h((size_t)n);
+    std::vector<float> hf((size_t)n);
+    double lo = WBX_INF_D;
+    for (int64_t i = 0; i < n; ++i) {
+        h[i] = 1.0 / (1.0 + exp(-g * ((double)i - (double)n / 2.0)));
+        hf[i] = (float)h[i];
+        lo = h[i] < lo ? h[i] : lo;
     }
-    *w64 = ctx->w64;
-    *w32 = ctx->w32;
+    WBX_HIP_CHECK(hipMemcpy(w64, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    WBX_HIP_CHECK(hipMemcpy(w32, hf.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    if (w_min) *w_min = lo;
     return WBX_OK;
+}
+
+int cached_weights(DeviceCtx *ctx, int64_t n, double g, const double **w64, const float **w32) {
+    for (const DeviceCtx::Weights &e : ctx->weights)
+        if (e.n == n && e.g == g) {
+            *w64 = e.w64;
+            *w32 = e.w32;
+            return WBX_OK;
+        }
+    // A bounded cache: past 64 MiB of vectors the device is drained once and
+    // every entry released (a workload cycling through that many (n, g) pairs
+    // pays one synchronisation per 64 MiB, not one per change).
+    const size_t bytes = (size_t)n * (sizeof(double) + sizeof(float));
+    if (ctx->weight_bytes + bytes > ((size_t)64 << 20) && !ctx->weights.empty()) {
+        WBX_HIP_CHECK(hipDeviceSynchronize());
+        for (DeviceCtx::Weights &e : ctx->weights) {
+            (void)hipFree(e.w64);
+            (void)hipFree(e.w32);
+        }
+        ctx->weights.clear();
+        ctx->weight_bytes = 0;
+    }
+    DeviceCtx::Weights e{n, g, nullptr, nullptr, 0.0};
+    if (hipMalloc(&e.w64, sizeof(double) * n) != hipSuccess) return fail(WBX_ENOMEM, "weights allocation failed");
+    if (hipMalloc(&e.w32, sizeof(float) * n) != hipSuccess) {
+        (void)hipFree(e.w64);
+        return fail(WBX_ENOMEM, "weights allocation failed");
+    }
+    const int rc = fill_weights(n, g, e.w64, e.w32, &e.w_min);
+    if (rc) {
+        (void)hipFree(e.w64);
+        (void)hipFree(e.w32);
+        return rc;
+    }
+    ctx->weights.push_back(e);
+    ctx->weight_bytes += bytes;
+    *w64 = e.w64;
+    *w32 = e.w32;
+    return WBX_OK;
+}
+
+double cached_weights_min(const DeviceCtx *ctx, const double *w64) {
+    if (!ctx) return 0.0;
+    for (const DeviceCtx::Weights &e : ctx->weights)
+        if (e.w64 == w64) return e.w_min;
+    return 0.0;
 }
 
 }  // namespace wbx

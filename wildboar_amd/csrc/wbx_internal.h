@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/wbx.h"
 
 #define WBX_INF_F __builtin_inff()
@@ -143,11 +145,18 @@ struct DeviceCtx {
     Workspace ws_in;  // inputs rounded to float32 for the FP32_ACC64 mode
     Workspace ws_dv;  // average slopes of the inputs (derivative metrics)
     int n_cu = 0;
-    int64_t w_n = -1;
-    double w_g = 0.0;
-    double *w64 = nullptr;
-    float *w32 = nullptr;
-    double w_min = 0.0;  // smallest cached weight (the lane64 kernels need > 0)
+    // WDTW weight vectors by (n, g): entries are never freed while the
+    // context lives (kernels in flight may read any of them), so switching
+    // between parameters does not synchronise the device.
+    struct Weights {
+        int64_t n;
+        double g;
+        double *w64;
+        float *w32;
+        double w_min;  // smallest weight (the lane64 kernels need > 0)
+    };
+    std::vector<Weights> weights;
+    size_t weight_bytes = 0;
 };
 
 DeviceCtx *ctx_current();              // current device's context (created lazily)
@@ -155,8 +164,12 @@ int ws_reserve(DeviceCtx *ctx, size_t bytes, void **out);
 int ws_in_reserve(DeviceCtx *ctx, size_t bytes, void **out);
 int ws_dv_reserve(DeviceCtx *ctx, size_t bytes, void **out);
 // WDTW weights for width n and penalty g, computed on the host with libm exp
-// and cached on the device (one entry per device).
+// and cached on the device (one entry per (n, g) per device).
 int cached_weights(DeviceCtx *ctx, int64_t n, double g, const double **w64, const float **w32);
+// Smallest weight of a cached float64 vector (0 if w64 is not one).
+double cached_weights_min(const DeviceCtx *ctx, const double *w64);
+// Fills a caller-owned pair of device vectors with the weights for (n, g).
+int fill_weights(int64_t n, double g, double *w64, float *w32, double *w_min);
 
 // ---------------------------------------------------------------- launchers
 // Plain one-pair-per-block DTW (any band, any lengths, WDTW quirk): writes the
