@@ -231,6 +231,18 @@ int wbx_dev_pairwise(const wbx_metric *metric, int dtype, const void *X,
                      int64_t ny, int64_t y_stride, int64_t ty, double *out,
                      void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): wbx_dev_pairwise with the same arguments and results, but its
+ * staged inputs and per-pair costs live in a fixed per-GPU device array of
+ * 64 MiB, so it calls no hipMalloc / hipFree / memset.  Sizes needing more
+ * scratch return WBX_EINVAL; wdtw, ddtw, wddtw and FP32_ACC64 return
+ * WBX_EUNSUPPORTED.  Calls that overlap on one GPU share its array and race. */
+int wbx_dev_pairwise_noalloc(const wbx_metric *metric, int dtype, const void *X,
+                             int64_t nx, int64_t x_stride, int64_t tx,
+                             const void *Y, int64_t ny, int64_t y_stride,
+                             int64_t ty, double *out, void *stream);
+
 /* Paired as wbx_paired: out[i] = D(Y[i], X[i]). */
 int wbx_dev_paired(const wbx_metric *metric, int dtype, const void *X,
                    int64_t x_stride, const void *Y, int64_t y_stride,

@@ -37,6 +37,33 @@ struct DevGuard {
     explicit DevGuard(int dev) : lk(g_dev_mu[dev & 63]) {}
 };
 
+// ------------------------------------------------------ allocation-free scratch
+// Workspace of wbx_dev_pairwise_noalloc: a fixed array in device memory, one
+// copy per GPU the library runs on (held until the process ends).  Host code
+// never uses the array's name as an address: each device's copy lives at its
+// own address, found once per device with hipGetSymbolAddress (with that
+// device current) and kept in g_scratch_ptr.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[64] = {};
+
+// Address of the current device's scratch (device `dev` must be current).
+int scratch_setup(int dev, void **out) {
+    if (dev < 0 || dev >= 64) return fail(WBX_EUNSUPPORTED, "device %d: at most 64 devices", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr)
+            return fail(WBX_EDEVICE, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev,
+                        hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    if (out) *out = g_scratch_ptr[dev];
+    return WBX_OK;
+}
+
 // Carves one workspace reservation into aligned regions.
 struct Carve {
     size_t total = 0;
@@ -180,11 +207,16 @@ struct ProfState {
     std::vector<int> kind;
 } g_prof;
 
+// Set while the allocation-free entry point runs on this thread: no timing
+// events are created then.
+thread_local bool g_no_alloc = false;
+
 struct ProfScope {
     hipEvent_t a = nullptr, b = nullptr;
     hipStream_t st;
     int k;
     ProfScope(hipStream_t s, int kind_) : st(s), k(kind_) {
+        if (g_no_alloc) return;
         std::lock_guard<std::mutex> lk(g_prof.mu);
         if (!g_prof.on) return;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
@@ -228,7 +260,8 @@ int run_pairs(const wbx_metric *m, const PairSpec &ps, Series<T> xs, Series<T> y
             return launch_lane_dtw(ps, xs, ys, m->r, nullptr, sq, st, w_fixed, pen);
         }
     } else {
-        if (g_policy.load() == 0 && !xs.lens && !ys.lens && lane64_ok(ctx_current(), weights) &&
+        if (g_policy.load() == 0 && !xs.lens && !ys.lens &&
+            (!weights || lane64_ok(ctx_current(), weights)) &&
             lane64_pairs_class(make_band(xs.len, ys.len, m->r, w_fixed).B, weights != nullptr)) {
             ProfScope pf(st, KERN_LANE);
             return launch_lane64_pairs(ps, xs, ys, m->r, weights, sq, st, w_fixed, pen);
@@ -276,46 +309,70 @@ size_t stage_bytes(int dtype, bool want_double, int64_t n, int64_t len) {
     } while (0)
 
 // -------------------------------------------------------------- pairwise
+// Workspace of one pairwise call: staged inputs (when the element type is
+// not the compute type) and the squared cost of every pair.
+struct PairwiseLayout {
+    PairSpec ps{};
+    size_t o_x = 0, o_y = 0, o_sq = 0, total = 0;
+};
+
+template <typename T>
+PairwiseLayout pairwise_layout(int dtype, int64_t nx, int64_t tx, int64_t ny, int64_t ty,
+                               bool singleton) {
+    PairwiseLayout L;
+    if (singleton) {
+        L.ps.mode = PAIR_TRIANGLE;
+        L.ps.ny = nx;
+        L.ps.n_pairs = nx * (nx - 1) / 2;
+    } else {
+        L.ps.mode = PAIR_GRID;
+        L.ps.ny = ny;
+        L.ps.n_pairs = nx * ny;
+    }
+    const bool dbl = std::is_same<T, double>::value;
+    Carve cv;
+    L.o_x = cv.take(stage_bytes(dtype, dbl, nx, tx));
+    L.o_y = singleton ? 0 : cv.take(stage_bytes(dtype, dbl, ny, ty));
+    L.o_sq = cv.take((size_t)L.ps.n_pairs * sizeof(T));
+    L.total = cv.total;
+    return L;
+}
+
+// Everything one pairwise call enqueues, in the workspace `ws` of L.total
+// bytes (kernels only: no allocation, no memset).
+template <typename T>
+int pairwise_run(const PairwiseLayout &L, const wbx_metric *m, int dtype, const void *X,
+                 int64_t nx, int64_t xstr, int64_t tx, const void *Y, int64_t ystr, int64_t ty,
+                 bool singleton, const T *w, char *ws, double *out, hipStream_t st, int w_fixed) {
+    Staged<T> sx, sy;
+    RC(stage<T>(dtype, X, nx, tx, xstr, ws, L.o_x, &sx, st));
+    if (singleton)
+        sy = sx;
+    else
+        RC(stage<T>(dtype, Y, L.ps.ny, ty, ystr, ws, L.o_y, &sy, st));
+    T *sq = (T *)(ws + L.o_sq);
+    RC(run_pairs<T>(m, L.ps, series<T>(sx.ptr, sx.stride, tx), series<T>(sy.ptr, sy.stride, ty), w,
+                    sq, st, w_fixed));
+    if (singleton) RC(launch_fill_f64(out, nx * nx, 0.0, st));
+    return launch_finish<T>(L.ps, sq, out, singleton ? nx : L.ps.ny, st);
+}
+
 template <typename T>
 int pairwise_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *X, int64_t nx,
                   int64_t xstr, int64_t tx, const void *Y, int64_t ny, int64_t ystr, int64_t ty,
                   bool singleton, double *out, hipStream_t st, int w_fixed) {
-    PairSpec ps{};
-    if (singleton) {
-        ps.mode = PAIR_TRIANGLE;
-        ps.ny = nx;
-        ps.n_pairs = nx * (nx - 1) / 2;
-    } else {
-        ps.mode = PAIR_GRID;
-        ps.ny = ny;
-        ps.n_pairs = nx * ny;
-    }
-    const bool dbl = std::is_same<T, double>::value;
-    Carve cv;
-    const size_t o_x = cv.take(stage_bytes(dtype, dbl, nx, tx));
-    const size_t o_y = singleton ? 0 : cv.take(stage_bytes(dtype, dbl, ny, ty));
-    const size_t o_sq = cv.take((size_t)ps.n_pairs * sizeof(T));
+    const PairwiseLayout L = pairwise_layout<T>(dtype, nx, tx, ny, ty, singleton);
     void *wsp = nullptr;
-    RC(ws_reserve(ctx, cv.total, &wsp));
-    char *ws = (char *)wsp;
-    Staged<T> sx, sy;
-    RC(stage<T>(dtype, X, nx, tx, xstr, ws, o_x, &sx, st));
-    if (singleton)
-        sy = sx;
-    else
-        RC(stage<T>(dtype, Y, ny, ty, ystr, ws, o_y, &sy, st));
+    RC(ws_reserve(ctx, L.total, &wsp));
     const T *w = nullptr;
     if (m->kind == WBX_WDTW) {
         const double *w64;
         const float *w32;
         RC(cached_weights(ctx, tx > ty ? tx : ty, m->g, &w64, &w32));
-        w = dbl ? (const T *)w64 : (const T *)w32;
+        w = std::is_same<T, double>::value ? (const T *)w64 : (const T *)w32;
     }
-    T *sq = (T *)(ws + o_sq);
-    RC(run_pairs<T>(m, ps, series<T>(sx.ptr, sx.stride, tx), series<T>(sy.ptr, sy.stride, ty), w,
-                    sq, st, w_fixed));
-    if (singleton) RC(launch_fill_f64(out, nx * nx, 0.0, st));
-    return launch_finish<T>(ps, sq, out, singleton ? nx : ny, st);
+    return pairwise_run<T>(L, m, dtype, X, nx, xstr, tx, Y, ystr, ty, singleton, w, (char *)wsp,
+                           out, st, w_fixed);
 }
 
 // -------------------------------------------------------------- argmin
@@ -948,7 +1005,8 @@ int wbx_device_count(void) {
 
 int wbx_init(int device) {
     WBX_HIP_CHECK(hipSetDevice(device));
-    return ctx_current() ? WBX_OK : WBX_EDEVICE;
+    if (!ctx_current()) return WBX_EDEVICE;
+    return scratch_setup(device, nullptr);
 }
 
 // Not part of the reference interface: selects the kernel family (0 = the
@@ -1044,6 +1102,59 @@ int wbx_dev_pairwise(const wbx_metric *metric, int dtype, const void *X, int64_t
                                         singleton, out, st, w_fixed);
     int rc2 = end(ctx, st);
     return rc ? rc : rc2;
+}
+
+// wbx_dev_pairwise without allocation: the staged inputs and the per-pair
+// costs go to the fixed per-device scratch (kScratchBytes), so nothing is
+// allocated or freed and no memset is enqueued.  Metrics that need weights or
+// slopes (wdtw, ddtw, wddtw) and the FP32_ACC64 mode are refused.  Calls that
+// overlap on one GPU share its scratch and race.
+int wbx_dev_pairwise_noalloc(const wbx_metric *metric, int dtype, const void *X, int64_t nx,
+                             int64_t x_stride, int64_t tx, const void *Y, int64_t ny,
+                             int64_t y_stride, int64_t ty, double *out, void *stream) {
+    RC(check_metric(metric));
+    RC(check_sizes(nx, tx));
+    if (dtype != WBX_F32 && dtype != WBX_F64) return fail(WBX_EINVAL, "unknown dtype %d", dtype);
+    if (!X || !out) return fail(WBX_EINVAL, "NULL array");
+    const bool singleton = (Y == nullptr);
+    if (singleton) {
+        ny = nx;
+        ty = tx;
+        y_stride = x_stride;
+    } else {
+        RC(check_sizes(ny, ty));
+    }
+    if (x_stride < tx || y_stride < ty) return fail(WBX_EINVAL, "row stride below the series length");
+    if (metric->kind == WBX_WDTW || metric->kind == WBX_DDTW || metric->kind == WBX_WDDTW)
+        return fail(WBX_EUNSUPPORTED,
+                    "the allocation-free pairwise serves euclidean, dtw and adtw (this metric "
+                    "needs weights or slopes: use wbx_dev_pairwise)");
+    if (metric->precision == WBX_FP32_ACC64)
+        return fail(WBX_EUNSUPPORTED, "the allocation-free pairwise serves FP64 and FP32");
+    if (metric->kind == WBX_EUCLIDEAN && !singleton && tx != ty)
+        return fail(WBX_EINVAL, "Illegal n_timestep (%lld != %lld) for non-elastic distance measure",
+                    (long long)tx, (long long)ty);
+    int dev = 0;
+    WBX_HIP_CHECK(hipGetDevice(&dev));
+    void *ws = nullptr;
+    RC(scratch_setup(dev, &ws));
+    const bool dbl = metric->precision == WBX_FP64;
+    const PairwiseLayout L = dbl ? pairwise_layout<double>(dtype, nx, tx, ny, ty, singleton)
+                                 : pairwise_layout<float>(dtype, nx, tx, ny, ty, singleton);
+    if (L.total > kScratchBytes)
+        return fail(WBX_EINVAL,
+                    "allocation-free pairwise of %lld x %lld series needs %zu bytes of scratch; "
+                    "the maximum is %zu (use wbx_dev_pairwise)",
+                    (long long)nx, (long long)ny, L.total, kScratchBytes);
+    hipStream_t st = (hipStream_t)stream;
+    g_no_alloc = true;
+    const int rc =
+        dbl ? pairwise_run<double>(L, metric, dtype, X, nx, x_stride, tx, Y, y_stride, ty,
+                                   singleton, nullptr, (char *)ws, out, st, 0)
+            : pairwise_run<float>(L, metric, dtype, X, nx, x_stride, tx, Y, y_stride, ty, singleton,
+                                  nullptr, (char *)ws, out, st, 0);
+    g_no_alloc = false;
+    return rc;
 }
 
 int wbx_dev_paired(const wbx_metric *metric, int dtype, const void *X, int64_t x_stride,
