@@ -89,9 +89,31 @@ def test_knn_and_nearest_neighbors_match_the_oracle():
         assert i.shape == (80, 3) and not (i == np.arange(80)[:, None]).any()
 
 
+def test_multivariate_neighbours_match_the_restated_selection():
+    # (n, 3, t) inputs: the dim="mean" matrix, partial selection (kth = k - 1
+    # for kneighbors, kth = k for the class shares: _neighbors.py:115-135, 270-272)
+    rng = np.random.default_rng(140)
+    X = np.cumsum(rng.standard_normal((40, 3, 32)), axis=2)
+    Q = np.cumsum(rng.standard_normal((9, 3, 32)), axis=2)
+    y = rng.integers(0, 3, 40)
+    D = np.mean([oracle.pairwise(Q[:, d], X[:, d], metric="dtw", r=0.2) for d in range(3)], axis=0)
+    nn = NearestNeighbors(4, metric="dtw", metric_params={"r": 0.2}).fit(X)
+    d, i = nn.kneighbors(Q)
+    part = np.argpartition(D, 3, axis=1)[:, :4]
+    pd = np.take_along_axis(D, part, axis=1)
+    order = np.argsort(pd, axis=1)
+    np.testing.assert_array_equal(i, np.take_along_axis(part, order, axis=1))
+    np.testing.assert_array_equal(d, np.take_along_axis(pd, order, axis=1))
+    clf = KNeighborsClassifier(4, metric="dtw", metric_params={"r": 0.2}).fit(X, y)
+    votes = np.unique(y, return_inverse=True)[1][np.argpartition(D, 4, axis=1)[:, :4]]
+    want = np.stack([np.sum(votes == c, axis=1) / 4 for c in range(3)], axis=1)
+    np.testing.assert_array_equal(clf.predict_proba(Q), want)
+
+
 def _kmeans_restated(x, k, metric, r, random_state, max_iter=300, tol=1e-3, g=None):
     """_neighbors.py:483-564 over the oracle (one init)."""
-    rs = np.random.RandomState(random_state)
+    rs = (random_state if isinstance(random_state, np.random.RandomState)
+          else np.random.RandomState(random_state))
     if g is not None:
         metric = "wdtw"
     C = x[rs.choice(x.shape[0], size=k, replace=False)].copy()
@@ -135,25 +157,16 @@ def test_kmeans_matches_the_restated_loop(metric, g):
         X[:7], C, metric=pm, r=0.2, g=g or 0.05).argmin(axis=1))
 
 
-@pytest.mark.parametrize("algorithm,init", [("fast", "random"), ("fast", "auto"), ("pam", "auto"),
-                                            ("pam", "random"), ("fast", "min")])
-def test_kmedoids_on_gpu_distances_equals_precomputed_oracle(algorithm, init):
-    from wildboar_amd.neighbors import KMedoids
-
-    X = np.concatenate([rw(98, 15, 40), rw(99, 15, 40) + 3.0, rw(100, 15, 40) - 3.0])
-    D = oracle.pairwise(X, None, metric="dtw", r=0.3)
-    a = KMedoids(3, metric="dtw", metric_params={"r": 0.3}, algorithm=algorithm, init=init,
-                 random_state=5).fit(X)
-    b = KMedoids(3, metric="precomputed", algorithm=algorithm, init=init,
-                 random_state=5).fit(D)
-    np.testing.assert_array_equal(a.labels_, b.labels_)
-    np.testing.assert_array_equal(a.medoid_indices_, b.medoid_indices_)
-    assert a.inertia_ == b.inertia_
-    np.testing.assert_array_equal(a.predict(X[:5]),
-                                  oracle.pairwise(X[:5], X[a.medoid_indices_], metric="dtw",
-                                                  r=0.3).argmin(axis=1))
-    if algorithm == "pam":
-        assert sorted(m // 15 for m in a.medoid_indices_) == [0, 1, 2]
+def test_kmeans_several_inits_keep_the_best_run():
+    X = np.concatenate([rw(101, 15, 40), rw(102, 15, 40) + 2.5, rw(103, 15, 40) - 2.5])
+    km = KMeans(3, metric="dtw", r=0.3, n_init=3, random_state=4, max_iter=8).fit(X)
+    rs = np.random.RandomState(4)
+    runs = [_kmeans_restated(X, 3, "dtw", 0.3, rs, max_iter=8) for _ in range(3)]
+    best = min(range(3), key=lambda q: (runs[q][1], q))
+    a, cost, C, it = runs[best]
+    np.testing.assert_array_equal(km.labels_, a)
+    np.testing.assert_array_equal(km.cluster_centers_, C)
+    assert km.inertia_ == cost and km.n_iter_ == it
 
 
 @pytest.mark.parametrize("r,g,weighted", [(0.2, None, False), (1.0, None, True), (0.3, 0.1, False)])
