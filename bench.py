@@ -181,6 +181,66 @@ def run_batched(torch, dev, n_pairs, t_pad, reps):
     return out
 
 
+# ------------------------------------------------------- pairwise metric legs
+def run_pairwise_legs(torch, dev, X64, Y64, reps):
+    """The other legs of the pairwise 2000 x 2000 x 512 workload on the same
+    inputs: dtw r = 0.05 and r = 1.0, wdtw g = 0.05 (r = 0.1) and euclidean in
+    float32, and dtw r = 0.1 / r = 1.0 in float64 (the reference's
+    arithmetic).  Each leg: time per call, algorithmic cells/s, the kernel
+    family that ran, and a few rows against the float64 oracle (relative
+    error for float32, mismatches for float64)."""
+    import oracle
+    from wildboar_amd import _lib, device
+
+    n, t = X64.shape
+    X32 = torch.from_numpy(X64.astype(np.float32)).to(dev)
+    Y32 = torch.from_numpy(Y64.astype(np.float32)).to(dev)
+    Xd = torch.from_numpy(X64).to(dev)
+    Yd = torch.from_numpy(Y64).to(dev)
+    Xr = X64.astype(np.float32).astype(np.float64)
+    Yr = Y64.astype(np.float32).astype(np.float64)
+    out = torch.empty((n, n), dtype=torch.float64, device=dev)
+    legs = [("fp32_dtw_r0.05", "dtw", 0.05, 0.05, "fp32"), ("fp32_dtw_r1.0", "dtw", 1.0, 0.05, "fp32"),
+            ("fp32_wdtw_r0.1_g0.05", "wdtw", 0.1, 0.05, "fp32"),
+            ("fp32_euclidean", "euclidean", 1.0, 0.05, "fp32"),
+            ("fp64_dtw_r0.1", "dtw", 0.1, 0.05, "fp64"), ("fp64_dtw_r1.0", "dtw", 1.0, 0.05, "fp64")]
+    res = {"workload": f"pairwise_distance of {n}x{t} against {n}x{t} (seeds 10 / 11)"}
+    kinds = {1: "euclid", 2: "lane / strip", 3: "plain block"}
+    for name, metric, r, g, prec in legs:
+        Xa, Ya = (X32, Y32) if prec == "fp32" else (Xd, Yd)
+
+        def call():
+            device.pairwise(Xa, Ya, metric=metric, r=r, g=g, precision=prec, out=out)
+
+        call()
+        torch.cuda.synchronize()
+        _lib.prof_enable(True)
+        _lib.prof_take()
+        k = 1 if (r == 1.0 and metric != "euclidean") else reps
+        dt = timed(torch, call, k)
+        kern_ms, launches, kind = _lib.prof_take()
+        _lib.prof_enable(False)
+        if metric == "euclidean":
+            work, unit = float(n) * n * t, "MAC/s"
+        else:
+            work, unit = float(n) * n * band_cells(t, t, r), "cells/s"
+        rows = 2 if r == 1.0 else 4
+        got = out[:rows].cpu().numpy()
+        ox, oy = (Xr, Yr) if prec == "fp32" else (X64, Y64)
+        want = oracle.pairwise(ox[:rows], oy, metric=metric, r=r, g=g)
+        leg = {"ms": dt * 1e3, "kernel_ms": kern_ms / max(launches, 1), "work": work,
+               "value": work / dt, "unit": unit, "kernel": kinds.get(kind, kind)}
+        if prec == "fp32":
+            leg["max_rel_err_rows"] = float(np.max(np.abs(got - want) / np.abs(want)))
+        else:
+            leg["mismatch_rows"] = int((got != want).sum())
+        leg["rows_checked"] = rows
+        res[name] = leg
+        log(f"pairwise {name}: {dt * 1e3:.2f} ms, {work / dt / 1e12:.3f} T{unit} "
+            f"({leg['kernel']}); checked {rows} rows")
+    return res
+
+
 # ----------------------------------------------------------------------- train
 def run_train(torch, dev, reps, dba_steps):
     from wildboar_amd import device
@@ -417,6 +477,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="headline workload only")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-legs", action="store_true", help="skip the other pairwise metric legs")
     ap.add_argument("--no-train", action="store_true")
     ap.add_argument("--no-subsequence", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -548,10 +609,16 @@ def main():
         idx, _ = device.argmin(X, Y, 1, metric="dtw", r=r, precision="fp32")
         torch.cuda.synchronize()
         acc["argmin_1nn_ms"] = (time.perf_counter() - ta) * 1e3
-        oi, _ = oracle.argmin(Xs[:rows], Ys, k=1, metric="dtw", r=r)
-        got = labels[idx[:rows, 0].cpu().numpy()]
-        acc["label_mismatch"] = int((got != labels[oi[:, 0]]).sum())
-        acc["label_rows_checked"] = rows
+        # 1-NN labels of every row against the oracle's early-abandoning
+        # argmin (the reference's heap loop) on all host cores
+        tl = time.perf_counter()
+        oi, _ = oracle.argmin(Xs, Ys, k=1, metric="dtw", r=r, n_threads=os.cpu_count() or 1)
+        tl = time.perf_counter() - tl
+        gi = idx[:, 0].cpu().numpy()
+        acc["label_mismatch"] = int((labels[gi] != labels[oi[:, 0]]).sum())
+        acc["index_mismatch"] = int((gi != oi[:, 0]).sum())
+        acc["label_rows_checked"] = n
+        acc["oracle_argmin_seconds"] = tl
 
         # Plain one-pair-per-block kernel on a slice of the same workload.
         prow = 64
@@ -569,6 +636,8 @@ def main():
         log(f"plain {acc['plain_cells_per_s'] / 1e12:.4f} Tcells/s; max rel err "
             f"{acc['max_rel_err']:.2e}; label mismatches {acc['label_mismatch']}")
 
+        if not args.no_legs:
+            result["pairwise_legs"] = run_pairwise_legs(torch, dev, X64, Y64, 3)
         if not args.no_batched:
             result["batched"] = run_batched(torch, dev, args.batched_pairs, 2048, 5)
         if not args.no_train:

@@ -147,6 +147,32 @@ def test_pairwise_fp32_unequal(lx, ly, r):
     assert_fp32_close(np.atleast_2d(got), oracle.pairwise(X, Y, metric="dtw", r=r))
 
 
+@pytest.mark.parametrize("lx,ly,r,g", [(128, 128, 0.1, 0.05), (100, 130, 0.1, 0.3),
+                                        (130, 100, 0.2, 0.05), (37, 80, 0.2, 1.0),
+                                        (5, 3, 1.0, 0.05), (1, 9, 0.5, 0.05), (256, 256, 0.2, 0.05),
+                                        (64, 64, 0.0, 0.5)])
+def test_pairwise_fp32_wdtw_lane_kernel(lx, ly, r, g):
+    # FP32 WDTW on the lane kernel (per-slot weights, row-0 prefix weights
+    # w[j-1]); also the singleton and paired layouts, and the block kernel
+    X = np.random.default_rng(32).standard_normal((33, lx))
+    Y = np.random.default_rng(33).standard_normal((65, ly))
+    params = {"r": r, "g": g}
+    got = wd.pairwise_distance(X, Y, metric="wdtw", metric_params=params, precision="fp32")
+    want = oracle.pairwise(X, Y, metric="wdtw", r=r, g=g)
+    assert_fp32_close(np.atleast_2d(got), want)
+    try:
+        _lib.set_policy(1)
+        plain = wd.pairwise_distance(X, Y, metric="wdtw", metric_params=params, precision="fp32")
+    finally:
+        _lib.set_policy(0)
+    assert_fp32_close(np.atleast_2d(plain), want)
+    if lx == ly:
+        got = wd.pairwise_distance(X, metric="wdtw", metric_params=params, precision="fp32")
+        assert_fp32_close(got, oracle.pairwise(X, None, metric="wdtw", r=r, g=g))
+        got = wd.paired_distance(X, Y[:33], metric="wdtw", metric_params=params, precision="fp32")
+        assert_fp32_close(got, oracle.paired(X, Y[:33], metric="wdtw", r=r, g=g))
+
+
 def test_pairwise_fp32_plain_kernel_matches():
     X = rw(24, 40, 200)
     Y = rw(25, 50, 200)

@@ -158,7 +158,8 @@ def metric_struct(kind, precision, r=1.0, g=0.05, p=1.0):
 
 
 def set_policy(policy):
-    """0: fastest kernels (default); 1: the plain one-pair-per-block kernel only."""
+    """0: fastest kernels (default); 1: the plain one-pair-per-block kernel only;
+    2: the group kernel (n lanes per pair) wherever it applies."""
     check(load().wbx_set_policy(int(policy)))
 
 

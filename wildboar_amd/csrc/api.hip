@@ -29,7 +29,7 @@ using namespace wbx;
 
 namespace {
 
-std::atomic<int> g_policy{0};  // 0 = fastest kernel available, 1 = plain kernel only
+std::atomic<int> g_policy{0};  // 0 fastest kernels, 1 plain kernel only, 2 group kernel preferred
 std::mutex g_dev_mu[64];
 
 struct DevGuard {
@@ -234,11 +234,12 @@ struct ProfScope {
     }
 };
 
-enum { KERN_EUCLID = 1, KERN_LANE = 2, KERN_PLAIN = 3, KERN_BATCH = 4, KERN_SUBSEQ = 5 };
+enum { KERN_EUCLID = 1, KERN_LANE = 2, KERN_PLAIN = 3, KERN_BATCH = 4, KERN_SUBSEQ = 5,
+       KERN_GROUP = 6, KERN_FULL = 7 };
 
-// The float64 lane kernels multiply +inf (columns outside a series) by the
-// weights, so they take WDTW only when every cached weight is > 0.
-bool lane64_ok(const DeviceCtx *ctx, const double *weights) {
+// The lane kernels multiply +inf (columns outside a series) by the weights,
+// so they take WDTW only when every cached weight (float64 or float32) is > 0.
+bool lane64_ok(const DeviceCtx *ctx, const void *weights) {
     return !weights || cached_weights_min(ctx, weights) > 0.0;
 }
 
@@ -253,18 +254,46 @@ int run_pairs(const wbx_metric *m, const PairSpec &ps, Series<T> xs, Series<T> y
         return launch_euclid_pairs<T>(ps, xs, ys, sq, st);
     }
     const double *pen = m->kind == WBX_ADTW ? &m->p : nullptr;
-    if constexpr (std::is_same<T, float>::value) {
-        if (g_policy.load() == 0 && !weights && !xs.lens && !ys.lens &&
-            lane_supported(xs.len, ys.len, m->r, 0, w_fixed)) {
-            ProfScope pf(st, KERN_LANE);
-            return launch_lane_dtw(ps, xs, ys, m->r, nullptr, sq, st, w_fixed, pen);
+    const int policy = g_policy.load();
+    // Register-band kernels need one geometry, and every weight > 0 (a zero
+    // weight would meet the +inf of a column outside the series as NaN).
+    if (policy != 1 && !xs.lens && !ys.lens && (!weights || lane64_ok(ctx_current(), weights))) {
+        const Band bd = make_band(xs.len, ys.len, m->r, w_fixed);
+        const int B = bd.B;
+        const bool wt = weights != nullptr;
+        // the band is the whole matrix (w >= min(lx, ly), r = 1): lanes own
+        // columns, no slot outside the matrix
+        FullGeo fg{};
+        if (!wt && !pen && bd.w >= (xs.len < ys.len ? xs.len : ys.len) && policy != 2 &&
+            full_plan(xs.len, ys.len, !std::is_same<T, float>::value, &fg)) {
+            ProfScope pf(st, KERN_FULL);
+            return launch_full_dtw<T>(ps, xs, ys, sq, st);
         }
-    } else {
-        if (g_policy.load() == 0 && !xs.lens && !ys.lens &&
-            (!weights || lane64_ok(ctx_current(), weights)) &&
-            lane64_pairs_class(make_band(xs.len, ys.len, m->r, w_fixed).B, weights != nullptr)) {
+        // one pair per lane while that is faster than a group of lanes per
+        // pair (measured on MI355X, 1000 x 1000 x 256: float32 DTW lanes up
+        // to 112 slots; float32 WDTW and float64 lanes up to 40 slots -- at
+        // 49 slots the group kernel is 1.1-1.4x faster, at 23 the lanes
+        // 1.6-1.8x); ADTW stays on the lanes (no group form)
+        bool lane;
+        if constexpr (std::is_same<T, float>::value)
+            lane = lane_supported(xs.len, ys.len, m->r, wt, w_fixed) && (!wt || B <= 40);
+        else
+            lane = lane64_pairs_class(B, wt) != 0 && (B <= 40 || pen);
+        GroupGeo gg{};
+        const bool group = !pen && group_plan(B, bd.lx, !std::is_same<T, float>::value, wt, &gg);
+        if (policy == 2 && group) lane = false;  // group kernel wherever it applies
+        if (lane) {
             ProfScope pf(st, KERN_LANE);
-            return launch_lane64_pairs(ps, xs, ys, m->r, weights, sq, st, w_fixed, pen);
+            if constexpr (std::is_same<T, float>::value)
+                return launch_lane_dtw(ps, xs, ys, m->r, weights, sq, st, w_fixed, pen);
+            else
+                return launch_lane64_pairs(ps, xs, ys, m->r, weights, sq, st, w_fixed, pen);
+        }
+        // wider bands and the remaining weighted ones: a group of lanes per pair
+        if (group) {
+            ProfScope pf(st, KERN_GROUP);
+            const int nw = xs.len > ys.len ? xs.len : ys.len;
+            return launch_group_dtw<T>(ps, xs, ys, m->r, weights, nw, sq, st, w_fixed);
         }
     }
     ProfScope pf(st, KERN_PLAIN);
@@ -668,7 +697,7 @@ int subseq_other_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void
     const Series<double> xs = series<double>(xp, xs_stride, tt);
     ProfScope pf(st, KERN_SUBSEQ);
     int lane_cap = 0;
-    if (g_policy.load() == 0 && lane64_ok(ctx, w64)) {
+    if (g_policy.load() != 1 && lane64_ok(ctx, w64)) {
         RC(launch_subseq_lane64(m->r, sh, (int)nmin, (int)nmax, xs, n, w64, dist, idx, st, band_add,
                                 pen));
         lane_cap = lane64_cap(w64 != nullptr);
@@ -722,7 +751,7 @@ int subseq_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
         ProfScope pf(st, KERN_SUBSEQ);
         const Series<double> xs = series<double>(x64.ptr, x64.stride, t);
         int lane_cap = 0;
-        if (m->kind != WBX_EUCLIDEAN && g_policy.load() == 0 && lane64_ok(ctx, w64)) {
+        if (m->kind != WBX_EUCLIDEAN && g_policy.load() != 1 && lane64_ok(ctx, w64)) {
             RC(launch_subseq_lane64(m->r, sh, (int)min_len, (int)max_len, xs, n, w64, dist, idx,
                                     st));
             lane_cap = lane64_cap(w64 != nullptr);
@@ -811,7 +840,7 @@ int profile_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
     ProfScope pf(st, KERN_SUBSEQ);
     return launch_profile(mm.kind, mm.r, sh, paired, (int)min_len, (int)max_len,
                           series<double>(x64.ptr, x64.stride, t), n, nwp, w64,
-                          g_policy.load() == 0 && lane64_ok(ctx, w64), prof, st);
+                          g_policy.load() != 1 && lane64_ok(ctx, w64), prof, st);
 }
 
 // Host-pointer wrapper helpers.
@@ -878,7 +907,7 @@ int scaled_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *S,
     ProfScope pf(st, KERN_SUBSEQ);
     return launch_scaled_subseq(mm.kind, mm.r, s64.ptr, s_off, s_len, hl.data(), n_s, total,
                                 s_mean, s_std, x64.ptr, x64.stride, n, t, dist, idx, ws + o_k,
-                                g_policy.load() != 0, st);
+                                g_policy.load() == 1, st);
 }
 
 // The wrapped metrics of the same scan (ScaledSubsequenceMetricWrap,
@@ -985,7 +1014,7 @@ int scaled_profile_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const vo
                                           st);
     return launch_scaled_profile(mm.kind, mm.r, s64.ptr, s_off, s_len, hl.data(), n_s, total,
                                  s_mean, s_std, paired ? 1 : 0, x64.ptr, x64.stride, n, t, nwp,
-                                 threshold, prof, ws + o_k, g_policy.load() != 0, st);
+                                 threshold, prof, ws + o_k, g_policy.load() == 1, st);
 }
 
 }  // namespace
@@ -1010,9 +1039,10 @@ int wbx_init(int device) {
 }
 
 // Not part of the reference interface: selects the kernel family (0 = the
-// fastest available, 1 = the plain one-pair-per-block kernel everywhere).
+// fastest available, 1 = the plain one-pair-per-block kernel everywhere,
+// 2 = the group kernel wherever it applies; for tests and measurements).
 int wbx_set_policy(int policy) {
-    if (policy < 0 || policy > 1) return fail(WBX_EINVAL, "policy must be 0 or 1");
+    if (policy < 0 || policy > 2) return fail(WBX_EINVAL, "policy must be 0, 1 or 2");
     g_policy.store(policy);
     return WBX_OK;
 }

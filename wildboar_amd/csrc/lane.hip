@@ -18,6 +18,14 @@
 // 0 in slot off, which turns the first row into the reference's prefix sum.
 // Slots >= B (the padding up to BP) compute throw-away values; slot B is reset
 // to +inf after every row so slot B-1 never sees them.
+//
+// WDTW (WeightedDtwMetric, distance/_elastic.pyx:3322-3341) multiplies each
+// cell's squared difference by w[|i - j|]; in band coordinates |i - j| =
+// |k - off| is fixed per slot, so every lane of a launch uses the same weight
+// per slot: c = fma(d * d, wslot[k], m), four VALU instructions per cell.
+// The per-slot weights are staged in LDS once per block and re-read every row
+// (broadcast reads, no VGPRs held across rows).  Row 0 uses the reference's
+// prefix-sum weights w[j - 1] (w[0] for j = 0, :893-906), a second LDS row.
 #include "lane_common.h"
 
 namespace wbx {
@@ -28,9 +36,10 @@ namespace wbx {
 // AD: the amercing recursion (adtw_distance, distance/_elastic.pyx:943-1008),
 // m = min3(up + p, left + p, diag); row 0 runs with p = 0 (its prefix sum
 // carries no penalty).
-template <int BP, bool AD = false>
+template <int BP, bool AD = false, bool WT = false>
 __device__ __forceinline__ void lane_row(float (&row)[BP], const float *yw, float xi,
-                                         const float (&pen)[8], float p = 0.0f) {
+                                         const float (&pen)[8], float p = 0.0f,
+                                         const float *wslot = nullptr) {
     float left = WBX_INF_F;
 #pragma unroll
     for (int k = 0; k < BP; ++k) {
@@ -41,7 +50,11 @@ __device__ __forceinline__ void lane_row(float (&row)[BP], const float *yw, floa
         else
             m = fminf(fminf(up, left), row[k]);
         const float d = xi - yw[k];
-        const float c = __builtin_fmaf(d, d, m);
+        float c;
+        if constexpr (WT)
+            c = __builtin_fmaf(d * d, wslot[k], m);
+        else
+            c = __builtin_fmaf(d, d, m);
         row[k] = c;
         left = c;
     }
@@ -49,11 +62,23 @@ __device__ __forceinline__ void lane_row(float (&row)[BP], const float *yw, floa
     for (int q = 1; q < 8; ++q) row[BP - 8 + q] = fmaxf(row[BP - 8 + q], pen[q]);
 }
 
-template <int BP, int U, bool AD = false>
+template <int BP, int U, bool AD = false, bool WT = false>
 __global__ __launch_bounds__(64) void k_lane_dtw(PairSpec ps, Series<float> xs,
                                                  Series<float> ys, LaneGeo g, float pen_ad,
+                                                 const float *__restrict__ weights,
                                                  float *__restrict__ sq) {
     constexpr int W = BP + U - 1;
+    // WT: lw[0][k] row-0 weight of slot k, lw[1][k] = w[|k - off|]; slots
+    // outside the band get 1 (their cells are +inf or thrown away)
+    __shared__ float lw[WT ? 2 : 1][WT ? BP : 1];
+    if constexpr (WT) {
+        for (int k = threadIdx.x; k < BP; k += 64) {
+            const int j = k - g.off;  // column of slot k in row 0
+            lw[0][k] = (j >= 0 && k < g.B) ? weights[j > 0 ? j - 1 : 0] : 1.0f;
+            lw[1][k] = k < g.B ? weights[j < 0 ? -j : j] : 1.0f;
+        }
+        __syncthreads();
+    }
     const int64_t p0 = (int64_t)blockIdx.x * 64 + threadIdx.x;
     const int64_t p = p0 < ps.n_pairs ? p0 : ps.n_pairs - 1;
     int64_t a, b;
@@ -74,6 +99,13 @@ __global__ __launch_bounds__(64) void k_lane_dtw(PairSpec ps, Series<float> xs,
     for (int m = 0; m < W; ++m) yw[m] = ldy(Y, m - off, ly);
 
     int i_begin = 0;
+    if constexpr (WT) {
+        lane_row<BP, false, true>(row, yw, X[0], pen, 0.0f, lw[0]);
+#pragma unroll
+        for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
+        yw[W - 1] = ldy(Y, 1 - off + W - 1, ly);
+        i_begin = 1;
+    }
     if constexpr (AD) {
         lane_row<BP, true>(row, yw, X[0], pen, 0.0f);
 #pragma unroll
@@ -84,7 +116,7 @@ __global__ __launch_bounds__(64) void k_lane_dtw(PairSpec ps, Series<float> xs,
     // Leading rows one at a time so the rest is a whole number of U-groups.
     const int rem = (lx - i_begin) % U;
     for (int i = i_begin; i < i_begin + rem; ++i) {
-        lane_row<BP, AD>(row, yw, X[i], pen, pen_ad);
+        lane_row<BP, AD, WT>(row, yw, X[i], pen, pen_ad, WT ? lw[WT ? 1 : 0] : nullptr);
 #pragma unroll
         for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
         yw[W - 1] = ldy(Y, i + 1 - off + W - 1, ly);
@@ -97,7 +129,12 @@ __global__ __launch_bounds__(64) void k_lane_dtw(PairSpec ps, Series<float> xs,
             yn[u] = ldy(Y, i0 + U - off + W - U + u, ly);
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) lane_row<BP, AD>(row, yw + u, xv[u], pen, pen_ad);
+        for (int u = 0; u < U; ++u) {
+            // the weights are re-read from LDS every row rather than held in
+            // registers across the loop (the band row and y window fill them)
+            if constexpr (WT) asm volatile("" ::: "memory");
+            lane_row<BP, AD, WT>(row, yw + u, xv[u], pen, pen_ad, WT ? lw[WT ? 1 : 0] : nullptr);
+        }
 #pragma unroll
         for (int m = 0; m < W - U; ++m) yw[m] = yw[m + U];
 #pragma unroll
@@ -116,20 +153,23 @@ __global__ __launch_bounds__(64) void k_lane_dtw(PairSpec ps, Series<float> xs,
 
 template <int BP, int U>
 static void launch_one(const PairSpec &ps, Series<float> xs, Series<float> ys, LaneGeo g,
-                       const double *penalty, float *sq, hipStream_t st) {
+                       const double *penalty, const float *weights, float *sq, hipStream_t st) {
     dim3 grid((unsigned)((ps.n_pairs + 63) / 64));
     if (penalty)
-        hipLaunchKernelGGL((k_lane_dtw<BP, U, true>), grid, dim3(64), 0, st, ps, xs, ys, g,
-                           (float)*penalty, sq);
+        hipLaunchKernelGGL((k_lane_dtw<BP, U, true, false>), grid, dim3(64), 0, st, ps, xs, ys, g,
+                           (float)*penalty, nullptr, sq);
+    else if (weights)
+        hipLaunchKernelGGL((k_lane_dtw<BP, U, false, true>), grid, dim3(64), 0, st, ps, xs, ys, g,
+                           0.0f, weights, sq);
     else
-        hipLaunchKernelGGL((k_lane_dtw<BP, U, false>), grid, dim3(64), 0, st, ps, xs, ys, g, 0.0f,
-                           sq);
+        hipLaunchKernelGGL((k_lane_dtw<BP, U, false, false>), grid, dim3(64), 0, st, ps, xs, ys, g,
+                           0.0f, nullptr, sq);
 }
 
 constexpr int LANE_MAX_BP = 112;
 
 bool lane_supported(int lx, int ly, double r, int weighted, int w_fixed) {
-    if (weighted) return false;
+    (void)weighted;
     Band bd = make_band(lx, ly, r, w_fixed);
     return bd.B <= LANE_MAX_BP && lx >= 1 && ly >= 1;
 }
@@ -138,27 +178,27 @@ int launch_lane_dtw(const PairSpec &ps, Series<float> xs, Series<float> ys, doub
                     const float *weights, float *sq, hipStream_t st, int w_fixed,
                     const double *penalty) {
     if (ps.n_pairs <= 0) return WBX_OK;
-    if (weights) return fail(WBX_EUNSUPPORTED, "lane kernel: weighted DTW not supported");
+    if (weights && penalty) return fail(WBX_EUNSUPPORTED, "lane kernel: weights with a penalty");
     if (xs.lens || ys.lens) return fail(WBX_EUNSUPPORTED, "lane kernel: needs one geometry");
     Band bd = make_band(xs.len, ys.len, r, w_fixed);
     if (bd.B > LANE_MAX_BP) return fail(WBX_EUNSUPPORTED, "lane kernel: band %d too wide", bd.B);
     LaneGeo g{bd.lx, bd.ly, bd.off, bd.B, bd.ly - bd.lx + bd.off};
     const int BP = (bd.B + 7) / 8 * 8;
     switch (BP) {
-        case 8: launch_one<8, 8>(ps, xs, ys, g, penalty, sq, st); break;
-        case 16: launch_one<16, 8>(ps, xs, ys, g, penalty, sq, st); break;
-        case 24: launch_one<24, 8>(ps, xs, ys, g, penalty, sq, st); break;
-        case 32: launch_one<32, 8>(ps, xs, ys, g, penalty, sq, st); break;
-        case 40: launch_one<40, 8>(ps, xs, ys, g, penalty, sq, st); break;
-        case 48: launch_one<48, 8>(ps, xs, ys, g, penalty, sq, st); break;
-        case 56: launch_one<56, 8>(ps, xs, ys, g, penalty, sq, st); break;
-        case 64: launch_one<64, 8>(ps, xs, ys, g, penalty, sq, st); break;
-        case 72: launch_one<72, 8>(ps, xs, ys, g, penalty, sq, st); break;
-        case 80: launch_one<80, 8>(ps, xs, ys, g, penalty, sq, st); break;
-        case 88: launch_one<88, 8>(ps, xs, ys, g, penalty, sq, st); break;
-        case 96: launch_one<96, 4>(ps, xs, ys, g, penalty, sq, st); break;
-        case 104: launch_one<104, 4>(ps, xs, ys, g, penalty, sq, st); break;
-        case 112: launch_one<112, 4>(ps, xs, ys, g, penalty, sq, st); break;
+        case 8: launch_one<8, 8>(ps, xs, ys, g, penalty, weights, sq, st); break;
+        case 16: launch_one<16, 8>(ps, xs, ys, g, penalty, weights, sq, st); break;
+        case 24: launch_one<24, 8>(ps, xs, ys, g, penalty, weights, sq, st); break;
+        case 32: launch_one<32, 8>(ps, xs, ys, g, penalty, weights, sq, st); break;
+        case 40: launch_one<40, 8>(ps, xs, ys, g, penalty, weights, sq, st); break;
+        case 48: launch_one<48, 8>(ps, xs, ys, g, penalty, weights, sq, st); break;
+        case 56: launch_one<56, 8>(ps, xs, ys, g, penalty, weights, sq, st); break;
+        case 64: launch_one<64, 8>(ps, xs, ys, g, penalty, weights, sq, st); break;
+        case 72: launch_one<72, 8>(ps, xs, ys, g, penalty, weights, sq, st); break;
+        case 80: launch_one<80, 8>(ps, xs, ys, g, penalty, weights, sq, st); break;
+        case 88: launch_one<88, 8>(ps, xs, ys, g, penalty, weights, sq, st); break;
+        case 96: launch_one<96, 4>(ps, xs, ys, g, penalty, weights, sq, st); break;
+        case 104: launch_one<104, 4>(ps, xs, ys, g, penalty, weights, sq, st); break;
+        case 112: launch_one<112, 4>(ps, xs, ys, g, penalty, weights, sq, st); break;
         default: return fail(WBX_EUNSUPPORTED, "lane kernel: band %d", bd.B);
     }
     WBX_HIP_CHECK(hipGetLastError());

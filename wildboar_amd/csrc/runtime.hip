@@ -131,13 +131,14 @@ int cached_weights(DeviceCtx *ctx, int64_t n, double g, const double **w64, cons
         ctx->weights.clear();
         ctx->weight_bytes = 0;
     }
-    DeviceCtx::Weights e{n, g, nullptr, nullptr, 0.0};
+    DeviceCtx::Weights e{n, g, nullptr, nullptr, 0.0, 0.0f};
     if (hipMalloc(&e.w64, sizeof(double) * n) != hipSuccess) return fail(WBX_ENOMEM, "weights allocation failed");
     if (hipMalloc(&e.w32, sizeof(float) * n) != hipSuccess) {
         (void)hipFree(e.w64);
         return fail(WBX_ENOMEM, "weights allocation failed");
     }
     const int rc = fill_weights(n, g, e.w64, e.w32, &e.w_min);
+    e.w32_min = (float)e.w_min;  // rounding is monotone: the smallest float is the rounded smallest
     if (rc) {
         (void)hipFree(e.w64);
         (void)hipFree(e.w32);
@@ -150,10 +151,12 @@ int cached_weights(DeviceCtx *ctx, int64_t n, double g, const double **w64, cons
     return WBX_OK;
 }
 
-double cached_weights_min(const DeviceCtx *ctx, const double *w64) {
+double cached_weights_min(const DeviceCtx *ctx, const void *w) {
     if (!ctx) return 0.0;
-    for (const DeviceCtx::Weights &e : ctx->weights)
-        if (e.w64 == w64) return e.w_min;
+    for (const DeviceCtx::Weights &e : ctx->weights) {
+        if ((const void *)e.w64 == w) return e.w_min;
+        if ((const void *)e.w32 == w) return (double)e.w32_min;
+    }
     return 0.0;
 }
 

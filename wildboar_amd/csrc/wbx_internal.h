@@ -153,7 +153,8 @@ struct DeviceCtx {
         double g;
         double *w64;
         float *w32;
-        double w_min;  // smallest weight (the lane64 kernels need > 0)
+        double w_min;  // smallest weight (the lane kernels need > 0)
+        float w32_min;  // smallest float32 weight
     };
     std::vector<Weights> weights;
     size_t weight_bytes = 0;
@@ -166,8 +167,8 @@ int ws_dv_reserve(DeviceCtx *ctx, size_t bytes, void **out);
 // WDTW weights for width n and penalty g, computed on the host with libm exp
 // and cached on the device (one entry per (n, g) per device).
 int cached_weights(DeviceCtx *ctx, int64_t n, double g, const double **w64, const float **w32);
-// Smallest weight of a cached float64 vector (0 if w64 is not one).
-double cached_weights_min(const DeviceCtx *ctx, const double *w64);
+// Smallest weight of a cached float64 or float32 vector (0 if w is not one).
+double cached_weights_min(const DeviceCtx *ctx, const void *w);
 // Fills a caller-owned pair of device vectors with the weights for (n, g).
 int fill_weights(int64_t n, double g, double *w64, float *w32, double *w_min);
 
@@ -195,6 +196,30 @@ int launch_lane_dtw(const PairSpec &ps, Series<float> xs, Series<float> ys,
                     double r, const float *weights, float *sq, hipStream_t st, int w_fixed = 0,
                     const double *penalty = nullptr);
 bool lane_supported(int lx, int ly, double r, int weighted, int w_fixed = 0);
+
+// Group kernel (group.hip): one geometry per launch, n lanes per pair, 64 / n
+// pairs per wave; any band width, DTW and WDTW (every weight > 0), float32
+// (FMA) or float64 (the reference's arithmetic, bit-exact).
+struct GroupGeo {
+    int lx, ly, off, B, kf;  // band geometry shared by every pair
+    int n, a, P, R;          // lanes per pair, lanes with R + 1 slots, pairs per wave, strip size
+    int nw;                  // weight vector length
+};
+bool group_plan(int B, int lx, bool fp64, bool weighted, GroupGeo *g);
+
+// Full-matrix kernel (full.hip): DTW whose band covers the whole matrix
+// (w >= min(lx, ly), the default r = 1), one geometry per launch, lanes own
+// columns; float32 (FMA) or float64 (bit-exact).
+struct FullGeo {
+    int lx, ly;
+    int n, P, S;  // lanes per pair, pairs per wave, columns per lane
+};
+bool full_plan(int lx, int ly, bool fp64, FullGeo *g);
+template <typename T>
+int launch_full_dtw(const PairSpec &ps, Series<T> xs, Series<T> ys, T *sq, hipStream_t st);
+template <typename T>
+int launch_group_dtw(const PairSpec &ps, Series<T> xs, Series<T> ys, double r, const T *weights,
+                     int n_weights, T *sq, hipStream_t st, int w_fixed = 0);
 
 // sqrt + scatter of per-pair squared costs into the driver's output layout.
 template <typename T>
