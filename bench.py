@@ -482,7 +482,7 @@ def main():
     ap.add_argument("--no-subsequence", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--batched-pairs", type=int, default=50000)
-    ap.add_argument("--subsequence-series", type=int, default=1000)
+    ap.add_argument("--subsequence-series", type=int, default=10000)
     ap.add_argument("--dba-steps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", action="store_true",

@@ -81,13 +81,21 @@ int wbx_device_count(void);
 
 /* No counterpart in the reference (tuning / measurement).  Kernel family:
  * 0 = fastest available (default), 1 = the plain one-pair-per-block kernel
- * for every DTW request (the "plain" baseline of the benchmarks). */
+ * for every DTW request (the "plain" baseline of the benchmarks), 2 = the
+ * group kernel (n lanes per pair) wherever it applies. */
 int wbx_set_policy(int policy);
+/* No counterpart in the reference (the reference computes in float64).
+ * WBX_FP32 requests of pairwise / paired / argmin / batched pairs whose
+ * series are longer than `len` run as WBX_FP32_ACC64 (float32 data, float64
+ * accumulation): past ~2048 steps the float32 path cost drifts beyond a
+ * 1e-5 relative error.  Default 2048; 0 never promotes. */
+int wbx_set_acc64_length(int64_t len);
 /* No counterpart in the reference (measurement).  While enabled, HIP events
  * are recorded on the launch stream around every distance kernel;
  * wbx_prof_take waits for them, returns the summed milliseconds, the number
  * of kernels and the kind of the last one (1 euclidean, 2 lane-per-pair DTW,
- * 3 plain DTW, 4 batched DTW, 5 subsequence scan), and clears the record. */
+ * 3 plain DTW, 4 batched DTW, 5 subsequence scan, 6 group of lanes per pair,
+ * 7 full-matrix DTW), and clears the record. */
 int wbx_prof_enable(int on);
 int wbx_prof_take(double *total_ms, int64_t *launches, int *last_kind);
 /* No counterpart in the reference (measurement).  Runs a VALU probe made of

@@ -124,6 +124,56 @@ def test_grid_gradient_wave_reduced_rows(ny):
     np.testing.assert_allclose(Yd.grad.cpu().numpy(), wy, rtol=1e-5, atol=1e-5)
 
 
+def _grid_oracle(X, Y, G, r):
+    wx = np.zeros_like(X, dtype=np.float64)
+    wy = np.zeros_like(Y, dtype=np.float64)
+    for a in range(X.shape[0]):
+        for b in range(Y.shape[0]):
+            _, ox, oy = oracle.dtw_grad(X[a].astype(np.float64), Y[b].astype(np.float64), r)
+            wx[a] += G[a, b] * ox
+            wy[b] += G[a, b] * oy
+    return wx, wy
+
+
+def test_grid_gradient_more_x_rows_than_one_column_partial():
+    # 300 x series: the y-side column terms are summed in partials of 256
+    # series and then across partials
+    X = ints(23, 300, 16)
+    Y = ints(24, 3, 16)
+    Xd = torch.from_numpy(X).cuda().requires_grad_(True)
+    Yd = torch.from_numpy(Y).cuda().requires_grad_(True)
+    G = np.random.default_rng(25).uniform(0.1, 1.0, (300, 3))
+    (dtw(Xd, Yd, 0.3) * torch.from_numpy(G).cuda()).sum().backward()
+    wx, wy = _grid_oracle(X, Y, G, 0.3)
+    np.testing.assert_allclose(Xd.grad.cpu().numpy(), wx, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(Yd.grad.cpu().numpy(), wy, rtol=1e-5, atol=1e-4)
+
+
+@pytest.mark.parametrize("tx,ty", [(37, 50), (50, 37)])
+def test_gradient_unequal_lengths(tx, ty):
+    # unequal lengths: the band's clamps at both ends and the final cell's slot
+    X = ints(26, 6, tx)
+    Y = ints(27, 6, ty)
+    G = np.random.default_rng(28).uniform(0.5, 1.5, (6, 6))
+    Xd = torch.from_numpy(X).cuda().requires_grad_(True)
+    Yd = torch.from_numpy(Y).cuda().requires_grad_(True)
+    D = dtw(Xd, Yd, 0.2)
+    want = oracle.pairwise(X.astype(np.float64), Y.astype(np.float64), metric="dtw", r=0.2)
+    assert np.max(np.abs(D.detach().cpu().numpy() - want) / np.maximum(want, 1e-3)) <= 1e-5
+    (D * torch.from_numpy(G).cuda()).sum().backward()
+    wx, wy = _grid_oracle(X, Y, G, 0.2)
+    np.testing.assert_allclose(Xd.grad.cpu().numpy(), wx, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(Yd.grad.cpu().numpy(), wy, rtol=1e-5, atol=1e-5)
+    Xp = torch.from_numpy(X).cuda().requires_grad_(True)
+    Yp = torch.from_numpy(Y).cuda().requires_grad_(True)
+    (paired_dtw(Xp, Yp, 0.2) * torch.from_numpy(G[:, 0].copy()).cuda()).sum().backward()
+    for i in range(6):
+        _, ox, oy = oracle.dtw_grad(X[i].astype(np.float64), Y[i].astype(np.float64), 0.2)
+        scale = G[i, 0] * max(np.abs(ox).max(), np.abs(oy).max(), 1e-3)
+        assert_grad_close(Xp.grad.cpu().numpy()[i], G[i, 0] * ox, scale)
+        assert_grad_close(Yp.grad.cpu().numpy()[i], G[i, 0] * oy, scale)
+
+
 def same_path(x, y, r, band):
     pi, pj = oracle.mapping(oracle.cost_matrix(x, y, band))
     return pi, pj

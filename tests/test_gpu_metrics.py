@@ -206,3 +206,35 @@ def test_fp64_wide_classes_in_subsequence_scans_and_profiles(metric, params):
                                        g=p["p"] if metric == "adtw" else p["g"])
     np.testing.assert_array_equal(gi, oi)
     np.testing.assert_array_equal(gd, od)
+
+
+def test_fp32_long_series_accumulate_in_float64():
+    # FP32 requests on series past 2048 steps run with float64 accumulation:
+    # the result is the oracle's on the float32-rounded inputs, bit for bit,
+    # and within 1e-5 of the float64 oracle.  With the promotion off, the
+    # float32 recursion's drift at n = 4096 is stated (asserted below 1e-3,
+    # reported in the assertion message).
+    from wildboar_amd import _lib as lib
+
+    X = rw(140, 3, 4096)
+    Y = rw(141, 4, 4096)
+    Xr = X.astype(np.float32).astype(np.float64)
+    Yr = Y.astype(np.float32).astype(np.float64)
+    for r in (0.1, 1.0):
+        want = oracle.pairwise(X, Y, metric="dtw", r=r)
+        want_r = oracle.pairwise(Xr, Yr, metric="dtw", r=r)
+        got = wd.pairwise_distance(X, Y, metric="dtw", metric_params={"r": r}, precision="fp32")
+        np.testing.assert_array_equal(got, want_r)
+        assert np.max(np.abs(got - want) / want) <= 1e-5
+        lib.set_acc64_length(0)
+        try:
+            raw = wd.pairwise_distance(X, Y, metric="dtw", metric_params={"r": r},
+                                       precision="fp32")
+        finally:
+            lib.set_acc64_length(2048)
+        drift = float(np.max(np.abs(raw - want) / want))
+        assert drift <= 1e-3, f"float32 accumulation at n=4096, r={r}: max rel err {drift:.2e}"
+        print(f"n=4096 r={r}: float32 accumulation max rel err {drift:.3e}, "
+              f"float64 accumulation {np.max(np.abs(got - want) / want):.3e}")
+    idx = wd.argmin_distance(X, Y, k=2, metric="dtw", metric_params={"r": 0.1}, precision="fp32")
+    np.testing.assert_array_equal(idx, oracle.argmin(Xr, Yr, k=2, metric="dtw", r=0.1)[0])

@@ -25,7 +25,7 @@ PRECISIONS = {"fp64": FP64, "fp32": FP32, "fp32_acc64": FP32_ACC64}
 EXPORTS = (
     "wbx_init", "wbx_last_error", "wbx_version", "wbx_device_count",
     "wbx_pairwise", "wbx_paired", "wbx_argmin", "wbx_subsequence_pairwise",
-    "wbx_dev_pairwise", "wbx_dev_pairwise_noalloc", "wbx_dev_paired", "wbx_dev_argmin",
+    "wbx_dev_pairwise", "wbx_dev_pairwise_noalloc", "wbx_set_acc64_length", "wbx_dev_paired", "wbx_dev_argmin",
     "wbx_dev_subsequence_pairwise", "wbx_set_policy", "wbx_prof_enable", "wbx_prof_take",
     "wbx_prof_valu_peak", "wbx_dev_eos_lengths", "wbx_batch_create", "wbx_batch_free",
     "wbx_batch_set_launch", "wbx_batch_autotune", "wbx_batch_bucket_info", "wbx_batch_info", "wbx_batch_run", "wbx_dtw_train_sizes", "wbx_dev_dtw_train_forward",
@@ -110,6 +110,7 @@ def load():
             "wbx_subsequence_pairwise": (i32, [m, p, p, p, i64, p, i64, i64, i64, i64, p, p]),
             "wbx_dev_pairwise": (i32, [m, i32, p, i64, i64, i64, p, i64, i64, i64, p, p]),
             "wbx_dev_pairwise_noalloc": (i32, [m, i32, p, i64, i64, i64, p, i64, i64, i64, p, p]),
+            "wbx_set_acc64_length": (i32, [i64]),
             "wbx_dev_paired": (i32, [m, i32, p, i64, p, i64, i64, i64, p, p]),
             "wbx_dev_argmin": (i32, [m, i32, p, i64, i64, i64, p, i64, i64, i64, i64, p, p, p, p]),
             "wbx_dev_subsequence_pairwise": (i32, [m, i32, p, p, p, i64, p, i64, i64, i64, p, p, p]),
@@ -161,6 +162,12 @@ def set_policy(policy):
     """0: fastest kernels (default); 1: the plain one-pair-per-block kernel only;
     2: the group kernel (n lanes per pair) wherever it applies."""
     check(load().wbx_set_policy(int(policy)))
+
+
+def set_acc64_length(n):
+    """FP32 requests on series longer than ``n`` run with float64 accumulation
+    (default 2048; 0 never promotes)."""
+    check(load().wbx_set_acc64_length(int(n)))
 
 
 def prof_enable(on=True):

@@ -1052,6 +1052,15 @@ int wbx_set_policy(int policy) {
 // wbx_prof_take waits for them and returns the summed milliseconds, the number
 // of launches and the kind of the last one (1 euclid, 2 lane, 3 plain), then
 // clears the record.
+// Not part of the reference interface: the series length above which FP32
+// requests of pairwise / paired / argmin run with float64 accumulation
+// (FP32_ACC64); default 2048, 0 = never.
+int wbx_set_acc64_length(int64_t len) {
+    if (len < 0) return fail(WBX_EINVAL, "length must be >= 0");
+    set_acc64_length(len);
+    return WBX_OK;
+}
+
 int wbx_prof_enable(int on) {
     std::lock_guard<std::mutex> lk(g_prof.mu);
     g_prof.on = on != 0;
@@ -1105,6 +1114,7 @@ int wbx_dev_pairwise(const wbx_metric *metric, int dtype, const void *X, int64_t
     hipStream_t st = pick_stream(ctx, stream);
     RC(begin(ctx, st));
     wbx_metric mm = *metric;
+    promote_long(&mm, tx > ty ? tx : ty);
     int rc = acc64_prepare(ctx, &mm, dtype,
                            {{&X, nx, tx, &x_stride}, {singleton ? nullptr : &Y, ny, ty, &y_stride}},
                            st);
@@ -1200,6 +1210,7 @@ int wbx_dev_paired(const wbx_metric *metric, int dtype, const void *X, int64_t x
     // _paired_distance binds its rows to the caller's y: out[i] = D(Y[i], X[i]).
     PairSpec ps{PAIR_DIAG, n, n, nullptr, nullptr};
     wbx_metric mm = *metric;
+    promote_long(&mm, t);
     int rc = acc64_prepare(ctx, &mm, dtype, {{&X, n, t, &x_stride}, {&Y, n, t, &y_stride}}, st);
     int w_fixed = 0;
     if (!rc && (mm.kind == WBX_DDTW || mm.kind == WBX_WDDTW)) {
@@ -1271,6 +1282,7 @@ int wbx_dev_argmin(const wbx_metric *metric, int dtype, const void *X, int64_t n
     hipStream_t st = pick_stream(ctx, stream);
     RC(begin(ctx, st));
     wbx_metric mm = *metric;
+    promote_long(&mm, tx > ty ? tx : ty);
     int rc = acc64_prepare(ctx, &mm, dtype, {{&X, nx, tx, &x_stride}, {&Y, ny, ty, &y_stride}}, st);
     if (!rc && (mm.kind == WBX_DDTW || mm.kind == WBX_WDDTW)) {
         if ((tx < ty ? tx : ty) < 3) {

@@ -870,6 +870,9 @@ int wbx_batch_create(const wbx_metric *metric, int64_t n_pairs, int64_t t_pad,
     if (!ctx) return WBX_EDEVICE;
     BatchPlan *b = new BatchPlan();
     b->metric = *metric;
+    int32_t longest = 0;
+    for (int64_t q = 0; q < n_pairs; ++q) longest = std::max(longest, std::max(len_x[q], len_y[q]));
+    promote_long(&b->metric, longest);
     b->n_pairs = n_pairs;
     b->t_pad = t_pad;
     b->policy = policy;

@@ -1,4 +1,5 @@
 // runtime.hip -- error state, per-device context and scratch arena.
+#include <atomic>
 #include <math.h>
 #include <mutex>
 #include <stdarg.h>
@@ -91,6 +92,17 @@ int ws_in_reserve(DeviceCtx *ctx, size_t bytes, void **out) {
 
 int ws_dv_reserve(DeviceCtx *ctx, size_t bytes, void **out) {
     return arena_reserve(ctx->ws_dv, bytes, out);
+}
+
+// FP32 requests whose DP has more rows or columns than this run with float64
+// accumulation (FP32_ACC64).  0 disables the promotion.
+static std::atomic<int64_t> g_acc64_len{2048};
+
+void set_acc64_length(int64_t len) { g_acc64_len.store(len); }
+
+void promote_long(wbx_metric *m, int64_t len) {
+    const int64_t lim = g_acc64_len.load();
+    if (m->precision == WBX_FP32 && lim > 0 && len > lim) m->precision = WBX_FP32_ACC64;
 }
 
 // WDTW weights 1 / (1 + exp(-g (i - n/2))) with the host libm exp, as

@@ -161,6 +161,10 @@ struct DeviceCtx {
 };
 
 DeviceCtx *ctx_current();              // current device's context (created lazily)
+// FP32 with float64 accumulation past a DP length: past ~2048 rows the float32
+// path cost drifts beyond a 1e-5 relative error (default 2048, 0 = never).
+void set_acc64_length(int64_t len);
+void promote_long(wbx_metric *m, int64_t len);
 int ws_reserve(DeviceCtx *ctx, size_t bytes, void **out);
 int ws_in_reserve(DeviceCtx *ctx, size_t bytes, void **out);
 int ws_dv_reserve(DeviceCtx *ctx, size_t bytes, void **out);
