@@ -393,24 +393,23 @@ int wbx_batch_create(const wbx_metric *metric, int64_t n_pairs, int64_t t_pad,
                      const int32_t *len_x, const int32_t *len_y,
                      const double *r_per_pair, int policy, void **plan);
 int wbx_batch_free(void *plan);
-/* Launch shape of the strip kernel's buckets: at most waves_per_cu persistent
- * waves per CU for each bucket (1..32, default 8) and the buckets spread over
- * n_streams forked streams (1..4, default 4).  Drops a captured graph.  The
- * results do not depend on it. */
+/* Launch shape of the strip kernel: at most waves_per_cu persistent waves
+ * per CU (1..32; default 12 for float32, 8 for float64 strips).  n_streams
+ * (1..4) is accepted and ignored: a run uses the caller's stream only.
+ * Drops a captured graph.  The results do not depend on it. */
 int wbx_batch_set_launch(void *plan, int waves_per_cu, int n_streams);
-/* Autotuner: for each strip bucket (pairs of one strip size), times `reps`
- * launches of the bucket's kernel alone on `stream` with the caller's data
- * for waves per CU in {4, 6, 8, 12}, keeps the fastest, and spreads
- * the buckets over the streams longest-measured first.  tuned_ms = the sum of
- * the buckets' best times, or -1 when the choice came from the process-wide
- * cache of plans with the same bucket signature, or when the plan has no
- * strip pairs.  Results do not depend on it.  Drops a captured graph. */
+/* Autotuner: times `reps` strip launches alone on `stream` with the caller's
+ * data for waves per CU in {4, 6, 8, 10, 12, 16} and keeps the fastest.
+ * tuned_ms = the best time, or -1 when the choice came from the
+ * process-wide cache of plans with the same signature, or when the plan has
+ * no strip pairs.  Results do not depend on it.  Drops a captured graph. */
 int wbx_batch_autotune(void *plan, int dtype, const void *X, int64_t x_stride,
                        const void *Y, int64_t y_stride, void *stream, int reps,
                        float *tuned_ms);
-/* Bucket k of the plan: strip size R, pairs, persistent waves, stream index
- * and measured time (ms, -1 if not autotuned); k < 0 only reports the number
- * of buckets. */
+/* Strip size class k of the plan (the pairs of one strip size R, in queue
+ * order, widest first): R, pairs, the strip launch's persistent waves, stream
+ * index (always 0) and the tuned launch time (ms, -1 if not autotuned); k < 0
+ * only reports the number of classes. */
 int wbx_batch_bucket_info(void *plan, int k, int *n_buckets, int *R, int *pairs,
                           int *waves, int *stream, float *ms);
 /* Algorithmic band cells of the plan and how many pairs each kernel takes. */

@@ -122,12 +122,17 @@ def test_cfg2_nearest_neighbour_labels_exact(metric, params, precision):
 
 
 # ----------------------------------------------------------------- batched
-def _plan_R(lx, ly, r):
+# strip sizes the batched kernel is compiled for (float32 / float64 pairs)
+STRIP_R32 = (2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 18, 20, 22, 24, 26, 28, 30,
+             32)
+STRIP_R64 = (2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 18, 20, 24, 28, 32)
+
+
+def _plan_R(lx, ly, r, table):
     """Strip size the plan gives a pair: the smallest compiled R whose 64
-    lanes cover the band (shorter series on the rows)."""
+    lanes cover the band (shorter series on the rows) with n R <= B."""
     import math
 
-    table = (2, 3, 4, 6, 8, 10, 12, 14, 16, 20, 24, 28, 32, 40)
     a, c = min(lx, ly), max(lx, ly)
     w = max(math.floor(a * r), 1)
     B = (c - a) + 2 * w - 1
@@ -141,19 +146,23 @@ def _plan_R(lx, ly, r):
     return None
 
 
+def _pick_per_size(lx, ly, r, table, per):
+    Rs = np.array([_plan_R(int(a), int(b), float(q), table) for a, b, q in zip(lx, ly, r)])
+    pick = []
+    for R in np.unique(Rs[Rs != None]):  # noqa: E711
+        pick.extend(np.flatnonzero(Rs == R)[:per].tolist())
+    return np.array(sorted(pick)), set(int(R) for R in Rs if R is not None)
+
+
 def test_uneven_pairs_2048_every_strip_size():
     from bench import uneven_pairs
 
     X, Y, lx, ly, r = uneven_pairs(3000, 2048)
-    Rs = np.array([_plan_R(int(a), int(b), float(q)) for a, b, q in zip(lx, ly, r)])
-    pick = []
-    for R in np.unique(Rs):
-        pick.extend(np.flatnonzero(Rs == R)[:24].tolist())
-    pick = np.array(sorted(pick))
+    pick, sizes = _pick_per_size(lx, ly, r, STRIP_R32, 12)
     Xs, Ys, lxs, lys, rs = X[pick], Y[pick], lx[pick], ly[pick], r[pick]
     plan = DtwBatch(lxs, lys, 2048, rs, precision="fp32")
-    got_R = sorted(b["R"] for b in plan.buckets() if b["pairs"] > 0)
-    assert set(range(2, 33)) & set(got_R) == {2, 3, 4, 6, 8, 10, 12, 14, 16, 20, 24, 28, 32}
+    got_R = set(b["R"] for b in plan.buckets() if b["pairs"] > 0)
+    assert got_R == sizes and {2, 12, 24} <= got_R and max(got_R) >= 28
     assert plan.n_plain == 0
     Xd = torch.from_numpy(Xs).cuda()
     Yd = torch.from_numpy(Ys).cuda()
@@ -166,4 +175,30 @@ def test_uneven_pairs_2048_every_strip_size():
             got = plan.run(Xd, Yd, graph=graph)
         side.synchronize()
         assert rel_err(got.cpu().numpy(), want) <= FP32_RTOL
+    plan.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_acc64"])
+def test_uneven_pairs_2048_float64_strips_bitwise(precision):
+    # the float64 strip kernel: bit-identical to the oracle on the values it
+    # computes with (float64 as given, or the float32 data of the pairs)
+    from bench import uneven_pairs
+
+    X, Y, lx, ly, r = uneven_pairs(1500, 2048)
+    pick, sizes = _pick_per_size(lx, ly, r, STRIP_R64, 5)
+    Xs = X[pick].astype(np.float64)
+    Ys = Y[pick].astype(np.float64)
+    if precision == "fp64":  # float64 values that are not float32
+        rng = np.random.default_rng(9)
+        for arr, L in ((Xs, lx[pick]), (Ys, ly[pick])):
+            for p in range(len(pick)):
+                arr[p, :L[p]] += rng.standard_normal(L[p]) * 1e-3
+    lxs, lys, rs = lx[pick], ly[pick], r[pick]
+    plan = DtwBatch(lxs, lys, 2048, rs, precision=precision)
+    assert set(b["R"] for b in plan.buckets() if b["pairs"] > 0) == sizes
+    assert plan.n_plain == 0
+    got = plan.run(torch.from_numpy(Xs).cuda(), torch.from_numpy(Ys).cuda(), graph=False)
+    want = np.array([oracle.metric_distance(Xs[p, :lxs[p]], Ys[p, :lys[p]], "dtw", r=rs[p])
+                     for p in range(len(pick))])
+    np.testing.assert_array_equal(got.cpu().numpy(), want)
     plan.close()

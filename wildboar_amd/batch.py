@@ -6,9 +6,11 @@ marker (``variable_len.EOS``) after the last value; pair ``p`` is
 ``r[p]``.  The result is ``DTW(X[p], Y[p])`` exactly as ``DtwMetric``
 computes it for the unpadded pair (``distance/_elastic.pyx:3167-3186``).
 
-``DtwBatch`` plans once (host: geometry, routing, cost-sorted work lists) and
-then runs on device pointers; with ``graph=True`` the run is captured into a
-hipGraph on first use and replayed afterwards.
+``DtwBatch`` plans once (host: geometry, routing, one queue sorted by strip
+size and cost) and then runs on device pointers: one persistent strip launch
+on the caller's stream (float32, or float64 for ``fp64`` / ``fp32_acc64``).
+With ``graph=True`` the run is captured into a hipGraph on first use and
+replayed afterwards.
 """
 
 import ctypes
@@ -92,15 +94,17 @@ class DtwBatch:
         self.n_strip = ns.value
         self.n_plain = npl.value
 
-    def set_launch(self, waves_per_cu=8, n_streams=4):
-        """Launch shape of the strip buckets (results do not depend on it)."""
+    def set_launch(self, waves_per_cu=12, n_streams=1):
+        """Persistent waves per CU of the strip launch (results do not depend
+        on it); ``n_streams`` is accepted for compatibility and ignored."""
         _lib.check(self._lib.wbx_batch_set_launch(self._plan, int(waves_per_cu), int(n_streams)))
 
     def autotune(self, X, Y, reps=2):
-        """Time each strip bucket's launch shapes on (X, Y) and keep the fastest.
+        """Time the strip launch at several waves per CU on (X, Y) and keep
+        the fastest.
 
-        Returns the total of the buckets' best times in ms (None when the
-        choice came from the process-wide cache, or there is nothing to tune).
+        Returns the best time in ms (None when the choice came from the
+        process-wide cache, or there is nothing to tune).
         """
         self._check_inputs(X, Y)
         ms = ctypes.c_float()
@@ -111,7 +115,8 @@ class DtwBatch:
         return None if ms.value < 0 else float(ms.value)
 
     def buckets(self):
-        """Per strip bucket: dict(R, pairs, waves, stream, ms)."""
+        """Per strip size: dict(R, pairs, waves, stream, ms) (waves and ms of
+        the one strip launch)."""
         nb = ctypes.c_int()
         _lib.check(self._lib.wbx_batch_bucket_info(self._plan, -1, ctypes.byref(nb), None, None,
                                                    None, None, None))
