@@ -13,16 +13,21 @@ Printed JSON line (stdout):
   cpu_baseline  the float64 C restatement of the reference, all host cores,
                 on a bounded slice of the headline workload
   accuracy      FP32 max relative error against the float64 oracle, 1-NN
-                label mismatches, the plain one-pair-per-block kernel on a
-                slice of the same workload
+                labels and indices of all 2000 rows against the oracle's
+                argmin, the plain one-pair-per-block kernel on a slice of the
+                same workload
+  pairwise_legs the other legs of the same pairwise workload: dtw r = 0.05
+                and r = 1.0, wdtw g = 0.05, euclidean (float32), dtw r = 0.1
+                and r = 1.0 in float64; rates and oracle checks on a few rows
   batched       50,000 EOS-padded pairs of uneven length (64..2048) with
-                per-pair r in [0.05, 0.2]: the persistent strip kernels in one
-                hipGraph replay against the plain per-pair kernel
+                per-pair r in [0.05, 0.2]: the persistent strip launch in a
+                hipGraph replay against the plain per-pair kernel, its own
+                roofline, and the float64 strips
   train         differentiable DTW, 4096 x 64 pairs of length 256, r = 0.1:
                 forward, forward + backward, and 200 barycenter steps
   subsequence   pairwise_subsequence_distance, 4096 shapelets (lengths
-                16..128) over random-walk series of length 1024, dtw r = 0.1
-                and euclidean, with exact indices
+                16..128) over 10000 random-walk series of length 1024, dtw
+                r = 0.1 and euclidean, with exact indices
 """
 
 import argparse
@@ -134,30 +139,52 @@ def run_batched(torch, dev, n_pairs, t_pad, reps):
         if plain is False:
             # the autotuner's per-bucket launch shapes, timed the same way
             with torch.cuda.stream(side):
-                tuned_ms = plan.autotune(Xd, Yd)
+                tuned_ms = plan.autotune(Xd, Yd, reps=5)
+            tuned_kernel_ms = tuned_ms
             tuned = [{"R": bk["R"], "pairs": bk["pairs"], "waves": bk["waves"],
                       "stream": bk["stream"], "ms": round(bk["ms"], 4)} for bk in plan.buckets()]
             replay()
             replay()
             tuned_dt = timed(torch, replay, reps)
             assert np.array_equal(out.cpu().numpy(), res[False][1])
-            log(f"batched: autotuned {tuned_dt * 1e3:.3f} ms (buckets alone {tuned_ms:.3f} ms "
-                f"in total) vs default launch {dt * 1e3:.3f} ms")
+            log(f"batched: autotuned {tuned_dt * 1e3:.3f} ms (strip launch alone {tuned_ms:.3f} ms) "
+                f"vs default launch {dt * 1e3:.3f} ms")
         plan.close()
+    # the autotuned launch shape is the plan's own choice: its replay time is
+    # the batched result
     dt, got, cells, n_strip, n_plainp = res[False]
+    default_dt = dt
+    dt = tuned_dt
     pdt = res[True][0]
     ppdt = res["padded"][0]
+    # the float64 strips (FP64 mode: the reference's arithmetic, bit-exact)
+    plan64 = DtwBatch(lx, ly, t_pad, r, precision="fp64")
+    X64d, Y64d = Xd.double(), Yd.double()
+    out64 = torch.empty(n_pairs, dtype=torch.float64, device=dev)
+
+    def replay64():
+        with torch.cuda.stream(side):
+            plan64.run(X64d, Y64d, out=out64, graph=True)
+
+    replay64()
+    replay64()
+    dt64 = timed(torch, replay64, 3)
+    got64 = out64.cpu().numpy()
+    plan64.close()
     assert np.array_equal(res["padded"][1], res[True][1])
     pick = np.random.default_rng(0).choice(n_pairs, 200, replace=False)
     want = np.array([oracle.metric_distance(X[p, :lx[p]].astype(np.float64),
                                             Y[p, :ly[p]].astype(np.float64), "dtw", r=r[p])
                      for p in pick])
     err = float(np.max(np.abs(got[pick] - want) / want))
+    mism64 = int((got64[pick] != want).sum())
+    kern_s = tuned_kernel_ms / 1e3
     out = {
         "workload": f"{n_pairs} pairs, lengths U[64,{t_pad}] padded to {t_pad} with EOS, "
                     "r U[0.05,0.2] per pair, random walks + 5% Laplace spikes, FP32",
         "cells": int(cells),
         "ms": dt * 1e3,
+        "default_launch_ms": default_dt * 1e3,
         "value": cells / dt,
         "unit": "cells/s",
         "frac_of_peak": cells / dt / PEAK_CELLS,
@@ -174,10 +201,30 @@ def run_batched(torch, dev, n_pairs, t_pad, reps):
         "autotuned_ms": tuned_dt * 1e3,
         "autotuned_buckets": tuned,
         "graph": True,
+        # the strip launch alone (HIP events around it on its stream, by the
+        # autotuner): the batched path's own roofline
+        "roofline": {
+            "bound": "valu",
+            "achieved": cells / kern_s * 6 / 1e12,
+            "peak": 157.3,
+            "unit": "TFLOP/s",
+            "frac": cells / kern_s * 6 / 157.3e12,
+            "traffic": None,
+            "kernel": "k_strip_batch<float>",
+            "kernel_ms": kern_s * 1e3,
+            "achieved_cells_per_s": cells / kern_s,
+            "peak_cells_per_s": PEAK_CELLS,
+        },
+        "fp64": {"ms": dt64 * 1e3, "value": cells / dt64, "unit": "cells/s",
+                 "mismatch_200_pairs": mism64,
+                 "note": "FP64 mode (the reference's float64 arithmetic) on the float32 data "
+                         "widened, float64 strips; bit-exact against the oracle"},
     }
-    log(f"batched: {dt * 1e3:.3f} ms, {cells / dt / 1e12:.2f} Tcells/s; plain {pdt * 1e3:.2f} ms "
+    log(f"batched: {dt * 1e3:.3f} ms, {cells / dt / 1e12:.2f} Tcells/s (strip launch "
+        f"{kern_s * 1e3:.3f} ms, {cells / kern_s / 1e12:.2f} Tcells/s); plain {pdt * 1e3:.2f} ms "
         f"(x{pdt / dt:.1f}); plain over the padded length {ppdt * 1e3:.2f} ms (x{ppdt / dt:.1f}); "
-        f"max rel err {err:.2e}")
+        f"max rel err {err:.2e}; fp64 {dt64 * 1e3:.3f} ms ({cells / dt64 / 1e12:.2f} Tcells/s, "
+        f"{mism64} mismatches)")
     return out
 
 
