@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "subseq_fast.h"
+#include "subseq_mfma.h"
 #include "wbx_internal.h"
 
 namespace wbx {
@@ -514,8 +515,15 @@ int subseq_fast_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void 
     // classes by band width B = 2 w - 1 (odd, <= 63): one FP32 scan launch
     // each; class 0 = bands too wide for the scan (exact rescan)
     std::vector<std::vector<int32_t>> cls(64);
+    // euclidean shapelets up to kMfmaMaxLen long: the matrix-core scan
+    const bool use_mfma = euclid && g_policy.load() != 1;
+    std::vector<int32_t> mf_ids;
     for (int64_t j = 0; j < n_s; ++j) {
         int c = 1;
+        if (use_mfma && hl[j] <= kMfmaMaxLen) {
+            mf_ids.push_back((int32_t)j);
+            continue;
+        }
         if (!euclid) {
             const int w = band_w((int)hl[j], (int)hl[j], m->r);
             c = 2 * w - 1 <= 63 ? 2 * w - 1 : 0;
@@ -542,6 +550,12 @@ int subseq_fast_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void 
     const size_t o_ovf = cv.take((size_t)n * (size_t)n_s * 4);
     const size_t o_list = cv.take((size_t)n_s * 4);
     const size_t o_cnt = cv.take(80 * 4);  // 64 class counters, then the |x| max
+    MfmaPlan mf;
+    size_t o_mf = 0;
+    if (!mf_ids.empty()) {
+        mfma_plan(mf_ids, hl, t, &mf);
+        o_mf = cv.take(mfma_ws_bytes(mf, n));
+    }
     void *wsp = nullptr;
     RC(ws_reserve(ctx, cv.total, &wsp));
     char *ws = (char *)wsp;
@@ -555,6 +569,14 @@ int subseq_fast_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void 
     RC(launch_fill_i32(cnt, 80, 0, st));
     double absmax = 0.0;
     const bool exact_inputs = dtype == WBX_F32;
+    if (!mf_ids.empty()) {
+        ProfScope pf(st, KERN_SUBSEQ);
+        RC(mfma_scan(mf, ws + o_mf, exact_inputs ? 1 : 0, x32.ptr, x32.stride, s32.ptr, x64.ptr,
+                     x64.stride, s64.ptr, s_off, s_len, n, n_s, dist, idx, st));
+    }
+    bool any_cls = false;
+    for (int c = 0; c < 64; ++c) any_cls = any_cls || !cls[c].empty();
+    if (!any_cls) return WBX_OK;
     if (!exact_inputs) {
         unsigned *mx = (unsigned *)(cnt + 64);
         RC(launch_absmax_rows<double>((const double *)X, n, t, xstr, mx, st));
