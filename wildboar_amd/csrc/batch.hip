@@ -119,6 +119,7 @@ struct BatchPlan {
     void *d_stage = nullptr;
     void *d_zero = nullptr;  // zero buffer read by the lanes past a pair's band
     int zero_pad = 0;
+    int64_t zero_len = 0;  // elements of each half of d_zero (zeros, then +inf)
     int32_t *d_lx = nullptr, *d_ly = nullptr;
     double *d_r = nullptr;
     void *d_sq = nullptr;        // strip pairs' squared costs (T)
@@ -292,8 +293,22 @@ static int plan_build(BatchPlan *b, const int32_t *lx, const int32_t *ly, const 
     // (strip_read_lo / strip_read_hi with off <= t_pad, R <= Rmax)
     b->zero_pad = (int)(b->t_pad + 128);
     const int64_t zlen = b->zero_pad + strip_read_hi((int)b->t_pad, 0, std::max(Rmax, 2)) + 64;
-    RC(dmalloc((char **)&b->d_zero, (size_t)zlen * esz));
-    WBX_HIP_CHECK(hipMemset(b->d_zero, 0, (size_t)zlen * esz));
+    // zeros (x of the lanes past a band), then +inf (their y)
+    b->zero_len = zlen;
+    RC(dmalloc((char **)&b->d_zero, (size_t)(2 * zlen) * esz));
+    {
+        std::vector<unsigned char> zi((size_t)(2 * zlen) * esz, 0);
+        for (int64_t q = zlen; q < 2 * zlen; ++q) {
+            if (b->fp64) {
+                const double v = WBX_INF_D;
+                memcpy(&zi[(size_t)q * esz], &v, esz);
+            } else {
+                const float v = WBX_INF_F;
+                memcpy(&zi[(size_t)q * esz], &v, esz);
+            }
+        }
+        WBX_HIP_CHECK(hipMemcpy(b->d_zero, zi.data(), zi.size(), hipMemcpyHostToDevice));
+    }
     if (!staged.empty()) {
         // a staged row has t_pad + 64 elements before it (-64 - off >= -64 - t_pad)
         // and reads reach stage_hi past its start
@@ -379,7 +394,8 @@ static int plan_strip_arrays(BatchPlan *b, const C *Xc, int64_t xs, const C *Yc,
                            b->stage_stride, b->stage_pad);
         WBX_HIP_CHECK(hipGetLastError());
     }
-    *ar = StripArrays<C>{xb, yb, (const C *)b->d_stage, (const C *)b->d_zero, (int)xst, (int)yst,
+    *ar = StripArrays<C>{xb,  yb, (const C *)b->d_stage, (const C *)b->d_zero,
+                         (const C *)b->d_zero + b->zero_len, (int)xst, (int)yst,
                          b->stage_stride, b->stage_pad, b->zero_pad};
     return WBX_OK;
 }

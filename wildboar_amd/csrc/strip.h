@@ -22,10 +22,11 @@ struct StripItem {
 
 // Arrays of one launch (element offsets are 32-bit): X and Y with their row
 // strides, the staged rows (x row of stage e at spad + 2 e sstride, y row
-// sstride later) and a zero buffer read by the lanes past a pair's band.
+// sstride later), and the buffers read by the lanes past a pair's band: zeros
+// for x and +inf for y (their cells are +inf without a select).
 template <typename T>
 struct StripArrays {
-    const T *xb, *yb, *sb, *zb;
+    const T *xb, *yb, *sb, *zb, *ib;
     int xstride, ystride, sstride, spad;
     int zpad;  // offset of the zero buffer's middle (it covers every read of a pair)
 };

@@ -104,15 +104,31 @@ __device__ __forceinline__ V4<double> ld4(const double *__restrict__ b, int o) {
 // neighbour, min with that lane's v (v_min_f32 with DPP wave_shr:1: lane 0
 // has no source and keeps min(a, b), i.e. its left input is +inf).  The
 // s_nop gives the two wait states a DPP read needs after v was written.
+// Both minima are written in the asm: as fminf the compiler would quiet each
+// operand first (two v_max per row); no operand is a NaN here.
 __device__ __forceinline__ float min_left(float a, float b, float v) {
-    float t = fminf(a, b);
-    asm volatile("s_nop 1\n\tv_min_f32_dpp %0, %1, %0 wave_shr:1 row_mask:0xf bank_mask:0xf"
-                 : "+v"(t)
-                 : "v"(v));
+    float t;
+    asm volatile(
+        "v_min_f32 %0, %1, %2\n\ts_nop 0\n\t"
+        "v_min_f32_dpp %0, %3, %0 wave_shr:1 row_mask:0xf bank_mask:0xf"
+        : "=&v"(t)
+        : "v"(a), "v"(b), "v"(v));
     return t;
 }
 __device__ __forceinline__ double min_left(double a, double b, double v) {
     return __builtin_fmin(__builtin_fmin(a, shr1(v)), b);
+}
+
+// The up input of a lane's last slot: lane p + 1's value (DPP wave_shl:1);
+// lane 63 has no source and keeps the old value, +inf from the start, so the
+// destination needs no +inf written before every shift.
+__device__ __forceinline__ float up_in(float old, float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x130,
+                                                      0xF, 0xF, false));
+}
+__device__ __forceinline__ double up_in(double old, double v) {
+    (void)old;
+    return shl1(v);
 }
 
 template <typename T, int R, int U>
@@ -123,6 +139,7 @@ struct Strip {
     T yw[W];  // yw[q] = y[yj + q]
     T xv[U];  // x of the group's U rows
     T l;      // last real slot of the lane's last row
+    T uin;    // last up input (+inf in lane 63)
     int r;    // the lane's row at the start of the group
     int yj;
     int lyl;  // ly for the pair's lanes, 0 for lanes past the band (+inf columns)
@@ -143,7 +160,8 @@ __device__ __forceinline__ void strip_rows(Strip<T, R, U> &L, int lx) {
         // differences once above their branch and keeps them all live
         const T d = STEADY ? x - L.yw[u] : L.yw[u] - x;
         const T c0 = fma_or_add(d, min_left(L.row[1], L.row[0], L.l));
-        const T Uin = shl1(act ? c0 : L.row[0]);
+        const T Uin = up_in(L.uin, act ? c0 : L.row[0]);
+        L.uin = Uin;
         T left = c0;
         const T r0 = act ? c0 : L.row[0];
 #pragma unroll
@@ -168,6 +186,43 @@ __device__ __forceinline__ void strip_rows(Strip<T, R, U> &L, int lx) {
     }
 }
 
+// One group: the next group's x and the window's U new entries are issued
+// before the group's U rows; then the window moves on by U.
+template <typename T, int R, int U, bool STEADY>
+__device__ __forceinline__ void strip_group(Strip<T, R, U> &L, const T *__restrict__ Xa,
+                                            const T *__restrict__ Ya, int xo, int yo, int lx,
+                                            int ly, int s0, int c0l, int cnl) {
+    constexpr int W = Strip<T, R, U>::W;
+    const T INF = sinf_<T>();
+    T xn[U], yn[U];
+    const int jn = L.yj + W;
+#pragma unroll
+    for (int u = 0; u < U; u += 4) {
+        const V4<T> a = ld4(Xa, xo + L.r + U + u);
+        const V4<T> b = ld4(Ya, yo + jn + u);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            xn[u + q] = a.v[q];
+            yn[u + q] = b.v[q];
+        }
+    }
+    strip_rows<T, R, U, STEADY>(L, lx);
+#pragma unroll
+    for (int q = 0; q < W - U; ++q) L.yw[q] = L.yw[q + U];
+    if (s0 + c0l >= 0 && s0 + cnl < ly) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) L.yw[W - U + u] = yn[u];
+    } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            L.yw[W - U + u] = (unsigned)(jn + u) < (unsigned)L.lyl ? yn[u] : INF;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) L.xv[u] = xn[u];
+    L.r += U;
+    L.yj += U;
+}
+
 // One pair on the wave.  R: slots of a short lane (long lanes R + 1).
 template <typename T, int R>
 __device__ __forceinline__ void strip_pair(const StripItem &it, const StripArrays<T> &ar,
@@ -184,9 +239,11 @@ __device__ __forceinline__ void strip_pair(const StripItem &it, const StripArray
     const T *__restrict__ Ya;
     int xo, yo;
     if (!live) {
-        // lanes past the band: no columns (+inf cells), reading the zero
-        // buffer, which covers every offset a pair reads
-        Xa = Ya = ar.zb;
+        // lanes past the band: no columns (+inf cells); x from the zero
+        // buffer and y from the +inf buffer, which cover every offset a pair
+        // reads (d = 0 - inf, d * d = +inf: no select keeps them at +inf)
+        Xa = ar.zb;
+        Ya = ar.ib;
         xo = yo = ar.zpad;
     } else if (it.stage < 0) {
         Xa = it.swap ? ar.yb : ar.xb;
@@ -220,39 +277,31 @@ __device__ __forceinline__ void strip_pair(const StripItem &it, const StripArray
         for (int q = 0; q < 4; ++q) L.xv[u + q] = v.v[q];
     }
     L.l = L.shortl ? L.row[R - 1] : L.row[R];
+    L.uin = INF;
     // steps [0, lx + n - 1): lane n - 1 finishes row lx - 1 at step lx + n - 2;
     // a group is steady when lane n - 1 has reached row 0 and lane 0 stays
     // below row lx
     const int groups = (lx + it.n - 1 + U - 1) / U;
-    for (int g = 0; g < groups; ++g) {
-        // the next group's x and the window's U new entries, issued before
-        // this group's rows
-        T xn[U], yn[U];
-        const int jn = L.yj + W;
-#pragma unroll
-        for (int u = 0; u < U; u += 4) {
-            const V4<T> a = ld4(Xa, xo + L.r + U + u);
-            const V4<T> b = ld4(Ya, yo + jn + u);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                xn[u + q] = a.v[q];
-                yn[u + q] = b.v[q];
-            }
-        }
-        const int s0 = g * U;
-        if (s0 >= it.n - 1 && s0 + U <= lx)
-            strip_rows<T, R, U, true>(L, lx);
-        else
-            strip_rows<T, R, U, false>(L, lx);
-#pragma unroll
-        for (int q = 0; q < W - U; ++q) L.yw[q] = L.yw[q + U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            L.yw[W - U + u] = (unsigned)(jn + u) < (unsigned)L.lyl ? yn[u] : INF;
-            L.xv[u] = xn[u];
-        }
-        L.r += U;
-        L.yj += U;
+    // new window entries of group g: columns [jn, jn + U) with jn = g U + c_p,
+    // c_p = W - p + k0 - off increasing in p; between lane 0's first and lane
+    // n - 1's last inside [0, ly) no entry needs the +inf select
+    const int kl = it.n - 1 < it.a ? (it.n - 1) * S : it.a * S + (it.n - 1 - it.a) * R;
+    const int c0l = W - off, cnl = W - (it.n - 1) + kl - off + U - 1;
+    // fill groups [0, gs) and drain groups [ge, groups) with selects, the
+    // steady groups [gs, ge) without: each in a loop of its own, so the
+    // compiler keeps the band row in the same registers across a loop's
+    // iterations instead of merging two variants at every group
+    const int gs = std::min((it.n - 1 + U - 1) / U, groups);
+    const int ge = std::max(gs, std::min(lx / U, groups));
+    int g = 0;
+#pragma nounroll
+    for (int phase = 0; phase < 2; ++phase) {
+        const int gend = phase == 0 ? gs : groups;
+        for (; g < gend; ++g)
+            strip_group<T, R, U, false>(L, Xa, Ya, xo, yo, lx, it.ly, g * U, c0l, cnl);
+        if (phase == 0)
+            for (; g < ge; ++g)
+                strip_group<T, R, U, true>(L, Xa, Ya, xo, yo, lx, it.ly, g * U, c0l, cnl);
     }
     const int nslots = L.shortl ? R : S;
     if (live && kf >= k0 && kf < k0 + nslots) {
