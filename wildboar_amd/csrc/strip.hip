@@ -136,7 +136,7 @@ struct Strip {
     static constexpr int S = R + 1;
     static constexpr int W = S + U - 1;
     T row[S];
-    T yw[W];  // yw[q] = y[yj + q]
+    T yw[W + U];  // yw[q] = y[yj + q]; the last U are the next group's in a group pair
     T xv[U];  // x of the group's U rows
     T l;      // last real slot of the lane's last row
     T uin;    // last up input (+inf in lane 63)
@@ -148,17 +148,19 @@ struct Strip {
 
 // One group of U rows.  STEADY: every lane of the pair is inside its rows for
 // all U rows (lanes past the band compute +inf), so no cell needs a select.
-template <typename T, int R, int U, bool STEADY>
-__device__ __forceinline__ void strip_rows(Strip<T, R, U> &L, int lx) {
+// The window starts at yw[YO] (YO = U for the second group of a pair) and the
+// group's x values are xs.
+template <typename T, int R, int U, bool STEADY, int YO = 0>
+__device__ __forceinline__ void strip_rows(Strip<T, R, U> &L, int lx, const T (&xs)[U]) {
     constexpr int S = R + 1;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const bool act = STEADY || (unsigned)(L.r + u) < (unsigned)lx;
-        const T x = L.xv[u];
+        const bool act = STEADY || (unsigned)(L.r + YO + u) < (unsigned)lx;
+        const T x = xs[u];
         // the select variant writes the difference as y - x ((y - x)^2 ==
         // (x - y)^2 exactly): otherwise the compiler computes both variants'
         // differences once above their branch and keeps them all live
-        const T d = STEADY ? x - L.yw[u] : L.yw[u] - x;
+        const T d = STEADY ? x - L.yw[YO + u] : L.yw[YO + u] - x;
         const T c0 = fma_or_add(d, min_left(L.row[1], L.row[0], L.l));
         const T Uin = up_in(L.uin, act ? c0 : L.row[0]);
         L.uin = Uin;
@@ -173,8 +175,8 @@ __device__ __forceinline__ void strip_rows(Strip<T, R, U> &L, int lx) {
                 up = L.shortl ? Uin : L.row[m + 1];
             else
                 up = L.row[m + 1];
-            const T c = STEADY ? scell(up, left, L.row[m], x, L.yw[u + m])
-                               : scell(up, left, L.row[m], L.yw[u + m], x);
+            const T c = STEADY ? scell(up, left, L.row[m], x, L.yw[YO + u + m])
+                               : scell(up, left, L.row[m], L.yw[YO + u + m], x);
             L.row[m] = act ? c : L.row[m];  // the old slot m was needed only by cells m, m + 1
             left = c;
         }
@@ -206,7 +208,7 @@ __device__ __forceinline__ void strip_group(Strip<T, R, U> &L, const T *__restri
             yn[u + q] = b.v[q];
         }
     }
-    strip_rows<T, R, U, STEADY>(L, lx);
+    strip_rows<T, R, U, STEADY>(L, lx, L.xv);
 #pragma unroll
     for (int q = 0; q < W - U; ++q) L.yw[q] = L.yw[q + U];
     if (s0 + c0l >= 0 && s0 + cnl < ly) {
@@ -221,6 +223,59 @@ __device__ __forceinline__ void strip_group(Strip<T, R, U> &L, const T *__restri
     for (int u = 0; u < U; ++u) L.xv[u] = xn[u];
     L.r += U;
     L.yj += U;
+}
+
+// Two steady groups, with the second group's x and new window entries
+// loaded straight into registers the first group does not read (xb and
+// yw[W, W + U)): no copies of them, and one window shift per two groups.
+template <typename T, int R, int U>
+__device__ __forceinline__ void strip_group_pair(Strip<T, R, U> &L, const T *__restrict__ Xa,
+                                                 const T *__restrict__ Ya, int xo, int yo,
+                                                 int lx, int ly, int s0, int c0l, int cnl) {
+    constexpr int W = Strip<T, R, U>::W;
+    const T INF = sinf_<T>();
+    T xb[U], yn[U];
+    int jn = L.yj + W;
+#pragma unroll
+    for (int u = 0; u < U; u += 4) {
+        const V4<T> a = ld4(Xa, xo + L.r + U + u);
+        const V4<T> b = ld4(Ya, yo + jn + u);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            xb[u + q] = a.v[q];
+            L.yw[W + u + q] = b.v[q];
+        }
+    }
+    strip_rows<T, R, U, true, 0>(L, lx, L.xv);
+    if (!(s0 + c0l >= 0 && s0 + cnl < ly)) {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            L.yw[W + u] = (unsigned)(jn + u) < (unsigned)L.lyl ? L.yw[W + u] : INF;
+    }
+    jn += U;
+#pragma unroll
+    for (int u = 0; u < U; u += 4) {
+        const V4<T> a = ld4(Xa, xo + L.r + 2 * U + u);
+        const V4<T> b = ld4(Ya, yo + jn + u);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            L.xv[u + q] = a.v[q];
+            yn[u + q] = b.v[q];
+        }
+    }
+    strip_rows<T, R, U, true, U>(L, lx, xb);
+#pragma unroll
+    for (int q = 0; q < W - U; ++q) L.yw[q] = L.yw[q + 2 * U];
+    if (s0 + U + c0l >= 0 && s0 + U + cnl < ly) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) L.yw[W - U + u] = yn[u];
+    } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            L.yw[W - U + u] = (unsigned)(jn + u) < (unsigned)L.lyl ? yn[u] : INF;
+    }
+    L.r += 2 * U;
+    L.yj += 2 * U;
 }
 
 // One pair on the wave.  R: slots of a short lane (long lanes R + 1).
@@ -278,31 +333,39 @@ __device__ __forceinline__ void strip_pair(const StripItem &it, const StripArray
     }
     L.l = L.shortl ? L.row[R - 1] : L.row[R];
     L.uin = INF;
-    // steps [0, lx + n - 1): lane n - 1 finishes row lx - 1 at step lx + n - 2;
-    // a group is steady when lane n - 1 has reached row 0 and lane 0 stays
-    // below row lx
-    const int groups = (lx + it.n - 1 + U - 1) / U;
+    // The final cell (lx - 1, ly - 1) is slot kf of lane pk, which finishes
+    // row lx - 1 at step lx - 1 + pk: the pair runs steps [0, lx + pk), in G
+    // groups.  Lanes p < pk then run past row lx - 1 in the last steps; what
+    // they compute there reaches only lanes that are past their last row too
+    // (lane p + 1 takes lane p's row of one step earlier, lane p - 1 the row
+    // lane p works on), so it never reaches lane pk's rows.
+    const int pk = kf < it.a * S ? kf / S : it.a + (kf - it.a * S) / R;
+    const int G = (lx + pk + U - 1) / U;
     // new window entries of group g: columns [jn, jn + U) with jn = g U + c_p,
     // c_p = W - p + k0 - off increasing in p; between lane 0's first and lane
     // n - 1's last inside [0, ly) no entry needs the +inf select
     const int kl = it.n - 1 < it.a ? (it.n - 1) * S : it.a * S + (it.n - 1 - it.a) * R;
     const int c0l = W - off, cnl = W - (it.n - 1) + kl - off + U - 1;
-    // fill groups [0, gs) and drain groups [ge, groups) with selects, the
-    // steady groups [gs, ge) without: each in a loop of its own, so the
+    // Fill groups [0, gs) keep a lane's state until it reaches row 0, with a
+    // select per cell.  From gs on every lane has started: groups [gs, ge)
+    // are inside every lane's rows, and [ge, G - 1) run the same select-free
+    // rows (a lane past its last row computes values nothing reads); the last
+    // group keeps each lane's state after its last row with selects, which
+    // leaves lane pk at row lx - 1.  Each range is a loop of its own, so the
     // compiler keeps the band row in the same registers across a loop's
-    // iterations instead of merging two variants at every group
-    const int gs = std::min((it.n - 1 + U - 1) / U, groups);
-    const int ge = std::max(gs, std::min(lx / U, groups));
+    // iterations instead of merging two variants at every group.
+    const int gs = std::min((it.n - 1 + U - 1) / U, G);
+    const int ge = std::max(gs, std::min(lx / U, G));
     int g = 0;
-#pragma nounroll
-    for (int phase = 0; phase < 2; ++phase) {
-        const int gend = phase == 0 ? gs : groups;
-        for (; g < gend; ++g)
-            strip_group<T, R, U, false>(L, Xa, Ya, xo, yo, lx, it.ly, g * U, c0l, cnl);
-        if (phase == 0)
-            for (; g < ge; ++g)
-                strip_group<T, R, U, true>(L, Xa, Ya, xo, yo, lx, it.ly, g * U, c0l, cnl);
-    }
+    for (; g < gs; ++g) strip_group<T, R, U, false>(L, Xa, Ya, xo, yo, lx, it.ly, g * U, c0l, cnl);
+    // group pairs for the strips whose extra U registers keep the kernel
+    // within its register budget (R >= 24 is at the limit)
+    if constexpr (R <= 22)
+        for (; g + 1 < ge; g += 2)
+            strip_group_pair<T, R, U>(L, Xa, Ya, xo, yo, lx, it.ly, g * U, c0l, cnl);
+    for (; g < std::max(ge, G - 1); ++g)
+        strip_group<T, R, U, true>(L, Xa, Ya, xo, yo, lx, it.ly, g * U, c0l, cnl);
+    if (g < G) strip_group<T, R, U, false>(L, Xa, Ya, xo, yo, lx, it.ly, g * U, c0l, cnl);
     const int nslots = L.shortl ? R : S;
     if (live && kf >= k0 && kf < k0 + nslots) {
         T res = INF;
