@@ -277,6 +277,11 @@ def run_pairwise_legs(torch, dev, X64, Y64, reps):
         want = oracle.pairwise(ox[:rows], oy, metric=metric, r=r, g=g)
         leg = {"ms": dt * 1e3, "kernel_ms": kern_ms / max(launches, 1), "work": work,
                "value": work / dt, "unit": unit, "kernel": kinds.get(kind, kind)}
+        # the leg against the VALU peak of its own cell arithmetic
+        peak, basis = LEG_PEAKS[(metric, prec)]
+        leg["frac_of_peak"] = work / dt / peak
+        leg["peak"] = peak
+        leg["peak_basis"] = basis
         if prec == "fp32":
             leg["max_rel_err_rows"] = float(np.max(np.abs(got - want) / np.abs(want)))
         else:
@@ -288,10 +293,22 @@ def run_pairwise_legs(torch, dev, X64, Y64, reps):
     return res
 
 
+# VALU peak of each pairwise leg's cell arithmetic: 78.6e12 FP32 and 39.3e12
+# FP64 lane-instructions per second (the published 157.3 / 78.6 TFLOP/s
+# vector peaks, an FMA counted as 2 flops), divided by the instructions one
+# algorithmic cell needs.
+LEG_PEAKS = {
+    ("dtw", "fp32"): (78.6e12 / 3, "3 FP32 VALU per cell (sub, min3, fma)"),
+    ("wdtw", "fp32"): (78.6e12 / 4, "4 FP32 VALU per cell (sub, mul, min3, fma)"),
+    ("euclidean", "fp32"): (78.6e12 / 2, "2 FP32 VALU per multiply-add (sub, fma)"),
+    ("dtw", "fp64"): (39.3e12 / 5, "5 FP64 VALU per cell (sub, mul, min, min, add)"),
+}
+
+
 # ----------------------------------------------------------------------- train
 def run_train(torch, dev, reps, dba_steps):
     from wildboar_amd import device
-    from wildboar_amd.dtw import dtw, dtw_barycenter
+    from wildboar_amd.dtw import dtw, dtw_barycenter, dtw_state_bytes
 
     X = torch.from_numpy(random_walks(40, 4096, 256).astype(np.float32)).to(dev)
     P = torch.from_numpy(random_walks(41, 64, 256).astype(np.float32)).to(dev)
@@ -309,12 +326,23 @@ def run_train(torch, dev, reps, dba_steps):
     def fwd_bwd():
         dtw(Xg, Pg, 0.1).sum().backward()
 
+    def fwd_ck():
+        return dtw(Xg, Pg, 0.1, checkpoint=True)
+
+    def fwd_bwd_ck():
+        dtw(Xg, Pg, 0.1, checkpoint=True).sum().backward()
+
     for _ in range(3):
         inf()
         fwd_bwd()
+        fwd_bwd_ck()
     t_inf = timed(torch, inf, reps)
     t_fwd = timed(torch, fwd, reps)
     t_fb = timed(torch, fwd_bwd, reps)
+    t_fwd_ck = timed(torch, fwd_ck, reps)
+    t_fb_ck = timed(torch, fwd_bwd_ck, reps)
+    state_planes, _ = dtw_state_bytes(4096, 64, 256, 256, 0.1, checkpoint=False)
+    state_ck, _ = dtw_state_bytes(4096, 64, 256, 256, 0.1, checkpoint=True)
     D = torch.from_numpy(random_walks(42, 512, 256).astype(np.float32)).to(dev)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -329,11 +357,22 @@ def run_train(torch, dev, reps, dba_steps):
         "fwd_bwd_ms": t_fb * 1e3,
         "fwd_bwd_over_fwd": t_fb / t_fwd,
         "fwd_bwd_over_inference_fwd": t_fb / t_inf,
+        "state_bytes": state_planes,
+        "checkpointed": {
+            "note": "dtw(..., checkpoint=True): cost rows kept every 32 anti-diagonal steps, "
+                    "each band recomputed with its decisions in the backward",
+            "fwd_ms": t_fwd_ck * 1e3,
+            "fwd_bwd_ms": t_fb_ck * 1e3,
+            "fwd_bwd_over_fwd": t_fb_ck / t_fwd_ck,
+            "fwd_bwd_over_inference_fwd": t_fb_ck / t_inf,
+            "state_bytes": state_ck,
+        },
         "dba": {"series": "512x256 (seed 42)", "steps": dba_steps, "seconds": t_dba,
                 "loss_first": float(hist[0]), "loss_last": float(hist[-1])},
     }
     log(f"train: inference {t_inf * 1e3:.3f} ms, fwd {t_fwd * 1e3:.3f} ms, fwd+bwd "
-        f"{t_fb * 1e3:.3f} ms (x{t_fb / t_fwd:.2f}); DBA {dba_steps} steps {t_dba:.2f} s")
+        f"{t_fb * 1e3:.3f} ms (x{t_fb / t_fwd:.2f}); checkpointed fwd {t_fwd_ck * 1e3:.3f} ms, "
+        f"fwd+bwd {t_fb_ck * 1e3:.3f} ms; DBA {dba_steps} steps {t_dba:.2f} s")
     return res
 
 

@@ -428,8 +428,14 @@ int wbx_batch_run(void *plan, int dtype, const void *X, int64_t x_stride,
  * path of dtw_mapping's backtrack (distance/dtw.py:362-427, ties diag > up >
  * left), for DTW barycenter averaging (the role of dtw_average,
  * distance/dtw.py:429-690) and the counterfactual explainers.  FP32, metric
- * dtw, all series of X of length tx and of Y of length ty.
- * mode 0: all pairs, D is (nx, ny); mode 1: D[p] = DTW(X[p], Y[p]), nx == ny.
+ * dtw or wdtw (WeightedDtwMetric, distance/_elastic.pyx:3322-3341: weights
+ * sized max(tx, ty), each path cell's term weighted as the forward weighs
+ * it), all series of X of length tx and of Y of length ty.
+ * mode bit 0 clear: all pairs, D is (nx, ny); set: D[p] = DTW(X[p], Y[p]),
+ * nx == ny.  Mode bit 1 set: the forward keeps only the cost rows at
+ * checkpoints 32 anti-diagonal steps apart and the backward recomputes each
+ * band of steps from its checkpoint; WDTW and bands wider than 112 slots
+ * always take that form, DTW bands of one slot never do.
  * The forward writes `state` (state_bytes) that the backward reads; the
  * backward needs `scratch` (scratch_bytes).  G is dL/dD in D's layout; dX
  * (nx, tx) and dY (ny, ty) are float32 with row strides ldx, ldy. */

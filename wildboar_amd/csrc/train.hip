@@ -500,6 +500,36 @@ struct TrainGeo {
     int BP, NW;
 };
 
+#define RCT(x)               \
+    do {                     \
+        int _rc = (x);       \
+        if (_rc) return _rc; \
+    } while (0)
+
+// Which form serves a call: the decision planes of the lane kernels (DTW,
+// bands up to TRAIN_MAX_BP slots) or the band checkpoints of train_seg.hip
+// (WDTW, wider bands, and every call with mode bit 1 set).
+static int train_route(const wbx_metric *m, int mode, int64_t tx, int64_t ty, bool *seg) {
+    if (mode < 0 || mode > 3) return fail(WBX_EINVAL, "mode %d (0..3)", mode);
+    if (m->kind != WBX_DTW && m->kind != WBX_WDTW)
+        return fail(WBX_EUNSUPPORTED, "differentiable DTW supports metrics dtw and wdtw");
+    if (m->precision != WBX_FP32)
+        return fail(WBX_EUNSUPPORTED, "differentiable DTW runs in FP32");
+    if (tx < 1 || ty < 1 || tx > (1 << 24) || ty > (1 << 24))
+        return fail(WBX_EINVAL, "series lengths (%lld, %lld)", (long long)tx, (long long)ty);
+    const Band bd = make_band((int)tx, (int)ty, m->r);
+    // the checkpointed form splits the band over lanes of at least two slots;
+    // a one-slot DTW band (the diagonal) takes the planes
+    if (bd.B < 2) {
+        if (m->kind == WBX_WDTW)
+            return fail(WBX_EUNSUPPORTED, "differentiable WDTW needs a band of 2 slots or more");
+        *seg = false;
+        return WBX_OK;
+    }
+    *seg = (mode & 2) != 0 || m->kind == WBX_WDTW || bd.B > TRAIN_MAX_BP;
+    return WBX_OK;
+}
+
 static int train_geo(const wbx_metric *m, int64_t tx, int64_t ty, TrainGeo *out) {
     if (m->kind != WBX_DTW)
         return fail(WBX_EUNSUPPORTED, "differentiable DTW supports metric dtw only");
@@ -566,6 +596,10 @@ extern "C" {
 int wbx_dtw_train_sizes(const wbx_metric *metric, int mode, int64_t nx, int64_t ny, int64_t tx,
                         int64_t ty, int64_t *state_bytes, int64_t *scratch_bytes) {
     if (!metric) return fail(WBX_EINVAL, "metric is NULL");
+    bool seg;
+    RCT(train_route(metric, mode, tx, ty, &seg));
+    if (seg)
+        return seg_train_sizes(metric, mode & 1, nx, ny, tx, ty, state_bytes, scratch_bytes);
     TrainGeo tg;
     int rc = train_geo(metric, tx, ty, &tg);
     if (rc) return rc;
@@ -584,7 +618,12 @@ int wbx_dev_dtw_train_forward(const wbx_metric *metric, int mode, const float *X
                               int64_t y_stride, int64_t ty, double *D, void *state, void *stream) {
     if (!metric || !X || !Y || !D || !state) return fail(WBX_EINVAL, "NULL argument");
     if (nx < 1 || ny < 1 || tx < 1 || ty < 1) return fail(WBX_EINVAL, "empty input");
-    if (mode == 1 && nx != ny) return fail(WBX_EINVAL, "paired mode needs nx == ny");
+    if ((mode & 1) && nx != ny) return fail(WBX_EINVAL, "paired mode needs nx == ny");
+    bool seg;
+    RCT(train_route(metric, mode, tx, ty, &seg));
+    if (seg)
+        return seg_train_forward(metric, mode & 1, X, nx, x_stride, tx, Y, ny, y_stride, ty, D,
+                                 state, (hipStream_t)stream);
     TrainGeo tg;
     int rc = train_geo(metric, tx, ty, &tg);
     if (rc) return rc;
@@ -615,7 +654,13 @@ int wbx_dev_dtw_train_backward(const wbx_metric *metric, int mode, const float *
                                void *stream) {
     if (!metric || !X || !Y || !G || !state || !dX || !dY || !scratch)
         return fail(WBX_EINVAL, "NULL argument");
-    if (mode == 1 && nx != ny) return fail(WBX_EINVAL, "paired mode needs nx == ny");
+    if (nx < 1 || ny < 1 || tx < 1 || ty < 1) return fail(WBX_EINVAL, "empty input");
+    if ((mode & 1) && nx != ny) return fail(WBX_EINVAL, "paired mode needs nx == ny");
+    bool seg;
+    RCT(train_route(metric, mode, tx, ty, &seg));
+    if (seg)
+        return seg_train_backward(metric, mode & 1, X, nx, x_stride, tx, Y, ny, y_stride, ty, G,
+                                  state, dX, ldx, dY, ldy_, scratch, (hipStream_t)stream);
     TrainGeo tg;
     int rc = train_geo(metric, tx, ty, &tg);
     if (rc) return rc;

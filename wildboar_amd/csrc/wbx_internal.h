@@ -211,6 +211,18 @@ struct GroupGeo {
 };
 bool group_plan(int B, int lx, bool fp64, bool weighted, GroupGeo *g);
 
+// Differentiable DTW / WDTW with band checkpoints (train_seg.hip); mode 0:
+// all pairs, 1: paired.
+int seg_train_sizes(const wbx_metric *m, int mode, int64_t nx, int64_t ny, int64_t tx, int64_t ty,
+                    int64_t *state_bytes, int64_t *scratch_bytes);
+int seg_train_forward(const wbx_metric *m, int mode, const float *X, int64_t nx, int64_t x_stride,
+                      int64_t tx, const float *Y, int64_t ny, int64_t y_stride, int64_t ty,
+                      double *D, void *state, hipStream_t st);
+int seg_train_backward(const wbx_metric *m, int mode, const float *X, int64_t nx,
+                       int64_t x_stride, int64_t tx, const float *Y, int64_t ny, int64_t y_stride,
+                       int64_t ty, const double *G, const void *state, float *dX, int64_t ldx,
+                       float *dY, int64_t ldy, void *scratch, hipStream_t st);
+
 // Full-matrix kernel (full.hip): DTW whose band covers the whole matrix
 // (w >= min(lx, ly), the default r = 1), one geometry per launch, lanes own
 // columns; float32 (FMA) or float64 (bit-exact).

@@ -9,8 +9,13 @@ resolved diag > up > left):
     dD/dx_i = sum over path cells (i, j) of (x_i - y_j) / D,
     dD/dy_j = -sum over path cells (i, j) of (x_i - y_j) / D,  (0 at D = 0).
 
-The forward records the backtrack decision of every band cell (two bits per
-cell) and the backward walks them; nothing is recomputed on the CPU.
+For DTW bands up to 112 slots the forward records the backtrack decision of
+every band cell (two bits per cell) and the backward walks them.  WDTW
+(``metric="wdtw"``, the cell's weight multiplies its term), wider bands, and
+calls with ``checkpoint=True`` keep only the cost rows at checkpoints 32
+anti-diagonal steps apart; the backward recomputes each band of 32 steps from
+its checkpoint, with its decisions, and walks the path through it.  Nothing
+is recomputed on the CPU.
 
 ``dtw_barycenter`` averages series by gradient descent on the mean DTW
 distance (Adam), the gradient-based counterpart of ``dtw_average``
@@ -25,11 +30,14 @@ import ctypes
 import torch
 
 from . import _lib
-from .distance import _check_r
+from .distance import _check_g, _check_r
 
 
-def _metric(r):
-    return _lib.metric_struct(_lib.DTW, _lib.FP32, _check_r(r), 0.05)
+def _metric(r, metric="dtw", g=0.05):
+    if metric not in ("dtw", "wdtw"):
+        raise ValueError(f"differentiable DTW supports metric 'dtw' or 'wdtw', got {metric!r}")
+    return _lib.metric_struct(_lib.KINDS[metric], _lib.FP32, _check_r(r),
+                              _check_g(g) if metric == "wdtw" else 0.05)
 
 
 def _prep(t, name):
@@ -50,17 +58,17 @@ def _sizes(m, mode, nx, ny, tx, ty):
 
 class _DtwFunction(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, X, Y, r, mode):
+    def forward(ctx, X, Y, r, mode, metric, g):
         X = _prep(X, "X")
         Y = _prep(Y, "Y")
         nx, tx = X.shape
         ny, ty = Y.shape
-        if mode == 1 and nx != ny:
+        if (mode & 1) and nx != ny:
             raise ValueError("paired DTW needs the same number of x and y series")
-        m = _metric(r)
+        m = _metric(r, metric, g)
         state_bytes, _ = _sizes(m, mode, nx, ny, tx, ty)
         state = torch.empty(state_bytes, dtype=torch.uint8, device=X.device)
-        D = torch.empty((nx, ny) if mode == 0 else (nx,), dtype=torch.float64, device=X.device)
+        D = torch.empty((nx,) if mode & 1 else (nx, ny), dtype=torch.float64, device=X.device)
         stream = torch.cuda.current_stream().cuda_stream
         _lib.check(_lib.load().wbx_dev_dtw_train_forward(
             m, mode, X.data_ptr(), nx, X.stride(0), tx, Y.data_ptr(), ny, Y.stride(0), ty,
@@ -69,6 +77,8 @@ class _DtwFunction(torch.autograd.Function):
         ctx.state = state
         ctx.r = r
         ctx.mode = mode
+        ctx.metric = metric
+        ctx.g = g
         return D
 
     @staticmethod
@@ -76,7 +86,7 @@ class _DtwFunction(torch.autograd.Function):
         X, Y = ctx.saved_tensors
         nx, tx = X.shape
         ny, ty = Y.shape
-        m = _metric(ctx.r)
+        m = _metric(ctx.r, ctx.metric, ctx.g)
         _, scratch_bytes = _sizes(m, ctx.mode, nx, ny, tx, ty)
         scratch = torch.empty(scratch_bytes, dtype=torch.uint8, device=X.device)
         G = G.to(torch.float64).contiguous()
@@ -88,17 +98,32 @@ class _DtwFunction(torch.autograd.Function):
             G.data_ptr(), ctx.state.data_ptr(), dX.data_ptr(), dX.stride(0), dY.data_ptr(),
             dY.stride(0), scratch.data_ptr(), stream))
         ctx.state = None
-        return dX, dY, None, None
+        return dX, dY, None, None, None, None
 
 
-def dtw(X, Y, r=1.0):
-    """(nx, ny) float64 DTW distances of every X row against every Y row."""
-    return _DtwFunction.apply(X, Y, float(r), 0)
+def dtw_state_bytes(nx, ny, tx, ty, r=1.0, *, metric="dtw", g=0.05, checkpoint=False,
+                    paired=False):
+    """(state, scratch) bytes of one differentiable call: what the forward
+    keeps for the backward, and the backward's working memory."""
+    mode = (1 if paired else 0) | (2 if checkpoint else 0)
+    return _sizes(_metric(r, metric, g), mode, nx, ny, tx, ty)
 
 
-def paired_dtw(X, Y, r=1.0):
-    """(n,) float64 DTW distances of X[i] against Y[i]."""
-    return _DtwFunction.apply(X, Y, float(r), 1)
+def dtw(X, Y, r=1.0, *, metric="dtw", g=0.05, checkpoint=False):
+    """(nx, ny) float64 DTW (or WDTW, ``metric="wdtw"`` with weight ``g``)
+    distances of every X row against every Y row.
+
+    ``checkpoint=True`` keeps the cost rows at anti-diagonal band checkpoints
+    and recomputes each band in the backward (memory ~ B (t + n) / 32 floats
+    per pair) instead of recording two decision bits per band cell; WDTW and
+    bands wider than 112 slots always use it.
+    """
+    return _DtwFunction.apply(X, Y, float(r), 2 if checkpoint else 0, metric, float(g))
+
+
+def paired_dtw(X, Y, r=1.0, *, metric="dtw", g=0.05, checkpoint=False):
+    """(n,) float64 DTW (or WDTW) distances of X[i] against Y[i]; see ``dtw``."""
+    return _DtwFunction.apply(X, Y, float(r), 3 if checkpoint else 1, metric, float(g))
 
 
 def dtw_barycenter(X, r=1.0, *, steps=200, lr=1e-2, init=None, return_history=False):
