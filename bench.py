@@ -706,6 +706,30 @@ def main():
         acc["label_rows_checked"] = n
         acc["oracle_argmin_seconds"] = tl
 
+        # argmin with the library's LB_Keogh: only the pairs the reference's
+        # loop evaluates (plus seeds) are computed, float64, exact
+        from wildboar_amd.lb import DtwKeoghLowerBound
+
+        Xe = torch.from_numpy(Xs).to(dev)
+        Ye = torch.from_numpy(Ys).to(dev)
+        lbk = torch.from_numpy(DtwKeoghLowerBound(r).fit(Ys).transform(Xs)).to(dev)
+        device.argmin(Xe, Ye, 1, metric="dtw", r=r, precision="fp64", lower_bound=lbk)
+        _lib.prof_pruned()
+        t_lb = timed(torch, lambda: device.argmin(Xe, Ye, 1, metric="dtw", r=r, precision="fp64",
+                                                  lower_bound=lbk), 3)
+        evaluated = _lib.prof_pruned() / 3
+        idx_lb, _ = device.argmin(Xe, Ye, 1, metric="dtw", r=r, precision="fp64", lower_bound=lbk)
+        t_full = timed(torch, lambda: device.argmin(Xe, Ye, 1, metric="dtw", r=r, precision="fp64"),
+                       3)
+        acc["argmin_lb_keogh_fp64"] = {
+            "ms": t_lb * 1e3, "no_bound_ms": t_full * 1e3, "speedup": t_full / t_lb,
+            "pairs_evaluated_frac": evaluated / (n * n),
+            "index_mismatch": int((idx_lb[:, 0].cpu().numpy() != oi[:, 0]).sum()),
+            "note": "argmin_distance(k=1, lower_bound=DtwKeoghLowerBound(r).fit(Y).transform(X)) "
+                    "in float64 against the same call without a bound"}
+        log(f"argmin with LB_Keogh: {t_lb * 1e3:.2f} ms ({evaluated / (n * n):.3f} of the pairs) "
+            f"vs {t_full * 1e3:.2f} ms without a bound")
+
         # Plain one-pair-per-block kernel on a slice of the same workload.
         prow = 64
         _lib.set_policy(1)

@@ -98,6 +98,12 @@ int wbx_set_acc64_length(int64_t len);
  * 7 full-matrix DTW), and clears the record. */
 int wbx_prof_enable(int on);
 int wbx_prof_take(double *total_ms, int64_t *launches, int *last_kind);
+/* No counterpart in the reference (measurement).  The number of pairs whose
+ * distance argmin with a lower bound has computed on the current device since
+ * the last call (the reference evaluates only pairs whose bound is below its
+ * running threshold; the library computes those and a few seeds per row,
+ * _cdistance.pyx:1329-1340), then clears it. */
+int wbx_prof_pruned(int64_t *evaluated);
 /* No counterpart in the reference (measurement).  Runs a VALU probe made of
  * the DTW cell's instructions (sub, min3, fma) on every SIMD of the current
  * device and returns the achieved lane-instructions per second. */
@@ -123,8 +129,10 @@ int wbx_paired(const wbx_metric *metric, const double *X, const double *Y,
 
 /* Replaces _argmin_distance (distance/_cdistance.pyx:1348-1378): the k
  * nearest Y rows of each X row, in the reference's heap order (Heap,
- * utils/_misc.pyx:18-107).  lower_bound (nx, ny) may be NULL.  idx is
- * (nx, k) int64, dist is (nx, k). */
+ * utils/_misc.pyx:18-107).  lower_bound (nx, ny) may be NULL; with one and
+ * k <= 16 only the distances the reference evaluates (bound below its running
+ * threshold, :1329-1340) and a few seeds per row are computed, in float64.
+ * idx is (nx, k) int64, dist is (nx, k). */
 int wbx_argmin(const wbx_metric *metric, const double *X, int64_t nx,
                int64_t dims, int64_t tx, const double *Y, int64_t ny,
                int64_t ty, int64_t dim, int64_t k, const double *lower_bound,

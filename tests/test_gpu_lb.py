@@ -72,3 +72,53 @@ def test_argmin_with_keogh_lower_bound():
         pi, pd = wd.argmin_distance(X, Y, k=k, metric="dtw", metric_params={"r": 0.1},
                                     return_distance=True, sorted=True)
         np.testing.assert_array_equal(np.sort(gd, axis=1), pd)
+
+
+@pytest.mark.parametrize("k,tight", [(1, 0.95), (1, 0.5), (3, 0.9), (16, 0.8), (17, 0.9)])
+def test_argmin_pruned_by_valid_bound_is_exact(k, tight):
+    # With a lower bound, argmin computes only the distances the reference
+    # evaluates (its threshold before pair j is the k-th smallest distance of
+    # the pairs before j) plus a few seeds per row; the result must equal the
+    # oracle's sequential loop bit for bit, in heap order.  The bound here is
+    # the true distance times U[tight, 1] (valid, of varying tightness), and
+    # 1e300 on a few pairs (never evaluated: the checks refuse +inf bounds as
+    # the reference's do); k = 17 takes the unpruned path.
+    from wildboar_amd import _lib
+
+    X = rw(110, 30, 64)
+    Y = rw(111, 400, 64)
+    D = oracle.pairwise(X, Y, metric="dtw", r=0.1)
+    rng = np.random.default_rng(112)
+    lb = D * rng.uniform(tight, 1.0, D.shape)
+    lb[rng.random(D.shape) < 0.02] = 1e300
+    _lib.prof_pruned()
+    gi, gd = wd.argmin_distance(X, Y, k=k, metric="dtw", metric_params={"r": 0.1},
+                                lower_bound=lb, return_distance=True)
+    evaluated = _lib.prof_pruned()
+    oi, od = oracle.argmin(X, Y, k=k, metric="dtw", r=0.1, lower_bound=lb)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gd, od)
+    if k <= 16 and tight >= 0.9:
+        assert evaluated < 0.5 * X.shape[0] * Y.shape[0], evaluated
+    # FP32 requests with a bound are served by the same exact path
+    gi32, gd32 = wd.argmin_distance(X, Y, k=k, metric="dtw", metric_params={"r": 0.1},
+                                    lower_bound=lb, return_distance=True, precision="fp32")
+    np.testing.assert_array_equal(gi32, oi)
+    np.testing.assert_array_equal(gd32, od)
+
+
+def test_argmin_bound_violated_on_seeds_falls_back_exactly():
+    # a bound above the distance on the seeds (the smallest bounds) and
+    # elsewhere: the rows fall back to every pair and the sequential rule's
+    # (documented as incorrect) outcome is still reproduced exactly
+    X = rw(113, 12, 48)
+    Y = rw(114, 150, 48)
+    D = oracle.pairwise(X, Y, metric="wdtw", r=0.2, g=0.05)
+    rng = np.random.default_rng(115)
+    lb = D * rng.uniform(0.3, 1.6, D.shape)
+    for k in (1, 4):
+        gi, gd = wd.argmin_distance(X, Y, k=k, metric="wdtw", metric_params={"r": 0.2, "g": 0.05},
+                                    lower_bound=lb, return_distance=True)
+        oi, od = oracle.argmin(X, Y, k=k, metric="wdtw", r=0.2, g=0.05, lower_bound=lb)
+        np.testing.assert_array_equal(gi, oi)
+        np.testing.assert_array_equal(gd, od)

@@ -26,7 +26,7 @@ EXPORTS = (
     "wbx_init", "wbx_last_error", "wbx_version", "wbx_device_count",
     "wbx_pairwise", "wbx_paired", "wbx_argmin", "wbx_subsequence_pairwise",
     "wbx_dev_pairwise", "wbx_dev_pairwise_noalloc", "wbx_set_acc64_length", "wbx_dev_paired", "wbx_dev_argmin",
-    "wbx_dev_subsequence_pairwise", "wbx_set_policy", "wbx_prof_enable", "wbx_prof_take",
+    "wbx_dev_subsequence_pairwise", "wbx_set_policy", "wbx_prof_enable", "wbx_prof_take", "wbx_prof_pruned",
     "wbx_prof_valu_peak", "wbx_dev_eos_lengths", "wbx_batch_create", "wbx_batch_free",
     "wbx_batch_set_launch", "wbx_batch_autotune", "wbx_batch_bucket_info", "wbx_batch_info", "wbx_batch_run", "wbx_dtw_train_sizes", "wbx_dev_dtw_train_forward",
     "wbx_dev_dtw_train_backward", "wbx_argmin_subsequence", "wbx_subsequence_profile",
@@ -86,6 +86,7 @@ def load():
             "wbx_set_policy": (i32, [i32]),
             "wbx_prof_enable": (i32, [i32]),
             "wbx_prof_take": (i32, [p, p, p]),
+            "wbx_prof_pruned": (i32, [p]),
             "wbx_prof_valu_peak": (i32, [p]),
             "wbx_dev_eos_lengths": (i32, [i32, p, i64, i64, i64, p, p]),
             "wbx_batch_create": (i32, [m, i64, i64, p, p, p, i32, p]),
@@ -168,6 +169,14 @@ def set_acc64_length(n):
     """FP32 requests on series longer than ``n`` run with float64 accumulation
     (default 2048; 0 never promotes)."""
     check(load().wbx_set_acc64_length(int(n)))
+
+
+def prof_pruned():
+    """Pairs whose distance argmin with a lower bound computed since the last
+    call (current device)."""
+    n = ctypes.c_int64()
+    check(load().wbx_prof_pruned(ctypes.byref(n)))
+    return n.value
 
 
 def prof_enable(on=True):

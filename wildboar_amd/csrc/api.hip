@@ -406,13 +406,67 @@ int pairwise_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *X,
 }
 
 // -------------------------------------------------------------- argmin
+// argmin with a lower bound: the float64 distances of the pairs the
+// reference evaluates (prune.hip), then the heap replay with its skip rule.
+int argmin_pruned(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *X, int64_t nx,
+                  int64_t xstr, int64_t tx, const void *Y, int64_t ny, int64_t ystr, int64_t ty,
+                  int64_t k, const double *lb, int64_t *idx, double *dist, hipStream_t st) {
+    const int64_t np = nx * ny;
+    Carve cv;
+    const size_t o_x64 = cv.take(stage_bytes(dtype, true, nx, tx));
+    const size_t o_y64 = cv.take(stage_bytes(dtype, true, ny, ty));
+    const size_t o_d64 = cv.take((size_t)np * 8);
+    const size_t o_heap = cv.take((size_t)nx * (size_t)k * 16);
+    const size_t o_pa = cv.take((size_t)np * 4);
+    const size_t o_pb = cv.take((size_t)np * 4);
+    const size_t o_sq = cv.take((size_t)np * 8);
+    const size_t o_cnt = cv.take(8);
+    void *wsp = nullptr;
+    RC(ws_reserve(ctx, cv.total, &wsp));
+    char *ws = (char *)wsp;
+    Staged<double> x64, y64;
+    RC(stage<double>(dtype, X, nx, tx, xstr, ws, o_x64, &x64, st));
+    RC(stage<double>(dtype, Y, ny, ty, ystr, ws, o_y64, &y64, st));
+    const double *w64 = nullptr;
+    const float *w32 = nullptr;
+    if (m->kind == WBX_WDTW) RC(cached_weights(ctx, tx > ty ? tx : ty, m->g, &w64, &w32));
+    double *D64 = (double *)(ws + o_d64);
+    int32_t *pa = (int32_t *)(ws + o_pa), *pb = (int32_t *)(ws + o_pb);
+    double *sq = (double *)(ws + o_sq);
+    unsigned long long *cnt = (unsigned long long *)(ws + o_cnt);
+    wbx_metric m64 = *m;
+    m64.precision = WBX_FP64;
+    RC(launch_fill_f64(D64, np, __builtin_nan(""), st));  // NaN: not computed
+    // pass 0: the seeds; pass 1: the pairs the seeds cannot exclude
+    for (int pass = 0; pass < 2; ++pass) {
+        RC(launch_fill_i32((int *)cnt, 2, 0, st));
+        RC(pass == 0 ? launch_prune_seed(lb, nx, ny, k, pa, pb, cnt, st)
+                     : launch_prune_scan(D64, lb, nx, ny, k, pa, pb, cnt, st));
+        unsigned long long total = 0;
+        WBX_HIP_CHECK(hipMemcpyAsync(&total, cnt, 8, hipMemcpyDeviceToHost, st));
+        WBX_HIP_CHECK(hipStreamSynchronize(st));
+        ctx->prune_evaluated += (int64_t)total;
+        if (total == 0) continue;
+        PairSpec pl{PAIR_LIST, (int64_t)total, ny, pa, pb};
+        RC(run_pairs<double>(&m64, pl, series<double>(x64.ptr, x64.stride, tx),
+                             series<double>(y64.ptr, y64.stride, ty), w64, sq, st));
+        RC(launch_finish<double>(pl, sq, D64, ny, st));
+    }
+    return launch_argmin_replay(D64, nx, ny, k, lb, ws + o_heap, idx, dist, st);
+}
+
 int argmin_impl(DeviceCtx *ctx, const wbx_metric *m, int dtype, const void *X, int64_t nx,
                 int64_t xstr, int64_t tx, const void *Y, int64_t ny, int64_t ystr, int64_t ty,
                 int64_t k, const double *lb, int64_t *idx, double *dist, hipStream_t st) {
     const int64_t np = nx * ny;
+    // a lower bound: only the distances the reference evaluates (exact, float64)
+    if (lb && k <= prune_k_max() && g_policy.load() != 1)
+        return argmin_pruned(ctx, m, dtype, X, nx, xstr, tx, Y, ny, ystr, ty, k, lb, idx, dist, st);
     // the certified FP32 candidate bound covers the DTW recursion; ADTW runs
-    // in float64 throughout
-    const bool certified = m->precision == WBX_FP32 && k <= 32 && m->kind != WBX_ADTW;
+    // in float64 throughout.  The candidates are the pairs the sequence
+    // without a lower bound pushes; with a bound the reference pushes others
+    // (the first k it evaluates), so bounded calls take the float64 matrix.
+    const bool certified = m->precision == WBX_FP32 && k <= 32 && m->kind != WBX_ADTW && !lb;
     Carve cv;
     const size_t o_x64 = cv.take(stage_bytes(dtype, true, nx, tx));
     const size_t o_y64 = cv.take(stage_bytes(dtype, true, ny, ty));
@@ -1077,6 +1131,15 @@ int wbx_set_policy(int policy) {
 // Not part of the reference interface: the series length above which FP32
 // requests of pairwise / paired / argmin run with float64 accumulation
 // (FP32_ACC64); default 2048, 0 = never.
+int wbx_prof_pruned(int64_t *evaluated) {
+    if (!evaluated) return fail(WBX_EINVAL, "NULL argument");
+    DeviceCtx *ctx = ctx_current();
+    if (!ctx) return WBX_EDEVICE;
+    *evaluated = ctx->prune_evaluated;
+    ctx->prune_evaluated = 0;
+    return WBX_OK;
+}
+
 int wbx_set_acc64_length(int64_t len) {
     if (len < 0) return fail(WBX_EINVAL, "length must be >= 0");
     set_acc64_length(len);

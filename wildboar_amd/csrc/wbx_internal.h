@@ -158,6 +158,8 @@ struct DeviceCtx {
     };
     std::vector<Weights> weights;
     size_t weight_bytes = 0;
+    // pairs whose distance argmin with a lower bound computed (measurement)
+    int64_t prune_evaluated = 0;
 };
 
 DeviceCtx *ctx_current();              // current device's context (created lazily)
@@ -241,6 +243,15 @@ int launch_group_dtw(const PairSpec &ps, Series<T> xs, Series<T> ys, double r, c
 template <typename T>
 int launch_finish(const PairSpec &ps, const T *sq, double *out, int64_t ld,
                   hipStream_t st);
+
+// argmin with a lower bound (prune.hip): the seed pairs (first k and the
+// smallest bounds of every row), then every pair that the seeds cannot prove
+// the reference skips; pairs are appended to (pa, pb) at *cnt.
+int prune_k_max();
+int launch_prune_seed(const double *lb, int64_t nx, int64_t ny, int64_t k, int32_t *pa,
+                      int32_t *pb, unsigned long long *cnt, hipStream_t st);
+int launch_prune_scan(const double *D, const double *lb, int64_t nx, int64_t ny, int64_t k,
+                      int32_t *pa, int32_t *pb, unsigned long long *cnt, hipStream_t st);
 
 // Heap replay (_ArgMinBatch acceptance rule) per x row over a distance matrix.
 // heaps: nx * k * 16 bytes of scratch.
