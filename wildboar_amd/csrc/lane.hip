@@ -79,7 +79,7 @@ __global__ __launch_bounds__(64) void k_lane_dtw(PairSpec ps, Series<float> xs,
         }
         __syncthreads();
     }
-    const int64_t p0 = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    const int64_t p0 = xcd_block(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
     const int64_t p = p0 < ps.n_pairs ? p0 : ps.n_pairs - 1;
     int64_t a, b;
     decode_pair(ps, p, a, b);

@@ -72,6 +72,19 @@ __host__ __device__ inline int64_t band_cells(int lx, int ly, double r) {
     return cells;
 }
 
+// ---------------------------------------------------------------- launches
+// The logical index of block b of an n-block launch such that each XCD runs a
+// contiguous range: workgroups are dealt to the 8 XCDs of an MI355X round
+// robin (block b to XCD b % 8), so XCD x holds blocks x, x + 8, ... and gets
+// logical blocks [off(x), off(x) + count(x)).  Pairs that share a series then
+// meet in one XCD's L2 instead of every XCD reading every series.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t n) {
+    constexpr int64_t kXcd = 8;
+    const int64_t q = n / kXcd, r = n % kXcd;
+    const int64_t x = b % kXcd;
+    return x * q + (x < r ? x : r) + b / kXcd;
+}
+
 // ---------------------------------------------------------------- pairs
 // How a flat pair index p maps onto (a, b) = (row series, column series).
 enum PairMode : int {
