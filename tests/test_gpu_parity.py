@@ -138,6 +138,23 @@ def test_pairwise_fp32_dtw(n, r):
     assert_fp32_close(got, want)
 
 
+def test_pairwise_fp32_every_exact_band_width():
+    # Odd band widths 9 .. 111 run on kernels compiled for exactly that many
+    # slots (no padding slot, no per-row reset); widths 2w - 1 at equal
+    # lengths, 2w + 1 at lengths 60 / 62 (final cell off the band centre).
+    X = rw(26, 10, 60)
+    Y = rw(27, 12, 60)
+    Y2 = rw(28, 12, 62)
+    for w in range(5, 57):
+        r = (w + 0.5) / 60.0
+        got = wd.pairwise_distance(X, Y, metric="dtw", metric_params={"r": r}, precision="fp32")
+        assert_fp32_close(got, oracle.pairwise(X, Y, metric="dtw", r=r))
+        if 2 * w + 1 <= 111:
+            got = wd.pairwise_distance(X, Y2, metric="dtw", metric_params={"r": r},
+                                       precision="fp32")
+            assert_fp32_close(got, oracle.pairwise(X, Y2, metric="dtw", r=r))
+
+
 @pytest.mark.parametrize("lx,ly,r", [(100, 130, 0.1), (130, 100, 0.1), (37, 80, 0.2),
                                       (5, 3, 1.0), (1, 9, 0.5)])
 def test_pairwise_fp32_unequal(lx, ly, r):

@@ -1,152 +1,8 @@
-// lane.hip -- FP32 DTW with one pair per lane and the band row in registers.
-//
-// For pairs that share one geometry (lx, ly, w) -- every pair of a
-// pairwise/paired call on a fixed-width array, every window of one shapelet --
-// each of the 64 lanes of a wave runs the whole DP of its own pair.  The row
-// is kept in band coordinates (slot k <-> column j = i + k - off), where the
-// three predecessors of slot k are slot k+1 of the previous row (up), slot
-// k-1 of this row (left) and slot k of the previous row (diag).  Those are
-// static register indices, so a fully unrolled row is BP cells of
-//     d = x_i - y_j ; m = min3(up, left, diag) ; c = fma(d, d, m)
-// i.e. three VALU instructions per cell and no LDS or cross-lane traffic.
-//
-// The y values of a row are a window y[i-off .. i-off+BP) that slides by one
-// per row.  The kernel keeps a window of BP+U-1 registers, runs U rows against
-// it (row u of the group reads yw[k+u]) and then shifts it by U: the shift
-// costs (BP+U-1)/U moves per U*BP cells.  Columns outside [0, ly) read +inf,
-// which makes their cells +inf, and the row state before row 0 is +inf with a
-// 0 in slot off, which turns the first row into the reference's prefix sum.
-// Slots >= B (the padding up to BP) compute throw-away values; slot B is reset
-// to +inf after every row so slot B-1 never sees them.
-//
-// WDTW (WeightedDtwMetric, distance/_elastic.pyx:3322-3341) multiplies each
-// cell's squared difference by w[|i - j|]; in band coordinates |i - j| =
-// |k - off| is fixed per slot, so every lane of a launch uses the same weight
-// per slot: c = fma(d * d, wslot[k], m), four VALU instructions per cell.
-// The per-slot weights are staged in LDS once per block and re-read every row
-// (broadcast reads, no VGPRs held across rows).  Row 0 uses the reference's
-// prefix-sum weights w[j - 1] (w[0] for j = 0, :893-906), a second LDS row.
-#include "lane_common.h"
+// lane.hip -- launches of the lane-per-pair kernel (lane_kernel.h): bands
+// padded to a multiple of 8 slots, every metric variant (DTW, WDTW, ADTW).
+#include "lane_kernel.h"
 
 namespace wbx {
-
-
-// pen[q] is +inf for slot BP-8+q == B and -inf otherwise (uniform values), so
-// max(row, pen) resets slot B without a branch.
-// AD: the amercing recursion (adtw_distance, distance/_elastic.pyx:943-1008),
-// m = min3(up + p, left + p, diag); row 0 runs with p = 0 (its prefix sum
-// carries no penalty).
-template <int BP, bool AD = false, bool WT = false>
-__device__ __forceinline__ void lane_row(float (&row)[BP], const float *yw, float xi,
-                                         const float (&pen)[8], float p = 0.0f,
-                                         const float *wslot = nullptr) {
-    float left = WBX_INF_F;
-#pragma unroll
-    for (int k = 0; k < BP; ++k) {
-        const float up = (k + 1 < BP) ? row[k + 1] : WBX_INF_F;
-        float m;
-        if constexpr (AD)
-            m = fminf(fminf(up + p, left + p), row[k]);
-        else
-            m = fminf(fminf(up, left), row[k]);
-        const float d = xi - yw[k];
-        float c;
-        if constexpr (WT)
-            c = __builtin_fmaf(d * d, wslot[k], m);
-        else
-            c = __builtin_fmaf(d, d, m);
-        row[k] = c;
-        left = c;
-    }
-#pragma unroll
-    for (int q = 1; q < 8; ++q) row[BP - 8 + q] = fmaxf(row[BP - 8 + q], pen[q]);
-}
-
-template <int BP, int U, bool AD = false, bool WT = false>
-__global__ __launch_bounds__(64) void k_lane_dtw(PairSpec ps, Series<float> xs,
-                                                 Series<float> ys, LaneGeo g, float pen_ad,
-                                                 const float *__restrict__ weights,
-                                                 float *__restrict__ sq) {
-    constexpr int W = BP + U - 1;
-    // WT: lw[0][k] row-0 weight of slot k, lw[1][k] = w[|k - off|]; slots
-    // outside the band get 1 (their cells are +inf or thrown away)
-    __shared__ float lw[WT ? 2 : 1][WT ? BP : 1];
-    if constexpr (WT) {
-        for (int k = threadIdx.x; k < BP; k += 64) {
-            const int j = k - g.off;  // column of slot k in row 0
-            lw[0][k] = (j >= 0 && k < g.B) ? weights[j > 0 ? j - 1 : 0] : 1.0f;
-            lw[1][k] = k < g.B ? weights[j < 0 ? -j : j] : 1.0f;
-        }
-        __syncthreads();
-    }
-    const int64_t p0 = xcd_block(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
-    const int64_t p = p0 < ps.n_pairs ? p0 : ps.n_pairs - 1;
-    int64_t a, b;
-    decode_pair(ps, p, a, b);
-    const float *__restrict__ X = xs.row(a);
-    const float *__restrict__ Y = ys.row(b);
-    const int lx = g.lx, ly = g.ly, off = g.off, B = g.B;
-
-    float row[BP];
-#pragma unroll
-    for (int k = 0; k < BP; ++k) row[k] = (k == off) ? 0.0f : WBX_INF_F;
-    float pen[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) pen[q] = (BP - 8 + q == B) ? WBX_INF_F : -WBX_INF_F;
-
-    float yw[W];
-#pragma unroll
-    for (int m = 0; m < W; ++m) yw[m] = ldy(Y, m - off, ly);
-
-    int i_begin = 0;
-    if constexpr (WT) {
-        lane_row<BP, false, true>(row, yw, X[0], pen, 0.0f, lw[0]);
-#pragma unroll
-        for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
-        yw[W - 1] = ldy(Y, 1 - off + W - 1, ly);
-        i_begin = 1;
-    }
-    if constexpr (AD) {
-        lane_row<BP, true>(row, yw, X[0], pen, 0.0f);
-#pragma unroll
-        for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
-        yw[W - 1] = ldy(Y, 1 - off + W - 1, ly);
-        i_begin = 1;
-    }
-    // Leading rows one at a time so the rest is a whole number of U-groups.
-    const int rem = (lx - i_begin) % U;
-    for (int i = i_begin; i < i_begin + rem; ++i) {
-        lane_row<BP, AD, WT>(row, yw, X[i], pen, pen_ad, WT ? lw[WT ? 1 : 0] : nullptr);
-#pragma unroll
-        for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
-        yw[W - 1] = ldy(Y, i + 1 - off + W - 1, ly);
-    }
-    for (int i0 = i_begin + rem; i0 < lx; i0 += U) {
-        float xv[U], yn[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            xv[u] = X[i0 + u];
-            yn[u] = ldy(Y, i0 + U - off + W - U + u, ly);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            // the weights are re-read from LDS every row rather than held in
-            // registers across the loop (the band row and y window fill them)
-            if constexpr (WT) asm volatile("" ::: "memory");
-            lane_row<BP, AD, WT>(row, yw + u, xv[u], pen, pen_ad, WT ? lw[WT ? 1 : 0] : nullptr);
-        }
-#pragma unroll
-        for (int m = 0; m < W - U; ++m) yw[m] = yw[m + U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) yw[W - U + u] = yn[u];
-    }
-
-    float res = WBX_INF_F;
-#pragma unroll
-    for (int k = 0; k < BP; ++k)
-        if (k == g.kf) res = row[k];
-    if (p0 < ps.n_pairs) sq[p0] = res;
-}
 
 // The y load for group i0 fetches window slots W-U .. W-1 of the *next* group:
 // next window starts at i0 + U - off, its slot W-U+u is y[i0 + U - off + W-U+u].
@@ -183,6 +39,8 @@ int launch_lane_dtw(const PairSpec &ps, Series<float> xs, Series<float> ys, doub
     Band bd = make_band(xs.len, ys.len, r, w_fixed);
     if (bd.B > LANE_MAX_BP) return fail(WBX_EUNSUPPORTED, "lane kernel: band %d too wide", bd.B);
     LaneGeo g{bd.lx, bd.ly, bd.off, bd.B, bd.ly - bd.lx + bd.off};
+    if (!weights && !penalty && lane_exact_supported(bd.B))
+        return launch_lane_exact(ps, xs, ys, g, sq, st);
     const int BP = (bd.B + 7) / 8 * 8;
     switch (BP) {
         case 8: launch_one<8, 8>(ps, xs, ys, g, penalty, weights, sq, st); break;

@@ -12,6 +12,11 @@ struct LaneGeo {
     int lx, ly, off, B, kf;
 };
 
+// Plain DTW on a band of exactly B slots (odd B, 9 .. 111; lane_exact.hip).
+bool lane_exact_supported(int B);
+int launch_lane_exact(const PairSpec &ps, Series<float> xs, Series<float> ys, const LaneGeo &g,
+                      float *sq, hipStream_t st);
+
 // y[j] for j in [0, ly), +inf outside (the cells there become +inf).
 __device__ __forceinline__ float ldy(const float *__restrict__ Y, int j, int ly) {
     const int jc = j < 0 ? 0 : (j >= ly ? ly - 1 : j);
