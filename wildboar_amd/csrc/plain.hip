@@ -4,8 +4,8 @@
 //                 reference's band with three diagonals in LDS.  Any lengths,
 //                 any band, WDTW weights with the row-0 index quirk.  This is
 //                 the FP64 parity kernel and the "plain per-pair" baseline.
-//  k_euclid_tile  16x16 pairs per block, LDS tiles, each lane sums its pair
-//                 sequentially in the reference's order (exact in FP64).
+//  k_euclid_grid  64x64 pairs per block, LDS tiles, each lane sums 4x4 pairs,
+//                 each sequentially in the reference's order (exact in FP64).
 //  k_finish       sqrt + scatter into the driver's output layout.
 //  k_argmin_replay_wave  _ArgMinBatch's heap acceptance rule, one wave per x row.
 //
@@ -159,39 +159,85 @@ template int launch_plain_dtw<double>(const PairSpec &, Series<double>, Series<d
 // ------------------------------------------------------------- euclidean
 // euclidean_distance (distance/_metric.pyx:654-683) with S = x, T = y: windows
 // of y; per window dist = sum_j (y[i+j] - x[j])^2 accumulated in order.  Equal
-// lengths give one window.  Pairs are tiled 16 x 16 per block for GRID mode;
-// other modes use one lane per pair straight from memory.
-constexpr int ET = 16;   // tile edge
-constexpr int EC = 64;   // timesteps per LDS chunk
+// lengths give one window.  GRID mode: 64 x 64 pairs per block, 32 timesteps
+// of the 64 x rows and 64 y rows staged in LDS per chunk, each lane summing
+// 4 x 4 pairs (rows ty + 16 i, columns tx + 16 j: conflict-free LDS reads),
+// so a timestep costs 8 LDS reads for 16 pairs.  Every pair's sum stays
+// sequential in j (float64: multiply then add, the reference's arithmetic;
+// float32: fused).  Other modes use one lane per pair straight from memory.
+constexpr int EB = 64;   // tile edge (pairs)
+constexpr int EK = 32;   // timesteps per LDS chunk
+
+__device__ __forceinline__ void euclid_acc(float &acc, float v) { acc = __builtin_fmaf(v, v, acc); }
+__device__ __forceinline__ void euclid_acc(double &acc, double v) { acc += v * v; }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_euclid_grid(Series<T> xs, int64_t nx,
                                                      Series<T> ys, int64_t ny,
                                                      int t, T *__restrict__ sq) {
-    __shared__ T sx[ET][EC + 1];
-    __shared__ T sy[ET][EC + 1];
-    const int tx = threadIdx.x % ET, ty = threadIdx.x / ET;
-    const int64_t a0 = (int64_t)blockIdx.y * ET, b0 = (int64_t)blockIdx.x * ET;
-    const int64_t a = a0 + ty, b = b0 + tx;
-    T dist = (T)0;
-    for (int c0 = 0; c0 < t; c0 += EC) {
-        const int cn = t - c0 < EC ? t - c0 : EC;
-        for (int q = threadIdx.x; q < ET * EC; q += 256) {
-            const int rr = q / EC, cc = q % EC;
+    __shared__ T sx[EB][EK + 1];
+    __shared__ T sy[EB][EK + 1];
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const int64_t a0 = (int64_t)blockIdx.y * EB, b0 = (int64_t)blockIdx.x * EB;
+    T acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = (T)0;
+    // chunk c0's values are loaded into registers while chunk c0 - EK is
+    // summed, then stored to LDS
+    constexpr int PER = EB * EK / 256;  // values of each tile per thread
+    T px[PER], py[PER];
+    auto fetch = [&](int c0) {
+        const int cn = t - c0 < EK ? t - c0 : EK;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int q = threadIdx.x + 256 * u;
+            const int rr = q / EK, cc = q % EK;
+            T vx = (T)0, vy = (T)0;
             if (cc < cn) {
-                sx[rr][cc] = (a0 + rr < nx) ? xs.row(a0 + rr)[c0 + cc] : (T)0;
-                sy[rr][cc] = (b0 + rr < ny) ? ys.row(b0 + rr)[c0 + cc] : (T)0;
+                if (a0 + rr < nx) vx = xs.row(a0 + rr)[c0 + cc];
+                if (b0 + rr < ny) vy = ys.row(b0 + rr)[c0 + cc];
             }
+            px[u] = vx;
+            py[u] = vy;
+        }
+    };
+    fetch(0);
+    for (int c0 = 0; c0 < t; c0 += EK) {
+        const int cn = t - c0 < EK ? t - c0 : EK;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int q = threadIdx.x + 256 * u;
+            sx[q / EK][q % EK] = px[u];
+            sy[q / EK][q % EK] = py[u];
         }
         __syncthreads();
+        if (c0 + EK < t) fetch(c0 + EK);
         for (int cc = 0; cc < cn; ++cc) {
-            T v = sy[tx][cc];
-            v -= sx[ty][cc];
-            dist += v * v;
+            T xv[4], yv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                xv[i] = sx[ty + 16 * i][cc];
+                yv[i] = sy[tx + 16 * i][cc];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) euclid_acc(acc[i][j], yv[j] - xv[i]);
         }
         __syncthreads();
     }
-    if (a < nx && b < ny) sq[a * ny + b] = dist;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t a = a0 + ty + 16 * i;
+        if (a >= nx) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t b = b0 + tx + 16 * j;
+            if (b < ny) sq[a * ny + b] = acc[i][j];
+        }
+    }
 }
 
 template <typename T>
@@ -225,7 +271,7 @@ int launch_euclid_pairs(const PairSpec &ps, Series<T> xs, Series<T> ys, T *sq,
     const bool fixed = xs.lens == nullptr && ys.lens == nullptr && xs.len == ys.len;
     if (ps.mode == PAIR_GRID && fixed) {
         const int64_t nx = ps.n_pairs / ps.ny, ny = ps.ny;
-        dim3 grid((unsigned)((ny + ET - 1) / ET), (unsigned)((nx + ET - 1) / ET));
+        dim3 grid((unsigned)((ny + EB - 1) / EB), (unsigned)((nx + EB - 1) / EB));
         if (grid.y > 65535) return fail(WBX_EUNSUPPORTED, "too many rows for euclidean tiles");
         hipLaunchKernelGGL(k_euclid_grid<T>, grid, dim3(256), 0, st, xs, nx, ys, ny, xs.len, sq);
     } else {
