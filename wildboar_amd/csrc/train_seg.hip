@@ -425,15 +425,41 @@ __device__ __forceinline__ float seg_col_sum(uint32_t seg, const float *__restri
     return s;
 }
 
+// P[s * (len + 1) + i] = sum of series s's first i values in float64; one
+// wave per series (a fixed sum order: deterministic).
+__global__ __launch_bounds__(64) void k_seg_prefix(const float *__restrict__ X, int64_t stride,
+                                                   int len, double *__restrict__ P) {
+    const int64_t sidx = blockIdx.x;
+    const int lane = threadIdx.x;
+    const float *x = X + sidx * stride;
+    double *out = P + sidx * (int64_t)(len + 1);
+    if (lane == 0) out[0] = 0.0;
+    double carry = 0.0;
+    for (int i0 = 0; i0 < len; i0 += 64) {
+        const int i = i0 + lane;
+        double v = i < len ? (double)x[i] : 0.0;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const double o = __shfl_up(v, d);
+            if (lane >= d) v += o;
+        }
+        v += carry;
+        if (i < len) out[i + 1] = v;
+        carry = __shfl(v, 63);
+    }
+}
+
 // Block (c, p - p0): rows and columns c * 256 + threadIdx.x of pair p (rows first,
-// then columns): the gradient of the row / column, into dX / dY rows p (paired
+// then columns; DTW sums a segment from the float64 prefix sums of the other
+// series, WDTW cell by cell): the gradient of the row / column, into dX / dY rows p (paired
 // calls: ld = the caller's strides), or, for all pairs, over the segment word
 // itself (gx = path.rows, gy = path.cols as floats: each thread reads its own
 // word before writing it).
 template <bool WT>
 __global__ __launch_bounds__(256) void k_seg_grad_pairs(
     SegPath path, PairSpec ps, int64_t p0, Series<float> xs, Series<float> ys,
-    const float *__restrict__ wv, int nw, const float *__restrict__ coef, float *gx, int64_t ldx,
+    const float *__restrict__ wv, int nw, const float *__restrict__ coef,
+    const double *__restrict__ PX, const double *__restrict__ PY, float *gx, int64_t ldx,
     float *gy, int64_t ldy) {
     const int64_t p = p0 + blockIdx.y;
     const int lx = xs.len, ly = ys.len;
@@ -444,11 +470,27 @@ __global__ __launch_bounds__(256) void k_seg_grad_pairs(
     const float cf = coef[p];
     if (t < lx) {
         const uint32_t seg = path.rows[p * lx + t];
-        gx[p * ldx + t] = cf * seg_row_sum<WT>(seg, xs.row(a), ys.row(b), wv, nw - 1, t);
+        float s;
+        if constexpr (WT) {
+            s = seg_row_sum<WT>(seg, xs.row(a), ys.row(b), wv, nw - 1, t);
+        } else {  // n x_i - (y_j0 + ... + y_(j0+n-1)) from the prefix sums of y_b
+            const int j0 = (int)(seg >> 16), n = (int)(seg & 0xFFFFu);
+            const double *py = PY + b * (int64_t)(ly + 1);
+            s = (float)((double)n * (double)xs.row(a)[t] - (py[j0 + n] - py[j0]));
+        }
+        gx[p * ldx + t] = cf * s;
     } else {
         const int j = t - lx;
         const uint32_t seg = path.cols[p * ly + j];
-        gy[p * ldy + j] = cf * seg_col_sum<WT>(seg, xs.row(a), ys.row(b), wv, nw - 1, j);
+        float s;
+        if constexpr (WT) {
+            s = seg_col_sum<WT>(seg, xs.row(a), ys.row(b), wv, nw - 1, j);
+        } else {  // n y_j - (x_i0 + ... + x_(i0+n-1)) from the prefix sums of x_a
+            const int i0 = (int)(seg >> 16), n = (int)(seg & 0xFFFFu);
+            const double *px = PX + a * (int64_t)(lx + 1);
+            s = (float)((double)n * (double)ys.row(b)[j] - (px[i0 + n] - px[i0]));
+        }
+        gy[p * ldy + j] = cf * s;
     }
 }
 
@@ -611,11 +653,17 @@ template <bool WT>
 int seg_launch_grad(int mode, SegPath path, int64_t nx, int64_t ny, Series<float> xs,
                     Series<float> ys, const float *w, int nw, const float *sq, const double *G,
                     float *dX, int64_t ldx, float *dY, int64_t ldy, float *coef, float *part,
-                    hipStream_t st) {
+                    double *PX, double *PY, hipStream_t st) {
     const int64_t tx = xs.len, ty = ys.len;
     const int64_t np = mode == 1 ? nx : nx * ny;
     hipLaunchKernelGGL(k_seg_coef, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, sq, G, np,
                        coef);
+    if (!WT) {
+        hipLaunchKernelGGL(k_seg_prefix, dim3((unsigned)nx), dim3(64), 0, st, xs.base, xs.stride,
+                           (int)tx, PX);
+        hipLaunchKernelGGL(k_seg_prefix, dim3((unsigned)ny), dim3(64), 0, st, ys.base, ys.stride,
+                           (int)ty, PY);
+    }
     PairSpec ps{mode == 0 ? PAIR_GRID : PAIR_DIAG, np, mode == 0 ? ny : nx, nullptr, nullptr};
     // pairs along y (at most 65535 per launch), rows and columns along x
     const unsigned gxb = (unsigned)((tx + ty + 255) / 256);
@@ -623,7 +671,7 @@ int seg_launch_grad(int mode, SegPath path, int64_t nx, int64_t ny, Series<float
         for (int64_t p0 = 0; p0 < np; p0 += 65535) {
             const int64_t cnt = np - p0 < 65535 ? np - p0 : 65535;
             hipLaunchKernelGGL(k_seg_grad_pairs<WT>, dim3(gxb, (unsigned)cnt), dim3(256), 0, st,
-                               path, ps, p0, xs, ys, w, nw, coef, gx, lgx, gy, lgy);
+                               path, ps, p0, xs, ys, w, nw, coef, PX, PY, gx, lgx, gy, lgy);
         }
     };
     if (mode == 1) {
@@ -646,6 +694,35 @@ int seg_launch_grad(int mode, SegPath path, int64_t nx, int64_t ny, Series<float
 
 }  // namespace
 
+// Backward scratch: the path segments (one word per row and column of each
+// pair), G / D per pair, for all pairs the column partials, and the float64
+// prefix sums of X and Y (DTW segment sums).
+struct SegScratch {
+    int64_t rows, cols, coef, part, px, py, total;
+};
+
+static SegScratch seg_scratch(int mode, int64_t nx, int64_t ny, int64_t tx, int64_t ty) {
+    auto al = [](int64_t v) { return (v + 255) / 256 * 256; };
+    const int64_t np = mode == 0 ? nx * ny : nx;
+    const int64_t nc = (nx + kSegChunk - 1) / kSegChunk;
+    SegScratch s{};
+    int64_t o = 0;
+    s.rows = o;
+    o += al(np * tx * 4);
+    s.cols = o;
+    o += al(np * ty * 4);
+    s.coef = o;
+    o += al(np * 4);
+    s.part = o;
+    o += al(mode == 0 ? nc * ny * ty * 4 : 0);
+    s.px = o;
+    o += al(nx * (tx + 1) * 8);
+    s.py = o;
+    o += al(ny * (ty + 1) * 8);
+    s.total = o;
+    return s;
+}
+
 int seg_train_sizes(const wbx_metric *m, int mode, int64_t nx, int64_t ny, int64_t tx, int64_t ty,
                     int64_t *state_bytes, int64_t *scratch_bytes) {
     SegGeo g;
@@ -654,11 +731,8 @@ int seg_train_sizes(const wbx_metric *m, int mode, int64_t nx, int64_t ny, int64
     const int64_t np = mode == 0 ? nx * ny : nx;
     if (state_bytes) *state_bytes = seg_state(g, np).total;
     // the path segments (one word per row and column of each pair), G / D per
-    // pair, and for all pairs the column partials
-    const int64_t nc = (nx + kSegChunk - 1) / kSegChunk;
-    if (scratch_bytes)
-        *scratch_bytes =
-            ((np * (tx + ty + 1) * 4 + (mode == 0 ? nc * ny * ty * 4 : 0) + 255) / 256) * 256;
+    // pair, for all pairs the column partials, and the prefix sums of X and Y
+    if (scratch_bytes) *scratch_bytes = seg_scratch(mode, nx, ny, tx, ty).total;
     return WBX_OK;
 }
 
@@ -711,17 +785,20 @@ int seg_train_backward(const wbx_metric *m, int mode, const float *X, int64_t nx
     const float *sq = (const float *)(sb + ss.sq);
     const float *w = (const float *)(sb + ss.w);
     const float *ck = (const float *)(sb + ss.ck);
-    SegPath path{(uint32_t *)scratch, (uint32_t *)scratch + ps.n_pairs * tx};
-    float *coef = (float *)(path.cols + ps.n_pairs * ty);
-    float *part = coef + ps.n_pairs;
+    const SegScratch sc = seg_scratch(mode, nx, ny, tx, ty);
+    char *scb = (char *)scratch;
+    SegPath path{(uint32_t *)(scb + sc.rows), (uint32_t *)(scb + sc.cols)};
+    float *coef = (float *)(scb + sc.coef);
+    float *part = (float *)(scb + sc.part);
+    double *px = (double *)(scb + sc.px), *py = (double *)(scb + sc.py);
     Series<float> xs{X, x_stride, (int)tx, nullptr}, ys{Y, y_stride, (int)ty, nullptr};
     rc = g.nw > 0 ? seg_launch_bwd<true>(ps, xs, ys, g, w, ck, path, st)
                   : seg_launch_bwd<false>(ps, xs, ys, g, nullptr, ck, path, st);
     if (rc) return rc;
     return g.nw > 0 ? seg_launch_grad<true>(mode, path, nx, ny, xs, ys, w, g.nw, sq, G, dX, ldx,
-                                            dY, ldy, coef, part, st)
+                                            dY, ldy, coef, part, px, py, st)
                     : seg_launch_grad<false>(mode, path, nx, ny, xs, ys, nullptr, 1, sq, G, dX,
-                                             ldx, dY, ldy, coef, part, st);
+                                             ldx, dY, ldy, coef, part, px, py, st);
 }
 
 }  // namespace wbx
