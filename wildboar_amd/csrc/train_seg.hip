@@ -527,6 +527,58 @@ __global__ __launch_bounds__(256) void k_seg_sum_y_part(const float *__restrict_
     part[c * step + t] = s;
 }
 
+// DTW, all pairs: the two sums above with each pair's term formed from its
+// segment word and the prefix sums on the fly (the same float arithmetic as
+// k_seg_grad_pairs followed by k_seg_sum_x / k_seg_sum_y_part, without the
+// per-pair terms in memory).
+__global__ __launch_bounds__(256) void k_seg_sum_x_dtw(const uint32_t *__restrict__ rows,
+                                                       const float *__restrict__ coef,
+                                                       Series<float> xs, const double *__restrict__ PY,
+                                                       int64_t nx, int64_t ny, int ly,
+                                                       float *__restrict__ dX, int64_t ldx) {
+    const int lx = xs.len;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= nx * lx) return;
+    const int64_t a = t / lx;
+    const int i = (int)(t - a * lx);
+    const double xi = (double)xs.row(a)[i];
+    const uint32_t *src = rows + a * ny * lx + i;
+    const float *cf = coef + a * ny;
+    float s = 0.0f;
+    for (int64_t b = 0; b < ny; ++b) {
+        const uint32_t seg = src[b * lx];
+        const int j0 = (int)(seg >> 16), n = (int)(seg & 0xFFFFu);
+        const double *py = PY + b * (int64_t)(ly + 1);
+        s += cf[b] * (float)((double)n * xi - (py[j0 + n] - py[j0]));
+    }
+    dX[a * ldx + i] = s;
+}
+
+__global__ __launch_bounds__(256) void k_seg_sum_y_part_dtw(const uint32_t *__restrict__ cols,
+                                                            const float *__restrict__ coef,
+                                                            Series<float> ys,
+                                                            const double *__restrict__ PX, int64_t nx,
+                                                            int64_t ny, int lx,
+                                                            float *__restrict__ part) {
+    const int ly = ys.len;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= ny * ly) return;
+    const int64_t b = t / ly;
+    const int j = (int)(t - b * ly);
+    const int64_t c = blockIdx.y;
+    const int64_t a0 = c * kSegChunk, a1 = a0 + kSegChunk < nx ? a0 + kSegChunk : nx;
+    const double yj = (double)ys.row(b)[j];
+    const int64_t step = ny * ly;
+    float s = 0.0f;
+    for (int64_t a = a0; a < a1; ++a) {
+        const uint32_t seg = cols[a * step + t];
+        const int i0 = (int)(seg >> 16), n = (int)(seg & 0xFFFFu);
+        const double *px = PX + a * (int64_t)(lx + 1);
+        s += coef[a * ny + b] * (float)((double)n * yj - (px[i0 + n] - px[i0]));
+    }
+    part[c * step + t] = s;
+}
+
 __global__ __launch_bounds__(256) void k_seg_sum_y_final(const float *__restrict__ part, int nc,
                                                          int64_t ny, int ly,
                                                          float *__restrict__ dY, int64_t ldy) {
@@ -676,6 +728,15 @@ int seg_launch_grad(int mode, SegPath path, int64_t nx, int64_t ny, Series<float
     };
     if (mode == 1) {
         grads(dX, ldx, dY, ldy);
+    } else if (!WT) {
+        hipLaunchKernelGGL(k_seg_sum_x_dtw, dim3((unsigned)((nx * tx + 255) / 256)), dim3(256), 0,
+                           st, path.rows, coef, xs, PY, nx, ny, (int)ty, dX, ldx);
+        const int nc = (int)((nx + kSegChunk - 1) / kSegChunk);
+        const unsigned yb = (unsigned)((ny * ty + 255) / 256);
+        hipLaunchKernelGGL(k_seg_sum_y_part_dtw, dim3(yb, (unsigned)nc), dim3(256), 0, st,
+                           path.cols, coef, ys, PX, nx, ny, (int)tx, part);
+        hipLaunchKernelGGL(k_seg_sum_y_final, dim3(yb), dim3(256), 0, st, part, nc, ny, (int)ty,
+                           dY, ldy);
     } else {
         float *gx = (float *)path.rows, *gy = (float *)path.cols;
         grads(gx, tx, gy, ty);
