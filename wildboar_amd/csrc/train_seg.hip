@@ -255,18 +255,20 @@ __global__ __launch_bounds__(64) void k_seg_forward(PairSpec ps, Series<float> x
     float *__restrict__ ck = ckpt + ((p * g.nseg) * g.n + q) * S;
     const int64_t cstride = (int64_t)g.n * S;
     uint32_t nb = 0;
+    float x = sg_ldx(X, L.r, g.lx);  // x of the step, loaded one step ahead
     for (int s = 0; s < g.nsteps; ++s) {
         if (s % kSegK == 0 && valid) {
             float *dst = ck + (s / kSegK) * cstride;
 #pragma unroll
             for (int m = 0; m < S; ++m) dst[m] = L.row[m];
         }
-        const float x = sg_ldx(X, L.r, g.lx);
+        const float xn = sg_ldx(X, L.r + 1, g.lx);
         if (s < g.n || s >= g.lx)
             seg_step<R, WT, true, false>(L, x, g.lx, nb, nb);
         else
             seg_step<R, WT, false, false>(L, x, g.lx, nb, nb);
         seg_advance(L, Y, g.ly);
+        x = xn;
     }
     const int nslots = L.shortl ? R : S;
     if (valid && g.kf >= L.k0 && g.kf < L.k0 + nslots) {
@@ -318,12 +320,17 @@ __global__ __launch_bounds__(64) void k_seg_backward(PairSpec ps, Series<float> 
     for (int c = g.nseg - 1; c >= 0; --c) {
         const int s0 = c * kSegK;
         const int s1 = s0 + kSegK < g.nsteps ? s0 + kSegK : g.nsteps;
-        const float *src = ck + c * cstride;
+        // lanes of no pair read pair 0's first checkpoint (in bounds) and drop it
+        const float *src = valid ? ck + c * cstride : ckpt;
 #pragma unroll
-        for (int m = 0; m < S; ++m) L.row[m] = valid ? src[m] : WBX_INF_F;
+        for (int m = 0; m < S; ++m) {
+            const float v = src[m];
+            L.row[m] = valid ? v : WBX_INF_F;
+        }
         seg_restore(L, g, q, s0, Y);
+        float x = sg_ldx(X, L.r, g.lx);  // x of the step, loaded one step ahead
         for (int s = s0; s < s1; ++s) {
-            const float x = sg_ldx(X, L.r, g.lx);
+            const float xn = sg_ldx(X, L.r + 1, g.lx);
             uint32_t bd = 0, bu = 0;
             if (s < g.n || s >= g.lx)
                 seg_step<R, WT, true, true>(L, x, g.lx, bd, bu);
@@ -332,6 +339,7 @@ __global__ __launch_bounds__(64) void k_seg_backward(PairSpec ps, Series<float> 
             bits[s - s0][lane][0] = bd;
             bits[s - s0][lane][1] = bu;
             seg_advance(L, Y, g.ly);
+            x = xn;
         }
         __syncthreads();
         while (!done) {
