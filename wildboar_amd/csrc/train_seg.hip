@@ -46,6 +46,7 @@ namespace wbx {
 namespace {
 
 constexpr int kSegK = 32;  // steps per checkpoint band
+constexpr int kSegU = 4;   // steps per group (one window shift and load batch)
 
 struct SegGeo {
     int lx, ly, off, B, kf;
@@ -82,7 +83,7 @@ struct SLane {
     static_assert(R + 1 <= 32, "one decision word per plane, lane and step");
     static constexpr int S = R + 1;
     float row[S];
-    float yw[S];  // yw[m] = y[yj + m]: the columns of this step's slots
+    float yw[S + kSegU - 1];  // yw[m] = y[yj + m]: step u of a group reads yw[u + m]
     float wt[WT ? S : 1], w0[WT ? S : 1];
     float l;      // last real slot of the lane's previous row
     int r, yj, k0;
@@ -131,7 +132,7 @@ __device__ __forceinline__ void seg_restore(SLane<R, WT> &L, const SegGeo &g, in
     L.r = s - q;
     L.yj = L.r + L.k0 - g.off;
 #pragma unroll
-    for (int m = 0; m < S; ++m) L.yw[m] = sg_ldy(Y, L.yj + m, g.ly);
+    for (int m = 0; m < S + kSegU - 1; ++m) L.yw[m] = sg_ldy(Y, L.yj + m, g.ly);
 }
 
 // A cell of the replay: the same float32 value as the forward's
@@ -174,7 +175,7 @@ __device__ __forceinline__ float rec_cell(float mn, float dg, float up, float x,
 // One step: row L.r of the lane.  SEL keeps the state of a lane outside
 // [0, lx); REC shifts into bd / bu, slot by slot, whether the slot's minimum
 // equals its diag / up input (bd, bu start at 0).
-template <int R, bool WT, bool SEL, bool REC>
+template <int R, bool WT, bool SEL, bool REC, int YO = 0>
 __device__ __forceinline__ void seg_step(SLane<R, WT> &L, float x, int lx, uint32_t &bd,
                                          uint32_t &bu) {
     constexpr int S = R + 1;
@@ -198,7 +199,7 @@ __device__ __forceinline__ void seg_step(SLane<R, WT> &L, float x, int lx, uint3
     };
     float Lin = sg_shr1(L.l);
     Lin = L.first ? WBX_INF_F : Lin;
-    const float c0 = cell(L.row[1], Lin, L.row[0], L.yw[0], 0);
+    const float c0 = cell(L.row[1], Lin, L.row[0], L.yw[YO], 0);
     float Uin = sg_shl1(SEL && !act ? L.row[0] : c0);
     Uin = L.last ? WBX_INF_F : Uin;
     float left = c0;
@@ -212,7 +213,7 @@ __device__ __forceinline__ void seg_step(SLane<R, WT> &L, float x, int lx, uint3
             up = L.shortl ? Uin : L.row[m + 1];
         else
             up = L.row[m + 1];
-        const float c = cell(up, left, L.row[m], L.yw[m], m);
+        const float c = cell(up, left, L.row[m], L.yw[YO + m], m);
         L.row[m] = SEL ? (act ? c : L.row[m]) : c;
         left = c;
     }
@@ -220,15 +221,50 @@ __device__ __forceinline__ void seg_step(SLane<R, WT> &L, float x, int lx, uint3
     L.l = L.shortl ? L.row[R - 1] : L.row[R];
 }
 
+// One step without a group: the window moves on by one.
 template <int R, bool WT>
 __device__ __forceinline__ void seg_advance(SLane<R, WT> &L, const float *__restrict__ Y,
                                             int ly) {
-    constexpr int S = R + 1;
+    constexpr int W = R + kSegU;  // S + kSegU - 1 entries
 #pragma unroll
-    for (int m = 0; m < S - 1; ++m) L.yw[m] = L.yw[m + 1];
-    L.yw[S - 1] = sg_ldy(Y, L.yj + S, ly);
+    for (int m = 0; m < W - 1; ++m) L.yw[m] = L.yw[m + 1];
+    L.yw[W - 1] = sg_ldy(Y, L.yj + W, ly);
     L.r += 1;
     L.yj += 1;
+}
+
+// kSegU steps: the next group's x and window entries are loaded first, the
+// steps read the window at offsets 0 .. kSegU - 1, then it moves on by kSegU.
+// xv holds this group's x and receives the next group's.
+template <int R, bool WT, bool SEL, bool REC>
+__device__ __forceinline__ void seg_group(SLane<R, WT> &L, float (&xv)[kSegU],
+                                          const float *__restrict__ X, int lx,
+                                          const float *__restrict__ Y, int ly,
+                                          uint32_t (&bd)[kSegU], uint32_t (&bu)[kSegU]) {
+    constexpr int W = R + kSegU;
+    float xn[kSegU], yn[kSegU];
+#pragma unroll
+    for (int u = 0; u < kSegU; ++u) {
+        xn[u] = sg_ldx(X, L.r + kSegU + u, lx);
+        yn[u] = sg_ldy(Y, L.yj + W + u, ly);
+    }
+#define WBX_SEG_STEP(u)                                                  \
+    seg_step<R, WT, SEL, REC, u>(L, xv[u], lx, bd[u], bu[u]);            \
+    L.r += 1;
+    WBX_SEG_STEP(0)
+    WBX_SEG_STEP(1)
+    WBX_SEG_STEP(2)
+    WBX_SEG_STEP(3)
+#undef WBX_SEG_STEP
+    static_assert(kSegU == 4, "four steps per group");
+#pragma unroll
+    for (int m = 0; m < W - kSegU; ++m) L.yw[m] = L.yw[m + kSegU];
+#pragma unroll
+    for (int u = 0; u < kSegU; ++u) {
+        L.yw[W - kSegU + u] = yn[u];
+        xv[u] = xn[u];
+    }
+    L.yj += kSegU;
 }
 
 template <int R, bool WT>
@@ -254,21 +290,32 @@ __global__ __launch_bounds__(64) void k_seg_forward(PairSpec ps, Series<float> x
     seg_restore(L, g, q, 0, Y);
     float *__restrict__ ck = ckpt + ((p * g.nseg) * g.n + q) * S;
     const int64_t cstride = (int64_t)g.n * S;
-    uint32_t nb = 0;
-    float x = sg_ldx(X, L.r, g.lx);  // x of the step, loaded one step ahead
-    for (int s = 0; s < g.nsteps; ++s) {
+    uint32_t nb[kSegU] = {};
+    float xv[kSegU];
+#pragma unroll
+    for (int u = 0; u < kSegU; ++u) xv[u] = sg_ldx(X, L.r + u, g.lx);
+    int s = 0;
+    // WDTW's forward keeps single steps: its grouped form needs more
+    // registers than two waves per SIMD allow
+    for (; !WT && s + kSegU <= g.nsteps; s += kSegU) {  // kSegK is a multiple of kSegU
         if (s % kSegK == 0 && valid) {
             float *dst = ck + (s / kSegK) * cstride;
 #pragma unroll
             for (int m = 0; m < S; ++m) dst[m] = L.row[m];
         }
-        const float xn = sg_ldx(X, L.r + 1, g.lx);
-        if (s < g.n || s >= g.lx)
-            seg_step<R, WT, true, false>(L, x, g.lx, nb, nb);
+        if (s < g.n || s + kSegU > g.lx)
+            seg_group<R, WT, true, false>(L, xv, X, g.lx, Y, g.ly, nb, nb);
         else
-            seg_step<R, WT, false, false>(L, x, g.lx, nb, nb);
+            seg_group<R, WT, false, false>(L, xv, X, g.lx, Y, g.ly, nb, nb);
+    }
+    for (; s < g.nsteps; ++s) {  // the last steps one at a time (WDTW: every step)
+        if (s % kSegK == 0 && valid) {
+            float *dst = ck + (s / kSegK) * cstride;
+#pragma unroll
+            for (int m = 0; m < S; ++m) dst[m] = L.row[m];
+        }
+        seg_step<R, WT, true, false>(L, sg_ldx(X, L.r, g.lx), g.lx, nb[0], nb[0]);
         seg_advance(L, Y, g.ly);
-        x = xn;
     }
     const int nslots = L.shortl ? R : S;
     if (valid && g.kf >= L.k0 && g.kf < L.k0 + nslots) {
@@ -328,18 +375,28 @@ __global__ __launch_bounds__(64) void k_seg_backward(PairSpec ps, Series<float> 
             L.row[m] = valid ? v : WBX_INF_F;
         }
         seg_restore(L, g, q, s0, Y);
-        float x = sg_ldx(X, L.r, g.lx);  // x of the step, loaded one step ahead
-        for (int s = s0; s < s1; ++s) {
-            const float xn = sg_ldx(X, L.r + 1, g.lx);
-            uint32_t bd = 0, bu = 0;
-            if (s < g.n || s >= g.lx)
-                seg_step<R, WT, true, true>(L, x, g.lx, bd, bu);
+        float xv[kSegU];
+#pragma unroll
+        for (int u = 0; u < kSegU; ++u) xv[u] = sg_ldx(X, L.r + u, g.lx);
+        int s = s0;
+        for (; s + kSegU <= s1; s += kSegU) {
+            uint32_t bd[kSegU] = {}, bu[kSegU] = {};
+            if (s < g.n || s + kSegU > g.lx)
+                seg_group<R, WT, true, true>(L, xv, X, g.lx, Y, g.ly, bd, bu);
             else
-                seg_step<R, WT, false, true>(L, x, g.lx, bd, bu);
+                seg_group<R, WT, false, true>(L, xv, X, g.lx, Y, g.ly, bd, bu);
+#pragma unroll
+            for (int u = 0; u < kSegU; ++u) {
+                bits[s + u - s0][lane][0] = bd[u];
+                bits[s + u - s0][lane][1] = bu[u];
+            }
+        }
+        for (; s < s1; ++s) {  // the pair's last steps one at a time
+            uint32_t bd = 0, bu = 0;
+            seg_step<R, WT, true, true>(L, sg_ldx(X, L.r, g.lx), g.lx, bd, bu);
             bits[s - s0][lane][0] = bd;
             bits[s - s0][lane][1] = bu;
             seg_advance(L, Y, g.ly);
-            x = xn;
         }
         __syncthreads();
         while (!done) {
