@@ -610,6 +610,7 @@ __global__ __launch_bounds__(256) void k_seg_sum_x_dtw(const uint32_t *__restric
     const uint32_t *src = rows + a * ny * lx + i;
     const float *cf = coef + a * ny;
     float s = 0.0f;
+#pragma unroll 8
     for (int64_t b = 0; b < ny; ++b) {
         const uint32_t seg = src[b * lx];
         const int j0 = (int)(seg >> 16), n = (int)(seg & 0xFFFFu);
@@ -635,6 +636,7 @@ __global__ __launch_bounds__(256) void k_seg_sum_y_part_dtw(const uint32_t *__re
     const double yj = (double)ys.row(b)[j];
     const int64_t step = ny * ly;
     float s = 0.0f;
+#pragma unroll 8
     for (int64_t a = a0; a < a1; ++a) {
         const uint32_t seg = cols[a * step + t];
         const int i0 = (int)(seg >> 16), n = (int)(seg & 0xFFFFu);
