@@ -45,15 +45,17 @@ __device__ __forceinline__ double inf_v<double>() {
     return WBX_INF_D;
 }
 
-// Value of lane - 1 (wave_shr:1) / lane + 1 (wave_shl:1); lanes without a
-// source read +inf.
+// Value of lane - 1 (wave_shr:1) / lane + 1 (wave_shl:1).  The lanes
+// without a source (0 and 63) are the first / last lane of a group, whose
+// value every caller replaces by +inf, so float32 keeps v there (no register
+// of +inf written before each shift).
 __device__ __forceinline__ float shr1(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(WBX_INF_F), __float_as_int(v),
-                                                      0x138, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x138,
+                                                      0xF, 0xF, false));
 }
 __device__ __forceinline__ float shl1(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(WBX_INF_F), __float_as_int(v),
-                                                      0x130, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130,
+                                                      0xF, 0xF, false));
 }
 __device__ __forceinline__ double shr1(double v) {
     const int2 b = *reinterpret_cast<const int2 *>(&v);
@@ -231,26 +233,41 @@ __global__ __launch_bounds__(64) void k_group_dtw(PairSpec ps, Series<T> xs, Ser
             xv[u] = ldx(X, L.r + u, lx);
             yn[u] = ldyg(Y, L.yj + W + u, ly);
         }
-        for (int s = rem; s < steady; s += U) {
-            // next group's x and new y entries, issued before this group
-            T xn[U], ynn[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                xn[u] = ldx(X, L.r + U + u, lx);
-                ynn[u] = ldyg(Y, L.yj + W + U + u, ly);
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) gstep<T, R, U, WT, false, false>(L, u, xv[u], lx, w0);
+        // groups in pairs: the second group's x and y entries go to registers
+        // the first does not read, and the next pair's to xv / yn, so no
+        // group copies its successor's loads
+        auto shift_in = [&](const T(&ynew)[U]) {
 #pragma unroll
             for (int m = 0; m < W - U; ++m) L.yw[m] = L.yw[m + U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                L.yw[W - U + u] = yn[u];
-                xv[u] = xn[u];
-                yn[u] = ynn[u];
-            }
+            for (int u = 0; u < U; ++u) L.yw[W - U + u] = ynew[u];
             L.r += U;
             L.yj += U;
+        };
+        int s = rem;
+        for (; s + 2 * U <= steady; s += 2 * U) {
+            T xb[U], yb[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                xb[u] = ldx(X, L.r + U + u, lx);
+                yb[u] = ldyg(Y, L.yj + W + U + u, ly);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) gstep<T, R, U, WT, false, false>(L, u, xv[u], lx, w0);
+            shift_in(yn);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                xv[u] = ldx(X, L.r + U + u, lx);
+                yn[u] = ldyg(Y, L.yj + W + U + u, ly);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) gstep<T, R, U, WT, false, false>(L, u, xb[u], lx, w0);
+            shift_in(yb);
+        }
+        if (s < steady) {  // one group left
+#pragma unroll
+            for (int u = 0; u < U; ++u) gstep<T, R, U, WT, false, false>(L, u, xv[u], lx, w0);
+            shift_in(yn);
         }
     } else {
         steady = 0;
