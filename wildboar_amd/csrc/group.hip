@@ -107,6 +107,51 @@ __device__ __forceinline__ T ldyg(const T *__restrict__ Y, int j, int ly) {
     return (unsigned)j < (unsigned)ly ? v : inf_v<T>();
 }
 
+// Four consecutive elements at p (16 bytes for float32, two 16-byte loads for
+// float64; only 4-byte / 8-byte alignment is assumed).
+__device__ __forceinline__ void ld4g(const float *__restrict__ p, float *v) {
+    typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+    const f4u q = *(const f4u *)p;
+    v[0] = q[0];
+    v[1] = q[1];
+    v[2] = q[2];
+    v[3] = q[3];
+}
+__device__ __forceinline__ void ld4g(const double *__restrict__ p, double *v) {
+    typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
+    const d2u q0 = *(const d2u *)p;
+    const d2u q1 = *(const d2u *)(p + 2);
+    v[0] = q0[0];
+    v[1] = q0[1];
+    v[2] = q1[0];
+    v[3] = q1[1];
+}
+
+// A steady group's U x values (rows xr ..) and U new window entries
+// (columns yc ..).  When every lane of the wave reads inside its rows and
+// columns the loads are plain vector loads; otherwise each value is clamped
+// into the row (x) or replaced by +inf outside [0, ly) (y).
+template <typename T, int U>
+__device__ __forceinline__ void group_loads(const T *__restrict__ X, const T *__restrict__ Y,
+                                            int xr, int yc, int lx, int ly, T (&xo)[U],
+                                            T (&yo)[U]) {
+    static_assert(U % 4 == 0, "groups of four");
+    const bool in = xr >= 0 && xr + U <= lx && yc >= 0 && yc + U <= ly;
+    if (__ballot(!in) == 0) {  // uniform
+#pragma unroll
+        for (int u = 0; u < U; u += 4) {
+            ld4g(X + xr + u, xo + u);
+            ld4g(Y + yc + u, yo + u);
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            xo[u] = ldx(X, xr + u, lx);
+            yo[u] = ldyg(Y, yc + u, ly);
+        }
+    }
+}
+
 // Per-lane state of a group member.
 template <typename T, int R, int U, bool WT>
 struct GLane {
@@ -247,19 +292,11 @@ __global__ __launch_bounds__(64) void k_group_dtw(PairSpec ps, Series<T> xs, Ser
         int s = rem;
         for (; s + 2 * U <= steady; s += 2 * U) {
             T xb[U], yb[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                xb[u] = ldx(X, L.r + U + u, lx);
-                yb[u] = ldyg(Y, L.yj + W + U + u, ly);
-            }
+            group_loads<T, U>(X, Y, L.r + U, L.yj + W + U, lx, ly, xb, yb);
 #pragma unroll
             for (int u = 0; u < U; ++u) gstep<T, R, U, WT, false, false>(L, u, xv[u], lx, w0);
             shift_in(yn);
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                xv[u] = ldx(X, L.r + U + u, lx);
-                yn[u] = ldyg(Y, L.yj + W + U + u, ly);
-            }
+            group_loads<T, U>(X, Y, L.r + U, L.yj + W + U, lx, ly, xv, yn);
 #pragma unroll
             for (int u = 0; u < U; ++u) gstep<T, R, U, WT, false, false>(L, u, xb[u], lx, w0);
             shift_in(yb);
