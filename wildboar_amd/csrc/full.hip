@@ -100,9 +100,16 @@ __device__ __forceinline__ T fldx(const T *__restrict__ X, int i, int lx) {
     return X[i < 0 ? 0 : (i >= lx ? lx - 1 : i)];
 }
 
+// Waves per SIMD the register allocation is held to: float64 lanes of 24
+// columns fit three waves' 168 registers without spills (184 otherwise).
+template <typename T, int S>
+constexpr int full_waves() {
+    return sizeof(T) == 8 && S == 24 ? 3 : 1;
+}
+
 template <typename T, int S, int U>
-__global__ __launch_bounds__(64) void k_full_dtw(PairSpec ps, Series<T> xs, Series<T> ys,
-                                                 FullGeo g, T *__restrict__ sq) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(full_waves<T, S>()))) void
+k_full_dtw(PairSpec ps, Series<T> xs, Series<T> ys, FullGeo g, T *__restrict__ sq) {
     const int lane = threadIdx.x;
     const int gi = lane / g.n;
     const int q = lane - gi * g.n;

@@ -102,13 +102,30 @@ __device__ __forceinline__ void cand_finish(const CandList &cl, const int *s_win
     }
 }
 
+// Four floats at a 4-byte aligned address (one 16-byte load).
+typedef float f4a __attribute__((ext_vector_type(4), aligned(4)));
+
+// Waves per SIMD the register allocation is held to, by band class: left
+// alone the allocator spreads the window-batch set-up over up to 230
+// registers (two or three waves); 13 and 15 fit four waves and 17 .. 23
+// three without spills (wider classes would spill).
+constexpr int subseq_dtw32_waves(int BP) {
+    return BP <= 11 ? 1 : BP <= 15 ? 4 : BP <= 23 ? 3 : 1;
+}
+
 template <int BP, int U>
-__global__ __launch_bounds__(64) void k_subseq_dtw32(TaskMap tasks, SubGeo sg,
-                                                     SubCand *__restrict__ cands) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(subseq_dtw32_waves(BP)))) void
+k_subseq_dtw32(TaskMap tasks, SubGeo sg, SubCand *__restrict__ cands) {
+    static_assert(U % 4 == 0, "groups of whole 16-byte loads");
     constexpr int W = BP + U - 1;
     const int lane = threadIdx.x;
     const SubTask tk = tasks.at(blockIdx.x);
+    // the shapelet is the same for every lane and is not written by the
+    // launch: read through the constant address space, its values are scalar
+    // loads into SGPRs
     const float *__restrict__ S = sg.S + sg.off[tk.shapelet];
+    const __attribute__((address_space(4))) float *Sc =
+        (const __attribute__((address_space(4))) float *)S;
     const int L = (int)sg.len[tk.shapelet];
     const int off = band_w(L, L, sg.r) - 1;  // the band has 2 w - 1 == BP slots
     const int nwin = sg.T - L + 1;
@@ -126,8 +143,28 @@ __global__ __launch_bounds__(64) void k_subseq_dtw32(TaskMap tasks, SubGeo sg,
 #pragma unroll
         for (int k = 0; k < BP; ++k) row[k] = (k == off) ? 0.0f : WBX_INF_F;
         float yw[W];
+        if (b0 >= off && W - off <= L) {
+            // uniform: every lane's columns -off .. W - off - 1 are inside the
+            // series row (ws >= off) and end inside the window; the first off
+            // are left of it (+inf)
+            const float *__restrict__ Yb = Y - off;
 #pragma unroll
-        for (int m = 0; m < W; ++m) yw[m] = ldy(Y, m - off, L);
+            for (int q = 0; q < W; q += 4) {
+                if (q + 4 <= W) {
+                    const f4a v = *(const f4a *)(Yb + q);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) yw[q + u] = v[u];
+                } else {
+#pragma unroll
+                    for (int u = 0; q + u < W; ++u) yw[q + u] = Yb[q + u];
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < W; ++m) yw[m] = m < off ? WBX_INF_F : yw[m];
+        } else {
+#pragma unroll
+            for (int m = 0; m < W; ++m) yw[m] = ldy(Y, m - off, L);
+        }
         const int rem = L % U;
         for (int i = 0; i < rem; ++i) {
             const float xi = S[i];
@@ -146,12 +183,28 @@ __global__ __launch_bounds__(64) void k_subseq_dtw32(TaskMap tasks, SubGeo sg,
             yw[W - 1] = ldy(Y, i + 1 - off + W - 1, L);
         }
         for (int i0 = rem; i0 < L; i0 += U) {
+            // This group's new window entries, columns jn .. jn + U - 1
+            // (jn >= 0).  Every lane has the same shapelet and row, so when
+            // they lie inside the window (the usual case) they are plain
+            // vector loads, waited for only at the window shift.  The
+            // shapelet values are scalar loads (the shapelet stays in the
+            // scalar cache).
             float yn[U];
+            const int jn = i0 - off + W;
+            if (jn + U <= L) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) yn[u] = ldy(Y, i0 + U - off + W - U + u, L);
+                for (int u = 0; u < U; u += 4) {
+                    const f4a b = *(const f4a *)(Y + jn + u);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) yn[u + q] = b[q];
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; ++u) yn[u] = ldy(Y, jn + u, L);
+            }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const float xi = S[i0 + u];
+                const float xi = Sc[i0 + u];
                 float left = WBX_INF_F;
 #pragma unroll
                 for (int k = 0; k < BP; ++k) {
