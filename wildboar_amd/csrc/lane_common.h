@@ -17,6 +17,9 @@ bool lane_exact_supported(int B);
 int launch_lane_exact(const PairSpec &ps, Series<float> xs, Series<float> ys, const LaneGeo &g,
                       float *sq, hipStream_t st);
 
+// Four floats at a 4-byte aligned address (one 16-byte load).
+typedef float f4a __attribute__((ext_vector_type(4), aligned(4)));
+
 // y[j] for j in [0, ly), +inf outside (the cells there become +inf).
 __device__ __forceinline__ float ldy(const float *__restrict__ Y, int j, int ly) {
     const int jc = j < 0 ? 0 : (j >= ly ? ly - 1 : j);

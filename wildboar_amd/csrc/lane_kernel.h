@@ -130,24 +130,79 @@ __device__ __forceinline__ void lane_dtw_body(PairSpec ps, Series<float> xs,
         for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
         yw[W - 1] = ldy(Y, i + 1 - off + W - 1, ly);
     }
-    for (int i0 = i_begin + rem; i0 < lx; i0 += U) {
-        float xv[U], yn[U];
+    if constexpr (BP >= 112 || (WT && BP > 40) || (AD && BP >= 80)) {
+        // the widest class, weighted bands past 40 slots and amerced ones
+        // from 80 (their weights and penalties take registers too) have none
+        // left for the next group's x: each group's loads are issued at its
+        // start
+        for (int i0 = i_begin + rem; i0 < lx; i0 += U) {
+            float xv[U], yn[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            xv[u] = X[i0 + u];
-            yn[u] = ldy(Y, i0 + U - off + W - U + u, ly);
+            for (int u = 0; u < U; ++u) {
+                xv[u] = X[i0 + u];
+                yn[u] = ldy(Y, i0 + U - off + W - U + u, ly);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if constexpr (WT) asm volatile("" ::: "memory");
+                lane_row<BP, AD, WT, EX>(row, yw + u, xv[u], pen, pen_ad,
+                                         WT ? lw[WT ? 1 : 0] : nullptr);
+            }
+#pragma unroll
+            for (int m = 0; m < W - U; ++m) yw[m] = yw[m + U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) yw[W - U + u] = yn[u];
         }
+    } else {
+        // x of the current group of U rows; the loop loads the next group's
+        // with the group's new window entries (columns jn .. jn + U - 1,
+        // jn > 0), so nothing a row reads waits on a load issued in the same
+        // group.  The lanes share the geometry: when both lie inside their
+        // rows (every group but the last one or two) they are plain 16-byte
+        // loads, else x is clamped into the row (never used past it) and y is
+        // +inf outside it.
+        static_assert(U % 4 == 0, "groups of whole 16-byte loads");
+        float xv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            // the weights are re-read from LDS every row rather than held in
-            // registers across the loop (the band row and y window fill them)
-            if constexpr (WT) asm volatile("" ::: "memory");
-            lane_row<BP, AD, WT, EX>(row, yw + u, xv[u], pen, pen_ad, WT ? lw[WT ? 1 : 0] : nullptr);
+        for (int u = 0; u < U; ++u) xv[u] = X[min(i_begin + rem + u, lx - 1)];
+        for (int i0 = i_begin + rem; i0 < lx; i0 += U) {
+            float xn[U], yn[U];
+            const int jn = i0 - off + W;
+            if (i0 + 2 * U <= lx && jn + U <= ly) {
+#pragma unroll
+                for (int u = 0; u < U; u += 4) {
+                    const f4a a = *(const f4a *)(X + i0 + U + u);
+                    const f4a b = *(const f4a *)(Y + jn + u);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        xn[u + q] = a[q];
+                        yn[u + q] = b[q];
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    xn[u] = X[min(i0 + U + u, lx - 1)];
+                    yn[u] = ldy(Y, jn + u, ly);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                // the weights are re-read from LDS every row rather than held
+                // in registers across the loop (the band row and y window
+                // fill them)
+                if constexpr (WT) asm volatile("" ::: "memory");
+                lane_row<BP, AD, WT, EX>(row, yw + u, xv[u], pen, pen_ad,
+                                         WT ? lw[WT ? 1 : 0] : nullptr);
+            }
+#pragma unroll
+            for (int m = 0; m < W - U; ++m) yw[m] = yw[m + U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                yw[W - U + u] = yn[u];
+                xv[u] = xn[u];
+            }
         }
-#pragma unroll
-        for (int m = 0; m < W - U; ++m) yw[m] = yw[m + U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) yw[W - U + u] = yn[u];
     }
 
     float res = WBX_INF_F;
@@ -157,11 +212,13 @@ __device__ __forceinline__ void lane_dtw_body(PairSpec ps, Series<float> xs,
     if (p0 < ps.n_pairs) sq[p0] = res;
 }
 
+// Plain DTW on padded bands is held to two waves per SIMD like the exact
+// kernels (at 104 and 112 slots the allocator would take one wave's worth);
+// the weighted and amerced variants need more registers than two waves have.
 template <int BP, int U, bool AD = false, bool WT = false>
-__global__ __launch_bounds__(64) void k_lane_dtw(PairSpec ps, Series<float> xs, Series<float> ys,
-                                                 LaneGeo g, float pen_ad,
-                                                 const float *__restrict__ weights,
-                                                 float *__restrict__ sq) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AD || WT ? 1 : 2))) void
+k_lane_dtw(PairSpec ps, Series<float> xs, Series<float> ys, LaneGeo g, float pen_ad,
+           const float *__restrict__ weights, float *__restrict__ sq) {
     lane_dtw_body<BP, U, AD, WT, false>(ps, xs, ys, g, pen_ad, weights, sq);
 }
 

@@ -102,9 +102,6 @@ __device__ __forceinline__ void cand_finish(const CandList &cl, const int *s_win
     }
 }
 
-// Four floats at a 4-byte aligned address (one 16-byte load).
-typedef float f4a __attribute__((ext_vector_type(4), aligned(4)));
-
 // Waves per SIMD the register allocation is held to, by band class: left
 // alone the allocator spreads the window-batch set-up over up to 230
 // registers (two or three waves); 13 and 15 fit four waves and 17 .. 23
