@@ -89,7 +89,22 @@ __device__ __forceinline__ void row64(double (&row)[BP], const double *yw, doubl
 
 // Squared DTW cost of X[0, lx) against Y[0, ly) with band offset `off`,
 // width B <= BP, final slot kf.  weights: wn >= max(lx, ly) values, all > 0.
-template <int BP, int U, bool WT, bool AD = false>
+// Whether a kernel's register class loads the next group's x and window
+// entries during the current group: the classes where the extra registers
+// cost no wave per SIMD and no spills (the others issue each group's loads at
+// its start).  kind 0: pairs, 1: window profiles, 2: subsequence minimum.
+template <int KIND, int BP, bool WT, bool AD>
+constexpr bool lane64_prefetch() {
+    if constexpr (KIND == 0)
+        return WT ? (BP == 16 || BP == 32 || BP == 48)
+                  : AD ? (BP >= 24 && BP <= 40) : (BP == 16 || (BP >= 32 && BP <= 56) || BP == 72);
+    else if constexpr (KIND == 1)
+        return BP <= (WT ? 56 : 48);
+    else
+        return WT || AD ? BP <= 40 : (BP == 8 || (BP >= 32 && BP <= 48));
+}
+
+template <int BP, int U, bool WT, bool AD = false, bool PF = false>
 __device__ __forceinline__ double lane64_cost(const double *__restrict__ X, int lx,
                                               const double *__restrict__ Y, int ly, int off,
                                               int B, int kf,
@@ -134,19 +149,63 @@ __device__ __forceinline__ double lane64_cost(const double *__restrict__ X, int 
         for (int m = 0; m < W - 1; ++m) yw[m] = yw[m + 1];
         yw[W - 1] = ldy64(Y, i + 1 - off + W - 1, ly);
     }
-    for (int i0 = 1 + rem; i0 < lx; i0 += U) {
-        double xv[U], yn[U];
+    if constexpr (!PF) {
+        for (int i0 = 1 + rem; i0 < lx; i0 += U) {
+            double xv[U], yn[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            xv[u] = X[i0 + u];
-            yn[u] = ldy64(Y, i0 + U - off + W - U + u, ly);
+            for (int u = 0; u < U; ++u) {
+                xv[u] = X[i0 + u];
+                yn[u] = ldy64(Y, i0 + U - off + W - U + u, ly);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) row64<BP, WT, AD>(row, yw + u, xv[u], pen, wk, p);
+#pragma unroll
+            for (int m = 0; m < W - U; ++m) yw[m] = yw[m + U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) yw[W - U + u] = yn[u];
         }
+    } else {
+        // x of the current group of U rows; the loop loads the next group's
+        // with the group's new window entries (columns jn .. jn + U - 1,
+        // jn > 0), so no row waits on a load issued in the same group.  The
+        // lanes share lx, ly and the band: when both lie inside (every group
+        // but the last one or two) they are plain 16-byte loads, else x is
+        // clamped into its row (never used past it) and y is +inf outside.
+        static_assert(U % 2 == 0, "groups of whole 16-byte loads");
+        typedef double d2a __attribute__((ext_vector_type(2), aligned(8)));
+        double xv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) row64<BP, WT, AD>(row, yw + u, xv[u], pen, wk, p);
+        for (int u = 0; u < U; ++u) xv[u] = X[min(1 + rem + u, lx - 1)];
+        for (int i0 = 1 + rem; i0 < lx; i0 += U) {
+            double xn[U], yn[U];
+            const int jn = i0 - off + W;
+            if (i0 + 2 * U <= lx && jn + U <= ly) {
 #pragma unroll
-        for (int m = 0; m < W - U; ++m) yw[m] = yw[m + U];
+                for (int u = 0; u < U; u += 2) {
+                    const d2a a = *(const d2a *)(X + i0 + U + u);
+                    const d2a b = *(const d2a *)(Y + jn + u);
+                    xn[u] = a[0];
+                    xn[u + 1] = a[1];
+                    yn[u] = b[0];
+                    yn[u + 1] = b[1];
+                }
+            } else {
 #pragma unroll
-        for (int u = 0; u < U; ++u) yw[W - U + u] = yn[u];
+                for (int u = 0; u < U; ++u) {
+                    xn[u] = X[min(i0 + U + u, lx - 1)];
+                    yn[u] = ldy64(Y, jn + u, ly);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) row64<BP, WT, AD>(row, yw + u, xv[u], pen, wk, p);
+#pragma unroll
+            for (int m = 0; m < W - U; ++m) yw[m] = yw[m + U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                yw[W - U + u] = yn[u];
+                xv[u] = xn[u];
+            }
+        }
     }
     double res = WBX_INF_D;
 #pragma unroll
@@ -164,8 +223,8 @@ __global__ __launch_bounds__(64) void k_lane64_pairs(PairSpec ps, Series<double>
     const int64_t p = p0 < ps.n_pairs ? p0 : ps.n_pairs - 1;
     int64_t a, b;
     decode_pair(ps, p, a, b);
-    const double c = lane64_cost<BP, U, WT, AD>(xs.row(a), xs.len, ys.row(b), ys.len, off, B,
-                                                kf, weights, wn, pen);
+    const double c = lane64_cost<BP, U, WT, AD, lane64_prefetch<0, BP, WT, AD>()>(
+        xs.row(a), xs.len, ys.row(b), ys.len, off, B, kf, weights, wn, pen);
     if (p0 < ps.n_pairs) sq[p0] = c;
 }
 
@@ -190,7 +249,8 @@ __global__ __launch_bounds__(64) void k_profile_lane64(Shapelets sh, int paired,
     const int s = b * 64 + threadIdx.x;
     const int sc = s < nwin ? s : nwin - 1;
     const int off = (B - 1) / 2;
-    const double c = lane64_cost<BP, U, WT>(S, L, xs.row(q) + sc, L, off, B, off, weights, t);
+    const double c = lane64_cost<BP, U, WT, false, lane64_prefetch<1, BP, WT, false>()>(
+        S, L, xs.row(q) + sc, L, off, B, off, weights, t);
     if (s < nwp) prof[q * nwp + s] = s < nwin ? c : WBX_INF_D;
 }
 
@@ -224,7 +284,8 @@ __global__ __launch_bounds__(64) void k_subseq_lane64(Shapelets sh, Series<doubl
     for (int b0 = 0; b0 < nwin; b0 += 64) {
         const int s = b0 + threadIdx.x;
         const int sc = s < nwin ? s : nwin - 1;
-        const double c = lane64_cost<BP, U, WT, AD>(S, L, x + sc, L, off, B, off, weights, t, pen);
+        const double c = lane64_cost<BP, U, WT, AD, lane64_prefetch<2, BP, WT, AD>()>(
+            S, L, x + sc, L, off, B, off, weights, t, pen);
         if (s < nwin && c < best) {
             best = c;
             best_s = s;
