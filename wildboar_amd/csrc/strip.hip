@@ -386,10 +386,18 @@ __device__ __forceinline__ void strip_pair(const StripItem &it, const StripArray
 #define WBX_STRIP_R64_T0(X) \
     X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(10) X(12) X(14) X(16) X(18) X(20) X(24) X(28) X(32)
 
+// Waves per SIMD the register allocation is held to: the float64 strips fit
+// two waves' 256 registers without spills (the allocator takes 285, one wave,
+// otherwise); the float32 ones sit at three waves by themselves.
+template <typename T>
+constexpr int strip_waves() {
+    return sizeof(T) == 8 ? 2 : 1;
+}
+
 template <typename T, int TIER>
-__global__ __launch_bounds__(64) void k_strip_batch(const StripItem *__restrict__ items,
-                                                    int n_items, int *__restrict__ counter,
-                                                    StripArrays<T> ar, T *__restrict__ sq) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(strip_waves<T>()))) void
+k_strip_batch(const StripItem *__restrict__ items, int n_items, int *__restrict__ counter,
+              StripArrays<T> ar, T *__restrict__ sq) {
     const int p = threadIdx.x;
     while (true) {
         int k = 0;
