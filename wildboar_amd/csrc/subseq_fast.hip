@@ -25,6 +25,7 @@
 
 #include "block_dtw.h"
 #include "lane_common.h"
+#include "lane64_cost.h"
 #include "subseq_fast.h"
 
 namespace wbx {
@@ -352,34 +353,11 @@ __global__ __launch_bounds__(64) void k_subseq_cand64(TaskMap tasks,
         }
         res = dist;
     } else {
+        // the band row in registers with a sliding window of the candidate's
+        // values (the float64 lane kernels' cell arithmetic, bit for bit)
         const int w = band_w(L, L, r);
-        const int off = w - 1, B = 2 * w - 1;
-        double row[BP];
-#pragma unroll
-        for (int k = 0; k < BP; ++k) row[k] = (k == off) ? 0.0 : WBX_INF_D;
-        for (int i = 0; i < L; ++i) {
-            const double xi = S[i];
-            double left = WBX_INF_D;
-#pragma unroll
-            for (int k = 0; k < BP; ++k) {
-                const double up = (k + 1 < BP) ? row[k + 1] : WBX_INF_D;
-                const int j = i + k - off;
-                const double yj = ((unsigned)j < (unsigned)L) ? Y[j < 0 ? 0 : (j >= L ? L - 1 : j)]
-                                                              : WBX_INF_D;
-                const double v = xi - yj;
-                const double dg = row[k];
-                const double mn = (left < up ? left : up);
-                const double m = (dg < mn ? dg : mn);
-                double c = m + v * v;
-                if (k >= B) c = WBX_INF_D;
-                row[k] = c;
-                left = c;
-            }
-        }
-        res = WBX_INF_D;
-#pragma unroll
-        for (int k = 0; k < BP; ++k)
-            if (k == off) res = row[k];
+        res = lane64_cost<BP, 4, false, false, true>(S, L, Y, L, w - 1, 2 * w - 1, w - 1, nullptr,
+                                                    L);
     }
     cost64[q] = res;
 }
