@@ -155,6 +155,24 @@ def test_pairwise_fp32_every_exact_band_width():
             assert_fp32_close(got, oracle.pairwise(X, Y2, metric="dtw", r=r))
 
 
+def test_pairwise_fp32_every_padded_band_width():
+    # Lengths 60 / 61 give even band widths B = 2w, padded to a multiple of 8
+    # with a reset slot: every padded class 8 .. 112 (the widest keeps its
+    # group's loads at the group's start, the others load the next group's
+    # during the current one), and the amerced variant on both sides of 80.
+    X = rw(36, 10, 60)
+    Y = rw(37, 12, 61)
+    for w in range(4, 57):
+        r = (w + 0.5) / 60.0
+        got = wd.pairwise_distance(X, Y, metric="dtw", metric_params={"r": r}, precision="fp32")
+        assert_fp32_close(got, oracle.pairwise(X, Y, metric="dtw", r=r))
+    for w in (12, 30, 39, 45, 56):
+        r = (w + 0.5) / 60.0
+        got = wd.pairwise_distance(X, Y, metric="adtw", metric_params={"r": r, "p": 0.5},
+                                   precision="fp32")
+        assert_fp32_close(got, oracle.pairwise(X, Y, metric="adtw", r=r, p=0.5))
+
+
 @pytest.mark.parametrize("lx,ly,r", [(100, 130, 0.1), (130, 100, 0.1), (37, 80, 0.2),
                                       (5, 3, 1.0), (1, 9, 0.5)])
 def test_pairwise_fp32_unequal(lx, ly, r):
