@@ -173,6 +173,24 @@ def test_pairwise_fp32_every_padded_band_width():
         assert_fp32_close(got, oracle.pairwise(X, Y, metric="adtw", r=r, p=0.5))
 
 
+def test_pairwise_fp64_lane_classes_bitwise():
+    # float64 lanes: DTW up to 40 slots at equal and unequal lengths, ADTW in
+    # every class to 104 slots (the classes that load the next group during
+    # the current one and those that do not); the reference's arithmetic, so
+    # bit for bit.
+    X = rw(46, 8, 60)
+    for ly in (60, 61):
+        Y = rw(47, 9, ly)
+        for w in range(2, 21):
+            r = (w + 0.5) / 60.0
+            got = wd.pairwise_distance(X, Y, metric="dtw", metric_params={"r": r})
+            np.testing.assert_array_equal(got, oracle.pairwise(X, Y, metric="dtw", r=r))
+        for w in (6, 12, 16, 20, 26, 30, 36, 44, 52):
+            r = (w + 0.5) / 60.0
+            got = wd.pairwise_distance(X, Y, metric="adtw", metric_params={"r": r, "p": 0.5})
+            np.testing.assert_array_equal(got, oracle.pairwise(X, Y, metric="adtw", r=r, p=0.5))
+
+
 @pytest.mark.parametrize("lx,ly,r", [(100, 130, 0.1), (130, 100, 0.1), (37, 80, 0.2),
                                       (5, 3, 1.0), (1, 9, 0.5)])
 def test_pairwise_fp32_unequal(lx, ly, r):
