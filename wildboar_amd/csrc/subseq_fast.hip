@@ -354,7 +354,9 @@ __global__ __launch_bounds__(64) void k_subseq_cand64(TaskMap tasks,
         res = dist;
     } else {
         // the band row in registers with a sliding window of the candidate's
-        // values (the float64 lane kernels' cell arithmetic, bit for bit)
+        // values (the float64 lane kernels' cell arithmetic, bit for bit);
+        // a launch holds one band class, B in (BP - 8, BP), whose padding
+        // slot B the routine resets every row
         const int w = band_w(L, L, r);
         res = lane64_cost<BP, 4, false, false, true>(S, L, Y, L, w - 1, 2 * w - 1, w - 1, nullptr,
                                                     L);
