@@ -268,23 +268,32 @@ __global__ __launch_bounds__(64) void k_lane_backtrack(PairSpec ps, LaneGeo g,
     const double coefd = (active && D > 0.0) ? G[p] / D : 0.0;
     const float coef = (float)coefd;
     // Columns are finished in decreasing order, each exactly once: their
-    // terms go through a CW-entry shift window and out as one vector store per
-    // CW columns (columns >= ly of the top window count as finished zeros).
+    // terms go to the lane's CW-column window in LDS (slot col % CW) and out
+    // as one vector store per CW columns when column col % CW == 0 completes
+    // it (columns >= ly of the top window stay the zeros written first).  A
+    // column costs two LDS writes, not a shift of two register windows, in a
+    // loop every lane of the wave runs as often as its busiest lane.
     constexpr int CW = 8;
     typedef float fcw __attribute__((ext_vector_type(CW)));
+    __shared__ float wbuf[2][CW][64];  // u, v of column slot e, lane
+#pragma unroll
+    for (int e = 0; e < CW; ++e) wbuf[0][e][lane] = wbuf[1][e][lane] = 0.0f;
     fcw *__restrict__ gww = (fcw *)(go.gw + p * go.lyp);
     const fcw *__restrict__ ypw = (const fcw *)(go.YP + b * go.lyp);
-    fcw wu_ = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    fcw wv_ = wu_;
     fcw yv_ = ypw[(ly - 1) / CW];  // y of the first window to complete
-    int fill = (CW - ly % CW) % CW;
     auto finish = [&](int col, float u, float v) {
-        wu_ = fcw{u, wu_[0], wu_[1], wu_[2], wu_[3], wu_[4], wu_[5], wu_[6]};
-        wv_ = fcw{v, wv_[0], wv_[1], wv_[2], wv_[3], wv_[4], wv_[5], wv_[6]};
-        if (++fill == CW) {  // col is a multiple of CW: columns col .. col + CW - 1
-            if (active) gww[col / CW] = wu_ * yv_ - wv_;
+        const int e = col & (CW - 1);
+        wbuf[0][e][lane] = u;
+        wbuf[1][e][lane] = v;
+        if (e == 0) {  // columns col .. col + CW - 1 are complete
+            fcw wu, wv;
+#pragma unroll
+            for (int q = 0; q < CW; ++q) {
+                wu[q] = wbuf[0][q][lane];
+                wv[q] = wbuf[1][q][lane];
+            }
+            if (active) gww[col / CW] = wu * yv_ - wv;
             yv_ = ypw[col >= CW ? col / CW - 1 : 0];  // the next window down
-            fill = 0;
         }
     };
 
