@@ -300,7 +300,9 @@ __global__ __launch_bounds__(64) void k_lane_backtrack(PairSpec ps, LaneGeo g,
     // Rows are taken RB at a time: first the path through the RB rows (bits
     // only, already in registers) while every x and prefix load those rows need
     // is issued, then the sums; the next RB rows' bits are fetched in between.
-    constexpr int RB = 5;
+    // RB is as many rows as keep the kernel within 128 registers (four waves
+    // per SIMD): more rows per batch are fewer memory round trips per pair.
+    constexpr int RB = BP <= 32 ? 7 : BP <= 64 ? 6 : BP <= 96 ? 5 : 4;
     uint32_t wd[RB][NW], wu[RB][NW];
     auto load_bits = [&](int ih) {
 #pragma unroll
