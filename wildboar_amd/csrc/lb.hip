@@ -5,7 +5,8 @@
 //   (distance/_elastic.pyx:98-150, used by _dtw_envelop :1076-1092).  Min and
 //   max are exact, so the window scan gives the same values.
 // k_lb_keogh: out[i, j] = sqrt(sum_t d_t) with d_t = (x - u)^2 above the
-//   envelope, (x - l)^2 below it, 0 inside; summed in t order
+//   envelope, (x - l)^2 below it, 0 inside; summed in t order, 64 x 64 pairs
+//   per block over LDS tiles
 //   (cumulative_bound, _elastic.pyx:229-260, with mean 0 and std 1, as
 //   _dtw_lb_keogh calls it :1095-1116).
 // k_lb_kim: DtwKimLowerBound.transform (distance/lb.py:206-312): squared
@@ -45,29 +46,82 @@ __device__ __forceinline__ double sq(double a, double b) {
     return s * s;
 }
 
+// 64 x 64 pairs per block: the 64 query rows and the 64 envelopes' upper and
+// lower rows staged in LDS in chunks of 16 timesteps, 4 x 4 pairs per thread
+// (rows ty + 16 i, envelopes tx + 16 j).  Each pair's sum runs over t in order
+// with the same terms as one thread per pair (a query read by every lane of a
+// wave, each envelope row by one lane, was 64 cache lines per step).
+constexpr int kLbB = 64, kLbK = 16;
+
 __global__ __launch_bounds__(256) void k_lb_keogh(const double *__restrict__ X, int64_t nx,
                                                   int64_t x_stride,
                                                   const double *__restrict__ lower,
                                                   const double *__restrict__ upper,
                                                   int64_t ny, int t,
                                                   double *__restrict__ out) {
-    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (g >= nx * ny) return;
-    const int64_t i = g / ny, j = g - i * ny;
-    const double *__restrict__ x = X + i * x_stride;
-    const double *__restrict__ lo = lower + j * t;
-    const double *__restrict__ up = upper + j * t;
-    double m = 0.0;
-    for (int q = 0; q < t; ++q) {
-        const double v = x[q];
-        double d = 0.0;
-        if (v > up[q])
-            d = sq(v, up[q]);
-        else if (v < lo[q])
-            d = sq(v, lo[q]);
-        m += d;
+    __shared__ double sx[kLbB][kLbK + 1];
+    __shared__ double su[kLbB][kLbK + 1];
+    __shared__ double sl[kLbB][kLbK + 1];
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const int64_t a0 = (int64_t)blockIdx.y * kLbB, b0 = (int64_t)blockIdx.x * kLbB;
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    constexpr int PER = kLbB * kLbK / 256;  // values of each tile per thread
+    for (int c0 = 0; c0 < t; c0 += kLbK) {
+        const int cn = t - c0 < kLbK ? t - c0 : kLbK;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int q = threadIdx.x + 256 * u;
+            const int rr = q / kLbK, cc = q % kLbK;
+            double vx = 0.0, vu = 0.0, vl = 0.0;
+            if (cc < cn) {
+                if (a0 + rr < nx) vx = X[(a0 + rr) * x_stride + c0 + cc];
+                if (b0 + rr < ny) {
+                    vu = upper[(b0 + rr) * t + c0 + cc];
+                    vl = lower[(b0 + rr) * t + c0 + cc];
+                }
+            }
+            sx[rr][cc] = vx;
+            su[rr][cc] = vu;
+            sl[rr][cc] = vl;
+        }
+        __syncthreads();
+        for (int cc = 0; cc < cn; ++cc) {
+            double xv[4], uv[4], lv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                xv[i] = sx[ty + 16 * i][cc];
+                uv[i] = su[tx + 16 * i][cc];
+                lv[i] = sl[tx + 16 * i][cc];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const double v = xv[i];
+                    double d = 0.0;
+                    if (v > uv[j])
+                        d = sq(v, uv[j]);
+                    else if (v < lv[j])
+                        d = sq(v, lv[j]);
+                    acc[i][j] += d;
+                }
+        }
+        __syncthreads();
     }
-    out[g] = sqrt(m);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t a = a0 + ty + 16 * i;
+        if (a >= nx) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t b = b0 + tx + 16 * j;
+            if (b < ny) out[a * ny + b] = sqrt(acc[i][j]);
+        }
+    }
 }
 
 __device__ __forceinline__ double dmin(double a, double b) { return b < a ? b : a; }
@@ -115,9 +169,10 @@ int launch_envelope(const double *X, int64_t n, int64_t stride, int t, int r, do
 int launch_lb_keogh(const double *X, int64_t nx, int64_t x_stride, const double *lower,
                     const double *upper, int64_t ny, int t, double *out, hipStream_t st) {
     if (nx * ny <= 0) return WBX_OK;
-    if ((nx * ny + 255) / 256 > 0x7fffffff) return fail(WBX_EUNSUPPORTED, "LB_Keogh too large");
-    hipLaunchKernelGGL(k_lb_keogh, grid_for(nx * ny), dim3(256), 0, st, X, nx, x_stride, lower,
-                       upper, ny, t, out);
+    const int64_t gx = (ny + kLbB - 1) / kLbB, gy = (nx + kLbB - 1) / kLbB;
+    if (gx > 0x7fffffff || gy > 65535) return fail(WBX_EUNSUPPORTED, "LB_Keogh too large");
+    hipLaunchKernelGGL(k_lb_keogh, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, X, nx,
+                       x_stride, lower, upper, ny, t, out);
     WBX_HIP_CHECK(hipGetLastError());
     return WBX_OK;
 }
