@@ -36,31 +36,52 @@ __device__ __forceinline__ void push_pair(int32_t *pa, int32_t *pb, unsigned lon
     pb[q] = (int32_t)j;
 }
 
-// One thread per row: the first k pairs and the kSeed pairs of smallest bound
-// (bound < +inf; ties by index).
+// Lexicographic minimum of (v, j) over the 64 lanes (every lane gets it).
+__device__ __forceinline__ void wave_min_vj(double &v, int &j) {
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        const double ov = __shfl_xor(v, m);
+        const int oj = __shfl_xor(j, m);
+        if (ov < v || (ov == v && oj < j)) {
+            v = ov;
+            j = oj;
+        }
+    }
+}
+
+// One wave per row: the first k pairs (bound < +inf) and the kSeed pairs of
+// smallest bound among j >= k (ties by index).  Each lane keeps the kSeed
+// smallest of its columns k + lane, k + lane + 64, ... (sorted insertion in
+// increasing j, so an equal value never displaces an earlier one); kSeed
+// rounds of a wave-wide (value, index) minimum then take the row's kSeed
+// smallest -- the set a scan of the row in index order keeps.
 __global__ __launch_bounds__(64) void k_prune_seed(const double *__restrict__ lb, int64_t nx,
                                                    int64_t ny, int64_t k, int32_t *__restrict__ pa,
                                                    int32_t *__restrict__ pb,
                                                    unsigned long long *__restrict__ cnt) {
-    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    const int64_t i = blockIdx.x;
     if (i >= nx) return;
+    const int lane = threadIdx.x;
     const double *row = lb + i * ny;
+    for (int64_t j = lane; j < k && j < ny; j += 64)
+        if (row[j] < WBX_INF_D) push_pair(pa, pb, cnt, i, j);
+    constexpr int kNone = 0x7fffffff;
     double sv[kSeed];
-    int64_t sj[kSeed];
+    int sj[kSeed];
 #pragma unroll
     for (int q = 0; q < kSeed; ++q) {
         sv[q] = WBX_INF_D;
-        sj[q] = -1;
+        sj[q] = kNone;
     }
-    for (int64_t j = k; j < ny; ++j) {
+    for (int64_t j = k + lane; j < ny; j += 64) {
         double v = row[j];
-        int64_t jj = j;
+        int jj = (int)j;
         if (!(v < sv[kSeed - 1])) continue;
 #pragma unroll
         for (int q = 0; q < kSeed; ++q) {  // sorted insertion
             if (v < sv[q]) {
                 const double tv = sv[q];
-                const int64_t tj = sj[q];
+                const int tj = sj[q];
                 sv[q] = v;
                 sj[q] = jj;
                 v = tv;
@@ -68,36 +89,34 @@ __global__ __launch_bounds__(64) void k_prune_seed(const double *__restrict__ lb
             }
         }
     }
-    for (int64_t j = 0; j < k; ++j)
-        if (row[j] < WBX_INF_D) push_pair(pa, pb, cnt, i, j);
+    for (int r = 0; r < kSeed; ++r) {
+        double mv = sv[0];
+        int mj = sj[0];
+        wave_min_vj(mv, mj);
+        if (mj == kNone) break;  // uniform: no lane has a finite bound left
+        if (lane == 0) push_pair(pa, pb, cnt, i, mj);
+        if (sj[0] == mj) {  // the lane holding it pops its head
 #pragma unroll
-    for (int q = 0; q < kSeed; ++q)
-        if (sj[q] >= 0) push_pair(pa, pb, cnt, i, sj[q]);
+            for (int q = 0; q + 1 < kSeed; ++q) {
+                sv[q] = sv[q + 1];
+                sj[q] = sj[q + 1];
+            }
+            sv[kSeed - 1] = WBX_INF_D;
+            sj[kSeed - 1] = kNone;
+        }
+    }
 }
 
-// One thread per row: with D holding the known distances (NaN elsewhere),
-// every pair with lb[j] < U(j) whose distance is not known is listed.
-__global__ __launch_bounds__(64) void k_prune_scan(const double *__restrict__ D,
-                                                   const double *__restrict__ lb, int64_t nx,
-                                                   int64_t ny, int k, int32_t *__restrict__ pa,
-                                                   int32_t *__restrict__ pb,
-                                                   unsigned long long *__restrict__ cnt) {
-    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (i >= nx) return;
-    const double *drow = D + i * ny;
-    const double *lrow = lb + i * ny;
-    // the bound must hold on the known pairs; else every pair with a finite
-    // bound is listed (the replay is then exact for any bound)
-    bool holds = true;
-    for (int64_t j = 0; j < ny; ++j) {
-        const double v = drow[j];
-        if (v == v && lrow[j] > v) holds = false;
-    }
-    double top[kKMax];  // the k smallest known values so far, ascending
-#pragma unroll
-    for (int q = 0; q < kKMax; ++q) top[q] = WBX_INF_D;
-    double U = WBX_INF_D;
-    for (int64_t j = 0; j < ny; ++j) {
+// Row i's scan from column j0 to j1 with the running state (top, U): every
+// pair with lb[j] < U(j) whose distance is not known is listed; a known value
+// below U enters the k smallest.
+__device__ __forceinline__ void scan_span(const double *__restrict__ drow,
+                                          const double *__restrict__ lrow, int64_t i, int64_t j0,
+                                          int64_t j1, int k, bool holds, double (&top)[kKMax],
+                                          double &U, int32_t *__restrict__ pa,
+                                          int32_t *__restrict__ pb,
+                                          unsigned long long *__restrict__ cnt) {
+    for (int64_t j = j0; j < j1; ++j) {
         const double v = drow[j];
         const bool known = v == v;
         if (!known && lrow[j] < (holds ? U : WBX_INF_D)) push_pair(pa, pb, cnt, i, j);
@@ -118,6 +137,83 @@ __global__ __launch_bounds__(64) void k_prune_scan(const double *__restrict__ D,
     }
 }
 
+constexpr int kKnownCap = 512;  // known distances of a row gathered in LDS
+
+// One wave per row: with D holding the known distances (NaN elsewhere),
+// every pair with lb[j] < U(j) whose distance is not known is listed, U(j)
+// the k-th smallest known value before j.  The bound must hold on the known
+// pairs, else every pair with a finite bound is listed (the replay is then
+// exact for any bound).  Lane l scans the columns of its chunk in order,
+// starting from the state after the known values before the chunk (gathered
+// in index order in LDS); a row with more than kKnownCap known values is
+// scanned by one lane.
+__global__ __launch_bounds__(64) void k_prune_scan(const double *__restrict__ D,
+                                                   const double *__restrict__ lb, int64_t nx,
+                                                   int64_t ny, int k, int32_t *__restrict__ pa,
+                                                   int32_t *__restrict__ pb,
+                                                   unsigned long long *__restrict__ cnt) {
+    __shared__ double kv[kKnownCap];
+    const int64_t i = blockIdx.x;
+    if (i >= nx) return;
+    const int lane = threadIdx.x;
+    const double *drow = D + i * ny;
+    const double *lrow = lb + i * ny;
+    bool bad = false;
+    for (int64_t j = lane; j < ny; j += 64) {
+        const double v = drow[j];
+        if (v == v && lrow[j] > v) bad = true;
+    }
+    const bool holds = __ballot(bad) == 0ull;
+    const int64_t C = (ny + 63) / 64;
+    const int64_t j0 = lane * C < ny ? lane * C : ny;
+    const int64_t j1 = j0 + C < ny ? j0 + C : ny;
+    int nk = 0;
+    for (int64_t j = j0; j < j1; ++j) {
+        const double v = drow[j];
+        nk += v == v ? 1 : 0;
+    }
+    int inc = nk;  // inclusive prefix over the lanes
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(inc, d);
+        if (lane >= d) inc += t;
+    }
+    const int total = __shfl(inc, 63);
+    double top[kKMax];
+#pragma unroll
+    for (int q = 0; q < kKMax; ++q) top[q] = WBX_INF_D;
+    double U = WBX_INF_D;
+    if (total > kKnownCap) {  // uniform
+        if (lane == 0) scan_span(drow, lrow, i, 0, ny, k, holds, top, U, pa, pb, cnt);
+        return;
+    }
+    int o = inc - nk;
+    for (int64_t j = j0; j < j1; ++j) {
+        const double v = drow[j];
+        if (v == v) kv[o++] = v;
+    }
+    __syncthreads();
+    const int before = inc - nk;  // known values before this lane's chunk
+    for (int e = 0; e < before; ++e) {
+        const double v = kv[e];
+        if (v < U) {
+            double c = v;
+#pragma unroll
+            for (int q = 0; q < kKMax; ++q) {
+                if (c < top[q]) {
+                    const double t = top[q];
+                    top[q] = c;
+                    c = t;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kKMax; ++q)
+                if (q == k - 1) U = top[q];
+        }
+    }
+    scan_span(drow, lrow, i, j0, j1, k, holds, top, U, pa, pb, cnt);
+}
+
 }  // namespace
 
 int prune_k_max() { return kKMax; }
@@ -125,8 +221,9 @@ int prune_k_max() { return kKMax; }
 int launch_prune_seed(const double *lb, int64_t nx, int64_t ny, int64_t k, int32_t *pa,
                       int32_t *pb, unsigned long long *cnt, hipStream_t st) {
     if (k > kKMax) return fail(WBX_EINVAL, "bound pruning serves k <= %d", kKMax);
-    hipLaunchKernelGGL(k_prune_seed, dim3((unsigned)((nx + 63) / 64)), dim3(64), 0, st, lb, nx, ny,
-                       k, pa, pb, cnt);
+    if (nx > 0x7fffffff) return fail(WBX_EUNSUPPORTED, "bound pruning: too many rows");
+    hipLaunchKernelGGL(k_prune_seed, dim3((unsigned)nx), dim3(64), 0, st, lb, nx, ny, k, pa, pb,
+                       cnt);
     WBX_HIP_CHECK(hipGetLastError());
     return WBX_OK;
 }
@@ -134,8 +231,9 @@ int launch_prune_seed(const double *lb, int64_t nx, int64_t ny, int64_t k, int32
 int launch_prune_scan(const double *D, const double *lb, int64_t nx, int64_t ny, int64_t k,
                       int32_t *pa, int32_t *pb, unsigned long long *cnt, hipStream_t st) {
     if (k > kKMax) return fail(WBX_EINVAL, "bound pruning serves k <= %d", kKMax);
-    hipLaunchKernelGGL(k_prune_scan, dim3((unsigned)((nx + 63) / 64)), dim3(64), 0, st, D, lb, nx,
-                       ny, (int)k, pa, pb, cnt);
+    if (nx > 0x7fffffff) return fail(WBX_EUNSUPPORTED, "bound pruning: too many rows");
+    hipLaunchKernelGGL(k_prune_scan, dim3((unsigned)nx), dim3(64), 0, st, D, lb, nx, ny, (int)k,
+                       pa, pb, cnt);
     WBX_HIP_CHECK(hipGetLastError());
     return WBX_OK;
 }
