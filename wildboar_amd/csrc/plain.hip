@@ -175,8 +175,10 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_euclid_grid(Series<T> xs, int64_t nx,
                                                      Series<T> ys, int64_t ny,
                                                      int t, T *__restrict__ sq) {
-    __shared__ T sx[EB][EK + 1];
-    __shared__ T sy[EB][EK + 1];
+    // rows padded to EK + 2: even, so a pair of timesteps is one aligned
+    // 2-element LDS read, and 16 rows read together fall in distinct banks
+    __shared__ __attribute__((aligned(16))) T sx[EB][EK + 2];
+    __shared__ __attribute__((aligned(16))) T sy[EB][EK + 2];
     const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
     const int64_t a0 = (int64_t)blockIdx.y * EB, b0 = (int64_t)blockIdx.x * EB;
     T acc[4][4];
@@ -214,7 +216,23 @@ __global__ __launch_bounds__(256) void k_euclid_grid(Series<T> xs, int64_t nx,
         }
         __syncthreads();
         if (c0 + EK < t) fetch(c0 + EK);
-        for (int cc = 0; cc < cn; ++cc) {
+        int cc = 0;
+        for (; cc + 2 <= cn; cc += 2) {  // two timesteps per LDS read, in order
+            typedef T t2 __attribute__((ext_vector_type(2)));
+            t2 xv[4], yv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                xv[i] = *(const t2 *)&sx[ty + 16 * i][cc];
+                yv[i] = *(const t2 *)&sy[tx + 16 * i][cc];
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) euclid_acc(acc[i][j], yv[j][h] - xv[i][h]);
+        }
+        for (; cc < cn; ++cc) {
             T xv[4], yv[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
