@@ -173,6 +173,19 @@ def test_pairwise_fp32_every_padded_band_width():
         assert_fp32_close(got, oracle.pairwise(X, Y, metric="adtw", r=r, p=0.5))
 
 
+@pytest.mark.parametrize("t", [1, 2, 31, 33, 63, 65, 97])
+def test_pairwise_euclidean_tile_edges(t):
+    # Equal lengths take the 64 x 64-pair LDS tiles, which read two timesteps
+    # at a time: odd lengths end on a single timestep, short ones inside the
+    # first chunk, and 70 x 67 rows leave partial tiles on both edges.
+    X = np.random.default_rng(48).standard_normal((70, t))
+    Y = np.random.default_rng(49).standard_normal((67, t))
+    want = oracle.pairwise(X, Y, metric="euclidean")
+    np.testing.assert_array_equal(wd.pairwise_distance(X, Y, metric="euclidean"), want)
+    got = wd.pairwise_distance(X, Y, metric="euclidean", precision="fp32")
+    assert_fp32_close(got, want)
+
+
 def test_pairwise_fp64_lane_classes_bitwise():
     # float64 lanes: DTW up to 40 slots at equal and unequal lengths, ADTW in
     # every class to 104 slots (the classes that load the next group during
