@@ -461,6 +461,9 @@ def run_fp64(torch, dev):
     res["profile_dtw"]["mismatch_4_rows"] = int((prof != want).sum())
     both("profile_euclidean", lambda: device.subsequence_profile(pd, X, metric="euclidean"),
          2000.0 * nwin * 128, unit="MAC/s")
+    res["profile_euclidean"]["note"] = (
+        "one kernel under both policies (the block policy switches only the elastic "
+        "kernels): speedup_vs_block compares two timings of the same launch")
     # 2. k = 5 best windows, pattern i (L U[16, 128]) against series i
     L = rng.integers(16, 129, 2000)
     pats = [Xh[int(rng.integers(2000)), a:a + l].copy()
